@@ -1,0 +1,177 @@
+/*
+ * openferric_amd -- C ABI of the MI355X (gfx950) Monte Carlo pricing engine.
+ *
+ * A drop-in for OpenFerric's Monte Carlo pricing hot path: plain C types,
+ * caller-owned host arrays, no exceptions across the boundary. Status codes
+ * map 1:1 onto the reference's PricingError variants
+ * (src/core/engine.rs:292-302) plus OFG_DEVICE_ERROR for HIP failures; the
+ * message of the last failure on the calling thread is ofg_last_error().
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef OPENFERRIC_AMD_H
+#define OPENFERRIC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: PricingError (src/core/engine.rs:292-302) ---------- */
+enum ofg_status {
+    OFG_OK = 0,
+    OFG_INVALID_INPUT = 1,          /* PricingError::InvalidInput */
+    OFG_CONVERGENCE_FAILURE = 2,    /* PricingError::ConvergenceFailure */
+    OFG_MARKET_DATA_MISSING = 3,    /* PricingError::MarketDataMissing */
+    OFG_NUMERICAL_ERROR = 4,        /* PricingError::NumericalError */
+    OFG_DEVICE_ERROR = 5            /* HIP runtime / device failure */
+};
+
+/* ---- enumerations ------------------------------------------------------ */
+enum ofg_model { OFG_MODEL_GBM = 0, OFG_MODEL_HESTON = 1 };
+/* MonteCarloInstrument impls (src/engines/monte_carlo/mc_engine.rs:441-584) */
+enum ofg_payoff { OFG_PAYOFF_VANILLA = 0, OFG_PAYOFF_ASIAN = 1, OFG_PAYOFF_BARRIER = 2 };
+enum ofg_option_type { OFG_CALL = 0, OFG_PUT = 1 };              /* core/types.rs:13-19 */
+enum ofg_averaging { OFG_ARITHMETIC = 0, OFG_GEOMETRIC = 1 };     /* core/types.rs:98-104 */
+enum ofg_strike_type { OFG_FIXED = 0, OFG_FLOATING = 1 };        /* core/types.rs:106-112 */
+enum ofg_barrier_dir { OFG_UP = 0, OFG_DOWN = 1 };               /* core/types.rs:50-56 */
+enum ofg_barrier_style { OFG_IN = 0, OFG_OUT = 1 };              /* core/types.rs:58-64 */
+
+/* Where a path's standard normals come from. */
+enum ofg_rng_mode {
+    /* The reference's own draws: per-sample xoshiro256++ stream seeded with
+       SplitMix64(seed + 7919*i), Acklam inverse CDF of the clamped uniform
+       (src/mc/simulation.rs:215-225, src/math/fast_rng.rs). Per-path parity
+       with the reference. */
+    OFG_RNG_REFERENCE = 0,
+    /* Caller-supplied normals, step-major [step][sample] per option. */
+    OFG_RNG_EXTERNAL = 1,
+    /* Throughput mode: the same per-sample xoshiro256++ streams feeding a
+       branch-free Box-Muller transform. Statistically equivalent draws. */
+    OFG_RNG_FAST = 2
+};
+
+enum ofg_greek_mode { OFG_GREEKS_NONE = 0, OFG_GREEKS_ADJOINT = 1, OFG_GREEKS_FD_CRN = 2 };
+
+/* One priced contract: instrument + market + model + path/step counts.
+ * Mirrors (VanillaOption | AsianOption | BarrierOption, Market, Gbm | Heston)
+ * as consumed by MonteCarloPricingEngine::price (mc_engine.rs:875-975). */
+typedef struct ofg_option {
+    int32_t model;          /* ofg_model */
+    int32_t payoff;         /* ofg_payoff */
+    int32_t option_type;    /* ofg_option_type */
+    int32_t averaging;      /* Asian: ofg_averaging */
+    int32_t strike_type;    /* Asian: ofg_strike_type */
+    int32_t barrier_dir;    /* Barrier: ofg_barrier_dir */
+    int32_t barrier_style;  /* Barrier: ofg_barrier_style */
+    int32_t n_obs;          /* Asian: number of observation times */
+    double spot, strike, expiry;
+    double rate, div_yield; /* GBM/Heston drift mu = rate - div_yield */
+    double vol;             /* GBM sigma (Market::vol_for, flat) */
+    double kappa, theta, xi, rho, v0;   /* Heston (models/stochastic.rs:37-44) */
+    double barrier_level, rebate;
+    const double *obs_times;            /* Asian observation times (host) */
+    int64_t num_paths;      /* MonteCarloPricingEngine::num_paths */
+    int64_t num_steps;      /* MonteCarloPricingEngine::num_steps */
+    uint64_t seed;          /* MonteCarloPricingEngine::seed */
+} ofg_option;
+
+/* Engine configuration shared by all options of one call. */
+typedef struct ofg_engine_cfg {
+    int32_t antithetic;       /* VarianceReduction::Antithetic */
+    int32_t control_variate;  /* VarianceReduction::ControlVariate (built-in CVs) */
+    int32_t rng_mode;         /* ofg_rng_mode */
+    int32_t greek_mode;       /* ofg_greek_mode */
+    double bump_rel;          /* FD_CRN relative bump (e.g. 1e-4) */
+    /* OFG_RNG_EXTERNAL: host arrays, options concatenated in call order,
+       each option laid out [step][sample] (samples = ceil(paths/2) with
+       antithetic). normals2 only for Heston. */
+    const double *normals1;
+    const double *normals2;
+    int32_t tune_block;       /* 0 = autotuned/default; else threads/block */
+    int32_t tune_item_steps;  /* 0 = default; else lane-steps per work item */
+} ofg_engine_cfg;
+
+/* PricingResult (src/core/engine.rs:277-287) flattened, plus the Heston
+ * parameter gradient and timing side channel. */
+typedef struct ofg_result {
+    double price;
+    double stderr_;           /* PricingResult::stderr */
+    int32_t has_stderr;
+    int32_t has_greeks;
+    double delta, gamma, vega, theta, rho;   /* Greeks (engine.rs:15-28) */
+    /* dPrice/d{S0, v0, kappa, theta, xi, rho} (Heston adjoint) */
+    double heston_grad[6];
+    /* Diagnostics keys NumPaths, NumSteps, Vol (mc_engine.rs:964-967) */
+    double num_paths, num_steps, vol;
+    double kernel_ms;         /* device time of this call's kernels */
+} ofg_result;
+
+typedef struct ofg_ctx ofg_ctx;
+typedef struct ofg_batch ofg_batch;
+
+/* Context: owns the HIP stream, device work buffers and tuning cache for one
+ * GPU. Replaces the process-wide wgpu context of the existing GPU engine
+ * (src/engines/gpu/gpu_mc.rs:276-283). NULL on failure (see ofg_last_error). */
+ofg_ctx *ofg_create(int32_t device);
+void ofg_destroy(ofg_ctx *ctx);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *ofg_last_error(void);
+
+/* Library version string. */
+const char *ofg_version(void);
+
+/* Prices n options. Replaces PricingEngine::price for
+ * MonteCarloPricingEngine (src/engines/monte_carlo/mc_engine.rs:875-975)
+ * and the simulate+reduce loop MonteCarloEngine::run
+ * (src/mc/simulation.rs:182-348); a Heston option replaces
+ * MonteCarloEngine::run with HestonPathGenerator (simulation.rs:98-116).
+ * With greek_mode != NONE also fills the Greeks
+ * (PricingEngine::price_with_greeks_aad, mc_engine.rs:977-987). out[n]. */
+int32_t ofg_price(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                  const ofg_engine_cfg *cfg, ofg_result *out);
+
+/* Per-sample payoffs of ONE option: x[i] (and control variate y[i]) exactly
+ * as the fold body of MonteCarloEngine::run computes them
+ * (simulation.rs:212-252), antithetic legs averaged. xs/ys hold
+ * samples = antithetic ? ceil(paths/2) : paths doubles; ys may be NULL. */
+int32_t ofg_sample_payoffs(ofg_ctx *ctx, const ofg_option *opt,
+                           const ofg_engine_cfg *cfg, double *xs, double *ys);
+
+/* Pathwise adjoint Greeks. Replaces mc_european_pathwise_aad
+ * (src/engines/monte_carlo/mc_aad.rs:75-219) and heston_price_delta_aad
+ * (mc_aad.rs:279-342), generalised to Asian/barrier payoffs and the full
+ * Heston parameter gradient. Equivalent to ofg_price with
+ * greek_mode = OFG_GREEKS_ADJOINT. */
+int32_t ofg_price_greeks(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                         const ofg_engine_cfg *cfg, ofg_result *out);
+
+/* Batched pricing for repeated replay: builds the work queue, uploads the
+ * contracts once and captures [queue reset, persistent pricing kernel,
+ * segmented reduce] into a hipGraph. Replaces a loop of
+ * PricingEngine::price calls (mc_engine.rs:875). */
+ofg_batch *ofg_batch_create(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                            const ofg_engine_cfg *cfg, int32_t use_graph);
+/* Replays the batch; kernel_ms (may be NULL) receives its device time. */
+int32_t ofg_batch_run(ofg_batch *batch, double *kernel_ms);
+/* Copies the last run's results (n entries). */
+int32_t ofg_batch_results(ofg_batch *batch, ofg_result *out);
+/* Device time of the persistent pricing kernel alone in the last run. */
+double ofg_batch_pricing_kernel_ms(ofg_batch *batch);
+void ofg_batch_destroy(ofg_batch *batch);
+
+/* Autotuner: times candidate launch shapes (threads per block, work-item
+ * size) on a scaled-down copy of the given workload, stores the winner in
+ * the context (and in cache_path when non-NULL) keyed by the workload class.
+ * best_block / best_item_steps may be NULL. */
+int32_t ofg_autotune(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                     const ofg_engine_cfg *cfg, const char *cache_path,
+                     int32_t *best_block, int32_t *best_item_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPENFERRIC_AMD_H */

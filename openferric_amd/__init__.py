@@ -1,0 +1,27 @@
+"""openferric_amd: MI355X (gfx950) Monte Carlo pricing engine for OpenFerric's
+MC hot path (GBM / Heston paths, payoffs, reduction, pathwise Greeks).
+
+The compute path is libopenferric_amd.so (hand-written HIP kernels behind a C
+ABI, include/openferric_amd.h). This package mirrors the reference's Rust
+PricingEngine contract on top of it.
+"""
+from .core import (AsianOption, AsianSpec, Averaging, BarrierDirection, BarrierOption,
+                   BarrierSpec, BarrierStyle, ConvergenceFailure, Diagnostics, ExerciseStyle, Gbm,
+                   Greeks, Heston, InvalidInput, Market, MarketDataMissing, NumericalError,
+                   OptionType, PricingError, PricingResult, StrikeType, VanillaOption)
+from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEngine,
+                      HestonAadConfig, HestonMonteCarloPricingEngine, RngMode, VarianceReduction,
+                      autotune, engine_cfg, heston_price_delta_aad, mc_european_pathwise_aad,
+                      option_struct, price_raw, sample_payoffs)
+from ._lib import DeviceError
+
+__all__ = [
+    "AsianOption", "AsianSpec", "Averaging", "BarrierDirection", "BarrierOption", "BarrierSpec",
+    "BarrierStyle", "ConvergenceFailure", "Diagnostics", "ExerciseStyle", "Gbm", "Greeks",
+    "Heston", "InvalidInput", "Market", "MarketDataMissing", "NumericalError", "OptionType",
+    "PricingError", "PricingResult", "StrikeType", "VanillaOption", "BatchPricer",
+    "GpuArithmeticAsianMC", "GpuMonteCarloPricingEngine", "HestonAadConfig",
+    "HestonMonteCarloPricingEngine", "RngMode", "VarianceReduction", "autotune", "engine_cfg",
+    "heston_price_delta_aad", "mc_european_pathwise_aad", "option_struct", "price_raw",
+    "sample_payoffs", "DeviceError",
+]

@@ -1,0 +1,142 @@
+"""ctypes binding of libopenferric_amd.so (the C ABI in include/openferric_amd.h).
+
+The product path has no CPU fallback: if the HIP library is missing or no GPU
+is present, calls raise DeviceError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libopenferric_amd.so")
+
+OFG_OK = 0
+OFG_INVALID_INPUT = 1
+OFG_CONVERGENCE_FAILURE = 2
+OFG_MARKET_DATA_MISSING = 3
+OFG_NUMERICAL_ERROR = 4
+OFG_DEVICE_ERROR = 5
+
+MODEL_GBM, MODEL_HESTON = 0, 1
+PAYOFF_VANILLA, PAYOFF_ASIAN, PAYOFF_BARRIER = 0, 1, 2
+CALL, PUT = 0, 1
+ARITHMETIC, GEOMETRIC = 0, 1
+FIXED, FLOATING = 0, 1
+UP, DOWN = 0, 1
+IN, OUT = 0, 1
+RNG_REFERENCE, RNG_EXTERNAL, RNG_FAST = 0, 1, 2
+GREEKS_NONE, GREEKS_ADJOINT, GREEKS_FD_CRN = 0, 1, 2
+
+EXPORTED_SYMBOLS = [
+    "ofg_create", "ofg_destroy", "ofg_last_error", "ofg_version", "ofg_price",
+    "ofg_sample_payoffs", "ofg_price_greeks", "ofg_batch_create", "ofg_batch_run",
+    "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
+]
+
+
+class OfgOption(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32), ("payoff", C.c_int32), ("option_type", C.c_int32),
+        ("averaging", C.c_int32), ("strike_type", C.c_int32), ("barrier_dir", C.c_int32),
+        ("barrier_style", C.c_int32), ("n_obs", C.c_int32),
+        ("spot", C.c_double), ("strike", C.c_double), ("expiry", C.c_double),
+        ("rate", C.c_double), ("div_yield", C.c_double), ("vol", C.c_double),
+        ("kappa", C.c_double), ("theta", C.c_double), ("xi", C.c_double),
+        ("rho", C.c_double), ("v0", C.c_double),
+        ("barrier_level", C.c_double), ("rebate", C.c_double),
+        ("obs_times", C.POINTER(C.c_double)),
+        ("num_paths", C.c_int64), ("num_steps", C.c_int64), ("seed", C.c_uint64),
+    ]
+
+
+class OfgEngineCfg(C.Structure):
+    _fields_ = [
+        ("antithetic", C.c_int32), ("control_variate", C.c_int32),
+        ("rng_mode", C.c_int32), ("greek_mode", C.c_int32),
+        ("bump_rel", C.c_double),
+        ("normals1", C.POINTER(C.c_double)), ("normals2", C.POINTER(C.c_double)),
+        ("tune_block", C.c_int32), ("tune_item_steps", C.c_int32),
+    ]
+
+
+class OfgResult(C.Structure):
+    _fields_ = [
+        ("price", C.c_double), ("stderr_", C.c_double),
+        ("has_stderr", C.c_int32), ("has_greeks", C.c_int32),
+        ("delta", C.c_double), ("gamma", C.c_double), ("vega", C.c_double),
+        ("theta", C.c_double), ("rho", C.c_double),
+        ("heston_grad", C.c_double * 6),
+        ("num_paths", C.c_double), ("num_steps", C.c_double), ("vol", C.c_double),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+class DeviceError(RuntimeError):
+    """HIP library missing or device failure (OFG_DEVICE_ERROR)."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load() -> C.CDLL:
+    """Loads the HIP library; raises DeviceError when it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise DeviceError(
+                f"openferric_amd: {LIB_PATH} is not built; run `python -m openferric_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        pv = C.c_void_p
+        L.ofg_create.argtypes = [C.c_int32]; L.ofg_create.restype = pv
+        L.ofg_destroy.argtypes = [pv]; L.ofg_destroy.restype = None
+        L.ofg_last_error.argtypes = []; L.ofg_last_error.restype = C.c_char_p
+        L.ofg_version.argtypes = []; L.ofg_version.restype = C.c_char_p
+        L.ofg_price.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t, C.POINTER(OfgEngineCfg),
+                                C.POINTER(OfgResult)]
+        L.ofg_price.restype = C.c_int32
+        L.ofg_price_greeks.argtypes = L.ofg_price.argtypes
+        L.ofg_price_greeks.restype = C.c_int32
+        L.ofg_sample_payoffs.argtypes = [pv, C.POINTER(OfgOption), C.POINTER(OfgEngineCfg),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ofg_sample_payoffs.restype = C.c_int32
+        L.ofg_batch_create.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t,
+                                       C.POINTER(OfgEngineCfg), C.c_int32]
+        L.ofg_batch_create.restype = pv
+        L.ofg_batch_run.argtypes = [pv, C.POINTER(C.c_double)]; L.ofg_batch_run.restype = C.c_int32
+        L.ofg_batch_results.argtypes = [pv, C.POINTER(OfgResult)]
+        L.ofg_batch_results.restype = C.c_int32
+        L.ofg_batch_pricing_kernel_ms.argtypes = [pv]
+        L.ofg_batch_pricing_kernel_ms.restype = C.c_double
+        L.ofg_batch_destroy.argtypes = [pv]; L.ofg_batch_destroy.restype = None
+        L.ofg_autotune.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t, C.POINTER(OfgEngineCfg),
+                                   C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.ofg_autotune.restype = C.c_int32
+        _lib = L
+        return L
+
+
+def last_error() -> str:
+    return load().ofg_last_error().decode()
+
+
+_ctx = {}
+
+
+def context(device: int = 0):
+    """Process-wide context per device (the engine's OnceLock analogue)."""
+    with _lock:
+        c = _ctx.get(device)
+    if c:
+        return c
+    L = load()
+    c = L.ofg_create(device)
+    if not c:
+        raise DeviceError(f"ofg_create({device}) failed: {last_error()}")
+    with _lock:
+        _ctx[device] = c
+    return c

@@ -1,0 +1,979 @@
+// Host side of the openferric_amd C ABI (include/openferric_amd.h).
+//
+// Responsibilities: input validation with the reference's error semantics
+// (src/engines/monte_carlo/mc_engine.rs:875-912, src/instruments/*.rs
+// validate()), the instrument -> path-generator wiring of
+// MonteCarloPricingEngine::price (mc_engine.rs:889-945), the built-in control
+// variates (mc_engine.rs:458-485, 546-583), building the device work queue,
+// launching the kernels on the context's stream, hipGraph capture of batches,
+// and the autotuner. No exception crosses the C ABI.
+#include "../../include/openferric_amd.h"
+#include "ofg_internal.hpp"
+#include "ofg_adjoint.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct OfgError : std::runtime_error {
+    int32_t code;
+    OfgError(int32_t c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int32_t code, const std::string &msg) { throw OfgError(code, msg); }
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        fail(OFG_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Closed forms used by the built-in control variates. These restate the
+// reference formulas so that the CV expectation is the reference's own:
+// hart_norm_cdf (src/math/fast_norm.rs:27-50), bs_price generic branch
+// (src/engines/analytic/black_scholes.rs:62-107) and
+// geometric_asian_discrete_fixed_closed_form (src/pricing/asian.rs:86-136).
+// ---------------------------------------------------------------------------
+double hart_norm_cdf(double x) {
+    const double P = 0.2316419;
+    const double A1 = 0.319381530, A2 = -0.356563782, A3 = 1.781477937, A4 = -1.821255978,
+                 A5 = 1.330274429;
+    const double z = std::fabs(x);
+    const double t = 1.0 / std::fma(P, z, 1.0);
+    const double poly = std::fma(std::fma(std::fma(std::fma(A5, t, A4), t, A3), t, A2), t, A1) * t;
+    const double pdf = 0.3989422804014327 * std::exp((-0.5) * z * z);
+    const double cdf_pos = std::fma(pdf, -poly, 1.0);
+    const double sign = std::signbit(x) ? 1.0 : 0.0;
+    return std::fma(sign, 1.0 - 2.0 * cdf_pos, cdf_pos);
+}
+
+double vanilla_payoff(int32_t type, double spot, double strike) {
+    return type == OFG_CALL ? std::fmax(spot - strike, 0.0) : std::fmax(strike - spot, 0.0);
+}
+
+double bs_price(int32_t type, double spot, double strike, double rate, double q, double vol,
+                double expiry) {
+    if (expiry <= 0.0) return vanilla_payoff(type, spot, strike);
+    const double df_r = std::exp(-rate * expiry);
+    const double df_q = std::exp(-q * expiry);
+    if (vol <= 0.0) {
+        return type == OFG_CALL ? std::fmax(spot * df_q - strike * df_r, 0.0)
+                                : std::fmax(strike * df_r - spot * df_q, 0.0);
+    }
+    const double sig_sqrt_t = vol * std::sqrt(expiry);
+    const double d1 =
+        (std::log(spot / strike) + std::fma(0.5 * vol, vol, rate - q) * expiry) / sig_sqrt_t;
+    const double d2 = d1 - sig_sqrt_t;
+    const double nd1 = hart_norm_cdf(d1), nd2 = hart_norm_cdf(d2);
+    const double s_df_q = spot * df_q;
+    const double k_df_r = strike * df_r;
+    const double call = std::fma(s_df_q, nd1, -(k_df_r * nd2));
+    return type == OFG_CALL ? call : call - s_df_q + k_df_r;
+}
+
+double geometric_asian_discrete(int32_t type, double s0, double k, double r, double q,
+                                double sigma, const double *obs, int64_t n_obs) {
+    if (n_obs == 0) return vanilla_payoff(type, s0, k);
+    double maturity = -INFINITY;
+    for (int64_t i = 0; i < n_obs; ++i) maturity = std::fmax(maturity, obs[i]);
+    maturity = std::fmax(maturity, 0.0);
+    if (maturity <= 0.0 || sigma <= 0.0) return std::exp(-r * maturity) * vanilla_payoff(type, s0, k);
+    const double n = (double)n_obs;
+    double sum_t = 0.0;
+    for (int64_t i = 0; i < n_obs; ++i) sum_t += obs[i];
+    const double mean_t = sum_t / n;
+    double cov_sum = 0.0;
+    for (int64_t i = 0; i < n_obs; ++i)
+        for (int64_t j = 0; j < n_obs; ++j) cov_sum += std::fmin(obs[i], obs[j]);
+    const double var = sigma * sigma * cov_sum / (n * n);
+    const double m = std::log(s0) + std::fma(-0.5 * sigma, sigma, r - q) * mean_t;
+    const double df = std::exp(-r * maturity);
+    if (var <= 0.0) return df * vanilla_payoff(type, std::exp(m), k);
+    const double sqrt_v = std::sqrt(var);
+    const double d1 = (m - std::log(k) + var) / sqrt_v;
+    const double d2 = d1 - sqrt_v;
+    const double eg = std::exp(m + 0.5 * var);
+    return type == OFG_CALL ? df * (eg * hart_norm_cdf(d1) - k * hart_norm_cdf(d2))
+                            : df * (k * hart_norm_cdf(-d2) - eg * hart_norm_cdf(-d1));
+}
+
+// ---------------------------------------------------------------------------
+// Validation (reference error strings)
+// ---------------------------------------------------------------------------
+void validate_instrument(const ofg_option &o) {
+    switch (o.payoff) {
+    case OFG_PAYOFF_VANILLA:  // instruments/vanilla.rs:69-96
+        if (!(o.strike > 0.0)) fail(OFG_INVALID_INPUT, "vanilla strike must be > 0");
+        if (o.expiry < 0.0) fail(OFG_INVALID_INPUT, "vanilla expiry must be >= 0");
+        break;
+    case OFG_PAYOFF_ASIAN:  // instruments/asian.rs:39-65
+        if (o.expiry < 0.0) fail(OFG_INVALID_INPUT, "asian expiry must be >= 0");
+        if (!(o.strike > 0.0)) fail(OFG_INVALID_INPUT, "asian strike must be > 0");
+        if (o.n_obs <= 0 || o.obs_times == nullptr)
+            fail(OFG_INVALID_INPUT, "asian observation_times cannot be empty");
+        for (int32_t i = 0; i < o.n_obs; ++i) {
+            if (o.obs_times[i] < 0.0 || o.obs_times[i] > o.expiry)
+                fail(OFG_INVALID_INPUT, "asian observation_times must lie in [0, expiry]");
+        }
+        if (o.averaging != OFG_ARITHMETIC && o.averaging != OFG_GEOMETRIC)
+            fail(OFG_INVALID_INPUT, "unknown averaging");
+        if (o.strike_type != OFG_FIXED && o.strike_type != OFG_FLOATING)
+            fail(OFG_INVALID_INPUT, "unknown strike type");
+        break;
+    case OFG_PAYOFF_BARRIER:  // instruments/barrier.rs:36-58
+        if (!(o.strike > 0.0)) fail(OFG_INVALID_INPUT, "barrier strike must be > 0");
+        if (o.expiry < 0.0) fail(OFG_INVALID_INPUT, "barrier expiry must be >= 0");
+        if (!(o.barrier_level > 0.0)) fail(OFG_INVALID_INPUT, "barrier level must be > 0");
+        if (o.rebate < 0.0) fail(OFG_INVALID_INPUT, "barrier rebate must be >= 0");
+        if (o.barrier_dir != OFG_UP && o.barrier_dir != OFG_DOWN)
+            fail(OFG_INVALID_INPUT, "unknown barrier direction");
+        if (o.barrier_style != OFG_IN && o.barrier_style != OFG_OUT)
+            fail(OFG_INVALID_INPUT, "unknown barrier style");
+        break;
+    default:
+        fail(OFG_INVALID_INPUT, "unknown payoff kind");
+    }
+    if (o.option_type != OFG_CALL && o.option_type != OFG_PUT)
+        fail(OFG_INVALID_INPUT, "unknown option type");
+}
+
+// Heston::validate (src/models/stochastic.rs:47-54)
+bool heston_valid(const ofg_option &o) {
+    return o.kappa > 0.0 && o.theta >= 0.0 && o.xi >= 0.0 && o.v0 >= 0.0 && o.rho > -1.0 &&
+           o.rho < 1.0;
+}
+
+// payoff_from_path(&[spot]) of the maturity-zero shortcut (mc_engine.rs:896-904)
+double payoff_at_spot(const ofg_option &o) {
+    if (o.payoff == OFG_PAYOFF_BARRIER) {
+        const bool hit = o.barrier_dir == OFG_UP ? o.spot >= o.barrier_level : o.spot <= o.barrier_level;
+        const bool active = o.barrier_style == OFG_IN ? hit : !hit;
+        return active ? vanilla_payoff(o.option_type, o.spot, o.strike) : o.rebate;
+    }
+    // Vanilla, and Asian whose average over a one-point path is the spot.
+    if (o.payoff == OFG_PAYOFF_ASIAN && o.strike_type == OFG_FLOATING)
+        return vanilla_payoff(o.option_type, o.spot, o.spot);
+    return vanilla_payoff(o.option_type, o.spot, o.strike);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device buffers
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t bytes) {
+        if (bytes <= cap) return;
+        if (p) hip_check(hipFree(p), "hipFree");
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes, 256);
+        hip_check(hipMalloc(&p, want), "hipMalloc");
+        cap = want;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Host-side plan of one pricing launch.
+struct Plan {
+    std::vector<ofg::OptDev> opts;
+    std::vector<double> obs_w;
+    std::vector<ofg::WorkItem> items;
+    std::vector<int32_t> order;
+    std::vector<int64_t> item_first;
+    std::vector<int32_t> item_count;
+    std::vector<int32_t> active;      // plan index -> caller option index
+    std::vector<double> vols;         // per caller option
+    int64_t total_samples = 0;
+    int64_t normals_elems = 0;
+    bool any_heston = false;
+};
+
+struct Tuning {
+    int32_t block = 256;
+    int32_t item_steps = 8192;
+};
+
+}  // namespace
+
+struct ofg_ctx {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    std::mutex mu;
+    DevBuf opts, obs_w, items, order, item_first, item_count, partials, outs, head, xs, ys, n1,
+        n2, adj;
+    std::map<std::string, Tuning> tuned;
+    int cus = 256;
+};
+
+struct ofg_batch {
+    ofg_ctx *ctx = nullptr;
+    Plan plan;
+    ofg_engine_cfg cfg{};
+    DevBuf opts, obs_w, items, order, item_first, item_count, partials, outs, head;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    ofg::LaunchArgs args{};
+    std::vector<ofg::OptOut> host_out;
+    std::vector<ofg_result> results;
+    double last_ms = 0.0, last_pricing_ms = 0.0;
+    bool use_graph = false;
+    bool ran = false;
+};
+
+namespace {
+
+std::string workload_class(const std::vector<ofg::OptDev> &opts, int32_t rng_mode) {
+    // Keyed by model mix, payoff mix, RNG mode and a step bucket.
+    int gbm = 0, hes = 0, kinds[4] = {0, 0, 0, 0};
+    double steps = 0;
+    for (const auto &o : opts) {
+        (o.model == OFG_MODEL_GBM ? gbm : hes)++;
+        kinds[o.kind & 3]++;
+        steps += o.steps;
+    }
+    const double mean_steps = opts.empty() ? 0 : steps / (double)opts.size();
+    const int bucket = mean_steps < 64 ? 0 : mean_steps < 256 ? 1 : mean_steps < 1024 ? 2 : 3;
+    std::ostringstream k;
+    k << "rng" << rng_mode << "_m" << (gbm ? "g" : "") << (hes ? "h" : "") << "_k"
+      << (kinds[0] ? "v" : "") << (kinds[1] ? "a" : "") << (kinds[2] ? "l" : "")
+      << (kinds[3] ? "b" : "") << "_s" << bucket;
+    return k.str();
+}
+
+// Builds the device descriptors, observation weights and work items.
+// `vols` is Market::vol_for (flat) per option; for Heston options ignored.
+Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const Tuning &tune) {
+    Plan p;
+    p.vols.resize(n);
+    int64_t out_offset = 0, normals_offset = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = in[i];
+        validate_instrument(o);
+        if (o.model != OFG_MODEL_GBM && o.model != OFG_MODEL_HESTON)
+            fail(OFG_INVALID_INPUT, "unknown model");
+        if (o.num_paths <= 0) fail(OFG_INVALID_INPUT, "num_paths must be > 0");
+        if (o.num_steps <= 0) fail(OFG_INVALID_INPUT, "num_steps must be > 0");
+        if (o.num_steps > 1000000) fail(OFG_INVALID_INPUT, "num_steps must be <= 1000000");
+        if (o.expiry < 0.0) fail(OFG_INVALID_INPUT, "instrument maturity must be >= 0");
+        if (o.model == OFG_MODEL_HESTON && !heston_valid(o))
+            fail(OFG_INVALID_INPUT, "invalid Heston parameters");
+        p.vols[i] = o.vol;
+        if (o.expiry == 0.0) continue;  // maturity-zero shortcut, no simulation
+        if (o.model == OFG_MODEL_GBM && o.vol <= 0.0)
+            fail(OFG_INVALID_INPUT, "market volatility must be > 0");
+
+        ofg::OptDev d{};
+        d.model = o.model;
+        d.steps = (int32_t)o.num_steps;
+        const int64_t samples = cfg.antithetic ? (o.num_paths + 1) / 2 : o.num_paths;
+        d.samples = samples;
+        d.out_offset = out_offset;
+        d.normals_offset = normals_offset;
+        d.obs_offset = (int64_t)p.obs_w.size();
+        d.seed = o.seed;
+        d.s0 = o.spot;
+        d.strike = o.strike;
+        d.mu = o.rate - o.div_yield;  // mc_engine.rs:916
+        d.sigma = o.vol;
+        // GbmPathGenerator::generate_into constants (simulation.rs:59-62)
+        d.dt = o.expiry / (double)o.num_steps;
+        d.sqrt_dt = std::sqrt(d.dt);
+        d.drift = (d.mu - 0.5 * d.sigma * d.sigma) * d.dt;
+        d.diffusion = d.sigma * d.sqrt_dt;
+        d.kappa = o.kappa;
+        d.theta = o.theta;
+        d.xi = o.xi;
+        d.rho = o.rho;
+        d.rho_c = std::sqrt(1.0 - o.rho * o.rho);
+        d.v0 = o.v0;
+        d.barrier = o.barrier_level;
+        d.rebate = o.rebate;
+        d.nobs = (double)std::max<int32_t>(o.n_obs, 0);
+        d.log_s0 = std::log(std::fmax(o.spot, 1e-12));
+        d.df = std::exp(-o.rate * o.expiry);  // mc_engine.rs:945
+        d.antithetic = cfg.antithetic ? 1 : 0;
+        int32_t flags = 0;
+        if (o.option_type == OFG_PUT) flags |= ofg::F_PUT;
+
+        // observation weights over path indices 0..steps
+        const int64_t len = o.num_steps + 1;
+        if (o.payoff == OFG_PAYOFF_ASIAN) {
+            std::vector<double> w((size_t)len, 0.0);
+            const double last_idx = (double)(len - 1);
+            for (int32_t k = 0; k < o.n_obs; ++k) {
+                // average_for_observations index map (mc_engine.rs:103-104)
+                const double x = std::round((o.obs_times[k] / o.expiry) * last_idx);
+                int64_t idx = !(x > 0.0) ? 0 : (x >= 9.2e18 ? INT64_MAX : (int64_t)x);
+                idx = std::min<int64_t>(idx, len - 1);
+                w[(size_t)idx] += 1.0;
+            }
+            p.obs_w.insert(p.obs_w.end(), w.begin(), w.end());
+        } else {
+            p.obs_w.push_back(0.0);  // keep offsets valid
+        }
+
+        // payoff class and control variate (mc_engine.rs:441-584, 934-943)
+        bool need_log = false;
+        if (o.payoff == OFG_PAYOFF_VANILLA) {
+            d.kind = ofg::PK_VANILLA;
+        } else if (o.payoff == OFG_PAYOFF_BARRIER) {
+            d.kind = ofg::PK_BARRIER;
+            if (o.barrier_dir == OFG_DOWN) flags |= ofg::F_DOWN;
+            if (o.barrier_style == OFG_IN) flags |= ofg::F_KNOCK_IN;
+        } else {
+            d.kind = ofg::PK_ASIAN;
+            if (o.strike_type == OFG_FLOATING) flags |= ofg::F_FLOATING;
+            if (o.averaging == OFG_GEOMETRIC) {
+                flags |= ofg::F_GEOMETRIC;
+                need_log = true;
+            }
+        }
+        if (cfg.control_variate && o.model == OFG_MODEL_GBM) {
+            const double df = d.df;
+            if (o.payoff == OFG_PAYOFF_VANILLA) {
+                // black_scholes_price_with_dividend (mc_engine.rs:66-77, 466-474)
+                const double adj = o.spot * std::exp(-o.div_yield * o.expiry);
+                const double e = bs_price(o.option_type, adj, o.strike, o.rate, 0.0, o.vol, o.expiry);
+                d.cv_expected = e / df;
+                d.has_cv = 1;
+                flags |= ofg::F_CV;
+            } else if (o.payoff == OFG_PAYOFF_ASIAN && o.averaging == OFG_ARITHMETIC &&
+                       o.strike_type == OFG_FIXED) {
+                const double e = geometric_asian_discrete(o.option_type, o.spot, o.strike, o.rate,
+                                                          o.div_yield, o.vol, o.obs_times, o.n_obs);
+                d.cv_expected = e / df;
+                d.has_cv = 1;
+                flags |= ofg::F_CV | ofg::F_CV_GEO;
+                need_log = true;
+            }
+        }
+        if (need_log) d.kind = ofg::PK_ASIAN_LOG;
+        d.flags = flags;
+        if (o.model == OFG_MODEL_HESTON) p.any_heston = true;
+
+        const int32_t plan_idx = (int32_t)p.opts.size();
+        p.opts.push_back(d);
+        p.active.push_back((int32_t)i);
+
+        // work items: samples_per_lane chosen so an item is ~tune.item_steps
+        // lane-steps of work (Heston and antithetic legs weigh more).
+        const double leg_w = (o.model == OFG_MODEL_HESTON ? 1.9 : 1.0) * (cfg.antithetic ? 1.6 : 1.0);
+        const double per_sample = (double)o.num_steps * leg_w;
+        int64_t spl = (int64_t)std::llround((double)tune.item_steps / per_sample);
+        spl = std::max<int64_t>(1, std::min<int64_t>(spl, 1024));
+        const int64_t item_samples = 64 * spl;
+        p.item_first.push_back((int64_t)p.items.size());
+        int32_t count = 0;
+        for (int64_t b = 0; b < samples; b += item_samples) {
+            ofg::WorkItem wi{};
+            wi.option = plan_idx;
+            wi.begin = b;
+            wi.end = std::min(samples, b + item_samples);
+            p.items.push_back(wi);
+            ++count;
+        }
+        p.item_count.push_back(count);
+        out_offset += samples;
+        normals_offset += samples * o.num_steps;
+    }
+    if (p.items.size() > (size_t)INT32_MAX) fail(OFG_INVALID_INPUT, "too many work items");
+    p.total_samples = out_offset;
+    p.normals_elems = normals_offset;
+    // cost-descending queue order (longest-processing-time first)
+    std::vector<double> cost(p.items.size());
+    for (size_t k = 0; k < p.items.size(); ++k) {
+        const auto &wi = p.items[k];
+        const auto &d = p.opts[(size_t)wi.option];
+        const double leg_w = (d.model == OFG_MODEL_HESTON ? 1.9 : 1.0) * (d.antithetic ? 1.6 : 1.0);
+        const double lanes_steps = std::ceil((double)(wi.end - wi.begin) / 64.0) * d.steps;
+        cost[k] = lanes_steps * leg_w;
+    }
+    p.order.resize(p.items.size());
+    std::iota(p.order.begin(), p.order.end(), 0);
+    std::stable_sort(p.order.begin(), p.order.end(),
+                     [&](int32_t a, int32_t b) { return cost[(size_t)a] > cost[(size_t)b]; });
+    return p;
+}
+
+template <class T>
+void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+    b.ensure(std::max<size_t>(v.size() * sizeof(T), 16));
+    if (!v.empty())
+        hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync H2D");
+}
+
+void fill_result(ofg_result &r, const ofg_option &o, const ofg::OptOut *oo, double ms) {
+    std::memset(&r, 0, sizeof r);
+    r.num_paths = (double)o.num_paths;
+    r.num_steps = (double)o.num_steps;
+    r.vol = o.vol;
+    r.kernel_ms = ms;
+    if (oo == nullptr) {  // maturity-zero shortcut: no diagnostics, stderr 0
+        r.price = payoff_at_spot(o);
+        r.stderr_ = 0.0;
+        r.has_stderr = 1;
+        r.num_paths = r.num_steps = r.vol = 0.0;
+        return;
+    }
+    r.price = oo->price;
+    r.stderr_ = oo->stderr_;
+    r.has_stderr = 1;
+}
+
+Tuning tuning_for(ofg_ctx *ctx, const std::vector<ofg::OptDev> &opts, const ofg_engine_cfg &cfg) {
+    Tuning t;
+    auto it = ctx->tuned.find(workload_class(opts, cfg.rng_mode));
+    if (it != ctx->tuned.end()) t = it->second;
+    if (cfg.tune_block > 0) t.block = cfg.tune_block;
+    if (cfg.tune_item_steps > 0) t.item_steps = cfg.tune_item_steps;
+    return t;
+}
+
+// Runs a plan on the context stream; outs receives one OptOut per plan option.
+double run_plan(ofg_ctx *ctx, const Plan &p, const ofg_engine_cfg &cfg, const Tuning &t,
+                std::vector<ofg::OptOut> &outs, double *xs_host, double *ys_host,
+                const double *n1_host, const double *n2_host, double *pricing_ms) {
+    hipStream_t s = ctx->stream;
+    const size_t nopt = p.opts.size();
+    outs.assign(nopt, ofg::OptOut{});
+    if (nopt == 0) return 0.0;
+    upload(ctx->opts, p.opts, s);
+    upload(ctx->obs_w, p.obs_w, s);
+    upload(ctx->items, p.items, s);
+    upload(ctx->order, p.order, s);
+    upload(ctx->item_first, p.item_first, s);
+    upload(ctx->item_count, p.item_count, s);
+    ctx->partials.ensure(p.items.size() * sizeof(ofg::Partial));
+    ctx->outs.ensure(nopt * sizeof(ofg::OptOut));
+    ctx->head.ensure(256);
+    double *xs_dev = nullptr, *ys_dev = nullptr;
+    if (xs_host) {
+        ctx->xs.ensure((size_t)p.total_samples * sizeof(double));
+        xs_dev = ctx->xs.as<double>();
+    }
+    if (ys_host) {
+        ctx->ys.ensure((size_t)p.total_samples * sizeof(double));
+        ys_dev = ctx->ys.as<double>();
+    }
+    const double *n1_dev = nullptr, *n2_dev = nullptr;
+    if (cfg.rng_mode == OFG_RNG_EXTERNAL) {
+        if (!n1_host) fail(OFG_INVALID_INPUT, "external normals required for OFG_RNG_EXTERNAL");
+        if (p.any_heston && !n2_host)
+            fail(OFG_INVALID_INPUT, "Heston needs a second external normal stream");
+        const size_t bytes = (size_t)p.normals_elems * sizeof(double);
+        ctx->n1.ensure(bytes);
+        hip_check(hipMemcpyAsync(ctx->n1.p, n1_host, bytes, hipMemcpyHostToDevice, s), "H2D normals");
+        n1_dev = ctx->n1.as<double>();
+        if (n2_host) {
+            ctx->n2.ensure(bytes);
+            hip_check(hipMemcpyAsync(ctx->n2.p, n2_host, bytes, hipMemcpyHostToDevice, s), "H2D normals");
+            n2_dev = ctx->n2.as<double>();
+        }
+    }
+    ofg::LaunchArgs a{};
+    a.opts = ctx->opts.as<ofg::OptDev>();
+    a.items = ctx->items.as<ofg::WorkItem>();
+    a.order = ctx->order.as<int32_t>();
+    a.n_items = (int32_t)p.items.size();
+    a.queue_head = ctx->head.as<uint32_t>();
+    a.partials = ctx->partials.as<ofg::Partial>();
+    a.obs_w = ctx->obs_w.as<double>();
+    a.xs = xs_dev;
+    a.ys = ys_dev;
+    a.normals1 = n1_dev;
+    a.normals2 = n2_dev;
+    a.rng_mode = cfg.rng_mode;
+    a.block = t.block;
+    const int grid_full = ofg::persistent_grid(t.block, cfg.rng_mode, ctx->device);
+    const int waves_per_block = t.block / 64;
+    const int need_blocks = (int)std::min<int64_t>(grid_full, (p.items.size() + waves_per_block - 1) / waves_per_block);
+    a.grid = std::max(1, need_blocks);
+
+    hip_check(ofg::launch_reset_queue(a.queue_head, s), "reset queue");
+    hip_check(hipEventRecord(ctx->ev0, s), "event");
+    hip_check(ofg::launch_pricing(a, s), "pricing kernel launch");
+    hip_check(hipEventRecord(ctx->ev1, s), "event");
+    hip_check(ofg::launch_reduce(a.opts, ctx->item_first.as<int64_t>(), ctx->item_count.as<int32_t>(),
+                                 a.partials, ctx->outs.as<ofg::OptOut>(), (int32_t)nopt, s),
+              "reduce kernel launch");
+    hip_check(hipEventRecord(ctx->ev2, s), "event");
+    hip_check(hipMemcpyAsync(outs.data(), ctx->outs.p, nopt * sizeof(ofg::OptOut),
+                             hipMemcpyDeviceToHost, s),
+              "D2H results");
+    if (xs_host)
+        hip_check(hipMemcpyAsync(xs_host, xs_dev, (size_t)p.total_samples * sizeof(double),
+                                 hipMemcpyDeviceToHost, s), "D2H xs");
+    if (ys_host)
+        hip_check(hipMemcpyAsync(ys_host, ys_dev, (size_t)p.total_samples * sizeof(double),
+                                 hipMemcpyDeviceToHost, s), "D2H ys");
+    hip_check(hipStreamSynchronize(s), "kernel execution");
+    float ms_all = 0.f, ms_price = 0.f;
+    hip_check(hipEventElapsedTime(&ms_price, ctx->ev0, ctx->ev1), "event time");
+    hip_check(hipEventElapsedTime(&ms_all, ctx->ev0, ctx->ev2), "event time");
+    if (pricing_ms) *pricing_ms = ms_price;
+    return ms_all;
+}
+
+template <class F>
+int32_t guarded(F &&f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const OfgError &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return OFG_NUMERICAL_ERROR;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return OFG_NUMERICAL_ERROR;
+    }
+}
+
+int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
+                   ofg_result *out);
+
+// Central finite differences with common random numbers: every bumped copy
+// uses the same seed, so the same streams (greeks/bsm.rs:130-212 harness,
+// priced in one batched launch).
+int32_t price_fd(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
+                 ofg_result *out) {
+    const double h = cfg->bump_rel > 0 ? cfg->bump_rel : 1e-4;
+    std::vector<ofg_option> all;
+    struct Spec { size_t base; int nb; };
+    std::vector<Spec> spec(n);
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = opts[i];
+        spec[i].base = all.size();
+        all.push_back(o);
+        auto add = [&](double ofg_option::*f, double rel_abs_floor) {
+            const double b = std::max(std::fabs(o.*f) * h, rel_abs_floor);
+            ofg_option up = o, dn = o;
+            up.*f += b;
+            dn.*f -= b;
+            all.push_back(up);
+            all.push_back(dn);
+            return b;
+        };
+        (void)add;
+        if (o.model == OFG_MODEL_GBM) {
+            add(&ofg_option::spot, 1e-8);
+            add(&ofg_option::vol, 1e-8);
+            add(&ofg_option::rate, h);
+            add(&ofg_option::expiry, 1e-8);
+            spec[i].nb = 4;
+        } else {
+            add(&ofg_option::spot, 1e-8);
+            add(&ofg_option::v0, 1e-8);
+            add(&ofg_option::kappa, 1e-8);
+            add(&ofg_option::theta, 1e-8);
+            add(&ofg_option::xi, 1e-8);
+            add(&ofg_option::rho, 1e-8);
+            spec[i].nb = 6;
+        }
+    }
+    ofg_engine_cfg c = *cfg;
+    c.greek_mode = OFG_GREEKS_NONE;
+    std::vector<ofg_result> res(all.size());
+    const int32_t rc = price_impl(ctx, all.data(), all.size(), &c, res.data());
+    if (rc != OFG_OK) return rc;
+    double ms = 0;
+    for (auto &r : res) ms = std::max(ms, r.kernel_ms);
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = opts[i];
+        ofg_result r = res[spec[i].base];
+        r.has_greeks = 1;
+        r.kernel_ms = ms;
+        auto fd = [&](int k, double ofg_option::*f, double floor_) {
+            const double b = std::max(std::fabs(o.*f) * h, floor_);
+            const double up = res[spec[i].base + 1 + 2 * k].price;
+            const double dn = res[spec[i].base + 2 + 2 * k].price;
+            return std::make_pair((up - dn) / (2.0 * b), (up - 2.0 * r.price + dn) / (b * b));
+        };
+        if (o.model == OFG_MODEL_GBM) {
+            auto d = fd(0, &ofg_option::spot, 1e-8);
+            r.delta = d.first;
+            r.gamma = d.second;
+            r.vega = fd(1, &ofg_option::vol, 1e-8).first;
+            r.rho = fd(2, &ofg_option::rate, h).first;
+            r.theta = -fd(3, &ofg_option::expiry, 1e-8).first;
+        } else {
+            auto d = fd(0, &ofg_option::spot, 1e-8);
+            r.delta = d.first;
+            r.gamma = d.second;
+            r.heston_grad[0] = d.first;
+            r.heston_grad[1] = fd(1, &ofg_option::v0, 1e-8).first;
+            r.heston_grad[2] = fd(2, &ofg_option::kappa, 1e-8).first;
+            r.heston_grad[3] = fd(3, &ofg_option::theta, 1e-8).first;
+            r.heston_grad[4] = fd(4, &ofg_option::xi, 1e-8).first;
+            r.heston_grad[5] = fd(5, &ofg_option::rho, 1e-8).first;
+        }
+        out[i] = r;
+    }
+    return OFG_OK;
+}
+
+int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
+                   ofg_result *out) {
+    if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
+    if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+    if (cfg->greek_mode == OFG_GREEKS_FD_CRN) return price_fd(ctx, opts, n, cfg, out);
+    if (cfg->greek_mode == OFG_GREEKS_ADJOINT) return ofg::adjoint_price(ctx, opts, n, cfg, out);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    // First pass with default tuning only to classify the workload.
+    Plan probe = build_plan(opts, n, *cfg, Tuning{});
+    const Tuning t = tuning_for(ctx, probe.opts, *cfg);
+    Plan p = (t.item_steps == Tuning{}.item_steps) ? std::move(probe) : build_plan(opts, n, *cfg, t);
+    std::vector<ofg::OptOut> outs;
+    double pricing_ms = 0;
+    const double ms = run_plan(ctx, p, *cfg, t, outs, nullptr, nullptr, cfg->normals1,
+                               cfg->normals2, &pricing_ms);
+    std::vector<const ofg::OptOut *> map(n, nullptr);
+    for (size_t k = 0; k < p.active.size(); ++k) map[(size_t)p.active[k]] = &outs[k];
+    for (size_t i = 0; i < n; ++i) fill_result(out[i], opts[i], map[i], ms);
+    return OFG_OK;
+}
+
+}  // namespace
+
+// Exposed to the adjoint module.
+namespace ofg {
+void host_fail(int32_t code, const std::string &msg) { fail(code, msg); }
+void host_hip_check(hipError_t e, const char *what) { hip_check(e, what); }
+void host_validate(const ofg_option &o) {
+    validate_instrument(o);
+    if (o.num_paths <= 0) fail(OFG_INVALID_INPUT, "num_paths must be > 0");
+    if (o.num_steps <= 0) fail(OFG_INVALID_INPUT, "num_steps must be > 0");
+    if (o.model == OFG_MODEL_HESTON && !heston_valid(o))
+        fail(OFG_INVALID_INPUT, "invalid Heston parameters");
+}
+double host_payoff_at_spot(const ofg_option &o) { return payoff_at_spot(o); }
+hipStream_t ctx_stream(ofg_ctx *ctx) { return ctx->stream; }
+int32_t ctx_device(ofg_ctx *ctx) { return ctx->device; }
+std::mutex &ctx_mutex(ofg_ctx *ctx) { return ctx->mu; }
+void *ctx_adjoint_scratch(ofg_ctx *ctx, size_t bytes) {
+    ctx->adj.ensure(bytes);
+    return ctx->adj.p;
+}
+}  // namespace ofg
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *ofg_last_error(void) { return g_last_error.c_str(); }
+
+const char *ofg_version(void) { return "openferric_amd 0.1.0 (gfx950)"; }
+
+ofg_ctx *ofg_create(int32_t device) {
+    ofg_ctx *ctx = nullptr;
+    const int32_t rc = guarded([&]() -> int32_t {
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count) fail(OFG_DEVICE_ERROR, "no such HIP device");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        auto c = std::make_unique<ofg_ctx>();
+        c->device = device;
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev2), "hipEventCreate");
+        hip_check(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device),
+                  "hipDeviceGetAttribute");
+        ctx = c.release();
+        return OFG_OK;
+    });
+    return rc == OFG_OK ? ctx : nullptr;
+}
+
+void ofg_destroy(ofg_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf *b : {&ctx->opts, &ctx->obs_w, &ctx->items, &ctx->order, &ctx->item_first,
+                      &ctx->item_count, &ctx->partials, &ctx->outs, &ctx->head, &ctx->xs, &ctx->ys,
+                      &ctx->n1, &ctx->n2, &ctx->adj})
+        b->release();
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipEventDestroy(ctx->ev2);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int32_t ofg_price(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
+                  ofg_result *out) {
+    return guarded([&]() { return price_impl(ctx, opts, n, cfg, out); });
+}
+
+int32_t ofg_price_greeks(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                         const ofg_engine_cfg *cfg, ofg_result *out) {
+    return guarded([&]() -> int32_t {
+        if (!cfg) fail(OFG_INVALID_INPUT, "null argument");
+        ofg_engine_cfg c = *cfg;
+        c.greek_mode = OFG_GREEKS_ADJOINT;
+        return price_impl(ctx, opts, n, &c, out);
+    });
+}
+
+int32_t ofg_sample_payoffs(ofg_ctx *ctx, const ofg_option *opt, const ofg_engine_cfg *cfg,
+                           double *xs, double *ys) {
+    return guarded([&]() -> int32_t {
+        if (!ctx || !opt || !cfg || !xs) fail(OFG_INVALID_INPUT, "null argument");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (opt->expiry == 0.0) fail(OFG_INVALID_INPUT, "maturity-zero option has no paths");
+        Plan p = build_plan(opt, 1, *cfg, tuning_for(ctx, {}, *cfg));
+        std::vector<ofg::OptOut> outs;
+        run_plan(ctx, p, *cfg, tuning_for(ctx, p.opts, *cfg), outs, xs, ys, cfg->normals1,
+                 cfg->normals2, nullptr);
+        return OFG_OK;
+    });
+}
+
+ofg_batch *ofg_batch_create(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                            const ofg_engine_cfg *cfg, int32_t use_graph) {
+    ofg_batch *out = nullptr;
+    guarded([&]() -> int32_t {
+        if (!ctx || !opts || !cfg || n == 0) fail(OFG_INVALID_INPUT, "null argument");
+        if (cfg->rng_mode == OFG_RNG_EXTERNAL)
+            fail(OFG_INVALID_INPUT, "batches use generated normals");
+        if (cfg->greek_mode != OFG_GREEKS_NONE)
+            fail(OFG_INVALID_INPUT, "batches price only; use ofg_price_greeks for Greeks");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        auto b = std::make_unique<ofg_batch>();
+        b->ctx = ctx;
+        b->cfg = *cfg;
+        Plan probe = build_plan(opts, n, *cfg, Tuning{});
+        const Tuning t = tuning_for(ctx, probe.opts, *cfg);
+        b->plan = build_plan(opts, n, *cfg, t);
+        b->results.resize(n);
+        for (size_t i = 0; i < n; ++i) fill_result(b->results[i], opts[i], nullptr, 0.0);
+        const Plan &p = b->plan;
+        hipStream_t s = ctx->stream;
+        upload(b->opts, p.opts, s);
+        upload(b->obs_w, p.obs_w, s);
+        upload(b->items, p.items, s);
+        upload(b->order, p.order, s);
+        upload(b->item_first, p.item_first, s);
+        upload(b->item_count, p.item_count, s);
+        b->partials.ensure(std::max<size_t>(p.items.size(), 1) * sizeof(ofg::Partial));
+        b->outs.ensure(std::max<size_t>(p.opts.size(), 1) * sizeof(ofg::OptOut));
+        b->head.ensure(256);
+        hip_check(hipStreamSynchronize(s), "upload");
+        ofg::LaunchArgs &a = b->args;
+        a.opts = b->opts.as<ofg::OptDev>();
+        a.items = b->items.as<ofg::WorkItem>();
+        a.order = b->order.as<int32_t>();
+        a.n_items = (int32_t)p.items.size();
+        a.queue_head = b->head.as<uint32_t>();
+        a.partials = b->partials.as<ofg::Partial>();
+        a.obs_w = b->obs_w.as<double>();
+        a.rng_mode = cfg->rng_mode;
+        a.block = t.block;
+        const int grid_full = ofg::persistent_grid(t.block, cfg->rng_mode, ctx->device);
+        const int wpb = t.block / 64;
+        a.grid = std::max(1, (int)std::min<int64_t>(grid_full, ((int64_t)p.items.size() + wpb - 1) / wpb));
+        hip_check(hipEventCreate(&b->ev0), "hipEventCreate");
+        hip_check(hipEventCreate(&b->ev1), "hipEventCreate");
+        hip_check(hipEventCreate(&b->ev2), "hipEventCreate");
+        b->use_graph = use_graph != 0 && !p.opts.empty();
+        if (b->use_graph) {
+            // Queue reset is a kernel, not a memset node, so replays stay exact.
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+            hipError_t e1 = ofg::launch_reset_queue(a.queue_head, s);
+            hipError_t e2 = ofg::launch_pricing(a, s);
+            hipError_t e3 = ofg::launch_reduce(a.opts, b->item_first.as<int64_t>(),
+                                               b->item_count.as<int32_t>(), a.partials,
+                                               b->outs.as<ofg::OptOut>(), (int32_t)p.opts.size(), s);
+            hipError_t e4 = hipStreamEndCapture(s, &b->graph);
+            hip_check(e1, "capture reset");
+            hip_check(e2, "capture pricing");
+            hip_check(e3, "capture reduce");
+            hip_check(e4, "end capture");
+            hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+        }
+        out = b.release();
+        return OFG_OK;
+    });
+    return out;
+}
+
+int32_t ofg_batch_run(ofg_batch *b, double *kernel_ms) {
+    return guarded([&]() -> int32_t {
+        if (!b) fail(OFG_INVALID_INPUT, "null batch");
+        ofg_ctx *ctx = b->ctx;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        hipStream_t s = ctx->stream;
+        const Plan &p = b->plan;
+        if (p.opts.empty()) {
+            if (kernel_ms) *kernel_ms = 0.0;
+            return OFG_OK;
+        }
+        hip_check(hipEventRecord(b->ev0, s), "event");
+        if (b->use_graph) {
+            hip_check(hipGraphLaunch(b->exec, s), "graph launch");
+            hip_check(hipEventRecord(b->ev2, s), "event");
+        } else {
+            hip_check(ofg::launch_reset_queue(b->args.queue_head, s), "reset queue");
+            hip_check(ofg::launch_pricing(b->args, s), "pricing kernel");
+            hip_check(hipEventRecord(b->ev1, s), "event");
+            hip_check(ofg::launch_reduce(b->args.opts, b->item_first.as<int64_t>(),
+                                         b->item_count.as<int32_t>(), b->args.partials,
+                                         b->outs.as<ofg::OptOut>(), (int32_t)p.opts.size(), s),
+                      "reduce kernel");
+            hip_check(hipEventRecord(b->ev2, s), "event");
+        }
+        hip_check(hipStreamSynchronize(s), "batch execution");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, b->ev0, b->ev2), "event time");
+        b->last_ms = ms;
+        if (!b->use_graph) {
+            float mp = 0.f;
+            hip_check(hipEventElapsedTime(&mp, b->ev0, b->ev1), "event time");
+            b->last_pricing_ms = mp;
+        } else {
+            b->last_pricing_ms = ms;
+        }
+        if (kernel_ms) *kernel_ms = ms;
+        b->ran = true;
+        return OFG_OK;
+    });
+}
+
+int32_t ofg_batch_results(ofg_batch *b, ofg_result *out) {
+    return guarded([&]() -> int32_t {
+        if (!b || !out) fail(OFG_INVALID_INPUT, "null argument");
+        ofg_ctx *ctx = b->ctx;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        const Plan &p = b->plan;
+        b->host_out.resize(p.opts.size());
+        if (!p.opts.empty()) {
+            if (!b->ran) fail(OFG_INVALID_INPUT, "batch has not been run");
+            hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+            hip_check(hipMemcpy(b->host_out.data(), b->outs.p, p.opts.size() * sizeof(ofg::OptOut),
+                                hipMemcpyDeviceToHost),
+                      "D2H batch results");
+        }
+        for (size_t i = 0; i < b->results.size(); ++i) out[i] = b->results[i];
+        for (size_t k = 0; k < p.active.size(); ++k) {
+            ofg_result &r = out[(size_t)p.active[k]];
+            r.price = b->host_out[k].price;
+            r.stderr_ = b->host_out[k].stderr_;
+            r.has_stderr = 1;
+            r.kernel_ms = b->last_ms;
+        }
+        return OFG_OK;
+    });
+}
+
+double ofg_batch_pricing_kernel_ms(ofg_batch *b) { return b ? b->last_pricing_ms : 0.0; }
+
+void ofg_batch_destroy(ofg_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (DevBuf *d : {&b->opts, &b->obs_w, &b->items, &b->order, &b->item_first, &b->item_count,
+                      &b->partials, &b->outs, &b->head})
+        d->release();
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ev2) (void)hipEventDestroy(b->ev2);
+    delete b;
+}
+
+int32_t ofg_autotune(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
+                     const char *cache_path, int32_t *best_block, int32_t *best_item_steps) {
+    return guarded([&]() -> int32_t {
+        if (!ctx || !opts || !cfg || n == 0) fail(OFG_INVALID_INPUT, "null argument");
+        if (cfg->rng_mode == OFG_RNG_EXTERNAL) fail(OFG_INVALID_INPUT, "autotune needs generated normals");
+        // Scaled-down copy: same options, path counts scaled so the sample
+        // workload is ~2e9 path-steps (or the full workload if smaller).
+        double total = 0;
+        for (size_t i = 0; i < n; ++i) total += (double)opts[i].num_paths * (double)opts[i].num_steps;
+        const double scale = std::min(1.0, 2e9 / std::max(total, 1.0));
+        std::vector<ofg_option> small(opts, opts + n);
+        for (auto &o : small) o.num_paths = std::max<int64_t>(64, (int64_t)(o.num_paths * scale));
+        ofg_engine_cfg c = *cfg;
+        c.greek_mode = OFG_GREEKS_NONE;
+        const int32_t blocks[] = {64, 128, 256, 512};
+        const int32_t items[] = {2048, 8192, 32768};
+        Tuning best;
+        double best_ms = 1e300;
+        std::vector<ofg_result> res(n);
+        std::string cls;
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            cls = workload_class(build_plan(small.data(), n, c, Tuning{}).opts, c.rng_mode);
+        }
+        for (int32_t bl : blocks) {
+            for (int32_t is : items) {
+                c.tune_block = bl;
+                c.tune_item_steps = is;
+                double ms = 1e300;
+                for (int rep = 0; rep < 2; ++rep) {
+                    const int32_t rc = price_impl(ctx, small.data(), n, &c, res.data());
+                    if (rc != OFG_OK) return rc;
+                    ms = std::min(ms, res[0].kernel_ms);
+                }
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best.block = bl;
+                    best.item_steps = is;
+                }
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->tuned[cls] = best;
+        }
+        if (cache_path) {
+            std::ofstream f(cache_path, std::ios::app);
+            f << cls << " " << best.block << " " << best.item_steps << " " << best_ms << "\n";
+        }
+        if (best_block) *best_block = best.block;
+        if (best_item_steps) *best_item_steps = best.item_steps;
+        return OFG_OK;
+    });
+}
+
+}  // extern "C"

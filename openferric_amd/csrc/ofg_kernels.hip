@@ -1,0 +1,415 @@
+// Monte Carlo pricing kernels for MI355X (gfx950), FP64 on the VALU.
+//
+// Layout of the work:
+//   * a persistent grid of wavefronts pops work items from a device queue
+//     (one atomic per item, cost-descending order), so thousands of options
+//     with uneven path and step counts share one launch;
+//   * a work item is a sample range of ONE option, so every lane of a wave
+//     runs the same step count and the same payoff (no step divergence, all
+//     contract parameters are wave-uniform and live in SGPRs);
+//   * a lane simulates one sample at a time with the whole path state in
+//     registers: spot, RNG state, running Asian sums or barrier flag; nothing
+//     per step touches memory except (wave-uniform) observation weights;
+//   * per item the wave reduces its five FP64 sums with a fixed butterfly
+//     and writes them to the item's own slot, and a segmented reduce sums the
+//     slots of each option in item order, so prices do not depend on the
+//     queue schedule.
+//
+// Reference semantics: MonteCarloEngine::run's fold body
+// (src/mc/simulation.rs:212-262), GbmPathGenerator::generate_into
+// (simulation.rs:58-71), HestonPathGenerator + Heston::step_euler
+// (simulation.rs:98-116, src/models/stochastic.rs:56-71) and the payoffs of
+// src/engines/monte_carlo/mc_engine.rs:59-121, 441-584.
+#include "ofg_internal.hpp"
+#include "ofg_math.hpp"
+
+namespace ofg {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ double vanilla_payoff(bool put, double spot, double strike) {
+    return put ? fmax(strike - spot, 0.0) : fmax(spot - strike, 0.0);
+}
+
+// Running state of one path leg, held in registers.
+struct Leg {
+    double s;      // spot
+    double sum;    // weighted sum of observed spots (Asian)
+    double ls;     // log spot tracked additively (geometric Asian)
+    double lsum;   // weighted sum of observed log spots
+    bool hit;      // barrier touched (inclusive, path[0] included)
+};
+
+template <int PK>
+__device__ __forceinline__ void leg_init(Leg &g, const OptDev &o, double w0, double bsign) {
+    g.s = o.s0;
+    if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) g.sum = w0 * o.s0;
+    if constexpr (PK == PK_ASIAN_LOG) {
+        g.ls = o.log_s0;
+        g.lsum = w0 * o.log_s0;
+    }
+    if constexpr (PK == PK_BARRIER) g.hit = bsign * o.s0 >= bsign * o.barrier;
+}
+
+// Observation bookkeeping after the spot of path index j+1 is known.
+template <int PK>
+__device__ __forceinline__ void leg_observe(Leg &g, double w, double bsign, double barrier) {
+    if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) g.sum = fma(w, g.s, g.sum);
+    if constexpr (PK == PK_ASIAN_LOG) g.lsum = fma(w, g.ls, g.lsum);
+    if constexpr (PK == PK_BARRIER) g.hit = g.hit || (bsign * g.s >= bsign * barrier);
+}
+
+// One GBM step (simulation.rs:67-70): s *= exp(fma(diffusion, z, drift)).
+template <int PK>
+__device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z) {
+    const double e = fma(o.diffusion, z, o.drift);
+    g.s *= exp_fp64(e);
+    if constexpr (PK == PK_ASIAN_LOG) g.ls += e;
+}
+
+// One Heston Euler step (stochastic.rs:56-71) plus the generator clamps
+// (simulation.rs:112-113). Operation order follows the reference; the file is
+// compiled with -ffp-contract=off.
+template <int PK>
+__device__ __forceinline__ void heston_step(Leg &g, double &v, const OptDev &o, double z1,
+                                            double z2) {
+    const double v_pos = fmax(v, 0.0);
+    const double sv = sqrt(v_pos);
+    const double zs = o.rho * z1 + o.rho_c * z2;
+    const double vn = v + o.kappa * (o.theta - v_pos) * o.dt + o.xi * sv * o.sqrt_dt * z1;
+    const double e = (o.mu - 0.5 * v_pos) * o.dt + sv * o.sqrt_dt * zs;
+    const double sn = g.s * exp_fp64(e);
+    const bool floor_hit = !(sn >= 1e-12);
+    g.s = floor_hit ? 1e-12 : sn;   // f64::max(sn, 1e-12), NaN -> 1e-12
+    if constexpr (PK == PK_ASIAN_LOG) g.ls = floor_hit ? -27.631021115928547 : g.ls + e;
+    v = fmax(vn, 0.0);
+}
+
+template <int PK>
+__device__ __forceinline__ void leg_payoff(const Leg &g, const OptDev &o, double &x, double &y) {
+    const bool put = o.flags & F_PUT;
+    if constexpr (PK == PK_VANILLA) {
+        x = vanilla_payoff(put, g.s, o.strike);
+    } else if constexpr (PK == PK_BARRIER) {
+        const bool active = (o.flags & F_KNOCK_IN) ? g.hit : !g.hit;
+        x = active ? vanilla_payoff(put, g.s, o.strike) : o.rebate;
+    } else {
+        double avg;
+        if constexpr (PK == PK_ASIAN_LOG) {
+            avg = (o.flags & F_GEOMETRIC) ? exp_fp64(g.lsum / o.nobs) : g.sum / o.nobs;
+        } else {
+            avg = g.sum / o.nobs;
+        }
+        x = (o.flags & F_FLOATING) ? vanilla_payoff(put, g.s, avg) : vanilla_payoff(put, avg, o.strike);
+    }
+    y = 0.0;
+    if (o.flags & F_CV) {
+        if constexpr (PK == PK_ASIAN_LOG) {
+            if (o.flags & F_CV_GEO) {
+                y = vanilla_payoff(put, exp_fp64(g.lsum / o.nobs), o.strike);
+                return;
+            }
+        }
+        y = vanilla_payoff(put, g.s, o.strike);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Normal sources
+// ------------------------------------------------------------------------
+
+struct ExternalNormals {
+    const double *p1, *p2;
+    int64_t stride;
+    __device__ __forceinline__ double z1(int j) const { return p1[(int64_t)j * stride]; }
+    __device__ __forceinline__ double z2(int j) const { return p2[(int64_t)j * stride]; }
+};
+
+// Draws two 64-bit words and returns a Box-Muller pair.
+__device__ __forceinline__ void fast_pair(Xoshiro256pp &rng, double &a, double &b) {
+    const uint64_t w1 = rng.next_u64();
+    const uint64_t w2 = rng.next_u64();
+    box_muller(w1, w2, a, b);
+}
+
+// ------------------------------------------------------------------------
+// Sample simulation: one (possibly antithetic) sample -> (x, y)
+// ------------------------------------------------------------------------
+
+template <int MODEL, int PK, int RNG, bool ANTI>
+__device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
+                                                const double *__restrict__ w,
+                                                const LaunchArgs &a, double &x, double &y) {
+    const double bsign = (o.flags & F_DOWN) ? -1.0 : 1.0;
+    const double barrier = o.barrier;
+    const int steps = o.steps;
+    Leg ga, gb;
+    leg_init<PK>(ga, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0, bsign);
+    if constexpr (ANTI) leg_init<PK>(gb, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0, bsign);
+
+    Xoshiro256pp rng;
+    ExternalNormals ext;
+    if constexpr (RNG == OFG_RNG_REFERENCE || RNG == OFG_RNG_FAST) {
+        rng.seed(stream_seed(o.seed, (uint64_t)sample));
+    } else {
+        ext.p1 = a.normals1 + o.normals_offset + sample;
+        ext.p2 = a.normals2 ? a.normals2 + o.normals_offset + sample : nullptr;
+        ext.stride = o.samples;
+    }
+
+    if constexpr (MODEL == OFG_MODEL_GBM) {
+        if constexpr (RNG == OFG_RNG_FAST) {
+            int j = 0;
+            for (; j + 1 < steps; j += 2) {
+                double za, zb;
+                fast_pair(rng, za, zb);
+                gbm_step<PK>(ga, o, za);
+                if constexpr (ANTI) gbm_step<PK>(gb, o, -za);
+                const double w1 = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 1] : 0.0;
+                leg_observe<PK>(ga, w1, bsign, barrier);
+                if constexpr (ANTI) leg_observe<PK>(gb, w1, bsign, barrier);
+                gbm_step<PK>(ga, o, zb);
+                if constexpr (ANTI) gbm_step<PK>(gb, o, -zb);
+                const double w2 = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 2] : 0.0;
+                leg_observe<PK>(ga, w2, bsign, barrier);
+                if constexpr (ANTI) leg_observe<PK>(gb, w2, bsign, barrier);
+            }
+            if (j < steps) {
+                double za, zb;
+                fast_pair(rng, za, zb);
+                gbm_step<PK>(ga, o, za);
+                if constexpr (ANTI) gbm_step<PK>(gb, o, -za);
+                const double w1 = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 1] : 0.0;
+                leg_observe<PK>(ga, w1, bsign, barrier);
+                if constexpr (ANTI) leg_observe<PK>(gb, w1, bsign, barrier);
+            }
+        } else {
+            for (int j = 0; j < steps; ++j) {
+                double z;
+                if constexpr (RNG == OFG_RNG_REFERENCE) z = reference_normal(rng);
+                else z = ext.z1(j);
+                gbm_step<PK>(ga, o, z);
+                if constexpr (ANTI) gbm_step<PK>(gb, o, -z);
+                const double wj = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 1] : 0.0;
+                leg_observe<PK>(ga, wj, bsign, barrier);
+                if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
+            }
+        }
+    } else {
+        double va = o.v0, vb = o.v0;
+        for (int j = 0; j < steps; ++j) {
+            double z1, z2;
+            if constexpr (RNG == OFG_RNG_REFERENCE) {
+                z1 = reference_normal(rng);
+                z2 = reference_normal(rng);
+            } else if constexpr (RNG == OFG_RNG_FAST) {
+                fast_pair(rng, z1, z2);
+            } else {
+                z1 = ext.z1(j);
+                z2 = ext.z2(j);
+            }
+            heston_step<PK>(ga, va, o, z1, z2);
+            if constexpr (ANTI) heston_step<PK>(gb, vb, o, -z1, -z2);
+            const double wj = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 1] : 0.0;
+            leg_observe<PK>(ga, wj, bsign, barrier);
+            if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
+        }
+    }
+
+    leg_payoff<PK>(ga, o, x, y);
+    if constexpr (ANTI) {
+        double xa, ya;
+        leg_payoff<PK>(gb, o, xa, ya);
+        x = 0.5 * (x + xa);
+        y = 0.5 * (y + ya);
+    }
+}
+
+struct Acc {
+    double sx, sx2, sy, sxy, sy2;
+};
+
+template <int MODEL, int PK, int RNG, bool ANTI>
+__device__ void run_item(const OptDev &o, const WorkItem &wi, int lane, const LaunchArgs &a,
+                         Acc &acc) {
+    const double *__restrict__ w = a.obs_w + o.obs_offset;
+    for (int64_t base = wi.begin; base < wi.end; base += WAVE) {
+        const int64_t sample = base + lane;
+        if (sample < wi.end) {
+            double x, y;
+            simulate_sample<MODEL, PK, RNG, ANTI>(o, sample, w, a, x, y);
+            acc.sx += x;
+            acc.sx2 += x * x;
+            acc.sy += y;
+            acc.sxy += x * y;
+            acc.sy2 += y * y;
+            if (a.xs) a.xs[o.out_offset + sample] = x;
+            if (a.ys) a.ys[o.out_offset + sample] = y;
+        }
+    }
+}
+
+template <int MODEL, int RNG, bool ANTI>
+__device__ __forceinline__ void dispatch_kind(const OptDev &o, const WorkItem &wi, int lane,
+                                              const LaunchArgs &a, Acc &acc) {
+    switch (o.kind) {
+    case PK_VANILLA: run_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, acc); break;
+    case PK_ASIAN: run_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, acc); break;
+    case PK_ASIAN_LOG: run_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, acc); break;
+    default: run_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, acc); break;
+    }
+}
+
+template <int RNG, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void price_kernel(LaunchArgs a) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (;;) {
+        uint32_t q = 0;
+        if (lane == 0) q = atomicAdd(a.queue_head, 1u);
+        q = __builtin_amdgcn_readfirstlane(q);
+        if (q >= (uint32_t)a.n_items) break;
+        const int32_t it = __builtin_amdgcn_readfirstlane(a.order[q]);
+        const WorkItem wi = a.items[it];
+        const OptDev o = a.opts[wi.option];
+        Acc acc{0.0, 0.0, 0.0, 0.0, 0.0};
+        if (o.model == OFG_MODEL_GBM) {
+            if (o.antithetic) dispatch_kind<OFG_MODEL_GBM, RNG, true>(o, wi, lane, a, acc);
+            else dispatch_kind<OFG_MODEL_GBM, RNG, false>(o, wi, lane, a, acc);
+        } else {
+            if (o.antithetic) dispatch_kind<OFG_MODEL_HESTON, RNG, true>(o, wi, lane, a, acc);
+            else dispatch_kind<OFG_MODEL_HESTON, RNG, false>(o, wi, lane, a, acc);
+        }
+        const double sx = wave_sum(acc.sx);
+        const double sx2 = wave_sum(acc.sx2);
+        const double sy = wave_sum(acc.sy);
+        const double sxy = wave_sum(acc.sxy);
+        const double sy2 = wave_sum(acc.sy2);
+        if (lane == 0) {
+            Partial p;
+            p.sx = sx;
+            p.sx2 = sx2;
+            p.sy = sy;
+            p.sxy = sxy;
+            p.sy2 = sy2;
+            p.n = (double)(wi.end - wi.begin);
+            a.partials[it] = p;
+        }
+    }
+}
+
+__global__ void reset_queue_kernel(uint32_t *head) {
+    if (threadIdx.x == 0) *head = 0u;
+}
+
+// One wavefront per option: lanes stride over the option's item slots, then a
+// fixed butterfly; lane 0 closes the statistics.
+__global__ __launch_bounds__(256) void reduce_kernel(const OptDev *__restrict__ opts,
+                                                     const int64_t *__restrict__ item_first,
+                                                     const int32_t *__restrict__ item_count,
+                                                     const Partial *__restrict__ partials,
+                                                     OptOut *__restrict__ out, int32_t n_opts) {
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (gw >= n_opts) return;
+    const int64_t first = item_first[gw];
+    const int32_t count = item_count[gw];
+    double sx = 0, sx2 = 0, sy = 0, sxy = 0, sy2 = 0, n = 0;
+    for (int32_t i = lane; i < count; i += WAVE) {
+        const Partial p = partials[first + i];
+        sx += p.sx;
+        sx2 += p.sx2;
+        sy += p.sy;
+        sxy += p.sxy;
+        sy2 += p.sy2;
+        n += p.n;
+    }
+    sx = wave_sum(sx);
+    sx2 = wave_sum(sx2);
+    sy = wave_sum(sy);
+    sxy = wave_sum(sxy);
+    sy2 = wave_sum(sy2);
+    n = wave_sum(n);
+    if (lane == 0) {
+        const OptDev &o = opts[gw];
+        OptOut r;
+        close_stats(sx, sx2, sy, sxy, sy2, n, o.has_cv, o.cv_expected, o.df, &r.price, &r.stderr_);
+        r.sx = sx;
+        r.sx2 = sx2;
+        r.sy = sy;
+        r.sxy = sxy;
+        r.sy2 = sy2;
+        r.n = n;
+        out[gw] = r;
+    }
+}
+
+template <int RNG>
+hipError_t launch_rng(const LaunchArgs &a, hipStream_t stream) {
+    switch (a.block) {
+    case 64: price_kernel<RNG, 64><<<a.grid, 64, 0, stream>>>(a); break;
+    case 128: price_kernel<RNG, 128><<<a.grid, 128, 0, stream>>>(a); break;
+    case 512: price_kernel<RNG, 512><<<a.grid, 512, 0, stream>>>(a); break;
+    default: price_kernel<RNG, 256><<<a.grid, 256, 0, stream>>>(a); break;
+    }
+    return hipGetLastError();
+}
+
+template <int RNG>
+const void *kernel_ptr(int block) {
+    switch (block) {
+    case 64: return (const void *)price_kernel<RNG, 64>;
+    case 128: return (const void *)price_kernel<RNG, 128>;
+    case 512: return (const void *)price_kernel<RNG, 512>;
+    default: return (const void *)price_kernel<RNG, 256>;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_reset_queue(uint32_t *head, hipStream_t stream) {
+    reset_queue_kernel<<<1, 64, 0, stream>>>(head);
+    return hipGetLastError();
+}
+
+hipError_t launch_pricing(const LaunchArgs &a, hipStream_t stream) {
+    switch (a.rng_mode) {
+    case OFG_RNG_EXTERNAL: return launch_rng<OFG_RNG_EXTERNAL>(a, stream);
+    case OFG_RNG_FAST: return launch_rng<OFG_RNG_FAST>(a, stream);
+    default: return launch_rng<OFG_RNG_REFERENCE>(a, stream);
+    }
+}
+
+hipError_t launch_reduce(const OptDev *opts, const int64_t *item_first, const int32_t *item_count,
+                         const Partial *partials, OptOut *out, int32_t n_opts,
+                         hipStream_t stream) {
+    if (n_opts <= 0) return hipSuccess;
+    const int waves_per_block = 4;
+    const int grid = (n_opts + waves_per_block - 1) / waves_per_block;
+    reduce_kernel<<<grid, 256, 0, stream>>>(opts, item_first, item_count, partials, out, n_opts);
+    return hipGetLastError();
+}
+
+int persistent_grid(int block, int rng_mode, int device) {
+    const void *k = rng_mode == OFG_RNG_EXTERNAL ? kernel_ptr<OFG_RNG_EXTERNAL>(block)
+                    : rng_mode == OFG_RNG_FAST   ? kernel_ptr<OFG_RNG_FAST>(block)
+                                                 : kernel_ptr<OFG_RNG_REFERENCE>(block);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    return per_cu * cus;
+}
+
+}  // namespace ofg

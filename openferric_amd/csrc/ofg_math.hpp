@@ -1,0 +1,194 @@
+// Device math for the Monte Carlo path kernels (gfx950, FP64 on the VALU).
+//
+// Everything here is __host__ __device__ so the same code can be exercised on
+// the CPU by the unit tests (tests/test_device_math_cpu.py builds a host-only
+// shim of this header).
+//
+// Contents:
+//   * SplitMix64 / xoshiro256++ streams seeded exactly as the reference
+//     (src/math/fast_rng.rs:35-73, 181-214) so a path's draws are the
+//     reference's draws.
+//   * Acklam inverse normal CDF (src/math/fast_norm.rs:53-139), restated with
+//     the two tails merged into one branch (bitwise the same values: the upper
+//     tail is the negated lower-tail formula applied to 1 - p).
+//   * exp: Cody-Waite reduction by ln2 into |r| <= ln2/2 and a degree-11
+//     polynomial (Chebyshev-node interpolant of e^r, max relative error
+//     1.7e-17 in exact arithmetic), scaled with ldexp.
+//   * Box-Muller pair generator for the fast RNG mode.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OFG_HD __host__ __device__ __forceinline__
+
+namespace ofg {
+
+// ---------------------------------------------------------------- RNG -----
+
+OFG_HD uint64_t splitmix64_next(uint64_t &state) {
+    state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+OFG_HD uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+// Per-path xoshiro256++ stream (fast_rng.rs:30-73). State lives in 8 VGPRs.
+struct Xoshiro256pp {
+    uint64_t s0, s1, s2, s3;
+
+    // seed_from_u64 (fast_rng.rs:35-48): four SplitMix64 outputs; an all-zero
+    // state is replaced by s0 = 1.
+    OFG_HD void seed(uint64_t seed) {
+        uint64_t sm = seed;
+        s0 = splitmix64_next(sm);
+        s1 = splitmix64_next(sm);
+        s2 = splitmix64_next(sm);
+        s3 = splitmix64_next(sm);
+        if ((s0 | s1 | s2 | s3) == 0) s0 = 1;
+    }
+
+    OFG_HD uint64_t next_u64() {
+        const uint64_t result = rotl64(s0 + s3, 23) + s0;
+        const uint64_t t = s1 << 17;
+        s2 ^= s0;
+        s3 ^= s1;
+        s1 ^= s2;
+        s0 ^= s3;
+        s2 ^= t;
+        s3 = rotl64(s3, 45);
+        return result;
+    }
+
+    // next_f64 (fast_rng.rs:69-73): (x >> 11) * 2^-53, exact in double.
+    OFG_HD double next_f64() {
+        const uint64_t x = next_u64() >> 11;
+        return (double)x * (1.0 / 9007199254740992.0);
+    }
+};
+
+// stream_seed (fast_rng.rs:202-205), wrapping.
+OFG_HD uint64_t stream_seed(uint64_t base, uint64_t index) { return base + index * 7919ull; }
+
+// uniform_open01 (fast_rng.rs:218-221): clamp to [eps, 1 - eps].
+OFG_HD double uniform_open01(double u) {
+    const double lo = 2.220446049250313e-16;
+    const double hi = 1.0 - 2.220446049250313e-16;
+    u = u < lo ? lo : u;
+    return u > hi ? hi : u;
+}
+
+// ------------------------------------------------------- elementary math --
+
+// exp(x) for finite x in the double range. |x| well inside +-700 for all
+// path increments; outside that ldexp saturates to 0 / inf as exp does.
+OFG_HD double exp_fp64(double x) {
+    const double LOG2E = 1.4426950408889634;
+    const double LN2_HI = 6.93147180369123816490e-01;  // 0x3fe62e42fee00000
+    const double LN2_LO = 1.90821492927058770002e-10;  // 0x3dea39ef35793c76
+    const double k = rint(x * LOG2E);
+    double r = fma(-k, LN2_HI, x);
+    r = fma(-k, LN2_LO, r);
+    double p = 2.5110037605963777e-08;
+    p = fma(p, r, 2.763263963904103e-07);
+    p = fma(p, r, 2.755724091857897e-06);
+    p = fma(p, r, 2.4801485482328494e-05);
+    p = fma(p, r, 0.00019841269890047113);
+    p = fma(p, r, 0.0013888888952314775);
+    p = fma(p, r, 0.008333333333319601);
+    p = fma(p, r, 0.0416666666664881);
+    p = fma(p, r, 0.1666666666666668);
+    p = fma(p, r, 0.5000000000000019);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
+// --------------------------------------------------- inverse normal CDF --
+
+// Acklam coefficients as published (identical to fast_norm.rs:66-94).
+#define OFG_AK_A0 -3.969683028665376e1
+#define OFG_AK_A1 2.209460984245205e2
+#define OFG_AK_A2 -2.759285104469687e2
+#define OFG_AK_A3 1.383577518672690e2
+#define OFG_AK_A4 -3.066479806614716e1
+#define OFG_AK_A5 2.506628277459239
+#define OFG_AK_B0 -5.447609879822406e1
+#define OFG_AK_B1 1.615858368580409e2
+#define OFG_AK_B2 -1.556989798598866e2
+#define OFG_AK_B3 6.680131188771972e1
+#define OFG_AK_B4 -1.328068155288572e1
+#define OFG_AK_C0 -7.784894002430293e-3
+#define OFG_AK_C1 -3.223964580411365e-1
+#define OFG_AK_C2 -2.400758277161838
+#define OFG_AK_C3 -2.549732539343734
+#define OFG_AK_C4 4.374664141464968
+#define OFG_AK_C5 2.938163982698783
+#define OFG_AK_D0 7.784695709041462e-3
+#define OFG_AK_D1 3.224671290700398e-1
+#define OFG_AK_D2 2.445134137142996
+#define OFG_AK_D3 3.754408661907416
+
+// Acklam on p already clamped to [eps, 1 - eps] by uniform_open01, so the
+// NaN / 0 / 1 guards of the reference never fire on this path.
+OFG_HD double acklam_inv_cdf(double p) {
+    const double P_LOW = 0.02425;
+    const double P_HIGH = 1.0 - P_LOW;
+    double z;
+    if (p < P_LOW || p > P_HIGH) {
+        const bool lower = p < P_LOW;
+        const double pt = lower ? p : 1.0 - p;
+        const double q = sqrt(-2.0 * log(pt));
+        double num = fma(OFG_AK_C0, q, OFG_AK_C1);
+        num = fma(num, q, OFG_AK_C2);
+        num = fma(num, q, OFG_AK_C3);
+        num = fma(num, q, OFG_AK_C4);
+        num = fma(num, q, OFG_AK_C5);
+        double den = fma(OFG_AK_D0, q, OFG_AK_D1);
+        den = fma(den, q, OFG_AK_D2);
+        den = fma(den, q, OFG_AK_D3);
+        den = fma(den, q, 1.0);
+        const double r = num / den;
+        z = lower ? r : -r;
+    } else {
+        const double q = p - 0.5;
+        const double r = q * q;
+        double num = fma(OFG_AK_A0, r, OFG_AK_A1);
+        num = fma(num, r, OFG_AK_A2);
+        num = fma(num, r, OFG_AK_A3);
+        num = fma(num, r, OFG_AK_A4);
+        num = fma(num, r, OFG_AK_A5);
+        double den = fma(OFG_AK_B0, r, OFG_AK_B1);
+        den = fma(den, r, OFG_AK_B2);
+        den = fma(den, r, OFG_AK_B3);
+        den = fma(den, r, OFG_AK_B4);
+        den = fma(den, r, 1.0);
+        z = num * q / den;
+    }
+    return z;
+}
+
+// sample_standard_normal (fast_rng.rs:223-226) on a xoshiro stream.
+OFG_HD double reference_normal(Xoshiro256pp &rng) {
+    return acklam_inv_cdf(uniform_open01(rng.next_f64()));
+}
+
+// -------------------------------------------------------- Box-Muller ----
+
+// Two independent standard normals from two 64-bit words:
+//   u1 in (0, 1] from 52 bits of w1, angle 2*pi*u2 from 52 bits of w2.
+OFG_HD void box_muller(uint64_t w1, uint64_t w2, double &z0, double &z1) {
+    // u1 = 1 - [0,1) grid of 2^-52 steps, so u1 in (0, 1] and log(u1) finite.
+    const double u1 = 1.0 - (double)(w1 >> 12) * (1.0 / 4503599627370496.0);
+    const double u2 = (double)(w2 >> 12) * (1.0 / 4503599627370496.0);
+    const double rad = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    z0 = rad * c;
+    z1 = rad * s;
+}
+
+}  // namespace ofg

@@ -1,0 +1,393 @@
+"""GPU pricing engines behind the reference's PricingEngine contract.
+
+`GpuMonteCarloPricingEngine` is the drop-in for MonteCarloPricingEngine
+(src/engines/monte_carlo/mc_engine.rs:587-670, 871-988): same constructor
+(num_paths, num_steps, seed), same builders, `price(instrument, market)`
+returning PricingResult and `price_with_greeks_aad`. Every call goes through
+the C ABI of libopenferric_amd.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .core import (AsianOption, Averaging, BarrierDirection, BarrierOption, BarrierStyle,
+                   ConvergenceFailure, Diagnostics, ExerciseStyle, Greeks, Heston, InvalidInput,
+                   Market, MarketDataMissing, NumericalError, OptionType, PricingResult,
+                   StrikeType, VanillaOption)
+
+
+class VarianceReduction(enum.Enum):
+    """mc_engine.rs:31-41"""
+    None_ = 0
+    Antithetic = 1
+    ControlVariate = 2
+
+
+class RngMode(enum.Enum):
+    Reference = _lib.RNG_REFERENCE
+    External = _lib.RNG_EXTERNAL
+    Fast = _lib.RNG_FAST
+
+
+_ERRORS = {
+    _lib.OFG_INVALID_INPUT: InvalidInput,
+    _lib.OFG_CONVERGENCE_FAILURE: ConvergenceFailure,
+    _lib.OFG_MARKET_DATA_MISSING: MarketDataMissing,
+    _lib.OFG_NUMERICAL_ERROR: NumericalError,
+    _lib.OFG_DEVICE_ERROR: _lib.DeviceError,
+}
+
+
+def _raise(rc: int):
+    if rc != _lib.OFG_OK:
+        raise _ERRORS.get(rc, NumericalError)(_lib.last_error())
+
+
+class _Keep:
+    """Keeps numpy arrays referenced by ctypes structs alive."""
+
+    def __init__(self):
+        self.refs = []
+
+    def arr(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        self.refs.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def option_struct(instrument, market: Market, num_paths: int, num_steps: int, seed: int,
+                  keep: _Keep, heston: Optional[Heston] = None) -> _lib.OfgOption:
+    """Flattens (instrument, market[, Heston]) into the C ABI's ofg_option."""
+    o = _lib.OfgOption()
+    o.model = _lib.MODEL_HESTON if heston is not None else _lib.MODEL_GBM
+    o.option_type = _lib.CALL if instrument.option_type is OptionType.Call else _lib.PUT
+    o.spot, o.strike, o.expiry = market.spot, instrument.strike, instrument.expiry
+    o.rate = market.rate
+    # drift mu = rate - dividend_yield (mc_engine.rs:916); a Heston model's own
+    # mu is honoured by shifting the carry.
+    o.div_yield = market.dividend_yield if heston is None else market.rate - heston.mu
+    if isinstance(instrument, AsianOption):
+        ref_strike = instrument.strike if instrument.asian.strike_type is StrikeType.Fixed \
+            else market.spot
+    else:
+        ref_strike = instrument.strike
+    o.vol = market.vol_for(max(ref_strike, 1e-12), instrument.expiry)
+    if heston is not None:
+        o.kappa, o.theta, o.xi, o.rho, o.v0 = heston.kappa, heston.theta, heston.xi, \
+            heston.rho, heston.v0
+    if isinstance(instrument, VanillaOption):
+        o.payoff = _lib.PAYOFF_VANILLA
+    elif isinstance(instrument, AsianOption):
+        o.payoff = _lib.PAYOFF_ASIAN
+        o.averaging = _lib.ARITHMETIC if instrument.asian.averaging is Averaging.Arithmetic \
+            else _lib.GEOMETRIC
+        o.strike_type = _lib.FIXED if instrument.asian.strike_type is StrikeType.Fixed \
+            else _lib.FLOATING
+        obs = list(instrument.asian.observation_times)
+        o.n_obs = len(obs)
+        o.obs_times = keep.arr(obs) if obs else None
+    elif isinstance(instrument, BarrierOption):
+        o.payoff = _lib.PAYOFF_BARRIER
+        o.barrier_dir = _lib.UP if instrument.barrier.direction is BarrierDirection.Up \
+            else _lib.DOWN
+        o.barrier_style = _lib.IN if instrument.barrier.style is BarrierStyle.In else _lib.OUT
+        o.barrier_level = instrument.barrier.level
+        o.rebate = instrument.barrier.rebate
+    else:
+        raise InvalidInput(f"unsupported instrument {type(instrument).__name__}")
+    o.num_paths, o.num_steps, o.seed = int(num_paths), int(num_steps), int(seed) & (2**64 - 1)
+    return o
+
+
+def engine_cfg(antithetic=False, control_variate=False, rng_mode=RngMode.Reference,
+               greek_mode=_lib.GREEKS_NONE, bump_rel=1e-4, normals1=None, normals2=None,
+               keep: Optional[_Keep] = None, tune_block=0, tune_item_steps=0) -> _lib.OfgEngineCfg:
+    c = _lib.OfgEngineCfg()
+    c.antithetic = 1 if antithetic else 0
+    c.control_variate = 1 if control_variate else 0
+    c.rng_mode = rng_mode.value if isinstance(rng_mode, RngMode) else int(rng_mode)
+    c.greek_mode = greek_mode
+    c.bump_rel = bump_rel
+    keep = keep or _Keep()
+    c.normals1 = keep.arr(normals1) if normals1 is not None else None
+    c.normals2 = keep.arr(normals2) if normals2 is not None else None
+    c.tune_block, c.tune_item_steps = tune_block, tune_item_steps
+    return c
+
+
+def price_raw(opts: Sequence[_lib.OfgOption], cfg: _lib.OfgEngineCfg, device: int = 0,
+              greeks: bool = False) -> List[_lib.OfgResult]:
+    """Calls ofg_price (or ofg_price_greeks) on an array of ofg_option."""
+    L = _lib.load()
+    ctx = _lib.context(device)
+    arr = (_lib.OfgOption * len(opts))(*opts)
+    out = (_lib.OfgResult * len(opts))()
+    fn = L.ofg_price_greeks if greeks else L.ofg_price
+    _raise(fn(ctx, arr, len(opts), C.byref(cfg), out))
+    return list(out)
+
+
+def sample_payoffs(opt: _lib.OfgOption, cfg: _lib.OfgEngineCfg, antithetic: bool,
+                   device: int = 0, want_y: bool = True) -> Tuple[np.ndarray, Optional[np.ndarray]]:
+    """Per-sample (x, y) of one option via ofg_sample_payoffs."""
+    L = _lib.load()
+    ctx = _lib.context(device)
+    samples = (opt.num_paths + 1) // 2 if antithetic else opt.num_paths
+    xs = np.zeros(samples)
+    ys = np.zeros(samples) if want_y else None
+    _raise(L.ofg_sample_payoffs(ctx, C.byref(opt), C.byref(cfg),
+                                xs.ctypes.data_as(C.POINTER(C.c_double)),
+                                ys.ctypes.data_as(C.POINTER(C.c_double)) if want_y else None))
+    return xs, ys
+
+
+def _to_result(r: _lib.OfgResult, with_greeks: bool, heston: bool) -> PricingResult:
+    diag = Diagnostics()
+    if r.num_paths or r.num_steps:
+        diag.insert("num_paths", r.num_paths)
+        diag.insert("num_steps", r.num_steps)
+        diag.insert("vol", r.vol)
+    res = PricingResult(price=r.price, stderr=r.stderr_ if r.has_stderr else None,
+                        diagnostics=diag, kernel_ms=r.kernel_ms)
+    if with_greeks and r.has_greeks:
+        res.greeks = Greeks(r.delta, r.gamma, r.vega, r.theta, r.rho)
+        if heston:
+            res.heston_gradient = dict(zip(["s0", "v0", "kappa", "theta", "xi", "rho"],
+                                           list(r.heston_grad)))
+    return res
+
+
+class GpuMonteCarloPricingEngine:
+    """Drop-in for MonteCarloPricingEngine (mc_engine.rs:587-670, 871-988)."""
+
+    def __init__(self, num_paths: int, num_steps: int, seed: int):
+        self.num_paths = num_paths
+        self.num_steps = num_steps
+        self.seed = seed
+        self.variance_reduction = VarianceReduction.None_
+        self.rng_mode = RngMode.Reference
+        self.device = 0
+
+    def with_variance_reduction(self, vr: VarianceReduction) -> "GpuMonteCarloPricingEngine":
+        self.variance_reduction = vr
+        return self
+
+    def with_seed(self, seed: int) -> "GpuMonteCarloPricingEngine":
+        self.seed = seed
+        return self
+
+    def with_rng_mode(self, mode: RngMode) -> "GpuMonteCarloPricingEngine":
+        self.rng_mode = mode
+        return self
+
+    def with_device(self, device: int) -> "GpuMonteCarloPricingEngine":
+        self.device = device
+        return self
+
+    def _validate(self, instrument):
+        instrument.validate()
+        if isinstance(instrument, VanillaOption) and instrument.exercise is not ExerciseStyle.European:
+            # the MC payoff ignores the exercise style like the reference's
+            # MonteCarloInstrument impl (mc_engine.rs:441-456)
+            pass
+
+    def price(self, instrument, market: Market) -> PricingResult:
+        self._validate(instrument)
+        keep = _Keep()
+        o = option_struct(instrument, market, self.num_paths, self.num_steps, self.seed, keep)
+        cfg = engine_cfg(antithetic=self.variance_reduction is VarianceReduction.Antithetic,
+                         control_variate=self.variance_reduction is VarianceReduction.ControlVariate,
+                         rng_mode=self.rng_mode, keep=keep)
+        r = price_raw([o], cfg, self.device)[0]
+        return _to_result(r, False, False)
+
+    def price_with_greeks_aad(self, instrument, market: Market) -> PricingResult:
+        """Adjoint Greeks for any supported payoff (the reference only has
+        them for VanillaOption, mc_engine.rs:977-987)."""
+        self._validate(instrument)
+        keep = _Keep()
+        o = option_struct(instrument, market, self.num_paths, self.num_steps, self.seed, keep)
+        cfg = engine_cfg(antithetic=self.variance_reduction is VarianceReduction.Antithetic,
+                         rng_mode=self.rng_mode, keep=keep)
+        r = price_raw([o], cfg, self.device, greeks=True)[0]
+        return _to_result(r, True, False)
+
+
+class GpuArithmeticAsianMC:
+    """Drop-in for ArithmeticAsianMC (mc_engine.rs:672-869): antithetic always,
+    geometric-Asian control variate on by default."""
+
+    def __init__(self, paths: int, steps: int, seed: int):
+        self.paths, self.steps, self.seed = paths, steps, seed
+        self.control_variate = True
+        self.rng_mode = RngMode.Reference
+
+    def with_control_variate(self, cv: bool) -> "GpuArithmeticAsianMC":
+        self.control_variate = cv
+        return self
+
+    def with_rng_mode(self, mode: RngMode) -> "GpuArithmeticAsianMC":
+        self.rng_mode = mode
+        return self
+
+    def price(self, instrument: AsianOption, market: Market) -> PricingResult:
+        instrument.validate()
+        if instrument.asian.averaging is not Averaging.Arithmetic:
+            raise InvalidInput("ArithmeticAsianMC requires Averaging::Arithmetic")
+        if instrument.asian.strike_type is not StrikeType.Fixed:
+            raise InvalidInput("ArithmeticAsianMC currently supports StrikeType::Fixed only")
+        if self.paths == 0:
+            raise InvalidInput("arithmetic asian mc paths must be > 0")
+        if self.steps == 0:
+            raise InvalidInput("arithmetic asian mc steps must be > 0")
+        vol = market.vol_for(instrument.strike, instrument.expiry)
+        if not (vol > 0.0) or vol != vol or vol == float("inf"):
+            raise InvalidInput("market volatility must be finite and > 0")
+        keep = _Keep()
+        o = option_struct(instrument, market, self.paths, self.steps, self.seed, keep)
+        cfg = engine_cfg(antithetic=True, control_variate=self.control_variate,
+                         rng_mode=self.rng_mode, keep=keep)
+        r = price_raw([o], cfg)[0]
+        return _to_result(r, False, False)
+
+
+class HestonMonteCarloPricingEngine:
+    """PricingEngine for Heston paths: MonteCarloEngine::run with
+    HestonPathGenerator (simulation.rs:79-117, 182-348) behind the engine
+    contract. The Heston model's mu is the drift; market.rate discounts."""
+
+    def __init__(self, model: Heston, num_paths: int, num_steps: int, seed: int):
+        self.model = model
+        self.num_paths, self.num_steps, self.seed = num_paths, num_steps, seed
+        self.antithetic = False
+        self.rng_mode = RngMode.Reference
+
+    def with_antithetic(self, a: bool) -> "HestonMonteCarloPricingEngine":
+        self.antithetic = a
+        return self
+
+    def with_rng_mode(self, mode: RngMode) -> "HestonMonteCarloPricingEngine":
+        self.rng_mode = mode
+        return self
+
+    def price(self, instrument, market: Market) -> PricingResult:
+        instrument.validate()
+        keep = _Keep()
+        o = option_struct(instrument, market, self.num_paths, self.num_steps, self.seed, keep,
+                          heston=self.model)
+        cfg = engine_cfg(antithetic=self.antithetic, rng_mode=self.rng_mode, keep=keep)
+        return _to_result(price_raw([o], cfg)[0], False, True)
+
+    def price_with_greeks_aad(self, instrument, market: Market) -> PricingResult:
+        instrument.validate()
+        keep = _Keep()
+        o = option_struct(instrument, market, self.num_paths, self.num_steps, self.seed, keep,
+                          heston=self.model)
+        cfg = engine_cfg(antithetic=self.antithetic, rng_mode=self.rng_mode, keep=keep)
+        return _to_result(price_raw([o], cfg, greeks=True)[0], True, True)
+
+
+@dataclass
+class HestonAadConfig:
+    """mc_aad.rs:221-237"""
+    num_paths: int
+    num_steps: int
+    seed: int
+
+
+def heston_price_delta_aad(option_type: OptionType, strike: float, maturity: float, spot: float,
+                           rate: float, model: Heston, config: HestonAadConfig) -> Tuple[float, float]:
+    """GPU counterpart of heston_price_delta_aad (mc_aad.rs:279-342): price
+    and dPrice/dS0 on the reference draws (no antithetic)."""
+    if config.num_paths == 0:
+        raise InvalidInput("num_paths must be > 0")
+    if config.num_steps == 0:
+        raise InvalidInput("num_steps must be > 0")
+    if maturity <= 0.0:
+        intrinsic = max(spot - strike, 0.0) if option_type is OptionType.Call \
+            else max(strike - spot, 0.0)
+        return intrinsic, 0.0
+    if not model.validate():
+        raise InvalidInput("invalid Heston parameters")
+    eng = HestonMonteCarloPricingEngine(model, config.num_paths, config.num_steps, config.seed)
+    mkt = Market(spot, rate, 0.0, 0.2)
+    res = eng.price_with_greeks_aad(VanillaOption(option_type, strike, maturity), mkt)
+    return res.price, res.greeks.delta
+
+
+def mc_european_pathwise_aad(engine: GpuMonteCarloPricingEngine, instrument: VanillaOption,
+                             market: Market) -> PricingResult:
+    """GPU counterpart of mc_european_pathwise_aad (mc_aad.rs:75-219)."""
+    instrument.validate()
+    if instrument.exercise is not ExerciseStyle.European:
+        raise InvalidInput("mc pathwise AAD supports European exercise only")
+    if engine.num_paths == 0:
+        raise InvalidInput("num_paths must be > 0")
+    if engine.num_steps == 0:
+        raise InvalidInput("num_steps must be > 0")
+    if instrument.expiry <= 0.0:
+        intrinsic = max(market.spot - instrument.strike, 0.0) \
+            if instrument.option_type is OptionType.Call else max(instrument.strike - market.spot, 0.0)
+        return PricingResult(intrinsic, 0.0, Greeks(0.0, 0.0, 0.0, 0.0, 0.0))
+    vol = market.vol_for(instrument.strike, instrument.expiry)
+    if not (vol > 0.0) or vol == float("inf"):
+        raise InvalidInput("market volatility must be finite and > 0")
+    return engine.price_with_greeks_aad(instrument, market)
+
+
+class BatchPricer:
+    """Many contracts priced in one persistent-kernel launch, replayed through
+    a captured hipGraph (ofg_batch_*)."""
+
+    def __init__(self, opts: Sequence[_lib.OfgOption], cfg: _lib.OfgEngineCfg, use_graph=True,
+                 device: int = 0):
+        self._L = _lib.load()
+        self._ctx = _lib.context(device)
+        self.n = len(opts)
+        self._arr = (_lib.OfgOption * self.n)(*opts)
+        self._cfg = cfg
+        self._b = self._L.ofg_batch_create(self._ctx, self._arr, self.n, C.byref(cfg),
+                                           1 if use_graph else 0)
+        if not self._b:
+            raise InvalidInput(_lib.last_error())
+
+    def run(self) -> float:
+        ms = C.c_double()
+        _raise(self._L.ofg_batch_run(self._b, C.byref(ms)))
+        return ms.value
+
+    def pricing_kernel_ms(self) -> float:
+        return self._L.ofg_batch_pricing_kernel_ms(self._b)
+
+    def results(self) -> List[_lib.OfgResult]:
+        out = (_lib.OfgResult * self.n)()
+        _raise(self._L.ofg_batch_results(self._b, out))
+        return list(out)
+
+    def close(self):
+        if self._b:
+            self._L.ofg_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(opts: Sequence[_lib.OfgOption], cfg: _lib.OfgEngineCfg, cache_path: Optional[str] = None,
+             device: int = 0) -> Tuple[int, int]:
+    L = _lib.load()
+    ctx = _lib.context(device)
+    arr = (_lib.OfgOption * len(opts))(*opts)
+    b, it = C.c_int32(), C.c_int32()
+    _raise(L.ofg_autotune(ctx, arr, len(opts), C.byref(cfg),
+                          cache_path.encode() if cache_path else None, C.byref(b), C.byref(it)))
+    return b.value, it.value

@@ -1,0 +1,141 @@
+/*
+ * CPU oracle for the OpenFerric Monte Carlo pricing path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is a plain-C restatement of the
+ * reference Rust algorithm (rosssaunders/openferric) used as the checker for
+ * the HIP product path and as the timed CPU baseline. Only tests/, the smoke
+ * check and the benchmark's cpu_baseline leg may load it; the product library
+ * (openferric_amd) never links, calls or falls back to it.
+ *
+ * Every function cites the reference file:line it restates. Operation order,
+ * fused multiply-adds (Rust `mul_add` -> C `fma`) and libm calls follow the
+ * reference; the file is compiled with -ffp-contract=off so no other
+ * contraction happens.
+ *
+ * Parity status per function (how each is pinned, see tests/test_oracle_*.py):
+ *   or_splitmix64_next, or_xoshiro_*   known-answer vectors + hand-derived
+ *                                      unit-state transitions
+ *   or_acklam_inv_cdf                  mpmath/scipy high-precision inverse
+ *                                      (rel err <= 1.15e-9), branch points
+ *   or_hart_norm_cdf                   A&S table (reference fast_norm tests)
+ *   or_gbm_generate / or_heston_*      closed forms (zero noise, xi = 0)
+ *   or_mc_run                          BS / QuantLib / Kemna-Turnbull
+ *                                      statistical checks, exact small sums
+ *   or_bs_*                            QuantLib Haug fixture table
+ *   or_geometric_asian_discrete        QuantLib 5.3425606635 fixture
+ *   or_mc_european_pathwise_aad        BS Greeks at the reference tolerances
+ *   or_heston_price_delta_aad          CRN central bump (reference test)
+ */
+#ifndef OPENFERRIC_ORACLE_H
+#define OPENFERRIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- numerics ------------------------------------------------------- */
+uint64_t or_splitmix64_next(uint64_t *state);
+void or_xoshiro_seed(uint64_t seed, uint64_t s[4]);
+uint64_t or_xoshiro_next_u64(uint64_t s[4]);
+double or_xoshiro_next_f64(uint64_t s[4]);
+uint64_t or_stream_seed(uint64_t base, uint64_t index);
+double or_uniform_open01(double u);
+double or_acklam_inv_cdf(double p);
+double or_hart_norm_cdf(double x);
+double or_fast_norm_pdf(double x);
+
+/* ---- option / model description -------------------------------------- */
+enum { OR_GBM = 0, OR_HESTON = 1 };
+enum { OR_VANILLA = 0, OR_ASIAN = 1, OR_BARRIER = 2 };
+enum { OR_CALL = 0, OR_PUT = 1 };
+enum { OR_ARITH = 0, OR_GEO = 1 };
+enum { OR_FIXED = 0, OR_FLOATING = 1 };
+enum { OR_UP = 0, OR_DOWN = 1 };
+enum { OR_IN = 0, OR_OUT = 1 };
+
+typedef struct {
+    int32_t model;        /* OR_GBM | OR_HESTON */
+    int32_t payoff;       /* OR_VANILLA | OR_ASIAN | OR_BARRIER */
+    int32_t option_type;  /* OR_CALL | OR_PUT */
+    int32_t averaging;    /* Asian: OR_ARITH | OR_GEO */
+    int32_t strike_type;  /* Asian: OR_FIXED | OR_FLOATING */
+    int32_t barrier_dir;  /* OR_UP | OR_DOWN */
+    int32_t barrier_style;/* OR_IN | OR_OUT */
+    int32_t n_obs;        /* Asian observation count */
+    double spot, strike, expiry, rate, div_yield, vol;
+    double kappa, theta, xi, rho, v0;
+    double barrier_level, rebate;
+    const double *obs_times;
+} or_option;
+
+typedef struct {
+    int64_t num_paths;
+    int64_t num_steps;
+    uint64_t seed;
+    int32_t antithetic;
+    int32_t control_variate;   /* 0 none; 1 = instrument's built-in CV */
+    double cv_expected;        /* E[y] undiscounted, used when control_variate */
+    int32_t cv_kind;           /* 0 terminal vanilla payoff; 1 geometric Asian */
+    int32_t pad;
+} or_cfg;
+
+/* Generators (simulation.rs:58-71, :98-116) */
+void or_gbm_generate(double mu, double sigma, double s0, double maturity,
+                     int64_t steps, const double *z, double *out);
+void or_heston_generate(double mu, double kappa, double theta, double xi,
+                        double rho, double v0, double s0, double maturity,
+                        int64_t steps, const double *z1, const double *z2,
+                        double *out);
+void or_heston_step_euler(double mu, double kappa, double theta, double xi,
+                          double rho, double s, double v, double dt,
+                          double z1, double z2, double *s_next, double *v_next);
+
+/* Payoff of one path of length steps+1 (mc_engine.rs:59-121, 441-584). */
+double or_payoff(const or_option *o, const double *path, int64_t len);
+/* Control-variate evaluator on one path (mc_engine.rs:481-483, 573-581). */
+double or_cv_eval(const or_option *o, int32_t cv_kind, const double *path, int64_t len);
+
+/* Fill the normals the reference draws for sample i (simulation.rs:215-225):
+ * z1[j], and z2[j] interleaved when streams == 2. */
+void or_sample_normals(uint64_t base_seed, int64_t sample, int64_t steps,
+                       int32_t streams, double *z1, double *z2);
+
+/* MonteCarloEngine::run (simulation.rs:182-348) with the instrument wiring of
+ * MonteCarloPricingEngine::price (mc_engine.rs:875-975). nthreads <= 1 runs the
+ * serial fold; larger values split samples into contiguous per-thread folds
+ * that are then tree-reduced (the rayon fold/reduce structure). xs/ys, when
+ * non-NULL, receive the per-sample (antithetic-averaged) x and y values.
+ * normals1/normals2, when non-NULL, replace the RNG with caller-supplied draws
+ * laid out [sample][step]. Returns 0 on success. */
+int or_mc_run(const or_option *o, const or_cfg *cfg, int32_t nthreads,
+              const double *normals1, const double *normals2,
+              double *price, double *stderr_out, double *xs, double *ys);
+
+/* Closed forms */
+double or_bs_price(int32_t option_type, double spot, double strike, double rate,
+                   double q, double vol, double expiry);
+double or_bs_price_exact(int32_t option_type, double spot, double strike,
+                         double rate, double q, double vol, double expiry);
+void or_bs_greeks(int32_t option_type, double spot, double strike, double rate,
+                  double q, double vol, double expiry, double out[5]);
+double or_geometric_asian_discrete(int32_t option_type, double s0, double k,
+                                   double r, double q, double sigma,
+                                   const double *obs, int64_t n_obs);
+/* Expected value of the built-in control variate, undiscounted. */
+double or_builtin_cv_expected(const or_option *o, int32_t *cv_kind);
+
+/* mc_european_pathwise_aad (mc_aad.rs:75-219). out: price, stderr, delta,
+ * vega, theta, rho. Returns 0 on success. */
+int or_mc_european_pathwise_aad(const or_option *o, int64_t num_paths,
+                                int64_t num_steps, uint64_t seed,
+                                int32_t antithetic, double out[6]);
+/* heston_price_delta_aad (mc_aad.rs:279-342). out: price, delta. */
+int or_heston_price_delta_aad(const or_option *o, int64_t num_paths,
+                              int64_t num_steps, uint64_t seed, double out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,215 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the timed
+CPU baseline. Only tests/, the smoke check and the benchmark's cpu_baseline
+leg import this module. The product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+GBM, HESTON = 0, 1
+VANILLA, ASIAN, BARRIER = 0, 1, 2
+CALL, PUT = 0, 1
+ARITH, GEO = 0, 1
+FIXED, FLOATING = 0, 1
+UP, DOWN = 0, 1
+IN, OUT = 0, 1
+
+
+class OrOption(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32), ("payoff", C.c_int32), ("option_type", C.c_int32),
+        ("averaging", C.c_int32), ("strike_type", C.c_int32),
+        ("barrier_dir", C.c_int32), ("barrier_style", C.c_int32), ("n_obs", C.c_int32),
+        ("spot", C.c_double), ("strike", C.c_double), ("expiry", C.c_double),
+        ("rate", C.c_double), ("div_yield", C.c_double), ("vol", C.c_double),
+        ("kappa", C.c_double), ("theta", C.c_double), ("xi", C.c_double),
+        ("rho", C.c_double), ("v0", C.c_double),
+        ("barrier_level", C.c_double), ("rebate", C.c_double),
+        ("obs_times", C.POINTER(C.c_double)),
+    ]
+
+
+class OrCfg(C.Structure):
+    _fields_ = [
+        ("num_paths", C.c_int64), ("num_steps", C.c_int64), ("seed", C.c_uint64),
+        ("antithetic", C.c_int32), ("control_variate", C.c_int32),
+        ("cv_expected", C.c_double), ("cv_kind", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with make (gcc); returns the library path."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"],
+                       check=True, capture_output=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        d, i32, i64, u64 = C.c_double, C.c_int32, C.c_int64, C.c_uint64
+        pd = C.POINTER(C.c_double)
+        pu64 = C.POINTER(C.c_uint64)
+        L.or_splitmix64_next.argtypes = [pu64]; L.or_splitmix64_next.restype = u64
+        L.or_xoshiro_seed.argtypes = [u64, pu64]; L.or_xoshiro_seed.restype = None
+        L.or_xoshiro_next_u64.argtypes = [pu64]; L.or_xoshiro_next_u64.restype = u64
+        L.or_xoshiro_next_f64.argtypes = [pu64]; L.or_xoshiro_next_f64.restype = d
+        L.or_stream_seed.argtypes = [u64, u64]; L.or_stream_seed.restype = u64
+        L.or_uniform_open01.argtypes = [d]; L.or_uniform_open01.restype = d
+        L.or_acklam_inv_cdf.argtypes = [d]; L.or_acklam_inv_cdf.restype = d
+        L.or_hart_norm_cdf.argtypes = [d]; L.or_hart_norm_cdf.restype = d
+        L.or_fast_norm_pdf.argtypes = [d]; L.or_fast_norm_pdf.restype = d
+        L.or_gbm_generate.argtypes = [d, d, d, d, i64, pd, pd]; L.or_gbm_generate.restype = None
+        L.or_heston_generate.argtypes = [d, d, d, d, d, d, d, d, i64, pd, pd, pd]
+        L.or_heston_generate.restype = None
+        L.or_heston_step_euler.argtypes = [d, d, d, d, d, d, d, d, d, d, pd, pd]
+        L.or_heston_step_euler.restype = None
+        L.or_payoff.argtypes = [C.POINTER(OrOption), pd, i64]; L.or_payoff.restype = d
+        L.or_cv_eval.argtypes = [C.POINTER(OrOption), i32, pd, i64]; L.or_cv_eval.restype = d
+        L.or_sample_normals.argtypes = [u64, i64, i64, i32, pd, pd]
+        L.or_sample_normals.restype = None
+        L.or_mc_run.argtypes = [C.POINTER(OrOption), C.POINTER(OrCfg), i32, pd, pd, pd, pd, pd, pd]
+        L.or_mc_run.restype = C.c_int
+        L.or_bs_price.argtypes = [i32, d, d, d, d, d, d]; L.or_bs_price.restype = d
+        L.or_bs_price_exact.argtypes = [i32, d, d, d, d, d, d]; L.or_bs_price_exact.restype = d
+        L.or_bs_greeks.argtypes = [i32, d, d, d, d, d, d, pd]; L.or_bs_greeks.restype = None
+        L.or_geometric_asian_discrete.argtypes = [i32, d, d, d, d, d, pd, i64]
+        L.or_geometric_asian_discrete.restype = d
+        L.or_builtin_cv_expected.argtypes = [C.POINTER(OrOption), C.POINTER(i32)]
+        L.or_builtin_cv_expected.restype = d
+        L.or_mc_european_pathwise_aad.argtypes = [C.POINTER(OrOption), i64, i64, u64, i32, pd]
+        L.or_mc_european_pathwise_aad.restype = C.c_int
+        L.or_heston_price_delta_aad.argtypes = [C.POINTER(OrOption), i64, i64, u64, pd]
+        L.or_heston_price_delta_aad.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_option(model=GBM, payoff=VANILLA, option_type=CALL, spot=100.0, strike=100.0,
+                expiry=1.0, rate=0.05, div_yield=0.0, vol=0.2, kappa=0.0, theta=0.0, xi=0.0,
+                rho=0.0, v0=0.0, averaging=ARITH, strike_type=FIXED, barrier_dir=UP,
+                barrier_style=OUT, barrier_level=0.0, rebate=0.0, obs_times=None):
+    """Builds an OrOption; the returned object keeps obs_times alive."""
+    o = OrOption()
+    o.model, o.payoff, o.option_type = model, payoff, option_type
+    o.averaging, o.strike_type = averaging, strike_type
+    o.barrier_dir, o.barrier_style = barrier_dir, barrier_style
+    o.spot, o.strike, o.expiry, o.rate, o.div_yield, o.vol = spot, strike, expiry, rate, div_yield, vol
+    o.kappa, o.theta, o.xi, o.rho, o.v0 = kappa, theta, xi, rho, v0
+    o.barrier_level, o.rebate = barrier_level, rebate
+    if obs_times is not None:
+        arr = np.ascontiguousarray(obs_times, dtype=np.float64)
+        o._obs = arr
+        o.obs_times = _ptr(arr)
+        o.n_obs = len(arr)
+    else:
+        o.obs_times = None
+        o.n_obs = 0
+    return o
+
+
+def builtin_cv_expected(o):
+    kind = C.c_int32(0)
+    e = lib().or_builtin_cv_expected(C.byref(o), C.byref(kind))
+    return e, kind.value
+
+
+def mc_run(o, num_paths, num_steps, seed, antithetic=False, control_variate=False,
+           nthreads=1, normals1=None, normals2=None, want_samples=False):
+    """MonteCarloEngine::run with the instrument's payoff (and built-in CV).
+
+    Returns (price, stderr) or (price, stderr, xs, ys) with want_samples."""
+    cfg = OrCfg()
+    cfg.num_paths, cfg.num_steps, cfg.seed = num_paths, num_steps, seed
+    cfg.antithetic = 1 if antithetic else 0
+    cfg.control_variate = 0
+    if control_variate:
+        e, kind = builtin_cv_expected(o)
+        if kind >= 0:
+            cfg.control_variate, cfg.cv_expected, cfg.cv_kind = 1, e, kind
+    samples = (num_paths + 1) // 2 if antithetic else num_paths
+    xs = np.zeros(samples) if want_samples else None
+    ys = np.zeros(samples) if want_samples else None
+    n1 = np.ascontiguousarray(normals1, dtype=np.float64) if normals1 is not None else None
+    n2 = np.ascontiguousarray(normals2, dtype=np.float64) if normals2 is not None else None
+    price, se = C.c_double(), C.c_double()
+    rc = lib().or_mc_run(C.byref(o), C.byref(cfg), nthreads, _ptr(n1), _ptr(n2),
+                         C.byref(price), C.byref(se), _ptr(xs), _ptr(ys))
+    if rc != 0:
+        raise ValueError(f"or_mc_run failed rc={rc}")
+    if want_samples:
+        return price.value, se.value, xs, ys
+    return price.value, se.value
+
+
+def sample_normals(seed, sample, steps, streams):
+    z1 = np.zeros(steps)
+    z2 = np.zeros(steps)
+    lib().or_sample_normals(seed, sample, steps, streams, _ptr(z1), _ptr(z2))
+    return z1, z2
+
+
+def normals_matrix(seed, samples, steps, streams):
+    """[sample][step] normals exactly as the reference draws them."""
+    z1 = np.zeros((samples, steps))
+    z2 = np.zeros((samples, steps))
+    for i in range(samples):
+        a, b = sample_normals(seed, i, steps, streams)
+        z1[i] = a
+        z2[i] = b
+    return z1, z2
+
+
+def bs_price(option_type, spot, strike, rate, q, vol, expiry, exact=False):
+    f = lib().or_bs_price_exact if exact else lib().or_bs_price
+    return f(option_type, spot, strike, rate, q, vol, expiry)
+
+
+def bs_greeks(option_type, spot, strike, rate, q, vol, expiry):
+    out = np.zeros(5)
+    lib().or_bs_greeks(option_type, spot, strike, rate, q, vol, expiry, _ptr(out))
+    return dict(zip(["delta", "gamma", "vega", "theta", "rho"], out.tolist()))
+
+
+def geometric_asian_discrete(option_type, s0, k, r, q, sigma, obs):
+    arr = np.ascontiguousarray(obs, dtype=np.float64)
+    return lib().or_geometric_asian_discrete(option_type, s0, k, r, q, sigma, _ptr(arr), len(arr))
+
+
+def mc_european_pathwise_aad(o, num_paths, num_steps, seed, antithetic):
+    out = np.zeros(6)
+    rc = lib().or_mc_european_pathwise_aad(C.byref(o), num_paths, num_steps, seed,
+                                           1 if antithetic else 0, _ptr(out))
+    if rc:
+        raise ValueError("aad failed")
+    return dict(zip(["price", "stderr", "delta", "vega", "theta", "rho"], out.tolist()))
+
+
+def heston_price_delta_aad(o, num_paths, num_steps, seed):
+    out = np.zeros(2)
+    rc = lib().or_heston_price_delta_aad(C.byref(o), num_paths, num_steps, seed, _ptr(out))
+    if rc:
+        raise ValueError("heston aad failed")
+    return float(out[0]), float(out[1])
