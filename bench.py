@@ -53,7 +53,7 @@ def make_workload(n_opts=4096, seed=WORKLOAD_SEED, path_scale=1.0):
                  spot=100.0, strike=float(strike[i]), expiry=T, rate=float(rate[i]), q=0.0,
                  vol=float(sigma[i]), num_steps=n,
                  num_paths=max(64, int(paths[i] * path_scale)),
-                 obs_times=[j * T / n for j in range(1, n + 1)] if asian else None)
+                 obs_times=[T * (j / n) for j in range(1, n + 1)] if asian else None)
         opts.append(d)
     return opts
 

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -31,6 +32,22 @@
 namespace {
 
 thread_local std::string g_last_error;
+
+// OFG_TRACE=1 prints each host-side stage to stderr (flushed), so a run that
+// stalls shows where.
+bool trace_on() {
+    static const bool on = [] {
+        const char *e = std::getenv("OFG_TRACE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+void trace(const char *what) {
+    if (trace_on()) {
+        std::fprintf(stderr, "[ofg] %s\n", what);
+        std::fflush(stderr);
+    }
+}
 
 struct OfgError : std::runtime_error {
     int32_t code;
@@ -215,6 +232,9 @@ struct Plan {
     int64_t total_samples = 0;
     int64_t normals_elems = 0;
     bool any_heston = false;
+    int32_t antithetic = 0;
+    int32_t model_begin[2] = {0, 0};   // order[] range of each model's items
+    int32_t model_end[2] = {0, 0};
 };
 
 struct Tuning {
@@ -244,6 +264,7 @@ struct ofg_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     ofg::LaunchArgs args{};
+    Tuning tune;
     std::vector<ofg::OptOut> host_out;
     std::vector<ofg_result> results;
     double last_ms = 0.0, last_pricing_ms = 0.0;
@@ -421,8 +442,19 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
     }
     p.order.resize(p.items.size());
     std::iota(p.order.begin(), p.order.end(), 0);
-    std::stable_sort(p.order.begin(), p.order.end(),
-                     [&](int32_t a, int32_t b) { return cost[(size_t)a] > cost[(size_t)b]; });
+    auto model_of = [&](int32_t k) { return p.opts[(size_t)p.items[(size_t)k].option].model; };
+    std::stable_sort(p.order.begin(), p.order.end(), [&](int32_t a, int32_t b) {
+        const int ma = model_of(a), mb = model_of(b);
+        if (ma != mb) return ma < mb;
+        return cost[(size_t)a] > cost[(size_t)b];
+    });
+    p.antithetic = cfg.antithetic ? 1 : 0;
+    for (int m = 0; m < 2; ++m) {
+        auto lo = std::find_if(p.order.begin(), p.order.end(), [&](int32_t k) { return model_of(k) == m; });
+        auto hi = std::find_if(lo, p.order.end(), [&](int32_t k) { return model_of(k) != m; });
+        p.model_begin[m] = (int32_t)(lo - p.order.begin());
+        p.model_end[m] = (int32_t)(hi - p.order.begin());
+    }
     return p;
 }
 
@@ -459,6 +491,27 @@ Tuning tuning_for(ofg_ctx *ctx, const std::vector<ofg::OptDev> &opts, const ofg_
     if (cfg.tune_block > 0) t.block = cfg.tune_block;
     if (cfg.tune_item_steps > 0) t.item_steps = cfg.tune_item_steps;
     return t;
+}
+
+// Enqueues the queue reset and one persistent pricing launch per model.
+void enqueue_pricing(const ofg::LaunchArgs &base, const Plan &p, const Tuning &t, int device,
+                     uint32_t *heads, hipStream_t s) {
+    hip_check(ofg::launch_reset_queue(heads, 2, s), "reset queue");
+    for (int m = 0; m < 2; ++m) {
+        const int32_t n = p.model_end[m] - p.model_begin[m];
+        if (n <= 0) continue;
+        ofg::LaunchArgs a = base;
+        a.model = m;
+        a.antithetic = p.antithetic;
+        a.order_begin = p.model_begin[m];
+        a.order_end = p.model_end[m];
+        a.queue_head = heads + m;
+        a.block = t.block;
+        const int full = ofg::persistent_grid(t.block, a.rng_mode, m, a.antithetic, device);
+        const int wpb = t.block / 64;
+        a.grid = std::max(1, (int)std::min<int64_t>(full, ((int64_t)n + wpb - 1) / wpb));
+        hip_check(ofg::launch_pricing(a, s), "pricing kernel launch");
+    }
 }
 
 // Runs a plan on the context stream; outs receives one OptOut per plan option.
@@ -506,8 +559,6 @@ double run_plan(ofg_ctx *ctx, const Plan &p, const ofg_engine_cfg &cfg, const Tu
     a.opts = ctx->opts.as<ofg::OptDev>();
     a.items = ctx->items.as<ofg::WorkItem>();
     a.order = ctx->order.as<int32_t>();
-    a.n_items = (int32_t)p.items.size();
-    a.queue_head = ctx->head.as<uint32_t>();
     a.partials = ctx->partials.as<ofg::Partial>();
     a.obs_w = ctx->obs_w.as<double>();
     a.xs = xs_dev;
@@ -515,15 +566,11 @@ double run_plan(ofg_ctx *ctx, const Plan &p, const ofg_engine_cfg &cfg, const Tu
     a.normals1 = n1_dev;
     a.normals2 = n2_dev;
     a.rng_mode = cfg.rng_mode;
-    a.block = t.block;
-    const int grid_full = ofg::persistent_grid(t.block, cfg.rng_mode, ctx->device);
-    const int waves_per_block = t.block / 64;
-    const int need_blocks = (int)std::min<int64_t>(grid_full, (p.items.size() + waves_per_block - 1) / waves_per_block);
-    a.grid = std::max(1, need_blocks);
 
-    hip_check(ofg::launch_reset_queue(a.queue_head, s), "reset queue");
+    trace("run_plan: uploads enqueued");
     hip_check(hipEventRecord(ctx->ev0, s), "event");
-    hip_check(ofg::launch_pricing(a, s), "pricing kernel launch");
+    enqueue_pricing(a, p, t, ctx->device, ctx->head.as<uint32_t>(), s);
+    trace("run_plan: kernels enqueued");
     hip_check(hipEventRecord(ctx->ev1, s), "event");
     hip_check(ofg::launch_reduce(a.opts, ctx->item_first.as<int64_t>(), ctx->item_count.as<int32_t>(),
                                  a.partials, ctx->outs.as<ofg::OptOut>(), (int32_t)nopt, s),
@@ -539,6 +586,7 @@ double run_plan(ofg_ctx *ctx, const Plan &p, const ofg_engine_cfg &cfg, const Tu
         hip_check(hipMemcpyAsync(ys_host, ys_dev, (size_t)p.total_samples * sizeof(double),
                                  hipMemcpyDeviceToHost, s), "D2H ys");
     hip_check(hipStreamSynchronize(s), "kernel execution");
+    trace("run_plan: synchronized");
     float ms_all = 0.f, ms_price = 0.f;
     hip_check(hipEventElapsedTime(&ms_price, ctx->ev0, ctx->ev1), "event time");
     hip_check(hipEventElapsedTime(&ms_all, ctx->ev0, ctx->ev2), "event time");
@@ -704,7 +752,9 @@ ofg_ctx *ofg_create(int32_t device) {
     ofg_ctx *ctx = nullptr;
     const int32_t rc = guarded([&]() -> int32_t {
         int count = 0;
+        trace("create: hipGetDeviceCount");
         hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        trace("create: device count ok");
         if (device < 0 || device >= count) fail(OFG_DEVICE_ERROR, "no such HIP device");
         hip_check(hipSetDevice(device), "hipSetDevice");
         auto c = std::make_unique<ofg_ctx>();
@@ -716,6 +766,7 @@ ofg_ctx *ofg_create(int32_t device) {
         hip_check(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device),
                   "hipDeviceGetAttribute");
         ctx = c.release();
+        trace("create: done");
         return OFG_OK;
     });
     return rc == OFG_OK ? ctx : nullptr;
@@ -801,15 +852,10 @@ ofg_batch *ofg_batch_create(ofg_ctx *ctx, const ofg_option *opts, size_t n,
         a.opts = b->opts.as<ofg::OptDev>();
         a.items = b->items.as<ofg::WorkItem>();
         a.order = b->order.as<int32_t>();
-        a.n_items = (int32_t)p.items.size();
-        a.queue_head = b->head.as<uint32_t>();
         a.partials = b->partials.as<ofg::Partial>();
         a.obs_w = b->obs_w.as<double>();
         a.rng_mode = cfg->rng_mode;
-        a.block = t.block;
-        const int grid_full = ofg::persistent_grid(t.block, cfg->rng_mode, ctx->device);
-        const int wpb = t.block / 64;
-        a.grid = std::max(1, (int)std::min<int64_t>(grid_full, ((int64_t)p.items.size() + wpb - 1) / wpb));
+        b->tune = t;
         hip_check(hipEventCreate(&b->ev0), "hipEventCreate");
         hip_check(hipEventCreate(&b->ev1), "hipEventCreate");
         hip_check(hipEventCreate(&b->ev2), "hipEventCreate");
@@ -817,8 +863,12 @@ ofg_batch *ofg_batch_create(ofg_ctx *ctx, const ofg_option *opts, size_t n,
         if (b->use_graph) {
             // Queue reset is a kernel, not a memset node, so replays stay exact.
             hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-            hipError_t e1 = ofg::launch_reset_queue(a.queue_head, s);
-            hipError_t e2 = ofg::launch_pricing(a, s);
+            hipError_t e1 = hipSuccess, e2 = hipSuccess;
+            try {
+                enqueue_pricing(a, p, t, ctx->device, b->head.as<uint32_t>(), s);
+            } catch (const OfgError &) {
+                e2 = hipErrorLaunchFailure;
+            }
             hipError_t e3 = ofg::launch_reduce(a.opts, b->item_first.as<int64_t>(),
                                                b->item_count.as<int32_t>(), a.partials,
                                                b->outs.as<ofg::OptOut>(), (int32_t)p.opts.size(), s);
@@ -852,8 +902,7 @@ int32_t ofg_batch_run(ofg_batch *b, double *kernel_ms) {
             hip_check(hipGraphLaunch(b->exec, s), "graph launch");
             hip_check(hipEventRecord(b->ev2, s), "event");
         } else {
-            hip_check(ofg::launch_reset_queue(b->args.queue_head, s), "reset queue");
-            hip_check(ofg::launch_pricing(b->args, s), "pricing kernel");
+            enqueue_pricing(b->args, p, b->tune, ctx->device, b->head.as<uint32_t>(), s);
             hip_check(hipEventRecord(b->ev1, s), "event");
             hip_check(ofg::launch_reduce(b->args.opts, b->item_first.as<int64_t>(),
                                          b->item_count.as<int32_t>(), b->args.partials,

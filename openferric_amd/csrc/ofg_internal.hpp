@@ -99,23 +99,27 @@ struct LaunchArgs {
     const OptDev *opts;
     const WorkItem *items;
     const int32_t *order;     // queue order (cost-descending) of item ids
-    int32_t n_items;
-    uint32_t *queue_head;
+    int32_t order_begin;      // this launch pops order[order_begin, order_end)
+    int32_t order_end;
+    uint32_t *queue_head;     // this launch's queue counter
     Partial *partials;
     const double *obs_w;      // observation weights, per option steps+1
     double *xs, *ys;          // optional per-sample outputs
     const double *normals1, *normals2;  // external normals
     int32_t rng_mode;
+    int32_t model;            // every item of one launch has this model
+    int32_t antithetic;       // and this antithetic flag
     int32_t block;            // threads per block
     int32_t grid;             // persistent blocks
 };
 
-hipError_t launch_reset_queue(uint32_t *head, hipStream_t stream);
+// Zeroes n queue counters (a kernel, so it is safe inside captured graphs).
+hipError_t launch_reset_queue(uint32_t *head, int32_t n, hipStream_t stream);
 hipError_t launch_pricing(const LaunchArgs &a, hipStream_t stream);
 hipError_t launch_reduce(const OptDev *opts, const int64_t *item_first, const int32_t *item_count,
                          const Partial *partials, OptOut *out, int32_t n_opts,
                          hipStream_t stream);
 // Occupancy-derived persistent grid size for a block size.
-int persistent_grid(int block, int rng_mode, int device);
+int persistent_grid(int block, int rng_mode, int model, int antithetic, int device);
 
 }  // namespace ofg

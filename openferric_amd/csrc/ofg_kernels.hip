@@ -268,25 +268,23 @@ __device__ __forceinline__ void dispatch_kind(const OptDev &o, const WorkItem &w
     }
 }
 
-template <int RNG, int BLOCK>
+template <int RNG, int MODEL, bool ANTI, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void price_kernel(LaunchArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t span = (uint32_t)(a.order_end - a.order_begin);
     for (;;) {
-        uint32_t q = 0;
-        if (lane == 0) q = atomicAdd(a.queue_head, 1u);
-        q = __builtin_amdgcn_readfirstlane(q);
-        if (q >= (uint32_t)a.n_items) break;
-        const int32_t it = __builtin_amdgcn_readfirstlane(a.order[q]);
+        // Every lane executes the atomic (lane 0 adds 1, the others 0) so the
+        // loop holds no divergent branch on the loop-invariant `lane == 0`;
+        // with one, the compiler may unswitch the loop and leave lanes 1..63
+        // in a copy that never sees the popped index.
+        const uint32_t q = __builtin_amdgcn_readfirstlane(
+            atomicAdd(a.queue_head, lane == 0 ? 1u : 0u));
+        if (q >= span) break;
+        const int32_t it = __builtin_amdgcn_readfirstlane(a.order[a.order_begin + (int32_t)q]);
         const WorkItem wi = a.items[it];
         const OptDev o = a.opts[wi.option];
         Acc acc{0.0, 0.0, 0.0, 0.0, 0.0};
-        if (o.model == OFG_MODEL_GBM) {
-            if (o.antithetic) dispatch_kind<OFG_MODEL_GBM, RNG, true>(o, wi, lane, a, acc);
-            else dispatch_kind<OFG_MODEL_GBM, RNG, false>(o, wi, lane, a, acc);
-        } else {
-            if (o.antithetic) dispatch_kind<OFG_MODEL_HESTON, RNG, true>(o, wi, lane, a, acc);
-            else dispatch_kind<OFG_MODEL_HESTON, RNG, false>(o, wi, lane, a, acc);
-        }
+        dispatch_kind<MODEL, RNG, ANTI>(o, wi, lane, a, acc);
         const double sx = wave_sum(acc.sx);
         const double sx2 = wave_sum(acc.sx2);
         const double sy = wave_sum(acc.sy);
@@ -305,8 +303,8 @@ __global__ __launch_bounds__(BLOCK) void price_kernel(LaunchArgs a) {
     }
 }
 
-__global__ void reset_queue_kernel(uint32_t *head) {
-    if (threadIdx.x == 0) *head = 0u;
+__global__ void reset_queue_kernel(uint32_t *head, int32_t n) {
+    if ((int32_t)threadIdx.x < n) head[threadIdx.x] = 0u;
 }
 
 // One wavefront per option: lanes stride over the option's item slots, then a
@@ -351,31 +349,50 @@ __global__ __launch_bounds__(256) void reduce_kernel(const OptDev *__restrict__ 
     }
 }
 
-template <int RNG>
-hipError_t launch_rng(const LaunchArgs &a, hipStream_t stream) {
+template <int RNG, int MODEL, bool ANTI>
+hipError_t launch_variant(const LaunchArgs &a, hipStream_t stream) {
     switch (a.block) {
-    case 64: price_kernel<RNG, 64><<<a.grid, 64, 0, stream>>>(a); break;
-    case 128: price_kernel<RNG, 128><<<a.grid, 128, 0, stream>>>(a); break;
-    case 512: price_kernel<RNG, 512><<<a.grid, 512, 0, stream>>>(a); break;
-    default: price_kernel<RNG, 256><<<a.grid, 256, 0, stream>>>(a); break;
+    case 64: price_kernel<RNG, MODEL, ANTI, 64><<<a.grid, 64, 0, stream>>>(a); break;
+    case 128: price_kernel<RNG, MODEL, ANTI, 128><<<a.grid, 128, 0, stream>>>(a); break;
+    case 512: price_kernel<RNG, MODEL, ANTI, 512><<<a.grid, 512, 0, stream>>>(a); break;
+    default: price_kernel<RNG, MODEL, ANTI, 256><<<a.grid, 256, 0, stream>>>(a); break;
     }
     return hipGetLastError();
 }
 
-template <int RNG>
-const void *kernel_ptr(int block) {
+template <int RNG, int MODEL, bool ANTI>
+const void *variant_ptr(int block) {
     switch (block) {
-    case 64: return (const void *)price_kernel<RNG, 64>;
-    case 128: return (const void *)price_kernel<RNG, 128>;
-    case 512: return (const void *)price_kernel<RNG, 512>;
-    default: return (const void *)price_kernel<RNG, 256>;
+    case 64: return (const void *)price_kernel<RNG, MODEL, ANTI, 64>;
+    case 128: return (const void *)price_kernel<RNG, MODEL, ANTI, 128>;
+    case 512: return (const void *)price_kernel<RNG, MODEL, ANTI, 512>;
+    default: return (const void *)price_kernel<RNG, MODEL, ANTI, 256>;
     }
+}
+
+template <int RNG>
+hipError_t launch_rng(const LaunchArgs &a, hipStream_t stream) {
+    if (a.model == OFG_MODEL_GBM) {
+        return a.antithetic ? launch_variant<RNG, OFG_MODEL_GBM, true>(a, stream)
+                            : launch_variant<RNG, OFG_MODEL_GBM, false>(a, stream);
+    }
+    return a.antithetic ? launch_variant<RNG, OFG_MODEL_HESTON, true>(a, stream)
+                        : launch_variant<RNG, OFG_MODEL_HESTON, false>(a, stream);
+}
+
+template <int RNG>
+const void *kernel_ptr(int block, int model, int anti) {
+    if (model == OFG_MODEL_GBM)
+        return anti ? variant_ptr<RNG, OFG_MODEL_GBM, true>(block)
+                    : variant_ptr<RNG, OFG_MODEL_GBM, false>(block);
+    return anti ? variant_ptr<RNG, OFG_MODEL_HESTON, true>(block)
+                : variant_ptr<RNG, OFG_MODEL_HESTON, false>(block);
 }
 
 }  // namespace
 
-hipError_t launch_reset_queue(uint32_t *head, hipStream_t stream) {
-    reset_queue_kernel<<<1, 64, 0, stream>>>(head);
+hipError_t launch_reset_queue(uint32_t *head, int32_t n, hipStream_t stream) {
+    reset_queue_kernel<<<1, 64, 0, stream>>>(head, n);
     return hipGetLastError();
 }
 
@@ -397,10 +414,10 @@ hipError_t launch_reduce(const OptDev *opts, const int64_t *item_first, const in
     return hipGetLastError();
 }
 
-int persistent_grid(int block, int rng_mode, int device) {
-    const void *k = rng_mode == OFG_RNG_EXTERNAL ? kernel_ptr<OFG_RNG_EXTERNAL>(block)
-                    : rng_mode == OFG_RNG_FAST   ? kernel_ptr<OFG_RNG_FAST>(block)
-                                                 : kernel_ptr<OFG_RNG_REFERENCE>(block);
+int persistent_grid(int block, int rng_mode, int model, int antithetic, int device) {
+    const void *k = rng_mode == OFG_RNG_EXTERNAL ? kernel_ptr<OFG_RNG_EXTERNAL>(block, model, antithetic)
+                    : rng_mode == OFG_RNG_FAST   ? kernel_ptr<OFG_RNG_FAST>(block, model, antithetic)
+                                                 : kernel_ptr<OFG_RNG_REFERENCE>(block, model, antithetic);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess ||
         per_cu <= 0)

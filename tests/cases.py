@@ -43,7 +43,7 @@ def make_pair(model="gbm", payoff="vanilla", option_type="call", spot=100.0, str
 
 def daily_obs(expiry, steps):
     """Observation at every step t_j = j T / n, j = 1..n."""
-    return [j * expiry / steps for j in range(1, steps + 1)]
+    return [expiry * (j / steps) for j in range(1, steps + 1)]
 
 
 def step_major(z):
