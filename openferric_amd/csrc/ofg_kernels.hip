@@ -23,6 +23,8 @@
 #include "ofg_internal.hpp"
 #include "ofg_math.hpp"
 
+#include <type_traits>
+
 namespace ofg {
 
 namespace {
@@ -133,12 +135,6 @@ struct ExternalNormals {
     __device__ __forceinline__ double z2(int j) const { return p2[(int64_t)j * stride]; }
 };
 
-// Draws two 64-bit words and returns a Box-Muller pair.
-__device__ __forceinline__ void fast_pair(Xoshiro256pp &rng, double &a, double &b) {
-    const uint64_t w1 = rng.next_u64();
-    const uint64_t w2 = rng.next_u64();
-    box_muller(w1, w2, a, b);
-}
 
 // ------------------------------------------------------------------------
 // Sample simulation: one (possibly antithetic) sample -> (x, y)
@@ -155,7 +151,10 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
     leg_init<PK>(ga, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0, bsign);
     if constexpr (ANTI) leg_init<PK>(gb, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0, bsign);
 
-    Xoshiro256pp rng;
+    // Reference draws use the reference's xoshiro256++; the fast mode uses
+    // xoshiro256+ with Box-Muller (four normals per three 64-bit draws).
+    using Rng = typename std::conditional<RNG == OFG_RNG_FAST, Xoshiro256p, Xoshiro256pp>::type;
+    Rng rng;
     ExternalNormals ext;
     if constexpr (RNG == OFG_RNG_REFERENCE || RNG == OFG_RNG_FAST) {
         rng.seed(stream_seed(o.seed, (uint64_t)sample));
@@ -164,63 +163,76 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
         ext.p2 = a.normals2 ? a.normals2 + o.normals_offset + sample : nullptr;
         ext.stride = o.samples;
     }
+    constexpr bool ASIAN = PK == PK_ASIAN || PK == PK_ASIAN_LOG;
 
     if constexpr (MODEL == OFG_MODEL_GBM) {
+        // one GBM step on both legs, then the observation of path index j+1
+        auto advance = [&](double z, int j) {
+            gbm_step<PK>(ga, o, z);
+            if constexpr (ANTI) gbm_step<PK>(gb, o, -z);
+            const double wj = ASIAN ? w[j + 1] : 0.0;
+            leg_observe<PK>(ga, wj, bsign, barrier);
+            if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
+        };
         if constexpr (RNG == OFG_RNG_FAST) {
             int j = 0;
-            for (; j + 1 < steps; j += 2) {
-                double za, zb;
-                fast_pair(rng, za, zb);
-                gbm_step<PK>(ga, o, za);
-                if constexpr (ANTI) gbm_step<PK>(gb, o, -za);
-                const double w1 = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 1] : 0.0;
-                leg_observe<PK>(ga, w1, bsign, barrier);
-                if constexpr (ANTI) leg_observe<PK>(gb, w1, bsign, barrier);
-                gbm_step<PK>(ga, o, zb);
-                if constexpr (ANTI) gbm_step<PK>(gb, o, -zb);
-                const double w2 = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 2] : 0.0;
-                leg_observe<PK>(ga, w2, bsign, barrier);
-                if constexpr (ANTI) leg_observe<PK>(gb, w2, bsign, barrier);
+            for (; j + 3 < steps; j += 4) {
+                double z0, z1, z2, z3;
+                normals4(rng, z0, z1, z2, z3);
+                advance(z0, j);
+                advance(z1, j + 1);
+                advance(z2, j + 2);
+                advance(z3, j + 3);
             }
             if (j < steps) {
-                double za, zb;
-                fast_pair(rng, za, zb);
-                gbm_step<PK>(ga, o, za);
-                if constexpr (ANTI) gbm_step<PK>(gb, o, -za);
-                const double w1 = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 1] : 0.0;
-                leg_observe<PK>(ga, w1, bsign, barrier);
-                if constexpr (ANTI) leg_observe<PK>(gb, w1, bsign, barrier);
+                double z0, z1, z2, z3;
+                normals4(rng, z0, z1, z2, z3);
+                advance(z0, j);
+                if (j + 1 < steps) advance(z1, j + 1);
+                if (j + 2 < steps) advance(z2, j + 2);
             }
         } else {
             for (int j = 0; j < steps; ++j) {
                 double z;
                 if constexpr (RNG == OFG_RNG_REFERENCE) z = reference_normal(rng);
                 else z = ext.z1(j);
-                gbm_step<PK>(ga, o, z);
-                if constexpr (ANTI) gbm_step<PK>(gb, o, -z);
-                const double wj = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 1] : 0.0;
-                leg_observe<PK>(ga, wj, bsign, barrier);
-                if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
+                advance(z, j);
             }
         }
     } else {
         double va = o.v0, vb = o.v0;
-        for (int j = 0; j < steps; ++j) {
-            double z1, z2;
-            if constexpr (RNG == OFG_RNG_REFERENCE) {
-                z1 = reference_normal(rng);
-                z2 = reference_normal(rng);
-            } else if constexpr (RNG == OFG_RNG_FAST) {
-                fast_pair(rng, z1, z2);
-            } else {
-                z1 = ext.z1(j);
-                z2 = ext.z2(j);
-            }
+        auto advance = [&](double z1, double z2, int j) {
             heston_step<PK>(ga, va, o, z1, z2);
             if constexpr (ANTI) heston_step<PK>(gb, vb, o, -z1, -z2);
-            const double wj = (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[j + 1] : 0.0;
+            const double wj = ASIAN ? w[j + 1] : 0.0;
             leg_observe<PK>(ga, wj, bsign, barrier);
             if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
+        };
+        if constexpr (RNG == OFG_RNG_FAST) {
+            int j = 0;
+            for (; j + 1 < steps; j += 2) {
+                double z0, z1, z2, z3;
+                normals4(rng, z0, z1, z2, z3);
+                advance(z0, z1, j);
+                advance(z2, z3, j + 1);
+            }
+            if (j < steps) {
+                double z0, z1, z2, z3;
+                normals4(rng, z0, z1, z2, z3);
+                advance(z0, z1, j);
+            }
+        } else {
+            for (int j = 0; j < steps; ++j) {
+                double z1, z2;
+                if constexpr (RNG == OFG_RNG_REFERENCE) {
+                    z1 = reference_normal(rng);
+                    z2 = reference_normal(rng);
+                } else {
+                    z1 = ext.z1(j);
+                    z2 = ext.z2(j);
+                }
+                advance(z1, z2, j);
+            }
         }
     }
 

@@ -14,7 +14,8 @@
 //   * exp: Cody-Waite reduction by ln2 into |r| <= ln2/2 and a degree-11
 //     polynomial (Chebyshev-node interpolant of e^r, max relative error
 //     1.7e-17 in exact arithmetic), scaled with ldexp.
-//   * Box-Muller pair generator for the fast RNG mode.
+//   * fast RNG mode: xoshiro256+ streams and a branch-free Box-Muller
+//     transform with its own log, sqrt and sin/cos on [-pi/4, pi/4].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -176,19 +177,158 @@ OFG_HD double reference_normal(Xoshiro256pp &rng) {
     return acklam_inv_cdf(uniform_open01(rng.next_f64()));
 }
 
-// -------------------------------------------------------- Box-Muller ----
+// ------------------------------------------------ fast-mode generator ---
 
-// Two independent standard normals from two 64-bit words:
-//   u1 in (0, 1] from 52 bits of w1, angle 2*pi*u2 from 52 bits of w2.
-OFG_HD void box_muller(uint64_t w1, uint64_t w2, double &z0, double &z1) {
-    // u1 = 1 - [0,1) grid of 2^-52 steps, so u1 in (0, 1] and log(u1) finite.
-    const double u1 = 1.0 - (double)(w1 >> 12) * (1.0 / 4503599627370496.0);
-    const double u2 = (double)(w2 >> 12) * (1.0 / 4503599627370496.0);
-    const double rad = sqrt(-2.0 * log(u1));
-    double s, c;
-    sincospi(2.0 * u2, &s, &c);
-    z0 = rad * c;
-    z1 = rad * s;
+// xoshiro256+ (Blackman & Vigna 2018): the same state and transition as
+// xoshiro256++ with the cheaper "+" scrambler (s0 + s3), the authors'
+// recommendation for floating-point output from the upper bits. Seeded by the
+// same SplitMix64 expansion of stream_seed(seed, sample).
+struct Xoshiro256p {
+    uint64_t s0, s1, s2, s3;
+
+    OFG_HD void seed(uint64_t seed) {
+        uint64_t sm = seed;
+        s0 = splitmix64_next(sm);
+        s1 = splitmix64_next(sm);
+        s2 = splitmix64_next(sm);
+        s3 = splitmix64_next(sm);
+        if ((s0 | s1 | s2 | s3) == 0) s0 = 1;
+    }
+
+    OFG_HD uint64_t next_u64() {
+        const uint64_t result = s0 + s3;
+        const uint64_t t = s1 << 17;
+        s2 ^= s0;
+        s3 ^= s1;
+        s1 ^= s2;
+        s0 ^= s3;
+        s2 ^= t;
+        s3 = rotl64(s3, 45);
+        return result;
+    }
+};
+
+OFG_HD double bits_as_double(uint64_t b) {
+    double d;
+    __builtin_memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+OFG_HD double rcp_seed(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcp(d);
+#else
+    return 1.0 / d;
+#endif
+}
+
+OFG_HD double rsq_seed(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsq(d);
+#else
+    return 1.0 / sqrt(d);
+#endif
+}
+
+// Natural log of a positive normal double, branch-free.
+// x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1),
+// evaluated as 2s + 2s z Q(z), z = s^2 <= 0.0295, Q a degree-6 Chebyshev-node
+// interpolant of (atanh(sqrt z)/sqrt z - 1)/z (max relative error of the
+// reconstructed log 4.6e-18 in exact arithmetic). The division is a
+// reciprocal seed with one Newton step and a final residual correction.
+OFG_HD double log_fp64(double x) {
+    int e;
+    double m = frexp(x, &e);
+    const bool lo = m < 0.70710678118654752;
+    m = lo ? m + m : m;
+    const double ed = (double)(lo ? e - 1 : e);
+    const double f = m - 1.0;
+    const double d = f + 2.0;
+    double r = rcp_seed(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    double s = f * r;
+    s = fma(fma(-d, s, f), r, s);
+    const double z = s * s;
+    double q = 0.07308224842521703;
+    q = fma(q, z, 0.07665860800278021);
+    q = fma(q, z, 0.09091444562630861);
+    q = fma(q, z, 0.1111110556739754);
+    q = fma(q, z, 0.14285714312987743);
+    q = fma(q, z, 0.19999999999949752);
+    q = fma(q, z, 0.3333333333333335);
+    const double s2 = s + s;
+    const double lm = fma(s2 * z, q, s2);
+    const double LN2_HI = 6.93147180369123816490e-01;
+    const double LN2_LO = 1.90821492927058770002e-10;
+    return fma(ed, LN2_HI, fma(ed, LN2_LO, lm));
+}
+
+// sqrt of a positive finite double from the reciprocal square-root seed and
+// one Goldschmidt refinement.
+OFG_HD double sqrt_pos(double t) {
+    const double y = rsq_seed(t);
+    double g = t * y;
+    double h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, t);
+    return fma(d, h, g);
+}
+
+// sin and cos of |t| <= pi/4: degree-5 Chebyshev-node interpolants in
+// w = t^2 of (sin t / t - 1)/w and (cos t - 1 + w/2)/w^2 (max relative error
+// 2.0e-17 and 1.3e-18 in exact arithmetic).
+OFG_HD void sincos_octant(double t, double &sn, double &cs) {
+    const double w = t * t;
+    double ps = 1.5918129294866608e-10;
+    ps = fma(ps, w, -2.5051131845003624e-08);
+    ps = fma(ps, w, 2.755731610255244e-06);
+    ps = fma(ps, w, -0.00019841269836758574);
+    ps = fma(ps, w, 0.008333333333330948);
+    ps = fma(ps, w, -0.16666666666666666);
+    sn = fma(t * w, ps, t);
+    double pc = -1.1382632425521717e-11;
+    pc = fma(pc, w, 2.08761462684032e-09);
+    pc = fma(pc, w, -2.7557317271729793e-07);
+    pc = fma(pc, w, 2.480158729876569e-05);
+    pc = fma(pc, w, -0.0013888888888887398);
+    pc = fma(pc, w, 0.041666666666666664);
+    cs = fma(w * w, pc, fma(-0.5, w, 1.0));
+}
+
+// Box-Muller pair from 52 bits of `wu` (radius), 3 bits of `wu` (symmetry)
+// and 32 bits `wt` (angle), all branch-free:
+//   u  = (k + 1/2) 2^-52 in (0, 1), k = wu >> 12, so log u is finite and < 0;
+//   R  = sqrt(-2 log u);
+//   an angle uniform on [-pi/4, pi/4) from wt, mapped to the whole circle by
+//   one of the 8 symmetries of the square (swap, two sign flips) chosen by
+//   bits 8..10 of wu: the image is uniform on the circle.
+OFG_HD void box_muller_pair(uint64_t wu, uint32_t wt, double &z0, double &z1) {
+    const double one_k = bits_as_double(0x3FF0000000000000ull | (wu >> 12));  // 1 + k 2^-52
+    const double u = one_k - 0.99999999999999988898;                        // exact
+    const double rad = sqrt_pos(-2.0 * log_fp64(u));
+    const double t = fma((double)wt, 3.657295198167899e-10, -0.7853981633974483);  // pi / 2^33, pi / 4
+    double sn, cs;
+    sincos_octant(t, sn, cs);
+    const bool swap = (wu >> 8) & 1u;
+    double a = swap ? sn : cs;
+    double b = swap ? cs : sn;
+    a = ((wu >> 9) & 1u) ? -a : a;
+    b = ((wu >> 10) & 1u) ? -b : b;
+    z0 = rad * a;
+    z1 = rad * b;
+}
+
+// Four standard normals from three 64-bit draws (two Box-Muller pairs; the
+// third draw supplies both angles).
+template <class Rng>
+OFG_HD void normals4(Rng &rng, double &z0, double &z1, double &z2, double &z3) {
+    const uint64_t wa = rng.next_u64();
+    const uint64_t wb = rng.next_u64();
+    const uint64_t wc = rng.next_u64();
+    box_muller_pair(wa, (uint32_t)(wc >> 32), z0, z1);
+    box_muller_pair(wb, (uint32_t)wc, z2, z3);
 }
 
 }  // namespace ofg
