@@ -343,6 +343,7 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
         d.nobs = (double)std::max<int32_t>(o.n_obs, 0);
         d.log_s0 = std::log(std::fmax(o.spot, 1e-12));
         d.df = std::exp(-o.rate * o.expiry);  // mc_engine.rs:945
+        d.rate = o.rate;
         d.antithetic = cfg.antithetic ? 1 : 0;
         int32_t flags = 0;
         if (o.option_type == OFG_PUT) flags |= ofg::F_PUT;
@@ -694,12 +695,147 @@ int32_t price_fd(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engin
     return OFG_OK;
 }
 
+// Pathwise adjoint Greeks: forward + backward sweeps per sample in one
+// persistent launch per model (ofg_adjoint.hip), then a segmented reduce.
+int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
+                     ofg_result *out) {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ofg_engine_cfg c = *cfg;
+    c.control_variate = 0;  // mc_aad.rs: plain estimator of price and Greeks
+    c.greek_mode = OFG_GREEKS_NONE;
+    Tuning t;
+    t.block = cfg->tune_block > 0 ? std::min(cfg->tune_block, 256) : 256;
+    t.item_steps = cfg->tune_item_steps > 0 ? cfg->tune_item_steps : 4096;
+    Plan p = build_plan(opts, n, c, t);
+    hipStream_t s = ctx->stream;
+    const size_t nopt = p.opts.size();
+    std::vector<ofg::AdjOut> outs(nopt);
+    double ms = 0.0;
+    if (nopt) {
+        upload(ctx->opts, p.opts, s);
+        upload(ctx->obs_w, p.obs_w, s);
+        upload(ctx->items, p.items, s);
+        upload(ctx->order, p.order, s);
+        upload(ctx->item_first, p.item_first, s);
+        upload(ctx->item_count, p.item_count, s);
+        ctx->partials.ensure(p.items.size() * sizeof(ofg::AdjPartial));
+        ctx->outs.ensure(nopt * sizeof(ofg::AdjOut));
+        ctx->head.ensure(256);
+        const double *n1_dev = nullptr, *n2_dev = nullptr;
+        if (c.rng_mode == OFG_RNG_EXTERNAL) {
+            if (!c.normals1) fail(OFG_INVALID_INPUT, "external normals required for OFG_RNG_EXTERNAL");
+            if (p.any_heston && !c.normals2)
+                fail(OFG_INVALID_INPUT, "Heston needs a second external normal stream");
+            const size_t bytes = (size_t)p.normals_elems * sizeof(double);
+            ctx->n1.ensure(bytes);
+            hip_check(hipMemcpyAsync(ctx->n1.p, c.normals1, bytes, hipMemcpyHostToDevice, s), "H2D normals");
+            n1_dev = ctx->n1.as<double>();
+            if (c.normals2) {
+                ctx->n2.ensure(bytes);
+                hip_check(hipMemcpyAsync(ctx->n2.p, c.normals2, bytes, hipMemcpyHostToDevice, s), "H2D normals");
+                n2_dev = ctx->n2.as<double>();
+            }
+        }
+        // per-model launch shapes and checkpoint scratch
+        ofg::AdjArgs la[2];
+        size_t scratch_doubles = 0;
+        for (int m = 0; m < 2; ++m) {
+            const int32_t cnt = p.model_end[m] - p.model_begin[m];
+            if (cnt <= 0) continue;
+            int max_steps = 1;
+            for (const auto &d : p.opts)
+                if (d.model == m) max_steps = std::max(max_steps, (int)d.steps);
+            const int nv = m == OFG_MODEL_HESTON ? 2 : 1;
+            const int seg = ofg::adj_segment(m);
+            const int64_t nseg = (max_steps + seg - 1) / seg;
+            ofg::AdjArgs &a = la[m];
+            a = ofg::AdjArgs{};
+            a.opts = ctx->opts.as<ofg::OptDev>();
+            a.items = ctx->items.as<ofg::WorkItem>();
+            a.order = ctx->order.as<int32_t>();
+            a.order_begin = p.model_begin[m];
+            a.order_end = p.model_end[m];
+            a.queue_head = ctx->head.as<uint32_t>() + m;
+            a.partials = ctx->partials.as<ofg::AdjPartial>();
+            a.obs_w = ctx->obs_w.as<double>();
+            a.normals1 = n1_dev;
+            a.normals2 = n2_dev;
+            a.wave_stride = nseg * nv * 64;
+            a.rng_mode = c.rng_mode;
+            a.model = m;
+            a.antithetic = p.antithetic;
+            a.block = t.block;
+            const int wpb = t.block / 64;
+            const int full = ofg::adjoint_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device);
+            a.grid = std::max(1, (int)std::min<int64_t>(full, ((int64_t)cnt + wpb - 1) / wpb));
+            scratch_doubles = std::max(scratch_doubles, (size_t)a.grid * wpb * (size_t)a.wave_stride);
+        }
+        ctx->adj.ensure(scratch_doubles * sizeof(double));
+        hip_check(ofg::launch_reset_queue(ctx->head.as<uint32_t>(), 2, s), "reset queue");
+        hip_check(hipEventRecord(ctx->ev0, s), "event");
+        for (int m = 0; m < 2; ++m) {
+            if (p.model_end[m] - p.model_begin[m] <= 0) continue;
+            la[m].scratch = ctx->adj.as<double>();
+            hip_check(ofg::launch_adjoint(la[m], s), "adjoint kernel launch");
+        }
+        hip_check(hipEventRecord(ctx->ev1, s), "event");
+        hip_check(ofg::launch_adjoint_reduce(ctx->opts.as<ofg::OptDev>(), ctx->item_first.as<int64_t>(),
+                                             ctx->item_count.as<int32_t>(),
+                                             ctx->partials.as<ofg::AdjPartial>(),
+                                             ctx->outs.as<ofg::AdjOut>(), (int32_t)nopt, s),
+                  "adjoint reduce launch");
+        hip_check(hipEventRecord(ctx->ev2, s), "event");
+        hip_check(hipMemcpyAsync(outs.data(), ctx->outs.p, nopt * sizeof(ofg::AdjOut),
+                                 hipMemcpyDeviceToHost, s), "D2H adjoint results");
+        hip_check(hipStreamSynchronize(s), "adjoint execution");
+        float f = 0.f;
+        hip_check(hipEventElapsedTime(&f, ctx->ev0, ctx->ev2), "event time");
+        ms = f;
+    }
+    std::vector<const ofg::AdjOut *> map(n, nullptr);
+    for (size_t k = 0; k < p.active.size(); ++k) map[(size_t)p.active[k]] = &outs[k];
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = opts[i];
+        ofg_result &r = out[i];
+        fill_result(r, o, nullptr, ms);
+        r.has_greeks = 1;
+        const ofg::AdjOut *a = map[i];
+        if (!a) {  // maturity zero: intrinsic value, zero Greeks (mc_aad.rs:96-112)
+            r.kernel_ms = ms;
+            continue;
+        }
+        r.price = a->price;
+        r.stderr_ = a->stderr_;
+        r.has_stderr = 1;
+        r.num_paths = (double)o.num_paths;
+        r.num_steps = (double)o.num_steps;
+        r.vol = o.vol;
+        r.delta = a->g[0];
+        r.gamma = 0.0;
+        r.rho = a->g[2];
+        r.theta = -a->g[3];
+        if (o.model == OFG_MODEL_GBM) {
+            r.vega = a->g[1];
+        } else {
+            r.vega = a->g[1] * 2.0 * std::sqrt(o.v0);  // d price / d sqrt(v0)
+            r.heston_grad[0] = a->g[0];
+            r.heston_grad[1] = a->g[1];
+            r.heston_grad[2] = a->g[4];
+            r.heston_grad[3] = a->g[5];
+            r.heston_grad[4] = a->g[6];
+            r.heston_grad[5] = a->g[7];
+        }
+    }
+    return OFG_OK;
+}
+
 int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
                    ofg_result *out) {
     if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
     if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
     if (cfg->greek_mode == OFG_GREEKS_FD_CRN) return price_fd(ctx, opts, n, cfg, out);
-    if (cfg->greek_mode == OFG_GREEKS_ADJOINT) return ofg::adjoint_price(ctx, opts, n, cfg, out);
+    if (cfg->greek_mode == OFG_GREEKS_ADJOINT) return adjoint_impl(ctx, opts, n, cfg, out);
     std::lock_guard<std::mutex> lock(ctx->mu);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     // First pass with default tuning only to classify the workload.
@@ -717,27 +853,6 @@ int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_eng
 }
 
 }  // namespace
-
-// Exposed to the adjoint module.
-namespace ofg {
-void host_fail(int32_t code, const std::string &msg) { fail(code, msg); }
-void host_hip_check(hipError_t e, const char *what) { hip_check(e, what); }
-void host_validate(const ofg_option &o) {
-    validate_instrument(o);
-    if (o.num_paths <= 0) fail(OFG_INVALID_INPUT, "num_paths must be > 0");
-    if (o.num_steps <= 0) fail(OFG_INVALID_INPUT, "num_steps must be > 0");
-    if (o.model == OFG_MODEL_HESTON && !heston_valid(o))
-        fail(OFG_INVALID_INPUT, "invalid Heston parameters");
-}
-double host_payoff_at_spot(const ofg_option &o) { return payoff_at_spot(o); }
-hipStream_t ctx_stream(ofg_ctx *ctx) { return ctx->stream; }
-int32_t ctx_device(ofg_ctx *ctx) { return ctx->device; }
-std::mutex &ctx_mutex(ofg_ctx *ctx) { return ctx->mu; }
-void *ctx_adjoint_scratch(ofg_ctx *ctx, size_t bytes) {
-    ctx->adj.ensure(bytes);
-    return ctx->adj.p;
-}
-}  // namespace ofg
 
 // ===========================================================================
 // C ABI

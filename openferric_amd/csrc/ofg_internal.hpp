@@ -43,7 +43,7 @@ struct alignas(16) OptDev {
     double dt, sqrt_dt, drift, diffusion;
     double kappa, theta, xi, rho, rho_c, v0;
     double barrier, rebate, nobs, log_s0;
-    double df, cv_expected;
+    double df, cv_expected, rate;
     int32_t antithetic;
     int32_t has_cv;
 };

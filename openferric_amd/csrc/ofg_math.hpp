@@ -69,7 +69,28 @@ struct Xoshiro256pp {
         const uint64_t x = next_u64() >> 11;
         return (double)x * (1.0 / 9007199254740992.0);
     }
+
+    // Steps back over the last draw and returns it again (the adjoint's
+    // backward sweep regenerates a path's draws in reverse order).
+    OFG_HD uint64_t prev_u64();
 };
+
+// Inverse of the xoshiro256 state transition (shared by ++ and +): the
+// transition is an invertible GF(2)-linear map, so a stream can be walked
+// backwards. With old state (a,b,c,d) and new (A,B,C,D):
+//   d ^ b = rotr(D, 45); a = A ^ (d ^ b); B ^ C = b ^ (b << 17), solved for b
+//   by b = x ^ x<<17 ^ x<<34 ^ x<<51; c = C ^ a ^ (b << 17); d = (d ^ b) ^ b.
+OFG_HD void xoshiro256_unstep(uint64_t &s0, uint64_t &s1, uint64_t &s2, uint64_t &s3) {
+    const uint64_t d1 = (s3 >> 45) | (s3 << 19);
+    const uint64_t a = s0 ^ d1;
+    const uint64_t x = s1 ^ s2;
+    const uint64_t b = x ^ (x << 17) ^ (x << 34) ^ (x << 51);
+    const uint64_t c = s2 ^ a ^ (b << 17);
+    s0 = a;
+    s1 = b;
+    s2 = c;
+    s3 = d1 ^ b;
+}
 
 // stream_seed (fast_rng.rs:202-205), wrapping.
 OFG_HD uint64_t stream_seed(uint64_t base, uint64_t index) { return base + index * 7919ull; }
@@ -195,6 +216,9 @@ struct Xoshiro256p {
         if ((s0 | s1 | s2 | s3) == 0) s0 = 1;
     }
 
+    // Steps back over the last draw and returns it again.
+    OFG_HD uint64_t prev_u64();
+
     OFG_HD uint64_t next_u64() {
         const uint64_t result = s0 + s3;
         const uint64_t t = s1 << 17;
@@ -207,6 +231,16 @@ struct Xoshiro256p {
         return result;
     }
 };
+
+OFG_HD uint64_t Xoshiro256pp::prev_u64() {
+    xoshiro256_unstep(s0, s1, s2, s3);
+    return rotl64(s0 + s3, 23) + s0;
+}
+
+OFG_HD uint64_t Xoshiro256p::prev_u64() {
+    xoshiro256_unstep(s0, s1, s2, s3);
+    return s0 + s3;
+}
 
 OFG_HD double bits_as_double(uint64_t b) {
     double d;

@@ -1,0 +1,113 @@
+// Path-state building blocks shared by the pricing and adjoint kernels: one
+// GBM step (src/mc/simulation.rs:58-71), one Heston Euler step with the
+// generator clamps (src/models/stochastic.rs:56-71, simulation.rs:98-116),
+// observation bookkeeping and the payoffs of
+// src/engines/monte_carlo/mc_engine.rs:59-121, 441-584.
+#pragma once
+
+#include "ofg_internal.hpp"
+#include "ofg_math.hpp"
+
+namespace ofg {
+
+__device__ __forceinline__ double vanilla_payoff(bool put, double spot, double strike) {
+    return put ? fmax(strike - spot, 0.0) : fmax(spot - strike, 0.0);
+}
+
+// Running state of one path leg, held in registers.
+struct Leg {
+    double s;      // spot
+    double sum;    // weighted sum of observed spots (Asian)
+    double ls;     // log spot tracked additively (geometric Asian)
+    double lsum;   // weighted sum of observed log spots
+    bool hit;      // barrier touched (inclusive, path[0] included)
+};
+
+template <int PK>
+__device__ __forceinline__ void leg_init(Leg &g, const OptDev &o, double w0, double bsign) {
+    g.s = o.s0;
+    if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) g.sum = w0 * o.s0;
+    if constexpr (PK == PK_ASIAN_LOG) {
+        g.ls = o.log_s0;
+        g.lsum = w0 * o.log_s0;
+    }
+    if constexpr (PK == PK_BARRIER) g.hit = bsign * o.s0 >= bsign * o.barrier;
+}
+
+// Observation bookkeeping after the spot of path index j+1 is known.
+template <int PK>
+__device__ __forceinline__ void leg_observe(Leg &g, double w, double bsign, double barrier) {
+    if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) g.sum = fma(w, g.s, g.sum);
+    if constexpr (PK == PK_ASIAN_LOG) g.lsum = fma(w, g.ls, g.lsum);
+    if constexpr (PK == PK_BARRIER) g.hit = g.hit || (bsign * g.s >= bsign * barrier);
+}
+
+// One GBM step (simulation.rs:67-70): s *= exp(fma(diffusion, z, drift)).
+template <int PK>
+__device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z) {
+    const double e = fma(o.diffusion, z, o.drift);
+    g.s *= exp_fp64(e);
+    if constexpr (PK == PK_ASIAN_LOG) g.ls += e;
+}
+
+// One Heston Euler step (stochastic.rs:56-71) plus the generator clamps
+// (simulation.rs:112-113). Operation order follows the reference; the file is
+// compiled with -ffp-contract=off.
+template <int PK>
+__device__ __forceinline__ void heston_step(Leg &g, double &v, const OptDev &o, double z1,
+                                            double z2) {
+    const double v_pos = fmax(v, 0.0);
+    const double sv = sqrt(v_pos);
+    const double zs = o.rho * z1 + o.rho_c * z2;
+    const double vn = v + o.kappa * (o.theta - v_pos) * o.dt + o.xi * sv * o.sqrt_dt * z1;
+    const double e = (o.mu - 0.5 * v_pos) * o.dt + sv * o.sqrt_dt * zs;
+    const double sn = g.s * exp_fp64(e);
+    const bool floor_hit = !(sn >= 1e-12);
+    g.s = floor_hit ? 1e-12 : sn;   // f64::max(sn, 1e-12), NaN -> 1e-12
+    if constexpr (PK == PK_ASIAN_LOG) g.ls = floor_hit ? -27.631021115928547 : g.ls + e;
+    v = fmax(vn, 0.0);
+}
+
+template <int PK>
+__device__ __forceinline__ void leg_payoff(const Leg &g, const OptDev &o, double &x, double &y) {
+    const bool put = o.flags & F_PUT;
+    if constexpr (PK == PK_VANILLA) {
+        x = vanilla_payoff(put, g.s, o.strike);
+    } else if constexpr (PK == PK_BARRIER) {
+        const bool active = (o.flags & F_KNOCK_IN) ? g.hit : !g.hit;
+        x = active ? vanilla_payoff(put, g.s, o.strike) : o.rebate;
+    } else {
+        double avg;
+        if constexpr (PK == PK_ASIAN_LOG) {
+            avg = (o.flags & F_GEOMETRIC) ? exp_fp64(g.lsum / o.nobs) : g.sum / o.nobs;
+        } else {
+            avg = g.sum / o.nobs;
+        }
+        x = (o.flags & F_FLOATING) ? vanilla_payoff(put, g.s, avg) : vanilla_payoff(put, avg, o.strike);
+    }
+    y = 0.0;
+    if (o.flags & F_CV) {
+        if constexpr (PK == PK_ASIAN_LOG) {
+            if (o.flags & F_CV_GEO) {
+                y = vanilla_payoff(put, exp_fp64(g.lsum / o.nobs), o.strike);
+                return;
+            }
+        }
+        y = vanilla_payoff(put, g.s, o.strike);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Normal sources
+// ------------------------------------------------------------------------
+
+struct ExternalNormals {
+    const double *p1, *p2;
+    int64_t stride;
+    __device__ __forceinline__ double z1(int j) const { return p1[(int64_t)j * stride]; }
+    __device__ __forceinline__ double z2(int j) const { return p2[(int64_t)j * stride]; }
+};
+
+
+
+}  // namespace ofg

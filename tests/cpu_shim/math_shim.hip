@@ -27,3 +27,24 @@ void shim_box_muller(uint64_t wu, uint32_t wt, double *z0, double *z1) {
     ofg::box_muller_pair(wu, wt, *z0, *z1);
 }
 }
+
+extern "C" {
+// Draws n values forward, then walks back n steps; writes both sequences and
+// returns 1 if the state after walking back equals the initial state.
+int shim_xoshiro_roundtrip(uint64_t seed, int plus, int64_t n, uint64_t *fwd, uint64_t *bwd) {
+    if (plus) {
+        ofg::Xoshiro256p r;
+        r.seed(seed);
+        const ofg::Xoshiro256p r0 = r;
+        for (int64_t i = 0; i < n; ++i) fwd[i] = r.next_u64();
+        for (int64_t i = 0; i < n; ++i) bwd[i] = r.prev_u64();
+        return r.s0 == r0.s0 && r.s1 == r0.s1 && r.s2 == r0.s2 && r.s3 == r0.s3;
+    }
+    ofg::Xoshiro256pp r;
+    r.seed(seed);
+    const ofg::Xoshiro256pp r0 = r;
+    for (int64_t i = 0; i < n; ++i) fwd[i] = r.next_u64();
+    for (int64_t i = 0; i < n; ++i) bwd[i] = r.prev_u64();
+    return r.s0 == r0.s0 && r.s1 == r0.s1 && r.s2 == r0.s2 && r.s3 == r0.s3;
+}
+}

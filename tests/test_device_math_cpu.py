@@ -148,3 +148,16 @@ def test_fast_normals_distribution(shim):
     assert abs(np.corrcoef(zz[:-1], zz[1:])[0, 1]) < 5 / np.sqrt(zz.size)
     assert abs(np.corrcoef(zz[0::2], zz[1::2])[0, 1]) < 5 / np.sqrt(zz.size / 2)
     assert abs(np.corrcoef(zz[0::2] ** 2, zz[1::2] ** 2)[0, 1]) < 5 / np.sqrt(zz.size / 2)
+
+
+def test_xoshiro_inverse_walks_the_stream_back(shim):
+    shim.shim_xoshiro_roundtrip.argtypes = [C.c_uint64, C.c_int, C.c_int64,
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    shim.shim_xoshiro_roundtrip.restype = C.c_int
+    for plus in (0, 1):
+        for seed in (0, 42, 2**64 - 1, 123456789):
+            n = 1000
+            f = (C.c_uint64 * n)()
+            b = (C.c_uint64 * n)()
+            assert shim.shim_xoshiro_roundtrip(seed, plus, n, f, b) == 1
+            assert list(b) == list(f)[::-1]

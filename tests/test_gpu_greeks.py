@@ -1,0 +1,161 @@
+"""Adjoint Greeks on the GPU (C ABI calls).
+
+Checks:
+  * GBM European on the reference draws against the oracle's restatement of
+    mc_european_pathwise_aad (same draws): price and Greeks to 1e-10 relative;
+  * the reference's own assertion (mc_aad.rs:353-409) against BS Greeks;
+  * Heston delta against the oracle's heston_price_delta_aad on the same draws;
+  * every adjoint gradient (GBM S0/sigma/r/T, Heston S0/v0/kappa/theta/xi/rho)
+    against central finite differences with common random numbers priced on
+    the GPU (same seeds, so the same draws), for European, Asian and barrier
+    payoffs, in both RNG modes.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.cases import make_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(rng="reference", antithetic=False, greek=0, bump=1e-4):
+    from openferric_amd import engine_cfg, RngMode
+    mode = {"reference": RngMode.Reference, "fast": RngMode.Fast}[rng]
+    return engine_cfg(antithetic=antithetic, rng_mode=mode, greek_mode=greek, bump_rel=bump)
+
+
+def _adjoint(g, rng="reference", antithetic=False):
+    from openferric_amd import price_raw
+    return price_raw([g], _cfg(rng, antithetic), greeks=True)[0]
+
+
+def test_gbm_european_matches_oracle_tape_aad(oracle, gpu):
+    case = dict(rate=0.03, q=0.01, vol=0.2, num_paths=40_000, num_steps=64, seed=42)
+    oo, g, keep = make_pair(**case)
+    for anti in (True, False):
+        ref = oracle.mc_european_pathwise_aad(oo, 40_000, 64, 42, anti)
+        r = _adjoint(g, antithetic=anti)
+        assert r.has_greeks
+        assert abs(r.price - ref["price"]) <= 1e-12 * ref["price"]
+        for name, got in (("delta", r.delta), ("vega", r.vega), ("rho", r.rho),
+                          ("theta", r.theta)):
+            assert abs(got - ref[name]) <= 1e-10 * max(1.0, abs(ref[name])), (name, got, ref[name])
+
+
+def test_gbm_european_reference_assertion(oracle, gpu):
+    """mc_aad.rs:353-409: 120k antithetic paths x 64 steps vs BS Greeks."""
+    oo, g, keep = make_pair(rate=0.03, q=0.01, vol=0.2, num_paths=120_000, num_steps=64, seed=42)
+    r = _adjoint(g, antithetic=True)
+    ref = oracle.bs_greeks(oracle.CALL, 100, 100, 0.03, 0.01, 0.2, 1.0)
+    assert abs(r.delta - ref["delta"]) < 6e-3
+    assert abs(r.vega - ref["vega"]) < 8e-2
+    assert abs(r.rho - ref["rho"]) < 7e-2
+    assert abs(r.theta - ref["theta"]) < 9e-2
+
+
+HESTON = dict(model="heston", spot=100.0, strike=70.0, expiry=1.0, rate=0.03, kappa=1.8,
+              theta=0.04, xi=0.6, rho=-0.5, v0=0.04, num_paths=60_000, num_steps=64, seed=7)
+
+
+def test_heston_delta_matches_oracle_forward_dual(oracle, gpu):
+    """heston_price_delta_aad semantics (mc_aad.rs:239-342) on the same draws."""
+    oo, g, keep = make_pair(**HESTON)
+    p_ref, d_ref = oracle.heston_price_delta_aad(oo, 60_000, 64, 7)
+    r = _adjoint(g)
+    assert abs(r.price - p_ref) <= 1e-11 * p_ref
+    assert abs(r.delta - d_ref) <= 1e-10 * abs(d_ref)
+    assert abs(r.heston_grad[0] - d_ref) <= 1e-10 * abs(d_ref)
+
+
+def _full(case):
+    import inspect
+    defaults = {k: v.default for k, v in inspect.signature(make_pair).parameters.items()}
+    return {**defaults, **case}
+
+
+def _fd(case, field, h, rng, antithetic):
+    """CRN central difference of the GPU price in one ofg_option field."""
+    from openferric_amd import price_raw
+    case = _full(case)
+    up = dict(case)
+    dn = dict(case)
+    up[field] = case[field] + h
+    dn[field] = case[field] - h
+    _, gu, k1 = make_pair(**up)
+    _, gd, k2 = make_pair(**dn)
+    ru, rd = price_raw([gu, gd], _cfg(rng, antithetic))
+    return (ru.price - rd.price) / (2 * h)
+
+
+GBM_FIELDS = [("spot", "delta"), ("vol", "vega"), ("rate", "rho"), ("expiry", "theta")]
+HESTON_FIELDS = [("spot", 0), ("v0", 1), ("kappa", 2), ("theta", 3), ("xi", 4), ("rho", 5)]
+
+
+def _check_fd(case, rng, antithetic, rel_tol, abs_tol):
+    case = _full(case)
+    oo, g, keep = make_pair(**case)
+    r = _adjoint(g, rng, antithetic)
+    heston = case.get("model") == "heston"
+    fields = HESTON_FIELDS if heston else GBM_FIELDS
+    for field, slot in fields:
+        # small bumps: with common random numbers, a larger bump lets paths
+        # cross the payoff kink between the two revaluations and the FD picks
+        # up that sampling noise; the allowance covers FD round-off.
+        h = 1e-7 * max(abs(case.get(field, 0.0)), 1e-2)
+        fd = _fd(case, field, h, rng, antithetic)
+        tol = rel_tol * abs(fd) + abs_tol + 20 * 2.2e-16 * abs(r.price) / h
+        if heston:
+            got = r.heston_grad[slot]
+        else:
+            got = getattr(r, slot)
+            if slot == "theta":
+                fd = -fd
+        assert abs(got - fd) <= tol, (field, got, fd)
+
+
+@pytest.mark.parametrize("rng", ["reference", "fast"])
+def test_gbm_greeks_vs_crn_fd(gpu, rng):
+    for payoff, extra in [("vanilla", {}),
+                          ("asian", dict(obs_times=[m / 12.0 for m in range(1, 13)])),
+                          ("asian", dict(obs_times=[m / 12.0 for m in range(1, 13)],
+                                         averaging="geo", option_type="put")),
+                          ("barrier", dict(barrier_dir="up", barrier_style="out",
+                                           barrier_level=250.0))]:
+        case = dict(payoff=payoff, strike=95.0, rate=0.04, q=0.01, vol=0.25, expiry=1.5,
+                    num_paths=20_000, num_steps=48, seed=11, **extra)
+        _check_fd(case, rng, True, 2e-4, 1e-6)
+
+
+@pytest.mark.parametrize("rng", ["reference", "fast"])
+def test_heston_greeks_vs_crn_fd(gpu, rng):
+    # Feller satisfied (2 kappa theta > xi^2) so the variance floor is never hit
+    case = dict(HESTON, strike=100.0, kappa=2.0, theta=0.05, xi=0.3, rho=-0.6, v0=0.045,
+                num_paths=20_000, num_steps=50)
+    _check_fd(case, rng, False, 5e-4, 1e-6)
+    case_b = dict(case, payoff="barrier", barrier_dir="up", barrier_style="out",
+                  barrier_level=200.0)
+    _check_fd(case_b, rng, True, 5e-4, 1e-6)
+
+
+def test_adjoint_price_equals_pricing_kernel(gpu):
+    """The adjoint's forward sweep is the pricing path: same plain price."""
+    from openferric_amd import price_raw
+    for case in [dict(num_paths=10_000, num_steps=37, seed=5),
+                 dict(HESTON, num_paths=10_000, num_steps=37)]:
+        for rng in ("reference", "fast"):
+            oo, g, keep = make_pair(**case)
+            p = price_raw([g], _cfg(rng, True))[0]
+            a = _adjoint(g, rng, True)
+            assert abs(a.price - p.price) <= 1e-13 * abs(p.price)
+
+
+def test_fd_greek_mode_matches_adjoint(gpu):
+    from openferric_amd import price_raw, _lib
+    oo, g, keep = make_pair(num_paths=40_000, num_steps=32, seed=3)
+    fd = price_raw([g], _cfg("fast", True, greek=_lib.GREEKS_FD_CRN, bump=1e-5))[0]
+    ad = _adjoint(g, "fast", True)
+    for name in ("delta", "vega", "rho", "theta"):
+        a, f = getattr(ad, name), getattr(fd, name)
+        assert abs(a - f) <= 1e-3 * abs(f) + 1e-5, (name, a, f)
