@@ -63,8 +63,14 @@ def to_ofg(d, seed=42):
     from openferric_amd.engines import _Keep
     keep = _Keep()
     o = _lib.OfgOption()
-    o.model = _lib.MODEL_GBM
-    o.payoff = _lib.PAYOFF_ASIAN if d["payoff"] == "asian" else _lib.PAYOFF_VANILLA
+    o.model = _lib.MODEL_HESTON if d.get("model") == "heston" else _lib.MODEL_GBM
+    o.payoff = {"asian": _lib.PAYOFF_ASIAN, "barrier": _lib.PAYOFF_BARRIER}.get(
+        d["payoff"], _lib.PAYOFF_VANILLA)
+    if o.model == _lib.MODEL_HESTON:
+        o.kappa, o.theta, o.xi, o.rho, o.v0 = d["kappa"], d["theta"], d["xi"], d["rho"], d["v0"]
+    if d["payoff"] == "barrier":
+        o.barrier_dir, o.barrier_style = _lib.UP, _lib.OUT
+        o.barrier_level, o.rebate = d["barrier_level"], 0.0
     o.option_type = _lib.CALL if d["option_type"] == "call" else _lib.PUT
     o.spot, o.strike, o.expiry = d["spot"], d["strike"], d["expiry"]
     o.rate, o.div_yield, o.vol = d["rate"], d["q"], d["vol"]
@@ -76,11 +82,150 @@ def to_ofg(d, seed=42):
     return o, keep
 
 
+def greeks_workload(n_opts=64, seed=WORKLOAD_SEED + 4, paths=1 << 20, steps=252):
+    """Adjoint Greeks workload: GBM arithmetic-Asian calls, daily observation,
+    S0 = 100, K U[80,120], sigma U[0.15,0.45], r = 0.05, q = 0, T = 1."""
+    rng = np.random.default_rng(seed)
+    strike = rng.uniform(80.0, 120.0, n_opts)
+    sigma = rng.uniform(0.15, 0.45, n_opts)
+    obs = [1.0 * (j / steps) for j in range(1, steps + 1)]
+    return [dict(payoff="asian", option_type="call", spot=100.0, strike=float(strike[i]),
+                 expiry=1.0, rate=0.05, q=0.0, vol=float(sigma[i]), num_steps=steps,
+                 num_paths=paths, obs_times=obs) for i in range(n_opts)]
+
+
+def heston_workload(n_opts=256, paths=1 << 20, steps=365, seed=WORKLOAD_SEED + 3):
+    """Heston up-and-out calls, daily monitoring: kappa U[0.5,5], theta
+    U[0.01,0.1], xi U[0.1,1], rho U[-0.9,0], v0 U[0.01,0.1], barrier = spot x
+    U[1.1,1.5], S0 = 100, K U[80,120], r U[0,0.08], q = 0, T = 1."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_opts):
+        out.append(dict(model="heston", payoff="barrier", option_type="call", spot=100.0,
+                        strike=float(rng.uniform(80, 120)), expiry=1.0,
+                        rate=float(rng.uniform(0, 0.08)), q=0.0, vol=0.2,
+                        kappa=float(rng.uniform(0.5, 5)), theta=float(rng.uniform(0.01, 0.1)),
+                        xi=float(rng.uniform(0.1, 1.0)), rho=float(rng.uniform(-0.9, 0.0)),
+                        v0=float(rng.uniform(0.01, 0.1)),
+                        barrier_level=100.0 * float(rng.uniform(1.1, 1.5)),
+                        num_steps=steps, num_paths=paths, obs_times=None))
+    return out
+
+
+def heston_greeks_workload(n_opts=1024, paths=1 << 18, seed=WORKLOAD_SEED + 5):
+    """Batched Heston calibration gradients: alternating European / up-and-out
+    calls, T U[0.05,5], steps ceil(252 T), Heston ranges as above."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_opts):
+        T = float(rng.uniform(0.05, 5.0))
+        barrier = i % 2 == 1
+        out.append(dict(model="heston", payoff="barrier" if barrier else "vanilla",
+                        option_type="call", spot=100.0, strike=float(rng.uniform(80, 120)),
+                        expiry=T, rate=float(rng.uniform(0, 0.08)), q=0.0, vol=0.2,
+                        kappa=float(rng.uniform(0.5, 5)), theta=float(rng.uniform(0.01, 0.1)),
+                        xi=float(rng.uniform(0.1, 1.0)), rho=float(rng.uniform(-0.9, 0.0)),
+                        v0=float(rng.uniform(0.01, 0.1)),
+                        barrier_level=100.0 * float(rng.uniform(1.1, 1.5)),
+                        num_steps=int(np.ceil(252 * T)), num_paths=paths, obs_times=None))
+    return out
+
+
 def path_steps_and_flops(opts):
     ps = sum(d["num_paths"] * d["num_steps"] for d in opts)
     fl = sum(d["num_paths"] * d["num_steps"] * (FLOP_ASIAN_STEP if d["payoff"] == "asian"
                                                 else FLOP_GBM_STEP) for d in opts)
     return ps, fl
+
+
+FLOP_HESTON_STEP = 124       # canonical FLOP per Heston path-step
+
+
+def greeks_leg(rng_mode):
+    """Adjoint vs bump-and-revalue on the Greeks workload (same GPU, same
+    draws). Greeks: delta, vega, rho, theta (4 per option); bump-and-revalue
+    = central differences with common random numbers, 2k+1 = 9 pricing passes
+    in one batched launch."""
+    from openferric_amd import engine_cfg, price_raw, _lib
+    opts = greeks_workload()
+    structs, keeps = zip(*[to_ofg(d) for d in opts])
+    structs = list(structs)
+    base = engine_cfg(rng_mode=rng_mode)
+    price_raw(structs, base)  # warm-up
+    p = price_raw(structs, base)
+    adj = price_raw(structs, base, greeks=True)
+    adj = price_raw(structs, base, greeks=True)
+    fd = price_raw(structs, engine_cfg(rng_mode=rng_mode, greek_mode=_lib.GREEKS_FD_CRN,
+                                       bump_rel=1e-4))
+    t_price, t_adj, t_fd = p[0].kernel_ms, adj[0].kernel_ms, fd[0].kernel_ms
+    n_greeks = 4 * len(opts)
+    rel = []
+    for a, f in zip(adj, fd):
+        for k in ("delta", "vega", "rho", "theta"):
+            fa, ff = getattr(a, k), getattr(f, k)
+            rel.append(abs(fa - ff) / max(abs(ff), 1e-12))
+    ps = sum(d["num_paths"] * d["num_steps"] for d in opts)
+    return {"workload": "64 GBM arithmetic-Asian calls (daily obs), 2^20 paths x 252 steps, "
+                        "K U[80,120], sigma U[0.15,0.45]",
+            "greeks": ["delta", "vega", "rho", "theta"],
+            "pricing_ms": t_price, "adjoint_ms": t_adj, "bump_ms": t_fd,
+            "adjoint_cost_in_pricing_passes": t_adj / t_price,
+            "adjoint_greeks_per_s": n_greeks / (t_adj * 1e-3),
+            "bump_greeks_per_s": n_greeks / (t_fd * 1e-3),
+            "adjoint_speedup_vs_bump": t_fd / t_adj,
+            "pricing_path_steps_per_s": ps / (t_price * 1e-3),
+            "adjoint_vs_bump_max_rel_diff": max(rel), "adjoint_vs_bump_median_rel_diff":
+                float(np.median(rel))}
+
+
+def heston_leg(rng_mode):
+    """Heston up-and-out pricing: 256 options x 2^20 paths x 365 steps."""
+    from openferric_amd import BatchPricer, engine_cfg
+    opts = heston_workload()
+    structs, keeps = zip(*[to_ofg(d) for d in opts])
+    b = BatchPricer(list(structs), engine_cfg(rng_mode=rng_mode), use_graph=True)
+    b.run()
+    ms = [b.run() for _ in range(2)]
+    b.close()
+    ps = sum(d["num_paths"] * d["num_steps"] for d in opts)
+    k_ms = float(np.mean(ms))
+    v = ps / (k_ms * 1e-3)
+    return {"workload": "256 Heston up-and-out calls, 2^20 paths x 365 steps", "path_steps": ps,
+            "kernel_ms": k_ms, "path_steps_per_s": v,
+            "frac_fp64_peak": v * FLOP_HESTON_STEP / 1e12 / FP64_PEAK_TFLOPS}
+
+
+def heston_greeks_leg(rng_mode):
+    """Batched Heston adjoint: 1024 options, 2^18 paths, steps ceil(252 T)."""
+    from openferric_amd import engine_cfg, price_raw
+    opts = heston_greeks_workload()
+    structs, keeps = zip(*[to_ofg(d) for d in opts])
+    structs = list(structs)
+    cfg = engine_cfg(rng_mode=rng_mode)
+    p = price_raw(structs, cfg)
+    a = price_raw(structs, cfg, greeks=True)
+    ps = sum(d["num_paths"] * d["num_steps"] for d in opts)
+    return {"workload": "1024 Heston European/up-and-out calls, 2^18 paths, steps ceil(252T)",
+            "path_steps": ps, "pricing_ms": p[0].kernel_ms, "adjoint_ms": a[0].kernel_ms,
+            "adjoint_cost_in_pricing_passes": a[0].kernel_ms / p[0].kernel_ms,
+            "adjoint_greeks_per_s": 6 * len(opts) / (a[0].kernel_ms * 1e-3)}
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB (a plain copy kernel of known
+    size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n}), flush=True)
 
 
 def cpu_threads():
@@ -94,18 +239,18 @@ def cpu_threads():
     return max(1, n)
 
 
-def cpu_baseline(opts, target_s=12.0):
+def cpu_baseline(opts, target_s=15.0):
     """Times the oracle (C restatement of MonteCarloEngine::run, all host
     threads) on a fixed subset of the workload: every 32nd option, paths
-    scaled so the sample is ~target_s of CPU work."""
+    scaled so the measured sample is ~target_s of CPU work (the scale is
+    found by short calibration runs, the last run is the one reported)."""
     from oracle import oracle as O
     O.build()
     sub = opts[::32]
     threads = cpu_threads()
-    # calibrate on a small slice
-    probe = [dict(d, num_paths=max(64, d["num_paths"] // 200)) for d in sub[:8]]
 
-    def run(ds):
+    def run(scale):
+        ds = [dict(d, num_paths=max(64, int(d["num_paths"] * scale))) for d in sub]
         ps = 0
         t0 = time.perf_counter()
         for d in ds:
@@ -118,14 +263,14 @@ def cpu_baseline(opts, target_s=12.0):
             ps += d["num_paths"] * d["num_steps"]
         return ps, time.perf_counter() - t0
 
-    ps, dt = run(probe)
-    rate = ps / max(dt, 1e-9)
-    full_ps, _ = path_steps_and_flops(sub)
-    scale = min(1.0, target_s * rate / max(full_ps, 1))
-    sample = [dict(d, num_paths=max(64, int(d["num_paths"] * scale))) for d in sub]
-    ps, dt = run(sample)
+    scale = 1.0 / 256
+    for _ in range(6):
+        ps, dt = run(scale)
+        if dt >= 0.5 * target_s or scale >= 1.0:
+            break
+        scale = min(1.0, scale * target_s / max(dt, 1e-3))
     return {"value": ps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sample)} options (every 32nd of the batch), paths x{scale:.4f}, "
+            "sample": f"{len(sub)} options (every 32nd of the batch), paths x{scale:.4f}, "
                       f"{ps:.3e} path-steps in {dt:.1f} s, oracle C restatement, "
                       f"OpenMP {threads} threads"}
 
@@ -139,7 +284,16 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--path-scale", type=float, default=1.0,
                     help="scale per-option path counts (profiling runs only)")
+    ap.add_argument("--no-greeks", action="store_true", help="skip the adjoint Greeks leg")
+    ap.add_argument("--heston", action="store_true", help="add the Heston barrier pricing leg")
+    ap.add_argument("--heston-greeks", action="store_true",
+                    help="add the batched Heston adjoint leg")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run one D2D copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md)"}))
@@ -199,6 +353,15 @@ def main():
                      "kernel_ms_avg": k_ms,
                      "flop_model": "65 FLOP per GBM path-step, 66 with Asian running sum"},
     }
+    if not args.no_greeks:
+        print("[bench] adjoint greeks leg", file=sys.stderr, flush=True)
+        out["greeks"] = greeks_leg(RngMode.Fast)
+    if args.heston:
+        print("[bench] heston leg", file=sys.stderr, flush=True)
+        out["heston"] = heston_leg(RngMode.Fast)
+    if args.heston_greeks:
+        print("[bench] heston greeks leg", file=sys.stderr, flush=True)
+        out["heston_greeks"] = heston_greeks_leg(RngMode.Fast)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(opts)

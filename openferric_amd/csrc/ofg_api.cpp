@@ -622,6 +622,11 @@ int32_t price_fd(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engin
                  ofg_result *out) {
     const double h = cfg->bump_rel > 0 ? cfg->bump_rel : 1e-4;
     std::vector<ofg_option> all;
+    // Bumped expiries carry their Asian schedule scaled by T'/T, so the
+    // observation index map round((t/T) n) is unchanged (the adjoint holds it
+    // fixed as well) and every fixing stays inside [0, T'].
+    std::vector<std::vector<double>> scaled;
+    scaled.reserve(2 * n);
     struct Spec { size_t base; int nb; };
     std::vector<Spec> spec(n);
     for (size_t i = 0; i < n; ++i) {
@@ -643,6 +648,14 @@ int32_t price_fd(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engin
             add(&ofg_option::vol, 1e-8);
             add(&ofg_option::rate, h);
             add(&ofg_option::expiry, 1e-8);
+            if (o.payoff == OFG_PAYOFF_ASIAN && o.n_obs > 0 && o.obs_times && o.expiry > 0.0) {
+                for (int side = 0; side < 2; ++side) {
+                    ofg_option &b = all[all.size() - 2 + side];
+                    scaled.emplace_back(o.obs_times, o.obs_times + o.n_obs);
+                    for (double &t : scaled.back()) t = std::fmin(t * (b.expiry / o.expiry), b.expiry);
+                    b.obs_times = scaled.back().data();
+                }
+            }
             spec[i].nb = 4;
         } else {
             add(&ofg_option::spot, 1e-8);
