@@ -31,6 +31,10 @@ FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector peak
 FLOP_GBM_STEP = 65            # canonical FLOP per GBM path-step
 FLOP_ASIAN_STEP = 66          # + arithmetic running sum
 WORKLOAD_SEED = 0x0FE22C1C
+# HBM bytes per launch of the pricing kernel on the default workload, from
+# rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) scaled by a 1 GiB
+# copy kernel (profiles/price_kernel_fast_gbm_batch4096_counters.md).
+PRICE_KERNEL_HBM_BYTES = 57.6e6 + 35.7e6
 
 
 def make_workload(n_opts=4096, seed=WORKLOAD_SEED, path_scale=1.0):
@@ -241,12 +245,12 @@ def cpu_threads():
 
 def cpu_baseline(opts, target_s=15.0):
     """Times the oracle (C restatement of MonteCarloEngine::run, all host
-    threads) on a fixed subset of the workload: every 32nd option, paths
+    threads) on a fixed subset of the workload: every 8th option, paths
     scaled so the measured sample is ~target_s of CPU work (the scale is
     found by short calibration runs, the last run is the one reported)."""
     from oracle import oracle as O
     O.build()
-    sub = opts[::32]
+    sub = opts[::8]
     threads = cpu_threads()
 
     def run(scale):
@@ -270,7 +274,7 @@ def cpu_baseline(opts, target_s=15.0):
             break
         scale = min(1.0, scale * target_s / max(dt, 1e-3))
     return {"value": ps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sub)} options (every 32nd of the batch), paths x{scale:.4f}, "
+            "sample": f"{len(sub)} options (every 8th of the batch), paths x{scale:.4f}, "
                       f"{ps:.3e} path-steps in {dt:.1f} s, oracle C restatement, "
                       f"OpenMP {threads} threads"}
 
@@ -348,7 +352,8 @@ def main():
                    "options": len(opts), "path_steps": ps, "graph": not args.no_graph,
                    "path_scale": args.path_scale},
         "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": PRICE_KERNEL_HBM_BYTES if args.path_scale == 1.0 else None,
                      "kernel": "price_kernel<RNG_FAST,256>",
                      "kernel_ms_avg": k_ms,
                      "flop_model": "65 FLOP per GBM path-step, 66 with Asian running sum"},

@@ -147,22 +147,6 @@ __device__ __forceinline__ void seg_reverse(RngOf<RNG> &rng, const ExternalNorma
     }
 }
 
-// Heston step returning the growth factor; bitwise the same operations as
-// heston_step in ofg_paths.hpp.
-__device__ __forceinline__ void heston_fwd(double s, double v, const OptDev &o, double z1,
-                                           double z2, double &s_next, double &v_next,
-                                           double &growth) {
-    const double v_pos = fmax(v, 0.0);
-    const double sv = sqrt(v_pos);
-    const double zs = o.rho * z1 + o.rho_c * z2;
-    const double vn = v + o.kappa * (o.theta - v_pos) * o.dt + o.xi * sv * o.sqrt_dt * z1;
-    const double e = (o.mu - 0.5 * v_pos) * o.dt + sv * o.sqrt_dt * zs;
-    growth = exp_fp64(e);
-    const double sn = s * growth;
-    s_next = !(sn >= 1e-12) ? 1e-12 : sn;
-    v_next = fmax(vn, 0.0);
-}
-
 struct Grad {
     double g[ADJ_NG];
 };
@@ -246,7 +230,7 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
         for (int k = 0; k < K; ++k) {
             if (k < len) {
                 if constexpr (MODEL == OFG_MODEL_GBM) gbm_step<PK>(g, o, z1[k]);
-                else heston_step<PK>(g, v, o, z1[k], z2[k]);
+                else heston_step<PK, RNG != OFG_RNG_FAST>(g, v, o, z1[k], z2[k]);
                 leg_observe<PK>(g, ASIAN ? w[j0 + k + 1] : 0.0, bsign, o.barrier);
             }
         }
@@ -282,7 +266,8 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
                     E[k] = exp_fp64(fma(o.diffusion, z1[k], o.drift));
                     S[k + 1] = S[k] * E[k];
                 } else {
-                    heston_fwd(S[k], V[k], o, z1[k], z2[k], S[k + 1], V[k + 1], E[k]);
+                    double e_unused;
+                    heston_euler<RNG != OFG_RNG_FAST>(S[k], V[k], o, z1[k], z2[k], S[k + 1], V[k + 1], E[k], e_unused);
                 }
             }
         }

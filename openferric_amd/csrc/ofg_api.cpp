@@ -344,6 +344,11 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
         d.log_s0 = std::log(std::fmax(o.spot, 1e-12));
         d.df = std::exp(-o.rate * o.expiry);  // mc_engine.rs:945
         d.rate = o.rate;
+        d.kdt = o.kappa * d.dt;
+        d.kthdt = o.kappa * o.theta * d.dt;
+        d.xsdt = o.xi * d.sqrt_dt;
+        d.mudt = d.mu * d.dt;
+        d.hdt = 0.5 * d.dt;
         d.antithetic = cfg.antithetic ? 1 : 0;
         int32_t flags = 0;
         if (o.option_type == OFG_PUT) flags |= ofg::F_PUT;

@@ -44,6 +44,8 @@ struct alignas(16) OptDev {
     double kappa, theta, xi, rho, rho_c, v0;
     double barrier, rebate, nobs, log_s0;
     double df, cv_expected, rate;
+    // Heston step constants of the fast mode's re-associated Euler step
+    double kdt, kthdt, xsdt, mudt, hdt;
     int32_t antithetic;
     int32_t has_cv;
 };

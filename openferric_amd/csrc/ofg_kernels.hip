@@ -104,8 +104,8 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
     } else {
         double va = o.v0, vb = o.v0;
         auto advance = [&](double z1, double z2, int j) {
-            heston_step<PK>(ga, va, o, z1, z2);
-            if constexpr (ANTI) heston_step<PK>(gb, vb, o, -z1, -z2);
+            heston_step<PK, RNG != OFG_RNG_FAST>(ga, va, o, z1, z2);
+            if constexpr (ANTI) heston_step<PK, RNG != OFG_RNG_FAST>(gb, vb, o, -z1, -z2);
             const double wj = ASIAN ? w[j + 1] : 0.0;
             leg_observe<PK>(ga, wj, bsign, barrier);
             if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);

@@ -51,21 +51,46 @@ __device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z) {
 }
 
 // One Heston Euler step (stochastic.rs:56-71) plus the generator clamps
-// (simulation.rs:112-113). Operation order follows the reference; the file is
-// compiled with -ffp-contract=off.
-template <int PK>
+// (simulation.rs:112-113), returning the spot growth factor exp(e).
+//   REF = true : the reference's operation order (parity modes; the file is
+//                compiled with -ffp-contract=off);
+//   REF = false: the same scheme re-associated into fused multiply-adds with
+//                per-option constants (fast mode): kappa dt, kappa theta dt,
+//                xi sqrt(dt), mu dt, dt / 2.
+template <bool REF>
+__device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o, double z1,
+                                             double z2, double &s_next, double &v_next,
+                                             double &growth, double &e) {
+    const double v_pos = fmax(v, 0.0);
+    double vn;
+    if constexpr (REF) {
+        const double sv = sqrt(v_pos);
+        const double zs = o.rho * z1 + o.rho_c * z2;
+        vn = v + o.kappa * (o.theta - v_pos) * o.dt + o.xi * sv * o.sqrt_dt * z1;
+        e = (o.mu - 0.5 * v_pos) * o.dt + sv * o.sqrt_dt * zs;
+    } else {
+        const double sv = v_pos > 0.0 ? sqrt_pos(v_pos) : 0.0;
+        const double zs = fma(o.rho, z1, o.rho_c * z2);
+        vn = fma(o.xsdt * sv, z1, fma(-o.kdt, v_pos, v + o.kthdt));
+        e = fma(sv * o.sqrt_dt, zs, fma(-o.hdt, v_pos, o.mudt));
+    }
+    growth = exp_fp64(e);
+    const double sn = s * growth;
+    s_next = !(sn >= 1e-12) ? 1e-12 : sn;   // f64::max(sn, 1e-12), NaN -> 1e-12
+    v_next = fmax(vn, 0.0);
+}
+
+template <int PK, bool REF>
 __device__ __forceinline__ void heston_step(Leg &g, double &v, const OptDev &o, double z1,
                                             double z2) {
-    const double v_pos = fmax(v, 0.0);
-    const double sv = sqrt(v_pos);
-    const double zs = o.rho * z1 + o.rho_c * z2;
-    const double vn = v + o.kappa * (o.theta - v_pos) * o.dt + o.xi * sv * o.sqrt_dt * z1;
-    const double e = (o.mu - 0.5 * v_pos) * o.dt + sv * o.sqrt_dt * zs;
-    const double sn = g.s * exp_fp64(e);
-    const bool floor_hit = !(sn >= 1e-12);
-    g.s = floor_hit ? 1e-12 : sn;   // f64::max(sn, 1e-12), NaN -> 1e-12
-    if constexpr (PK == PK_ASIAN_LOG) g.ls = floor_hit ? -27.631021115928547 : g.ls + e;
-    v = fmax(vn, 0.0);
+    double s_next, v_next, growth, e;
+    heston_euler<REF>(g.s, v, o, z1, z2, s_next, v_next, growth, e);
+    if constexpr (PK == PK_ASIAN_LOG) {
+        const bool floor_hit = !(g.s * growth >= 1e-12);
+        g.ls = floor_hit ? -27.631021115928547 : g.ls + e;   // ln(1e-12) at the floor
+    }
+    g.s = s_next;
+    v = v_next;
 }
 
 template <int PK>
