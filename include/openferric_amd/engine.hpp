@@ -1,0 +1,371 @@
+// C++ mirror of OpenFerric's pricing contract over the openferric_amd C ABI.
+//
+// Same names, argument meaning and error behaviour as the reference's Rust
+// types, so the GPU engine is a drop-in beside MonteCarloPricingEngine:
+//   Greeks, PricingResult, PricingError          src/core/engine.rs:15-315
+//   OptionType, BarrierSpec, AsianSpec, ...      src/core/types.rs:13-122
+//   VanillaOption / AsianOption / BarrierOption  src/instruments/*.rs
+//   Market (flat vol)                            src/market/market.rs:202-414
+//   Heston                                       src/models/stochastic.rs:37-72
+//   PricingEngine<I>::price / price_with_greeks_aad (engine.rs:37-52)
+// Errors are C++ exceptions carrying the PricingError variant (Rust returns
+// Result<_, PricingError>); every call goes through libopenferric_amd.so.
+#pragma once
+
+#include "../openferric_amd.h"
+
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace openferric_amd {
+
+// ---------------------------------------------------------------- errors --
+class PricingError : public std::runtime_error {
+public:
+    enum class Kind { InvalidInput, ConvergenceFailure, MarketDataMissing, NumericalError, Device };
+    PricingError(Kind k, const std::string &msg) : std::runtime_error(prefix(k) + msg), kind_(k) {}
+    Kind kind() const { return kind_; }
+
+private:
+    static std::string prefix(Kind k) {
+        switch (k) {
+        case Kind::InvalidInput: return "invalid input: ";
+        case Kind::ConvergenceFailure: return "convergence failure: ";
+        case Kind::MarketDataMissing: return "market data missing: ";
+        case Kind::NumericalError: return "numerical error: ";
+        default: return "device error: ";
+        }
+    }
+    Kind kind_;
+};
+
+inline void check(int32_t rc) {
+    if (rc == OFG_OK) return;
+    const std::string msg = ofg_last_error();
+    switch (rc) {
+    case OFG_INVALID_INPUT: throw PricingError(PricingError::Kind::InvalidInput, msg);
+    case OFG_CONVERGENCE_FAILURE: throw PricingError(PricingError::Kind::ConvergenceFailure, msg);
+    case OFG_MARKET_DATA_MISSING: throw PricingError(PricingError::Kind::MarketDataMissing, msg);
+    case OFG_NUMERICAL_ERROR: throw PricingError(PricingError::Kind::NumericalError, msg);
+    default: throw PricingError(PricingError::Kind::Device, msg);
+    }
+}
+
+// ------------------------------------------------------------ contract --
+enum class OptionType { Call, Put };
+enum class ExerciseStyle { European, American };
+enum class BarrierDirection { Up, Down };
+enum class BarrierStyle { In, Out };
+enum class Averaging { Arithmetic, Geometric };
+enum class StrikeType { Fixed, Floating };
+
+struct Greeks {
+    double delta = 0, gamma = 0, vega = 0, theta = 0, rho = 0;
+};
+
+// Diagnostics: the reference's 8-slot key/value store (engine.rs:191-274).
+class Diagnostics {
+public:
+    static constexpr size_t CAPACITY = 8;
+    std::optional<double> insert(const std::string &key, double v) {
+        auto it = map_.find(key);
+        if (it != map_.end()) {
+            const double prev = it->second;
+            it->second = v;
+            return prev;
+        }
+        if (map_.size() >= CAPACITY)
+            throw std::length_error("diagnostics capacity exceeded (8)");
+        map_[key] = v;
+        return std::nullopt;
+    }
+    std::optional<double> get(const std::string &key) const {
+        auto it = map_.find(key);
+        if (it == map_.end()) return std::nullopt;
+        return it->second;
+    }
+    size_t len() const { return map_.size(); }
+    bool is_empty() const { return map_.empty(); }
+
+private:
+    std::map<std::string, double> map_;
+};
+
+struct PricingResult {
+    double price = 0;
+    std::optional<double> stderr_;
+    std::optional<Greeks> greeks;
+    Diagnostics diagnostics;
+    // dPrice/d{S0, v0, kappa, theta, xi, rho} for Heston adjoint runs.
+    std::optional<std::vector<double>> heston_gradient;
+    double kernel_ms = 0;
+};
+
+struct VanillaOption {
+    OptionType option_type = OptionType::Call;
+    double strike = 0, expiry = 0;
+    ExerciseStyle exercise = ExerciseStyle::European;
+    static VanillaOption european_call(double k, double t) { return {OptionType::Call, k, t}; }
+    static VanillaOption european_put(double k, double t) { return {OptionType::Put, k, t}; }
+};
+
+struct AsianSpec {
+    Averaging averaging = Averaging::Arithmetic;
+    StrikeType strike_type = StrikeType::Fixed;
+    std::vector<double> observation_times;
+};
+
+struct AsianOption {
+    OptionType option_type = OptionType::Call;
+    double strike = 0, expiry = 0;
+    AsianSpec asian;
+};
+
+struct BarrierSpec {
+    BarrierDirection direction = BarrierDirection::Up;
+    BarrierStyle style = BarrierStyle::Out;
+    double level = 0, rebate = 0;
+};
+
+struct BarrierOption {
+    OptionType option_type = OptionType::Call;
+    double strike = 0, expiry = 0;
+    BarrierSpec barrier;
+};
+
+// Market with a flat vol source; builder validation as MarketBuilder::build.
+struct Market {
+    double spot = 0, rate = 0, dividend_yield = 0, flat_vol = 0;
+    double vol_for(double, double) const { return flat_vol; }
+
+    class Builder {
+    public:
+        Builder &spot(double s) { spot_ = s; return *this; }
+        Builder &rate(double r) { rate_ = r; return *this; }
+        Builder &dividend_yield(double q) { q_ = q; return *this; }
+        Builder &flat_vol(double v) { vol_ = v; return *this; }
+        Market build() const {
+            if (!spot_) throw PricingError(PricingError::Kind::InvalidInput, "market spot is required");
+            if (*spot_ <= 0.0) throw PricingError(PricingError::Kind::InvalidInput, "market spot must be > 0");
+            if (!vol_)
+                throw PricingError(PricingError::Kind::InvalidInput,
+                                   "either market flat_vol or vol_surface is required");
+            if (*vol_ <= 0.0) throw PricingError(PricingError::Kind::InvalidInput, "market flat_vol must be > 0");
+            return Market{*spot_, rate_, q_, *vol_};
+        }
+
+    private:
+        std::optional<double> spot_, vol_;
+        double rate_ = 0, q_ = 0;
+    };
+    static Builder builder() { return Builder{}; }
+};
+
+struct Heston {
+    double mu, kappa, theta, xi, rho, v0;
+    bool validate() const {
+        return kappa > 0.0 && theta >= 0.0 && xi >= 0.0 && v0 >= 0.0 && rho > -1.0 && rho < 1.0;
+    }
+};
+
+enum class VarianceReduction { None, Antithetic, ControlVariate };
+enum class RngMode { Reference = OFG_RNG_REFERENCE, Fast = OFG_RNG_FAST };
+
+// ------------------------------------------------------------- context --
+// One process-wide context per device (the engine's OnceLock analogue).
+inline ofg_ctx *device_context(int device = 0) {
+    static std::mutex mu;
+    static std::map<int, ofg_ctx *> ctxs;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ctxs.find(device);
+    if (it != ctxs.end()) return it->second;
+    ofg_ctx *c = ofg_create(device);
+    if (!c) throw PricingError(PricingError::Kind::Device, ofg_last_error());
+    ctxs[device] = c;
+    return c;
+}
+
+namespace detail {
+inline ofg_option base(OptionType t, double k, double T, const Market &m, int64_t paths,
+                       int64_t steps, uint64_t seed) {
+    ofg_option o{};
+    o.model = OFG_MODEL_GBM;
+    o.option_type = t == OptionType::Call ? OFG_CALL : OFG_PUT;
+    o.spot = m.spot;
+    o.strike = k;
+    o.expiry = T;
+    o.rate = m.rate;
+    o.div_yield = m.dividend_yield;
+    o.num_paths = paths;
+    o.num_steps = steps;
+    o.seed = seed;
+    return o;
+}
+inline ofg_option to_ofg(const VanillaOption &i, const Market &m, int64_t p, int64_t s, uint64_t seed) {
+    ofg_option o = base(i.option_type, i.strike, i.expiry, m, p, s, seed);
+    o.payoff = OFG_PAYOFF_VANILLA;
+    o.vol = m.vol_for(i.strike, i.expiry);
+    return o;
+}
+inline ofg_option to_ofg(const AsianOption &i, const Market &m, int64_t p, int64_t s, uint64_t seed) {
+    ofg_option o = base(i.option_type, i.strike, i.expiry, m, p, s, seed);
+    o.payoff = OFG_PAYOFF_ASIAN;
+    o.averaging = i.asian.averaging == Averaging::Arithmetic ? OFG_ARITHMETIC : OFG_GEOMETRIC;
+    o.strike_type = i.asian.strike_type == StrikeType::Fixed ? OFG_FIXED : OFG_FLOATING;
+    o.n_obs = (int32_t)i.asian.observation_times.size();
+    o.obs_times = i.asian.observation_times.data();
+    const double ref = i.asian.strike_type == StrikeType::Fixed ? i.strike : m.spot;
+    o.vol = m.vol_for(std::fmax(ref, 1e-12), i.expiry);
+    return o;
+}
+inline ofg_option to_ofg(const BarrierOption &i, const Market &m, int64_t p, int64_t s, uint64_t seed) {
+    ofg_option o = base(i.option_type, i.strike, i.expiry, m, p, s, seed);
+    o.payoff = OFG_PAYOFF_BARRIER;
+    o.barrier_dir = i.barrier.direction == BarrierDirection::Up ? OFG_UP : OFG_DOWN;
+    o.barrier_style = i.barrier.style == BarrierStyle::In ? OFG_IN : OFG_OUT;
+    o.barrier_level = i.barrier.level;
+    o.rebate = i.barrier.rebate;
+    o.vol = m.vol_for(i.strike, i.expiry);
+    return o;
+}
+
+inline PricingResult to_result(const ofg_result &r, bool greeks, bool heston) {
+    PricingResult out;
+    out.price = r.price;
+    if (r.has_stderr) out.stderr_ = r.stderr_;
+    if (r.num_paths != 0.0 || r.num_steps != 0.0) {
+        out.diagnostics.insert("num_paths", r.num_paths);
+        out.diagnostics.insert("num_steps", r.num_steps);
+        out.diagnostics.insert("vol", r.vol);
+    }
+    if (greeks && r.has_greeks) {
+        out.greeks = Greeks{r.delta, r.gamma, r.vega, r.theta, r.rho};
+        if (heston) out.heston_gradient = std::vector<double>(r.heston_grad, r.heston_grad + 6);
+    }
+    out.kernel_ms = r.kernel_ms;
+    return out;
+}
+}  // namespace detail
+
+// PricingEngine<I> (engine.rs:37-52).
+template <class I>
+class PricingEngine {
+public:
+    virtual ~PricingEngine() = default;
+    virtual PricingResult price(const I &instrument, const Market &market) const = 0;
+    virtual PricingResult price_with_greeks_aad(const I &instrument, const Market &market) const {
+        return price(instrument, market);
+    }
+};
+
+// Drop-in for MonteCarloPricingEngine (mc_engine.rs:587-670, 871-988):
+// every supported instrument, GBM paths on the GPU.
+class GpuMonteCarloPricingEngine : public PricingEngine<VanillaOption>,
+                                   public PricingEngine<AsianOption>,
+                                   public PricingEngine<BarrierOption> {
+public:
+    GpuMonteCarloPricingEngine(int64_t num_paths, int64_t num_steps, uint64_t seed)
+        : num_paths(num_paths), num_steps(num_steps), seed(seed) {}
+    GpuMonteCarloPricingEngine &with_variance_reduction(VarianceReduction vr) {
+        variance_reduction = vr;
+        return *this;
+    }
+    GpuMonteCarloPricingEngine &with_seed(uint64_t s) {
+        seed = s;
+        return *this;
+    }
+    GpuMonteCarloPricingEngine &with_rng_mode(RngMode m) {
+        rng_mode = m;
+        return *this;
+    }
+
+    PricingResult price(const VanillaOption &i, const Market &m) const override { return run(i, m, false); }
+    PricingResult price(const AsianOption &i, const Market &m) const override { return run(i, m, false); }
+    PricingResult price(const BarrierOption &i, const Market &m) const override { return run(i, m, false); }
+    PricingResult price_with_greeks_aad(const VanillaOption &i, const Market &m) const override {
+        return run(i, m, true);
+    }
+    PricingResult price_with_greeks_aad(const AsianOption &i, const Market &m) const override {
+        return run(i, m, true);
+    }
+    PricingResult price_with_greeks_aad(const BarrierOption &i, const Market &m) const override {
+        return run(i, m, true);
+    }
+
+    int64_t num_paths, num_steps;
+    uint64_t seed;
+    VarianceReduction variance_reduction = VarianceReduction::None;
+    RngMode rng_mode = RngMode::Reference;
+    int device = 0;
+
+private:
+    template <class I>
+    PricingResult run(const I &i, const Market &m, bool greeks) const {
+        ofg_option o = detail::to_ofg(i, m, num_paths, num_steps, seed);
+        ofg_engine_cfg c{};
+        c.antithetic = variance_reduction == VarianceReduction::Antithetic;
+        c.control_variate = variance_reduction == VarianceReduction::ControlVariate && !greeks;
+        c.rng_mode = (int32_t)rng_mode;
+        ofg_result r{};
+        ofg_ctx *ctx = device_context(device);
+        check(greeks ? ofg_price_greeks(ctx, &o, 1, &c, &r) : ofg_price(ctx, &o, 1, &c, &r));
+        return detail::to_result(r, greeks, false);
+    }
+};
+
+// PricingEngine for Heston paths (MonteCarloEngine::run with
+// HestonPathGenerator, simulation.rs:79-117, behind the engine contract).
+class HestonMonteCarloPricingEngine : public PricingEngine<VanillaOption>,
+                                      public PricingEngine<BarrierOption> {
+public:
+    HestonMonteCarloPricingEngine(Heston model, int64_t num_paths, int64_t num_steps, uint64_t seed)
+        : model(model), num_paths(num_paths), num_steps(num_steps), seed(seed) {}
+    HestonMonteCarloPricingEngine &with_antithetic(bool a) {
+        antithetic = a;
+        return *this;
+    }
+    PricingResult price(const VanillaOption &i, const Market &m) const override { return run(i, m, false); }
+    PricingResult price(const BarrierOption &i, const Market &m) const override { return run(i, m, false); }
+    PricingResult price_with_greeks_aad(const VanillaOption &i, const Market &m) const override {
+        return run(i, m, true);
+    }
+    PricingResult price_with_greeks_aad(const BarrierOption &i, const Market &m) const override {
+        return run(i, m, true);
+    }
+
+    Heston model;
+    int64_t num_paths, num_steps;
+    uint64_t seed;
+    bool antithetic = false;
+    RngMode rng_mode = RngMode::Reference;
+    int device = 0;
+
+private:
+    template <class I>
+    PricingResult run(const I &i, const Market &m, bool greeks) const {
+        ofg_option o = detail::to_ofg(i, m, num_paths, num_steps, seed);
+        o.model = OFG_MODEL_HESTON;
+        o.div_yield = m.rate - model.mu;   // drift mu = rate - div_yield
+        o.kappa = model.kappa;
+        o.theta = model.theta;
+        o.xi = model.xi;
+        o.rho = model.rho;
+        o.v0 = model.v0;
+        ofg_engine_cfg c{};
+        c.antithetic = antithetic;
+        c.rng_mode = (int32_t)rng_mode;
+        ofg_result r{};
+        ofg_ctx *ctx = device_context(device);
+        check(greeks ? ofg_price_greeks(ctx, &o, 1, &c, &r) : ofg_price(ctx, &o, 1, &c, &r));
+        return detail::to_result(r, greeks, true);
+    }
+};
+
+}  // namespace openferric_amd

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libopenferric_amd.so")
 SOURCES = ["ofg_kernels.hip", "ofg_adjoint.hip", "ofg_api.cpp"]
-HEADERS = ["ofg_math.hpp", "ofg_internal.hpp", "ofg_adjoint.hpp"]
+HEADERS = ["ofg_math.hpp", "ofg_internal.hpp", "ofg_adjoint.hpp", "ofg_paths.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -56,6 +56,23 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_cpp_tests(verbose: bool = False) -> str:
+    """Builds tests/cpp/test_engine (the reference's engine tests on the C++
+    mirror of the pricing contract), linked against the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_engine.cpp")
+    exe = os.path.join(root, "tests", "cpp", "test_engine")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), src,
+           "-L", HERE, "-lopenferric_amd", "-Wl,-rpath,$ORIGIN/../../openferric_amd", "-o", exe]
+    if verbose:
+        print("g++ test_engine.cpp", flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("C++ mirror test build failed")
+    return exe
+
+
 def dump_asm(src: str = "ofg_kernels.hip", out: str | None = None) -> str:
     """Writes the gfx950 assembly of one source (for instruction counts)."""
     out = out or os.path.join(HERE, "_build", src + ".s")
@@ -67,3 +84,4 @@ def dump_asm(src: str = "ofg_kernels.hip", out: str | None = None) -> str:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cpp_tests(verbose=True))

@@ -1,0 +1,193 @@
+"""Engine surface on the GPU: batches and graph replay, autotuner, error codes
+through the C ABI, the Python and C++ mirrors of PricingEngine, torch autograd."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.cases import daily_obs, make_pair
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _mixed_batch(n=48, seed=3, heston_every=5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        steps = int(rng.integers(5, 120))
+        T = float(rng.uniform(0.2, 2.0))
+        kind = ["vanilla", "asian", "barrier"][i % 3]
+        kw = dict(payoff=kind, option_type="call" if i % 2 else "put",
+                  strike=float(rng.uniform(80, 120)), expiry=T, rate=float(rng.uniform(0, 0.06)),
+                  vol=float(rng.uniform(0.1, 0.5)), num_steps=steps,
+                  num_paths=int(rng.integers(100, 20000)), seed=int(rng.integers(1, 1000)))
+        if kind == "asian":
+            kw["obs_times"] = daily_obs(T, steps)
+        if kind == "barrier":
+            kw.update(barrier_dir="up", barrier_style="out", barrier_level=140.0, rebate=0.5)
+        if i % heston_every == 0:
+            kw.update(model="heston", kappa=1.5, theta=0.04, xi=0.5, rho=-0.6, v0=0.05)
+        out.append(kw)
+    return out
+
+
+@pytest.mark.parametrize("rng,anti", [("reference", False), ("fast", True)])
+def test_batch_equals_individual_prices_and_replays(gpu, rng, anti):
+    """One persistent launch over uneven options gives each option exactly
+    the price of pricing it alone; graph replays repeat it bit for bit."""
+    from openferric_amd import BatchPricer, RngMode, engine_cfg, price_raw
+    mode = {"reference": RngMode.Reference, "fast": RngMode.Fast}[rng]
+    cases = _mixed_batch()
+    pairs = [make_pair(**c) for c in cases]
+    opts = [p[1] for p in pairs]
+    cfg = engine_cfg(antithetic=anti, rng_mode=mode)
+    single = [price_raw([o], cfg)[0].price for o in opts]
+    together = [r.price for r in price_raw(opts, cfg)]
+    np.testing.assert_allclose(together, single, rtol=1e-13, atol=0)
+    for use_graph in (True, False):
+        b = BatchPricer(opts, cfg, use_graph=use_graph)
+        prices = []
+        for _ in range(3):
+            b.run()
+            prices.append([r.price for r in b.results()])
+            # eager work and a sync between replays
+            price_raw(opts[:2], cfg)
+        b.close()
+        for p in prices:
+            np.testing.assert_allclose(p, single, rtol=1e-13, atol=0)
+        assert prices[0] == prices[1] == prices[2]
+
+
+def test_prices_do_not_depend_on_launch_shape(gpu):
+    from openferric_amd import RngMode, engine_cfg, price_raw
+    cases = _mixed_batch(n=12, seed=9)
+    opts = [make_pair(**c)[1] for c in cases]
+    ref = None
+    for block in (64, 128, 256, 512):
+        for item in (512, 4096, 65536):
+            cfg = engine_cfg(rng_mode=RngMode.Fast, tune_block=block, tune_item_steps=item)
+            p = np.array([r.price for r in price_raw(opts, cfg)])
+            if ref is None:
+                ref = p
+            np.testing.assert_allclose(p, ref, rtol=1e-12, atol=1e-14)
+
+
+def test_autotuner_picks_a_shape_and_caches(gpu, tmp_path):
+    from openferric_amd import RngMode, autotune, engine_cfg
+    cases = _mixed_batch(n=8, seed=5, heston_every=100)
+    opts = [make_pair(**c)[1] for c in cases]
+    cache = str(tmp_path / "tune.txt")
+    block, item = autotune(opts, engine_cfg(rng_mode=RngMode.Fast), cache)
+    assert block in (64, 128, 256, 512) and item in (2048, 8192, 32768)
+    assert open(cache).read().split()[1:3] == [str(block), str(item)]
+
+
+def test_error_codes_through_the_c_abi(gpu):
+    from openferric_amd import InvalidInput, engine_cfg, price_raw
+    bad = [
+        (dict(strike=-1.0), "vanilla strike must be > 0"),
+        (dict(expiry=-1.0), "vanilla expiry must be >= 0"),
+        (dict(num_paths=0), "num_paths must be > 0"),
+        (dict(num_steps=0), "num_steps must be > 0"),
+        (dict(vol=0.0), "market volatility must be > 0"),
+        (dict(payoff="asian", obs_times=[0.5, 1.5]),
+         "asian observation_times must lie in [0, expiry]"),
+        (dict(payoff="barrier", barrier_level=0.0), "barrier level must be > 0"),
+        (dict(payoff="barrier", barrier_level=120.0, rebate=-1.0), "barrier rebate must be >= 0"),
+        (dict(model="heston", kappa=0.0, theta=0.04, xi=0.5, rho=-0.5, v0=0.04),
+         "invalid Heston parameters"),
+        (dict(model="heston", kappa=1.0, theta=0.04, xi=0.5, rho=1.0, v0=0.04),
+         "invalid Heston parameters"),
+    ]
+    for kw, msg in bad:
+        _, g, keep = make_pair(**kw)
+        with pytest.raises(InvalidInput) as e:
+            price_raw([g], engine_cfg())
+        assert msg in str(e.value)
+    # maturity zero: intrinsic value, stderr 0 (mc_engine.rs:896-904)
+    _, g, keep = make_pair(option_type="put", strike=110.0, expiry=0.0)
+    r = price_raw([g], engine_cfg())[0]
+    assert r.price == 10.0 and r.stderr_ == 0.0 and r.num_paths == 0.0
+    # external mode without normals is an error, not a silent fallback
+    _, g, keep = make_pair()
+    from openferric_amd import RngMode
+    with pytest.raises(InvalidInput):
+        price_raw([g], engine_cfg(rng_mode=RngMode.External))
+
+
+def test_python_mirror_engine(gpu, oracle):
+    import openferric_amd as ofa
+    m = ofa.Market.builder().spot(100.0).rate(0.05).dividend_yield(0.0).flat_vol(0.2).build()
+    opt = ofa.VanillaOption.european_call(100.0, 1.0)
+    r = ofa.GpuMonteCarloPricingEngine(100_000, 252, 42).price(opt, m)
+    oo = oracle.make_option()
+    p_ref, se_ref = oracle.mc_run(oo, 100_000, 252, 42)
+    assert abs(r.price - p_ref) <= 1e-11 * p_ref and abs(r.stderr - se_ref) <= 1e-8 * se_ref
+    assert r.diagnostics.get("num_paths") == 100_000 and r.diagnostics.get("vol") == 0.2
+    a = ofa.AsianOption(ofa.OptionType.Call, 100.0, 1.0,
+                        ofa.AsianSpec(ofa.Averaging.Arithmetic, ofa.StrikeType.Fixed,
+                                      [k / 11.0 for k in range(12)]))
+    ra = ofa.GpuArithmeticAsianMC(100_000, 252, 42).price(a, m)
+    assert abs(ra.price - 5.73) / 5.73 <= 0.02
+    oa = oracle.make_option(payoff=oracle.ASIAN, obs_times=[k / 11.0 for k in range(12)])
+    pa, _ = oracle.mc_run(oa, 100_000, 252, 42, antithetic=True, control_variate=True)
+    assert abs(ra.price - pa) <= 1e-11 * pa
+    b = ofa.BarrierOption.builder().call().strike(100.0).expiry(1.0).up_and_out(130.0).build()
+    rb = ofa.GpuMonteCarloPricingEngine(20_000, 64, 1).price(b, m)
+    assert 0 < rb.price < r.price
+    h = ofa.Heston(0.03, 1.8, 0.04, 0.6, -0.5, 0.04)
+    p, d = ofa.heston_price_delta_aad(ofa.OptionType.Call, 70.0, 1.0, 100.0, 0.03, h,
+                                      ofa.HestonAadConfig(60_000, 64, 7))
+    ho = oracle.make_option(model=oracle.HESTON, strike=70.0, rate=0.03, kappa=1.8, theta=0.04,
+                            xi=0.6, rho=-0.5, v0=0.04)
+    p_ref, d_ref = oracle.heston_price_delta_aad(ho, 60_000, 64, 7)
+    assert abs(p - p_ref) <= 1e-11 * p_ref and abs(d - d_ref) <= 1e-10 * abs(d_ref)
+    g = ofa.mc_european_pathwise_aad(
+        ofa.GpuMonteCarloPricingEngine(40_000, 64, 42).with_variance_reduction(
+            ofa.VarianceReduction.Antithetic),
+        opt, ofa.Market(100.0, 0.03, 0.01, 0.2))
+    gr = oracle.mc_european_pathwise_aad(oracle.make_option(rate=0.03, div_yield=0.01),
+                                         40_000, 64, 42, True)
+    assert abs(g.greeks.delta - gr["delta"]) <= 1e-10 and abs(g.greeks.vega - gr["vega"]) <= 1e-9
+
+
+def test_cpp_mirror_program(gpu):
+    exe = os.path.join(ROOT, "tests", "cpp", "test_engine")
+    assert os.path.exists(exe), "tests/cpp/test_engine is not built"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failure(s)" in r.stdout
+
+
+def test_torch_autograd_gbm_and_heston(gpu):
+    torch = pytest.importorskip("torch")
+    from openferric_amd import RngMode, engine_cfg, price_raw
+    from openferric_amd.torch_ops import gbm_mc_price, heston_mc_price
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    t = lambda v: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=True)
+    spot, vol, rate, T = t([100.0, 95.0]), t([0.2, 0.3]), t([0.03, 0.05]), t([1.0, 0.5])
+    K = torch.tensor([100.0, 90.0], dtype=torch.float64, device=dev)
+    p = gbm_mc_price(spot, vol, rate, T, K, num_paths=40_000, num_steps=32, seed=3,
+                     rng_mode=RngMode.Reference)
+    (p * torch.tensor([1.0, 2.0], dtype=torch.float64, device=dev)).sum().backward()
+    for i, w in enumerate((1.0, 2.0)):
+        _, g, keep = make_pair(spot=[100.0, 95.0][i], vol=[0.2, 0.3][i], rate=[0.03, 0.05][i],
+                               expiry=[1.0, 0.5][i], strike=[100.0, 90.0][i], num_paths=40_000,
+                               num_steps=32, seed=3)
+        r = price_raw([g], engine_cfg(rng_mode=RngMode.Reference), greeks=True)[0]
+        assert abs(p[i].item() - r.price) <= 1e-12 * r.price
+        assert abs(spot.grad[i].item() - w * r.delta) <= 1e-12
+        assert abs(vol.grad[i].item() - w * r.vega) <= 1e-10
+        assert abs(rate.grad[i].item() - w * r.rho) <= 1e-10
+        assert abs(T.grad[i].item() + w * r.theta) <= 1e-10
+    hs = [t([x]) for x in (100.0, 0.04, 1.5, 0.05, 0.4, -0.6, 0.02)]
+    ph = heston_mc_price(*hs, torch.tensor([1.0], dtype=torch.float64, device=dev),
+                         torch.tensor([100.0], dtype=torch.float64, device=dev),
+                         num_paths=20_000, num_steps=40, seed=5)
+    ph.sum().backward()
+    assert all(x.grad is not None and torch.isfinite(x.grad).all() for x in hs)
+    assert hs[0].grad.item() > 0  # call delta
