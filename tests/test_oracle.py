@@ -431,3 +431,32 @@ def test_heston_oracle_mc_against_semi_analytic(oracle):
     p, se = oracle.mc_run(o, 400_000, 252, 9, nthreads=8)
     ref = heston_call(100, 100, 1.0, 0.02, 0.0, 2.0, 0.04, 0.3, -0.7, 0.04)
     assert abs(p - ref) <= 3 * se + 0.01
+
+
+def test_bs_greeks_quantlib_atm_and_finite_differences(oracle):
+    # greeks/bsm.rs:246-257: QuantLib ATM call Greeks
+    g = oracle.bs_greeks(oracle.CALL, 100.0, 100.0, 0.05, 0.0, 0.2, 1.0)
+    assert abs(g["delta"] - 0.6368) < 5e-4
+    assert abs(g["gamma"] - 0.01876) < 5e-5
+    assert abs(g["theta"] - -6.414) < 5e-3
+    assert abs(g["vega"] - 37.524) < 5e-3
+    assert abs(g["rho"] - 51.522) < 2.0
+    # closed-form Greeks are the derivatives of the closed-form price
+    # (central differences of the exact-CDF price, both option types, q != 0)
+    for t in (oracle.CALL, oracle.PUT):
+        args = dict(spot=95.0, strike=100.0, rate=0.03, q=0.01, vol=0.25, expiry=0.8)
+        g = oracle.bs_greeks(t, *args.values())
+
+        def px(**kw):
+            a = dict(args, **kw)
+            return oracle.bs_price(t, a["spot"], a["strike"], a["rate"], a["q"], a["vol"],
+                                   a["expiry"], exact=True)
+
+        h = 1e-4
+        fd = {"delta": (px(spot=95.0 + h) - px(spot=95.0 - h)) / (2 * h),
+              "gamma": (px(spot=95.0 + h) - 2 * px() + px(spot=95.0 - h)) / h ** 2,
+              "vega": (px(vol=0.25 + h) - px(vol=0.25 - h)) / (2 * h),
+              "rho": (px(rate=0.03 + h) - px(rate=0.03 - h)) / (2 * h),
+              "theta": -(px(expiry=0.8 + h) - px(expiry=0.8 - h)) / (2 * h)}
+        for k, v in fd.items():
+            assert abs(g[k] - v) <= 1e-5 * max(1.0, abs(v)), (t, k, g[k], v)
