@@ -53,14 +53,14 @@ using RngOf = typename std::conditional<RNG == OFG_RNG_FAST, Xoshiro256p, Xoshir
 template <int RNG, int MODEL, int K = adj_segment(MODEL)>
 __device__ __forceinline__ void seg_forward(RngOf<RNG> &rng, const ExternalNormals &ext, int j0,
                                             int len, double sign, double (&z1)[K],
-                                            double (&z2)[K]) {
+                                            double (&z2)[K], const double *tab) {
     if constexpr (RNG == OFG_RNG_FAST) {
         if constexpr (MODEL == OFG_MODEL_GBM) {
 #pragma unroll
             for (int g = 0; g < K / 4; ++g) {
                 if (4 * g < len) {
                     double a, b, c, d;
-                    normals4(rng, a, b, c, d);
+                    normals4(rng, a, b, c, d, tab);
                     z1[4 * g] = sign * a;
                     z1[4 * g + 1] = sign * b;
                     z1[4 * g + 2] = sign * c;
@@ -72,7 +72,7 @@ __device__ __forceinline__ void seg_forward(RngOf<RNG> &rng, const ExternalNorma
             for (int g = 0; g < K / 2; ++g) {
                 if (2 * g < len) {
                     double a, b, c, d;
-                    normals4(rng, a, b, c, d);
+                    normals4(rng, a, b, c, d, tab);
                     z1[2 * g] = sign * a;
                     z2[2 * g] = sign * b;
                     z1[2 * g + 1] = sign * c;
@@ -104,7 +104,7 @@ __device__ __forceinline__ void seg_forward(RngOf<RNG> &rng, const ExternalNorma
 template <int RNG, int MODEL, int K = adj_segment(MODEL)>
 __device__ __forceinline__ void seg_reverse(RngOf<RNG> &rng, const ExternalNormals &ext, int j0,
                                             int len, double sign, double (&z1)[K],
-                                            double (&z2)[K]) {
+                                            double (&z2)[K], const double *tab) {
     if constexpr (RNG == OFG_RNG_FAST) {
         constexpr int G = MODEL == OFG_MODEL_GBM ? K / 4 : K / 2;
         constexpr int SPG = MODEL == OFG_MODEL_GBM ? 4 : 2;  // steps per group
@@ -115,8 +115,8 @@ __device__ __forceinline__ void seg_reverse(RngOf<RNG> &rng, const ExternalNorma
                 const uint64_t wb = rng.prev_u64();
                 const uint64_t wa = rng.prev_u64();
                 double a, b, c, d;
-                box_muller_pair(wa, (uint32_t)(wc >> 32), a, b);
-                box_muller_pair(wb, (uint32_t)wc, c, d);
+                box_muller_pair(wa, (uint32_t)(wc >> 32), a, b, tab);
+                box_muller_pair(wb, (uint32_t)wc, c, d, tab);
                 if constexpr (MODEL == OFG_MODEL_GBM) {
                     z1[4 * g] = sign * a;
                     z1[4 * g + 1] = sign * b;
@@ -143,7 +143,7 @@ __device__ __forceinline__ void seg_reverse(RngOf<RNG> &rng, const ExternalNorma
             }
         }
     } else {
-        seg_forward<RNG, MODEL, K>(rng, ext, j0, len, sign, z1, z2);
+        seg_forward<RNG, MODEL, K>(rng, ext, j0, len, sign, z1, z2, tab);
     }
 }
 
@@ -196,7 +196,7 @@ __device__ __forceinline__ double leg_payoff_seeds(const Leg &g, const OptDev &o
 template <int MODEL, int PK, int RNG>
 __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
                               const double *__restrict__ w, const AdjArgs &a, double *ck,
-                              int lane, Grad &gr) {
+                              int lane, const double *tab, Grad &gr) {
     constexpr int K = adj_segment(MODEL);
     const int steps = o.steps;
     const int nseg = (steps + K - 1) / K;
@@ -225,7 +225,7 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
         if constexpr (NV == 2) row[WAVE] = v;
         const int j0 = c * K;
         const int len = min(K, steps - j0);
-        seg_forward<RNG, MODEL>(rng, ext, j0, len, sign, z1, z2);
+        seg_forward<RNG, MODEL>(rng, ext, j0, len, sign, z1, z2, tab);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (k < len) {
@@ -255,7 +255,7 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
         const double *row = ck + (int64_t)c * NV * WAVE + lane;
         const int j0 = c * K;
         const int len = min(K, steps - j0);
-        seg_reverse<RNG, MODEL>(rng, ext, j0, len, sign, z1, z2);
+        seg_reverse<RNG, MODEL>(rng, ext, j0, len, sign, z1, z2, tab);
         double S[K + 1], V[K + 1], E[K];
         S[0] = row[0];
         if constexpr (NV == 2) V[0] = row[WAVE];
@@ -337,7 +337,7 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
 
 template <int MODEL, int PK, int RNG, bool ANTI>
 __device__ void adjoint_item(const OptDev &o, const WorkItem &wi, int lane, const AdjArgs &a,
-                             double *ck, double &spv, double &spv2, Grad &acc) {
+                             double *ck, const double *tab, double &spv, double &spv2, Grad &acc) {
     const double *__restrict__ w = a.obs_w + o.obs_offset;
     for (int64_t base = wi.begin; base < wi.end; base += WAVE) {
         const int64_t sample = base + lane;
@@ -345,9 +345,9 @@ __device__ void adjoint_item(const OptDev &o, const WorkItem &wi, int lane, cons
             Grad g;
 #pragma unroll
             for (int k = 0; k < ADJ_NG; ++k) g.g[k] = 0.0;
-            double pv = adjoint_leg<MODEL, PK, RNG>(o, sample, 1.0, w, a, ck, lane, g);
+            double pv = adjoint_leg<MODEL, PK, RNG>(o, sample, 1.0, w, a, ck, lane, tab, g);
             if constexpr (ANTI) {
-                pv += adjoint_leg<MODEL, PK, RNG>(o, sample, -1.0, w, a, ck, lane, g);
+                pv += adjoint_leg<MODEL, PK, RNG>(o, sample, -1.0, w, a, ck, lane, tab, g);
                 pv *= 0.5;
 #pragma unroll
                 for (int k = 0; k < ADJ_NG; ++k) g.g[k] *= 0.5;
@@ -363,6 +363,11 @@ __device__ void adjoint_item(const OptDev &o, const WorkItem &wi, int lane, cons
 template <int RNG, int MODEL, bool ANTI, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ double tab[2 * OFG_LOG_TABLE_N];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < 2 * OFG_LOG_TABLE_N; i += BLOCK) tab[i] = OFG_LOG_TABLE[i];
+        __syncthreads();
+    }
     const int64_t slot = (int64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
     double *ck = a.scratch + slot * a.wave_stride;
     const uint32_t span = (uint32_t)(a.order_end - a.order_begin);
@@ -379,10 +384,10 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
 #pragma unroll
         for (int k = 0; k < ADJ_NG; ++k) acc.g[k] = 0.0;
         switch (o.kind) {
-        case PK_VANILLA: adjoint_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, ck, spv, spv2, acc); break;
-        case PK_ASIAN: adjoint_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, ck, spv, spv2, acc); break;
-        case PK_ASIAN_LOG: adjoint_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, ck, spv, spv2, acc); break;
-        default: adjoint_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, ck, spv, spv2, acc); break;
+        case PK_VANILLA: adjoint_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
+        case PK_ASIAN: adjoint_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
+        case PK_ASIAN_LOG: adjoint_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
+        default: adjoint_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
         }
         AdjPartial p;
         p.spv = wave_sum(spv);

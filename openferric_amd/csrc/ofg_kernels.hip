@@ -45,7 +45,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <int MODEL, int PK, int RNG, bool ANTI>
 __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
                                                 const double *__restrict__ w,
-                                                const LaunchArgs &a, double &x, double &y) {
+                                                const LaunchArgs &a, const double *tab, double &x,
+                                                double &y) {
     const double bsign = (o.flags & F_DOWN) ? -1.0 : 1.0;
     const double barrier = o.barrier;
     const int steps = o.steps;
@@ -80,7 +81,7 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
             int j = 0;
             for (; j + 3 < steps; j += 4) {
                 double z0, z1, z2, z3;
-                normals4(rng, z0, z1, z2, z3);
+                normals4(rng, z0, z1, z2, z3, tab);
                 advance(z0, j);
                 advance(z1, j + 1);
                 advance(z2, j + 2);
@@ -88,7 +89,7 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
             }
             if (j < steps) {
                 double z0, z1, z2, z3;
-                normals4(rng, z0, z1, z2, z3);
+                normals4(rng, z0, z1, z2, z3, tab);
                 advance(z0, j);
                 if (j + 1 < steps) advance(z1, j + 1);
                 if (j + 2 < steps) advance(z2, j + 2);
@@ -114,13 +115,13 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
             int j = 0;
             for (; j + 1 < steps; j += 2) {
                 double z0, z1, z2, z3;
-                normals4(rng, z0, z1, z2, z3);
+                normals4(rng, z0, z1, z2, z3, tab);
                 advance(z0, z1, j);
                 advance(z2, z3, j + 1);
             }
             if (j < steps) {
                 double z0, z1, z2, z3;
-                normals4(rng, z0, z1, z2, z3);
+                normals4(rng, z0, z1, z2, z3, tab);
                 advance(z0, z1, j);
             }
         } else {
@@ -153,13 +154,13 @@ struct Acc {
 
 template <int MODEL, int PK, int RNG, bool ANTI>
 __device__ void run_item(const OptDev &o, const WorkItem &wi, int lane, const LaunchArgs &a,
-                         Acc &acc) {
+                         const double *tab, Acc &acc) {
     const double *__restrict__ w = a.obs_w + o.obs_offset;
     for (int64_t base = wi.begin; base < wi.end; base += WAVE) {
         const int64_t sample = base + lane;
         if (sample < wi.end) {
             double x, y;
-            simulate_sample<MODEL, PK, RNG, ANTI>(o, sample, w, a, x, y);
+            simulate_sample<MODEL, PK, RNG, ANTI>(o, sample, w, a, tab, x, y);
             acc.sx += x;
             acc.sx2 += x * x;
             acc.sy += y;
@@ -173,18 +174,24 @@ __device__ void run_item(const OptDev &o, const WorkItem &wi, int lane, const La
 
 template <int MODEL, int RNG, bool ANTI>
 __device__ __forceinline__ void dispatch_kind(const OptDev &o, const WorkItem &wi, int lane,
-                                              const LaunchArgs &a, Acc &acc) {
+                                              const LaunchArgs &a, const double *tab, Acc &acc) {
     switch (o.kind) {
-    case PK_VANILLA: run_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, acc); break;
-    case PK_ASIAN: run_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, acc); break;
-    case PK_ASIAN_LOG: run_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, acc); break;
-    default: run_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, acc); break;
+    case PK_VANILLA: run_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
+    case PK_ASIAN: run_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
+    case PK_ASIAN_LOG: run_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
+    default: run_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
     }
 }
 
 template <int RNG, int MODEL, bool ANTI, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void price_kernel(LaunchArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
+    // log table of the fast-mode Box-Muller, staged once per block in LDS
+    __shared__ double tab[2 * OFG_LOG_TABLE_N];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < 2 * OFG_LOG_TABLE_N; i += BLOCK) tab[i] = OFG_LOG_TABLE[i];
+        __syncthreads();
+    }
     const uint32_t span = (uint32_t)(a.order_end - a.order_begin);
     for (;;) {
         // Every lane executes the atomic (lane 0 adds 1, the others 0) so the
@@ -198,7 +205,7 @@ __global__ __launch_bounds__(BLOCK) void price_kernel(LaunchArgs a) {
         const WorkItem wi = a.items[it];
         const OptDev o = a.opts[wi.option];
         Acc acc{0.0, 0.0, 0.0, 0.0, 0.0};
-        dispatch_kind<MODEL, RNG, ANTI>(o, wi, lane, a, acc);
+        dispatch_kind<MODEL, RNG, ANTI>(o, wi, lane, a, tab, acc);
         const double sx = wave_sum(acc.sx);
         const double sx2 = wave_sum(acc.sx2);
         const double sy = wave_sum(acc.sy);

@@ -264,37 +264,169 @@ OFG_HD double rsq_seed(double d) {
 #endif
 }
 
-// Natural log of a positive normal double, branch-free.
-// x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1),
-// evaluated as 2s + 2s z Q(z), z = s^2 <= 0.0295, Q a degree-6 Chebyshev-node
-// interpolant of (atanh(sqrt z)/sqrt z - 1)/z (max relative error of the
-// reconstructed log 4.6e-18 in exact arithmetic). The division is a
-// reciprocal seed with one Newton step and a final residual correction.
-OFG_HD double log_fp64(double x) {
+// Log table for log_fp64: for m in [0.5, 1), 128 intervals of width 1/256
+// with midpoints c_i = 0.5 + (i + 1/2)/256; entry i holds T1 = 1/c_i rounded to
+// double and T2 = -log(T1) (computed to 50 digits, then rounded), so that
+// log m = T2 + log1p(m T1 - 1) with |m T1 - 1| <= 0.00391 (values generated
+// with mpmath for this project). The last interval uses c = 1 (T1 = 1,
+// T2 = 0): r = m - 1 is then exact and log x keeps full relative accuracy as
+// x -> 1 from below.
+#define OFG_LOG_TABLE_N 128
+static constexpr double OFG_LOG_TABLE[2 * OFG_LOG_TABLE_N] = {
+    1.9922178988326849, -0.689248540144288,
+    1.9768339768339769, -0.6814965633399701,
+    1.9616858237547892, -0.6738042177168143,
+    1.946768060836502, -0.6661705928617432,
+    1.9320754716981132, -0.6585947990532856,
+    1.9176029962546817, -0.6510759666392583,
+    1.903345724907063, -0.6436132454376686,
+    1.8892988929889298, -0.6362058041598069,
+    1.8754578754578755, -0.628852829854548,
+    1.8618181818181818, -0.6215535273729365,
+    1.848375451263538, -0.6143071188521694,
+    1.8351254480286738, -0.6071128432181422,
+    1.8220640569395017, -0.599969955705762,
+    1.8091872791519434, -0.5928777273962701,
+    1.7964912280701755, -0.5858354447708572,
+    1.78397212543554, -0.5788424092798867,
+    1.7716262975778547, -0.5718979369270757,
+    1.7594501718213058, -0.5650013578680152,
+    1.7474402730375427, -0.5581520160224405,
+    1.735593220338983, -0.5513492686996879,
+    1.7239057239057238, -0.5445924862368081,
+    1.7123745819397993, -0.5378810516488214,
+    1.7009966777408638, -0.5312143602906321,
+    1.6897689768976898, -0.5245918195301387,
+    1.678688524590164, -0.5180128484320962,
+    1.6677524429967427, -0.5114768774523106,
+    1.656957928802589, -0.5049833481417624,
+    1.6463022508038585, -0.49853171286027365,
+    1.6357827476038338, -0.4921214344993545,
+    1.6253968253968254, -0.4857519862138747,
+    1.615141955835962, -0.47942285116222716,
+    1.6050156739811912, -0.47313352225466315,
+    1.5950155763239875, -0.4668835019094919,
+    1.585139318885449, -0.4606723018168513,
+    1.5753846153846154, -0.4544994427097703,
+    1.5657492354740061, -0.4483644541422544,
+    1.5562310030395137, -0.4422668742741359,
+    1.5468277945619335, -0.43620624966244487,
+    1.5375375375375375, -0.43018213505906394,
+    1.528358208955224, -0.42419409321444135,
+    1.5192878338278932, -0.4182416946871461,
+    1.5103244837758112, -0.41232451765905753,
+    1.501466275659824, -0.40644214775599097,
+    1.4927113702623906, -0.4005941778735679,
+    1.4840579710144928, -0.39478020800814806,
+    1.4755043227665705, -0.3889998450926485,
+    1.4670487106017192, -0.38325270283708057,
+    1.4586894586894588, -0.37753840157364205,
+    1.4504249291784703, -0.37185656810621104,
+    1.4422535211267606, -0.36620683556409206,
+    1.4341736694677871, -0.36058884325986873,
+    1.426183844011142, -0.3550022365512289,
+    1.4182825484764543, -0.3494466667066269,
+    1.4104683195592287, -0.343921790774657,
+    1.4027397260273973, -0.3384272714570163,
+    1.3950953678474114, -0.3329627769849375,
+    1.3875338753387534, -0.3275279809989806,
+    1.3800539083557952, -0.3221225624320727,
+    1.3726541554959786, -0.31674620539569226,
+    1.3653333333333333, -0.31139859906909695,
+    1.3580901856763925, -0.306079437591497,
+    1.3509234828496042, -0.3007884199570814,
+    1.3438320209973753, -0.2955252499128068,
+    1.3368146214099217, -0.2902896358588618,
+    1.3298701298701299, -0.28508129075172356,
+    1.322997416020672, -0.279899932009726,
+    1.3161953727506426, -0.2747452814210614,
+    1.3094629156010231, -0.26961706505414207,
+    1.3027989821882953, -0.2645150131702466,
+    1.2962025316455696, -0.2594388601383859,
+    1.2896725440806045, -0.25438834435231733,
+    1.2832080200501252, -0.24936320814964427,
+    1.2768079800498753, -0.24436319773293858,
+    1.2704714640198511, -0.23938806309282482,
+    1.2641975308641975, -0.23443755793296864,
+    1.257985257985258, -0.22951143959691278,
+    1.2518337408312958, -0.22460946899670603,
+    1.245742092457421, -0.2197314105432732,
+    1.2397094430992737, -0.21487703207847508,
+    1.2337349397590363, -0.21004610480880959,
+    1.2278177458033572, -0.20523840324070627,
+    1.2219570405727924, -0.2004537051173701,
+    1.2161520190023754, -0.19569179135712642,
+    1.210401891252955, -0.1909524459932298,
+    1.204705882352941, -0.18623545611509087,
+    1.199063231850117, -0.18154061181088324,
+    1.1934731934731935, -0.1768677061114908,
+    1.1879350348027842, -0.17221653493575995,
+    1.1824480369515011, -0.16758689703701793,
+    1.1770114942528735, -0.16297859395082367,
+    1.17162471395881, -0.15839142994391764,
+    1.1662870159453302, -0.15382521196433638,
+    1.1609977324263039, -0.14927974959266183,
+    1.1557562076749435, -0.14475485499437207,
+    1.150561797752809, -0.14025034287326765,
+    1.145413870246085, -0.13576603042593893,
+    1.1403118040089086, -0.13130173729725345,
+    1.1352549889135255, -0.12685728553682943,
+    1.130242825607064, -0.12243249955647377,
+    1.1252747252747253, -0.11802720608855737,
+    1.1203501094091903, -0.11364123414530306,
+    1.1154684095860568, -0.10927441497896273,
+    1.1106290672451193, -0.10492658204285929,
+    1.1058315334773219, -0.10059757095327378,
+    1.1010752688172043, -0.09628721945215148,
+    1.0963597430406853, -0.09199536737061052,
+    1.091684434968017, -0.0877218565932284,
+    1.0870488322717622, -0.08346653102309001,
+    1.0824524312896406, -0.07922923654757486,
+    1.0778947368421052, -0.07500982100486656,
+    1.0733752620545074, -0.07080813415116662,
+    1.068893528183716, -0.06662402762859244,
+    1.0644490644490645, -0.06245735493374666,
+    1.060041407867495, -0.05830797138693517,
+    1.0556701030927835, -0.054175734102024614,
+    1.051334702258727, -0.05006050195691803,
+    1.047034764826176, -0.04596213556463585,
+    1.0427698574338085, -0.04188049724498711,
+    1.0385395537525355, -0.037815450996817664,
+    1.0343434343434343, -0.033766862470817484,
+    1.0301810865191148, -0.029734598942879144,
+    1.0260521042084167, -0.025718529287989036,
+    1.0219560878243512, -0.021718523954642903,
+    1.0178926441351888, -0.017734454939768475,
+    1.0138613861386139, -0.01376619576414797,
+    1.009861932938856, -0.00981362144832467,
+    1.005893909626719, -0.005876608488984971,
+    1.0, 0.0,
+};
+
+// Natural log of x in (0, 1] (the Box-Muller argument), branch-free:
+// x = m 2^e (frexp, m in [0.5, 1)), log x = e ln2 + T2[i] + log1p(r),
+// r = fma(m, T1[i], -1), log1p by its Taylor series through r^7 (truncation
+// 1.7e-18 relative). `tab` points at OFG_LOG_TABLE, or at a copy in LDS.
+OFG_HD double log_fp64(double x, const double *tab) {
     int e;
-    double m = frexp(x, &e);
-    const bool lo = m < 0.70710678118654752;
-    m = lo ? m + m : m;
-    const double ed = (double)(lo ? e - 1 : e);
-    const double f = m - 1.0;
-    const double d = f + 2.0;
-    double r = rcp_seed(d);
-    r = fma(r, fma(-d, r, 1.0), r);
-    double s = f * r;
-    s = fma(fma(-d, s, f), r, s);
-    const double z = s * s;
-    double q = 0.07308224842521703;
-    q = fma(q, z, 0.07665860800278021);
-    q = fma(q, z, 0.09091444562630861);
-    q = fma(q, z, 0.1111110556739754);
-    q = fma(q, z, 0.14285714312987743);
-    q = fma(q, z, 0.19999999999949752);
-    q = fma(q, z, 0.3333333333333335);
-    const double s2 = s + s;
-    const double lm = fma(s2 * z, q, s2);
+    const double m = frexp(x, &e);
+    uint64_t mb;
+    __builtin_memcpy(&mb, &m, sizeof mb);
+    const int i = (int)((mb >> 45) & 127u);  // top 7 fraction bits of m in [0.5, 1)
+    const double t1 = tab[2 * i];
+    const double t2 = tab[2 * i + 1];
+    const double r = fma(m, t1, -1.0);
+    double p = 1.0 / 7.0;
+    p = fma(p, r, -1.0 / 6.0);
+    p = fma(p, r, 1.0 / 5.0);
+    p = fma(p, r, -0.25);
+    p = fma(p, r, 1.0 / 3.0);
+    p = fma(p, r, -0.5);
+    const double l1p = fma(r * r, p, r);
+    const double ed = (double)e;
     const double LN2_HI = 6.93147180369123816490e-01;
     const double LN2_LO = 1.90821492927058770002e-10;
-    return fma(ed, LN2_HI, fma(ed, LN2_LO, lm));
+    return fma(ed, LN2_HI, t2 + fma(ed, LN2_LO, l1p));
 }
 
 // sqrt of a positive finite double from the reciprocal square-root seed and
@@ -338,10 +470,10 @@ OFG_HD void sincos_octant(double t, double &sn, double &cs) {
 //   an angle uniform on [-pi/4, pi/4) from wt, mapped to the whole circle by
 //   one of the 8 symmetries of the square (swap, two sign flips) chosen by
 //   bits 8..10 of wu: the image is uniform on the circle.
-OFG_HD void box_muller_pair(uint64_t wu, uint32_t wt, double &z0, double &z1) {
+OFG_HD void box_muller_pair(uint64_t wu, uint32_t wt, double &z0, double &z1, const double *tab) {
     const double one_k = bits_as_double(0x3FF0000000000000ull | (wu >> 12));  // 1 + k 2^-52
     const double u = one_k - 0.99999999999999988898;                        // exact
-    const double rad = sqrt_pos(-2.0 * log_fp64(u));
+    const double rad = sqrt_pos(-2.0 * log_fp64(u, tab));
     const double t = fma((double)wt, 3.657295198167899e-10, -0.7853981633974483);  // pi / 2^33, pi / 4
     double sn, cs;
     sincos_octant(t, sn, cs);
@@ -357,12 +489,12 @@ OFG_HD void box_muller_pair(uint64_t wu, uint32_t wt, double &z0, double &z1) {
 // Four standard normals from three 64-bit draws (two Box-Muller pairs; the
 // third draw supplies both angles).
 template <class Rng>
-OFG_HD void normals4(Rng &rng, double &z0, double &z1, double &z2, double &z3) {
+OFG_HD void normals4(Rng &rng, double &z0, double &z1, double &z2, double &z3, const double *tab) {
     const uint64_t wa = rng.next_u64();
     const uint64_t wb = rng.next_u64();
     const uint64_t wc = rng.next_u64();
-    box_muller_pair(wa, (uint32_t)(wc >> 32), z0, z1);
-    box_muller_pair(wb, (uint32_t)wc, z2, z3);
+    box_muller_pair(wa, (uint32_t)(wc >> 32), z0, z1, tab);
+    box_muller_pair(wb, (uint32_t)wc, z2, z3, tab);
 }
 
 }  // namespace ofg

@@ -14,17 +14,17 @@ void shim_reference_normals(uint64_t seed, uint64_t sample, int64_t n, double *o
 }
 
 extern "C" {
-double shim_log(double x) { return ofg::log_fp64(x); }
+double shim_log(double x) { return ofg::log_fp64(x, ofg::OFG_LOG_TABLE); }
 double shim_sqrt(double x) { return ofg::sqrt_pos(x); }
 void shim_sincos(double t, double *s, double *c) { ofg::sincos_octant(t, *s, *c); }
 void shim_fast_normals(uint64_t seed, uint64_t sample, int64_t n4, double *out) {
     ofg::Xoshiro256p r;
     r.seed(ofg::stream_seed(seed, sample));
     for (int64_t k = 0; k < n4; ++k)
-        ofg::normals4(r, out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
+        ofg::normals4(r, out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3], ofg::OFG_LOG_TABLE);
 }
 void shim_box_muller(uint64_t wu, uint32_t wt, double *z0, double *z1) {
-    ofg::box_muller_pair(wu, wt, *z0, *z1);
+    ofg::box_muller_pair(wu, wt, *z0, *z1, ofg::OFG_LOG_TABLE);
 }
 }
 

@@ -78,13 +78,13 @@ def test_reference_draws_match_oracle_bitwise(shim, oracle):
 def test_log_sqrt_sincos_accuracy(shim):
     mp.mp.dps = 30
     rng = np.random.default_rng(11)
-    xs = np.concatenate([rng.uniform(0, 1, 3000), np.geomspace(1.1e-16, 1.0, 3000),
-                         1 - np.geomspace(1.1e-16, 0.5, 1000), rng.uniform(1, 1e6, 500)])
+    # domain (0, 1): the Box-Muller argument
+    xs = np.concatenate([rng.uniform(0, 1, 3000), np.geomspace(2.2e-308, 0.999, 3000),
+                         1 - np.geomspace(1.1e-16, 0.5, 1000)])
     worst = 0.0
     for x in xs:
         x = float(x)
-        if x == 1.0:
-            assert shim.shim_log(x) == 0.0
+        if x >= 1.0:
             continue
         ref = mp.log(mp.mpf(x))
         worst = max(worst, float(abs((mp.mpf(shim.shim_log(x)) - ref) / ref)))
