@@ -73,6 +73,25 @@ def build_cpp_tests(verbose: bool = False) -> str:
     return exe
 
 
+def build_math_shim(verbose: bool = False) -> str:
+    """Builds tests/cpu_shim/libofg_math_shim.so: a host-only build of the
+    device math header, used by the tests to check the kernels' own math on
+    the CPU and to recompute fast-mode paths for the GPU parity tests."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpu_shim", "math_shim.hip")
+    out = os.path.join(root, "tests", "cpu_shim", "libofg_math_shim.so")
+    cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-ffp-contract=off", "-fPIC",
+           "-shared", "-o", out + ".tmp", src]
+    if verbose:
+        print("hipcc (host only) math_shim.hip", flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("host build of ofg_math.hpp failed")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def dump_asm(src: str = "ofg_kernels.hip", out: str | None = None) -> str:
     """Writes the gfx950 assembly of one source (for instruction counts)."""
     out = out or os.path.join(HERE, "_build", src + ".s")
@@ -85,3 +104,4 @@ def dump_asm(src: str = "ofg_kernels.hip", out: str | None = None) -> str:
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_cpp_tests(verbose=True))
+    print(build_math_shim(verbose=True))

@@ -229,8 +229,8 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (k < len) {
-                if constexpr (MODEL == OFG_MODEL_GBM) gbm_step<PK>(g, o, z1[k]);
-                else heston_step<PK, RNG != OFG_RNG_FAST>(g, v, o, z1[k], z2[k]);
+                if constexpr (MODEL == OFG_MODEL_GBM) gbm_step<PK, RNG == OFG_RNG_FAST>(g, o, z1[k], tab);
+                else heston_step<PK, RNG != OFG_RNG_FAST>(g, v, o, z1[k], z2[k], tab);
                 leg_observe<PK>(g, ASIAN ? w[j0 + k + 1] : 0.0, bsign, o.barrier);
             }
         }
@@ -263,11 +263,12 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
         for (int k = 0; k < K; ++k) {
             if (k < len) {
                 if constexpr (MODEL == OFG_MODEL_GBM) {
-                    E[k] = exp_fp64(fma(o.diffusion, z1[k], o.drift));
+                    double x_unused;
+                    E[k] = gbm_growth<RNG == OFG_RNG_FAST>(o, z1[k], tab, x_unused);
                     S[k + 1] = S[k] * E[k];
                 } else {
                     double e_unused;
-                    heston_euler<RNG != OFG_RNG_FAST>(S[k], V[k], o, z1[k], z2[k], S[k + 1], V[k + 1], E[k], e_unused);
+                    heston_euler<RNG != OFG_RNG_FAST>(S[k], V[k], o, z1[k], z2[k], S[k + 1], V[k + 1], E[k], e_unused, tab);
                 }
             }
         }
@@ -363,9 +364,9 @@ __device__ void adjoint_item(const OptDev &o, const WorkItem &wi, int lane, cons
 template <int RNG, int MODEL, bool ANTI, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
-    __shared__ double tab[2 * OFG_LOG_TABLE_N];
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
-        for (int i = threadIdx.x; i < 2 * OFG_LOG_TABLE_N; i += BLOCK) tab[i] = OFG_LOG_TABLE[i];
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += BLOCK) tab[i] = OFG_FAST_TABLE[i];
         __syncthreads();
     }
     const int64_t slot = (int64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;

@@ -9,6 +9,7 @@
 // and the autotuner. No exception crosses the C ABI.
 #include "../../include/openferric_amd.h"
 #include "ofg_internal.hpp"
+#include "ofg_math.hpp"
 #include "ofg_adjoint.hpp"
 
 #include <hip/hip_runtime.h>
@@ -349,6 +350,17 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
         d.xsdt = o.xi * d.sqrt_dt;
         d.mudt = d.mu * d.dt;
         d.hdt = 0.5 * d.dt;
+        d.drift2 = d.drift * OFG_INV_LN2_256;
+        d.diffusion2 = d.diffusion * OFG_INV_LN2_256;
+        d.sqrt_dt2 = d.sqrt_dt * OFG_INV_LN2_256;
+        d.mudt2 = d.mudt * OFG_INV_LN2_256;
+        d.hdt2 = d.hdt * OFG_INV_LN2_256;
+        // fast GBM steps evaluate 2^(y/256), y = diffusion2 z + drift2 with
+        // |z| <= sqrt(106 ln 2) < 8.6; keep |y| inside its domain
+        if (cfg.rng_mode == OFG_RNG_FAST && o.model == OFG_MODEL_GBM &&
+            !(std::fabs(d.drift2) + 8.6 * d.diffusion2 < OFG_EXP2_256_YMAX))
+            fail(OFG_INVALID_INPUT,
+                 "fast rng mode needs per-step log-returns below 2.9e6 in magnitude");
         d.antithetic = cfg.antithetic ? 1 : 0;
         int32_t flags = 0;
         if (o.option_type == OFG_PUT) flags |= ofg::F_PUT;

@@ -46,6 +46,8 @@ struct alignas(16) OptDev {
     double df, cv_expected, rate;
     // Heston step constants of the fast mode's re-associated Euler step
     double kdt, kthdt, xsdt, mudt, hdt;
+    // fast mode: exponent constants pre-scaled by 256/ln2 for exp2_256
+    double drift2, diffusion2, sqrt_dt2, mudt2, hdt2;
     int32_t antithetic;
     int32_t has_cv;
 };

@@ -71,8 +71,8 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
     if constexpr (MODEL == OFG_MODEL_GBM) {
         // one GBM step on both legs, then the observation of path index j+1
         auto advance = [&](double z, int j) {
-            gbm_step<PK>(ga, o, z);
-            if constexpr (ANTI) gbm_step<PK>(gb, o, -z);
+            gbm_step<PK, RNG == OFG_RNG_FAST>(ga, o, z, tab);
+            if constexpr (ANTI) gbm_step<PK, RNG == OFG_RNG_FAST>(gb, o, -z, tab);
             const double wj = ASIAN ? w[j + 1] : 0.0;
             leg_observe<PK>(ga, wj, bsign, barrier);
             if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
@@ -105,8 +105,8 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
     } else {
         double va = o.v0, vb = o.v0;
         auto advance = [&](double z1, double z2, int j) {
-            heston_step<PK, RNG != OFG_RNG_FAST>(ga, va, o, z1, z2);
-            if constexpr (ANTI) heston_step<PK, RNG != OFG_RNG_FAST>(gb, vb, o, -z1, -z2);
+            heston_step<PK, RNG != OFG_RNG_FAST>(ga, va, o, z1, z2, tab);
+            if constexpr (ANTI) heston_step<PK, RNG != OFG_RNG_FAST>(gb, vb, o, -z1, -z2, tab);
             const double wj = ASIAN ? w[j + 1] : 0.0;
             leg_observe<PK>(ga, wj, bsign, barrier);
             if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
@@ -187,9 +187,9 @@ template <int RNG, int MODEL, bool ANTI, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void price_kernel(LaunchArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     // log table of the fast-mode Box-Muller, staged once per block in LDS
-    __shared__ double tab[2 * OFG_LOG_TABLE_N];
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
-        for (int i = threadIdx.x; i < 2 * OFG_LOG_TABLE_N; i += BLOCK) tab[i] = OFG_LOG_TABLE[i];
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += BLOCK) tab[i] = OFG_FAST_TABLE[i];
         __syncthreads();
     }
     const uint32_t span = (uint32_t)(a.order_end - a.order_begin);

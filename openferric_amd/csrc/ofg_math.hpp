@@ -15,7 +15,8 @@
 //     polynomial (Chebyshev-node interpolant of e^r, max relative error
 //     1.7e-17 in exact arithmetic), scaled with ldexp.
 //   * fast RNG mode: xoshiro256+ streams and a branch-free Box-Muller
-//     transform with its own log, sqrt and sin/cos on [-pi/4, pi/4].
+//     transform with table-driven -2 log, sin/cos of a full turn and a
+//     Newton sqrt; a table-driven 2^(y/256) for the path steps.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -264,226 +265,82 @@ OFG_HD double rsq_seed(double d) {
 #endif
 }
 
-// Log table for log_fp64: for m in [0.5, 1), 128 intervals of width 1/256
-// with midpoints c_i = 0.5 + (i + 1/2)/256; entry i holds T1 = 1/c_i rounded to
-// double and T2 = -log(T1) (computed to 50 digits, then rounded), so that
-// log m = T2 + log1p(m T1 - 1) with |m T1 - 1| <= 0.00391 (values generated
-// with mpmath for this project). The last interval uses c = 1 (T1 = 1,
-// T2 = 0): r = m - 1 is then exact and log x keeps full relative accuracy as
-// x -> 1 from below.
-#define OFG_LOG_TABLE_N 128
-static constexpr double OFG_LOG_TABLE[2 * OFG_LOG_TABLE_N] = {
-    1.9922178988326849, -0.689248540144288,
-    1.9768339768339769, -0.6814965633399701,
-    1.9616858237547892, -0.6738042177168143,
-    1.946768060836502, -0.6661705928617432,
-    1.9320754716981132, -0.6585947990532856,
-    1.9176029962546817, -0.6510759666392583,
-    1.903345724907063, -0.6436132454376686,
-    1.8892988929889298, -0.6362058041598069,
-    1.8754578754578755, -0.628852829854548,
-    1.8618181818181818, -0.6215535273729365,
-    1.848375451263538, -0.6143071188521694,
-    1.8351254480286738, -0.6071128432181422,
-    1.8220640569395017, -0.599969955705762,
-    1.8091872791519434, -0.5928777273962701,
-    1.7964912280701755, -0.5858354447708572,
-    1.78397212543554, -0.5788424092798867,
-    1.7716262975778547, -0.5718979369270757,
-    1.7594501718213058, -0.5650013578680152,
-    1.7474402730375427, -0.5581520160224405,
-    1.735593220338983, -0.5513492686996879,
-    1.7239057239057238, -0.5445924862368081,
-    1.7123745819397993, -0.5378810516488214,
-    1.7009966777408638, -0.5312143602906321,
-    1.6897689768976898, -0.5245918195301387,
-    1.678688524590164, -0.5180128484320962,
-    1.6677524429967427, -0.5114768774523106,
-    1.656957928802589, -0.5049833481417624,
-    1.6463022508038585, -0.49853171286027365,
-    1.6357827476038338, -0.4921214344993545,
-    1.6253968253968254, -0.4857519862138747,
-    1.615141955835962, -0.47942285116222716,
-    1.6050156739811912, -0.47313352225466315,
-    1.5950155763239875, -0.4668835019094919,
-    1.585139318885449, -0.4606723018168513,
-    1.5753846153846154, -0.4544994427097703,
-    1.5657492354740061, -0.4483644541422544,
-    1.5562310030395137, -0.4422668742741359,
-    1.5468277945619335, -0.43620624966244487,
-    1.5375375375375375, -0.43018213505906394,
-    1.528358208955224, -0.42419409321444135,
-    1.5192878338278932, -0.4182416946871461,
-    1.5103244837758112, -0.41232451765905753,
-    1.501466275659824, -0.40644214775599097,
-    1.4927113702623906, -0.4005941778735679,
-    1.4840579710144928, -0.39478020800814806,
-    1.4755043227665705, -0.3889998450926485,
-    1.4670487106017192, -0.38325270283708057,
-    1.4586894586894588, -0.37753840157364205,
-    1.4504249291784703, -0.37185656810621104,
-    1.4422535211267606, -0.36620683556409206,
-    1.4341736694677871, -0.36058884325986873,
-    1.426183844011142, -0.3550022365512289,
-    1.4182825484764543, -0.3494466667066269,
-    1.4104683195592287, -0.343921790774657,
-    1.4027397260273973, -0.3384272714570163,
-    1.3950953678474114, -0.3329627769849375,
-    1.3875338753387534, -0.3275279809989806,
-    1.3800539083557952, -0.3221225624320727,
-    1.3726541554959786, -0.31674620539569226,
-    1.3653333333333333, -0.31139859906909695,
-    1.3580901856763925, -0.306079437591497,
-    1.3509234828496042, -0.3007884199570814,
-    1.3438320209973753, -0.2955252499128068,
-    1.3368146214099217, -0.2902896358588618,
-    1.3298701298701299, -0.28508129075172356,
-    1.322997416020672, -0.279899932009726,
-    1.3161953727506426, -0.2747452814210614,
-    1.3094629156010231, -0.26961706505414207,
-    1.3027989821882953, -0.2645150131702466,
-    1.2962025316455696, -0.2594388601383859,
-    1.2896725440806045, -0.25438834435231733,
-    1.2832080200501252, -0.24936320814964427,
-    1.2768079800498753, -0.24436319773293858,
-    1.2704714640198511, -0.23938806309282482,
-    1.2641975308641975, -0.23443755793296864,
-    1.257985257985258, -0.22951143959691278,
-    1.2518337408312958, -0.22460946899670603,
-    1.245742092457421, -0.2197314105432732,
-    1.2397094430992737, -0.21487703207847508,
-    1.2337349397590363, -0.21004610480880959,
-    1.2278177458033572, -0.20523840324070627,
-    1.2219570405727924, -0.2004537051173701,
-    1.2161520190023754, -0.19569179135712642,
-    1.210401891252955, -0.1909524459932298,
-    1.204705882352941, -0.18623545611509087,
-    1.199063231850117, -0.18154061181088324,
-    1.1934731934731935, -0.1768677061114908,
-    1.1879350348027842, -0.17221653493575995,
-    1.1824480369515011, -0.16758689703701793,
-    1.1770114942528735, -0.16297859395082367,
-    1.17162471395881, -0.15839142994391764,
-    1.1662870159453302, -0.15382521196433638,
-    1.1609977324263039, -0.14927974959266183,
-    1.1557562076749435, -0.14475485499437207,
-    1.150561797752809, -0.14025034287326765,
-    1.145413870246085, -0.13576603042593893,
-    1.1403118040089086, -0.13130173729725345,
-    1.1352549889135255, -0.12685728553682943,
-    1.130242825607064, -0.12243249955647377,
-    1.1252747252747253, -0.11802720608855737,
-    1.1203501094091903, -0.11364123414530306,
-    1.1154684095860568, -0.10927441497896273,
-    1.1106290672451193, -0.10492658204285929,
-    1.1058315334773219, -0.10059757095327378,
-    1.1010752688172043, -0.09628721945215148,
-    1.0963597430406853, -0.09199536737061052,
-    1.091684434968017, -0.0877218565932284,
-    1.0870488322717622, -0.08346653102309001,
-    1.0824524312896406, -0.07922923654757486,
-    1.0778947368421052, -0.07500982100486656,
-    1.0733752620545074, -0.07080813415116662,
-    1.068893528183716, -0.06662402762859244,
-    1.0644490644490645, -0.06245735493374666,
-    1.060041407867495, -0.05830797138693517,
-    1.0556701030927835, -0.054175734102024614,
-    1.051334702258727, -0.05006050195691803,
-    1.047034764826176, -0.04596213556463585,
-    1.0427698574338085, -0.04188049724498711,
-    1.0385395537525355, -0.037815450996817664,
-    1.0343434343434343, -0.033766862470817484,
-    1.0301810865191148, -0.029734598942879144,
-    1.0260521042084167, -0.025718529287989036,
-    1.0219560878243512, -0.021718523954642903,
-    1.0178926441351888, -0.017734454939768475,
-    1.0138613861386139, -0.01376619576414797,
-    1.009861932938856, -0.00981362144832467,
-    1.005893909626719, -0.005876608488984971,
-    1.0, 0.0,
-};
+// Constant tables of the fast mode (log, exp2, sin/cos), generated by
+// gen_fast_tables.py. The kernels stage a copy in LDS; `tab` below points at
+// OFG_FAST_TABLE itself or at that copy.
+#include "ofg_fast_tables.inc"
+#define OFG_TAB_LOG 0
+#define OFG_TAB_EXP2 256
+#define OFG_TAB_SINCOS 512
 
-// Natural log of x in (0, 1] (the Box-Muller argument), branch-free:
-// x = m 2^e (frexp, m in [0.5, 1)), log x = e ln2 + T2[i] + log1p(r),
-// r = fma(m, T1[i], -1), log1p by its Taylor series through r^7 (truncation
-// 1.7e-18 relative). `tab` points at OFG_LOG_TABLE, or at a copy in LDS.
-OFG_HD double log_fp64(double x, const double *tab) {
+// -2 log x for x in (0, 1] (the Box-Muller radius argument), branch-free:
+// x = m 2^e (frexp, m in [0.5, 1)), interval i of m from its top 7 fraction
+// bits, r = m T1[i] - 1 (|r| <= 0.0039), and
+//   -2 log x = -2 e ln2 + 2 log T1[i] + r q(r),
+// q(r) = -2 log1p(r) / r by its Taylor series through r^6 (truncation
+// 1.4e-20 absolute). ln2 is taken as the double nearest to it: the error,
+// |e| 4.6e-17, stays below 3.4e-17 of the result because |result| >= 1.38 |e|.
+OFG_HD double neg2log_fp64(double x, const double *tab) {
     int e;
     const double m = frexp(x, &e);
     uint64_t mb;
     __builtin_memcpy(&mb, &m, sizeof mb);
-    const int i = (int)((mb >> 45) & 127u);  // top 7 fraction bits of m in [0.5, 1)
-    const double t1 = tab[2 * i];
-    const double t2 = tab[2 * i + 1];
+    const int i = (int)((mb >> 45) & 127u);
+    const double t1 = tab[OFG_TAB_LOG + 2 * i];
+    const double t2 = tab[OFG_TAB_LOG + 2 * i + 1];
     const double r = fma(m, t1, -1.0);
-    double p = 1.0 / 7.0;
-    p = fma(p, r, -1.0 / 6.0);
-    p = fma(p, r, 1.0 / 5.0);
-    p = fma(p, r, -0.25);
-    p = fma(p, r, 1.0 / 3.0);
-    p = fma(p, r, -0.5);
-    const double l1p = fma(r * r, p, r);
-    const double ed = (double)e;
-    const double LN2_HI = 6.93147180369123816490e-01;
-    const double LN2_LO = 1.90821492927058770002e-10;
-    return fma(ed, LN2_HI, t2 + fma(ed, LN2_LO, l1p));
+    double q = -2.0 / 7.0;
+    q = fma(q, r, 1.0 / 3.0);
+    q = fma(q, r, -0.4);
+    q = fma(q, r, 0.5);
+    q = fma(q, r, -2.0 / 3.0);
+    q = fma(q, r, 1.0);
+    q = fma(q, r, -2.0);
+    return fma((double)e, -1.3862943611198906, t2 + r * q);
 }
 
-// sqrt of a positive finite double from the reciprocal square-root seed and
-// one Goldschmidt refinement.
+// sqrt of a positive finite double: the reciprocal square-root seed y
+// (relative error ~2^-23 on gfx950), then two Newton corrections
+// g <- g + (t - g^2) y/2 (errors ~1.5 eps^2, then ~1.5 eps^3).
 OFG_HD double sqrt_pos(double t) {
     const double y = rsq_seed(t);
+    const double h = 0.5 * y;
     double g = t * y;
-    double h = 0.5 * y;
-    const double r = fma(-g, h, 0.5);
-    g = fma(g, r, g);
-    h = fma(h, r, h);
-    const double d = fma(-g, g, t);
-    return fma(d, h, g);
+    g = fma(fma(-g, g, t), h, g);
+    return fma(fma(-g, g, t), h, g);
 }
 
-// sin and cos of |t| <= pi/4: degree-5 Chebyshev-node interpolants in
-// w = t^2 of (sin t / t - 1)/w and (cos t - 1 + w/2)/w^2 (max relative error
-// 2.0e-17 and 1.3e-18 in exact arithmetic).
-OFG_HD void sincos_octant(double t, double &sn, double &cs) {
-    const double w = t * t;
-    double ps = 1.5918129294866608e-10;
-    ps = fma(ps, w, -2.5051131845003624e-08);
-    ps = fma(ps, w, 2.755731610255244e-06);
-    ps = fma(ps, w, -0.00019841269836758574);
-    ps = fma(ps, w, 0.008333333333330948);
-    ps = fma(ps, w, -0.16666666666666666);
-    sn = fma(t * w, ps, t);
-    double pc = -1.1382632425521717e-11;
-    pc = fma(pc, w, 2.08761462684032e-09);
-    pc = fma(pc, w, -2.7557317271729793e-07);
-    pc = fma(pc, w, 2.480158729876569e-05);
-    pc = fma(pc, w, -0.0013888888888887398);
-    pc = fma(pc, w, 0.041666666666666664);
-    cs = fma(w * w, pc, fma(-0.5, w, 1.0));
+// (sin, cos) of the angle 2 pi (wt + 1/2) / 2^32: the table holds the pair at
+// a_i = 2 pi (i + 1/2) / 512 for the top 9 bits i of wt; the remaining 23
+// bits give d = angle - a_i, |d| <= pi/512, and the pair is rotated by d:
+//   sin d = d (1 - w/6 + w^2/120)  (relative truncation 1.1e-17),
+//   cos d = 1 + w (-1/2 + c w)     (c fitted, absolute error 7.9e-18),
+// w = d^2.
+OFG_HD void sincos_turn(uint32_t wt, double &sn, double &cs, const double *tab) {
+    const uint32_t i = wt >> 23;
+    const double sa = tab[OFG_TAB_SINCOS + 2 * i];
+    const double ca = tab[OFG_TAB_SINCOS + 2 * i + 1];
+    // 2 pi / 2^32 and 2 pi (1/2 - 2^22) / 2^32
+    const double d = fma((double)(wt & 0x7FFFFFu), 1.4629180792671596e-09, -0.0061359224200835255);
+    const double w = d * d;
+    const double sd = fma(d * w, fma(w, 1.0 / 120.0, -1.0 / 6.0), d);
+    const double cm = w * fma(w, 0.041666619912862315, -0.5);
+    sn = fma(ca, sd, fma(sa, cm, sa));
+    cs = fma(-sa, sd, fma(ca, cm, ca));
 }
 
-// Box-Muller pair from 52 bits of `wu` (radius), 3 bits of `wu` (symmetry)
-// and 32 bits `wt` (angle), all branch-free:
-//   u  = (k + 1/2) 2^-52 in (0, 1), k = wu >> 12, so log u is finite and < 0;
-//   R  = sqrt(-2 log u);
-//   an angle uniform on [-pi/4, pi/4) from wt, mapped to the whole circle by
-//   one of the 8 symmetries of the square (swap, two sign flips) chosen by
-//   bits 8..10 of wu: the image is uniform on the circle.
+// Box-Muller pair from the top 52 bits of `wu` (radius) and 32 bits `wt`
+// (angle), branch-free:
+//   u = (k + 1/2) 2^-52 in (0, 1), k = wu >> 12, so log u is finite and < 0;
+//   (z0, z1) = sqrt(-2 log u) (cos, sin)(2 pi (wt + 1/2) / 2^32).
 OFG_HD void box_muller_pair(uint64_t wu, uint32_t wt, double &z0, double &z1, const double *tab) {
     const double one_k = bits_as_double(0x3FF0000000000000ull | (wu >> 12));  // 1 + k 2^-52
     const double u = one_k - 0.99999999999999988898;                        // exact
-    const double rad = sqrt_pos(-2.0 * log_fp64(u, tab));
-    const double t = fma((double)wt, 3.657295198167899e-10, -0.7853981633974483);  // pi / 2^33, pi / 4
+    const double rad = sqrt_pos(neg2log_fp64(u, tab));
     double sn, cs;
-    sincos_octant(t, sn, cs);
-    const bool swap = (wu >> 8) & 1u;
-    double a = swap ? sn : cs;
-    double b = swap ? cs : sn;
-    a = ((wu >> 9) & 1u) ? -a : a;
-    b = ((wu >> 10) & 1u) ? -b : b;
-    z0 = rad * a;
-    z1 = rad * b;
+    sincos_turn(wt, sn, cs, tab);
+    z0 = rad * cs;
+    z1 = rad * sn;
 }
 
 // Four standard normals from three 64-bit draws (two Box-Muller pairs; the
@@ -496,5 +353,33 @@ OFG_HD void normals4(Rng &rng, double &z0, double &z1, double &z2, double &z3, c
     box_muller_pair(wa, (uint32_t)(wc >> 32), z0, z1, tab);
     box_muller_pair(wb, (uint32_t)wc, z2, z3, tab);
 }
+
+// 2^(y/256) for |y| < 2^31, the fast mode's exp: callers scale the exponent
+// by 256/ln2 once per option (x = y ln2/256, so |x| < 5.8e6; beyond |x| ~ 745
+// ldexp already returns 0 or inf as exp does). Adding 1.5 2^52 rounds y to
+// the integer k held in the low 32 bits; k splits into the table index
+// j = k mod 256 and the binary exponent floor(k/256), and f = y - k is exact
+// (|f| <= 1/2); 2^(f/256) - 1 is its Taylor series through f^4 (truncation
+// 3.8e-17 relative).
+OFG_HD double exp2_256(double y, const double *tab) {
+    const double kd = y + 6755399441055744.0;
+    const double k = kd - 6755399441055744.0;
+    const double f = y - k;
+    uint64_t kb;
+    __builtin_memcpy(&kb, &kd, sizeof kb);
+    const int ki = (int)(uint32_t)kb;
+    const double t = tab[OFG_TAB_EXP2 + (ki & 255)];
+    // (ln2/256)^n / n!, n = 4, 3, 2, 1
+    double q = fma(f, 2.239395190875157e-12, 3.3083026805413713e-09);
+    q = fma(q, f, 3.6655655969101062e-06);
+    q = fma(q, f, 0.0027076061740622863);
+    return ldexp(fma(t, f * q, t), ki >> 8);
+}
+
+// ln2 / 256 and its inverse: the fast mode's exponent scale; the bound on
+// |y| that exp2_256 accepts.
+#define OFG_LN2_256 0.0027076061740622863
+#define OFG_INV_LN2_256 369.3299304675746
+#define OFG_EXP2_256_YMAX 1073741824.0
 
 }  // namespace ofg

@@ -42,12 +42,28 @@ __device__ __forceinline__ void leg_observe(Leg &g, double w, double bsign, doub
     if constexpr (PK == PK_BARRIER) g.hit = g.hit || (bsign * g.s >= bsign * barrier);
 }
 
-// One GBM step (simulation.rs:67-70): s *= exp(fma(diffusion, z, drift)).
-template <int PK>
-__device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z) {
-    const double e = fma(o.diffusion, z, o.drift);
-    g.s *= exp_fp64(e);
-    if constexpr (PK == PK_ASIAN_LOG) g.ls += e;
+// Growth factor exp(x) of one GBM step (simulation.rs:67-70), x = diffusion z
+// + drift, and x itself. FAST evaluates it as 2^(y/256) with y = x 256/ln2
+// formed from the pre-scaled constants (one rounding, as for x).
+template <bool FAST>
+__device__ __forceinline__ double gbm_growth(const OptDev &o, double z, const double *tab,
+                                             double &x) {
+    if constexpr (FAST) {
+        const double y = fma(o.diffusion2, z, o.drift2);
+        x = y * OFG_LN2_256;
+        return exp2_256(y, tab);
+    } else {
+        x = fma(o.diffusion, z, o.drift);
+        return exp_fp64(x);
+    }
+}
+
+// One GBM step: s *= exp(fma(diffusion, z, drift)).
+template <int PK, bool FAST>
+__device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z, const double *tab) {
+    double x;
+    g.s *= gbm_growth<FAST>(o, z, tab, x);
+    if constexpr (PK == PK_ASIAN_LOG) g.ls += x;
 }
 
 // One Heston Euler step (stochastic.rs:56-71) plus the generator clamps
@@ -56,11 +72,12 @@ __device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z) {
 //                compiled with -ffp-contract=off);
 //   REF = false: the same scheme re-associated into fused multiply-adds with
 //                per-option constants (fast mode): kappa dt, kappa theta dt,
-//                xi sqrt(dt), mu dt, dt / 2.
+//                xi sqrt(dt), and the log-spot increment's mu dt, dt / 2,
+//                sqrt(dt) pre-scaled by 256/ln2 for exp2_256.
 template <bool REF>
 __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o, double z1,
                                              double z2, double &s_next, double &v_next,
-                                             double &growth, double &e) {
+                                             double &growth, double &e, const double *tab) {
     const double v_pos = fmax(v, 0.0);
     double vn;
     if constexpr (REF) {
@@ -68,13 +85,18 @@ __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o
         const double zs = o.rho * z1 + o.rho_c * z2;
         vn = v + o.kappa * (o.theta - v_pos) * o.dt + o.xi * sv * o.sqrt_dt * z1;
         e = (o.mu - 0.5 * v_pos) * o.dt + sv * o.sqrt_dt * zs;
+        growth = exp_fp64(e);
     } else {
         const double sv = v_pos > 0.0 ? sqrt_pos(v_pos) : 0.0;
         const double zs = fma(o.rho, z1, o.rho_c * z2);
         vn = fma(o.xsdt * sv, z1, fma(-o.kdt, v_pos, v + o.kthdt));
-        e = fma(sv * o.sqrt_dt, zs, fma(-o.hdt, v_pos, o.mudt));
+        // v has no a-priori bound: clamp y into exp2_256's domain (past
+        // |y| = 2^30 the growth is 0 or inf either way)
+        double y = fma(sv * o.sqrt_dt2, zs, fma(-o.hdt2, v_pos, o.mudt2));
+        y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
+        e = y * OFG_LN2_256;
+        growth = exp2_256(y, tab);
     }
-    growth = exp_fp64(e);
     const double sn = s * growth;
     s_next = !(sn >= 1e-12) ? 1e-12 : sn;   // f64::max(sn, 1e-12), NaN -> 1e-12
     v_next = fmax(vn, 0.0);
@@ -82,9 +104,9 @@ __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o
 
 template <int PK, bool REF>
 __device__ __forceinline__ void heston_step(Leg &g, double &v, const OptDev &o, double z1,
-                                            double z2) {
+                                            double z2, const double *tab) {
     double s_next, v_next, growth, e;
-    heston_euler<REF>(g.s, v, o, z1, z2, s_next, v_next, growth, e);
+    heston_euler<REF>(g.s, v, o, z1, z2, s_next, v_next, growth, e, tab);
     if constexpr (PK == PK_ASIAN_LOG) {
         const bool floor_hit = !(g.s * growth >= 1e-12);
         g.ls = floor_hit ? -27.631021115928547 : g.ls + e;   // ln(1e-12) at the floor

@@ -1,46 +1,15 @@
 """The device math header built for the host (CPU only).
 
-Checks the exact functions the kernels run: exp_fp64 against high-precision
-exp, the Acklam restatement and the reference-draw stream against the oracle
-(bitwise: same operations, and the host build uses the same libm).
+Checks the exact functions the kernels run: exp_fp64 and the fast mode's
+-2 log, sqrt, exp2 and sin/cos against mpmath, the Acklam restatement and the
+reference-draw stream against the oracle (bitwise: same operations, and the
+host build uses the same libm), the fast normals' distribution and the
+xoshiro inverse.
 """
 import ctypes as C
-import os
-import subprocess
-import tempfile
 
 import mpmath as mp
 import numpy as np
-import pytest
-
-HERE = os.path.dirname(os.path.abspath(__file__))
-
-
-@pytest.fixture(scope="module")
-def shim():
-    out = os.path.join(tempfile.gettempdir(), f"ofg_math_shim_{os.getpid()}.so")
-    cmd = ["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-host-only", "-O2", "-ffp-contract=off",
-           "-fPIC", "-shared", "-o", out, os.path.join(HERE, "cpu_shim", "math_shim.hip")]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        pytest.fail("host build of ofg_math.hpp failed:\n" + r.stderr)
-    L = C.CDLL(out)
-    L.shim_exp.argtypes = [C.c_double]; L.shim_exp.restype = C.c_double
-    L.shim_acklam.argtypes = [C.c_double]; L.shim_acklam.restype = C.c_double
-    L.shim_reference_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_int64,
-                                         C.POINTER(C.c_double)]
-    L.shim_reference_normals.restype = None
-    L.shim_log.argtypes = [C.c_double]; L.shim_log.restype = C.c_double
-    L.shim_sqrt.argtypes = [C.c_double]; L.shim_sqrt.restype = C.c_double
-    L.shim_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-    L.shim_sincos.restype = None
-    L.shim_fast_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.POINTER(C.c_double)]
-    L.shim_fast_normals.restype = None
-    L.shim_box_muller.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double),
-                                  C.POINTER(C.c_double)]
-    L.shim_box_muller.restype = None
-    yield L
-    os.unlink(out)
 
 
 def test_exp_fp64_accuracy(shim):
@@ -75,52 +44,94 @@ def test_reference_draws_match_oracle_bitwise(shim, oracle):
         assert np.array_equal(a, z1)
 
 
-def test_log_sqrt_sincos_accuracy(shim):
+def test_neg2log_sqrt_accuracy(shim):
     mp.mp.dps = 30
     rng = np.random.default_rng(11)
-    # domain (0, 1): the Box-Muller argument
+    # domain (0, 1): the Box-Muller argument, down to the smallest normal
     xs = np.concatenate([rng.uniform(0, 1, 3000), np.geomspace(2.2e-308, 0.999, 3000),
-                         1 - np.geomspace(1.1e-16, 0.5, 1000)])
+                         1 - np.geomspace(1.1e-16, 0.5, 1000), [2.0**-53, 0.5, 0.25]])
     worst = 0.0
     for x in xs:
         x = float(x)
         if x >= 1.0:
             continue
-        ref = mp.log(mp.mpf(x))
-        worst = max(worst, float(abs((mp.mpf(shim.shim_log(x)) - ref) / ref)))
-    assert worst <= 4e-16, worst
+        ref = -2 * mp.log(mp.mpf(x))
+        worst = max(worst, float(abs((mp.mpf(shim.shim_neg2log(x)) - ref) / ref)))
+    assert worst <= 3.5e-16, worst
+    # every table interval, both ends, at three binary exponents
+    for e in (0, -1, -40):
+        for i in range(128):
+            for m in (0.5 + i / 256, 0.5 + (i + 1) / 256 - 2.0**-53):
+                x = m * 2.0**e
+                ref = -2 * mp.log(mp.mpf(x))
+                assert abs((mp.mpf(shim.shim_neg2log(x)) - ref) / ref) <= 3.5e-16, x
     worst = 0.0
     for x in np.geomspace(1e-17, 80.0, 3000):
         ref = mp.sqrt(mp.mpf(float(x)))
         worst = max(worst, float(abs((mp.mpf(shim.shim_sqrt(float(x))) - ref) / ref)))
     assert worst <= 2.3e-16, worst
-    ws = ws2 = 0.0
+
+
+def test_exp2_256_accuracy_and_range(shim):
+    """2^(y/256) against mpmath; y covers the exponents of path steps, every
+    table index with both signs of k, and the saturated ends."""
+    mp.mp.dps = 30
+    ys = np.concatenate([np.linspace(-30, 30, 4001) * 369.3299304675746,
+                         np.arange(-1024, 1024) + 0.5, np.arange(-1024, 1024, dtype=float),
+                         np.random.default_rng(4).uniform(-700, 700, 2000) * 369.3299304675746])
+    worst = 0.0
+    for y in ys:
+        got = shim.shim_exp2_256(float(y))
+        ref = mp.power(2, mp.mpf(float(y)) / 256)
+        worst = max(worst, float(abs(mp.mpf(got) - ref) / ref))
+    assert worst <= 2.5e-16, worst
+    assert shim.shim_exp2_256(0.0) == 1.0 and shim.shim_exp2_256(256.0) == 2.0
+    assert shim.shim_exp2_256(-256.0) == 0.5
+    for big in (1e6, 1e9, 2.0**30):
+        assert shim.shim_exp2_256(big) == float("inf")
+        assert shim.shim_exp2_256(-big) == 0.0
+
+
+def test_sincos_of_a_turn(shim):
+    """(sin, cos) of 2 pi (wt + 1/2) / 2^32 over random words and every
+    table-interval boundary."""
+    mp.mp.dps = 30
+    rng = np.random.default_rng(12)
+    wts = list(rng.integers(0, 2**32, 4000))
+    for i in range(512):
+        for off in (0, 1, (1 << 22) - 1, 1 << 22, (1 << 23) - 1):
+            wts.append((i << 23) + off)
+    wts += [0, 2**32 - 1, 2**30, 2**31, 3 * 2**30]
     s, c = C.c_double(), C.c_double()
-    for t in np.linspace(-np.pi / 4, np.pi / 4, 4001):
-        shim.shim_sincos(float(t), C.byref(s), C.byref(c))
-        if t != 0:
-            ws = max(ws, float(abs((mp.mpf(s.value) - mp.sin(float(t))) / mp.sin(float(t)))))
-        ws2 = max(ws2, float(abs((mp.mpf(c.value) - mp.cos(float(t))) / mp.cos(float(t)))))
-    assert ws <= 3e-16 and ws2 <= 3e-16, (ws, ws2)
+    worst = 0.0
+    for wt in wts:
+        wt = int(wt)
+        shim.shim_sincos_turn(wt, C.byref(s), C.byref(c))
+        a = 2 * mp.pi * (mp.mpf(wt) + mp.mpf(1) / 2) / 2**32
+        worst = max(worst, float(abs(s.value - mp.sin(a))), float(abs(c.value - mp.cos(a))))
+    assert worst <= 1.7e-16, worst  # absolute, 1.5 ulp of 1
 
 
-def test_box_muller_extremes_and_symmetries(shim):
+def test_box_muller_extremes_and_angles(shim):
     z0, z1 = C.c_double(), C.c_double()
     # largest radius: k = 0 -> u = 2^-53, R = sqrt(106 ln 2)
-    shim.shim_box_muller(0, 2**31, C.byref(z0), C.byref(z1))
-    r = np.hypot(z0.value, z1.value)
-    assert abs(r - np.sqrt(-2 * np.log(2.0**-53))) < 1e-14
-    # angle 0 (wt = 2^31) and no symmetry bits -> (R, 0)
-    assert abs(z1.value) < 1e-15 and z0.value > 0
+    rmax = np.sqrt(-2 * np.log(2.0**-53))
+    shim.shim_box_muller(0, 0, C.byref(z0), C.byref(z1))
+    assert abs(np.hypot(z0.value, z1.value) - rmax) < 1e-14
+    # angle of wt = 0 is pi / 2^32: (R, R pi 2^-32), to sincos_turn's absolute bound
+    assert abs(z0.value - rmax) < 1e-14 and abs(z1.value - rmax * np.pi / 2**32) < 1.7e-16 * rmax
+    # quarter, half and three-quarter turns
+    for wt, (cx, cy) in [(2**30, (0, 1)), (2**31, (-1, 0)), (3 * 2**30, (0, -1))]:
+        shim.shim_box_muller(0, wt, C.byref(z0), C.byref(z1))
+        assert abs(z0.value - cx * rmax) < 1e-8 and abs(z1.value - cy * rmax) < 1e-8
     # smallest radius: k = 2^52 - 1 -> u = 1 - 2^-53
     shim.shim_box_muller(((1 << 52) - 1) << 12, 2**31, C.byref(z0), C.byref(z1))
     assert 0 < np.hypot(z0.value, z1.value) < 2e-8
-    # the 8 symmetry codes map one angle to 8 distinct points of the circle
-    pts = set()
-    for code in range(8):
-        shim.shim_box_muller((1 << 63) | (code << 8), 2**31 + 2**29, C.byref(z0), C.byref(z1))
-        pts.add((round(z0.value, 12), round(z1.value, 12)))
-    assert len(pts) == 8
+    # radius of a mid-range u, against mpmath
+    k = 123456789012345
+    shim.shim_box_muller(k << 12, 12345, C.byref(z0), C.byref(z1))
+    u = (mp.mpf(k) + mp.mpf(1) / 2) * mp.mpf(2) ** -52
+    assert abs(np.hypot(z0.value, z1.value) / float(mp.sqrt(-2 * mp.log(u))) - 1) < 5e-16
 
 
 def test_fast_normals_distribution(shim):
@@ -151,9 +162,6 @@ def test_fast_normals_distribution(shim):
 
 
 def test_xoshiro_inverse_walks_the_stream_back(shim):
-    shim.shim_xoshiro_roundtrip.argtypes = [C.c_uint64, C.c_int, C.c_int64,
-                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
-    shim.shim_xoshiro_roundtrip.restype = C.c_int
     for plus in (0, 1):
         for seed in (0, 42, 2**64 - 1, 123456789):
             n = 1000

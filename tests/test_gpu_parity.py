@@ -193,3 +193,39 @@ def test_fast_mode_within_3se_of_bs(oracle, gpu):
     assert abs(r.price - bs) <= 3 * r.stderr_
     p_ref, se_ref = oracle.mc_run(oo, 100_000, 252, 42, antithetic=True)
     assert abs(r.price - p_ref) <= 3 * np.hypot(r.stderr_, se_ref)
+
+
+@pytest.mark.parametrize("steps", [1, 7, 64, 253])
+def test_fast_mode_gbm_paths_equal_host_build(gpu, shim, steps):
+    """Fast-mode GBM paths on the GPU equal, sample by sample, the host build
+    of the same device math (tests/cpu_shim): xoshiro256+ stream, Box-Muller
+    from the LDS tables, 2^(y/256) steps. Only the GPU's reciprocal-sqrt seed
+    and rounding of fused operations may move the last bits. A call struck at
+    1e-9 makes the payoff S_T - 1e-9."""
+    from openferric_amd import sample_payoffs
+    n, seed, s0, r, vol, T = 3000, 17, 100.0, 0.04, 0.3, 1.5
+    _, g, keep = make_pair(strike=1e-9, spot=s0, rate=r, vol=vol, expiry=T, num_paths=n,
+                           num_steps=steps, seed=seed)
+    xs, _ = sample_payoffs(g, _cfg("fast"), False, want_y=False)
+    host = np.array([shim.shim_fast_gbm_terminal(seed, i, s0, r, vol, T, steps)
+                     for i in range(n)]) - 1e-9
+    np.testing.assert_allclose(xs, host, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("steps", [1, 6, 101])
+def test_fast_mode_heston_paths_equal_host_build(gpu, shim, steps):
+    """As above for the fast-mode Heston step (re-associated Euler step,
+    2^(y/256) growth, generator clamps), including a path set whose variance
+    hits zero often (Feller condition violated)."""
+    from openferric_amd import sample_payoffs
+    n, seed, s0, r, T = 2000, 23, 100.0, 0.03, 2.0
+    for kappa, theta, xi, rho, v0 in [(1.5, 0.04, 0.5, -0.6, 0.05), (0.5, 0.02, 1.0, -0.9, 0.01)]:
+        _, g, keep = make_pair(model="heston", strike=1e-9, spot=s0, rate=r, expiry=T,
+                               kappa=kappa, theta=theta, xi=xi, rho=rho, v0=v0, num_paths=n,
+                               num_steps=steps, seed=seed)
+        xs, _ = sample_payoffs(g, _cfg("fast"), False, want_y=False)
+        host = np.array([shim.shim_fast_heston_terminal(seed, i, s0, r, kappa, theta, xi, rho,
+                                                         v0, T, steps)
+                         for i in range(n)])
+        host = np.maximum(host - 1e-9, 0.0)
+        np.testing.assert_allclose(xs, host, rtol=1e-12, atol=1e-12)
