@@ -5,11 +5,11 @@
 
 Every entry is computed with mpmath at 50 digits and rounded once to double.
 Layout of OFG_FAST_TABLE (doubles):
-  [0, 256)     log: 128 pairs (T1, 2 log T1). Interval i of m in [0.5, 1) has
-               midpoint c_i = 0.5 + (i + 1/2)/256 and T1 = 1/c_i rounded; the
+  [0, 512)     log: 256 pairs (T1, 2 log T1). Interval i of m in [0.5, 1) has
+               midpoint c_i = 0.5 + (i + 1/2)/512 and T1 = 1/c_i rounded; the
                last interval uses T1 = 1 so that r = m - 1 is exact near 1.
-  [256, 512)   exp2: 2^(j/256), j = 0..255.
-  [512, 1536)  sincos: 512 pairs (sin a_i, cos a_i), a_i = 2 pi (i + 1/2) / 512.
+  [512, 768)   exp2: 2^(j/256), j = 0..255.
+  [768, 1792)  sincos: 512 pairs (sin a_i, cos a_i), a_i = 2 pi (i + 1/2) / 512.
 It also prints the fitted coefficient of the cos correction polynomial.
 """
 import os
@@ -26,11 +26,11 @@ def _d(x) -> str:
 
 def log_pairs():
     out = []
-    for i in range(128):
-        if i == 127:
+    for i in range(256):
+        if i == 255:
             t1 = 1.0
         else:
-            c = mp.mpf(1) / 2 + (mp.mpf(i) + mp.mpf(1) / 2) / 256
+            c = mp.mpf(1) / 2 + (mp.mpf(i) + mp.mpf(1) / 2) / 512
             t1 = float(1 / c)
         out.append((t1, 2 * mp.log(mp.mpf(t1))))
     return out
@@ -71,9 +71,9 @@ def cos_fit():
 
 def main():
     lines = ["// Generated by gen_fast_tables.py (mpmath, 50 digits, rounded once).",
-             "// Layout: [0,256) log pairs (T1, 2 log T1); [256,512) 2^(j/256);",
-             "// [512,1536) (sin, cos) of 2 pi (i + 1/2) / 512.",
-             "#define OFG_FAST_TABLE_LEN 1536",
+             "// Layout: [0,512) log pairs (T1, 2 log T1); [512,768) 2^(j/256);",
+             "// [768,1792) (sin, cos) of 2 pi (i + 1/2) / 512.",
+             "#define OFG_FAST_TABLE_LEN 1792",
              "static constexpr double OFG_FAST_TABLE[OFG_FAST_TABLE_LEN] = {"]
     for t1, t2 in log_pairs():
         lines.append(f"    {_d(t1)}, {_d(t2)},")

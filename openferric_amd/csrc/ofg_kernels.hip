@@ -183,8 +183,21 @@ __device__ __forceinline__ void dispatch_kind(const OptDev &o, const WorkItem &w
     }
 }
 
+// Minimum waves per SIMD asked of the register allocator: the plain fast GBM
+// loop fits 128 VGPRs (4 waves) without spilling.
+template <int RNG, int MODEL, bool ANTI>
+constexpr int price_waves_per_eu() {
+    return (RNG == OFG_RNG_FAST && MODEL == OFG_MODEL_GBM && !ANTI) ? 4 : 1;
+}
 template <int RNG, int MODEL, bool ANTI, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void price_kernel(LaunchArgs a) {
+constexpr int price_waves_target() {
+    // one-wave blocks are held to fewer waves by the LDS table anyway
+    return BLOCK >= 128 ? price_waves_per_eu<RNG, MODEL, ANTI>() : 1;
+}
+
+template <int RNG, int MODEL, bool ANTI, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
+    price_waves_target<RNG, MODEL, ANTI, BLOCK>()))) void price_kernel(LaunchArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     // log table of the fast-mode Box-Muller, staged once per block in LDS
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];

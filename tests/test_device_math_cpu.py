@@ -60,8 +60,8 @@ def test_neg2log_sqrt_accuracy(shim):
     assert worst <= 3.5e-16, worst
     # every table interval, both ends, at three binary exponents
     for e in (0, -1, -40):
-        for i in range(128):
-            for m in (0.5 + i / 256, 0.5 + (i + 1) / 256 - 2.0**-53):
+        for i in range(256):
+            for m in (0.5 + i / 512, 0.5 + (i + 1) / 512 - 2.0**-53):
                 x = m * 2.0**e
                 ref = -2 * mp.log(mp.mpf(x))
                 assert abs((mp.mpf(shim.shim_neg2log(x)) - ref) / ref) <= 3.5e-16, x
