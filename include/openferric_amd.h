@@ -109,6 +109,16 @@ typedef struct ofg_result {
     double kernel_ms;         /* device time of this call's kernels */
 } ofg_result;
 
+/* The five estimates of MonteCarloGreeksEngine::run_estimators
+ * (src/engines/monte_carlo/mc_greeks.rs:131-243, MonteCarloGreekEstimate
+ * :235-242): discounted means over the samples. */
+typedef struct ofg_mc_greeks {
+    double pathwise_delta, pathwise_gamma;
+    double lr_delta, lr_gamma, lr_vega;
+    double num_samples;       /* ceil(paths/2) with antithetic, else paths */
+    double kernel_ms;
+} ofg_mc_greeks;
+
 typedef struct ofg_ctx ofg_ctx;
 typedef struct ofg_batch ofg_batch;
 
@@ -148,6 +158,20 @@ int32_t ofg_sample_payoffs(ofg_ctx *ctx, const ofg_option *opt,
  * greek_mode = OFG_GREEKS_ADJOINT. */
 int32_t ofg_price_greeks(ofg_ctx *ctx, const ofg_option *opts, size_t n,
                          const ofg_engine_cfg *cfg, ofg_result *out);
+
+/* Pathwise and likelihood-ratio Greeks of European GBM vanillas from one
+ * exact-terminal normal per sample, n options in one persistent launch.
+ * Replaces MonteCarloGreeksEngine::estimate_pathwise and
+ * ::estimate_likelihood_ratio (src/engines/monte_carlo/mc_greeks.rs:93-129;
+ * estimators :131-284). opts[i] must be GBM vanillas (num_steps is ignored:
+ * the estimator steps once over T; div_yield is the market's effective
+ * yield); cfg supplies antithetic and rng_mode (control_variate and
+ * greek_mode are ignored); spot_bump_rel is the relative spot bump of the
+ * pathwise gamma (h = max(spot * rel, 1e-6)). Validation and messages follow
+ * run_estimators; expiry 0 gives all-zero estimates. out[n]. */
+int32_t ofg_mc_greeks_estimate(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                               const ofg_engine_cfg *cfg, double spot_bump_rel,
+                               ofg_mc_greeks *out);
 
 /* Batched pricing for repeated replay: builds the work queue, uploads the
  * contracts once and captures [queue reset, persistent pricing kernel,

@@ -11,7 +11,8 @@ from .core import (AsianOption, AsianSpec, Averaging, BarrierDirection, BarrierO
                    OptionType, PricingError, PricingResult, StrikeType, VanillaOption)
 from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEngine,
                       HestonAadConfig, HestonMonteCarloPricingEngine, RngMode, VarianceReduction,
-                      autotune, engine_cfg, heston_price_delta_aad, mc_european_pathwise_aad,
+                      MonteCarloGreeksEngine, autotune, engine_cfg, heston_price_delta_aad,
+                      mc_european_parallel, mc_european_pathwise_aad, mc_european_sequential,
                       option_struct, price_raw, sample_payoffs)
 from ._lib import DeviceError
 
@@ -23,5 +24,6 @@ __all__ = [
     "GpuArithmeticAsianMC", "GpuMonteCarloPricingEngine", "HestonAadConfig",
     "HestonMonteCarloPricingEngine", "RngMode", "VarianceReduction", "autotune", "engine_cfg",
     "heston_price_delta_aad", "mc_european_pathwise_aad", "option_struct", "price_raw",
-    "sample_payoffs", "DeviceError",
+    "sample_payoffs", "DeviceError", "MonteCarloGreeksEngine", "mc_european_parallel",
+    "mc_european_sequential",
 ]

@@ -33,6 +33,7 @@ EXPORTED_SYMBOLS = [
     "ofg_create", "ofg_destroy", "ofg_last_error", "ofg_version", "ofg_price",
     "ofg_sample_payoffs", "ofg_price_greeks", "ofg_batch_create", "ofg_batch_run",
     "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
+    "ofg_mc_greeks_estimate",
 ]
 
 
@@ -70,6 +71,14 @@ class OfgResult(C.Structure):
         ("heston_grad", C.c_double * 6),
         ("num_paths", C.c_double), ("num_steps", C.c_double), ("vol", C.c_double),
         ("kernel_ms", C.c_double),
+    ]
+
+
+class OfgMcGreeks(C.Structure):
+    _fields_ = [
+        ("pathwise_delta", C.c_double), ("pathwise_gamma", C.c_double),
+        ("lr_delta", C.c_double), ("lr_gamma", C.c_double), ("lr_vega", C.c_double),
+        ("num_samples", C.c_double), ("kernel_ms", C.c_double),
     ]
 
 
@@ -116,6 +125,10 @@ def load() -> C.CDLL:
         L.ofg_autotune.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t, C.POINTER(OfgEngineCfg),
                                    C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.ofg_autotune.restype = C.c_int32
+        L.ofg_mc_greeks_estimate.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t,
+                                             C.POINTER(OfgEngineCfg), C.c_double,
+                                             C.POINTER(OfgMcGreeks)]
+        L.ofg_mc_greeks_estimate.restype = C.c_int32
         _lib = L
         return L
 

@@ -12,8 +12,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libopenferric_amd.so")
-SOURCES = ["ofg_kernels.hip", "ofg_adjoint.hip", "ofg_api.cpp"]
-HEADERS = ["ofg_math.hpp", "ofg_internal.hpp", "ofg_adjoint.hpp", "ofg_paths.hpp"]
+SOURCES = ["ofg_kernels.hip", "ofg_adjoint.hip", "ofg_mcgreeks.hip", "ofg_api.cpp"]
+HEADERS = ["ofg_math.hpp", "ofg_internal.hpp", "ofg_adjoint.hpp", "ofg_paths.hpp",
+           "ofg_mcgreeks.hpp", "ofg_fast_tables.inc"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -7,6 +7,7 @@ Heston): same names, argument meaning and validation messages.
 from __future__ import annotations
 
 import enum
+import math
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -334,7 +335,14 @@ class Market:
         return False
 
     def effective_dividend_yield(self, maturity: float) -> float:
-        return self.dividend_yield
+        """market.rs:275-282 with an empty dividend schedule
+        (dividends.rs:151-217): r - ln(F/S)/T with F = S exp((r - q) T), which
+        equals q up to rounding."""
+        if maturity <= 0.0 or self.spot <= 0.0:
+            return self.dividend_yield
+        forward = self.spot * math.exp((self.rate - self.dividend_yield) * maturity)
+        ratio = max(forward / self.spot, 1.0e-16)
+        return self.rate - math.log(ratio) / maturity
 
 
 class MarketBuilder:

@@ -11,6 +11,7 @@
 #include "ofg_internal.hpp"
 #include "ofg_math.hpp"
 #include "ofg_adjoint.hpp"
+#include "ofg_mcgreeks.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -632,6 +633,119 @@ int32_t guarded(F &&f) {
 int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
                    ofg_result *out);
 
+// MonteCarloGreeksEngine::run_estimators (mc_greeks.rs:131-243): validation
+// in the reference's order, one exact-terminal step per sample on the GPU,
+// then discount * sum / n per estimate.
+int32_t mc_greeks_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
+                       double spot_bump_rel, ofg_mc_greeks *out) {
+    if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
+    if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+    std::vector<ofg_option> sim;
+    std::vector<size_t> sim_of;
+    std::vector<double> ups, dns;
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = opts[i];
+        std::memset(&out[i], 0, sizeof out[i]);
+        validate_instrument(o);
+        if (o.payoff != OFG_PAYOFF_VANILLA || o.model != OFG_MODEL_GBM)
+            fail(OFG_INVALID_INPUT, "MonteCarloGreeksEngine supports European vanillas under GBM only");
+        if (o.num_paths <= 0) fail(OFG_INVALID_INPUT, "num_paths must be > 0");
+        if (!(spot_bump_rel > 0.0) || !std::isfinite(spot_bump_rel))
+            fail(OFG_INVALID_INPUT, "spot_bump_rel must be finite and > 0");
+        if (o.expiry <= 0.0) continue;  // default (all-zero) estimate
+        if (o.spot <= 0.0) fail(OFG_INVALID_INPUT, "market spot must be > 0");
+        if (o.vol <= 0.0 || !std::isfinite(o.vol))
+            fail(OFG_INVALID_INPUT, "market volatility must be finite and > 0");
+        ofg_option s = o;
+        s.num_steps = 1;  // exact terminal value: one step over T
+        sim.push_back(s);
+        sim_of.push_back(i);
+        const double h = std::max(o.spot * spot_bump_rel, 1.0e-6);
+        ups.push_back(o.spot + h);
+        dns.push_back(std::max(o.spot - h, 1.0e-8));
+    }
+    if (sim.empty()) return OFG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ofg_engine_cfg c = *cfg;
+    c.control_variate = 0;
+    c.greek_mode = OFG_GREEKS_NONE;
+    Tuning t;
+    t.block = cfg->tune_block > 0 ? std::min(cfg->tune_block, 256) : 256;
+    t.item_steps = cfg->tune_item_steps > 0 ? cfg->tune_item_steps : 4096;
+    Plan p = build_plan(sim.data(), sim.size(), c, t);
+    const size_t nopt = p.opts.size();
+    for (size_t k = 0; k < nopt; ++k) {
+        p.opts[k].spot_up = ups[(size_t)p.active[k]];
+        p.opts[k].spot_dn = dns[(size_t)p.active[k]];
+    }
+    hipStream_t s = ctx->stream;
+    upload(ctx->opts, p.opts, s);
+    upload(ctx->items, p.items, s);
+    upload(ctx->order, p.order, s);
+    upload(ctx->item_first, p.item_first, s);
+    upload(ctx->item_count, p.item_count, s);
+    ctx->partials.ensure(p.items.size() * sizeof(ofg::AdjPartial));
+    ctx->outs.ensure(nopt * sizeof(ofg::AdjOut));
+    ctx->head.ensure(256);
+    const double *n1_dev = nullptr;
+    if (c.rng_mode == OFG_RNG_EXTERNAL) {
+        if (!c.normals1) fail(OFG_INVALID_INPUT, "external normals required for OFG_RNG_EXTERNAL");
+        const size_t bytes = (size_t)p.normals_elems * sizeof(double);
+        ctx->n1.ensure(bytes);
+        hip_check(hipMemcpyAsync(ctx->n1.p, c.normals1, bytes, hipMemcpyHostToDevice, s),
+                  "H2D normals");
+        n1_dev = ctx->n1.as<double>();
+    }
+    ofg::AdjArgs a{};
+    a.opts = ctx->opts.as<ofg::OptDev>();
+    a.items = ctx->items.as<ofg::WorkItem>();
+    a.order = ctx->order.as<int32_t>();
+    a.order_begin = p.model_begin[OFG_MODEL_GBM];
+    a.order_end = p.model_end[OFG_MODEL_GBM];
+    a.queue_head = ctx->head.as<uint32_t>();
+    a.partials = ctx->partials.as<ofg::AdjPartial>();
+    a.normals1 = n1_dev;
+    a.rng_mode = c.rng_mode;
+    a.model = OFG_MODEL_GBM;
+    a.antithetic = p.antithetic;
+    a.block = t.block;
+    const int wpb = t.block / 64;
+    const int32_t cnt = a.order_end - a.order_begin;
+    const int full = ofg::mc_greeks_grid(t.block, c.rng_mode, p.antithetic, ctx->device);
+    a.grid = std::max(1, (int)std::min<int64_t>(full, ((int64_t)cnt + wpb - 1) / wpb));
+    hip_check(ofg::launch_reset_queue(ctx->head.as<uint32_t>(), 1, s), "reset queue");
+    hip_check(hipEventRecord(ctx->ev0, s), "event");
+    hip_check(ofg::launch_mc_greeks(a, s), "mc greeks kernel launch");
+    hip_check(ofg::launch_mc_greeks_reduce(ctx->item_first.as<int64_t>(),
+                                           ctx->item_count.as<int32_t>(),
+                                           ctx->partials.as<ofg::AdjPartial>(),
+                                           ctx->outs.as<ofg::AdjOut>(), (int32_t)nopt, s),
+              "mc greeks reduce launch");
+    hip_check(hipEventRecord(ctx->ev2, s), "event");
+    std::vector<ofg::AdjOut> sums(nopt);
+    hip_check(hipMemcpyAsync(sums.data(), ctx->outs.p, nopt * sizeof(ofg::AdjOut),
+                             hipMemcpyDeviceToHost, s),
+              "D2H mc greeks");
+    hip_check(hipStreamSynchronize(s), "mc greeks execution");
+    float f = 0.f;
+    hip_check(hipEventElapsedTime(&f, ctx->ev0, ctx->ev2), "event time");
+    for (size_t k = 0; k < nopt; ++k) {
+        const ofg_option &o = sim[(size_t)p.active[k]];
+        ofg_mc_greeks &r = out[sim_of[(size_t)p.active[k]]];
+        const double discount = std::exp(-o.rate * o.expiry);
+        const double nn = sums[k].n;
+        r.pathwise_delta = discount * sums[k].g[0] / nn;
+        r.pathwise_gamma = discount * sums[k].g[1] / nn;
+        r.lr_delta = discount * sums[k].g[2] / nn;
+        r.lr_gamma = discount * sums[k].g[3] / nn;
+        r.lr_vega = discount * sums[k].g[4] / nn;
+        r.num_samples = nn;
+        r.kernel_ms = f;
+    }
+    return OFG_OK;
+}
+
 // Central finite differences with common random numbers: every bumped copy
 // uses the same seed, so the same streams (greeks/bsm.rs:130-212 harness,
 // priced in one batched launch).
@@ -945,6 +1059,12 @@ int32_t ofg_price_greeks(ofg_ctx *ctx, const ofg_option *opts, size_t n,
         c.greek_mode = OFG_GREEKS_ADJOINT;
         return price_impl(ctx, opts, n, &c, out);
     });
+}
+
+int32_t ofg_mc_greeks_estimate(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                               const ofg_engine_cfg *cfg, double spot_bump_rel,
+                               ofg_mc_greeks *out) {
+    return guarded([&]() { return mc_greeks_impl(ctx, opts, n, cfg, spot_bump_rel, out); });
 }
 
 int32_t ofg_sample_payoffs(ofg_ctx *ctx, const ofg_option *opt, const ofg_engine_cfg *cfg,

@@ -48,6 +48,8 @@ struct alignas(16) OptDev {
     double kdt, kthdt, xsdt, mudt, hdt;
     // fast mode: exponent constants pre-scaled by 256/ln2 for exp2_256
     double drift2, diffusion2, sqrt_dt2, mudt2, hdt2;
+    // Monte Carlo Greeks estimators: the bumped spots of the pathwise gamma
+    double spot_up, spot_dn;
     int32_t antithetic;
     int32_t has_cv;
 };

@@ -338,7 +338,113 @@ def mc_european_pathwise_aad(engine: GpuMonteCarloPricingEngine, instrument: Van
     vol = market.vol_for(instrument.strike, instrument.expiry)
     if not (vol > 0.0) or vol == float("inf"):
         raise InvalidInput("market volatility must be finite and > 0")
-    return engine.price_with_greeks_aad(instrument, market)
+    # the drift uses the market's effective dividend yield (mc_aad.rs:138)
+    eff = Market(market.spot, market.rate, market.effective_dividend_yield(instrument.expiry),
+                 market.flat_vol)
+    return engine.price_with_greeks_aad(instrument, eff)
+
+
+class MonteCarloGreeksEngine:
+    """Drop-in for MonteCarloGreeksEngine (mc_greeks.rs:20-129): pathwise and
+    likelihood-ratio Greeks of European GBM vanillas from one exact-terminal
+    normal per sample, computed by the GPU estimator kernel
+    (ofg_mc_greeks_estimate)."""
+
+    def __init__(self, num_paths: int, seed: int):
+        self.num_paths = num_paths
+        self.seed = seed
+        self.antithetic = True
+        self.spot_bump_rel = 1.0e-2
+        self.rng_mode = RngMode.Reference
+        self.device = 0
+
+    def with_antithetic(self, antithetic: bool) -> "MonteCarloGreeksEngine":
+        self.antithetic = antithetic
+        return self
+
+    def with_spot_bump_rel(self, spot_bump_rel: float) -> "MonteCarloGreeksEngine":
+        self.spot_bump_rel = max(spot_bump_rel, 1.0e-6)
+        return self
+
+    def with_seed(self, seed: int) -> "MonteCarloGreeksEngine":
+        self.seed = seed
+        return self
+
+    def with_rng_mode(self, mode: RngMode) -> "MonteCarloGreeksEngine":
+        self.rng_mode = mode
+        return self
+
+    def estimates(self, instrument: VanillaOption, market: Market) -> dict:
+        """All five discounted estimates (run_estimators, mc_greeks.rs:131-243)."""
+        instrument.validate()
+        if instrument.exercise is not ExerciseStyle.European:
+            raise InvalidInput("MonteCarloGreeksEngine supports European exercise only")
+        keep = _Keep()
+        eff = Market(market.spot, market.rate, market.effective_dividend_yield(instrument.expiry),
+                     market.flat_vol)
+        o = option_struct(instrument, eff, self.num_paths, 1, self.seed, keep)
+        cfg = engine_cfg(antithetic=self.antithetic, rng_mode=self.rng_mode, keep=keep)
+        L = _lib.load()
+        ctx = _lib.context(self.device)
+        out = _lib.OfgMcGreeks()
+        _raise(L.ofg_mc_greeks_estimate(ctx, C.byref(o), 1, C.byref(cfg), self.spot_bump_rel,
+                                        C.byref(out)))
+        return {"pathwise_delta": out.pathwise_delta, "pathwise_gamma": out.pathwise_gamma,
+                "lr_delta": out.lr_delta, "lr_gamma": out.lr_gamma, "lr_vega": out.lr_vega,
+                "num_samples": out.num_samples, "kernel_ms": out.kernel_ms}
+
+    def estimate_pathwise(self, instrument: VanillaOption, market: Market) -> Greeks:
+        """Pathwise delta and CRN-difference gamma (mc_greeks.rs:93-108)."""
+        e = self.estimates(instrument, market)
+        return Greeks(e["pathwise_delta"], e["pathwise_gamma"], 0.0, 0.0, 0.0)
+
+    def estimate_likelihood_ratio(self, instrument: VanillaOption, market: Market) -> Greeks:
+        """Likelihood-ratio delta, gamma and vega (mc_greeks.rs:110-129)."""
+        e = self.estimates(instrument, market)
+        return Greeks(e["lr_delta"], e["lr_gamma"], e["lr_vega"], 0.0, 0.0)
+
+
+_NAN_RESULT = float("nan")
+
+
+def _mc_european_exact(instrument: VanillaOption, market: Market, n_paths: int) -> PricingResult:
+    """mc_european_parallel / mc_european_sequential (mc_parallel.rs:264-434):
+    the terminal spot sampled exactly (one step over T, one normal per path,
+    no variance reduction); invalid inputs give a NaN price, not an error."""
+    if n_paths == 0 or instrument.exercise is not ExerciseStyle.European:
+        return PricingResult(_NAN_RESULT, None)
+    if instrument.expiry <= 0.0:
+        intrinsic = max(market.spot - instrument.strike, 0.0) \
+            if instrument.option_type is OptionType.Call else max(instrument.strike - market.spot, 0.0)
+        return PricingResult(intrinsic, 0.0)
+    vol = market.vol_for(instrument.strike, instrument.expiry)
+    if not (vol > 0.0) or vol == float("inf"):
+        return PricingResult(_NAN_RESULT, None)
+    keep = _Keep()
+    eff = Market(market.spot, market.rate, market.effective_dividend_yield(instrument.expiry),
+                 market.flat_vol)
+    o = option_struct(instrument, eff, n_paths, 1, 0xDEADBEEFCAFEBABE, keep)
+    r = price_raw([o], engine_cfg(rng_mode=RngMode.Fast, keep=keep))[0]
+    diag = Diagnostics()
+    diag.insert("num_paths", float(n_paths))
+    diag.insert("num_steps", 1.0)
+    diag.insert("vol", vol)
+    return PricingResult(r.price, r.stderr_, diagnostics=diag, kernel_ms=r.kernel_ms)
+
+
+def mc_european_parallel(instrument: VanillaOption, market: Market, n_paths: int,
+                         n_steps: int = 1) -> PricingResult:
+    """GPU counterpart of mc_european_parallel (mc_parallel.rs:264-362).
+    n_steps is ignored, as in the reference. The reference's draws depend on
+    its thread count and SIMD path, so parity is statistical; the reference's
+    `num_threads` diagnostic has no GPU counterpart and is not set."""
+    return _mc_european_exact(instrument, market, n_paths)
+
+
+def mc_european_sequential(instrument: VanillaOption, market: Market, n_paths: int,
+                           n_steps: int = 1) -> PricingResult:
+    """GPU counterpart of mc_european_sequential (mc_parallel.rs:365-434)."""
+    return _mc_european_exact(instrument, market, n_paths)
 
 
 class BatchPricer:

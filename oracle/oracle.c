@@ -720,3 +720,82 @@ int or_heston_price_delta_aad(const or_option *o, int64_t num_paths, int64_t num
     free(z2);
     return 0;
 }
+
+/* ===================================================================== */
+/* MonteCarloGreeksEngine, mc_greeks.rs:131-314                            */
+/* ===================================================================== */
+
+/* pathwise_delta, mc_greeks.rs:293-310 */
+static double mcg_pathwise_delta(int32_t type, double st, double strike, double growth) {
+    if (type == OR_CALL) return st > strike ? growth : 0.0;
+    return st < strike ? -growth : 0.0;
+}
+
+/* single_path_contribution, mc_greeks.rs:245-284 */
+static void mcg_single_path(int32_t type, double z, double spot, double strike,
+                            double spot_up, double spot_dn, double spot_span, double vol,
+                            double maturity, double drift, double sig_sqrt_t, double e[5]) {
+    const double growth = exp(fma(sig_sqrt_t, z, drift));
+    const double st = spot * growth;
+    const double payoff = vanilla_payoff(type, st, strike);
+    const double delta_pw = mcg_pathwise_delta(type, st, strike, growth);
+    const double delta_up = mcg_pathwise_delta(type, spot_up * growth, strike, growth);
+    const double delta_dn = mcg_pathwise_delta(type, spot_dn * growth, strike, growth);
+    const double gamma_pw = (delta_up - delta_dn) / spot_span;
+    const double sqrt_t = sqrt(maturity);
+    const double inv_spot_vol_sqrt_t = 1.0 / (spot * vol * sqrt_t);
+    const double z2 = z * z;
+    const double z2m1 = z2 - 1.0;
+    const double w_delta = z * inv_spot_vol_sqrt_t;
+    const double inv_spot2_vol2_t = inv_spot_vol_sqrt_t * inv_spot_vol_sqrt_t;
+    const double w_gamma = z2m1 * inv_spot2_vol2_t - z * inv_spot_vol_sqrt_t / spot;
+    const double w_vega = (z2m1 - vol * sqrt_t * z) / vol;
+    e[0] = delta_pw;
+    e[1] = gamma_pw;
+    e[2] = payoff * w_delta;
+    e[3] = payoff * w_gamma;
+    e[4] = payoff * w_vega;
+}
+
+/* run_estimators, mc_greeks.rs:131-243 (the non-parallel fold) */
+int or_mc_greeks_estimate(const or_option *o, int64_t num_paths, uint64_t seed,
+                          int32_t antithetic, double spot_bump_rel, double out[5]) {
+    for (int k = 0; k < 5; ++k) out[k] = 0.0;
+    if (!(o->strike > 0.0) || o->expiry < 0.0) return 1;
+    if (num_paths <= 0) return 1;
+    if (!(spot_bump_rel > 0.0) || !isfinite(spot_bump_rel)) return 1;
+    if (o->expiry <= 0.0) return 0;
+    const double spot = o->spot;
+    if (spot <= 0.0) return 1;
+    const double strike = o->strike, maturity = o->expiry, vol = o->vol;
+    if (vol <= 0.0 || !isfinite(vol)) return 1;
+    const double rate = o->rate, dividend = o->div_yield;
+    const double drift = (rate - dividend - 0.5 * vol * vol) * maturity;
+    const double sig_sqrt_t = vol * sqrt(maturity);
+    const double discount = exp(-rate * maturity);
+    double h = spot * spot_bump_rel;
+    if (h < 1.0e-6) h = 1.0e-6;
+    const double spot_up = spot + h;
+    double spot_dn = spot - h;
+    if (spot_dn < 1.0e-8) spot_dn = 1.0e-8;
+    const double spot_span = spot_up - spot_dn;
+    const int64_t samples = antithetic ? (num_paths + 1) / 2 : num_paths;
+    double sums[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < samples; ++i) {
+        uint64_t s[4];
+        or_xoshiro_seed(or_stream_seed(seed, (uint64_t)i), s);
+        const double z = or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(s)));
+        double b[5], a[5];
+        mcg_single_path(o->option_type, z, spot, strike, spot_up, spot_dn, spot_span, vol,
+                        maturity, drift, sig_sqrt_t, b);
+        if (antithetic) {
+            mcg_single_path(o->option_type, -z, spot, strike, spot_up, spot_dn, spot_span, vol,
+                            maturity, drift, sig_sqrt_t, a);
+            for (int k = 0; k < 5; ++k) b[k] = 0.5 * (b[k] + a[k]);
+        }
+        for (int k = 0; k < 5; ++k) sums[k] = sums[k] + b[k];
+    }
+    const double n = (double)samples;
+    for (int k = 0; k < 5; ++k) out[k] = discount * sums[k] / n;
+    return 0;
+}

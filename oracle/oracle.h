@@ -25,6 +25,9 @@
  *   or_geometric_asian_discrete        QuantLib 5.3425606635 fixture
  *   or_mc_european_pathwise_aad        BS Greeks at the reference tolerances
  *   or_heston_price_delta_aad          CRN central bump (reference test)
+ *   or_mc_greeks_estimate              BS delta/gamma/vega within 3 SE
+ *                                      (independent closed forms) and the
+ *                                      reference's QuantLib values
  */
 #ifndef OPENFERRIC_ORACLE_H
 #define OPENFERRIC_ORACLE_H
@@ -134,6 +137,13 @@ int or_mc_european_pathwise_aad(const or_option *o, int64_t num_paths,
 /* heston_price_delta_aad (mc_aad.rs:279-342). out: price, delta. */
 int or_heston_price_delta_aad(const or_option *o, int64_t num_paths,
                               int64_t num_steps, uint64_t seed, double out[2]);
+
+/* MonteCarloGreeksEngine::run_estimators (mc_greeks.rs:131-255), the
+ * sequential fold. out: pathwise delta, pathwise gamma, LR delta, LR gamma,
+ * LR vega (discounted means). o->div_yield is the market's effective yield.
+ * Returns 0 on success, 1 on invalid input. */
+int or_mc_greeks_estimate(const or_option *o, int64_t num_paths, uint64_t seed,
+                          int32_t antithetic, double spot_bump_rel, double out[5]);
 
 #ifdef __cplusplus
 }

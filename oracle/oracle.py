@@ -97,6 +97,8 @@ def lib():
         L.or_mc_european_pathwise_aad.restype = C.c_int
         L.or_heston_price_delta_aad.argtypes = [C.POINTER(OrOption), i64, i64, u64, pd]
         L.or_heston_price_delta_aad.restype = C.c_int
+        L.or_mc_greeks_estimate.argtypes = [C.POINTER(OrOption), i64, u64, i32, d, pd]
+        L.or_mc_greeks_estimate.restype = C.c_int
         _lib = L
     return _lib
 
@@ -213,3 +215,14 @@ def heston_price_delta_aad(o, num_paths, num_steps, seed):
     if rc:
         raise ValueError("heston aad failed")
     return float(out[0]), float(out[1])
+
+
+def mc_greeks_estimate(o, num_paths, seed, antithetic=True, spot_bump_rel=1e-2):
+    """MonteCarloGreeksEngine estimators: dict of pathwise delta/gamma and
+    likelihood-ratio delta/gamma/vega (mc_greeks.rs:131-243)."""
+    out = np.zeros(5)
+    rc = lib().or_mc_greeks_estimate(C.byref(o), num_paths, seed, 1 if antithetic else 0,
+                                     spot_bump_rel, _ptr(out))
+    if rc:
+        raise ValueError("invalid input")
+    return dict(zip(["pw_delta", "pw_gamma", "lr_delta", "lr_gamma", "lr_vega"], out.tolist()))

@@ -460,3 +460,64 @@ def test_bs_greeks_quantlib_atm_and_finite_differences(oracle):
               "theta": -(px(expiry=0.8 + h) - px(expiry=0.8 - h)) / (2 * h)}
         for k, v in fd.items():
             assert abs(g[k] - v) <= 1e-5 * max(1.0, abs(v)), (t, k, g[k], v)
+
+
+def test_mc_greeks_engine_reference_assertions(oracle):
+    """mc_greeks.rs:330-413: pathwise delta within 2 % and LR vega within 5 %
+    of the closed forms; the QuantLib reference values 0.6368, 0.01876,
+    37.524 (S=K=100, r=5 %, sigma=20 %, T=1, 100,000 antithetic paths, seed 42)."""
+    from scipy.stats import norm
+    o = oracle.make_option()
+    g = oracle.mc_greeks_estimate(o, 100_000, 42)
+    d1 = (math.log(100.0 / 100.0) + (0.05 + 0.5 * 0.04) * 1.0) / 0.2
+    assert abs(g["pw_delta"] - norm.cdf(d1)) / norm.cdf(d1) <= 0.02
+    vega = 100.0 * norm.pdf(d1)
+    assert abs(g["lr_vega"] - vega) / vega <= 0.05
+    assert abs(g["pw_delta"] - 0.6368) / 0.6368 < 0.02
+    assert abs(g["pw_gamma"] - 0.01876) / 0.01876 < 0.20
+    assert abs(g["lr_vega"] - 37.524) / 37.524 < 0.05
+
+
+@pytest.mark.parametrize("opt_type", ["call", "put"])
+def test_mc_greeks_engine_estimators_unbiased(oracle, opt_type):
+    """Each estimator against its exact expectation at 3 standard errors of
+    20 independent batches: pathwise and LR delta -> BS delta, LR gamma -> BS
+    gamma, LR vega -> BS vega, and the CRN difference of pathwise deltas ->
+    (Delta_BS(S+h) - Delta_BS(S-h)) / (2h) exactly (pathwise delta at a bumped
+    spot is unbiased for the BS delta there). A dropped term or wrong sign in
+    any score weight moves its mean by many standard errors."""
+    from scipy.stats import norm
+    S, K, r, q, vol, T, rel = 100.0, 105.0, 0.03, 0.01, 0.25, 0.75, 0.02
+    t = oracle.CALL if opt_type == "call" else oracle.PUT
+    o = oracle.make_option(option_type=t, strike=K, rate=r, div_yield=q, vol=vol, expiry=T)
+    batches = np.array([[*oracle.mc_greeks_estimate(o, 100_000, 1000 + b, True, rel).values()]
+                        for b in range(20)])
+    mean, se = batches.mean(axis=0), batches.std(axis=0, ddof=1) / np.sqrt(len(batches))
+
+    def bs_delta(s):
+        d1 = (math.log(s / K) + (r - q + 0.5 * vol * vol) * T) / (vol * math.sqrt(T))
+        return math.exp(-q * T) * (norm.cdf(d1) if t == oracle.CALL else norm.cdf(d1) - 1.0)
+
+    d1 = (math.log(S / K) + (r - q + 0.5 * vol * vol) * T) / (vol * math.sqrt(T))
+    gamma = math.exp(-q * T) * norm.pdf(d1) / (S * vol * math.sqrt(T))
+    vega = S * math.exp(-q * T) * norm.pdf(d1) * math.sqrt(T)
+    h = S * rel
+    expected = [bs_delta(S), (bs_delta(S + h) - bs_delta(S - h)) / (2 * h), bs_delta(S), gamma,
+                vega]
+    for k, e in enumerate(expected):
+        assert abs(mean[k] - e) <= 3.5 * se[k] + 1e-12, (k, mean[k], e, se[k])
+
+
+def test_mc_greeks_engine_edge_cases(oracle):
+    o = oracle.make_option(expiry=0.0)
+    assert list(oracle.mc_greeks_estimate(o, 1000, 1).values()) == [0.0] * 5
+    with pytest.raises(ValueError):
+        oracle.mc_greeks_estimate(oracle.make_option(), 0, 1)
+    with pytest.raises(ValueError):
+        oracle.mc_greeks_estimate(oracle.make_option(), 10, 1, True, 0.0)
+    with pytest.raises(ValueError):
+        oracle.mc_greeks_estimate(oracle.make_option(vol=0.0), 10, 1)
+    # non-antithetic uses ceil-free sample count and the plain estimator
+    a = oracle.mc_greeks_estimate(oracle.make_option(), 1, 7, False)
+    b = oracle.mc_greeks_estimate(oracle.make_option(), 2, 7, True)
+    assert a["lr_delta"] != b["lr_delta"]
