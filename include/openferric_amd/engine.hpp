@@ -144,6 +144,14 @@ struct BarrierOption {
 struct Market {
     double spot = 0, rate = 0, dividend_yield = 0, flat_vol = 0;
     double vol_for(double, double) const { return flat_vol; }
+    // market.rs:275-282 with no discrete dividends (dividends.rs:151-217):
+    // r - ln(F/S)/T, F = S exp((r - q) T); equal to q up to rounding.
+    double effective_dividend_yield(double maturity) const {
+        if (maturity <= 0.0 || spot <= 0.0) return dividend_yield;
+        const double forward = spot * std::exp((rate - dividend_yield) * maturity);
+        const double ratio = std::fmax(forward / spot, 1.0e-16);
+        return rate - std::log(ratio) / maturity;
+    }
 
     class Builder {
     public:
@@ -367,5 +375,159 @@ private:
         return detail::to_result(r, greeks, true);
     }
 };
+
+// MonteCarloGreeksEngine (mc_greeks.rs:20-129): pathwise delta, CRN
+// pathwise gamma and likelihood-ratio delta/gamma/vega of European GBM
+// vanillas, one exact-terminal normal per sample (ofg_mc_greeks_estimate).
+class MonteCarloGreeksEngine {
+public:
+    struct Estimates {
+        double pathwise_delta, pathwise_gamma, lr_delta, lr_gamma, lr_vega, num_samples;
+    };
+    MonteCarloGreeksEngine(int64_t num_paths, uint64_t seed) : paths_(num_paths), seed_(seed) {}
+    MonteCarloGreeksEngine &with_antithetic(bool a) { anti_ = a; return *this; }
+    MonteCarloGreeksEngine &with_spot_bump_rel(double rel) { bump_ = std::fmax(rel, 1.0e-6); return *this; }
+    MonteCarloGreeksEngine &with_seed(uint64_t seed) { seed_ = seed; return *this; }
+    MonteCarloGreeksEngine &with_rng_mode(RngMode m) { rng_ = m; return *this; }
+
+    Estimates estimates(const VanillaOption &i, const Market &m) const {
+        if (i.exercise != ExerciseStyle::European)
+            throw PricingError(PricingError::Kind::InvalidInput,
+                               "MonteCarloGreeksEngine supports European exercise only");
+        Market eff = m;
+        eff.dividend_yield = m.effective_dividend_yield(i.expiry);
+        ofg_option o = detail::to_ofg(i, eff, paths_, 1, seed_);
+        ofg_engine_cfg c{};
+        c.antithetic = anti_ ? 1 : 0;
+        c.rng_mode = (int32_t)rng_;
+        ofg_mc_greeks out{};
+        check(ofg_mc_greeks_estimate(device_context(), &o, 1, &c, bump_, &out));
+        return {out.pathwise_delta, out.pathwise_gamma, out.lr_delta, out.lr_gamma, out.lr_vega,
+                out.num_samples};
+    }
+    Greeks estimate_pathwise(const VanillaOption &i, const Market &m) const {
+        const Estimates e = estimates(i, m);
+        return Greeks{e.pathwise_delta, e.pathwise_gamma, 0.0, 0.0, 0.0};
+    }
+    Greeks estimate_likelihood_ratio(const VanillaOption &i, const Market &m) const {
+        const Estimates e = estimates(i, m);
+        return Greeks{e.lr_delta, e.lr_gamma, e.lr_vega, 0.0, 0.0};
+    }
+
+private:
+    int64_t paths_;
+    uint64_t seed_;
+    bool anti_ = true;
+    double bump_ = 1.0e-2;
+    RngMode rng_ = RngMode::Reference;
+};
+
+// mc_european_parallel / mc_european_sequential (mc_parallel.rs:264-434):
+// exact terminal sampling, one normal per path, fast draws; invalid inputs
+// give a NaN price (the reference returns, it does not fail).
+inline PricingResult mc_european_exact(const VanillaOption &i, const Market &m, int64_t n_paths) {
+    PricingResult r;
+    if (n_paths == 0 || i.exercise != ExerciseStyle::European) {
+        r.price = std::nan("");
+        return r;
+    }
+    if (i.expiry <= 0.0) {
+        r.price = i.option_type == OptionType::Call ? std::fmax(m.spot - i.strike, 0.0)
+                                                    : std::fmax(i.strike - m.spot, 0.0);
+        r.stderr_ = 0.0;
+        return r;
+    }
+    const double vol = m.vol_for(i.strike, i.expiry);
+    if (!(vol > 0.0) || !std::isfinite(vol)) {
+        r.price = std::nan("");
+        return r;
+    }
+    Market eff = m;
+    eff.dividend_yield = m.effective_dividend_yield(i.expiry);
+    ofg_option o = detail::to_ofg(i, eff, n_paths, 1, 0xDEADBEEFCAFEBABEull);
+    ofg_engine_cfg c{};
+    c.rng_mode = OFG_RNG_FAST;
+    ofg_result res{};
+    check(ofg_price(device_context(), &o, 1, &c, &res));
+    r.price = res.price;
+    r.stderr_ = res.stderr_;
+    r.diagnostics.insert("num_paths", (double)n_paths);
+    r.diagnostics.insert("num_steps", 1.0);
+    r.diagnostics.insert("vol", vol);
+    r.kernel_ms = res.kernel_ms;
+    return r;
+}
+inline PricingResult mc_european_parallel(const VanillaOption &i, const Market &m, int64_t n_paths,
+                                          int64_t /*n_steps*/ = 1) {
+    return mc_european_exact(i, m, n_paths);
+}
+inline PricingResult mc_european_sequential(const VanillaOption &i, const Market &m,
+                                            int64_t n_paths, int64_t /*n_steps*/ = 1) {
+    return mc_european_exact(i, m, n_paths);
+}
+
+// Direct MC helpers of pricing::asian (asian.rs:139-200) and pricing::barrier
+// (barrier.rs:199-239): GBM { mu: r, sigma }, antithetic, discount e^{-rt},
+// the Asian average over all steps + 1 path points, a zero payoff when the
+// barrier option is inactive. Return (price, stderr).
+struct AsianStrike {
+    bool floating = false;
+    double k = 0.0;
+    static AsianStrike Fixed(double k) { return AsianStrike{false, k}; }
+    static AsianStrike Floating() { return AsianStrike{true, 0.0}; }
+};
+
+namespace detail {
+inline std::pair<double, double> legacy_price(ofg_option o) {
+    ofg_engine_cfg c{};
+    c.antithetic = 1;
+    ofg_result r{};
+    check(ofg_price(device_context(), &o, 1, &c, &r));
+    return {r.price, r.stderr_};
+}
+inline std::pair<double, double> legacy_asian(OptionType t, AsianStrike strike, double s0, double r,
+                                              double sigma, double T, int64_t steps, int64_t paths,
+                                              uint64_t seed, bool geometric) {
+    const Market m{s0, r, 0.0, sigma};
+    ofg_option o = base(t, strike.floating ? s0 : strike.k, T, m, paths, steps, seed);
+    o.vol = sigma;
+    o.payoff = OFG_PAYOFF_ASIAN;
+    o.averaging = geometric ? OFG_GEOMETRIC : OFG_ARITHMETIC;
+    o.strike_type = strike.floating ? OFG_FLOATING : OFG_FIXED;
+    std::vector<double> obs;
+    for (int64_t j = 0; j <= steps; ++j) obs.push_back(T * ((double)j / (double)steps));
+    o.n_obs = (int32_t)obs.size();
+    o.obs_times = obs.data();
+    return legacy_price(o);
+}
+}  // namespace detail
+
+inline std::pair<double, double> arithmetic_asian_price_mc(OptionType t, AsianStrike strike,
+                                                           double s0, double r, double sigma,
+                                                           double T, int64_t steps,
+                                                           int64_t paths, uint64_t seed) {
+    return detail::legacy_asian(t, strike, s0, r, sigma, T, steps, paths, seed, false);
+}
+inline std::pair<double, double> geometric_asian_price_mc(OptionType t, AsianStrike strike,
+                                                          double s0, double r, double sigma,
+                                                          double T, int64_t steps, int64_t paths,
+                                                          uint64_t seed) {
+    return detail::legacy_asian(t, strike, s0, r, sigma, T, steps, paths, seed, true);
+}
+inline std::pair<double, double> barrier_price_mc(OptionType t, BarrierStyle style,
+                                                  BarrierDirection dir, double s0, double k,
+                                                  double barrier, double r, double sigma, double T,
+                                                  int64_t steps, int64_t num_paths,
+                                                  uint64_t seed) {
+    const Market m{s0, r, 0.0, sigma};
+    ofg_option o = detail::base(t, k, T, m, num_paths, steps, seed);
+    o.vol = sigma;
+    o.payoff = OFG_PAYOFF_BARRIER;
+    o.barrier_dir = dir == BarrierDirection::Up ? OFG_UP : OFG_DOWN;
+    o.barrier_style = style == BarrierStyle::In ? OFG_IN : OFG_OUT;
+    o.barrier_level = barrier;
+    o.rebate = 0.0;
+    return detail::legacy_price(o);
+}
 
 }  // namespace openferric_amd

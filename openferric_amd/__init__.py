@@ -13,6 +13,7 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       HestonAadConfig, HestonMonteCarloPricingEngine, RngMode, VarianceReduction,
                       MonteCarloGreeksEngine, autotune, engine_cfg, heston_price_delta_aad,
                       mc_european_parallel, mc_european_pathwise_aad, mc_european_sequential,
+                      mc_european_with_arena,
                       option_struct, price_raw, sample_payoffs)
 from ._lib import DeviceError
 
@@ -25,5 +26,5 @@ __all__ = [
     "HestonMonteCarloPricingEngine", "RngMode", "VarianceReduction", "autotune", "engine_cfg",
     "heston_price_delta_aad", "mc_european_pathwise_aad", "option_struct", "price_raw",
     "sample_payoffs", "DeviceError", "MonteCarloGreeksEngine", "mc_european_parallel",
-    "mc_european_sequential",
+    "mc_european_sequential", "mc_european_with_arena",
 ]

@@ -407,10 +407,12 @@ class MonteCarloGreeksEngine:
 _NAN_RESULT = float("nan")
 
 
-def _mc_european_exact(instrument: VanillaOption, market: Market, n_paths: int) -> PricingResult:
-    """mc_european_parallel / mc_european_sequential (mc_parallel.rs:264-434):
-    the terminal spot sampled exactly (one step over T, one normal per path,
-    no variance reduction); invalid inputs give a NaN price, not an error."""
+def _mc_european_exact(instrument: VanillaOption, market: Market, n_paths: int,
+                       seed: int = 0xDEADBEEFCAFEBABE, effective_q: bool = True) -> PricingResult:
+    """mc_european_parallel / mc_european_sequential (mc_parallel.rs:264-434)
+    and mc_european_with_arena (mc_engine.rs:186-343): the terminal spot
+    sampled exactly (one step over T, one normal per path, no variance
+    reduction); invalid inputs give a NaN price, not an error."""
     if n_paths == 0 or instrument.exercise is not ExerciseStyle.European:
         return PricingResult(_NAN_RESULT, None)
     if instrument.expiry <= 0.0:
@@ -421,9 +423,9 @@ def _mc_european_exact(instrument: VanillaOption, market: Market, n_paths: int) 
     if not (vol > 0.0) or vol == float("inf"):
         return PricingResult(_NAN_RESULT, None)
     keep = _Keep()
-    eff = Market(market.spot, market.rate, market.effective_dividend_yield(instrument.expiry),
-                 market.flat_vol)
-    o = option_struct(instrument, eff, n_paths, 1, 0xDEADBEEFCAFEBABE, keep)
+    q = market.effective_dividend_yield(instrument.expiry) if effective_q else market.dividend_yield
+    eff = Market(market.spot, market.rate, q, market.flat_vol)
+    o = option_struct(instrument, eff, n_paths, 1, seed, keep)
     r = price_raw([o], engine_cfg(rng_mode=RngMode.Fast, keep=keep))[0]
     diag = Diagnostics()
     diag.insert("num_paths", float(n_paths))
@@ -445,6 +447,21 @@ def mc_european_sequential(instrument: VanillaOption, market: Market, n_paths: i
                            n_steps: int = 1) -> PricingResult:
     """GPU counterpart of mc_european_sequential (mc_parallel.rs:365-434)."""
     return _mc_european_exact(instrument, market, n_paths)
+
+
+def mc_european_with_arena(instrument: VanillaOption, market: Market, n_paths: int,
+                           n_steps: int = 1, arena=None) -> PricingResult:
+    """GPU counterpart of mc_european_with_arena (mc_engine.rs:186-343):
+    exact terminal sampling with mu = r - q and seed 42. The arena (a host
+    payoff buffer in the reference) is accepted and unused: payoffs are
+    reduced on the device. An instrument that fails validation gives NaN."""
+    try:
+        instrument.validate()
+    except InvalidInput:
+        return PricingResult(_NAN_RESULT, None)
+    if n_paths == 0:
+        return PricingResult(_NAN_RESULT, None)
+    return _mc_european_exact(instrument, market, n_paths, seed=42, effective_q=False)
 
 
 class BatchPricer:

@@ -129,6 +129,54 @@ int main() {
                std::to_string(aad.greeks->delta) + " vs " + std::to_string(bump));
         expect(aad.heston_gradient && aad.heston_gradient->size() == 6, "heston_gradient_present");
     }
+    {  // mc_greeks.rs:330-413: pathwise delta (2 %), LR vega (5 %), QuantLib values
+        auto e = MonteCarloGreeksEngine(100000, 42);
+        auto pw = e.estimate_pathwise(call, m);
+        auto lr = e.estimate_likelihood_ratio(call, m);
+        const double d1 = (std::log(1.0) + (0.05 + 0.5 * 0.04) * 1.0) / 0.2;
+        auto N = [](double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); };
+        const double vega = 100.0 * std::exp(-0.5 * d1 * d1) / std::sqrt(2 * M_PI);
+        expect(std::fabs((pw.delta - N(d1)) / N(d1)) <= 0.02,
+               "pathwise_delta_converges_to_closed_form_call_delta", std::to_string(pw.delta));
+        expect(std::fabs((lr.vega - vega) / vega) <= 0.05,
+               "likelihood_ratio_vega_converges_to_closed_form_call_vega", std::to_string(lr.vega));
+        expect(std::fabs(pw.delta - 0.6368) / 0.6368 < 0.02 &&
+                   std::fabs(pw.gamma - 0.01876) / 0.01876 < 0.20 &&
+                   std::fabs(lr.vega - 37.524) / 37.524 < 0.05,
+               "quantlib_reference_values_are_reasonable");
+    }
+    {  // mc_parallel.rs: exact-terminal European against Black-Scholes
+        auto r = mc_european_parallel(call, m, 2000000, 252);
+        expect(std::fabs(r.price - BS_CALL) <= 3.0 * *r.stderr_ &&
+                   r.diagnostics.get("num_steps").value_or(0) == 1.0,
+               "mc_european_parallel_within_3se_of_bs", std::to_string(r.price));
+        expect(std::isnan(mc_european_sequential(call, m, 0).price),
+               "mc_european_sequential_zero_paths_is_nan");
+    }
+    {  // pricing/asian.rs:207-297 and pricing/barrier.rs:313-355
+        auto [mc, se] = geometric_asian_price_mc(OptionType::Call, AsianStrike::Fixed(100.0), 100.0,
+                                                 0.05, 0.2, 1.0, 252, 80000, 44);
+        const double mm = std::log(100.0) + (0.05 - 0.02) * 1.0 / 2.0, v = 0.04 / 3.0;
+        const double dd1 = (mm - std::log(100.0) + v) / std::sqrt(v);
+        auto N = [](double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); };
+        const double closed =
+            std::exp(-0.05) * (std::exp(mm + 0.5 * v) * N(dd1) - 100.0 * N(dd1 - std::sqrt(v)));
+        expect(std::fabs(mc - closed) <= 2.0 * se + 2e-2, "geometric_asian_mc_converges_to_closed_form",
+               std::to_string(mc) + " vs " + std::to_string(closed));
+        auto a = arithmetic_asian_price_mc(OptionType::Call, AsianStrike::Fixed(100.0), 100.0, 0.02,
+                                           0.2, 1.0, 128, 30000, 11);
+        auto g = geometric_asian_price_mc(OptionType::Call, AsianStrike::Fixed(100.0), 100.0, 0.02,
+                                          0.2, 1.0, 128, 30000, 11);
+        expect(a.first >= g.first - 0.05, "arithmetic_call_is_above_geometric_call_for_fixed_strike");
+        auto out = barrier_price_mc(OptionType::Call, BarrierStyle::Out, BarrierDirection::Up, 100.0,
+                                    100.0, 110.0, 0.01, 0.2, 1.0, 252, 30000, 10);
+        auto inn = barrier_price_mc(OptionType::Call, BarrierStyle::In, BarrierDirection::Up, 100.0,
+                                    100.0, 110.0, 0.01, 0.2, 1.0, 252, 30000, 10);
+        const double bd1 = (0.01 + 0.02) / 0.2;
+        const double vanilla = 100.0 * N(bd1) - 100.0 * std::exp(-0.01) * N(bd1 - 0.2);
+        expect(std::fabs(out.first + inn.first - vanilla) <= 2.5 * (out.second + inn.second),
+               "barrier_mc_parity_matches_vanilla");
+    }
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
 }

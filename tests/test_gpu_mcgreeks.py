@@ -4,7 +4,8 @@ helpers (C ABI calls):
 * MonteCarloGreeksEngine (mc_greeks.rs): pathwise and likelihood-ratio Greeks
   vs the oracle on the reference draws and external normals, and the fast
   mode against the Black-Scholes closed forms;
-* mc_european_parallel / _sequential (mc_parallel.rs:264-434);
+* mc_european_parallel / _sequential (mc_parallel.rs:264-434) and
+  mc_european_with_arena (mc_engine.rs:186-343);
 * arithmetic/geometric Asian and barrier helpers (pricing/asian.rs:139-200,
   pricing/barrier.rs:199-239) vs the oracle's engine on the same options.
 """
@@ -134,19 +135,19 @@ def test_mc_greeks_engine_mirror_and_reference_values(gpu):
 def test_mc_european_parallel_and_sequential(gpu):
     """Exact-terminal European: within 3 SE of Black-Scholes; NaN for zero
     paths, intrinsic value at zero expiry, num_steps diagnostic 1."""
+    from scipy.stats import norm
     import openferric_amd as ofa
-    from tests.cases import make_pair  # noqa: F401
     m = ofa.Market.builder().spot(100.0).rate(0.05).dividend_yield(0.02).flat_vol(0.25).build()
-    for opt, bs in [(ofa.VanillaOption.european_call(105.0, 1.5), None),
-                    (ofa.VanillaOption.european_put(95.0, 0.75), None)]:
-        from scipy.stats import norm
+    for opt in [ofa.VanillaOption.european_call(105.0, 1.5),
+                ofa.VanillaOption.european_put(95.0, 0.75)]:
         S, K, r, q, v, T = 100.0, opt.strike, 0.05, 0.02, 0.25, opt.expiry
         d1 = (math.log(S / K) + (r - q + 0.5 * v * v) * T) / (v * math.sqrt(T))
         d2 = d1 - v * math.sqrt(T)
         call = opt.option_type is ofa.OptionType.Call
         bs = (S * math.exp(-q * T) * norm.cdf(d1) - K * math.exp(-r * T) * norm.cdf(d2)) if call \
             else (K * math.exp(-r * T) * norm.cdf(-d2) - S * math.exp(-q * T) * norm.cdf(-d1))
-        for fn in (ofa.mc_european_parallel, ofa.mc_european_sequential):
+        for fn in (ofa.mc_european_parallel, ofa.mc_european_sequential,
+                   ofa.mc_european_with_arena):
             res = fn(opt, m, 2_000_000, 252)
             assert abs(res.price - bs) <= 3 * res.stderr, (res.price, bs, res.stderr)
             assert res.diagnostics.get("num_steps") == 1.0
@@ -155,6 +156,12 @@ def test_mc_european_parallel_and_sequential(gpu):
                                                 0, 1).price)
     r0 = ofa.mc_european_sequential(ofa.VanillaOption.european_put(110.0, 0.0), m, 10, 1)
     assert r0.price == 10.0 and r0.stderr == 0.0
+    # mc_engine.rs:1170-1194: 200,000 paths within 1 % of BS; invalid -> NaN
+    m0 = ofa.Market.builder().spot(100.0).rate(0.05).dividend_yield(0.0).flat_vol(0.2).build()
+    ra = ofa.mc_european_with_arena(ofa.VanillaOption.european_call(100.0, 1.0), m0, 200_000, 1)
+    assert abs(ra.price - 10.450583572185565) / 10.450583572185565 <= 0.01
+    assert math.isnan(ofa.mc_european_with_arena(ofa.VanillaOption.european_call(-1.0, 1.0), m0,
+                                                 100, 1).price)
 
 
 def _legacy_oracle(oracle, kind, option_type, s0, k, r, sigma, t, steps, paths, seed, **kw):
