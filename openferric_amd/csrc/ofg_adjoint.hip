@@ -158,10 +158,10 @@ __device__ __forceinline__ double leg_payoff_seeds(const Leg &g, const OptDev &o
                                                    double &ca, double &cg) {
     const bool put = o.flags & F_PUT;
     ct = ca = cg = 0.0;
-    if constexpr (PK == PK_VANILLA || PK == PK_BARRIER) {
+    if constexpr (PK == PK_VANILLA || is_barrier(PK)) {
         const double x = vanilla_payoff(put, g.s, o.strike);
         const double d = put ? (o.strike - g.s > 0.0 ? -1.0 : 0.0) : (g.s - o.strike > 0.0 ? 1.0 : 0.0);
-        if constexpr (PK == PK_BARRIER) {
+        if constexpr (is_barrier(PK)) {
             const bool active = (o.flags & F_KNOCK_IN) ? g.hit : !g.hit;
             ct = active ? d : 0.0;
             return active ? x : o.rebate;
@@ -200,7 +200,6 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
     constexpr int K = adj_segment(MODEL);
     const int steps = o.steps;
     const int nseg = (steps + K - 1) / K;
-    const double bsign = (o.flags & F_DOWN) ? -1.0 : 1.0;
     constexpr bool ASIAN = PK == PK_ASIAN || PK == PK_ASIAN_LOG;
     constexpr int NV = MODEL == OFG_MODEL_HESTON ? 2 : 1;
 
@@ -216,7 +215,7 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
 
     // ---- forward sweep -------------------------------------------------
     Leg g;
-    leg_init<PK>(g, o, ASIAN ? w[0] : 0.0, bsign);
+    leg_init<PK>(g, o, ASIAN ? w[0] : 0.0);
     double v = o.v0;
     double z1[K], z2[K];
     for (int c = 0; c < nseg; ++c) {
@@ -231,7 +230,7 @@ __device__ double adjoint_leg(const OptDev &o, int64_t sample, double sign,
             if (k < len) {
                 if constexpr (MODEL == OFG_MODEL_GBM) gbm_step<PK, RNG == OFG_RNG_FAST>(g, o, z1[k], tab);
                 else heston_step<PK, RNG != OFG_RNG_FAST>(g, v, o, z1[k], z2[k], tab);
-                leg_observe<PK>(g, ASIAN ? w[j0 + k + 1] : 0.0, bsign, o.barrier);
+                leg_observe<PK>(g, ASIAN ? w[j0 + k + 1] : 0.0, o.barrier);
             }
         }
     }
@@ -388,7 +387,8 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
         case PK_VANILLA: adjoint_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
         case PK_ASIAN: adjoint_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
         case PK_ASIAN_LOG: adjoint_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
-        default: adjoint_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
+        case PK_BARRIER: adjoint_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
+        default: adjoint_item<MODEL, PK_BARRIER_DOWN, RNG, ANTI>(o, wi, lane, a, ck, tab, spv, spv2, acc); break;
         }
         AdjPartial p;
         p.spv = wave_sum(spv);

@@ -388,7 +388,7 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
         if (o.payoff == OFG_PAYOFF_VANILLA) {
             d.kind = ofg::PK_VANILLA;
         } else if (o.payoff == OFG_PAYOFF_BARRIER) {
-            d.kind = ofg::PK_BARRIER;
+            d.kind = o.barrier_dir == OFG_DOWN ? ofg::PK_BARRIER_DOWN : ofg::PK_BARRIER;
             if (o.barrier_dir == OFG_DOWN) flags |= ofg::F_DOWN;
             if (o.barrier_style == OFG_IN) flags |= ofg::F_KNOCK_IN;
         } else {

@@ -14,8 +14,13 @@ enum PayoffKind : int32_t {
     PK_VANILLA = 0,    // terminal spot only
     PK_ASIAN = 1,      // weighted running sum of observed spots
     PK_ASIAN_LOG = 2,  // + running sum of observed log-spots (geometric)
-    PK_BARRIER = 3     // running hit flag (inclusive, path[0] included)
+    PK_BARRIER = 3,    // running hit flag (inclusive, path[0] included), up
+    PK_BARRIER_DOWN = 4  // the same, down barrier
 };
+
+__host__ __device__ constexpr bool is_barrier(int pk) {
+    return pk == PK_BARRIER || pk == PK_BARRIER_DOWN;
+}
 
 enum OptFlags : int32_t {
     F_PUT = 1,            // option type put (else call)

@@ -47,12 +47,11 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
                                                 const double *__restrict__ w,
                                                 const LaunchArgs &a, const double *tab, double &x,
                                                 double &y) {
-    const double bsign = (o.flags & F_DOWN) ? -1.0 : 1.0;
     const double barrier = o.barrier;
     const int steps = o.steps;
     Leg ga, gb;
-    leg_init<PK>(ga, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0, bsign);
-    if constexpr (ANTI) leg_init<PK>(gb, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0, bsign);
+    leg_init<PK>(ga, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0);
+    if constexpr (ANTI) leg_init<PK>(gb, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0);
 
     // Reference draws use the reference's xoshiro256++; the fast mode uses
     // xoshiro256+ with Box-Muller (four normals per three 64-bit draws).
@@ -74,8 +73,8 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
             gbm_step<PK, RNG == OFG_RNG_FAST>(ga, o, z, tab);
             if constexpr (ANTI) gbm_step<PK, RNG == OFG_RNG_FAST>(gb, o, -z, tab);
             const double wj = ASIAN ? w[j + 1] : 0.0;
-            leg_observe<PK>(ga, wj, bsign, barrier);
-            if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
+            leg_observe<PK>(ga, wj, barrier);
+            if constexpr (ANTI) leg_observe<PK>(gb, wj, barrier);
         };
         if constexpr (RNG == OFG_RNG_FAST) {
             int j = 0;
@@ -108,8 +107,8 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
             heston_step<PK, RNG != OFG_RNG_FAST>(ga, va, o, z1, z2, tab);
             if constexpr (ANTI) heston_step<PK, RNG != OFG_RNG_FAST>(gb, vb, o, -z1, -z2, tab);
             const double wj = ASIAN ? w[j + 1] : 0.0;
-            leg_observe<PK>(ga, wj, bsign, barrier);
-            if constexpr (ANTI) leg_observe<PK>(gb, wj, bsign, barrier);
+            leg_observe<PK>(ga, wj, barrier);
+            if constexpr (ANTI) leg_observe<PK>(gb, wj, barrier);
         };
         if constexpr (RNG == OFG_RNG_FAST) {
             int j = 0;
@@ -179,7 +178,8 @@ __device__ __forceinline__ void dispatch_kind(const OptDev &o, const WorkItem &w
     case PK_VANILLA: run_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
     case PK_ASIAN: run_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
     case PK_ASIAN_LOG: run_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
-    default: run_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
+    case PK_BARRIER: run_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
+    default: run_item<MODEL, PK_BARRIER_DOWN, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
     }
 }
 

@@ -24,22 +24,24 @@ struct Leg {
 };
 
 template <int PK>
-__device__ __forceinline__ void leg_init(Leg &g, const OptDev &o, double w0, double bsign) {
+__device__ __forceinline__ void leg_init(Leg &g, const OptDev &o, double w0) {
     g.s = o.s0;
     if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) g.sum = w0 * o.s0;
     if constexpr (PK == PK_ASIAN_LOG) {
         g.ls = o.log_s0;
         g.lsum = w0 * o.log_s0;
     }
-    if constexpr (PK == PK_BARRIER) g.hit = bsign * o.s0 >= bsign * o.barrier;
+    if constexpr (PK == PK_BARRIER) g.hit = o.s0 >= o.barrier;
+    if constexpr (PK == PK_BARRIER_DOWN) g.hit = o.s0 <= o.barrier;
 }
 
 // Observation bookkeeping after the spot of path index j+1 is known.
 template <int PK>
-__device__ __forceinline__ void leg_observe(Leg &g, double w, double bsign, double barrier) {
+__device__ __forceinline__ void leg_observe(Leg &g, double w, double barrier) {
     if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) g.sum = fma(w, g.s, g.sum);
     if constexpr (PK == PK_ASIAN_LOG) g.lsum = fma(w, g.ls, g.lsum);
-    if constexpr (PK == PK_BARRIER) g.hit = g.hit || (bsign * g.s >= bsign * barrier);
+    if constexpr (PK == PK_BARRIER) g.hit = g.hit || (g.s >= barrier);
+    if constexpr (PK == PK_BARRIER_DOWN) g.hit = g.hit || (g.s <= barrier);
 }
 
 // Growth factor exp(x) of one GBM step (simulation.rs:67-70), x = diffusion z
@@ -98,7 +100,7 @@ __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o
         growth = exp2_256(y, tab);
     }
     const double sn = s * growth;
-    s_next = !(sn >= 1e-12) ? 1e-12 : sn;   // f64::max(sn, 1e-12), NaN -> 1e-12
+    s_next = fmax(sn, 1e-12);   // f64::max(sn, 1e-12): maxNum, NaN -> 1e-12
     v_next = fmax(vn, 0.0);
 }
 
@@ -120,7 +122,7 @@ __device__ __forceinline__ void leg_payoff(const Leg &g, const OptDev &o, double
     const bool put = o.flags & F_PUT;
     if constexpr (PK == PK_VANILLA) {
         x = vanilla_payoff(put, g.s, o.strike);
-    } else if constexpr (PK == PK_BARRIER) {
+    } else if constexpr (is_barrier(PK)) {
         const bool active = (o.flags & F_KNOCK_IN) ? g.hit : !g.hit;
         x = active ? vanilla_payoff(put, g.s, o.strike) : o.rebate;
     } else {
