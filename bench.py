@@ -161,13 +161,19 @@ def greeks_leg(rng_mode):
     adj = price_raw(structs, base, greeks=True)
     fd = price_raw(structs, engine_cfg(rng_mode=rng_mode, greek_mode=_lib.GREEKS_FD_CRN,
                                        bump_rel=1e-4))
+    fwd_cfg = engine_cfg(rng_mode=rng_mode, greek_mode=_lib.GREEKS_FORWARD)
+    price_raw(structs, fwd_cfg)
+    fwd = price_raw(structs, fwd_cfg)
     t_price, t_adj, t_fd = p[0].kernel_ms, adj[0].kernel_ms, fd[0].kernel_ms
+    t_fwd = fwd[0].kernel_ms
     n_greeks = 4 * len(opts)
     rel = []
     for a, f in zip(adj, fd):
         for k in ("delta", "vega", "rho", "theta"):
             fa, ff = getattr(a, k), getattr(f, k)
             rel.append(abs(fa - ff) / max(abs(ff), 1e-12))
+    rel_fwd = [abs(getattr(a, k) - getattr(f, k)) / max(abs(getattr(a, k)), 1e-12)
+               for a, f in zip(adj, fwd) for k in ("delta", "vega", "rho", "theta")]
     ps = sum(d["num_paths"] * d["num_steps"] for d in opts)
     return {"workload": "64 GBM arithmetic-Asian calls (daily obs), 2^20 paths x 252 steps, "
                         "K U[80,120], sigma U[0.15,0.45]",
@@ -179,7 +185,12 @@ def greeks_leg(rng_mode):
             "adjoint_speedup_vs_bump": t_fd / t_adj,
             "pricing_path_steps_per_s": ps / (t_price * 1e-3),
             "adjoint_vs_bump_max_rel_diff": max(rel), "adjoint_vs_bump_median_rel_diff":
-                float(np.median(rel))}
+                float(np.median(rel)),
+            # the same pathwise estimator in one forward sweep (GBM running sums)
+            "forward_ms": t_fwd, "forward_cost_in_pricing_passes": t_fwd / t_price,
+            "forward_greeks_per_s": n_greeks / (t_fwd * 1e-3),
+            "forward_speedup_vs_bump": t_fd / t_fwd,
+            "forward_vs_adjoint_max_rel_diff": max(rel_fwd)}
 
 
 def heston_leg(rng_mode):

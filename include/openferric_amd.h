@@ -48,12 +48,25 @@ enum ofg_rng_mode {
     OFG_RNG_REFERENCE = 0,
     /* Caller-supplied normals, step-major [step][sample] per option. */
     OFG_RNG_EXTERNAL = 1,
-    /* Throughput mode: the same per-sample xoshiro256++ streams feeding a
-       branch-free Box-Muller transform. Statistically equivalent draws. */
+    /* Throughput mode: per-sample xoshiro256+ streams (same SplitMix64
+       seeding) feeding a branch-free, table-driven Box-Muller transform; path
+       exponentials as 2^(y/256). Statistically equivalent draws. */
     OFG_RNG_FAST = 2
 };
 
-enum ofg_greek_mode { OFG_GREEKS_NONE = 0, OFG_GREEKS_ADJOINT = 1, OFG_GREEKS_FD_CRN = 2 };
+/* How ofg_price fills the Greeks.
+   ADJOINT  pathwise reverse sweep (checkpointed adjoint kernel), GBM and Heston;
+   FD_CRN   central bump-and-revalue with common random numbers, all bumps in
+            one batched launch (greeks/bsm.rs:130-212);
+   FORWARD  the same pathwise estimator as ADJOINT evaluated in one forward
+            sweep: GBM options by running sums of the path (no reverse pass);
+            Heston options use the adjoint kernel. */
+enum ofg_greek_mode {
+    OFG_GREEKS_NONE = 0,
+    OFG_GREEKS_ADJOINT = 1,
+    OFG_GREEKS_FD_CRN = 2,
+    OFG_GREEKS_FORWARD = 3
+};
 
 /* One priced contract: instrument + market + model + path/step counts.
  * Mirrors (VanillaOption | AsianOption | BarrierOption, Market, Gbm | Heston)

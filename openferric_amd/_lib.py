@@ -27,7 +27,7 @@ FIXED, FLOATING = 0, 1
 UP, DOWN = 0, 1
 IN, OUT = 0, 1
 RNG_REFERENCE, RNG_EXTERNAL, RNG_FAST = 0, 1, 2
-GREEKS_NONE, GREEKS_ADJOINT, GREEKS_FD_CRN = 0, 1, 2
+GREEKS_NONE, GREEKS_ADJOINT, GREEKS_FD_CRN, GREEKS_FORWARD = 0, 1, 2, 3
 
 EXPORTED_SYMBOLS = [
     "ofg_create", "ofg_destroy", "ofg_last_error", "ofg_version", "ofg_price",

@@ -12,6 +12,7 @@
 #include "ofg_math.hpp"
 #include "ofg_adjoint.hpp"
 #include "ofg_mcgreeks.hpp"
+#include "ofg_pathwise.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -378,6 +379,9 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
                 idx = std::min<int64_t>(idx, len - 1);
                 w[(size_t)idx] += 1.0;
             }
+            double jsum = 0.0;
+            for (int64_t j = 0; j < len; ++j) jsum += (double)j * w[(size_t)j];
+            d.obs_jsum = jsum;
             p.obs_w.insert(p.obs_w.end(), w.begin(), w.end());
         } else {
             p.obs_w.push_back(0.0);  // keep offsets valid
@@ -847,6 +851,7 @@ int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     ofg_engine_cfg c = *cfg;
     c.control_variate = 0;  // mc_aad.rs: plain estimator of price and Greeks
+    const bool forward = cfg->greek_mode == OFG_GREEKS_FORWARD;
     c.greek_mode = OFG_GREEKS_NONE;
     Tuning t;
     t.block = cfg->tune_block > 0 ? std::min(cfg->tune_block, 256) : 256;
@@ -911,9 +916,13 @@ int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
             a.antithetic = p.antithetic;
             a.block = t.block;
             const int wpb = t.block / 64;
-            const int full = ofg::adjoint_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device);
+            const bool fwd = forward && m == OFG_MODEL_GBM;  // no checkpoints
+            const int full = fwd ? ofg::pathwise_grid(t.block, c.rng_mode, p.antithetic, ctx->device)
+                                 : ofg::adjoint_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device);
             a.grid = std::max(1, (int)std::min<int64_t>(full, ((int64_t)cnt + wpb - 1) / wpb));
-            scratch_doubles = std::max(scratch_doubles, (size_t)a.grid * wpb * (size_t)a.wave_stride);
+            if (!fwd)
+                scratch_doubles =
+                    std::max(scratch_doubles, (size_t)a.grid * wpb * (size_t)a.wave_stride);
         }
         ctx->adj.ensure(scratch_doubles * sizeof(double));
         hip_check(ofg::launch_reset_queue(ctx->head.as<uint32_t>(), 2, s), "reset queue");
@@ -921,7 +930,10 @@ int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
         for (int m = 0; m < 2; ++m) {
             if (p.model_end[m] - p.model_begin[m] <= 0) continue;
             la[m].scratch = ctx->adj.as<double>();
-            hip_check(ofg::launch_adjoint(la[m], s), "adjoint kernel launch");
+            if (forward && m == OFG_MODEL_GBM)
+                hip_check(ofg::launch_pathwise(la[m], s), "pathwise kernel launch");
+            else
+                hip_check(ofg::launch_adjoint(la[m], s), "adjoint kernel launch");
         }
         hip_check(hipEventRecord(ctx->ev1, s), "event");
         hip_check(ofg::launch_adjoint_reduce(ctx->opts.as<ofg::OptDev>(), ctx->item_first.as<int64_t>(),
@@ -979,7 +991,8 @@ int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_eng
     if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
     if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
     if (cfg->greek_mode == OFG_GREEKS_FD_CRN) return price_fd(ctx, opts, n, cfg, out);
-    if (cfg->greek_mode == OFG_GREEKS_ADJOINT) return adjoint_impl(ctx, opts, n, cfg, out);
+    if (cfg->greek_mode == OFG_GREEKS_ADJOINT || cfg->greek_mode == OFG_GREEKS_FORWARD)
+        return adjoint_impl(ctx, opts, n, cfg, out);
     std::lock_guard<std::mutex> lock(ctx->mu);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     // First pass with default tuning only to classify the workload.

@@ -55,6 +55,8 @@ struct alignas(16) OptDev {
     double drift2, diffusion2, sqrt_dt2, mudt2, hdt2;
     // Monte Carlo Greeks estimators: the bumped spots of the pathwise gamma
     double spot_up, spot_dn;
+    // sum_j j w_j of the observation weights (forward pathwise Greeks)
+    double obs_jsum;
     int32_t antithetic;
     int32_t has_cv;
 };

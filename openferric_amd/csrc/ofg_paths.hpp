@@ -146,6 +146,46 @@ __device__ __forceinline__ void leg_payoff(const Leg &g, const OptDev &o, double
     }
 }
 
+// Payoff value and derivative seeds of one leg:
+//   dx/dS_n = ct, and per observation j: dx/dS_j += ca w_j + cg w_j / S_j.
+template <int PK>
+__device__ __forceinline__ double leg_payoff_seeds(const Leg &g, const OptDev &o, double &ct,
+                                                   double &ca, double &cg) {
+    const bool put = o.flags & F_PUT;
+    ct = ca = cg = 0.0;
+    if constexpr (PK == PK_VANILLA || is_barrier(PK)) {
+        const double x = vanilla_payoff(put, g.s, o.strike);
+        const double d = put ? (o.strike - g.s > 0.0 ? -1.0 : 0.0) : (g.s - o.strike > 0.0 ? 1.0 : 0.0);
+        if constexpr (is_barrier(PK)) {
+            const bool active = (o.flags & F_KNOCK_IN) ? g.hit : !g.hit;
+            ct = active ? d : 0.0;
+            return active ? x : o.rebate;
+        } else {
+            ct = d;
+            return x;
+        }
+    } else {
+        bool geo = false;
+        if constexpr (PK == PK_ASIAN_LOG) geo = (o.flags & F_GEOMETRIC) != 0;
+        const double avg = geo ? exp_fp64(g.lsum / o.nobs) : g.sum / o.nobs;
+        double davg;
+        double x;
+        if (o.flags & F_FLOATING) {
+            x = vanilla_payoff(put, g.s, avg);
+            const bool on = put ? (avg - g.s > 0.0) : (g.s - avg > 0.0);
+            ct = on ? (put ? -1.0 : 1.0) : 0.0;
+            davg = on ? (put ? 1.0 : -1.0) : 0.0;
+        } else {
+            x = vanilla_payoff(put, avg, o.strike);
+            const bool on = put ? (o.strike - avg > 0.0) : (avg - o.strike > 0.0);
+            davg = on ? (put ? -1.0 : 1.0) : 0.0;
+        }
+        if (geo) cg = davg * avg / o.nobs;
+        else ca = davg / o.nobs;
+        return x;
+    }
+}
+
 // ------------------------------------------------------------------------
 // Normal sources
 // ------------------------------------------------------------------------

@@ -1,0 +1,265 @@
+// Forward pathwise Greeks of GBM paths for MI355X (gfx950), FP64: the same
+// pathwise estimator as the adjoint kernel (positive-part derivative
+// 1{x > 0}, zero derivative of barrier indicators, observation indices fixed
+// relative to T, d/dT through dt = T/n and the discount), evaluated in ONE
+// forward sweep with no checkpoints or reverse pass.
+//
+// Under GBM, ln S_j = ln S0 + j drift + diffusion Z_j with Z_j = z_1 + .. + z_j,
+// so d ln S_j / d theta = a + b j + c Z_j with per-parameter constants:
+//   S0: a = 1/S0;  sigma: b = -sigma dt, c = sqrt(dt);  r: b = dt;
+//   T:  b = drift / T, c = diffusion / (2T).
+// With the payoff seeds (d x/d S_n = ct, d x/d S_j += ca w_j + cg w_j / S_j)
+//   d x / d theta = ct S_n L_n + ca sum_j w_j S_j L_j + cg sum_j w_j L_j,
+// and the weighted sums follow from running sums by summation by parts
+// (R_k = sum_{i<=k} w_i S_i, the running arithmetic sum):
+//   sum_j j w_j S_j   = n R_n - sum_{k<n} R_k,
+//   sum_j w_j S_j Z_j = R_n Z_n - sum_i z_i R_{i-1},
+// so a vanilla or barrier path carries one extra add per step (Z) and an
+// arithmetic Asian path three (Z, sum R, sum z R). The draws and path
+// operations are the pricing kernel's, so prices equal the pricing pass.
+#include "ofg_adjoint.hpp"
+#include "ofg_internal.hpp"
+#include "ofg_math.hpp"
+#include "ofg_pathwise.hpp"
+#include "ofg_paths.hpp"
+
+#include <type_traits>
+
+namespace ofg {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+
+// Running state of one leg beyond the path state: sum of the running
+// weighted sums (Q) and of z times the running weighted sum before the step
+// (P); both zero for payoffs without an arithmetic average.
+struct Acc2 {
+    double q, p;
+};
+
+// Gradient of one leg's pv from its final state. Z is this leg's normal sum,
+// G2 = sum_j w_j Z_j (geometric averages only). Returns pv.
+template <int PK>
+__device__ __forceinline__ double leg_gradient(const Leg &g, const OptDev &o, double Z,
+                                               const Acc2 &s, double G2, double (&gr)[ADJ_NG]) {
+    double ct, ca, cg;
+    const double x = leg_payoff_seeds<PK>(g, o, ct, ca, cg);
+    const double pv = o.df * x;
+    const double n = (double)o.steps;
+    const double T = o.dt * n;
+    const double tS = ct * g.s;
+    double ta = tS, tb = tS * n, tc = tS * Z;
+    if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) {
+        const double A0 = g.sum;
+        const double A1 = fma(n, A0, -s.q);
+        const double A2 = fma(A0, Z, -s.p);
+        ta = fma(ca, A0, ta);
+        tb = fma(ca, A1, tb);
+        tc = fma(ca, A2, tc);
+    }
+    if constexpr (PK == PK_ASIAN_LOG) {
+        ta = fma(cg, o.nobs, ta);
+        tb = fma(cg, o.obs_jsum, tb);
+        tc = fma(cg, G2, tc);
+    }
+    gr[0] += o.df * ta / o.s0;
+    gr[1] += o.df * fma(-o.sigma * o.dt, tb, o.sqrt_dt * tc);
+    gr[2] += fma(o.df * o.dt, tb, -T * pv);
+    gr[3] += fma(o.df, fma(o.drift, tb, 0.5 * o.diffusion * tc) / T, -o.rate * pv);
+    return pv;
+}
+
+template <int PK, int RNG, bool ANTI>
+__device__ __forceinline__ double pathwise_sample(const OptDev &o, int64_t sample,
+                                                  const double *__restrict__ w, const AdjArgs &a,
+                                                  const double *tab, double (&gr)[ADJ_NG]) {
+    constexpr bool ASIAN = PK == PK_ASIAN || PK == PK_ASIAN_LOG;
+    constexpr bool FAST = RNG == OFG_RNG_FAST;
+    const double barrier = o.barrier;
+    const int steps = o.steps;
+    Leg ga, gb;
+    leg_init<PK>(ga, o, ASIAN ? w[0] : 0.0);
+    if constexpr (ANTI) leg_init<PK>(gb, o, ASIAN ? w[0] : 0.0);
+    Acc2 sa{0.0, 0.0}, sb{0.0, 0.0};
+    double Z = 0.0, G2 = 0.0;
+
+    using Rng = typename std::conditional<FAST, Xoshiro256p, Xoshiro256pp>::type;
+    Rng rng;
+    ExternalNormals ext;
+    if constexpr (RNG == OFG_RNG_EXTERNAL) {
+        ext.p1 = a.normals1 + o.normals_offset + sample;
+        ext.p2 = nullptr;
+        ext.stride = o.samples;
+    } else {
+        rng.seed(stream_seed(o.seed, (uint64_t)sample));
+    }
+
+    auto advance = [&](double z, int j) {
+        if constexpr (ASIAN) {
+            sa.q += ga.sum;
+            sa.p = fma(z, ga.sum, sa.p);
+            if constexpr (ANTI) {
+                sb.q += gb.sum;
+                sb.p = fma(-z, gb.sum, sb.p);
+            }
+        }
+        Z += z;
+        gbm_step<PK, FAST>(ga, o, z, tab);
+        if constexpr (ANTI) gbm_step<PK, FAST>(gb, o, -z, tab);
+        const double wj = ASIAN ? w[j + 1] : 0.0;
+        leg_observe<PK>(ga, wj, barrier);
+        if constexpr (ANTI) leg_observe<PK>(gb, wj, barrier);
+        if constexpr (PK == PK_ASIAN_LOG) G2 = fma(wj, Z, G2);
+    };
+    if constexpr (FAST) {
+        int j = 0;
+        for (; j + 3 < steps; j += 4) {
+            double z0, z1, z2, z3;
+            normals4(rng, z0, z1, z2, z3, tab);
+            advance(z0, j);
+            advance(z1, j + 1);
+            advance(z2, j + 2);
+            advance(z3, j + 3);
+        }
+        if (j < steps) {
+            double z0, z1, z2, z3;
+            normals4(rng, z0, z1, z2, z3, tab);
+            advance(z0, j);
+            if (j + 1 < steps) advance(z1, j + 1);
+            if (j + 2 < steps) advance(z2, j + 2);
+        }
+    } else {
+        for (int j = 0; j < steps; ++j) {
+            double z;
+            if constexpr (RNG == OFG_RNG_REFERENCE) z = reference_normal(rng);
+            else z = ext.z1(j);
+            advance(z, j);
+        }
+    }
+
+    double pv = leg_gradient<PK>(ga, o, Z, sa, G2, gr);
+    if constexpr (ANTI) {
+        double gb_grad[ADJ_NG] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        pv += leg_gradient<PK>(gb, o, -Z, sb, -G2, gb_grad);
+        pv *= 0.5;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gr[k] = 0.5 * (gr[k] + gb_grad[k]);
+    }
+    return pv;
+}
+
+template <int PK, int RNG, bool ANTI>
+__device__ void pathwise_item(const OptDev &o, const WorkItem &wi, int lane, const AdjArgs &a,
+                              const double *tab, double &spv, double &spv2, double (&acc)[ADJ_NG]) {
+    const double *__restrict__ w = a.obs_w + o.obs_offset;
+    for (int64_t base = wi.begin; base < wi.end; base += WAVE) {
+        const int64_t sample = base + lane;
+        if (sample < wi.end) {
+            double g[ADJ_NG] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            const double pv = pathwise_sample<PK, RNG, ANTI>(o, sample, w, a, tab, g);
+            spv += pv;
+            spv2 += pv * pv;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] += g[k];
+        }
+    }
+}
+
+template <int RNG, bool ANTI, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void pathwise_kernel(AdjArgs a) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    }
+    const uint32_t span = (uint32_t)(a.order_end - a.order_begin);
+    for (;;) {
+        // every lane takes part in the atomic (see the pricing kernel)
+        const uint32_t q = __builtin_amdgcn_readfirstlane(
+            atomicAdd(a.queue_head, lane == 0 ? 1u : 0u));
+        if (q >= span) break;
+        const int32_t it = __builtin_amdgcn_readfirstlane(a.order[a.order_begin + (int32_t)q]);
+        const WorkItem wi = a.items[it];
+        const OptDev o = a.opts[wi.option];
+        double spv = 0.0, spv2 = 0.0;
+        double acc[ADJ_NG] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        switch (o.kind) {
+        case PK_VANILLA: pathwise_item<PK_VANILLA, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        case PK_ASIAN: pathwise_item<PK_ASIAN, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        case PK_ASIAN_LOG: pathwise_item<PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        case PK_BARRIER: pathwise_item<PK_BARRIER, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        default: pathwise_item<PK_BARRIER_DOWN, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        }
+        AdjPartial p;
+        p.spv = wave_sum(spv);
+        p.spv2 = wave_sum(spv2);
+#pragma unroll
+        for (int k = 0; k < ADJ_NG; ++k) p.g[k] = k < 4 ? wave_sum(acc[k]) : 0.0;
+        p.n = (double)(wi.end - wi.begin);
+        if (lane == 0) a.partials[it] = p;
+    }
+}
+
+template <int RNG, bool ANTI>
+hipError_t launch_block(const AdjArgs &a, hipStream_t s) {
+    switch (a.block) {
+    case 64: pathwise_kernel<RNG, ANTI, 64><<<a.grid, 64, 0, s>>>(a); break;
+    case 128: pathwise_kernel<RNG, ANTI, 128><<<a.grid, 128, 0, s>>>(a); break;
+    default: pathwise_kernel<RNG, ANTI, 256><<<a.grid, 256, 0, s>>>(a); break;
+    }
+    return hipGetLastError();
+}
+
+template <int RNG, bool ANTI>
+const void *block_ptr(int block) {
+    switch (block) {
+    case 64: return (const void *)pathwise_kernel<RNG, ANTI, 64>;
+    case 128: return (const void *)pathwise_kernel<RNG, ANTI, 128>;
+    default: return (const void *)pathwise_kernel<RNG, ANTI, 256>;
+    }
+}
+
+template <int RNG>
+hipError_t launch_anti(const AdjArgs &a, hipStream_t s) {
+    return a.antithetic ? launch_block<RNG, true>(a, s) : launch_block<RNG, false>(a, s);
+}
+
+template <int RNG>
+const void *anti_ptr(int block, int anti) {
+    return anti ? block_ptr<RNG, true>(block) : block_ptr<RNG, false>(block);
+}
+
+}  // namespace
+
+hipError_t launch_pathwise(const AdjArgs &a, hipStream_t s) {
+    switch (a.rng_mode) {
+    case OFG_RNG_EXTERNAL: return launch_anti<OFG_RNG_EXTERNAL>(a, s);
+    case OFG_RNG_FAST: return launch_anti<OFG_RNG_FAST>(a, s);
+    default: return launch_anti<OFG_RNG_REFERENCE>(a, s);
+    }
+}
+
+int pathwise_grid(int block, int rng_mode, int antithetic, int device) {
+    const void *k = rng_mode == OFG_RNG_EXTERNAL ? anti_ptr<OFG_RNG_EXTERNAL>(block, antithetic)
+                    : rng_mode == OFG_RNG_FAST   ? anti_ptr<OFG_RNG_FAST>(block, antithetic)
+                                                 : anti_ptr<OFG_RNG_REFERENCE>(block, antithetic);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    return per_cu * cus;
+}
+
+}  // namespace ofg
