@@ -184,6 +184,10 @@ struct Heston {
 };
 
 enum class VarianceReduction { None, Antithetic, ControlVariate };
+// Differentiation of price_with_greeks_aad: checkpointed reverse sweep, or
+// the same pathwise estimator in one forward sweep (GBM; Heston uses the
+// adjoint).
+enum class GreekMethod { Adjoint = OFG_GREEKS_ADJOINT, Forward = OFG_GREEKS_FORWARD };
 enum class RngMode { Reference = OFG_RNG_REFERENCE, Fast = OFG_RNG_FAST };
 
 // ------------------------------------------------------------- context --
@@ -293,6 +297,10 @@ public:
         rng_mode = m;
         return *this;
     }
+    GpuMonteCarloPricingEngine &with_greek_method(GreekMethod g) {
+        greek_method = g;
+        return *this;
+    }
 
     PricingResult price(const VanillaOption &i, const Market &m) const override { return run(i, m, false); }
     PricingResult price(const AsianOption &i, const Market &m) const override { return run(i, m, false); }
@@ -311,6 +319,7 @@ public:
     uint64_t seed;
     VarianceReduction variance_reduction = VarianceReduction::None;
     RngMode rng_mode = RngMode::Reference;
+    GreekMethod greek_method = GreekMethod::Adjoint;
     int device = 0;
 
 private:
@@ -321,9 +330,10 @@ private:
         c.antithetic = variance_reduction == VarianceReduction::Antithetic;
         c.control_variate = variance_reduction == VarianceReduction::ControlVariate && !greeks;
         c.rng_mode = (int32_t)rng_mode;
+        c.greek_mode = greeks ? (int32_t)greek_method : OFG_GREEKS_NONE;
         ofg_result r{};
         ofg_ctx *ctx = device_context(device);
-        check(greeks ? ofg_price_greeks(ctx, &o, 1, &c, &r) : ofg_price(ctx, &o, 1, &c, &r));
+        check(ofg_price(ctx, &o, 1, &c, &r));
         return detail::to_result(r, greeks, false);
     }
 };

@@ -9,7 +9,7 @@ from .core import (AsianOption, AsianSpec, Averaging, BarrierDirection, BarrierO
                    BarrierSpec, BarrierStyle, ConvergenceFailure, Diagnostics, ExerciseStyle, Gbm,
                    Greeks, Heston, InvalidInput, Market, MarketDataMissing, NumericalError,
                    OptionType, PricingError, PricingResult, StrikeType, VanillaOption)
-from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEngine,
+from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEngine, GreekMethod,
                       HestonAadConfig, HestonMonteCarloPricingEngine, RngMode, VarianceReduction,
                       MonteCarloGreeksEngine, autotune, engine_cfg, heston_price_delta_aad,
                       mc_european_parallel, mc_european_pathwise_aad, mc_european_sequential,
@@ -26,5 +26,5 @@ __all__ = [
     "HestonMonteCarloPricingEngine", "RngMode", "VarianceReduction", "autotune", "engine_cfg",
     "heston_price_delta_aad", "mc_european_pathwise_aad", "option_struct", "price_raw",
     "sample_payoffs", "DeviceError", "MonteCarloGreeksEngine", "mc_european_parallel",
-    "mc_european_sequential", "mc_european_with_arena",
+    "mc_european_sequential", "mc_european_with_arena", "GreekMethod",
 ]

@@ -29,6 +29,14 @@ class VarianceReduction(enum.Enum):
     ControlVariate = 2
 
 
+class GreekMethod(enum.Enum):
+    """How price_with_greeks_aad differentiates: the checkpointed reverse
+    sweep (adjoint kernel), or the same pathwise estimator in one forward
+    sweep (GBM running sums; Heston falls back to the adjoint)."""
+    Adjoint = _lib.GREEKS_ADJOINT
+    Forward = _lib.GREEKS_FORWARD
+
+
 class RngMode(enum.Enum):
     Reference = _lib.RNG_REFERENCE
     External = _lib.RNG_EXTERNAL
@@ -172,7 +180,12 @@ class GpuMonteCarloPricingEngine:
         self.seed = seed
         self.variance_reduction = VarianceReduction.None_
         self.rng_mode = RngMode.Reference
+        self.greek_method = GreekMethod.Adjoint
         self.device = 0
+
+    def with_greek_method(self, method: GreekMethod) -> "GpuMonteCarloPricingEngine":
+        self.greek_method = method
+        return self
 
     def with_variance_reduction(self, vr: VarianceReduction) -> "GpuMonteCarloPricingEngine":
         self.variance_reduction = vr
@@ -214,8 +227,8 @@ class GpuMonteCarloPricingEngine:
         keep = _Keep()
         o = option_struct(instrument, market, self.num_paths, self.num_steps, self.seed, keep)
         cfg = engine_cfg(antithetic=self.variance_reduction is VarianceReduction.Antithetic,
-                         rng_mode=self.rng_mode, keep=keep)
-        r = price_raw([o], cfg, self.device, greeks=True)[0]
+                         rng_mode=self.rng_mode, greek_mode=self.greek_method.value, keep=keep)
+        r = price_raw([o], cfg, self.device)[0]
         return _to_result(r, True, False)
 
 

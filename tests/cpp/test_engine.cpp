@@ -112,6 +112,15 @@ int main() {
         const double theta = -100.0 * std::exp(-0.01) * pdf * 0.2 / 2.0 + 0.01 * 100.0 * std::exp(-0.01) * N(d1) -
                              0.03 * 100.0 * std::exp(-0.03) * N(d2);
         expect(r.greeks.has_value(), "pricing_engine_trait_has_aad_entry_point_for_mc");
+        auto f = GpuMonteCarloPricingEngine(120000, 64, 42)
+                     .with_variance_reduction(VarianceReduction::Antithetic)
+                     .with_greek_method(GreekMethod::Forward)
+                     .price_with_greeks_aad(call, mq);
+        expect(f.greeks && std::fabs(f.greeks->delta - r.greeks->delta) < 1e-10 &&
+                   std::fabs(f.greeks->vega - r.greeks->vega) < 1e-9 &&
+                   std::fabs(f.greeks->rho - r.greeks->rho) < 1e-9 &&
+                   std::fabs(f.greeks->theta - r.greeks->theta) < 1e-9,
+               "forward_pathwise_greeks_equal_adjoint");
         expect(std::fabs(r.greeks->delta - delta) < 6e-3 && std::fabs(r.greeks->vega - vega) < 8e-2 &&
                    std::fabs(r.greeks->rho - rho) < 7e-2 && std::fabs(r.greeks->theta - theta) < 9e-2,
                "mc_pathwise_aad_recovers_bs_first_order_greeks");
