@@ -358,7 +358,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched GBM: 4096 options (2048 European + 2048 arithmetic "
                                "Asian, daily observation), paths floor(10^U[4,6]), steps "
-                               "U{12..504}, plain MC, fast RNG (xoshiro256+ + Box-Muller), "
+                               "U{12..504}, plain MC, fast RNG (xoroshiro128+ + Box-Muller), "
                                "one persistent launch replayed from a hipGraph",
                    "options": len(opts), "path_steps": ps, "graph": not args.no_graph,
                    "path_scale": args.path_scale},

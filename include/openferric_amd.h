@@ -48,7 +48,7 @@ enum ofg_rng_mode {
     OFG_RNG_REFERENCE = 0,
     /* Caller-supplied normals, step-major [step][sample] per option. */
     OFG_RNG_EXTERNAL = 1,
-    /* Throughput mode: per-sample xoshiro256+ streams (same SplitMix64
+    /* Throughput mode: per-sample xoroshiro128+ streams (same SplitMix64
        seeding) feeding a branch-free, table-driven Box-Muller transform; path
        exponentials as 2^(y/256). Statistically equivalent draws. */
     OFG_RNG_FAST = 2

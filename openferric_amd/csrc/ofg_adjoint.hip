@@ -42,7 +42,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <int RNG>
-using RngOf = typename std::conditional<RNG == OFG_RNG_FAST, Xoshiro256p, Xoshiro256pp>::type;
+using RngOf = typename std::conditional<RNG == OFG_RNG_FAST, Xoroshiro128p, Xoshiro256pp>::type;
 
 // ---------------------------------------------------------------------------
 // Normals of one segment [j0, j0 + len), len <= K, forward or reverse.

@@ -54,8 +54,8 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
     if constexpr (ANTI) leg_init<PK>(gb, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0);
 
     // Reference draws use the reference's xoshiro256++; the fast mode uses
-    // xoshiro256+ with Box-Muller (four normals per three 64-bit draws).
-    using Rng = typename std::conditional<RNG == OFG_RNG_FAST, Xoshiro256p, Xoshiro256pp>::type;
+    // xoroshiro128+ with Box-Muller (four normals per three 64-bit draws).
+    using Rng = typename std::conditional<RNG == OFG_RNG_FAST, Xoroshiro128p, Xoshiro256pp>::type;
     Rng rng;
     ExternalNormals ext;
     if constexpr (RNG == OFG_RNG_REFERENCE || RNG == OFG_RNG_FAST) {

@@ -14,7 +14,7 @@
 //   * exp: Cody-Waite reduction by ln2 into |r| <= ln2/2 and a degree-11
 //     polynomial (Chebyshev-node interpolant of e^r, max relative error
 //     1.7e-17 in exact arithmetic), scaled with ldexp.
-//   * fast RNG mode: xoshiro256+ streams and a branch-free Box-Muller
+//   * fast RNG mode: xoroshiro128+ streams and a branch-free Box-Muller
 //     transform with table-driven -2 log, sin/cos of a full turn and a
 //     Newton sqrt; a table-driven 2^(y/256) for the path steps.
 #pragma once
@@ -36,7 +36,26 @@ OFG_HD uint64_t splitmix64_next(uint64_t &state) {
     return z ^ (z >> 31);
 }
 
-OFG_HD uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+// 64-bit rotate left by a constant 0 < k < 64, k != 32. On the device it is
+// two v_alignbit_b32 (funnel shifts of the 32-bit halves): for k < 32 the new
+// high word is ({hi, lo} >> (32 - k))[31:0] and the new low word
+// ({lo, hi} >> (32 - k))[31:0]; for k > 32 the halves swap roles.
+OFG_HD uint64_t rotl64(uint64_t x, int k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    uint32_t nh, nl;
+    if (k < 32) {
+        nh = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)(32 - k));
+        nl = __builtin_amdgcn_alignbit(lo, hi, (uint32_t)(32 - k));
+    } else {
+        nh = __builtin_amdgcn_alignbit(lo, hi, (uint32_t)(64 - k));
+        nl = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)(64 - k));
+    }
+    return ((uint64_t)nh << 32) | nl;
+#else
+    return (x << k) | (x >> (64 - k));
+#endif
+}
 
 // Per-path xoshiro256++ stream (fast_rng.rs:30-73). State lives in 8 VGPRs.
 struct Xoshiro256pp {
@@ -76,7 +95,7 @@ struct Xoshiro256pp {
     OFG_HD uint64_t prev_u64();
 };
 
-// Inverse of the xoshiro256 state transition (shared by ++ and +): the
+// Inverse of the xoshiro256 state transition: the
 // transition is an invertible GF(2)-linear map, so a stream can be walked
 // backwards. With old state (a,b,c,d) and new (A,B,C,D):
 //   d ^ b = rotr(D, 45); a = A ^ (d ^ b); B ^ C = b ^ (b << 17), solved for b
@@ -201,46 +220,44 @@ OFG_HD double reference_normal(Xoshiro256pp &rng) {
 
 // ------------------------------------------------ fast-mode generator ---
 
-// xoshiro256+ (Blackman & Vigna 2018): the same state and transition as
-// xoshiro256++ with the cheaper "+" scrambler (s0 + s3), the authors'
-// recommendation for floating-point output from the upper bits. Seeded by the
-// same SplitMix64 expansion of stream_seed(seed, sample).
-struct Xoshiro256p {
-    uint64_t s0, s1, s2, s3;
+// xoroshiro128+ (Blackman & Vigna, "Scrambled linear pseudorandom number
+// generators", 2018; parameters a = 24, b = 16, c = 37): 128 bits of state
+// and the "+" scrambler, the authors' generator for floating-point output from
+// the upper bits. One draw is 12 VALU operations (xoshiro256: 14) and the
+// state is 4 VGPRs. Seeded by two SplitMix64 outputs of stream_seed(seed,
+// sample); an all-zero state is replaced by s0 = 1.
+struct Xoroshiro128p {
+    uint64_t s0, s1;
 
     OFG_HD void seed(uint64_t seed) {
         uint64_t sm = seed;
         s0 = splitmix64_next(sm);
         s1 = splitmix64_next(sm);
-        s2 = splitmix64_next(sm);
-        s3 = splitmix64_next(sm);
-        if ((s0 | s1 | s2 | s3) == 0) s0 = 1;
+        if ((s0 | s1) == 0) s0 = 1;
     }
 
-    // Steps back over the last draw and returns it again.
-    OFG_HD uint64_t prev_u64();
-
     OFG_HD uint64_t next_u64() {
-        const uint64_t result = s0 + s3;
-        const uint64_t t = s1 << 17;
-        s2 ^= s0;
-        s3 ^= s1;
-        s1 ^= s2;
-        s0 ^= s3;
-        s2 ^= t;
-        s3 = rotl64(s3, 45);
+        const uint64_t result = s0 + s1;
+        const uint64_t t = s1 ^ s0;
+        s0 = rotl64(s0, 24) ^ t ^ (t << 16);
+        s1 = rotl64(t, 37);
         return result;
+    }
+
+    // Steps back over the last draw and returns it again. With t = old s0 ^
+    // old s1: t = rotr(s1, 37) and old s0 = rotr(s0 ^ t ^ (t << 16), 24).
+    OFG_HD uint64_t prev_u64() {
+        const uint64_t t = rotl64(s1, 27);
+        const uint64_t a = s0 ^ t ^ (t << 16);
+        s0 = rotl64(a, 40);
+        s1 = t ^ s0;
+        return s0 + s1;
     }
 };
 
 OFG_HD uint64_t Xoshiro256pp::prev_u64() {
     xoshiro256_unstep(s0, s1, s2, s3);
     return rotl64(s0 + s3, 23) + s0;
-}
-
-OFG_HD uint64_t Xoshiro256p::prev_u64() {
-    xoshiro256_unstep(s0, s1, s2, s3);
-    return s0 + s3;
 }
 
 OFG_HD double bits_as_double(uint64_t b) {

@@ -94,7 +94,7 @@ __device__ void mcg_item(const OptDev &o, const WorkItem &wi, int lane, const Ad
             rng.seed(stream_seed(o.seed, (uint64_t)sample));
             z = reference_normal(rng);
         } else if constexpr (RNG == OFG_RNG_FAST) {
-            Xoshiro256p rng;
+            Xoroshiro128p rng;
             rng.seed(stream_seed(o.seed, (uint64_t)sample));
             const uint64_t wa = rng.next_u64();
             const uint64_t wb = rng.next_u64();

@@ -90,7 +90,7 @@ __device__ __forceinline__ double pathwise_sample(const OptDev &o, int64_t sampl
     Acc2 sa{0.0, 0.0}, sb{0.0, 0.0};
     double Z = 0.0, G2 = 0.0;
 
-    using Rng = typename std::conditional<FAST, Xoshiro256p, Xoshiro256pp>::type;
+    using Rng = typename std::conditional<FAST, Xoroshiro128p, Xoshiro256pp>::type;
     Rng rng;
     ExternalNormals ext;
     if constexpr (RNG == OFG_RNG_EXTERNAL) {

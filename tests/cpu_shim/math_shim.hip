@@ -21,7 +21,7 @@ void shim_sincos_turn(uint32_t wt, double *s, double *c) {
     ofg::sincos_turn(wt, *s, *c, ofg::OFG_FAST_TABLE);
 }
 void shim_fast_normals(uint64_t seed, uint64_t sample, int64_t n4, double *out) {
-    ofg::Xoshiro256p r;
+    ofg::Xoroshiro128p r;
     r.seed(ofg::stream_seed(seed, sample));
     for (int64_t k = 0; k < n4; ++k)
         ofg::normals4(r, out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3],
@@ -38,7 +38,7 @@ double shim_fast_gbm_terminal(uint64_t seed, uint64_t sample, double s0, double 
     const double dt = expiry / (double)steps;
     const double drift2 = ((mu - 0.5 * sigma * sigma) * dt) * OFG_INV_LN2_256;
     const double diffusion2 = (sigma * sqrt(dt)) * OFG_INV_LN2_256;
-    ofg::Xoshiro256p r;
+    ofg::Xoroshiro128p r;
     r.seed(ofg::stream_seed(seed, sample));
     double s = s0;
     for (int j = 0; j < steps; j += 4) {
@@ -62,7 +62,7 @@ double shim_fast_heston_terminal(uint64_t seed, uint64_t sample, double s0, doub
     const double sqrt_dt2 = sqrt_dt * OFG_INV_LN2_256;
     const double mudt2 = (mu * dt) * OFG_INV_LN2_256;
     const double hdt2 = (0.5 * dt) * OFG_INV_LN2_256;
-    ofg::Xoshiro256p r;
+    ofg::Xoroshiro128p r;
     r.seed(ofg::stream_seed(seed, sample));
     double s = s0, v = v0;
     for (int j = 0; j < steps; j += 2) {
@@ -86,16 +86,23 @@ double shim_fast_heston_terminal(uint64_t seed, uint64_t sample, double s0, doub
 }
 
 extern "C" {
+// n outputs of xoroshiro128+ from an explicit state (no seeding).
+void shim_xoroshiro_from_state(uint64_t s0, uint64_t s1, int64_t n, uint64_t *out) {
+    ofg::Xoroshiro128p r;
+    r.s0 = s0;
+    r.s1 = s1;
+    for (int64_t i = 0; i < n; ++i) out[i] = r.next_u64();
+}
 // Draws n values forward, then walks back n steps; writes both sequences and
 // returns 1 if the state after walking back equals the initial state.
 int shim_xoshiro_roundtrip(uint64_t seed, int plus, int64_t n, uint64_t *fwd, uint64_t *bwd) {
     if (plus) {
-        ofg::Xoshiro256p r;
+        ofg::Xoroshiro128p r;
         r.seed(seed);
-        const ofg::Xoshiro256p r0 = r;
+        const ofg::Xoroshiro128p r0 = r;
         for (int64_t i = 0; i < n; ++i) fwd[i] = r.next_u64();
         for (int64_t i = 0; i < n; ++i) bwd[i] = r.prev_u64();
-        return r.s0 == r0.s0 && r.s1 == r0.s1 && r.s2 == r0.s2 && r.s3 == r0.s3;
+        return r.s0 == r0.s0 && r.s1 == r0.s1;
     }
     ofg::Xoshiro256pp r;
     r.seed(seed);

@@ -169,3 +169,17 @@ def test_xoshiro_inverse_walks_the_stream_back(shim):
             b = (C.c_uint64 * n)()
             assert shim.shim_xoshiro_roundtrip(seed, plus, n, f, b) == 1
             assert list(b) == list(f)[::-1]
+
+
+def test_xoroshiro128plus_unit_state_by_hand(shim):
+    """xoroshiro128+ from state (1, 2), derived by hand from the published
+    transition (a = 24, b = 16, c = 37): output s0 + s1 = 3; t = 1 ^ 2 = 3,
+    s0' = rotl(1, 24) ^ 3 ^ (3 << 16) = 0x1030003, s1' = rotl(3, 37) =
+    0x6000000000, so the second output is 0x6001030003."""
+    out = (C.c_uint64 * 2)()
+    shim.shim_xoroshiro_from_state(1, 2, 2, out)
+    assert list(out) == [3, 0x6001030003]
+    # from (2^63, 0): t = 2^63; s0' = rotl(2^63, 24) ^ 2^63 ^ 0 = 2^23 ^ 2^63;
+    # s1' = rotl(2^63, 37) = 2^36
+    shim.shim_xoroshiro_from_state(1 << 63, 0, 2, out)
+    assert list(out) == [1 << 63, ((1 << 23) ^ (1 << 63)) + (1 << 36)]

@@ -198,7 +198,7 @@ def test_fast_mode_within_3se_of_bs(oracle, gpu):
 @pytest.mark.parametrize("steps", [1, 7, 64, 253])
 def test_fast_mode_gbm_paths_equal_host_build(gpu, shim, steps):
     """Fast-mode GBM paths on the GPU equal, sample by sample, the host build
-    of the same device math (tests/cpu_shim): xoshiro256+ stream, Box-Muller
+    of the same device math (tests/cpu_shim): xoroshiro128+ stream, Box-Muller
     from the LDS tables, 2^(y/256) steps. Only the GPU's reciprocal-sqrt seed
     and rounding of fused operations may move the last bits. A call struck at
     1e-9 makes the payoff S_T - 1e-9."""
