@@ -212,18 +212,26 @@ def heston_leg(rng_mode):
 
 def heston_greeks_leg(rng_mode):
     """Batched Heston adjoint: 1024 options, 2^18 paths, steps ceil(252 T)."""
-    from openferric_amd import engine_cfg, price_raw
+    from openferric_amd import _lib, engine_cfg, price_raw
     opts = heston_greeks_workload()
     structs, keeps = zip(*[to_ofg(d) for d in opts])
     structs = list(structs)
     cfg = engine_cfg(rng_mode=rng_mode)
     p = price_raw(structs, cfg)
     a = price_raw(structs, cfg, greeks=True)
+    f = price_raw(structs, engine_cfg(rng_mode=rng_mode, greek_mode=_lib.GREEKS_FORWARD))
+    rel = [abs(y - x) / max(abs(x), 1e-12) for ra, rf in zip(a, f)
+           for x, y in zip(ra.heston_grad, rf.heston_grad)]
     ps = sum(d["num_paths"] * d["num_steps"] for d in opts)
     return {"workload": "1024 Heston European/up-and-out calls, 2^18 paths, steps ceil(252T)",
             "path_steps": ps, "pricing_ms": p[0].kernel_ms, "adjoint_ms": a[0].kernel_ms,
             "adjoint_cost_in_pricing_passes": a[0].kernel_ms / p[0].kernel_ms,
-            "adjoint_greeks_per_s": 6 * len(opts) / (a[0].kernel_ms * 1e-3)}
+            "adjoint_greeks_per_s": 6 * len(opts) / (a[0].kernel_ms * 1e-3),
+            # the same sensitivities by a forward tangent of the Euler step
+            "forward_ms": f[0].kernel_ms,
+            "forward_cost_in_pricing_passes": f[0].kernel_ms / p[0].kernel_ms,
+            "forward_greeks_per_s": 6 * len(opts) / (f[0].kernel_ms * 1e-3),
+            "forward_vs_adjoint_max_rel_diff": max(rel)}
 
 
 def copy_calibration(mib):

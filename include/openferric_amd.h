@@ -59,8 +59,9 @@ enum ofg_rng_mode {
    FD_CRN   central bump-and-revalue with common random numbers, all bumps in
             one batched launch (greeks/bsm.rs:130-212);
    FORWARD  the same pathwise estimator as ADJOINT evaluated in one forward
-            sweep: GBM options by running sums of the path (no reverse pass);
-            Heston options use the adjoint kernel. */
+            sweep, no reverse pass or checkpoints: GBM options by running
+            sums of the path, Heston options by a forward tangent of the
+            Euler step (all six Heston sensitivities). */
 enum ofg_greek_mode {
     OFG_GREEKS_NONE = 0,
     OFG_GREEKS_ADJOINT = 1,

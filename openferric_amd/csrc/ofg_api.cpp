@@ -916,8 +916,8 @@ int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
             a.antithetic = p.antithetic;
             a.block = t.block;
             const int wpb = t.block / 64;
-            const bool fwd = forward && m == OFG_MODEL_GBM;  // no checkpoints
-            const int full = fwd ? ofg::pathwise_grid(t.block, c.rng_mode, p.antithetic, ctx->device)
+            const bool fwd = forward;  // no checkpoints
+            const int full = fwd ? ofg::pathwise_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device)
                                  : ofg::adjoint_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device);
             a.grid = std::max(1, (int)std::min<int64_t>(full, ((int64_t)cnt + wpb - 1) / wpb));
             if (!fwd)
@@ -930,7 +930,7 @@ int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
         for (int m = 0; m < 2; ++m) {
             if (p.model_end[m] - p.model_begin[m] <= 0) continue;
             la[m].scratch = ctx->adj.as<double>();
-            if (forward && m == OFG_MODEL_GBM)
+            if (forward)
                 hip_check(ofg::launch_pathwise(la[m], s), "pathwise kernel launch");
             else
                 hip_check(ofg::launch_adjoint(la[m], s), "adjoint kernel launch");

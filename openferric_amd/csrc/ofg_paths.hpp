@@ -76,21 +76,29 @@ __device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z, cons
 //                per-option constants (fast mode): kappa dt, kappa theta dt,
 //                xi sqrt(dt), and the log-spot increment's mu dt, dt / 2,
 //                sqrt(dt) pre-scaled by 256/ln2 for exp2_256.
+// Intermediates of one Heston step that its tangent needs: sqrt of the
+// positive variance, the correlated normal, and the unclamped spot and
+// variance.
+struct HestonParts {
+    double sv, zs, sn, vn;
+};
+
 template <bool REF>
 __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o, double z1,
                                              double z2, double &s_next, double &v_next,
-                                             double &growth, double &e, const double *tab) {
+                                             double &growth, double &e, const double *tab,
+                                             HestonParts &hp) {
     const double v_pos = fmax(v, 0.0);
-    double vn;
+    double vn, sv, zs;
     if constexpr (REF) {
-        const double sv = sqrt(v_pos);
-        const double zs = o.rho * z1 + o.rho_c * z2;
+        sv = sqrt(v_pos);
+        zs = o.rho * z1 + o.rho_c * z2;
         vn = v + o.kappa * (o.theta - v_pos) * o.dt + o.xi * sv * o.sqrt_dt * z1;
         e = (o.mu - 0.5 * v_pos) * o.dt + sv * o.sqrt_dt * zs;
         growth = exp_fp64(e);
     } else {
-        const double sv = v_pos > 0.0 ? sqrt_pos(v_pos) : 0.0;
-        const double zs = fma(o.rho, z1, o.rho_c * z2);
+        sv = v_pos > 0.0 ? sqrt_pos(v_pos) : 0.0;
+        zs = fma(o.rho, z1, o.rho_c * z2);
         vn = fma(o.xsdt * sv, z1, fma(-o.kdt, v_pos, v + o.kthdt));
         // v has no a-priori bound: clamp y into exp2_256's domain (past
         // |y| = 2^30 the growth is 0 or inf either way)
@@ -102,6 +110,18 @@ __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o
     const double sn = s * growth;
     s_next = fmax(sn, 1e-12);   // f64::max(sn, 1e-12): maxNum, NaN -> 1e-12
     v_next = fmax(vn, 0.0);
+    hp.sv = sv;
+    hp.zs = zs;
+    hp.sn = sn;
+    hp.vn = vn;
+}
+
+template <bool REF>
+__device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o, double z1,
+                                             double z2, double &s_next, double &v_next,
+                                             double &growth, double &e, const double *tab) {
+    HestonParts hp;
+    heston_euler<REF>(s, v, o, z1, z2, s_next, v_next, growth, e, tab, hp);
 }
 
 template <int PK, bool REF>

@@ -1,4 +1,6 @@
-// Forward pathwise Greeks of GBM paths for MI355X (gfx950), FP64: the same
+// Forward pathwise Greeks for MI355X (gfx950), FP64: GBM by running sums of
+// the path (below), Heston by a forward tangent of the Euler step (further
+// down). GBM: the same
 // pathwise estimator as the adjoint kernel (positive-part derivative
 // 1{x > 0}, zero derivative of barrier indicators, observation indices fixed
 // relative to T, d/dT through dt = T/n and the discount), evaluated in ONE
@@ -155,7 +157,193 @@ __device__ __forceinline__ double pathwise_sample(const OptDev &o, int64_t sampl
     return pv;
 }
 
+// ---------------------------------------------------------------------------
+// Heston: forward tangent of the Euler step (no closed form for the variance
+// path). Each leg carries d ln S and d v for the gradient slots 0 S0, 1 v0,
+// 2 r, 3 T, 4 kappa, 5 theta, 6 xi, 7 rho, with the adjoint kernel's
+// conventions: the clamp indicators 1{sn >= 1e-12} and 1{vn > 0}, d v_pos/dv =
+// 1{v > 0}, d sqrt(v_pos)/d v_pos = 1/(2 sqrt) taken as 0 at v_pos = 0, and
+// d/dT through dt = T/n. Asian legs also carry sum_j w_j S_j d ln S_j (a) and
+// sum_j w_j d ln S_j (gq).
+// ---------------------------------------------------------------------------
+struct HTan {
+    double ls[ADJ_NG], dv[ADJ_NG], a[ADJ_NG], gq[ADJ_NG];
+};
+
+template <int PK>
+__device__ __forceinline__ void htan_init(HTan &t, const OptDev &o, double w0) {
+#pragma unroll
+    for (int k = 0; k < ADJ_NG; ++k) t.ls[k] = t.dv[k] = t.a[k] = t.gq[k] = 0.0;
+    t.ls[0] = 1.0 / o.s0;
+    t.dv[1] = 1.0;
+    if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) t.a[0] = w0;
+    if constexpr (PK == PK_ASIAN_LOG) t.gq[0] = w0 * t.ls[0];
+}
+
+template <int PK, bool REF>
+__device__ __forceinline__ void heston_tangent_step(Leg &g, double &v, HTan &t, const OptDev &o,
+                                                    double z1, double z2, const double *tab,
+                                                    double half_isdt, double inv_n) {
+    double s_next, v_next, growth, e;
+    HestonParts hp;
+    const double v_old = v;
+    heston_euler<REF>(g.s, v, o, z1, z2, s_next, v_next, growth, e, tab, hp);
+    if constexpr (PK == PK_ASIAN_LOG) {
+        const bool floor_hit = !(g.s * growth >= 1e-12);
+        g.ls = floor_hit ? -27.631021115928547 : g.ls + e;
+    }
+    g.s = s_next;
+    v = v_next;
+
+    const double v_pos = fmax(v_old, 0.0);
+    const double av = v_old > 0.0 ? 1.0 : 0.0;
+    const double hinv = hp.sv > 0.0 ? 0.5 / hp.sv : 0.0;
+    const double ind_s = hp.sn >= 1e-12 ? 1.0 : 0.0;
+    const double ind_v = hp.vn > 0.0 ? 1.0 : 0.0;
+    const double svd = av * hinv;   // d sqrt(v_pos) / d v
+    const double A = fma(o.xi * o.sqrt_dt * z1, svd, fma(-o.kappa * o.dt, av, 1.0));  // d vn / d v
+    const double B = fma(o.sqrt_dt * hp.zs, svd, -0.5 * o.dt * av);                 // d e / d v
+    const double sq = hp.sv * o.sqrt_dt;
+    const double dv_kappa = (o.theta - v_pos) * o.dt;
+    const double dv_theta = o.kappa * o.dt;
+    const double dv_xi = sq * z1;
+    const double de_rho = sq * fma(-o.rho / o.rho_c, z2, z1);
+    const double de_t = fma(hp.sv * hp.zs, half_isdt, o.mu - 0.5 * v_pos) * inv_n;
+    const double dv_t = fma(o.xi * hp.sv * z1, half_isdt, o.kappa * (o.theta - v_pos)) * inv_n;
+
+    // S0 and r never move v
+    t.ls[0] = t.ls[0] * ind_s;
+    t.ls[2] = (t.ls[2] + o.dt) * ind_s;
+    // v0
+    t.ls[1] = fma(t.dv[1], B, t.ls[1]) * ind_s;
+    t.dv[1] = t.dv[1] * A * ind_v;
+    // T
+    t.ls[3] = (t.ls[3] + fma(t.dv[3], B, de_t)) * ind_s;
+    t.dv[3] = fma(t.dv[3], A, dv_t) * ind_v;
+    // kappa, theta, xi (variance direct terms)
+    t.ls[4] = fma(t.dv[4], B, t.ls[4]) * ind_s;
+    t.dv[4] = fma(t.dv[4], A, dv_kappa) * ind_v;
+    t.ls[5] = fma(t.dv[5], B, t.ls[5]) * ind_s;
+    t.dv[5] = fma(t.dv[5], A, dv_theta) * ind_v;
+    t.ls[6] = fma(t.dv[6], B, t.ls[6]) * ind_s;
+    t.dv[6] = fma(t.dv[6], A, dv_xi) * ind_v;
+    // rho (log-spot direct term)
+    t.ls[7] = (t.ls[7] + fma(t.dv[7], B, de_rho)) * ind_s;
+    t.dv[7] = t.dv[7] * A * ind_v;
+}
+
+template <int PK>
+__device__ __forceinline__ void htan_observe(HTan &t, const Leg &g, double wj) {
+    if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) {
+        const double ws = wj * g.s;
+#pragma unroll
+        for (int k = 0; k < ADJ_NG; ++k) t.a[k] = fma(ws, t.ls[k], t.a[k]);
+    }
+    if constexpr (PK == PK_ASIAN_LOG) {
+#pragma unroll
+        for (int k = 0; k < ADJ_NG; ++k) t.gq[k] = fma(wj, t.ls[k], t.gq[k]);
+    }
+}
+
+template <int PK>
+__device__ __forceinline__ double heston_leg_gradient(const Leg &g, const OptDev &o,
+                                                      const HTan &t, double (&gr)[ADJ_NG]) {
+    double ct, ca, cg;
+    const double x = leg_payoff_seeds<PK>(g, o, ct, ca, cg);
+    const double pv = o.df * x;
+    const double T = o.dt * (double)o.steps;
+    const double tS = ct * g.s;
+#pragma unroll
+    for (int k = 0; k < ADJ_NG; ++k) {
+        double d = tS * t.ls[k];
+        if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) d = fma(ca, t.a[k], d);
+        if constexpr (PK == PK_ASIAN_LOG) d = fma(cg, t.gq[k], d);
+        gr[k] += o.df * d;
+    }
+    gr[2] -= T * pv;        // mu = r - q and the discount exp(-r T)
+    gr[3] -= o.rate * pv;   // the discount's T
+    return pv;
+}
+
 template <int PK, int RNG, bool ANTI>
+__device__ __forceinline__ double heston_tangent_sample(const OptDev &o, int64_t sample,
+                                                        const double *__restrict__ w,
+                                                        const AdjArgs &a, const double *tab,
+                                                        double (&gr)[ADJ_NG]) {
+    constexpr bool ASIAN = PK == PK_ASIAN || PK == PK_ASIAN_LOG;
+    constexpr bool REF = RNG != OFG_RNG_FAST;
+    const double barrier = o.barrier;
+    const int steps = o.steps;
+    const double half_isdt = 0.5 / o.sqrt_dt;
+    const double inv_n = 1.0 / (double)steps;
+    Leg ga, gb;
+    HTan ta, tb;
+    double va = o.v0, vb = o.v0;
+    leg_init<PK>(ga, o, ASIAN ? w[0] : 0.0);
+    htan_init<PK>(ta, o, ASIAN ? w[0] : 0.0);
+    if constexpr (ANTI) {
+        leg_init<PK>(gb, o, ASIAN ? w[0] : 0.0);
+        htan_init<PK>(tb, o, ASIAN ? w[0] : 0.0);
+    }
+    using Rng = typename std::conditional<RNG == OFG_RNG_FAST, Xoroshiro128p, Xoshiro256pp>::type;
+    Rng rng;
+    ExternalNormals ext;
+    if constexpr (RNG == OFG_RNG_EXTERNAL) {
+        ext.p1 = a.normals1 + o.normals_offset + sample;
+        ext.p2 = a.normals2 + o.normals_offset + sample;
+        ext.stride = o.samples;
+    } else {
+        rng.seed(stream_seed(o.seed, (uint64_t)sample));
+    }
+    auto advance = [&](double z1, double z2, int j) {
+        heston_tangent_step<PK, REF>(ga, va, ta, o, z1, z2, tab, half_isdt, inv_n);
+        if constexpr (ANTI) heston_tangent_step<PK, REF>(gb, vb, tb, o, -z1, -z2, tab, half_isdt, inv_n);
+        const double wj = ASIAN ? w[j + 1] : 0.0;
+        leg_observe<PK>(ga, wj, barrier);
+        htan_observe<PK>(ta, ga, wj);
+        if constexpr (ANTI) {
+            leg_observe<PK>(gb, wj, barrier);
+            htan_observe<PK>(tb, gb, wj);
+        }
+    };
+    if constexpr (RNG == OFG_RNG_FAST) {
+        int j = 0;
+        for (; j + 1 < steps; j += 2) {
+            double z0, z1, z2, z3;
+            normals4(rng, z0, z1, z2, z3, tab);
+            advance(z0, z1, j);
+            advance(z2, z3, j + 1);
+        }
+        if (j < steps) {
+            double z0, z1, z2, z3;
+            normals4(rng, z0, z1, z2, z3, tab);
+            advance(z0, z1, j);
+        }
+    } else {
+        for (int j = 0; j < steps; ++j) {
+            double z1, z2;
+            if constexpr (RNG == OFG_RNG_REFERENCE) {
+                z1 = reference_normal(rng);
+                z2 = reference_normal(rng);
+            } else {
+                z1 = ext.z1(j);
+                z2 = ext.z2(j);
+            }
+            advance(z1, z2, j);
+        }
+    }
+    double pv = heston_leg_gradient<PK>(ga, o, ta, gr);
+    if constexpr (ANTI) {
+        double g2[ADJ_NG] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        pv += heston_leg_gradient<PK>(gb, o, tb, g2);
+        pv *= 0.5;
+#pragma unroll
+        for (int k = 0; k < ADJ_NG; ++k) gr[k] = 0.5 * (gr[k] + g2[k]);
+    }
+    return pv;
+}
+
+template <int MODEL, int PK, int RNG, bool ANTI>
 __device__ void pathwise_item(const OptDev &o, const WorkItem &wi, int lane, const AdjArgs &a,
                               const double *tab, double &spv, double &spv2, double (&acc)[ADJ_NG]) {
     const double *__restrict__ w = a.obs_w + o.obs_offset;
@@ -163,17 +351,23 @@ __device__ void pathwise_item(const OptDev &o, const WorkItem &wi, int lane, con
         const int64_t sample = base + lane;
         if (sample < wi.end) {
             double g[ADJ_NG] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            const double pv = pathwise_sample<PK, RNG, ANTI>(o, sample, w, a, tab, g);
+            double pv;
+            if constexpr (MODEL == OFG_MODEL_GBM) pv = pathwise_sample<PK, RNG, ANTI>(o, sample, w, a, tab, g);
+            else pv = heston_tangent_sample<PK, RNG, ANTI>(o, sample, w, a, tab, g);
             spv += pv;
             spv2 += pv * pv;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) acc[k] += g[k];
+            for (int k = 0; k < ADJ_NG; ++k) acc[k] += g[k];
         }
     }
 }
 
-template <int RNG, bool ANTI, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void pathwise_kernel(AdjArgs a) {
+// The Heston tangent holds 16 (Asian: 32) extra doubles per leg: with one
+// leg, ask the register allocator for two waves per SIMD (it fits 256 VGPRs
+// without spilling; two antithetic legs do not).
+template <int RNG, int MODEL, bool ANTI, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
+    (MODEL == OFG_MODEL_HESTON && !ANTI) ? 2 : 1))) void pathwise_kernel(AdjArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
@@ -192,65 +386,74 @@ __global__ __launch_bounds__(BLOCK) void pathwise_kernel(AdjArgs a) {
         double spv = 0.0, spv2 = 0.0;
         double acc[ADJ_NG] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         switch (o.kind) {
-        case PK_VANILLA: pathwise_item<PK_VANILLA, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
-        case PK_ASIAN: pathwise_item<PK_ASIAN, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
-        case PK_ASIAN_LOG: pathwise_item<PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
-        case PK_BARRIER: pathwise_item<PK_BARRIER, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
-        default: pathwise_item<PK_BARRIER_DOWN, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        case PK_VANILLA: pathwise_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        case PK_ASIAN: pathwise_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        case PK_ASIAN_LOG: pathwise_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        case PK_BARRIER: pathwise_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
+        default: pathwise_item<MODEL, PK_BARRIER_DOWN, RNG, ANTI>(o, wi, lane, a, tab, spv, spv2, acc); break;
         }
         AdjPartial p;
         p.spv = wave_sum(spv);
         p.spv2 = wave_sum(spv2);
 #pragma unroll
-        for (int k = 0; k < ADJ_NG; ++k) p.g[k] = k < 4 ? wave_sum(acc[k]) : 0.0;
+        for (int k = 0; k < ADJ_NG; ++k)
+            p.g[k] = (MODEL == OFG_MODEL_HESTON || k < 4) ? wave_sum(acc[k]) : 0.0;
         p.n = (double)(wi.end - wi.begin);
         if (lane == 0) a.partials[it] = p;
     }
 }
 
-template <int RNG, bool ANTI>
+template <int RNG, int MODEL, bool ANTI>
 hipError_t launch_block(const AdjArgs &a, hipStream_t s) {
     switch (a.block) {
-    case 64: pathwise_kernel<RNG, ANTI, 64><<<a.grid, 64, 0, s>>>(a); break;
-    case 128: pathwise_kernel<RNG, ANTI, 128><<<a.grid, 128, 0, s>>>(a); break;
-    default: pathwise_kernel<RNG, ANTI, 256><<<a.grid, 256, 0, s>>>(a); break;
+    case 64: pathwise_kernel<RNG, MODEL, ANTI, 64><<<a.grid, 64, 0, s>>>(a); break;
+    case 128: pathwise_kernel<RNG, MODEL, ANTI, 128><<<a.grid, 128, 0, s>>>(a); break;
+    default: pathwise_kernel<RNG, MODEL, ANTI, 256><<<a.grid, 256, 0, s>>>(a); break;
     }
     return hipGetLastError();
 }
 
-template <int RNG, bool ANTI>
+template <int RNG, int MODEL, bool ANTI>
 const void *block_ptr(int block) {
     switch (block) {
-    case 64: return (const void *)pathwise_kernel<RNG, ANTI, 64>;
-    case 128: return (const void *)pathwise_kernel<RNG, ANTI, 128>;
-    default: return (const void *)pathwise_kernel<RNG, ANTI, 256>;
+    case 64: return (const void *)pathwise_kernel<RNG, MODEL, ANTI, 64>;
+    case 128: return (const void *)pathwise_kernel<RNG, MODEL, ANTI, 128>;
+    default: return (const void *)pathwise_kernel<RNG, MODEL, ANTI, 256>;
     }
 }
 
 template <int RNG>
-hipError_t launch_anti(const AdjArgs &a, hipStream_t s) {
-    return a.antithetic ? launch_block<RNG, true>(a, s) : launch_block<RNG, false>(a, s);
+hipError_t launch_model(const AdjArgs &a, hipStream_t s) {
+    if (a.model == OFG_MODEL_GBM)
+        return a.antithetic ? launch_block<RNG, OFG_MODEL_GBM, true>(a, s)
+                            : launch_block<RNG, OFG_MODEL_GBM, false>(a, s);
+    return a.antithetic ? launch_block<RNG, OFG_MODEL_HESTON, true>(a, s)
+                        : launch_block<RNG, OFG_MODEL_HESTON, false>(a, s);
 }
 
 template <int RNG>
-const void *anti_ptr(int block, int anti) {
-    return anti ? block_ptr<RNG, true>(block) : block_ptr<RNG, false>(block);
+const void *model_ptr(int block, int model, int anti) {
+    if (model == OFG_MODEL_GBM)
+        return anti ? block_ptr<RNG, OFG_MODEL_GBM, true>(block)
+                    : block_ptr<RNG, OFG_MODEL_GBM, false>(block);
+    return anti ? block_ptr<RNG, OFG_MODEL_HESTON, true>(block)
+                : block_ptr<RNG, OFG_MODEL_HESTON, false>(block);
 }
 
 }  // namespace
 
 hipError_t launch_pathwise(const AdjArgs &a, hipStream_t s) {
     switch (a.rng_mode) {
-    case OFG_RNG_EXTERNAL: return launch_anti<OFG_RNG_EXTERNAL>(a, s);
-    case OFG_RNG_FAST: return launch_anti<OFG_RNG_FAST>(a, s);
-    default: return launch_anti<OFG_RNG_REFERENCE>(a, s);
+    case OFG_RNG_EXTERNAL: return launch_model<OFG_RNG_EXTERNAL>(a, s);
+    case OFG_RNG_FAST: return launch_model<OFG_RNG_FAST>(a, s);
+    default: return launch_model<OFG_RNG_REFERENCE>(a, s);
     }
 }
 
-int pathwise_grid(int block, int rng_mode, int antithetic, int device) {
-    const void *k = rng_mode == OFG_RNG_EXTERNAL ? anti_ptr<OFG_RNG_EXTERNAL>(block, antithetic)
-                    : rng_mode == OFG_RNG_FAST   ? anti_ptr<OFG_RNG_FAST>(block, antithetic)
-                                                 : anti_ptr<OFG_RNG_REFERENCE>(block, antithetic);
+int pathwise_grid(int block, int rng_mode, int model, int antithetic, int device) {
+    const void *k = rng_mode == OFG_RNG_EXTERNAL ? model_ptr<OFG_RNG_EXTERNAL>(block, model, antithetic)
+                    : rng_mode == OFG_RNG_FAST   ? model_ptr<OFG_RNG_FAST>(block, model, antithetic)
+                                                 : model_ptr<OFG_RNG_REFERENCE>(block, model, antithetic);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess ||
         per_cu <= 0)
