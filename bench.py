@@ -34,7 +34,7 @@ WORKLOAD_SEED = 0x0FE22C1C
 # HBM bytes per launch of the pricing kernel on the default workload, from
 # rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) scaled by a 1 GiB
 # copy kernel (profiles/price_kernel_fast_gbm_batch4096_counters.md).
-PRICE_KERNEL_HBM_BYTES = 89.8e6 + 40.7e6
+PRICE_KERNEL_HBM_BYTES = 91.3e6 + 38.7e6
 
 
 def make_workload(n_opts=4096, seed=WORKLOAD_SEED, path_scale=1.0):
