@@ -301,6 +301,8 @@ def cpu_baseline(opts, target_s=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the default launch shape (block 256, 8192 lane-steps per item)")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -322,12 +324,20 @@ def main():
                                    "(multi-GPU sharding is described in DESIGN.md)"}))
         sys.exit(2)
 
-    from openferric_amd import BatchPricer, engine_cfg, RngMode
+    from openferric_amd import BatchPricer, autotune, engine_cfg, RngMode
     from openferric_amd import _lib
 
     opts = make_workload(path_scale=args.path_scale)
     structs, keeps = zip(*[to_ofg(d) for d in opts])
     cfg = engine_cfg(antithetic=False, control_variate=False, rng_mode=RngMode.Fast)
+    tuned = None
+    if not args.no_autotune:
+        # block size x work-item size, timed on a scaled copy of the batch; the
+        # choice is remembered by the context and used by the batch below
+        print("[bench] autotuning launch shape", file=sys.stderr, flush=True)
+        tuned = autotune(list(structs), cfg)
+        print(f"[bench] autotuned: block {tuned[0]}, item {tuned[1]} lane-steps",
+              file=sys.stderr, flush=True)
     print(f"[bench] building batch of {len(opts)} options", file=sys.stderr, flush=True)
     batch = BatchPricer(list(structs), cfg, use_graph=not args.no_graph)
     for w in range(args.warmup):
@@ -369,7 +379,9 @@ def main():
                                "U{12..504}, plain MC, fast RNG (xoroshiro128+ + Box-Muller), "
                                "one persistent launch replayed from a hipGraph",
                    "options": len(opts), "path_steps": ps, "graph": not args.no_graph,
-                   "path_scale": args.path_scale},
+                   "path_scale": args.path_scale,
+                   "autotuned": None if tuned is None else
+                   {"block": tuned[0], "item_lane_steps": tuned[1]}},
         "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": PRICE_KERNEL_HBM_BYTES if args.path_scale == 1.0 else None,

@@ -1254,10 +1254,11 @@ int32_t ofg_autotune(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
         if (!ctx || !opts || !cfg || n == 0) fail(OFG_INVALID_INPUT, "null argument");
         if (cfg->rng_mode == OFG_RNG_EXTERNAL) fail(OFG_INVALID_INPUT, "autotune needs generated normals");
         // Scaled-down copy: same options, path counts scaled so the sample
-        // workload is ~2e9 path-steps (or the full workload if smaller).
+        // workload is ~2e10 path-steps (or the full workload if smaller):
+        // large enough that the launch's tail behaves like the full batch's.
         double total = 0;
         for (size_t i = 0; i < n; ++i) total += (double)opts[i].num_paths * (double)opts[i].num_steps;
-        const double scale = std::min(1.0, 2e9 / std::max(total, 1.0));
+        const double scale = std::min(1.0, 2e10 / std::max(total, 1.0));
         std::vector<ofg_option> small(opts, opts + n);
         for (auto &o : small) o.num_paths = std::max<int64_t>(64, (int64_t)(o.num_paths * scale));
         ofg_engine_cfg c = *cfg;
@@ -1277,7 +1278,7 @@ int32_t ofg_autotune(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
                 c.tune_block = bl;
                 c.tune_item_steps = is;
                 double ms = 1e300;
-                for (int rep = 0; rep < 2; ++rep) {
+                for (int rep = 0; rep < 3; ++rep) {
                     const int32_t rc = price_impl(ctx, small.data(), n, &c, res.data());
                     if (rc != OFG_OK) return rc;
                     ms = std::min(ms, res[0].kernel_ms);
