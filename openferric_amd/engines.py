@@ -110,6 +110,7 @@ def option_struct(instrument, market: Market, num_paths: int, num_steps: int, se
     else:
         raise InvalidInput(f"unsupported instrument {type(instrument).__name__}")
     o.num_paths, o.num_steps, o.seed = int(num_paths), int(num_steps), int(seed) & (2**64 - 1)
+    o._keep = keep  # obs_times points into an array this keeps alive
     return o
 
 
@@ -126,6 +127,7 @@ def engine_cfg(antithetic=False, control_variate=False, rng_mode=RngMode.Referen
     c.normals1 = keep.arr(normals1) if normals1 is not None else None
     c.normals2 = keep.arr(normals2) if normals2 is not None else None
     c.tune_block, c.tune_item_steps = tune_block, tune_item_steps
+    c._keep = keep  # the struct holds raw pointers into these arrays
     return c
 
 
