@@ -133,6 +133,52 @@ typedef struct ofg_mc_greeks {
     double kernel_ms;
 } ofg_mc_greeks;
 
+/* ---- Longstaff-Schwartz least-squares Monte Carlo --------------------- */
+/* Instruments priced by LongstaffSchwartzEngine
+ * (src/engines/lsm/longstaff_schwartz.rs:448-735). */
+enum ofg_lsm_instrument {
+    OFG_LSM_VANILLA = 0,    /* VanillaOption (instruments/vanilla.rs) */
+    OFG_LSM_BERMUDAN = 1,   /* BermudanOption (instruments/bermudan.rs) */
+    OFG_LSM_BARRIER = 2     /* BarrierOption: the engine's plain MC pricer */
+};
+/* ExerciseStyle (core/types.rs) of a VanillaOption */
+enum ofg_exercise { OFG_EUROPEAN = 0, OFG_AMERICAN = 1, OFG_BERMUDAN = 2 };
+/* LsmDynamics (longstaff_schwartz.rs:25-39); BermudanOption only -- the
+ * VanillaOption and BarrierOption paths always simulate GBM. LOCAL_VOL uses
+ * the market's flat volatility as the surface. */
+enum ofg_lsm_dynamics { OFG_LSM_GBM = 0, OFG_LSM_LOCAL_VOL = 1, OFG_LSM_HESTON = 2 };
+
+/* One LongstaffSchwartzEngine::price call: instrument + market + engine. */
+typedef struct ofg_lsm_option {
+    int32_t instrument;     /* ofg_lsm_instrument */
+    int32_t option_type;    /* ofg_option_type */
+    int32_t exercise;       /* VanillaOption: ofg_exercise */
+    int32_t dynamics;       /* BermudanOption: ofg_lsm_dynamics */
+    int32_t n_dates;        /* exercise dates (Bermudan exercise / BermudanOption) */
+    int32_t barrier_dir;    /* BarrierOption: ofg_barrier_dir */
+    int32_t barrier_style;  /* BarrierOption: ofg_barrier_style */
+    int32_t pad;
+    double spot, strike, expiry;
+    double rate, div_yield; /* div_yield: Market::effective_dividend_yield(expiry) */
+    double vol;             /* Market::vol_for (flat) */
+    double kappa, theta, xi, rho, v0;   /* Heston dynamics (models/stochastic.rs:37-44) */
+    double barrier_level, rebate;
+    const double *dates;    /* exercise dates in years (host) */
+    const double *strikes;  /* BermudanOption::strike_schedule (host, n_dates) */
+    int64_t num_paths;      /* LongstaffSchwartzEngine::num_paths */
+    int64_t num_steps;      /* LongstaffSchwartzEngine::num_steps */
+    uint64_t seed;          /* LongstaffSchwartzEngine::seed */
+} ofg_lsm_option;
+
+/* ExerciseBoundaryPoint (longstaff_schwartz.rs:42-54). */
+typedef struct ofg_lsm_boundary {
+    double time, strike;
+    double boundary_spot;   /* valid when has_boundary */
+    int32_t has_boundary;
+    int32_t pad;
+    int64_t itm_paths, exercised_paths;
+} ofg_lsm_boundary;
+
 typedef struct ofg_ctx ofg_ctx;
 typedef struct ofg_batch ofg_batch;
 
@@ -208,6 +254,34 @@ void ofg_batch_destroy(ofg_batch *batch);
 int32_t ofg_autotune(ofg_ctx *ctx, const ofg_option *opts, size_t n,
                      const ofg_engine_cfg *cfg, const char *cache_path,
                      int32_t *best_block, int32_t *best_item_steps);
+
+/* Least-squares Monte Carlo: prices n options in one batched sequence of
+ * launches (path simulation, then one launch per exercise-date round).
+ * Replaces LongstaffSchwartzEngine's PricingEngine::price for VanillaOption
+ * (src/engines/lsm/longstaff_schwartz.rs:448-670), BermudanOption and
+ * price_bermudan_with_boundary (:291-417, :672-681), and BarrierOption
+ * (:683-735). cfg supplies rng_mode (REFERENCE: the engine's own single
+ * xoshiro256++ stream, drawn path after path exactly as the reference does;
+ * EXTERNAL: normals1 (and normals2, Heston) per option [step][path], options
+ * concatenated; FAST: per-path xoroshiro128+ Box-Muller streams);
+ * antithetic, control_variate and greek_mode are ignored (the engine has no
+ * variance reduction). out[n]: price, stderr and the diagnostics num_paths,
+ * num_steps, vol (NaN where the reference records no vol). boundary (may be
+ * NULL) holds num_steps slots per option, options in call order; for a
+ * BermudanOption it receives the exercise boundary in chronological order and
+ * boundary_len[i] its length (0 for other instruments); boundary_len may be
+ * NULL only with boundary. */
+int32_t ofg_lsm_price(ofg_ctx *ctx, const ofg_lsm_option *opts, size_t n,
+                      const ofg_engine_cfg *cfg, ofg_result *out,
+                      ofg_lsm_boundary *boundary, int64_t *boundary_len);
+
+/* Positions a xoshiro256++ state k draws ahead (state <- T^k state, T the
+ * generator's GF(2)-linear transition) in O(log k) polynomial steps. No
+ * counterpart in the reference (whose engine steps one draw at a time); it
+ * is how ofg_lsm_price's reference draws split the engine's single stream
+ * across paths, exported for callers that partition a stream the same way.
+ * Host-only, needs no GPU. */
+int32_t ofg_stream_jump(const uint64_t state[4], uint64_t k, uint64_t out[4]);
 
 #ifdef __cplusplus
 }

@@ -6,14 +6,16 @@ ABI, include/openferric_amd.h). This package mirrors the reference's Rust
 PricingEngine contract on top of it.
 """
 from .core import (AsianOption, AsianSpec, Averaging, BarrierDirection, BarrierOption,
-                   BarrierSpec, BarrierStyle, ConvergenceFailure, Diagnostics, ExerciseStyle, Gbm,
+                   BarrierSpec, BarrierStyle, BermudanLsmOutput, BermudanOption,
+                   ExerciseBoundaryPoint, ConvergenceFailure, Diagnostics, ExerciseStyle, Gbm,
                    Greeks, Heston, InvalidInput, Market, MarketDataMissing, NumericalError,
                    OptionType, PricingError, PricingResult, StrikeType, VanillaOption)
 from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEngine, GreekMethod,
                       HestonAadConfig, HestonMonteCarloPricingEngine, RngMode, VarianceReduction,
                       MonteCarloGreeksEngine, autotune, engine_cfg, heston_price_delta_aad,
                       mc_european_parallel, mc_european_pathwise_aad, mc_european_sequential,
-                      mc_european_with_arena,
+                      mc_european_with_arena, LongstaffSchwartzEngine, LsmDynamics,
+                      lsm_option_struct, lsm_price_raw, stream_jump,
                       option_struct, price_raw, sample_payoffs)
 from ._lib import DeviceError
 
@@ -26,5 +28,7 @@ __all__ = [
     "HestonMonteCarloPricingEngine", "RngMode", "VarianceReduction", "autotune", "engine_cfg",
     "heston_price_delta_aad", "mc_european_pathwise_aad", "option_struct", "price_raw",
     "sample_payoffs", "DeviceError", "MonteCarloGreeksEngine", "mc_european_parallel",
-    "mc_european_sequential", "mc_european_with_arena", "GreekMethod",
+    "mc_european_sequential", "mc_european_with_arena", "GreekMethod", "BermudanOption",
+    "BermudanLsmOutput", "ExerciseBoundaryPoint", "LongstaffSchwartzEngine", "LsmDynamics",
+    "lsm_option_struct", "lsm_price_raw", "stream_jump",
 ]

@@ -28,12 +28,15 @@ UP, DOWN = 0, 1
 IN, OUT = 0, 1
 RNG_REFERENCE, RNG_EXTERNAL, RNG_FAST = 0, 1, 2
 GREEKS_NONE, GREEKS_ADJOINT, GREEKS_FD_CRN, GREEKS_FORWARD = 0, 1, 2, 3
+LSM_VANILLA, LSM_BERMUDAN, LSM_BARRIER = 0, 1, 2
+EUROPEAN, AMERICAN, BERMUDAN = 0, 1, 2
+LSM_GBM, LSM_LOCAL_VOL, LSM_HESTON = 0, 1, 2
 
 EXPORTED_SYMBOLS = [
     "ofg_create", "ofg_destroy", "ofg_last_error", "ofg_version", "ofg_price",
     "ofg_sample_payoffs", "ofg_price_greeks", "ofg_batch_create", "ofg_batch_run",
     "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
-    "ofg_mc_greeks_estimate",
+    "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump",
 ]
 
 
@@ -79,6 +82,29 @@ class OfgMcGreeks(C.Structure):
         ("pathwise_delta", C.c_double), ("pathwise_gamma", C.c_double),
         ("lr_delta", C.c_double), ("lr_gamma", C.c_double), ("lr_vega", C.c_double),
         ("num_samples", C.c_double), ("kernel_ms", C.c_double),
+    ]
+
+
+class OfgLsmOption(C.Structure):
+    _fields_ = [
+        ("instrument", C.c_int32), ("option_type", C.c_int32), ("exercise", C.c_int32),
+        ("dynamics", C.c_int32), ("n_dates", C.c_int32), ("barrier_dir", C.c_int32),
+        ("barrier_style", C.c_int32), ("pad", C.c_int32),
+        ("spot", C.c_double), ("strike", C.c_double), ("expiry", C.c_double),
+        ("rate", C.c_double), ("div_yield", C.c_double), ("vol", C.c_double),
+        ("kappa", C.c_double), ("theta", C.c_double), ("xi", C.c_double),
+        ("rho", C.c_double), ("v0", C.c_double),
+        ("barrier_level", C.c_double), ("rebate", C.c_double),
+        ("dates", C.POINTER(C.c_double)), ("strikes", C.POINTER(C.c_double)),
+        ("num_paths", C.c_int64), ("num_steps", C.c_int64), ("seed", C.c_uint64),
+    ]
+
+
+class OfgLsmBoundary(C.Structure):
+    _fields_ = [
+        ("time", C.c_double), ("strike", C.c_double), ("boundary_spot", C.c_double),
+        ("has_boundary", C.c_int32), ("pad", C.c_int32),
+        ("itm_paths", C.c_int64), ("exercised_paths", C.c_int64),
     ]
 
 
@@ -129,6 +155,12 @@ def load() -> C.CDLL:
                                              C.POINTER(OfgEngineCfg), C.c_double,
                                              C.POINTER(OfgMcGreeks)]
         L.ofg_mc_greeks_estimate.restype = C.c_int32
+        L.ofg_lsm_price.argtypes = [pv, C.POINTER(OfgLsmOption), C.c_size_t,
+                                    C.POINTER(OfgEngineCfg), C.POINTER(OfgResult),
+                                    C.POINTER(OfgLsmBoundary), C.POINTER(C.c_int64)]
+        L.ofg_lsm_price.restype = C.c_int32
+        L.ofg_stream_jump.argtypes = [C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(C.c_uint64)]
+        L.ofg_stream_jump.restype = C.c_int32
         _lib = L
         return L
 

@@ -251,6 +251,72 @@ class BarrierOption:
         return "BarrierOption"
 
 
+@dataclass
+class BermudanOption:
+    """instruments/bermudan.rs:18-144: exercise dates with a per-date strike."""
+    option_type: OptionType
+    expiry: float
+    exercise_dates: Sequence[float]
+    strike_schedule: Sequence[float]
+
+    @staticmethod
+    def new(option_type: OptionType, expiry: float, exercise_dates: Sequence[float],
+            strike_schedule: Sequence[float]) -> "BermudanOption":
+        return BermudanOption(option_type, expiry, list(exercise_dates), list(strike_schedule))
+
+    @staticmethod
+    def with_constant_strike(option_type: OptionType, strike: float, expiry: float,
+                             exercise_dates: Sequence[float]) -> "BermudanOption":
+        return BermudanOption(option_type, expiry, list(exercise_dates),
+                              [strike] * len(exercise_dates))
+
+    def num_exercise_dates(self) -> int:
+        return len(self.exercise_dates)
+
+    def effective_schedule(self):
+        """(time, strike) pairs; the expiry with the last strike is appended
+        when the last date falls before it (bermudan.rs:85-103)."""
+        self.validate()
+        out = list(zip(self.exercise_dates, self.strike_schedule))
+        if self.expiry - out[-1][0] > 1.0e-12:
+            out.append((self.expiry, out[-1][1]))
+        return out
+
+    def validate(self) -> None:
+        if not (self.expiry > 0.0 and math.isfinite(self.expiry)):
+            raise InvalidInput("bermudan expiry must be finite and > 0")
+        if not self.exercise_dates:
+            raise InvalidInput("bermudan exercise_dates cannot be empty")
+        if len(self.exercise_dates) != len(self.strike_schedule):
+            raise InvalidInput("bermudan strike_schedule must match exercise_dates length")
+        if any(not math.isfinite(t) or t <= 0.0 or t > self.expiry for t in self.exercise_dates):
+            raise InvalidInput("bermudan exercise_dates must be finite and in (0, expiry]")
+        if any(b <= a for a, b in zip(self.exercise_dates, self.exercise_dates[1:])):
+            raise InvalidInput("bermudan exercise_dates must be strictly increasing")
+        if any(not math.isfinite(k) or k <= 0.0 for k in self.strike_schedule):
+            raise InvalidInput("bermudan strike_schedule entries must be finite and > 0")
+
+    def instrument_type(self) -> str:
+        return "BermudanOption"
+
+
+@dataclass
+class ExerciseBoundaryPoint:
+    """engines/lsm/longstaff_schwartz.rs:42-54"""
+    time: float
+    strike: float
+    boundary_spot: Optional[float]
+    itm_paths: int
+    exercised_paths: int
+
+
+@dataclass
+class BermudanLsmOutput:
+    """engines/lsm/longstaff_schwartz.rs:57-63"""
+    result: PricingResult
+    exercise_boundary: List[ExerciseBoundaryPoint]
+
+
 class BarrierOptionBuilder:
     """instruments/barrier.rs:66-178"""
 

@@ -13,6 +13,7 @@
 #include "ofg_adjoint.hpp"
 #include "ofg_mcgreeks.hpp"
 #include "ofg_pathwise.hpp"
+#include "ofg_host.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -32,38 +33,9 @@
 #include <string>
 #include <vector>
 
+using namespace ofg_host;
+
 namespace {
-
-thread_local std::string g_last_error;
-
-// OFG_TRACE=1 prints each host-side stage to stderr (flushed), so a run that
-// stalls shows where.
-bool trace_on() {
-    static const bool on = [] {
-        const char *e = std::getenv("OFG_TRACE");
-        return e && e[0] == '1';
-    }();
-    return on;
-}
-void trace(const char *what) {
-    if (trace_on()) {
-        std::fprintf(stderr, "[ofg] %s\n", what);
-        std::fflush(stderr);
-    }
-}
-
-struct OfgError : std::runtime_error {
-    int32_t code;
-    OfgError(int32_t c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-[[noreturn]] void fail(int32_t code, const std::string &msg) { throw OfgError(code, msg); }
-
-void hip_check(hipError_t e, const char *what) {
-    if (e != hipSuccess) {
-        fail(OFG_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Closed forms used by the built-in control variates. These restate the
@@ -135,10 +107,12 @@ double geometric_asian_discrete(int32_t type, double s0, double k, double r, dou
                             : df * (k * hart_norm_cdf(-d2) - eg * hart_norm_cdf(-d1));
 }
 
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Validation (reference error strings)
 // ---------------------------------------------------------------------------
-void validate_instrument(const ofg_option &o) {
+void ofg_host::validate_instrument(const ofg_option &o) {
     switch (o.payoff) {
     case OFG_PAYOFF_VANILLA:  // instruments/vanilla.rs:69-96
         if (!(o.strike > 0.0)) fail(OFG_INVALID_INPUT, "vanilla strike must be > 0");
@@ -175,6 +149,8 @@ void validate_instrument(const ofg_option &o) {
         fail(OFG_INVALID_INPUT, "unknown option type");
 }
 
+namespace {
+
 // Heston::validate (src/models/stochastic.rs:47-54)
 bool heston_valid(const ofg_option &o) {
     return o.kappa > 0.0 && o.theta >= 0.0 && o.xi >= 0.0 && o.v0 >= 0.0 && o.rho > -1.0 &&
@@ -197,30 +173,9 @@ double payoff_at_spot(const ofg_option &o) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Device buffers
+// Launch plans
 // ---------------------------------------------------------------------------
 namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    void ensure(size_t bytes) {
-        if (bytes <= cap) return;
-        if (p) hip_check(hipFree(p), "hipFree");
-        p = nullptr;
-        cap = 0;
-        const size_t want = std::max<size_t>(bytes, 256);
-        hip_check(hipMalloc(&p, want), "hipMalloc");
-        cap = want;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T>
-    T *as() const { return static_cast<T *>(p); }
-};
 
 // Host-side plan of one pricing launch.
 struct Plan {
@@ -240,23 +195,8 @@ struct Plan {
     int32_t model_end[2] = {0, 0};
 };
 
-struct Tuning {
-    int32_t block = 256;
-    int32_t item_steps = 8192;
-};
-
 }  // namespace
 
-struct ofg_ctx {
-    int32_t device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    std::mutex mu;
-    DevBuf opts, obs_w, items, order, item_first, item_count, partials, outs, head, xs, ys, n1,
-        n2, adj;
-    std::map<std::string, Tuning> tuned;
-    int cus = 256;
-};
 
 struct ofg_batch {
     ofg_ctx *ctx = nullptr;
@@ -617,22 +557,6 @@ double run_plan(ofg_ctx *ctx, const Plan &p, const ofg_engine_cfg &cfg, const Tu
     return ms_all;
 }
 
-template <class F>
-int32_t guarded(F &&f) {
-    try {
-        g_last_error.clear();
-        return f();
-    } catch (const OfgError &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return OFG_NUMERICAL_ERROR;
-    } catch (...) {
-        g_last_error = "unknown error";
-        return OFG_NUMERICAL_ERROR;
-    }
-}
 
 int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
                    ofg_result *out);
@@ -1048,6 +972,7 @@ void ofg_destroy(ofg_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    lsm_release(ctx);
     for (DevBuf *b : {&ctx->opts, &ctx->obs_w, &ctx->items, &ctx->order, &ctx->item_first,
                       &ctx->item_count, &ctx->partials, &ctx->outs, &ctx->head, &ctx->xs, &ctx->ys,
                       &ctx->n1, &ctx->n2, &ctx->adj})

@@ -1,0 +1,118 @@
+// Host-side pieces shared by the C ABI translation units: error handling
+// (status codes + thread-local message), device buffers and the per-GPU
+// context object.
+#pragma once
+
+#include "../../include/openferric_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+namespace ofg_host {
+
+inline thread_local std::string g_last_error;
+
+// OFG_TRACE=1 prints each host-side stage to stderr (flushed), so a run that
+// stalls shows where.
+inline bool trace_on() {
+    static const bool on = [] {
+        const char *e = std::getenv("OFG_TRACE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+inline void trace(const char *what) {
+    if (trace_on()) {
+        std::fprintf(stderr, "[ofg] %s\n", what);
+        std::fflush(stderr);
+    }
+}
+
+struct OfgError : std::runtime_error {
+    int32_t code;
+    OfgError(int32_t c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int32_t code, const std::string &msg) { throw OfgError(code, msg); }
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        fail(OFG_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t bytes) {
+        if (bytes <= cap) return;
+        if (p) hip_check(hipFree(p), "hipFree");
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes, 256);
+        hip_check(hipMalloc(&p, want), "hipMalloc");
+        cap = want;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+
+struct Tuning {
+    int32_t block = 256;
+    int32_t item_steps = 8192;
+};
+
+template <class F>
+inline int32_t guarded(F &&f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const OfgError &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return OFG_NUMERICAL_ERROR;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return OFG_NUMERICAL_ERROR;
+    }
+}
+
+struct LsmBuffers;   // ofg_lsm_api.cpp
+
+// Instrument validation with the reference's messages (instruments/*.rs
+// validate()); throws OfgError.
+void validate_instrument(const ofg_option &o);
+
+}  // namespace ofg_host
+
+struct ofg_ctx {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    std::mutex mu;
+    ofg_host::DevBuf opts, obs_w, items, order, item_first, item_count, partials, outs, head, xs, ys, n1,
+        n2, adj;
+    std::map<std::string, ofg_host::Tuning> tuned;
+    ofg_host::LsmBuffers *lsm = nullptr;   // Longstaff-Schwartz work buffers
+    int cus = 256;
+};
+
+namespace ofg_host {
+// Releases the context's Longstaff-Schwartz buffers (ofg_lsm_api.cpp).
+void lsm_release(ofg_ctx *ctx);
+}  // namespace ofg_host

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import enum
+import math
 from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
 
@@ -17,9 +18,10 @@ import numpy as np
 
 from . import _lib
 from .core import (AsianOption, Averaging, BarrierDirection, BarrierOption, BarrierStyle,
-                   ConvergenceFailure, Diagnostics, ExerciseStyle, Greeks, Heston, InvalidInput,
-                   Market, MarketDataMissing, NumericalError, OptionType, PricingResult,
-                   StrikeType, VanillaOption)
+                   BermudanLsmOutput, BermudanOption, ConvergenceFailure, Diagnostics,
+                   ExerciseBoundaryPoint, ExerciseStyle, Greeks, Heston, InvalidInput, Market,
+                   MarketDataMissing, NumericalError, OptionType, PricingResult, StrikeType,
+                   VanillaOption)
 
 
 class VarianceReduction(enum.Enum):
@@ -529,3 +531,178 @@ def autotune(opts: Sequence[_lib.OfgOption], cfg: _lib.OfgEngineCfg, cache_path:
     _raise(L.ofg_autotune(ctx, arr, len(opts), C.byref(cfg),
                           cache_path.encode() if cache_path else None, C.byref(b), C.byref(it)))
     return b.value, it.value
+
+
+# ------------------------------------------------------- Longstaff-Schwartz ---
+@dataclass(frozen=True)
+class LsmDynamics:
+    """LsmDynamics (engines/lsm/longstaff_schwartz.rs:25-39). `kind` is one of
+    "gbm", "local_vol" (the market's flat vol as the surface), "heston"."""
+    kind: str = "gbm"
+    kappa: float = 0.0
+    theta: float = 0.0
+    xi: float = 0.0
+    rho: float = 0.0
+    v0: float = 0.0
+
+
+_EXERCISE = {ExerciseStyle.European: _lib.EUROPEAN, ExerciseStyle.American: _lib.AMERICAN,
+             ExerciseStyle.Bermudan: _lib.BERMUDAN}
+_DYNAMICS = {"gbm": _lib.LSM_GBM, "local_vol": _lib.LSM_LOCAL_VOL, "heston": _lib.LSM_HESTON}
+
+
+def lsm_option_struct(instrument, market: Market, num_paths: int, num_steps: int, seed: int,
+                      keep: _Keep, dynamics: LsmDynamics = LsmDynamics()) -> _lib.OfgLsmOption:
+    """Flattens (instrument, market, engine) into the C ABI's ofg_lsm_option."""
+    o = _lib.OfgLsmOption()
+    o.option_type = _lib.CALL if instrument.option_type is OptionType.Call else _lib.PUT
+    o.spot, o.expiry, o.rate = market.spot, instrument.expiry, market.rate
+    o.div_yield = market.effective_dividend_yield(instrument.expiry)
+    if isinstance(instrument, VanillaOption):
+        o.instrument = _lib.LSM_VANILLA
+        o.exercise = _EXERCISE[instrument.exercise]
+        o.strike = instrument.strike
+        o.vol = market.vol_for(instrument.strike, instrument.expiry)
+        dates = list(instrument.bermudan_dates) if instrument.exercise is ExerciseStyle.Bermudan \
+            else []
+        o.n_dates = len(dates)
+        o.dates = keep.arr(dates) if dates else None
+    elif isinstance(instrument, BermudanOption):
+        o.instrument = _lib.LSM_BERMUDAN
+        o.dynamics = _DYNAMICS[dynamics.kind]
+        dates = list(instrument.exercise_dates)
+        strikes = list(instrument.strike_schedule)
+        o.n_dates = len(dates)
+        o.dates = keep.arr(dates) if dates else None
+        o.strikes = keep.arr(strikes) if strikes else None
+        o.strike = strikes[-1] if strikes else 0.0
+        o.vol = market.vol_for(o.strike, instrument.expiry)
+        o.kappa, o.theta, o.xi, o.rho, o.v0 = dynamics.kappa, dynamics.theta, dynamics.xi, \
+            dynamics.rho, dynamics.v0
+    elif isinstance(instrument, BarrierOption):
+        o.instrument = _lib.LSM_BARRIER
+        o.strike = instrument.strike
+        o.vol = market.vol_for(instrument.strike, instrument.expiry)
+        o.barrier_dir = _lib.UP if instrument.barrier.direction is BarrierDirection.Up \
+            else _lib.DOWN
+        o.barrier_style = _lib.IN if instrument.barrier.style is BarrierStyle.In else _lib.OUT
+        o.barrier_level = instrument.barrier.level
+        o.rebate = instrument.barrier.rebate
+    else:
+        raise InvalidInput(f"unsupported instrument {type(instrument).__name__}")
+    o.num_paths, o.num_steps, o.seed = int(num_paths), int(num_steps), int(seed) & (2**64 - 1)
+    o._keep = keep
+    return o
+
+
+def lsm_price_raw(opts: Sequence[_lib.OfgLsmOption], cfg: _lib.OfgEngineCfg, device: int = 0,
+                  want_boundary: bool = False):
+    """Calls ofg_lsm_price. Returns the results, and with want_boundary also
+    one list of boundary structs per option (empty unless BermudanOption)."""
+    L = _lib.load()
+    ctx = _lib.context(device)
+    n = len(opts)
+    arr = (_lib.OfgLsmOption * n)(*opts)
+    out = (_lib.OfgResult * n)()
+    bnd = blen = None
+    if want_boundary:
+        slots = sum(max(int(o.num_steps), 0) for o in opts)
+        bnd = (_lib.OfgLsmBoundary * max(slots, 1))()
+        blen = (C.c_int64 * max(n, 1))()
+    _raise(L.ofg_lsm_price(ctx, arr, n, C.byref(cfg), out, bnd, blen))
+    if not want_boundary:
+        return list(out)
+    per, slot = [], 0
+    for i, o in enumerate(opts):
+        per.append([bnd[slot + k] for k in range(blen[i])])
+        slot += max(int(o.num_steps), 0)
+    return list(out), per
+
+
+def _lsm_result(r: _lib.OfgResult, instrument, dynamics: LsmDynamics) -> PricingResult:
+    diag = Diagnostics()
+    if not math.isnan(r.num_paths):
+        diag.insert("num_paths", r.num_paths)
+        diag.insert("num_steps", r.num_steps)
+        if isinstance(instrument, BermudanOption):
+            diag.insert("exercise_dates", float(len(instrument.effective_schedule())))
+        if not math.isnan(r.vol):
+            diag.insert("vol", r.vol)
+    return PricingResult(price=r.price, stderr=r.stderr_, diagnostics=diag, kernel_ms=r.kernel_ms)
+
+
+class LongstaffSchwartzEngine:
+    """LongstaffSchwartzEngine (engines/lsm/longstaff_schwartz.rs:66-112)
+    on the GPU: VanillaOption (European / American / Bermudan exercise),
+    BermudanOption with GBM, flat local-vol or Heston dynamics and its
+    exercise boundary, and BarrierOption by plain Monte Carlo. The default
+    Reference RNG mode draws the reference engine's own single stream."""
+
+    def __init__(self, num_paths: int, num_steps: int, seed: int):
+        self.num_paths = num_paths
+        self.num_steps = num_steps
+        self.seed = seed
+        self.dynamics = LsmDynamics()
+        self.rng_mode = RngMode.Reference
+        self.device = 0
+
+    @staticmethod
+    def new(num_paths: int, num_steps: int, seed: int) -> "LongstaffSchwartzEngine":
+        return LongstaffSchwartzEngine(num_paths, num_steps, seed)
+
+    def with_local_vol_dynamics(self) -> "LongstaffSchwartzEngine":
+        self.dynamics = LsmDynamics("local_vol")
+        return self
+
+    def with_heston_dynamics(self, model: Heston) -> "LongstaffSchwartzEngine":
+        """The spot drift is the market's r - q; the model's mu is ignored
+        (longstaff_schwartz.rs:98-111)."""
+        self.dynamics = LsmDynamics("heston", model.kappa, model.theta, model.xi, model.rho,
+                                    model.v0)
+        return self
+
+    def with_rng_mode(self, mode: RngMode) -> "LongstaffSchwartzEngine":
+        self.rng_mode = mode
+        return self
+
+    def with_device(self, device: int) -> "LongstaffSchwartzEngine":
+        self.device = device
+        return self
+
+    def _structs(self, instruments, markets, keep):
+        return [lsm_option_struct(ins, mk, self.num_paths, self.num_steps, self.seed, keep,
+                                  self.dynamics) for ins, mk in zip(instruments, markets)]
+
+    def price(self, instrument, market: Market) -> PricingResult:
+        return self.price_batch([instrument], market)[0]
+
+    def price_batch(self, instruments, market) -> List[PricingResult]:
+        """Prices several instruments in one batched GPU sequence (one market
+        for all, or one per instrument)."""
+        markets = market if isinstance(market, (list, tuple)) else [market] * len(instruments)
+        keep = _Keep()
+        opts = self._structs(instruments, markets, keep)
+        res = lsm_price_raw(opts, engine_cfg(rng_mode=self.rng_mode), self.device)
+        return [_lsm_result(r, ins, self.dynamics) for r, ins in zip(res, instruments)]
+
+    def price_bermudan_with_boundary(self, instrument: BermudanOption,
+                                     market: Market) -> BermudanLsmOutput:
+        keep = _Keep()
+        opts = self._structs([instrument], [market], keep)
+        res, bnd = lsm_price_raw(opts, engine_cfg(rng_mode=self.rng_mode), self.device,
+                                 want_boundary=True)
+        pts = [ExerciseBoundaryPoint(time=b.time, strike=b.strike,
+                                     boundary_spot=b.boundary_spot if b.has_boundary else None,
+                                     itm_paths=int(b.itm_paths),
+                                     exercised_paths=int(b.exercised_paths)) for b in bnd[0]]
+        return BermudanLsmOutput(result=_lsm_result(res[0], instrument, self.dynamics),
+                                 exercise_boundary=pts)
+
+
+def stream_jump(state: Sequence[int], k: int) -> List[int]:
+    """ofg_stream_jump: the xoshiro256++ state k draws ahead (host-only)."""
+    L = _lib.load()
+    src = (C.c_uint64 * 4)(*[int(x) & (2**64 - 1) for x in state])
+    dst = (C.c_uint64 * 4)()
+    _raise(L.ofg_stream_jump(src, int(k) & (2**64 - 1), dst))
+    return list(dst)

@@ -54,6 +54,11 @@ uint64_t or_xoshiro_next_u64(uint64_t s[4]) {
     return result;
 }
 
+/* k successive next_u64 calls (brute-force stepping for jump checks) */
+void or_xoshiro_advance(uint64_t s[4], uint64_t k) {
+    for (uint64_t i = 0; i < k; ++i) (void)or_xoshiro_next_u64(s);
+}
+
 /* Xoshiro256PlusPlus::next_f64, fast_rng.rs:69-73 */
 double or_xoshiro_next_f64(uint64_t s[4]) {
     const uint64_t x = or_xoshiro_next_u64(s) >> 11;

@@ -43,6 +43,7 @@ uint64_t or_splitmix64_next(uint64_t *state);
 void or_xoshiro_seed(uint64_t seed, uint64_t s[4]);
 uint64_t or_xoshiro_next_u64(uint64_t s[4]);
 double or_xoshiro_next_f64(uint64_t s[4]);
+void or_xoshiro_advance(uint64_t s[4], uint64_t k);
 uint64_t or_stream_seed(uint64_t base, uint64_t index);
 double or_uniform_open01(double u);
 double or_acklam_inv_cdf(double p);
@@ -144,6 +145,46 @@ int or_heston_price_delta_aad(const or_option *o, int64_t num_paths,
  * Returns 0 on success, 1 on invalid input. */
 int or_mc_greeks_estimate(const or_option *o, int64_t num_paths, uint64_t seed,
                           int32_t antithetic, double spot_bump_rel, double out[5]);
+
+/* ---- Longstaff-Schwartz (oracle_lsm.c) ------------------------------- */
+enum { OR_LSM_VANILLA = 0, OR_LSM_BERMUDAN = 1, OR_LSM_BARRIER = 2 };
+enum { OR_EUROPEAN = 0, OR_AMERICAN = 1, OR_BERMUDAN = 2 };
+enum { OR_LSM_GBM = 0, OR_LSM_LOCAL_VOL = 1, OR_LSM_HESTON = 2 };
+
+typedef struct {
+    int32_t instrument;    /* OR_LSM_VANILLA | OR_LSM_BERMUDAN | OR_LSM_BARRIER */
+    int32_t option_type;   /* OR_CALL | OR_PUT */
+    int32_t exercise;      /* VanillaOption: OR_EUROPEAN | OR_AMERICAN | OR_BERMUDAN */
+    int32_t dynamics;      /* BermudanOption: OR_LSM_GBM | _LOCAL_VOL (flat) | _HESTON */
+    int32_t n_dates;       /* exercise dates (and strikes for BermudanOption) */
+    int32_t barrier_dir, barrier_style, pad;
+    double spot, strike, expiry, rate, div_yield, vol;   /* div_yield: effective */
+    double kappa, theta, xi, rho, v0;
+    double barrier_level, rebate;
+    const double *dates, *strikes;
+    int64_t num_paths, num_steps;
+    uint64_t seed;
+} or_lsm_option;
+
+/* ExerciseBoundaryPoint (longstaff_schwartz.rs:42-54) */
+typedef struct {
+    double time, strike, boundary_spot;
+    int32_t has_boundary, pad;
+    int64_t itm_paths, exercised_paths;
+} or_lsm_boundary;
+
+/* nalgebra LU (partial pivoting) solve of a 3x3 system, row-major a.
+ * Returns 0 when U has an exactly zero pivot (the reference then uses 0). */
+int or_lu3_solve(const double a[9], const double b[3], double x[3]);
+
+/* LongstaffSchwartzEngine::price for VanillaOption (:448-670), BermudanOption
+ * (price_bermudan_with_boundary, :291-417) and BarrierOption (:674-735).
+ * normals (may be NULL) replace the stream's draws in its own order (per
+ * path, per step, z1 then z2 for Heston). boundary (capacity num_steps, may
+ * be NULL) receives the exercise boundary in chronological order for
+ * BermudanOption; *n_boundary its length. Returns 0, 1 invalid input. */
+int or_lsm_price(const or_lsm_option *o, const double *normals, double *price,
+                 double *stderr_out, or_lsm_boundary *boundary, int64_t *n_boundary);
 
 #ifdef __cplusplus
 }

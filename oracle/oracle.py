@@ -46,6 +46,34 @@ class OrCfg(C.Structure):
     ]
 
 
+LSM_VANILLA, LSM_BERMUDAN, LSM_BARRIER = 0, 1, 2
+EUROPEAN, AMERICAN, BERMUDAN = 0, 1, 2
+LSM_GBM, LSM_LOCAL_VOL, LSM_HESTON = 0, 1, 2
+
+
+class OrLsmOption(C.Structure):
+    _fields_ = [
+        ("instrument", C.c_int32), ("option_type", C.c_int32), ("exercise", C.c_int32),
+        ("dynamics", C.c_int32), ("n_dates", C.c_int32), ("barrier_dir", C.c_int32),
+        ("barrier_style", C.c_int32), ("pad", C.c_int32),
+        ("spot", C.c_double), ("strike", C.c_double), ("expiry", C.c_double),
+        ("rate", C.c_double), ("div_yield", C.c_double), ("vol", C.c_double),
+        ("kappa", C.c_double), ("theta", C.c_double), ("xi", C.c_double),
+        ("rho", C.c_double), ("v0", C.c_double),
+        ("barrier_level", C.c_double), ("rebate", C.c_double),
+        ("dates", C.POINTER(C.c_double)), ("strikes", C.POINTER(C.c_double)),
+        ("num_paths", C.c_int64), ("num_steps", C.c_int64), ("seed", C.c_uint64),
+    ]
+
+
+class OrLsmBoundary(C.Structure):
+    _fields_ = [
+        ("time", C.c_double), ("strike", C.c_double), ("boundary_spot", C.c_double),
+        ("has_boundary", C.c_int32), ("pad", C.c_int32),
+        ("itm_paths", C.c_int64), ("exercised_paths", C.c_int64),
+    ]
+
+
 def build(force: bool = False) -> str:
     """Compile liboracle.so with make (gcc); returns the library path."""
     if force or not os.path.exists(_LIB_PATH):
@@ -70,6 +98,7 @@ def lib():
         L.or_xoshiro_seed.argtypes = [u64, pu64]; L.or_xoshiro_seed.restype = None
         L.or_xoshiro_next_u64.argtypes = [pu64]; L.or_xoshiro_next_u64.restype = u64
         L.or_xoshiro_next_f64.argtypes = [pu64]; L.or_xoshiro_next_f64.restype = d
+        L.or_xoshiro_advance.argtypes = [pu64, u64]; L.or_xoshiro_advance.restype = None
         L.or_stream_seed.argtypes = [u64, u64]; L.or_stream_seed.restype = u64
         L.or_uniform_open01.argtypes = [d]; L.or_uniform_open01.restype = d
         L.or_acklam_inv_cdf.argtypes = [d]; L.or_acklam_inv_cdf.restype = d
@@ -99,6 +128,10 @@ def lib():
         L.or_heston_price_delta_aad.restype = C.c_int
         L.or_mc_greeks_estimate.argtypes = [C.POINTER(OrOption), i64, u64, i32, d, pd]
         L.or_mc_greeks_estimate.restype = C.c_int
+        L.or_lu3_solve.argtypes = [pd, pd, pd]; L.or_lu3_solve.restype = C.c_int
+        L.or_lsm_price.argtypes = [C.POINTER(OrLsmOption), pd, pd, pd,
+                                   C.POINTER(OrLsmBoundary), C.POINTER(i64)]
+        L.or_lsm_price.restype = C.c_int
         _lib = L
     return _lib
 
@@ -226,3 +259,72 @@ def mc_greeks_estimate(o, num_paths, seed, antithetic=True, spot_bump_rel=1e-2):
     if rc:
         raise ValueError("invalid input")
     return dict(zip(["pw_delta", "pw_gamma", "lr_delta", "lr_gamma", "lr_vega"], out.tolist()))
+
+
+def make_lsm_option(instrument=LSM_VANILLA, option_type=PUT, exercise=AMERICAN,
+                    dynamics=LSM_GBM, spot=36.0, strike=40.0, expiry=1.0, rate=0.06,
+                    div_yield=0.0, vol=0.2, kappa=0.0, theta=0.0, xi=0.0, rho=0.0, v0=0.0,
+                    barrier_dir=UP, barrier_style=OUT, barrier_level=0.0, rebate=0.0,
+                    dates=None, strikes=None, num_paths=1000, num_steps=50, seed=42):
+    """Builds an OrLsmOption; the returned object keeps dates/strikes alive."""
+    o = OrLsmOption()
+    o.instrument, o.option_type, o.exercise, o.dynamics = instrument, option_type, exercise, dynamics
+    o.barrier_dir, o.barrier_style = barrier_dir, barrier_style
+    o.spot, o.strike, o.expiry, o.rate, o.div_yield, o.vol = spot, strike, expiry, rate, div_yield, vol
+    o.kappa, o.theta, o.xi, o.rho, o.v0 = kappa, theta, xi, rho, v0
+    o.barrier_level, o.rebate = barrier_level, rebate
+    o.num_paths, o.num_steps, o.seed = num_paths, num_steps, seed
+    d = np.ascontiguousarray(dates if dates is not None else [], dtype=np.float64)
+    k = np.ascontiguousarray(strikes if strikes is not None else [], dtype=np.float64)
+    o._keep = (d, k)
+    o.dates, o.strikes = _ptr(d), _ptr(k)
+    o.n_dates = len(d)
+    return o
+
+
+def lu3_solve(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(3)
+    ok = lib().or_lu3_solve(_ptr(a), _ptr(b), _ptr(x))
+    return x if ok else None
+
+
+def lsm_price(o, normals=None, want_boundary=False):
+    """LongstaffSchwartzEngine::price. normals, when given, replace the
+    stream's draws in its order ([path][step], Heston: [path][step][z1, z2]).
+    Returns (price, stderr) or (price, stderr, boundary list of dicts)."""
+    z = np.ascontiguousarray(normals, dtype=np.float64).ravel() if normals is not None else None
+    price, se = C.c_double(), C.c_double()
+    cap = max(int(o.num_steps), 1)
+    bnd = (OrLsmBoundary * cap)() if want_boundary else None
+    nb = C.c_int64(0)
+    rc = lib().or_lsm_price(C.byref(o), _ptr(z), C.byref(price), C.byref(se), bnd, C.byref(nb))
+    if rc != 0:
+        raise ValueError(f"or_lsm_price failed rc={rc}")
+    if not want_boundary:
+        return price.value, se.value
+    pts = [dict(time=b.time, strike=b.strike,
+                boundary_spot=b.boundary_spot if b.has_boundary else None,
+                itm_paths=b.itm_paths, exercised_paths=b.exercised_paths)
+           for b in bnd[:nb.value]]
+    return price.value, se.value, pts
+
+
+def lsm_stream_normals(seed, count):
+    """The first `count` draws of the engine's single stream (Acklam of the
+    clamped uniform of xoshiro256++ seeded by seed_from_u64(seed))."""
+    s = (C.c_uint64 * 4)()
+    L = lib()
+    L.or_xoshiro_seed(seed, s)
+    out = np.empty(count)
+    for i in range(count):
+        out[i] = L.or_acklam_inv_cdf(L.or_uniform_open01(L.or_xoshiro_next_f64(s)))
+    return out
+
+
+def xoshiro_advance(state, k):
+    """The xoshiro256++ state after k draws, by stepping."""
+    s = (C.c_uint64 * 4)(*[int(x) & (2**64 - 1) for x in state])
+    lib().or_xoshiro_advance(s, int(k))
+    return list(s)
