@@ -234,6 +234,94 @@ def heston_greeks_leg(rng_mode):
             "forward_vs_adjoint_max_rel_diff": max(rel)}
 
 
+# Longstaff & Schwartz (2001) Table 1: American puts, K = 40, r = 6 %, 50
+# exercise steps (tests/lsm_reference.rs of the reference)
+LSM_TABLE1 = [(s, v, t) for s in (36.0, 38.0, 40.0, 42.0, 44.0) for v in (0.2, 0.4)
+              for t in (1.0, 2.0)]
+LSM_STEPS = 50
+LSM_ROUND_BYTES = 32      # per path and backward round: spots at two dates + value in/out
+LSM_FINAL_BYTES = 16      # last round: spot at the first date + value in
+
+
+def lsm_structs(paths):
+    from openferric_amd import LongstaffSchwartzEngine, Market, VanillaOption
+    from openferric_amd.engines import _Keep, lsm_option_struct
+    keep = _Keep()
+    out = []
+    for s, v, t in LSM_TABLE1:
+        m = Market.builder().spot(s).rate(0.06).flat_vol(v).build()
+        out.append(lsm_option_struct(VanillaOption.american_put(40.0, t), m, paths, LSM_STEPS,
+                                     42, keep))
+    return out, keep
+
+
+def lsm_leg(paths=1 << 20, reps=3, cpu=True):
+    """Batched Longstaff-Schwartz: the 20 Table 1 American puts, 50 steps."""
+    from openferric_amd import RngMode, engine_cfg, lsm_last_timing, lsm_price_raw
+    structs, keep = lsm_structs(paths)
+    n_ex = LSM_STEPS - 1
+    round_bytes = len(structs) * paths * (LSM_ROUND_BYTES * (n_ex - 1) + LSM_FINAL_BYTES)
+    ps = len(structs) * paths * LSM_STEPS
+    out = {"workload": f"20 American puts (Longstaff-Schwartz Table 1), {paths} paths x "
+                       f"{LSM_STEPS} steps each, one batched call", "path_steps": ps,
+           "round_launches": n_ex, "round_bytes": round_bytes}
+    for name, mode in (("fast", RngMode.Fast), ("reference", RngMode.Reference)):
+        cfg = engine_cfg(rng_mode=mode)
+        lsm_price_raw(structs, cfg)   # warm-up (allocations)
+        tot, pth, rnd = [], [], []
+        for _ in range(reps):
+            res = lsm_price_raw(structs, cfg)
+            t = lsm_last_timing()
+            tot.append(res[0].kernel_ms)
+            pth.append(t["paths_ms"])
+            rnd.append(t["rounds_ms"])
+        k, pm, rm = float(np.mean(tot)), float(np.mean(pth)), float(np.mean(rnd))
+        gbs = round_bytes / (rm * 1e-3) / 1e9
+        out[name] = {"kernel_ms": k, "paths_ms": pm, "rounds_ms": rm,
+                     "options_per_s": len(structs) / (k * 1e-3), "path_steps_per_s": ps / (k * 1e-3),
+                     "path_kernel_path_steps_per_s": ps / (pm * 1e-3),
+                     "rounds_achieved_gb_s": gbs, "rounds_frac_hbm_peak": gbs / 8000.0,
+                     "prices_first3": [r.price for r in res[:3]]}
+    # the reference test's own size (tests/lsm_reference.rs: 100k paths, seed
+    # 42), reference draws: the same paths the reference engine simulates
+    small, keep_small = lsm_structs(100_000)
+    cfg = engine_cfg(rng_mode=RngMode.Reference)
+    lsm_price_raw(small, cfg)
+    ms = [lsm_price_raw(small, cfg)[0].kernel_ms for _ in range(reps)]
+    out["table1_reference_size"] = {
+        "workload": "the 20 Table 1 puts at 100000 paths x 50 steps, reference draws",
+        "kernel_ms": float(np.mean(ms)),
+        "path_steps_per_s": len(small) * 100_000 * LSM_STEPS / (float(np.mean(ms)) * 1e-3)}
+    if cpu:
+        out["cpu_baseline"] = lsm_cpu_baseline()
+        out["speedup_vs_cpu_reference_draws"] = (
+            out["reference"]["path_steps_per_s"] / out["cpu_baseline"]["value"])
+    return out
+
+
+def lsm_cpu_baseline(paths=100_000):
+    """The oracle's LongstaffSchwartzEngine restatement (sequential, as the
+    reference) on Table 1 options at the reference test's size, one option
+    per host thread."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    O.build()
+    threads = cpu_threads()
+    cases = [LSM_TABLE1[i % len(LSM_TABLE1)] for i in range(4 * threads)]
+    structs = [O.make_lsm_option(spot=s, strike=40.0, vol=v, expiry=t, rate=0.06,
+                                 num_paths=paths, num_steps=LSM_STEPS, seed=42)
+               for s, v, t in cases]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(O.lsm_price, structs))
+    dt = time.perf_counter() - t0
+    ps = len(structs) * paths * LSM_STEPS
+    return {"value": ps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{len(structs)} Table 1 American puts at {paths} paths x {LSM_STEPS} "
+                      f"steps, {threads} threads (one option each at a time), {dt:.2f} s, "
+                      f"oracle C restatement"}
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (a plain copy kernel of known
     size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -313,6 +401,9 @@ def main():
     ap.add_argument("--heston", action="store_true", help="add the Heston barrier pricing leg")
     ap.add_argument("--heston-greeks", action="store_true",
                     help="add the batched Heston adjoint leg")
+    ap.add_argument("--no-lsm", action="store_true", help="skip the Longstaff-Schwartz leg")
+    ap.add_argument("--lsm-only", action="store_true",
+                    help="run only the Longstaff-Schwartz leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one D2D copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -326,6 +417,10 @@ def main():
 
     from openferric_amd import BatchPricer, autotune, engine_cfg, RngMode
     from openferric_amd import _lib
+
+    if args.lsm_only:
+        print(json.dumps({"lsm": lsm_leg(cpu=not args.no_cpu)}), flush=True)
+        return
 
     opts = make_workload(path_scale=args.path_scale)
     structs, keeps = zip(*[to_ofg(d) for d in opts])
@@ -398,6 +493,9 @@ def main():
     if args.heston_greeks:
         print("[bench] heston greeks leg", file=sys.stderr, flush=True)
         out["heston_greeks"] = heston_greeks_leg(RngMode.Fast)
+    if not args.no_lsm:
+        print("[bench] longstaff-schwartz leg", file=sys.stderr, flush=True)
+        out["lsm"] = lsm_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(opts)

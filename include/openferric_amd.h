@@ -275,6 +275,13 @@ int32_t ofg_lsm_price(ofg_ctx *ctx, const ofg_lsm_option *opts, size_t n,
                       const ofg_engine_cfg *cfg, ofg_result *out,
                       ofg_lsm_boundary *boundary, int64_t *boundary_len);
 
+/* Device time of the context's last ofg_lsm_price call, split into path
+ * simulation (seed and path kernels) and the backward rounds, and the number
+ * of round launches. No counterpart in the reference; for measurement.
+ * Any output pointer may be NULL. */
+int32_t ofg_lsm_last_timing(ofg_ctx *ctx, double *paths_ms, double *rounds_ms,
+                            int32_t *rounds);
+
 /* Positions a xoshiro256++ state k draws ahead (state <- T^k state, T the
  * generator's GF(2)-linear transition) in O(log k) polynomial steps. No
  * counterpart in the reference (whose engine steps one draw at a time); it

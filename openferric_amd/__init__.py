@@ -15,7 +15,7 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       MonteCarloGreeksEngine, autotune, engine_cfg, heston_price_delta_aad,
                       mc_european_parallel, mc_european_pathwise_aad, mc_european_sequential,
                       mc_european_with_arena, LongstaffSchwartzEngine, LsmDynamics,
-                      lsm_option_struct, lsm_price_raw, stream_jump,
+                      lsm_last_timing, lsm_option_struct, lsm_price_raw, stream_jump,
                       option_struct, price_raw, sample_payoffs)
 from ._lib import DeviceError
 
@@ -30,5 +30,5 @@ __all__ = [
     "sample_payoffs", "DeviceError", "MonteCarloGreeksEngine", "mc_european_parallel",
     "mc_european_sequential", "mc_european_with_arena", "GreekMethod", "BermudanOption",
     "BermudanLsmOutput", "ExerciseBoundaryPoint", "LongstaffSchwartzEngine", "LsmDynamics",
-    "lsm_option_struct", "lsm_price_raw", "stream_jump",
+    "lsm_last_timing", "lsm_option_struct", "lsm_price_raw", "stream_jump",
 ]

@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = [
     "ofg_create", "ofg_destroy", "ofg_last_error", "ofg_version", "ofg_price",
     "ofg_sample_payoffs", "ofg_price_greeks", "ofg_batch_create", "ofg_batch_run",
     "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
-    "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump",
+    "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump", "ofg_lsm_last_timing",
 ]
 
 
@@ -161,6 +161,9 @@ def load() -> C.CDLL:
         L.ofg_lsm_price.restype = C.c_int32
         L.ofg_stream_jump.argtypes = [C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(C.c_uint64)]
         L.ofg_stream_jump.restype = C.c_int32
+        L.ofg_lsm_last_timing.argtypes = [pv, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int32)]
+        L.ofg_lsm_last_timing.restype = C.c_int32
         _lib = L
         return L
 

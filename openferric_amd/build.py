@@ -39,13 +39,26 @@ def _compile(src: str, obj: str, extra=()):
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every HIP/C++ source and links the shared library."""
+def _stale(src: str, obj: str) -> bool:
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in HEADERS] + \
+        [os.path.join(os.path.dirname(HERE), "include", "openferric_amd.h")]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps if os.path.exists(d))
+
+
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles every HIP/C++ source and links the shared library
+    (incremental: only sources older than their object or any header)."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
+        if incremental and not _stale(src, obj):
+            objs.append(obj)
+            continue
         if verbose:
             print("hipcc", src, flush=True)
         objs.append(_compile(src, obj))

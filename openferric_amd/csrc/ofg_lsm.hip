@@ -14,9 +14,10 @@
 // matrix and the value vector are coalesced. Launch 0 simulates the paths and
 // stores the spot at each exercise date; launch r >= 1 is one backward round.
 // A launch's reductions are deterministic: each tile writes its partial sums
-// to its own slot, and the last tile of an option to finish (a device-scope
-// counter) sums the slots in tile order and closes the round (solve, exercise
-// statistics, price).
+// to its own slot, and a close kernel (one block per option) sums the slots in
+// tile order and closes the round (solve, exercise statistics, price). The
+// kernel boundary orders the slots' writes before their reads; no fences or
+// atomics across workgroups are needed.
 #include "ofg_lsm.hpp"
 #include "ofg_math.hpp"
 #include "ofg_paths.hpp"
@@ -173,15 +174,11 @@ __device__ __forceinline__ RoundPlan round_plan(const LsmDev &o, int launch) {
     return p;
 }
 
-// Tile epilogue: partial sums to the tile's slot; the option's last tile sums
-// every slot in tile order and closes the round.
-__device__ void finish_tile(const LsmArgs &a, const LsmDev &o, int opt, int tile, LsmAcc &acc,
-                            const RoundPlan &rp) {
+// Tile epilogue: the block's partial sums to the tile's own slot.
+__device__ void finish_tile(const LsmArgs &a, const LsmDev &o, int tile, LsmAcc &acc) {
     __shared__ double lds[NWAVES][LSM_NACC];
     __shared__ double tot[LSM_NACC];
-    __shared__ int is_last;
-    const bool put = o.put != 0;
-    block_reduce(acc, put, lds, tot);
+    block_reduce(acc, o.put != 0, lds, tot);
     if (threadIdx.x == 0) {
         LsmPartial &p = a.partials[tile];
 #pragma unroll
@@ -191,15 +188,21 @@ __device__ void finish_tile(const LsmArgs &a, const LsmDev &o, int opt, int tile
         p.term_itm = tot[A_TERM];
         p.fsum = tot[A_FSUM];
         p.fsq = tot[A_FSQ];
-        // release: the slot is visible device-wide before the count is
-        __threadfence();
-        const uint32_t prev = atomicAdd(&a.done[opt], 1u);
-        is_last = prev == (uint32_t)(o.n_tiles - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    // acquire: every other tile's slot is visible after its count
-    __threadfence();
+}
+
+// Closes one launch for one option (one block per option, after the tile
+// kernel): sums the option's tile slots in tile order, then solves the
+// regression of the date just accumulated, records the exercise statistics of
+// the date just updated, and at time 0 the price and standard error.
+__global__ __launch_bounds__(LSM_BLOCK) void lsm_close_kernel(LsmArgs a) {
+    __shared__ double lds[NWAVES][LSM_NACC];
+    __shared__ double tot[LSM_NACC];
+    const int opt = blockIdx.x;
+    const LsmDev o = a.opts[opt];
+    if (o.barrier ? a.launch > 0 : a.launch > o.n_ex) return;
+    const RoundPlan rp = round_plan(o, a.launch);
+    const bool put = o.put != 0;
     LsmAcc s;
     acc_init(s, put);
     for (int t = threadIdx.x; t < o.n_tiles; t += LSM_BLOCK) {
@@ -249,8 +252,6 @@ __device__ void finish_tile(const LsmArgs &a, const LsmDev &o, int opt, int tile
         out.price = mean;
         out.stderr_ = sqrt(var / n);
     }
-    // ready for the next launch
-    a.done[opt] = 0u;
 }
 
 // ------------------------------------------------------------------------
@@ -266,12 +267,12 @@ __device__ __forceinline__ void drive(const LsmDev &o, const LsmArgs &a, int64_t
                                       const double *tab, Step &&step) {
     const int n = o.m.steps;
     if constexpr (RNG == OFG_RNG_REFERENCE) {
-        const uint64_t *sd = a.seeds + o.seed_offset + p;
+        const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
         Xoshiro256pp rng;
         rng.s0 = sd[0];
-        rng.s1 = sd[o.paths];
-        rng.s2 = sd[2 * o.paths];
-        rng.s3 = sd[3 * o.paths];
+        rng.s1 = sd[1];
+        rng.s2 = sd[2];
+        rng.s3 = sd[3];
         for (int t = 1; t <= n; ++t) {
             const double z1 = reference_normal(rng);
             const double z2 = TWO ? reference_normal(rng) : 0.0;
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(LSM_BLOCK) void lsm_paths_kernel(LsmArgs a) {
         else if (o.dynamics == LSM_LOCAL_VOL) lsm_path<RNG, LSM_LOCAL_VOL>(o, a, p, tab, rp, acc);
         else lsm_path<RNG, LSM_GBM>(o, a, p, tab, rp, acc);
     }
-    finish_tile(a, o, opt, tile, acc, rp);
+    finish_tile(a, o, tile, acc);
 }
 
 // ------------------------------------------------------------------------
@@ -445,33 +446,50 @@ __global__ __launch_bounds__(LSM_BLOCK) void lsm_round_kernel(LsmArgs a) {
     LsmAcc acc;
     acc_init(acc, put);
     const int64_t base = (int64_t)(tile - o.tile_first) * LSM_BLOCK * o.ppt;
-    for (int j = 0; j < o.ppt; ++j) {
-        const int64_t p = base + (int64_t)j * LSM_BLOCK + threadIdx.x;
-        if (p >= o.paths) break;
-        double v = vals[p];
-        if (dout.valid) {
-            // exercise decision (:618-626): continuation from the regression
-            const double s = s_u[p];
-            const double ex = lsm_intrinsic(put, s, du.strike);
-            if (ex > 0.0) {
-                const double cont = b0 + b1 * s + b2 * s * s;
-                if (ex > cont) {
-                    v = ex;
-                    acc.v[A_EXN] += 1.0;
-                    acc.v[A_EXT] = put ? fmax(acc.v[A_EXT], s) : fmin(acc.v[A_EXT], s);
-                }
+    // RU paths per thread per pass, all loads issued before any use
+    constexpr int RU = 4;
+    for (int j0 = 0; j0 < o.ppt; j0 += RU) {
+        double v[RU], su[RU], sr[RU];
+        bool ok[RU];
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const int64_t p = base + (int64_t)(j0 + u) * LSM_BLOCK + threadIdx.x;
+            ok[u] = j0 + u < o.ppt && p < o.paths;
+            v[u] = su[u] = sr[u] = 0.0;
+            if (ok[u]) {
+                v[u] = vals[p];
+                su[u] = s_u[p];
+                if (!rp.final_) sr[u] = s_r[p];
             }
         }
-        for (int g = 0; g < du.gap; ++g) v *= o.disc;
-        if (rp.final_) {
-            acc.v[A_FSUM] += v;
-            acc.v[A_FSQ] += v * v;
-        } else {
-            acc_regress(acc, put, s_r[p], strike_r, v);
-            vals[p] = v;
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            if (!ok[u]) continue;
+            double val = v[u];
+            if (dout.valid) {
+                // exercise decision (:618-626): continuation from the regression
+                const double s = su[u];
+                const double ex = lsm_intrinsic(put, s, du.strike);
+                if (ex > 0.0) {
+                    const double cont = b0 + b1 * s + b2 * s * s;
+                    if (ex > cont) {
+                        val = ex;
+                        acc.v[A_EXN] += 1.0;
+                        acc.v[A_EXT] = put ? fmax(acc.v[A_EXT], s) : fmin(acc.v[A_EXT], s);
+                    }
+                }
+            }
+            for (int g = 0; g < du.gap; ++g) val *= o.disc;
+            if (rp.final_) {
+                acc.v[A_FSUM] += val;
+                acc.v[A_FSQ] += val * val;
+            } else {
+                acc_regress(acc, put, sr[u], strike_r, val);
+                vals[base + (int64_t)(j0 + u) * LSM_BLOCK + threadIdx.x] = val;
+            }
         }
     }
-    finish_tile(a, o, opt, tile, acc, rp);
+    finish_tile(a, o, tile, acc);
 }
 
 // ------------------------------------------------------------------------
@@ -526,26 +544,17 @@ __global__ __launch_bounds__(256) void lsm_seed_kernel(const LsmSeedJob *__restr
     for (int q = 0; q < j.chunk; ++q) {
         const int64_t p = first + q;
         if (p >= j.paths) break;
-        out[p] = st.s0;
-        out[j.paths + p] = st.s1;
-        out[2 * j.paths + p] = st.s2;
-        out[3 * j.paths + p] = st.s3;
+        // one path's state is one contiguous 32-byte sector
+        out[4 * p] = st.s0;
+        out[4 * p + 1] = st.s1;
+        out[4 * p + 2] = st.s2;
+        out[4 * p + 3] = st.s3;
         for (int64_t d = 0; d < j.draws; ++d) (void)st.next_u64();
     }
 }
 
-__global__ void lsm_zero_kernel(uint32_t *done, int32_t n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) done[i] = 0u;
-}
 
 }  // namespace
-
-hipError_t launch_lsm_zero(uint32_t *done, int32_t n, hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
-    lsm_zero_kernel<<<(n + 255) / 256, 256, 0, stream>>>(done, n);
-    return hipGetLastError();
-}
 
 hipError_t launch_lsm_seeds(const LsmSeedJob *jobs, int32_t n_jobs, int64_t total_threads,
                             const uint64_t *polys, uint64_t *seeds, hipStream_t stream) {
@@ -572,6 +581,9 @@ hipError_t launch_lsm(const LsmArgs &a, hipStream_t stream) {
     } else {
         lsm_round_kernel<<<a.n_tiles, LSM_BLOCK, 0, stream>>>(a);
     }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    lsm_close_kernel<<<a.n_opts, LSM_BLOCK, 0, stream>>>(a);
     return hipGetLastError();
 }
 

@@ -10,9 +10,9 @@
 //   values [path]        the running discounted cash flow of each path,
 // and the backward induction is one launch per exercise date: apply the
 // exercise decision of the date regressed by the previous launch, discount to
-// the next earlier date, accumulate the regression sums there. Each launch
-// ends in a per-option last-block reduction (fixed tile order) that solves
-// the 3x3 normal equations for the next launch.
+// the next earlier date, accumulate the regression sums there. Each launch is
+// followed by a per-option close kernel (a fixed-order reduction of the tile
+// partials) that solves the 3x3 normal equations for the next launch.
 #pragma once
 
 #include "ofg_internal.hpp"
@@ -44,7 +44,7 @@ struct alignas(16) LsmDev {
     int64_t paths;
     int64_t s_offset;      // spots, [date][path]
     int64_t v_offset;      // values
-    int64_t seed_offset;   // reference-stream start states, [word][path]
+    int64_t seed_offset;   // reference-stream start states, [path][word]
     int64_t normals_offset;  // external normals, [step][path]
     double disc;           // exp(-r dt)
     double terminal_strike;
@@ -89,15 +89,14 @@ struct LsmArgs {
     LsmDateOut *date_out;
     LsmOptOut *opt_out;
     LsmPartial *partials;        // one slot per tile
-    uint32_t *done;              // per option: tiles finished in this launch
     double *spots;
     double *values;
     const uint64_t *seeds;
     const double *normals1, *normals2;
     int32_t launch;              // 0: path simulation; r >= 1: backward round r
     int32_t n_tiles;
+    int32_t n_opts;
     int32_t rng_mode;
-    int32_t pad;
 };
 
 // Start states of the reference stream for a group of paths: thread g of
@@ -109,7 +108,7 @@ struct LsmSeedJob {
     uint64_t base[4];            // xoshiro256++ state after seed_from_u64(seed)
     int64_t paths;
     int64_t draws;               // draws per path (steps, or 2 steps for Heston)
-    int64_t seed_offset;         // into the seeds array ([word][path] per option)
+    int64_t seed_offset;         // into the seeds array ([path][word] per option)
     int64_t thread_first;        // first global thread of this job
     int32_t chunk;
     int32_t n_polys;
@@ -117,7 +116,6 @@ struct LsmSeedJob {
     int32_t pad;
 };
 
-hipError_t launch_lsm_zero(uint32_t *done, int32_t n, hipStream_t stream);
 hipError_t launch_lsm_seeds(const LsmSeedJob *jobs, int32_t n_jobs, int64_t total_threads,
                             const uint64_t *polys, uint64_t *seeds, hipStream_t stream);
 hipError_t launch_lsm(const LsmArgs &a, hipStream_t stream);

@@ -32,16 +32,18 @@ using namespace ofg_host;
 namespace ofg_host {
 
 struct LsmBuffers {
-    DevBuf opts, tile_opt, dates, date_out, opt_out, partials, done, spots, values, seeds, polys,
-        jobs, n1, n2;
+    DevBuf opts, tile_opt, dates, date_out, opt_out, partials, spots, values, seeds, polys, jobs,
+        n1, n2;
+    // device time of the last call: seeds + path kernel, backward rounds
+    double paths_ms = 0.0, rounds_ms = 0.0;
+    int32_t rounds = 0;
 };
 
 void lsm_release(ofg_ctx *ctx) {
     if (!ctx || !ctx->lsm) return;
     LsmBuffers *b = ctx->lsm;
     for (DevBuf *d : {&b->opts, &b->tile_opt, &b->dates, &b->date_out, &b->opt_out, &b->partials,
-                      &b->done, &b->spots, &b->values, &b->seeds, &b->polys, &b->jobs, &b->n1,
-                      &b->n2})
+                      &b->spots, &b->values, &b->seeds, &b->polys, &b->jobs, &b->n1, &b->n2})
         d->release();
     delete b;
     ctx->lsm = nullptr;
@@ -254,9 +256,26 @@ void fill_path_constants(ofg::OptDev &d, const ofg_lsm_option &o, double vol, do
     (void)rng_mode;
 }
 
+// Paths per thread of one option's tiles. A tile streams LSM_BLOCK * ppt
+// paths per launch after a short dependent prologue (tile -> option -> date
+// data), so tiles are made as fat as the batch allows: about LSM_TARGET_TILES
+// tiles over the whole batch (a few per CU), at most LSM_MAX_TILES per option
+// (the close kernel's reduction) and 64 paths per thread.
+constexpr int64_t LSM_TARGET_TILES = 4096;
+
+int32_t tile_ppt(int64_t paths, int64_t total_paths) {
+    const int64_t blk = ofg::LSM_BLOCK;
+    const int64_t per_tile = std::max<int64_t>(1, total_paths / LSM_TARGET_TILES);
+    int64_t ppt = std::min<int64_t>(64, std::max<int64_t>(1, (per_tile + blk - 1) / blk));
+    const int64_t min_ppt = (paths + blk * ofg::LSM_MAX_TILES - 1) / (blk * ofg::LSM_MAX_TILES);
+    return (int32_t)std::max(ppt, min_ppt);
+}
+
 LsmPlan build_lsm_plan(const ofg_lsm_option *in, size_t n, int32_t rng_mode) {
     LsmPlan p;
     p.host.resize(n);
+    int64_t total_paths = 0;
+    for (size_t i = 0; i < n; ++i) total_paths += std::max<int64_t>(in[i].num_paths, 0);
     std::map<uint64_t, std::vector<Poly>> poly_cache;
     for (size_t i = 0; i < n; ++i) {
         const ofg_lsm_option &o = in[i];
@@ -380,9 +399,7 @@ LsmPlan build_lsm_plan(const ofg_lsm_option *in, size_t n, int32_t rng_mode) {
         d.put = o.option_type == OFG_PUT;
         d.n_ex = (int32_t)ex_steps.size();
         d.paths = paths;
-        d.ppt = (int32_t)std::max<int64_t>(
-            1, (paths + (int64_t)ofg::LSM_BLOCK * ofg::LSM_MAX_TILES - 1) /
-                   ((int64_t)ofg::LSM_BLOCK * ofg::LSM_MAX_TILES));
+        d.ppt = tile_ppt(paths, total_paths);
         const int64_t tile_paths = (int64_t)ofg::LSM_BLOCK * d.ppt;
         d.n_tiles = (int32_t)((paths + tile_paths - 1) / tile_paths);
         d.tile_first = (int32_t)p.tile_opt.size();
@@ -418,7 +435,9 @@ LsmPlan build_lsm_plan(const ofg_lsm_option *in, size_t n, int32_t rng_mode) {
             j.base[3] = st.s3;
             j.paths = paths;
             j.draws = steps * (dynamics == ofg::LSM_HESTON ? 2 : 1);
-            j.chunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(64, 512 / j.draws));
+            // a thread jumps once (about 128 x 256 transition steps for a
+            // 2^17-path option) and then walks `chunk` paths' draws
+            j.chunk = 64;
             const int64_t threads = (paths + j.chunk - 1) / j.chunk;
             int32_t bits = 0;
             while (bits < 63 && (int64_t(1) << bits) < threads) ++bits;
@@ -492,13 +511,12 @@ int32_t lsm_price_impl(ofg_ctx *ctx, const ofg_lsm_option *opts, size_t n,
         b.date_out.ensure(std::max<size_t>(p.dates.size(), 1) * sizeof(ofg::LsmDateOut));
         b.opt_out.ensure(p.opts.size() * sizeof(ofg::LsmOptOut));
         b.partials.ensure(p.tile_opt.size() * sizeof(ofg::LsmPartial));
-        b.done.ensure(p.opts.size() * sizeof(uint32_t));
         b.spots.ensure((size_t)std::max<int64_t>(p.spots, 1) * sizeof(double));
         b.values.ensure((size_t)std::max<int64_t>(p.values, 1) * sizeof(double));
         a.date_out = b.date_out.as<ofg::LsmDateOut>();
         a.opt_out = b.opt_out.as<ofg::LsmOptOut>();
         a.partials = b.partials.as<ofg::LsmPartial>();
-        a.done = b.done.as<uint32_t>();
+        a.n_opts = (int32_t)p.opts.size();
         a.spots = b.spots.as<double>();
         a.values = b.values.as<double>();
         a.rng_mode = rng;
@@ -526,17 +544,19 @@ int32_t lsm_price_impl(ofg_ctx *ctx, const ofg_lsm_option *opts, size_t n,
             polys = upload(b.polys, p.polys, s, "H2D jump polynomials");
         }
         hip_check(hipEventRecord(ctx->ev0, s), "event");
-        hip_check(ofg::launch_lsm_zero(a.done, (int32_t)p.opts.size(), s), "lsm zero launch");
         if (rng == OFG_RNG_REFERENCE)
             hip_check(ofg::launch_lsm_seeds(jobs, (int32_t)p.jobs.size(), p.seed_threads, polys,
                                             b.seeds.as<uint64_t>(), s),
                       "lsm seed kernel launch");
         trace("lsm: launches");
-        for (int32_t r = 0; r <= p.max_rounds; ++r) {
-            a.launch = r;
-            hip_check(ofg::launch_lsm(a, s), r == 0 ? "lsm path kernel launch" : "lsm round launch");
-        }
+        a.launch = 0;
+        hip_check(ofg::launch_lsm(a, s), "lsm path kernel launch");
         hip_check(hipEventRecord(ctx->ev1, s), "event");
+        for (int32_t r = 1; r <= p.max_rounds; ++r) {
+            a.launch = r;
+            hip_check(ofg::launch_lsm(a, s), "lsm round launch");
+        }
+        hip_check(hipEventRecord(ctx->ev2, s), "event");
         hip_check(hipMemcpyAsync(opt_out.data(), a.opt_out, opt_out.size() * sizeof(opt_out[0]),
                                  hipMemcpyDeviceToHost, s),
                   "D2H lsm results");
@@ -546,7 +566,12 @@ int32_t lsm_price_impl(ofg_ctx *ctx, const ofg_lsm_option *opts, size_t n,
                       "D2H lsm dates");
         hip_check(hipStreamSynchronize(s), "lsm kernel execution");
         trace("lsm: synchronized");
-        hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
+        float ms_paths = 0.f;
+        hip_check(hipEventElapsedTime(&ms_paths, ctx->ev0, ctx->ev1), "event time");
+        hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev2), "event time");
+        b.paths_ms = ms_paths;
+        b.rounds_ms = ms - ms_paths;
+        b.rounds = p.max_rounds;
     }
 
     int64_t slot = 0;
@@ -605,6 +630,19 @@ int32_t ofg_lsm_price(ofg_ctx *ctx, const ofg_lsm_option *opts, size_t n,
                       const ofg_engine_cfg *cfg, ofg_result *out, ofg_lsm_boundary *boundary,
                       int64_t *boundary_len) {
     return guarded([&]() { return lsm_price_impl(ctx, opts, n, cfg, out, boundary, boundary_len); });
+}
+
+int32_t ofg_lsm_last_timing(ofg_ctx *ctx, double *paths_ms, double *rounds_ms,
+                            int32_t *rounds) {
+    return guarded([&]() -> int32_t {
+        if (!ctx) fail(OFG_INVALID_INPUT, "null argument");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        const LsmBuffers *b = ctx->lsm;
+        if (paths_ms) *paths_ms = b ? b->paths_ms : 0.0;
+        if (rounds_ms) *rounds_ms = b ? b->rounds_ms : 0.0;
+        if (rounds) *rounds = b ? b->rounds : 0;
+        return OFG_OK;
+    });
 }
 
 int32_t ofg_stream_jump(const uint64_t state[4], uint64_t k, uint64_t out[4]) {

@@ -699,6 +699,15 @@ class LongstaffSchwartzEngine:
                                  exercise_boundary=pts)
 
 
+def lsm_last_timing(device: int = 0) -> dict:
+    """Device time of the last lsm_price_raw call on this device: path
+    simulation (seed + path kernels) and backward rounds."""
+    L = _lib.load()
+    a, b, r = C.c_double(), C.c_double(), C.c_int32()
+    _raise(L.ofg_lsm_last_timing(_lib.context(device), C.byref(a), C.byref(b), C.byref(r)))
+    return {"paths_ms": a.value, "rounds_ms": b.value, "rounds": r.value}
+
+
 def stream_jump(state: Sequence[int], k: int) -> List[int]:
     """ofg_stream_jump: the xoshiro256++ state k draws ahead (host-only)."""
     L = _lib.load()

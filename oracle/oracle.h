@@ -28,6 +28,13 @@
  *   or_mc_greeks_estimate              BS delta/gamma/vega within 3 SE
  *                                      (independent closed forms) and the
  *                                      reference's QuantLib values
+ *   or_lu3_solve                       numpy.linalg.solve, singular cases
+ *   or_lsm_price                       Longstaff-Schwartz Table 1 (0.10),
+ *                                      QuantLib FD Bermudan puts (0.5 %),
+ *                                      independent numpy lstsq LSM (1e-9,
+ *                                      exact counts), BS, CRR American
+ *   or_xoshiro_advance                 brute-force stepping (pins the
+ *                                      product's jump-ahead)
  */
 #ifndef OPENFERRIC_ORACLE_H
 #define OPENFERRIC_ORACLE_H
