@@ -275,6 +275,20 @@ int32_t ofg_lsm_price(ofg_ctx *ctx, const ofg_lsm_option *opts, size_t n,
                       const ofg_engine_cfg *cfg, ofg_result *out,
                       ofg_lsm_boundary *boundary, int64_t *boundary_len);
 
+/* Quasi-Monte Carlo European vanillas under GBM on Sobol points: replaces
+ * mc_european_qmc / mc_european_qmc_with_seed
+ * (src/engines/monte_carlo/mc_qmc.rs:29-144), n options in one launch.
+ * opts[i] must be GBM vanillas (the reference takes a European
+ * VanillaOption); num_paths = Sobol points, num_steps = dimensions (at most
+ * 21201), seed = the Sobol seed, div_yield = the market's effective yield.
+ * Each path's point is the reference sequence's point of the same index, so
+ * the draws are the reference's. As the reference, degenerate inputs return
+ * results, not errors: price NaN and has_stderr 0 for zero paths or steps or
+ * a vol that is not finite and positive; the payoff at the spot with
+ * stderr 0 for expiry <= 0; diagnostics (num_paths, num_steps, vol) NaN
+ * where the reference records none. out[n]. */
+int32_t ofg_qmc_price(ofg_ctx *ctx, const ofg_option *opts, size_t n, ofg_result *out);
+
 /* Device time of the context's last ofg_lsm_price call, split into path
  * simulation (seed and path kernels) and the backward rounds, and the number
  * of round launches. No counterpart in the reference; for measurement.

@@ -4,7 +4,10 @@
 // types, so the GPU engine is a drop-in beside MonteCarloPricingEngine:
 //   Greeks, PricingResult, PricingError          src/core/engine.rs:15-315
 //   OptionType, BarrierSpec, AsianSpec, ...      src/core/types.rs:13-122
-//   VanillaOption / AsianOption / BarrierOption  src/instruments/*.rs
+//   VanillaOption / AsianOption / BarrierOption / BermudanOption
+//                                                src/instruments/*.rs
+//   LongstaffSchwartzEngine                      src/engines/lsm/longstaff_schwartz.rs
+//   mc_european_qmc(_with_seed)                  src/engines/monte_carlo/mc_qmc.rs
 //   Market (flat vol)                            src/market/market.rs:202-414
 //   Heston                                       src/models/stochastic.rs:37-72
 //   PricingEngine<I>::price / price_with_greeks_aad (engine.rs:37-52)
@@ -60,7 +63,7 @@ inline void check(int32_t rc) {
 
 // ------------------------------------------------------------ contract --
 enum class OptionType { Call, Put };
-enum class ExerciseStyle { European, American };
+enum class ExerciseStyle { European, American, Bermudan };
 enum class BarrierDirection { Up, Down };
 enum class BarrierStyle { In, Out };
 enum class Averaging { Arithmetic, Geometric };
@@ -112,8 +115,35 @@ struct VanillaOption {
     OptionType option_type = OptionType::Call;
     double strike = 0, expiry = 0;
     ExerciseStyle exercise = ExerciseStyle::European;
+    std::vector<double> bermudan_dates;   // ExerciseStyle::Bermudan { dates }
     static VanillaOption european_call(double k, double t) { return {OptionType::Call, k, t}; }
     static VanillaOption european_put(double k, double t) { return {OptionType::Put, k, t}; }
+    static VanillaOption american_call(double k, double t) {
+        return {OptionType::Call, k, t, ExerciseStyle::American, {}};
+    }
+    static VanillaOption american_put(double k, double t) {
+        return {OptionType::Put, k, t, ExerciseStyle::American, {}};
+    }
+};
+
+// BermudanOption (instruments/bermudan.rs:18-144): exercise dates with a
+// strike per date; validated by the engine with the reference's messages.
+struct BermudanOption {
+    OptionType option_type = OptionType::Put;
+    double expiry = 0;
+    std::vector<double> exercise_dates, strike_schedule;
+    static BermudanOption with_constant_strike(OptionType t, double k, double expiry,
+                                               std::vector<double> dates) {
+        std::vector<double> ks(dates.size(), k);
+        return BermudanOption{t, expiry, std::move(dates), std::move(ks)};
+    }
+};
+
+// ExerciseBoundaryPoint / BermudanLsmOutput (longstaff_schwartz.rs:42-63).
+struct ExerciseBoundaryPoint {
+    double time = 0, strike = 0;
+    std::optional<double> boundary_spot;
+    int64_t itm_paths = 0, exercised_paths = 0;
 };
 
 struct AsianSpec {
@@ -475,6 +505,172 @@ inline PricingResult mc_european_sequential(const VanillaOption &i, const Market
                                             int64_t n_paths, int64_t /*n_steps*/ = 1) {
     return mc_european_exact(i, m, n_paths);
 }
+
+// mc_european_qmc_with_seed / mc_european_qmc (mc_qmc.rs:29-144): the
+// reference's Sobol points on the GPU, one path per point.
+inline PricingResult mc_european_qmc_with_seed(const VanillaOption &i, const Market &m,
+                                               int64_t n_paths, int64_t n_steps, uint64_t seed) {
+    PricingResult r;
+    if (i.exercise != ExerciseStyle::European) {
+        r.price = std::nan("");
+        return r;
+    }
+    Market eff = m;
+    eff.dividend_yield = m.effective_dividend_yield(i.expiry);
+    ofg_option o = detail::to_ofg(i, eff, n_paths, n_steps, seed);
+    ofg_result res{};
+    check(ofg_qmc_price(device_context(), &o, 1, &res));
+    r.price = res.price;
+    if (res.has_stderr) r.stderr_ = res.stderr_;
+    if (!std::isnan(res.num_paths)) {
+        r.diagnostics.insert("num_paths", res.num_paths);
+        r.diagnostics.insert("num_steps", res.num_steps);
+        r.diagnostics.insert("vol", res.vol);
+    }
+    r.kernel_ms = res.kernel_ms;
+    return r;
+}
+inline PricingResult mc_european_qmc(const VanillaOption &i, const Market &m, int64_t n_paths,
+                                     int64_t n_steps) {
+    return mc_european_qmc_with_seed(i, m, n_paths, n_steps, 0);
+}
+
+// LongstaffSchwartzEngine (longstaff_schwartz.rs:66-735) on the GPU:
+// VanillaOption (European / American / Bermudan exercise, GBM),
+// BermudanOption under GBM, flat local vol or Heston with its exercise
+// boundary, BarrierOption by plain Monte Carlo. Reference draws (the engine's
+// own single stream) unless with_rng_mode says otherwise.
+class LongstaffSchwartzEngine : public PricingEngine<VanillaOption>,
+                                public PricingEngine<BermudanOption>,
+                                public PricingEngine<BarrierOption> {
+public:
+    struct BermudanLsmOutput {
+        PricingResult result;
+        std::vector<ExerciseBoundaryPoint> exercise_boundary;
+    };
+
+    LongstaffSchwartzEngine(int64_t num_paths, int64_t num_steps, uint64_t seed)
+        : num_paths(num_paths), num_steps(num_steps), seed(seed) {}
+    LongstaffSchwartzEngine &with_local_vol_dynamics() {
+        dynamics_ = OFG_LSM_LOCAL_VOL;
+        return *this;
+    }
+    // The spot drift is the market's r - q; the model's mu is not used.
+    LongstaffSchwartzEngine &with_heston_dynamics(const Heston &h) {
+        dynamics_ = OFG_LSM_HESTON;
+        heston_ = h;
+        return *this;
+    }
+    LongstaffSchwartzEngine &with_rng_mode(RngMode m) {
+        rng_ = m;
+        return *this;
+    }
+
+    PricingResult price(const VanillaOption &i, const Market &m) const override {
+        ofg_lsm_option o = base(i.option_type, i.expiry, m);
+        o.instrument = OFG_LSM_VANILLA;
+        o.exercise = i.exercise == ExerciseStyle::European   ? OFG_EUROPEAN
+                     : i.exercise == ExerciseStyle::American ? OFG_AMERICAN
+                                                             : OFG_BERMUDAN;
+        o.strike = i.strike;
+        o.vol = m.vol_for(i.strike, i.expiry);
+        if (i.exercise == ExerciseStyle::Bermudan) {
+            o.n_dates = (int32_t)i.bermudan_dates.size();
+            o.dates = i.bermudan_dates.data();
+        }
+        return run(o, nullptr);
+    }
+    PricingResult price(const BermudanOption &i, const Market &m) const override {
+        return price_bermudan_with_boundary(i, m).result;
+    }
+    PricingResult price(const BarrierOption &i, const Market &m) const override {
+        ofg_lsm_option o = base(i.option_type, i.expiry, m);
+        o.instrument = OFG_LSM_BARRIER;
+        o.strike = i.strike;
+        o.vol = m.vol_for(i.strike, i.expiry);
+        o.barrier_dir = i.barrier.direction == BarrierDirection::Up ? OFG_UP : OFG_DOWN;
+        o.barrier_style = i.barrier.style == BarrierStyle::In ? OFG_IN : OFG_OUT;
+        o.barrier_level = i.barrier.level;
+        o.rebate = i.barrier.rebate;
+        return run(o, nullptr);
+    }
+    BermudanLsmOutput price_bermudan_with_boundary(const BermudanOption &i,
+                                                   const Market &m) const {
+        ofg_lsm_option o = base(i.option_type, i.expiry, m);
+        o.instrument = OFG_LSM_BERMUDAN;
+        o.dynamics = dynamics_;
+        o.n_dates = (int32_t)i.exercise_dates.size();
+        o.dates = i.exercise_dates.data();
+        if (i.strike_schedule.size() == i.exercise_dates.size()) o.strikes = i.strike_schedule.data();
+        o.strike = i.strike_schedule.empty() ? 0.0 : i.strike_schedule.back();
+        o.vol = m.vol_for(o.strike, i.expiry);
+        o.kappa = heston_.kappa;
+        o.theta = heston_.theta;
+        o.xi = heston_.xi;
+        o.rho = heston_.rho;
+        o.v0 = heston_.v0;
+        BermudanLsmOutput out;
+        out.result = run(o, &out.exercise_boundary);
+        // exercise_dates: the effective schedule's length (bermudan.rs:85-103)
+        if (out.result.diagnostics.get("num_paths")) {
+            const size_t n = i.exercise_dates.size();
+            const bool appended = n && i.expiry - i.exercise_dates.back() > 1.0e-12;
+            out.result.diagnostics.insert("exercise_dates", (double)(n + (appended ? 1 : 0)));
+        }
+        return out;
+    }
+
+    int64_t num_paths, num_steps;
+    uint64_t seed;
+
+private:
+    ofg_lsm_option base(OptionType t, double expiry, const Market &m) const {
+        ofg_lsm_option o{};
+        o.option_type = t == OptionType::Call ? OFG_CALL : OFG_PUT;
+        o.spot = m.spot;
+        o.expiry = expiry;
+        o.rate = m.rate;
+        o.div_yield = m.effective_dividend_yield(expiry);
+        o.num_paths = num_paths;
+        o.num_steps = num_steps;
+        o.seed = seed;
+        return o;
+    }
+    PricingResult run(const ofg_lsm_option &o, std::vector<ExerciseBoundaryPoint> *bnd) const {
+        ofg_engine_cfg c{};
+        c.rng_mode = (int32_t)rng_;
+        ofg_result res{};
+        std::vector<ofg_lsm_boundary> slots(bnd ? (size_t)std::max<int64_t>(num_steps, 1) : 0);
+        int64_t len = 0;
+        check(ofg_lsm_price(device_context(), &o, 1, &c, &res, bnd ? slots.data() : nullptr,
+                            bnd ? &len : nullptr));
+        PricingResult r;
+        r.price = res.price;
+        r.stderr_ = res.stderr_;
+        if (!std::isnan(res.num_paths)) {
+            r.diagnostics.insert("num_paths", res.num_paths);
+            r.diagnostics.insert("num_steps", res.num_steps);
+            if (!std::isnan(res.vol)) r.diagnostics.insert("vol", res.vol);
+        }
+        r.kernel_ms = res.kernel_ms;
+        if (bnd) {
+            for (int64_t k = 0; k < len; ++k) {
+                const ofg_lsm_boundary &b = slots[(size_t)k];
+                ExerciseBoundaryPoint p;
+                p.time = b.time;
+                p.strike = b.strike;
+                if (b.has_boundary) p.boundary_spot = b.boundary_spot;
+                p.itm_paths = b.itm_paths;
+                p.exercised_paths = b.exercised_paths;
+                bnd->push_back(p);
+            }
+        }
+        return r;
+    }
+    int32_t dynamics_ = OFG_LSM_GBM;
+    Heston heston_{0, 0, 0, 0, 0, 0};
+    RngMode rng_ = RngMode::Reference;
+};
 
 // Direct MC helpers of pricing::asian (asian.rs:139-200) and pricing::barrier
 // (barrier.rs:199-239): GBM { mu: r, sigma }, antithetic, discount e^{-rt},

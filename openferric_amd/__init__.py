@@ -16,6 +16,7 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       mc_european_parallel, mc_european_pathwise_aad, mc_european_sequential,
                       mc_european_with_arena, LongstaffSchwartzEngine, LsmDynamics,
                       lsm_last_timing, lsm_option_struct, lsm_price_raw, stream_jump,
+                      mc_european_qmc, mc_european_qmc_with_seed, qmc_price_raw,
                       option_struct, price_raw, sample_payoffs)
 from ._lib import DeviceError
 
@@ -31,4 +32,5 @@ __all__ = [
     "mc_european_sequential", "mc_european_with_arena", "GreekMethod", "BermudanOption",
     "BermudanLsmOutput", "ExerciseBoundaryPoint", "LongstaffSchwartzEngine", "LsmDynamics",
     "lsm_last_timing", "lsm_option_struct", "lsm_price_raw", "stream_jump",
+    "mc_european_qmc", "mc_european_qmc_with_seed", "qmc_price_raw",
 ]

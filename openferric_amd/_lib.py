@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = [
     "ofg_sample_payoffs", "ofg_price_greeks", "ofg_batch_create", "ofg_batch_run",
     "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
     "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump", "ofg_lsm_last_timing",
+    "ofg_qmc_price",
 ]
 
 
@@ -164,6 +165,8 @@ def load() -> C.CDLL:
         L.ofg_lsm_last_timing.argtypes = [pv, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                           C.POINTER(C.c_int32)]
         L.ofg_lsm_last_timing.restype = C.c_int32
+        L.ofg_qmc_price.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t, C.POINTER(OfgResult)]
+        L.ofg_qmc_price.restype = C.c_int32
         _lib = L
         return L
 

@@ -93,6 +93,7 @@ inline int32_t guarded(F &&f) {
 }
 
 struct LsmBuffers;   // ofg_lsm_api.cpp
+struct QmcBuffers;   // ofg_qmc_api.cpp
 
 // Instrument validation with the reference's messages (instruments/*.rs
 // validate()); throws OfgError.
@@ -109,10 +110,13 @@ struct ofg_ctx {
         n2, adj;
     std::map<std::string, ofg_host::Tuning> tuned;
     ofg_host::LsmBuffers *lsm = nullptr;   // Longstaff-Schwartz work buffers
+    ofg_host::QmcBuffers *qmc = nullptr;   // quasi-Monte Carlo tables and partials
     int cus = 256;
 };
 
 namespace ofg_host {
 // Releases the context's Longstaff-Schwartz buffers (ofg_lsm_api.cpp).
 void lsm_release(ofg_ctx *ctx);
+// Releases the context's quasi-Monte Carlo buffers (ofg_qmc_api.cpp).
+void qmc_release(ofg_ctx *ctx);
 }  // namespace ofg_host

@@ -258,24 +258,21 @@ void fill_path_constants(ofg::OptDev &d, const ofg_lsm_option &o, double vol, do
 
 // Paths per thread of one option's tiles. A tile streams LSM_BLOCK * ppt
 // paths per launch after a short dependent prologue (tile -> option -> date
-// data), so tiles are made as fat as the batch allows: about LSM_TARGET_TILES
-// tiles over the whole batch (a few per CU), at most LSM_MAX_TILES per option
-// (the close kernel's reduction) and 64 paths per thread.
-constexpr int64_t LSM_TARGET_TILES = 4096;
+// data), so tiles are made fat: about LSM_TILES_PER_OPTION per option (at
+// most 64 paths per thread). The tile size depends on the option alone, so
+// its summation order, and with it every bit of its price, does not depend
+// on what else is in the batch.
+constexpr int64_t LSM_TILES_PER_OPTION = 512;
 
-int32_t tile_ppt(int64_t paths, int64_t total_paths) {
+int32_t tile_ppt(int64_t paths) {
     const int64_t blk = ofg::LSM_BLOCK;
-    const int64_t per_tile = std::max<int64_t>(1, total_paths / LSM_TARGET_TILES);
-    int64_t ppt = std::min<int64_t>(64, std::max<int64_t>(1, (per_tile + blk - 1) / blk));
-    const int64_t min_ppt = (paths + blk * ofg::LSM_MAX_TILES - 1) / (blk * ofg::LSM_MAX_TILES);
-    return (int32_t)std::max(ppt, min_ppt);
+    const int64_t ppt = (paths + blk * LSM_TILES_PER_OPTION - 1) / (blk * LSM_TILES_PER_OPTION);
+    return (int32_t)std::min<int64_t>(64, std::max<int64_t>(1, ppt));
 }
 
 LsmPlan build_lsm_plan(const ofg_lsm_option *in, size_t n, int32_t rng_mode) {
     LsmPlan p;
     p.host.resize(n);
-    int64_t total_paths = 0;
-    for (size_t i = 0; i < n; ++i) total_paths += std::max<int64_t>(in[i].num_paths, 0);
     std::map<uint64_t, std::vector<Poly>> poly_cache;
     for (size_t i = 0; i < n; ++i) {
         const ofg_lsm_option &o = in[i];
@@ -399,7 +396,10 @@ LsmPlan build_lsm_plan(const ofg_lsm_option *in, size_t n, int32_t rng_mode) {
         d.put = o.option_type == OFG_PUT;
         d.n_ex = (int32_t)ex_steps.size();
         d.paths = paths;
-        d.ppt = tile_ppt(paths, total_paths);
+        d.ppt = tile_ppt(paths);
+        if ((paths + (int64_t)ofg::LSM_BLOCK * d.ppt - 1) / ((int64_t)ofg::LSM_BLOCK * d.ppt) >
+            ofg::LSM_MAX_TILES)
+            fail(OFG_INVALID_INPUT, "num_paths must be <= 16777216 per option");
         const int64_t tile_paths = (int64_t)ofg::LSM_BLOCK * d.ppt;
         d.n_tiles = (int32_t)((paths + tile_paths - 1) / tile_paths);
         d.tile_first = (int32_t)p.tile_opt.size();

@@ -451,6 +451,46 @@ def _mc_european_exact(instrument: VanillaOption, market: Market, n_paths: int,
     return PricingResult(r.price, r.stderr_, diagnostics=diag, kernel_ms=r.kernel_ms)
 
 
+def qmc_price_raw(opts: Sequence[_lib.OfgOption], device: int = 0) -> List[_lib.OfgResult]:
+    """Calls ofg_qmc_price on an array of ofg_option (GBM vanillas)."""
+    L = _lib.load()
+    ctx = _lib.context(device)
+    arr = (_lib.OfgOption * len(opts))(*opts)
+    out = (_lib.OfgResult * len(opts))()
+    _raise(L.ofg_qmc_price(ctx, arr, len(opts), out))
+    return list(out)
+
+
+def _qmc_result(r: _lib.OfgResult) -> PricingResult:
+    diag = Diagnostics()
+    if not math.isnan(r.num_paths):
+        diag.insert("num_paths", r.num_paths)
+        diag.insert("num_steps", r.num_steps)
+        diag.insert("vol", r.vol)
+    return PricingResult(r.price, r.stderr_ if r.has_stderr else None, diagnostics=diag,
+                         kernel_ms=r.kernel_ms)
+
+
+def mc_european_qmc_with_seed(instrument: VanillaOption, market: Market, n_paths: int,
+                              n_steps: int, seed: int, device: int = 0) -> PricingResult:
+    """mc_european_qmc_with_seed (engines/monte_carlo/mc_qmc.rs:40-144) on the
+    GPU: the reference's Sobol points, one path per point. Non-European
+    exercise gives a NaN price with no stderr, as the reference."""
+    if instrument.exercise is not ExerciseStyle.European:
+        return PricingResult(_NAN_RESULT, None)
+    keep = _Keep()
+    q = market.effective_dividend_yield(instrument.expiry)
+    eff = Market(market.spot, market.rate, q, market.flat_vol)
+    o = option_struct(instrument, eff, n_paths, n_steps, seed, keep)
+    return _qmc_result(qmc_price_raw([o], device)[0])
+
+
+def mc_european_qmc(instrument: VanillaOption, market: Market, n_paths: int,
+                    n_steps: int) -> PricingResult:
+    """mc_european_qmc (mc_qmc.rs:29-37): seed 0."""
+    return mc_european_qmc_with_seed(instrument, market, n_paths, n_steps, 0)
+
+
 def mc_european_parallel(instrument: VanillaOption, market: Market, n_paths: int,
                          n_steps: int = 1) -> PricingResult:
     """GPU counterpart of mc_european_parallel (mc_parallel.rs:264-362).

@@ -129,6 +129,12 @@ def lib():
         L.or_mc_greeks_estimate.argtypes = [C.POINTER(OrOption), i64, u64, i32, d, pd]
         L.or_mc_greeks_estimate.restype = C.c_int
         L.or_lu3_solve.argtypes = [pd, pd, pd]; L.or_lu3_solve.restype = C.c_int
+        L.or_sobol_directions.argtypes = [u64, u64, pu64]; L.or_sobol_directions.restype = None
+        L.or_sobol_scrambler.argtypes = [u64, u64]; L.or_sobol_scrambler.restype = u64
+        L.or_sobol_points.argtypes = [i64, u64, i64, pu64, pd]; L.or_sobol_points.restype = None
+        L.or_mc_european_qmc.argtypes = [C.POINTER(OrOption), i64, i64, u64, pd, pd,
+                                         C.POINTER(i32)]
+        L.or_mc_european_qmc.restype = None
         L.or_lsm_price.argtypes = [C.POINTER(OrLsmOption), pd, pd, pd,
                                    C.POINTER(OrLsmBoundary), C.POINTER(i64)]
         L.or_lsm_price.restype = C.c_int
@@ -328,3 +334,30 @@ def xoshiro_advance(state, k):
     s = (C.c_uint64 * 4)(*[int(x) & (2**64 - 1) for x in state])
     lib().or_xoshiro_advance(s, int(k))
     return list(s)
+
+
+def sobol_points(dims, seed, n):
+    """(words uint64 [n][dims], uniforms [n][dims]) of SobolSequence::new(dims,
+    seed) stepped n times."""
+    w = np.zeros((n, dims), dtype=np.uint64)
+    u = np.zeros((n, dims))
+    lib().or_sobol_points(dims, seed, n, w.ctypes.data_as(C.POINTER(C.c_uint64)), _ptr(u))
+    return w, u
+
+
+def sobol_directions(dim, seed):
+    v = (C.c_uint64 * 64)()
+    lib().or_sobol_directions(dim, seed, v)
+    return [int(x) for x in v]
+
+
+def sobol_scrambler(dim, seed):
+    return int(lib().or_sobol_scrambler(dim, seed))
+
+
+def mc_european_qmc(o, n_paths, n_steps, seed):
+    """mc_european_qmc_with_seed: (price, stderr or None)."""
+    p, se, has = C.c_double(), C.c_double(), C.c_int32()
+    lib().or_mc_european_qmc(C.byref(o), n_paths, n_steps, seed, C.byref(p), C.byref(se),
+                             C.byref(has))
+    return p.value, (se.value if has.value else None)

@@ -186,6 +186,63 @@ int main() {
         expect(std::fabs(out.first + inn.first - vanilla) <= 2.5 * (out.second + inn.second),
                "barrier_mc_parity_matches_vanilla");
     }
+    {  // tests/lsm_reference.rs: Longstaff-Schwartz Table 1 (100k x 50, seed 42, 0.10)
+        const double table[5][5] = {{36.0, 4.474, 4.831, 7.081, 8.480},
+                                    {38.0, 3.242, 3.746, 6.141, 7.647},
+                                    {40.0, 2.319, 2.875, 5.318, 6.912},
+                                    {42.0, 1.615, 2.216, 4.586, 6.241},
+                                    {44.0, 1.100, 1.678, 3.942, 5.642}};
+        const LongstaffSchwartzEngine lsm(100000, 50, 42);
+        double worst = 0.0;
+        for (const auto &row : table) {
+            // columns: (sigma, T) = (0.2, 1), (0.2, 2), (0.4, 1), (0.4, 2)
+            for (int c = 1; c <= 4; ++c) {
+                const double v = c <= 2 ? 0.2 : 0.4;
+                const double T = (c % 2) ? 1.0 : 2.0;
+                const Market mk = flat(row[0], 0.06, 0.0, v);
+                const double px = lsm.price(VanillaOption::american_put(40.0, T), mk).price;
+                worst = std::fmax(worst, std::fabs(px - row[c]));
+            }
+        }
+        expect(worst <= 0.10, "lsm_table1_american_puts_within_0_10", std::to_string(worst));
+        const Market mk = flat(40.0, 0.06, 0.0, 0.2);
+        const auto am = lsm.price(VanillaOption::american_put(40.0, 1.0), mk);
+        const auto eu = lsm.price(VanillaOption::european_put(40.0, 1.0), mk);
+        expect(am.price > eu.price, "lsm_american_put_above_european");
+        expect(am.diagnostics.get("num_paths").value_or(0) == 100000.0 &&
+                   am.diagnostics.get("vol").value_or(0) == 0.2,
+               "lsm_diagnostics");
+    }
+    {  // tests/bermudan_quantlib.rs: step-down strikes and Heston dynamics
+        const Market mk = flat(100.0, 0.03, 0.0, 0.25);
+        BermudanOption step{OptionType::Put, 1.0, {0.25, 0.5, 0.75, 1.0}, {100.0, 97.5, 95.0, 92.5}};
+        auto cst = BermudanOption::with_constant_strike(OptionType::Put, 100.0, 1.0,
+                                                        {0.25, 0.5, 0.75, 1.0});
+        const LongstaffSchwartzEngine lsm(250000, 100, 42);
+        expect(lsm.price(step, mk).price < lsm.price(cst, mk).price,
+               "bermudan_step_down_strike_prices_below_constant_strike");
+        const Market mh = flat(100.0, 0.02, 0.01, 0.20);
+        auto inst = BermudanOption::with_constant_strike(OptionType::Put, 100.0, 1.0,
+                                                         {0.2, 0.4, 0.6, 0.8, 1.0});
+        const auto out = LongstaffSchwartzEngine(220000, 100, 321)
+                             .with_heston_dynamics(Heston{0.0, 2.0, 0.04, 0.6, -0.5, 0.04})
+                             .price_bermudan_with_boundary(inst, mh);
+        bool any = false;
+        for (const auto &pt : out.exercise_boundary) any = any || pt.boundary_spot.has_value();
+        expect(std::isfinite(out.result.price) && out.result.price > 0.0 && any,
+               "bermudan_lsm_heston_dynamics_boundary");
+        expect(out.result.diagnostics.get("exercise_dates").value_or(0) == 5.0 &&
+                   !out.result.diagnostics.get("vol"),
+               "bermudan_lsm_diagnostics");
+    }
+    {  // mc_qmc.rs:184-197: QMC within 0.1 % of Black-Scholes at 10k points
+        const auto q = mc_european_qmc_with_seed(call, m, 10000, 1, 11);
+        expect(std::fabs(q.price - BS_CALL) / BS_CALL <= 0.001,
+               "qmc_price_matches_black_scholes_with_tight_error_at_10k_paths",
+               std::to_string(q.price));
+        const auto am = mc_european_qmc(VanillaOption::american_put(100.0, 1.0), m, 100, 4);
+        expect(std::isnan(am.price) && !am.stderr_, "qmc_non_european_is_nan");
+    }
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
 }
