@@ -322,6 +322,70 @@ def lsm_cpu_baseline(paths=100_000):
                       f"oracle C restatement"}
 
 
+QMC_OPTS, QMC_POINTS, QMC_DIMS = 64, 1 << 20, 64
+
+
+def qmc_workload(seed=WORKLOAD_SEED + 6):
+    """European calls/puts, K U[80,120], sigma U[0.1,0.5], r U[0,0.06],
+    T U[0.25,2], S0 = 100, each on its own Sobol seed."""
+    rng = np.random.default_rng(seed)
+    return [dict(option_type="call" if i % 2 == 0 else "put", spot=100.0,
+                 strike=float(rng.uniform(80, 120)), vol=float(rng.uniform(0.1, 0.5)),
+                 rate=float(rng.uniform(0.0, 0.06)), expiry=float(rng.uniform(0.25, 2.0)),
+                 seed=1000 + i) for i in range(QMC_OPTS)]
+
+
+def qmc_leg(reps=3, cpu=True):
+    """Sobol quasi-Monte Carlo: 64 European options, 2^20 points x 64
+    dimensions each, one launch (mc_european_qmc_with_seed per option)."""
+    from openferric_amd import _lib, qmc_price_raw
+    opts = qmc_workload()
+    structs = []
+    for d in opts:
+        o = _lib.OfgOption()
+        o.model, o.payoff = _lib.MODEL_GBM, _lib.PAYOFF_VANILLA
+        o.option_type = _lib.CALL if d["option_type"] == "call" else _lib.PUT
+        o.spot, o.strike, o.expiry, o.rate, o.vol = d["spot"], d["strike"], d["expiry"], \
+            d["rate"], d["vol"]
+        o.num_paths, o.num_steps, o.seed = QMC_POINTS, QMC_DIMS, d["seed"]
+        structs.append(o)
+    qmc_price_raw(structs)
+    ms = [qmc_price_raw(structs)[0].kernel_ms for _ in range(reps)]
+    k = float(np.mean(ms))
+    ps = QMC_OPTS * QMC_POINTS * QMC_DIMS
+    v = ps / (k * 1e-3)
+    out = {"workload": f"{QMC_OPTS} European options, 2^20 Sobol points x {QMC_DIMS} "
+                       "dimensions each (reference Sobol points), one launch",
+           "path_steps": ps, "kernel_ms": k, "path_steps_per_s": v,
+           "frac_fp64_peak": v * FLOP_GBM_STEP / 1e12 / FP64_PEAK_TFLOPS}
+    if cpu:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import oracle as O
+        O.build()
+        threads = cpu_threads()
+        pts = 1 << 17
+        sample = opts[:2 * threads]
+
+        def one(d):
+            oo = O.make_option(option_type=O.CALL if d["option_type"] == "call" else O.PUT,
+                               spot=d["spot"], strike=d["strike"], expiry=d["expiry"],
+                               rate=d["rate"], vol=d["vol"])
+            return O.mc_european_qmc(oo, pts, QMC_DIMS, d["seed"])
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            list(ex.map(one, sample))
+        dt = time.perf_counter() - t0
+        cps = len(sample) * pts * QMC_DIMS / dt
+        out["cpu_baseline"] = {"value": cps, "unit": "path-steps/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{len(sample)} options at 2^17 points x {QMC_DIMS} "
+                                         f"dimensions, one per thread, {dt:.2f} s, oracle C "
+                                         "restatement (sequential next_into as the reference)"}
+        out["speedup_vs_cpu"] = v / cps
+    return out
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (a plain copy kernel of known
     size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -404,6 +468,9 @@ def main():
     ap.add_argument("--no-lsm", action="store_true", help="skip the Longstaff-Schwartz leg")
     ap.add_argument("--lsm-only", action="store_true",
                     help="run only the Longstaff-Schwartz leg (profiling)")
+    ap.add_argument("--no-qmc", action="store_true", help="skip the quasi-Monte Carlo leg")
+    ap.add_argument("--qmc-only", action="store_true",
+                    help="run only the quasi-Monte Carlo leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one D2D copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -420,6 +487,9 @@ def main():
 
     if args.lsm_only:
         print(json.dumps({"lsm": lsm_leg(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.qmc_only:
+        print(json.dumps({"qmc": qmc_leg(cpu=not args.no_cpu)}), flush=True)
         return
 
     opts = make_workload(path_scale=args.path_scale)
@@ -496,6 +566,9 @@ def main():
     if not args.no_lsm:
         print("[bench] longstaff-schwartz leg", file=sys.stderr, flush=True)
         out["lsm"] = lsm_leg(cpu=not args.no_cpu)
+    if not args.no_qmc:
+        print("[bench] quasi-monte-carlo leg", file=sys.stderr, flush=True)
+        out["qmc"] = qmc_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(opts)
