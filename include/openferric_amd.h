@@ -179,6 +179,35 @@ typedef struct ofg_lsm_boundary {
     int64_t itm_paths, exercised_paths;
 } ofg_lsm_boundary;
 
+/* ---- multi-asset: worst-of autocallables ------------------------------ */
+/* Autocallable / PhoenixAutocallable (src/instruments/autocallable.rs) with
+ * the market inputs of price_autocallable (src/pricing/autocallable.rs:52). */
+typedef struct ofg_autocall {
+    int32_t phoenix;          /* 0 Autocallable, 1 PhoenixAutocallable */
+    int32_t memory;           /* Phoenix: memory coupons */
+    int32_t n_underlyings;    /* contract underlyings (at most 8) */
+    int32_t n_dates;
+    const int32_t *underlyings;      /* indices into the market arrays */
+    const double *autocall_dates;
+    double notional, maturity, autocall_barrier, coupon_rate;
+    double coupon_barrier;    /* Phoenix only */
+    double ki_barrier, ki_strike;
+    int32_t n_assets;         /* market arrays */
+    int32_t pad;
+    const double *spots, *vols;
+    const double *corr;       /* n_assets x n_assets, row-major */
+    double rate, div_yield;   /* r, q */
+    int64_t num_paths, num_steps;
+} ofg_autocall;
+
+/* AutocallableSensitivities (pricing/autocallable.rs:25-33). */
+typedef struct ofg_autocall_sens {
+    double delta[8];          /* per contract underlying */
+    double vega, cega;        /* cega NaN when no correlation bump was valid */
+    int32_t n_delta;
+    int32_t ok;               /* 0: the reference's sensitivities would fail */
+} ofg_autocall_sens;
+
 typedef struct ofg_ctx ofg_ctx;
 typedef struct ofg_batch ofg_batch;
 
@@ -288,6 +317,22 @@ int32_t ofg_lsm_price(ofg_ctx *ctx, const ofg_lsm_option *opts, size_t n,
  * stderr 0 for expiry <= 0; diagnostics (num_paths, num_steps, vol) NaN
  * where the reference records none. out[n]. */
 int32_t ofg_qmc_price(ofg_ctx *ctx, const ofg_option *opts, size_t n, ofg_result *out);
+
+/* Worst-of autocallables and Phoenix notes on correlated GBM paths: replaces
+ * price_autocallable / price_phoenix_autocallable and, with with_greeks,
+ * autocallable_sensitivities / phoenix_autocallable_sensitivities
+ * (src/pricing/autocallable.rs:52-231, bump-and-reprice :366-418), n contracts
+ * in one launch, every bump a scenario on the base price's draws.
+ * cfg->rng_mode: EXTERNAL (normals1 per contract [step][underlying][path],
+ * contracts concatenated) or FAST (per-path streams from the reference's
+ * seed 94210); the reference's single stream cannot be split across paths
+ * because a called path stops drawing. As price_autocallable, an invalid
+ * contract gives price NaN and has_stderr 0 (ofg_last_error holds the
+ * reference's message); greeks: delta = sum of deltas, vega, rho = cega.
+ * sens (may be NULL): the per-underlying sensitivities. out[n], sens[n]. */
+int32_t ofg_autocall_price(ofg_ctx *ctx, const ofg_autocall *contracts, size_t n,
+                           const ofg_engine_cfg *cfg, int32_t with_greeks, ofg_result *out,
+                           ofg_autocall_sens *sens);
 
 /* Device time of the context's last ofg_lsm_price call, split into path
  * simulation (seed and path kernels) and the backward rounds, and the number

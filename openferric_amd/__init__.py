@@ -18,6 +18,10 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       lsm_last_timing, lsm_option_struct, lsm_price_raw, stream_jump,
                       mc_european_qmc, mc_european_qmc_with_seed, qmc_price_raw,
                       option_struct, price_raw, sample_payoffs)
+from .multi import (Autocallable, AutocallableSensitivities, PhoenixAutocallable,
+                    autocall_price_raw, autocall_struct, autocallable_sensitivities,
+                    phoenix_autocallable_sensitivities, price_autocallable,
+                    price_phoenix_autocallable)
 from ._lib import DeviceError
 
 __all__ = [
@@ -33,4 +37,7 @@ __all__ = [
     "BermudanLsmOutput", "ExerciseBoundaryPoint", "LongstaffSchwartzEngine", "LsmDynamics",
     "lsm_last_timing", "lsm_option_struct", "lsm_price_raw", "stream_jump",
     "mc_european_qmc", "mc_european_qmc_with_seed", "qmc_price_raw",
+    "Autocallable", "AutocallableSensitivities", "PhoenixAutocallable", "autocall_price_raw",
+    "autocall_struct", "autocallable_sensitivities", "phoenix_autocallable_sensitivities",
+    "price_autocallable", "price_phoenix_autocallable",
 ]

@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = [
     "ofg_sample_payoffs", "ofg_price_greeks", "ofg_batch_create", "ofg_batch_run",
     "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
     "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump", "ofg_lsm_last_timing",
-    "ofg_qmc_price",
+    "ofg_qmc_price", "ofg_autocall_price",
 ]
 
 
@@ -109,6 +109,28 @@ class OfgLsmBoundary(C.Structure):
     ]
 
 
+class OfgAutocall(C.Structure):
+    _fields_ = [
+        ("phoenix", C.c_int32), ("memory", C.c_int32), ("n_underlyings", C.c_int32),
+        ("n_dates", C.c_int32), ("underlyings", C.POINTER(C.c_int32)),
+        ("autocall_dates", C.POINTER(C.c_double)),
+        ("notional", C.c_double), ("maturity", C.c_double), ("autocall_barrier", C.c_double),
+        ("coupon_rate", C.c_double), ("coupon_barrier", C.c_double),
+        ("ki_barrier", C.c_double), ("ki_strike", C.c_double),
+        ("n_assets", C.c_int32), ("pad", C.c_int32),
+        ("spots", C.POINTER(C.c_double)), ("vols", C.POINTER(C.c_double)),
+        ("corr", C.POINTER(C.c_double)), ("rate", C.c_double), ("div_yield", C.c_double),
+        ("num_paths", C.c_int64), ("num_steps", C.c_int64),
+    ]
+
+
+class OfgAutocallSens(C.Structure):
+    _fields_ = [
+        ("delta", C.c_double * 8), ("vega", C.c_double), ("cega", C.c_double),
+        ("n_delta", C.c_int32), ("ok", C.c_int32),
+    ]
+
+
 class DeviceError(RuntimeError):
     """HIP library missing or device failure (OFG_DEVICE_ERROR)."""
 
@@ -167,6 +189,10 @@ def load() -> C.CDLL:
         L.ofg_lsm_last_timing.restype = C.c_int32
         L.ofg_qmc_price.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t, C.POINTER(OfgResult)]
         L.ofg_qmc_price.restype = C.c_int32
+        L.ofg_autocall_price.argtypes = [pv, C.POINTER(OfgAutocall), C.c_size_t,
+                                         C.POINTER(OfgEngineCfg), C.c_int32,
+                                         C.POINTER(OfgResult), C.POINTER(OfgAutocallSens)]
+        L.ofg_autocall_price.restype = C.c_int32
         _lib = L
         return L
 

@@ -94,6 +94,7 @@ inline int32_t guarded(F &&f) {
 
 struct LsmBuffers;   // ofg_lsm_api.cpp
 struct QmcBuffers;   // ofg_qmc_api.cpp
+struct MultiBuffers; // ofg_multi_api.cpp
 
 // Instrument validation with the reference's messages (instruments/*.rs
 // validate()); throws OfgError.
@@ -111,6 +112,7 @@ struct ofg_ctx {
     std::map<std::string, ofg_host::Tuning> tuned;
     ofg_host::LsmBuffers *lsm = nullptr;   // Longstaff-Schwartz work buffers
     ofg_host::QmcBuffers *qmc = nullptr;   // quasi-Monte Carlo tables and partials
+    ofg_host::MultiBuffers *multi = nullptr;   // multi-asset scenarios and partials
     int cus = 256;
 };
 
@@ -119,4 +121,6 @@ namespace ofg_host {
 void lsm_release(ofg_ctx *ctx);
 // Releases the context's quasi-Monte Carlo buffers (ofg_qmc_api.cpp).
 void qmc_release(ofg_ctx *ctx);
+// Releases the context's multi-asset buffers (ofg_multi_api.cpp).
+void multi_release(ofg_ctx *ctx);
 }  // namespace ofg_host

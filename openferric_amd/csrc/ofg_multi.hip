@@ -1,0 +1,208 @@
+// Multi-asset path kernels for MI355X (gfx950), FP64 on the VALU: worst-of
+// autocallables and Phoenix notes (simulate_autocallable_paths,
+// src/pricing/autocallable.rs:441-579).
+//
+// Per path, the underlyings' spots live in registers (up to MA_MAX); each
+// step draws one normal per underlying, correlates them with the scenario's
+// lower Cholesky factor (correlate_normals, :659-667: a running sum from 0
+// in column order), moves each spot by exp(drift + vol_dt * corr) floored at
+// 1e-12, takes the worst performance s_i / s0_i, records a knock-in
+// (worst <= ki_barrier) and, on an observation step, pays Phoenix coupons
+// (with memory) and calls the note when worst >= autocall_barrier. A called
+// lane leaves the step loop; the wave continues with the rest.
+//
+// Draws: external normals [step][asset][path] of the contract (the same for
+// all of its scenarios), or fast per-path xoroshiro128+ Box-Muller streams
+// seeded by stream_seed(seed, path) (one pair per two underlyings per step).
+// Reductions: per-tile (sum, sum of squares) of pv - shift in the tile's own
+// slot, then one block per scenario sums them in tile order.
+#include "ofg_multi.hpp"
+#include "ofg_internal.hpp"
+#include "ofg_math.hpp"
+
+namespace ofg {
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int NWAVES = MA_BLOCK / WAVE;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+
+template <int RNG>
+__device__ __forceinline__ double autocall_path(const AcDev &o, const AcObs *obs, int64_t p,
+                                                const double *normals, const double *tab) {
+    constexpr bool FAST = RNG == OFG_RNG_FAST;
+    const int n = o.n;
+    double s[MA_MAX];
+#pragma unroll
+    for (int i = 0; i < MA_MAX; ++i) s[i] = o.s0[i];
+    Xoroshiro128p rng;
+    if constexpr (FAST) rng.seed(stream_seed(o.seed, (uint64_t)p));
+    const double *zp = normals + o.normals_offset + p;
+    int k = 0;
+    int next_step = o.n_obs > 0 ? obs[0].step : 0x7fffffff;
+    bool called = false, ki = false;
+    double worst_final = 1.0, pv = 0.0, pending = 0.0, prev_t = 0.0;
+    for (int step = 1; step <= o.steps; ++step) {
+        double z[MA_MAX];
+        if constexpr (FAST) {
+#pragma unroll
+            for (int i = 0; i < MA_MAX; i += 2) {
+                if (i < n) {
+                    const uint64_t wu = rng.next_u64();
+                    const uint64_t wt = rng.next_u64();
+                    box_muller_pair(wu, (uint32_t)(wt >> 32), z[i], z[i + 1], tab);
+                }
+            }
+        } else {
+            const int64_t row = (int64_t)(step - 1) * n;
+#pragma unroll
+            for (int i = 0; i < MA_MAX; ++i)
+                if (i < n) z[i] = zp[(row + i) * o.paths];
+        }
+        double worst = INFINITY;
+#pragma unroll
+        for (int i = 0; i < MA_MAX; ++i) {
+            if (i < n) {
+                const double *li = o.chol + i * (i + 1) / 2;
+                double c = 0.0;
+#pragma unroll
+                for (int j = 0; j <= i; ++j) c += li[j] * z[j];
+                double g;
+                if constexpr (FAST) {
+                    double y = o.drift2[i] + o.vol_dt2[i] * c;
+                    y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
+                    g = exp2_256(y, tab);
+                } else {
+                    g = exp_fp64(o.drift[i] + o.vol_dt[i] * c);
+                }
+                s[i] = fmax(s[i] * g, 1.0e-12);
+                worst = fmin(worst, s[i] / o.s0[i]);
+            }
+        }
+        worst_final = worst;
+        if (worst <= o.ki_barrier) ki = true;
+        if (step == next_step) {
+            const AcObs ob = obs[k];
+            if (o.phoenix) {
+                const double accrual = o.notional * o.coupon_rate * fmax(ob.time - prev_t, 0.0);
+                if (worst >= o.coupon_barrier) {
+                    const double due = o.memory ? pending + accrual : accrual;
+                    pv += ob.df * due;
+                    pending = 0.0;
+                } else if (o.memory) {
+                    pending += accrual;
+                }
+                prev_t = ob.time;
+            }
+            if (worst >= o.autocall_barrier) {
+                if (o.phoenix) pv += ob.df * o.notional;
+                else pv = ob.call_pv;
+                called = true;
+                break;
+            }
+            ++k;
+            next_step = k < o.n_obs ? obs[k].step : 0x7fffffff;
+        }
+    }
+    if (!called) {
+        double redemption;
+        if (ki) redemption = o.notional * (worst_final / o.ki_strike);
+        else if (!o.phoenix) redemption = o.redeem_plain;
+        else redemption = o.notional;
+        pv += o.df_maturity * redemption;
+    }
+    return pv;
+}
+
+template <int RNG>
+__global__ __launch_bounds__(MA_BLOCK) void autocall_kernel(AcArgs a) {
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    __shared__ double red[NWAVES][2];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += MA_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    }
+    const int tile = blockIdx.x;
+    const int sc = a.tile_scen[tile];
+    const AcDev &o = a.scen[sc];
+    const AcObs *obs = a.obs + o.obs_offset;
+    const int64_t base = (int64_t)(tile - o.tile_first) * MA_BLOCK * o.ppt;
+    double sum = 0.0, sq = 0.0;
+    for (int j = 0; j < o.ppt; ++j) {
+        const int64_t p = base + (int64_t)j * MA_BLOCK + threadIdx.x;
+        if (p >= o.paths) break;
+        const double x = autocall_path<RNG>(o, obs, p, a.normals, tab) - o.shift;
+        sum += x;
+        sq += x * x;
+    }
+    sum = wave_sum(sum);
+    sq = wave_sum(sq);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = sum;
+        red[wave][1] = sq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = red[0][0], q = red[0][1];
+        for (int w = 1; w < NWAVES; ++w) {
+            s += red[w][0];
+            q += red[w][1];
+        }
+        a.partials[tile].sum = s;
+        a.partials[tile].sum_sq = q;
+    }
+}
+
+// One block per scenario: tile slots in tile order; mean and standard error
+// of the discounted payoffs (autocallable.rs:566-578).
+__global__ __launch_bounds__(MA_BLOCK) void autocall_close_kernel(AcArgs a) {
+    __shared__ double red[NWAVES][2];
+    const int sc = blockIdx.x;
+    const AcDev &o = a.scen[sc];
+    double s = 0.0, q = 0.0;
+    for (int t = threadIdx.x; t < o.n_tiles; t += MA_BLOCK) {
+        s += a.partials[o.tile_first + t].sum;
+        q += a.partials[o.tile_first + t].sum_sq;
+    }
+    s = wave_sum(s);
+    q = wave_sum(q);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = s;
+        red[wave][1] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double sum = red[0][0], sq = red[0][1];
+    for (int w = 1; w < NWAVES; ++w) {
+        sum += red[w][0];
+        sq += red[w][1];
+    }
+    const double n = (double)o.paths;
+    const double var = o.paths > 1 ? fmax((sq - sum * sum / n) / (n - 1.0), 0.0) : 0.0;
+    a.out[sc].price = o.shift + sum / n;
+    a.out[sc].stderr_ = sqrt(var / n);
+}
+
+}  // namespace
+
+hipError_t launch_autocall(const AcArgs &a, hipStream_t stream) {
+    if (a.n_tiles <= 0) return hipSuccess;
+    if (a.rng_mode == OFG_RNG_FAST)
+        autocall_kernel<OFG_RNG_FAST><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+    else
+        autocall_kernel<OFG_RNG_EXTERNAL><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    autocall_close_kernel<<<a.n_scen, MA_BLOCK, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace ofg

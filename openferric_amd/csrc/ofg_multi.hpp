@@ -1,0 +1,75 @@
+// Device layouts and launch interface of the multi-asset path kernels
+// (ofg_multi.hip): worst-of autocallables and Phoenix notes on correlated GBM
+// paths (src/pricing/autocallable.rs). Shared with ofg_multi_api.cpp.
+//
+// A scenario is one contract under one set of market inputs (the base price,
+// or one bump of the reference's bump-and-reprice sensitivities); scenarios
+// of a contract share its draws, as the reference reprices every bump with
+// the same seed. Up to MA_MAX underlyings are held in registers per path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ofg {
+
+constexpr int MA_MAX = 8;
+constexpr int MA_BLOCK = 256;
+constexpr int MA_MAX_TILES = 1024;
+
+struct alignas(16) AcDev {
+    int32_t n;             // underlyings of the contract
+    int32_t phoenix;
+    int32_t memory;
+    int32_t n_obs;         // observation schedule entries
+    int32_t steps;
+    int32_t obs_offset;    // into the AcObs array
+    int32_t tile_first;
+    int32_t n_tiles;
+    int32_t ppt;           // paths per thread
+    int32_t pad;
+    int64_t paths;
+    int64_t normals_offset;   // external normals of the contract, [step][asset][path]
+    uint64_t seed;            // fast draws: stream_seed(seed, path)
+    double s0[MA_MAX], drift[MA_MAX], vol_dt[MA_MAX];
+    double drift2[MA_MAX], vol_dt2[MA_MAX];   // fast mode: scaled by 256/ln2
+    double chol[MA_MAX * (MA_MAX + 1) / 2];   // lower factor, row i at i(i+1)/2
+    double notional, maturity, autocall_barrier, coupon_rate, coupon_barrier;
+    double ki_barrier, ki_strike;
+    double df_maturity;       // exp(-r maturity)
+    double redeem_plain;      // notional (1 + c maturity): standard note, no knock-in
+    double shift;             // payoffs are summed as pv - shift (variance accuracy)
+};
+
+// One observation date: its path step, time, exp(-r t) and, for a standard
+// note, the called payoff exp(-r t) N (1 + c t).
+struct AcObs {
+    int32_t step;
+    int32_t pad;
+    double time, df, call_pv;
+};
+
+struct AcPartial {
+    double sum, sum_sq;
+};
+
+struct AcOut {
+    double price, stderr_;
+};
+
+struct AcArgs {
+    const AcDev *scen;
+    const int32_t *tile_scen;
+    const AcObs *obs;
+    AcPartial *partials;
+    AcOut *out;
+    const double *normals;
+    int32_t n_tiles;
+    int32_t n_scen;
+    int32_t rng_mode;
+    int32_t pad;
+};
+
+hipError_t launch_autocall(const AcArgs &a, hipStream_t stream);
+
+}  // namespace ofg

@@ -1,0 +1,196 @@
+"""Multi-asset contracts on the GPU: worst-of autocallables and Phoenix notes.
+
+Mirrors src/instruments/autocallable.rs (Autocallable, PhoenixAutocallable)
+and src/pricing/autocallable.rs (price_autocallable,
+autocallable_sensitivities, price_phoenix_autocallable,
+phoenix_autocallable_sensitivities) over the C ABI's ofg_autocall_price.
+Draws: the reference walks one xoshiro256++ stream whose position depends on
+when earlier paths were called, which no parallel simulator can reproduce;
+`rng_mode` is Fast (per-path streams from the reference's seed 94210) or
+External (caller normals, [path][step][underlying] per contract).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .core import Diagnostics, Greeks, InvalidInput, PricingResult
+from .engines import RngMode, _Keep, _raise
+
+
+@dataclass
+class Autocallable:
+    """instruments/autocallable.rs:18-34"""
+    underlyings: Sequence[int]
+    notional: float
+    autocall_dates: Sequence[float]
+    autocall_barrier: float
+    coupon_rate: float
+    ki_barrier: float
+    ki_strike: float
+    maturity: float
+
+    def validate(self) -> None:
+        if not self.underlyings:
+            raise InvalidInput("autocallable underlyings cannot be empty")
+        if self.notional <= 0.0:
+            raise InvalidInput("autocallable notional must be > 0")
+        if self.maturity <= 0.0:
+            raise InvalidInput("autocallable maturity must be > 0")
+        if not self.autocall_dates:
+            raise InvalidInput("autocallable autocall_dates cannot be empty")
+        if self.coupon_rate < 0.0:
+            raise InvalidInput("autocallable coupon_rate must be >= 0")
+        if self.autocall_barrier <= 0.0:
+            raise InvalidInput("autocallable autocall_barrier must be > 0")
+        if self.ki_barrier <= 0.0:
+            raise InvalidInput("autocallable ki_barrier must be > 0")
+        if self.ki_strike <= 0.0:
+            raise InvalidInput("autocallable ki_strike must be > 0")
+        if any(t <= 0.0 or t > self.maturity for t in self.autocall_dates):
+            raise InvalidInput("autocallable dates must lie in (0, maturity]")
+        if any(b <= a for a, b in zip(self.autocall_dates, self.autocall_dates[1:])):
+            raise InvalidInput("autocallable dates must be strictly increasing")
+        if len(set(self.underlyings)) != len(self.underlyings):
+            raise InvalidInput("autocallable underlyings must be unique")
+
+    def instrument_type(self) -> str:
+        return "Autocallable"
+
+
+@dataclass
+class PhoenixAutocallable:
+    """instruments/autocallable.rs:113-157"""
+    underlyings: Sequence[int]
+    notional: float
+    autocall_dates: Sequence[float]
+    autocall_barrier: float
+    coupon_barrier: float
+    coupon_rate: float
+    memory: bool
+    ki_barrier: float
+    ki_strike: float
+    maturity: float
+
+    def validate(self) -> None:
+        Autocallable(self.underlyings, self.notional, self.autocall_dates,
+                     self.autocall_barrier, self.coupon_rate, self.ki_barrier, self.ki_strike,
+                     self.maturity).validate()
+        if self.coupon_barrier <= 0.0:
+            raise InvalidInput("phoenix coupon_barrier must be > 0")
+
+    def instrument_type(self) -> str:
+        return "PhoenixAutocallable"
+
+
+@dataclass
+class AutocallableSensitivities:
+    """pricing/autocallable.rs:25-33"""
+    delta: List[float]
+    vega: float
+    cega: float
+
+
+def autocall_struct(contract, spots, vols, corr_matrix, r, q, n_paths, n_steps,
+                    keep: _Keep) -> _lib.OfgAutocall:
+    """Flattens a contract and its market inputs into the C ABI's ofg_autocall."""
+    a = _lib.OfgAutocall()
+    phoenix = isinstance(contract, PhoenixAutocallable)
+    a.phoenix = int(phoenix)
+    a.memory = int(bool(getattr(contract, "memory", False)))
+    und = np.ascontiguousarray(list(contract.underlyings), dtype=np.int32)
+    keep.refs.append(und)
+    a.underlyings = und.ctypes.data_as(C.POINTER(C.c_int32))
+    a.n_underlyings = len(und)
+    a.autocall_dates = keep.arr(list(contract.autocall_dates)) if contract.autocall_dates else None
+    a.n_dates = len(contract.autocall_dates)
+    a.notional, a.maturity = contract.notional, contract.maturity
+    a.autocall_barrier, a.coupon_rate = contract.autocall_barrier, contract.coupon_rate
+    a.coupon_barrier = contract.coupon_barrier if phoenix else 0.0
+    a.ki_barrier, a.ki_strike = contract.ki_barrier, contract.ki_strike
+    a.n_assets = len(spots)
+    a.spots = keep.arr(list(spots)) if len(spots) else None
+    if len(vols) != len(spots):
+        raise InvalidInput("autocallable spots and vols lengths must match")
+    a.vols = keep.arr(list(vols)) if len(vols) else None
+    cm = np.asarray(corr_matrix, dtype=np.float64)
+    if cm.shape != (len(spots), len(spots)):
+        raise InvalidInput("autocallable correlation matrix dimensions must match assets")
+    a.corr = keep.arr(cm.ravel()) if len(spots) else None
+    a.rate, a.div_yield = r, q
+    a.num_paths, a.num_steps = int(n_paths), int(n_steps)
+    a._keep = keep
+    return a
+
+
+def autocall_price_raw(structs, rng_mode=RngMode.Fast, normals=None, with_greeks=False,
+                       device: int = 0):
+    """Calls ofg_autocall_price. normals: per contract [path][step][underlying]
+    arrays (External mode). Returns (results, sensitivities)."""
+    from .engines import engine_cfg
+    L = _lib.load()
+    ctx = _lib.context(device)
+    n = len(structs)
+    z = None
+    if normals is not None:
+        # the C ABI's layout: [step][underlying][path] per contract
+        z = np.concatenate([np.ascontiguousarray(np.transpose(np.asarray(x), (1, 2, 0))).ravel()
+                            for x in normals])
+    cfg = engine_cfg(rng_mode=rng_mode, normals1=z)
+    arr = (_lib.OfgAutocall * n)(*structs)
+    out = (_lib.OfgResult * n)()
+    sens = (_lib.OfgAutocallSens * n)()
+    _raise(L.ofg_autocall_price(ctx, arr, n, C.byref(cfg), 1 if with_greeks else 0, out, sens))
+    return list(out), list(sens)
+
+
+def _result(r: _lib.OfgResult, n_dates: int, greeks: bool) -> PricingResult:
+    diag = Diagnostics()
+    if r.has_stderr:
+        diag.insert("num_paths", r.num_paths)
+        diag.insert("num_steps", r.num_steps)
+        diag.insert("observation_count", float(n_dates))
+    res = PricingResult(price=r.price, stderr=r.stderr_ if r.has_stderr else None,
+                        diagnostics=diag, kernel_ms=r.kernel_ms)
+    if greeks and r.has_greeks:
+        res.greeks = Greeks(delta=r.delta, gamma=0.0, vega=r.vega, theta=0.0, rho=r.rho)
+    return res
+
+
+def price_autocallable(autocall: Autocallable, spots, vols, corr_matrix, r, q, n_paths, n_steps,
+                       rng_mode=RngMode.Fast, normals=None) -> PricingResult:
+    """price_autocallable (pricing/autocallable.rs:52-99): price, stderr and
+    the bump-and-reprice Greeks (delta = sum of deltas, vega, rho = cega)."""
+    keep = _Keep()
+    s = autocall_struct(autocall, spots, vols, corr_matrix, r, q, n_paths, n_steps, keep)
+    res, _ = autocall_price_raw([s], rng_mode, None if normals is None else [normals], True)
+    return _result(res[0], len(autocall.autocall_dates), True)
+
+
+def price_phoenix_autocallable(phoenix: PhoenixAutocallable, spots, vols, corr_matrix, r, q,
+                               n_paths, n_steps, rng_mode=RngMode.Fast,
+                               normals=None) -> PricingResult:
+    """price_phoenix_autocallable (pricing/autocallable.rs:135-190)."""
+    return price_autocallable(phoenix, spots, vols, corr_matrix, r, q, n_paths, n_steps,
+                              rng_mode, normals)
+
+
+def autocallable_sensitivities(autocall, spots, vols, corr_matrix, r, q, n_paths, n_steps,
+                               rng_mode=RngMode.Fast, normals=None) -> AutocallableSensitivities:
+    """autocallable_sensitivities / phoenix_autocallable_sensitivities
+    (pricing/autocallable.rs:101-133, 192-223): all bumps in one launch."""
+    autocall.validate()
+    keep = _Keep()
+    s = autocall_struct(autocall, spots, vols, corr_matrix, r, q, n_paths, n_steps, keep)
+    res, sens = autocall_price_raw([s], rng_mode, None if normals is None else [normals], True)
+    if not res[0].has_stderr or not sens[0].ok:
+        raise InvalidInput(_lib.last_error() or "autocallable sensitivities failed")
+    sv = sens[0]
+    return AutocallableSensitivities([sv.delta[k] for k in range(sv.n_delta)], sv.vega, sv.cega)
+
+
+phoenix_autocallable_sensitivities = autocallable_sensitivities

@@ -35,6 +35,10 @@
  *                                      exact counts), BS, CRR American
  *   or_xoshiro_advance                 brute-force stepping (pins the
  *                                      product's jump-ahead)
+ *   or_autocall_price, or_cholesky_*   zero-vol closed forms (coupons, memory,
+ *                                      call dates), never/always called,
+ *                                      certain knock-in vs e^{(r-q)T} and
+ *                                      Margrabe, reference monotonicity tests
  *   or_sobol_*, or_mc_european_qmc     reference QMC assertions, (0,m,1)-net
  *                                      stratification of every dimension,
  *                                      Gray-code identity, BS within 0.1 %
@@ -195,6 +199,31 @@ int or_lu3_solve(const double a[9], const double b[3], double x[3]);
  * BermudanOption; *n_boundary its length. Returns 0, 1 invalid input. */
 int or_lsm_price(const or_lsm_option *o, const double *normals, double *price,
                  double *stderr_out, or_lsm_boundary *boundary, int64_t *n_boundary);
+
+/* ---- multi-asset: autocallables (oracle_multi.c) --------------------- */
+typedef struct {
+    int32_t n_assets;     /* the contract's underlyings, already selected */
+    int32_t phoenix;      /* 0: Autocallable, 1: PhoenixAutocallable */
+    int32_t memory;       /* Phoenix memory coupons */
+    int32_t n_dates;
+    double notional, maturity, autocall_barrier, coupon_rate, coupon_barrier;
+    double ki_barrier, ki_strike, rate, div_yield;
+    const double *spots, *vols;   /* n_assets */
+    const double *corr;           /* n_assets x n_assets, row-major */
+    const double *dates;          /* autocall_dates */
+    int64_t num_paths, num_steps;
+    uint64_t seed;
+} or_autocall;
+
+int or_cholesky_lower(const double *a, int32_t n, double *l);
+int32_t or_observation_schedule(const double *dates, int32_t n_dates, double maturity,
+                                int64_t n_steps, int64_t *steps_out, double *times_out);
+/* simulate_autocallable_paths (pricing/autocallable.rs:441-579). normals:
+ * NULL for the engine's single stream, else [path][step][asset]. Returns 0,
+ * 1 invalid counts, 2 correlation not PSD. */
+int or_autocall_price(const or_autocall *a, const double *normals, double *price,
+                      double *stderr_out);
+void or_bump_corr_matrix(const double *c, int32_t n, double bump, double *out);
 
 /* ---- quasi-Monte Carlo (oracle_qmc.c) -------------------------------- */
 /* build_direction_numbers / scrambler of SobolSequence (math/sobol.rs) */

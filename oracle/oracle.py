@@ -74,6 +74,19 @@ class OrLsmBoundary(C.Structure):
     ]
 
 
+class OrAutocall(C.Structure):
+    _fields_ = [
+        ("n_assets", C.c_int32), ("phoenix", C.c_int32), ("memory", C.c_int32),
+        ("n_dates", C.c_int32),
+        ("notional", C.c_double), ("maturity", C.c_double), ("autocall_barrier", C.c_double),
+        ("coupon_rate", C.c_double), ("coupon_barrier", C.c_double), ("ki_barrier", C.c_double),
+        ("ki_strike", C.c_double), ("rate", C.c_double), ("div_yield", C.c_double),
+        ("spots", C.POINTER(C.c_double)), ("vols", C.POINTER(C.c_double)),
+        ("corr", C.POINTER(C.c_double)), ("dates", C.POINTER(C.c_double)),
+        ("num_paths", C.c_int64), ("num_steps", C.c_int64), ("seed", C.c_uint64),
+    ]
+
+
 def build(force: bool = False) -> str:
     """Compile liboracle.so with make (gcc); returns the library path."""
     if force or not os.path.exists(_LIB_PATH):
@@ -129,6 +142,10 @@ def lib():
         L.or_mc_greeks_estimate.argtypes = [C.POINTER(OrOption), i64, u64, i32, d, pd]
         L.or_mc_greeks_estimate.restype = C.c_int
         L.or_lu3_solve.argtypes = [pd, pd, pd]; L.or_lu3_solve.restype = C.c_int
+        L.or_autocall_price.argtypes = [C.POINTER(OrAutocall), pd, pd, pd]
+        L.or_autocall_price.restype = C.c_int
+        L.or_cholesky_lower.argtypes = [pd, i32, pd]; L.or_cholesky_lower.restype = C.c_int
+        L.or_bump_corr_matrix.argtypes = [pd, i32, d, pd]; L.or_bump_corr_matrix.restype = None
         L.or_sobol_directions.argtypes = [u64, u64, pu64]; L.or_sobol_directions.restype = None
         L.or_sobol_scrambler.argtypes = [u64, u64]; L.or_sobol_scrambler.restype = u64
         L.or_sobol_points.argtypes = [i64, u64, i64, pu64, pd]; L.or_sobol_points.restype = None
@@ -361,3 +378,42 @@ def mc_european_qmc(o, n_paths, n_steps, seed):
     lib().or_mc_european_qmc(C.byref(o), n_paths, n_steps, seed, C.byref(p), C.byref(se),
                              C.byref(has))
     return p.value, (se.value if has.value else None)
+
+
+def make_autocall(spots, vols, corr, dates, notional=100.0, maturity=1.0, autocall_barrier=1.0,
+                  coupon_rate=0.08, coupon_barrier=0.0, ki_barrier=0.2, ki_strike=1.0, rate=0.01,
+                  div_yield=0.0, phoenix=False, memory=False, num_paths=1000, num_steps=64,
+                  seed=94_210):
+    """OrAutocall over the contract's (already selected) underlyings."""
+    a = OrAutocall()
+    sp = np.ascontiguousarray(spots, dtype=np.float64)
+    vo = np.ascontiguousarray(vols, dtype=np.float64)
+    co = np.ascontiguousarray(corr, dtype=np.float64).ravel()
+    dt = np.ascontiguousarray(dates, dtype=np.float64)
+    a._keep = (sp, vo, co, dt)
+    a.n_assets, a.phoenix, a.memory, a.n_dates = len(sp), int(phoenix), int(memory), len(dt)
+    a.notional, a.maturity, a.autocall_barrier = notional, maturity, autocall_barrier
+    a.coupon_rate, a.coupon_barrier, a.ki_barrier, a.ki_strike = coupon_rate, coupon_barrier, \
+        ki_barrier, ki_strike
+    a.rate, a.div_yield = rate, div_yield
+    a.spots, a.vols, a.corr, a.dates = _ptr(sp), _ptr(vo), _ptr(co), _ptr(dt)
+    a.num_paths, a.num_steps, a.seed = num_paths, num_steps, seed
+    return a
+
+
+def autocall_price(a, normals=None):
+    """simulate_autocallable_paths: (mean, stderr). normals [path][step][asset]."""
+    z = np.ascontiguousarray(normals, dtype=np.float64).ravel() if normals is not None else None
+    p, se = C.c_double(), C.c_double()
+    rc = lib().or_autocall_price(C.byref(a), _ptr(z), C.byref(p), C.byref(se))
+    if rc != 0:
+        raise ValueError(f"or_autocall_price failed rc={rc}")
+    return p.value, se.value
+
+
+def cholesky_lower(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    l = np.zeros((n, n))
+    ok = lib().or_cholesky_lower(_ptr(a.ravel()), n, _ptr(l))
+    return l if ok else None
