@@ -208,6 +208,40 @@ typedef struct ofg_autocall_sens {
     int32_t ok;               /* 0: the reference's sensitivities would fail */
 } ofg_autocall_sens;
 
+/* BasketOption (src/instruments/basket.rs:14-34) with its market inputs
+ * (price_basket_mc, src/pricing/basket.rs:99-120). */
+typedef enum ofg_basket_type {
+    OFG_BASKET_AVERAGE = 0,   /* sum_i w_i S_i(T) */
+    OFG_BASKET_BEST_OF = 1,   /* max_i S_i(T)/S_i(0) */
+    OFG_BASKET_WORST_OF = 2   /* min_i S_i(T)/S_i(0) */
+} ofg_basket_type;
+
+typedef struct ofg_basket {
+    int32_t basket_type;      /* ofg_basket_type */
+    int32_t is_call;
+    int32_t n_weights;        /* Average: must equal n_assets */
+    int32_t n_assets;         /* at most 16 */
+    const double *weights;
+    double strike, maturity;
+    const double *spots, *vols, *dividends;
+    const double *corr;       /* n_assets x n_assets, row-major */
+    double rate;
+    int64_t num_paths;
+    uint64_t seed;            /* the reference's MC_SEED is 3148159 */
+} ofg_basket;
+
+/* BasketSensitivities (pricing/basket.rs:57-65) and the diagnostics of
+ * price_basket_mc (:149-151). */
+typedef struct ofg_basket_sens {
+    double delta[16];         /* per asset */
+    double vega, cega;        /* cega NaN when no correlation bump was valid */
+    double rho;               /* average off-diagonal input correlation */
+    int32_t n_delta;
+    int32_t ok;               /* 0: basket_sensitivities would fail */
+    int32_t repaired;         /* 1: the input matrix was not PSD and was projected */
+    int32_t pad;
+} ofg_basket_sens;
+
 typedef struct ofg_ctx ofg_ctx;
 typedef struct ofg_batch ofg_batch;
 
@@ -333,6 +367,30 @@ int32_t ofg_qmc_price(ofg_ctx *ctx, const ofg_option *opts, size_t n, ofg_result
 int32_t ofg_autocall_price(ofg_ctx *ctx, const ofg_autocall *contracts, size_t n,
                            const ofg_engine_cfg *cfg, int32_t with_greeks, ofg_result *out,
                            ofg_autocall_sens *sens);
+
+/* Basket options under the Gaussian copula on one exact terminal step:
+ * replaces price_basket_mc and, with with_sens, basket_sensitivities
+ * (src/pricing/basket.rs:99-160, 456-597; simulation :599-683), n baskets in
+ * one launch, every bump a scenario on the base price's draws. A correlation
+ * matrix that is valid but not PSD is projected to the nearest correlation
+ * matrix (Higham 2002 alternating projections, as
+ * validate_or_repair_correlation_matrix, src/math/correlation.rs:340-358).
+ * cfg->rng_mode: REFERENCE (the engine's single xoshiro256++ stream from
+ * `seed`, split across paths by jump-ahead: the reference's draws),
+ * EXTERNAL (normals1 per basket [asset][path], baskets concatenated) or
+ * FAST. As price_basket_mc, an invalid basket gives price NaN and
+ * has_stderr 0 (ofg_last_error holds the reference's message); diagnostics
+ * num_paths in out, rho in sens. out[n], sens[n] (sens may be NULL). */
+int32_t ofg_basket_price(ofg_ctx *ctx, const ofg_basket *baskets, size_t n,
+                         const ofg_engine_cfg *cfg, int32_t with_sens, ofg_result *out,
+                         ofg_basket_sens *sens);
+
+/* Nearest correlation matrix (nearest_correlation_matrix_higham,
+ * src/math/correlation.rs:271-335): Higham's alternating projections with
+ * the reference's defaults (tol 1e-10, 100 iterations) when min_eig < -1e-10,
+ * else a copy. Writes out (n x n), *was_repaired. Host-only. */
+int32_t ofg_repair_correlation(const double *corr, int32_t n, double *out,
+                               int32_t *was_repaired);
 
 /* Device time of the context's last ofg_lsm_price call, split into path
  * simulation (seed and path kernels) and the backward rounds, and the number

@@ -18,10 +18,12 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       lsm_last_timing, lsm_option_struct, lsm_price_raw, stream_jump,
                       mc_european_qmc, mc_european_qmc_with_seed, qmc_price_raw,
                       option_struct, price_raw, sample_payoffs)
-from .multi import (Autocallable, AutocallableSensitivities, PhoenixAutocallable,
-                    autocall_price_raw, autocall_struct, autocallable_sensitivities,
-                    phoenix_autocallable_sensitivities, price_autocallable,
-                    price_phoenix_autocallable)
+from .multi import (Autocallable, AutocallableSensitivities, BasketOption,
+                    BasketSensitivities, BasketType, PhoenixAutocallable, autocall_price_raw,
+                    autocall_struct, autocallable_sensitivities, basket_price_raw,
+                    basket_sensitivities, basket_struct, phoenix_autocallable_sensitivities,
+                    price_autocallable, price_basket_mc, price_phoenix_autocallable,
+                    validate_or_repair_correlation_matrix)
 from ._lib import DeviceError
 
 __all__ = [
@@ -38,6 +40,9 @@ __all__ = [
     "lsm_last_timing", "lsm_option_struct", "lsm_price_raw", "stream_jump",
     "mc_european_qmc", "mc_european_qmc_with_seed", "qmc_price_raw",
     "Autocallable", "AutocallableSensitivities", "PhoenixAutocallable", "autocall_price_raw",
+    "BasketOption", "BasketSensitivities", "BasketType", "basket_price_raw",
+    "basket_sensitivities", "basket_struct", "price_basket_mc",
+    "validate_or_repair_correlation_matrix",
     "autocall_struct", "autocallable_sensitivities", "phoenix_autocallable_sensitivities",
     "price_autocallable", "price_phoenix_autocallable",
 ]

@@ -31,13 +31,14 @@ GREEKS_NONE, GREEKS_ADJOINT, GREEKS_FD_CRN, GREEKS_FORWARD = 0, 1, 2, 3
 LSM_VANILLA, LSM_BERMUDAN, LSM_BARRIER = 0, 1, 2
 EUROPEAN, AMERICAN, BERMUDAN = 0, 1, 2
 LSM_GBM, LSM_LOCAL_VOL, LSM_HESTON = 0, 1, 2
+BASKET_AVERAGE, BASKET_BEST_OF, BASKET_WORST_OF = 0, 1, 2
 
 EXPORTED_SYMBOLS = [
     "ofg_create", "ofg_destroy", "ofg_last_error", "ofg_version", "ofg_price",
     "ofg_sample_payoffs", "ofg_price_greeks", "ofg_batch_create", "ofg_batch_run",
     "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
     "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump", "ofg_lsm_last_timing",
-    "ofg_qmc_price", "ofg_autocall_price",
+    "ofg_qmc_price", "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation",
 ]
 
 
@@ -131,6 +132,25 @@ class OfgAutocallSens(C.Structure):
     ]
 
 
+class OfgBasket(C.Structure):
+    _fields_ = [
+        ("basket_type", C.c_int32), ("is_call", C.c_int32), ("n_weights", C.c_int32),
+        ("n_assets", C.c_int32), ("weights", C.POINTER(C.c_double)),
+        ("strike", C.c_double), ("maturity", C.c_double),
+        ("spots", C.POINTER(C.c_double)), ("vols", C.POINTER(C.c_double)),
+        ("dividends", C.POINTER(C.c_double)), ("corr", C.POINTER(C.c_double)),
+        ("rate", C.c_double), ("num_paths", C.c_int64), ("seed", C.c_uint64),
+    ]
+
+
+class OfgBasketSens(C.Structure):
+    _fields_ = [
+        ("delta", C.c_double * 16), ("vega", C.c_double), ("cega", C.c_double),
+        ("rho", C.c_double), ("n_delta", C.c_int32), ("ok", C.c_int32),
+        ("repaired", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
 class DeviceError(RuntimeError):
     """HIP library missing or device failure (OFG_DEVICE_ERROR)."""
 
@@ -193,6 +213,13 @@ def load() -> C.CDLL:
                                          C.POINTER(OfgEngineCfg), C.c_int32,
                                          C.POINTER(OfgResult), C.POINTER(OfgAutocallSens)]
         L.ofg_autocall_price.restype = C.c_int32
+        L.ofg_basket_price.argtypes = [pv, C.POINTER(OfgBasket), C.c_size_t,
+                                       C.POINTER(OfgEngineCfg), C.c_int32,
+                                       C.POINTER(OfgResult), C.POINTER(OfgBasketSens)]
+        L.ofg_basket_price.restype = C.c_int32
+        L.ofg_repair_correlation.argtypes = [C.POINTER(C.c_double), C.c_int32,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.ofg_repair_correlation.restype = C.c_int32
         _lib = L
         return L
 

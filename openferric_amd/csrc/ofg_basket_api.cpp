@@ -1,0 +1,561 @@
+// Host side of ofg_basket_price: price_basket_mc and basket_sensitivities
+// (src/pricing/basket.rs:99-160, 456-597) for n baskets in one launch, and
+// ofg_repair_correlation (validate_or_repair_correlation_matrix and
+// nearest_correlation_matrix_higham, src/math/correlation.rs:271-358).
+//
+// Validation follows the reference's order and messages (BasketOption::
+// validate, instruments/basket.rs:36-60; validate_inputs_matrix and
+// validate_market_inputs, pricing/basket.rs:802-860; validate_correlation_
+// matrix, math/correlation.rs:220-248). As price_basket_mc, an invalid basket
+// yields a NaN price and no stderr (the message is left in ofg_last_error).
+//
+// Every scenario (the base price, each bump) is prepared as the reference's
+// simulate_basket_once_matrix prepares it: a zero maturity prices the payoff
+// of the spots on the host; otherwise the correlation matrix is repaired when
+// it is not PSD and factored by cholesky_lower_psd (tolerance 1e-12).
+#include "../../include/openferric_amd.h"
+#include "ofg_host.hpp"
+#include "ofg_internal.hpp"
+#include "ofg_jump.hpp"
+#include "ofg_math.hpp"
+#include "ofg_multi.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace ofg_host;
+
+namespace ofg_host {
+
+struct BasketBuffers {
+    DevBuf scen, tiles, partials, out, normals, seeds, polys, jobs;
+};
+
+void basket_release(ofg_ctx *ctx) {
+    if (!ctx || !ctx->basket) return;
+    BasketBuffers *b = ctx->basket;
+    for (DevBuf *d : {&b->scen, &b->tiles, &b->partials, &b->out, &b->normals, &b->seeds,
+                      &b->polys, &b->jobs})
+        d->release();
+    delete b;
+    ctx->basket = nullptr;
+}
+
+}  // namespace ofg_host
+
+namespace {
+
+constexpr double SPOT_BUMP_REL = 0.01, VOL_BUMP_ABS = 0.01, CORR_BUMP = 0.01;  // basket.rs:28-30
+constexpr double PSD_TOL = 1.0e-10;    // PsdProjectionConfig::default (correlation.rs:26-33)
+constexpr int PSD_MAX_ITER = 100;
+
+using Mat = std::vector<double>;   // row-major n x n
+
+// ---------------------------------------------------------------------------
+// Symmetric eigen-decomposition by cyclic Jacobi rotations (n <= 64). The
+// reference uses nalgebra's SymmetricEigen; both converge to the exact
+// decomposition, so they differ only by rounding.
+// ---------------------------------------------------------------------------
+void jacobi_eigen(const Mat &a_in, int n, std::vector<double> &w, Mat &v) {
+    Mat a = a_in;
+    v.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; ++i) {
+            diag += a[(size_t)i * n + i] * a[(size_t)i * n + i];
+            for (int j = i + 1; j < n; ++j) off += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+        }
+        if (off == 0.0 || off <= 1e-36 * diag) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[(size_t)p * n + q];
+                if (apq == 0.0) continue;
+                const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) /
+                                 (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {   // columns p, q
+                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = c * akp - s * akq;
+                    a[(size_t)k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {   // rows p, q
+                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = c * apk - s * aqk;
+                    a[(size_t)q * n + k] = s * apk + c * aqk;
+                }
+                a[(size_t)p * n + q] = a[(size_t)q * n + p] = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+                    v[(size_t)k * n + p] = c * vkp - s * vkq;
+                    v[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    w.resize((size_t)n);
+    for (int i = 0; i < n; ++i) w[(size_t)i] = a[(size_t)i * n + i];
+}
+
+Mat symmetrize(const Mat &m, int n) {
+    Mat out((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            out[(size_t)i * n + j] = 0.5 * (m[(size_t)i * n + j] + m[(size_t)j * n + i]);
+    return out;
+}
+
+double min_eigenvalue(const Mat &m, int n) {
+    std::vector<double> w;
+    Mat v;
+    jacobi_eigen(m, n, w, v);
+    return *std::min_element(w.begin(), w.end());
+}
+
+// project_psd (correlation.rs:608-617): V max(D, 0) V^T, symmetrized.
+Mat project_psd(const Mat &m, int n) {
+    std::vector<double> w;
+    Mat v;
+    jacobi_eigen(symmetrize(m, n), n, w, v);
+    Mat out((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k)
+                s += v[(size_t)i * n + k] * std::max(w[(size_t)k], 0.0) * v[(size_t)j * n + k];
+            out[(size_t)i * n + j] = s;
+        }
+    return symmetrize(out, n);
+}
+
+// validate_correlation_matrix (correlation.rs:220-248); throws.
+void validate_corr(const Mat &c, int n) {
+    for (int i = 0; i < n; ++i) {
+        const double di = c[(size_t)i * n + i];
+        if (!std::isfinite(di) || std::fabs(di - 1.0) > 1.0e-10)
+            fail(OFG_INVALID_INPUT, "correlation matrix diagonal must be 1");
+        for (int j = 0; j < n; ++j) {
+            const double rho = c[(size_t)i * n + j];
+            if (!std::isfinite(rho) || !(rho >= -1.0 && rho <= 1.0))
+                fail(OFG_INVALID_INPUT, "correlation entries must be finite and in [-1, 1]");
+            if (std::fabs(rho - c[(size_t)j * n + i]) > 1.0e-10)
+                fail(OFG_INVALID_INPUT, "correlation matrix must be symmetric");
+        }
+    }
+}
+
+// nearest_correlation_matrix_higham (correlation.rs:271-335).
+Mat nearest_correlation(const Mat &m, int n) {
+    Mat y = symmetrize(m, n);
+    for (int i = 0; i < n; ++i) y[(size_t)i * n + i] = 1.0;
+    Mat ds((size_t)n * n, 0.0);
+    for (int it = 0; it < PSD_MAX_ITER; ++it) {
+        Mat r((size_t)n * n);
+        for (size_t k = 0; k < r.size(); ++k) r[k] = y[k] - ds[k];
+        r = symmetrize(r, n);
+        Mat x = project_psd(r, n);
+        for (size_t k = 0; k < r.size(); ++k) ds[k] = x[k] - r[k];
+        for (int i = 0; i < n; ++i) x[(size_t)i * n + i] = 1.0;
+        x = symmetrize(x, n);
+        double diff = 0.0;
+        for (size_t k = 0; k < x.size(); ++k) diff += (x[k] - y[k]) * (x[k] - y[k]);
+        y = x;
+        if (std::sqrt(diff) < PSD_TOL) break;
+    }
+    y = project_psd(symmetrize(y, n), n);
+    for (int i = 0; i < n; ++i) y[(size_t)i * n + i] = 1.0;
+    y = symmetrize(y, n);
+    for (int i = 0; i < n; ++i) {
+        y[(size_t)i * n + i] = 1.0;
+        for (int j = i + 1; j < n; ++j) {
+            const double v = std::min(1.0, std::max(-1.0, y[(size_t)i * n + j]));
+            y[(size_t)i * n + j] = y[(size_t)j * n + i] = v;
+        }
+    }
+    return y;
+}
+
+// validate_or_repair_correlation_matrix (correlation.rs:340-358); throws.
+Mat validate_or_repair(const Mat &c, int n, bool &repaired) {
+    validate_corr(c, n);
+    repaired = false;
+    if (min_eigenvalue(c, n) >= -PSD_TOL) return c;
+    Mat out = nearest_correlation(c, n);
+    validate_corr(out, n);
+    if (!(min_eigenvalue(out, n) >= -1.0e-8))
+        fail(OFG_INVALID_INPUT, "nearest-correlation projection did not produce PSD output");
+    repaired = true;
+    return out;
+}
+
+// cholesky_lower_psd (correlation.rs:362-389) with tolerance 1e-12.
+bool cholesky_psd(const Mat &a, int n, Mat &l) {
+    const double tol = 1.0e-12;
+    l.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double sum = a[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) sum -= l[(size_t)i * n + k] * l[(size_t)j * n + k];
+            if (i == j) {
+                if (sum < -tol) return false;
+                l[(size_t)i * n + j] = std::sqrt(std::fmax(sum, tol));
+            } else if (l[(size_t)j * n + j] > tol) {
+                l[(size_t)i * n + j] = sum / l[(size_t)j * n + j];
+            }
+        }
+    return true;
+}
+
+// validate_inputs_matrix (basket.rs:842-860) for one basket's inputs; throws.
+void validate_inputs(const ofg_basket &b, const std::vector<double> &spots,
+                     const std::vector<double> &vols, const Mat &corr) {
+    // BasketOption::validate (instruments/basket.rs:38-60)
+    if (b.strike < 0.0) fail(OFG_INVALID_INPUT, "basket strike must be >= 0");
+    if (b.maturity < 0.0) fail(OFG_INVALID_INPUT, "basket maturity must be >= 0");
+    for (int32_t k = 0; k < b.n_weights; ++k)
+        if (!std::isfinite(b.weights[k])) fail(OFG_INVALID_INPUT, "basket weights must be finite");
+    if (b.basket_type == OFG_BASKET_AVERAGE && b.n_weights <= 0)
+        fail(OFG_INVALID_INPUT, "average basket requires non-empty weights");
+    // validate_market_inputs (basket.rs:802-840)
+    if (b.num_paths <= 0) fail(OFG_INVALID_INPUT, "basket n_paths must be > 0");
+    if (b.n_assets <= 0) fail(OFG_INVALID_INPUT, "basket spots cannot be empty");
+    for (double s : spots)
+        if (s <= 0.0 || !std::isfinite(s)) fail(OFG_INVALID_INPUT, "basket spots must be finite and > 0");
+    for (double v : vols)
+        if (v < 0.0 || !std::isfinite(v)) fail(OFG_INVALID_INPUT, "basket vols must be finite and >= 0");
+    for (int32_t i = 0; i < b.n_assets; ++i)
+        if (!std::isfinite(b.dividends[i])) fail(OFG_INVALID_INPUT, "basket dividends must be finite");
+    if (b.basket_type == OFG_BASKET_AVERAGE && b.n_weights != b.n_assets)
+        fail(OFG_INVALID_INPUT, "average basket weights length must match number of assets");
+    validate_corr(corr, b.n_assets);
+}
+
+// terminal_payoff (basket.rs:759-784) of the spots themselves.
+double payoff_at_spots(const ofg_basket &b, const std::vector<double> &spots) {
+    double u;
+    if (b.basket_type == OFG_BASKET_AVERAGE) {
+        u = -0.0;
+        for (int32_t i = 0; i < b.n_assets; ++i) u += b.weights[i] * spots[(size_t)i];
+    } else if (b.basket_type == OFG_BASKET_BEST_OF) {
+        u = -INFINITY;
+        for (int32_t i = 0; i < b.n_assets; ++i) u = std::fmax(u, spots[(size_t)i] / spots[(size_t)i]);
+    } else {
+        u = INFINITY;
+        for (int32_t i = 0; i < b.n_assets; ++i) u = std::fmin(u, spots[(size_t)i] / spots[(size_t)i]);
+    }
+    return b.is_call ? std::fmax(u - b.strike, 0.0) : std::fmax(b.strike - u, 0.0);
+}
+
+// One scenario: a device descriptor or a host value (zero maturity).
+struct Scenario {
+    bool on_device = false;
+    double host_price = 0.0;
+    bool repaired = false;
+    ofg::BkDev d{};
+};
+
+// simulate_basket_once_matrix's preparation (basket.rs:610-641); throws.
+Scenario prepare(const ofg_basket &b, const std::vector<double> &spots,
+                 const std::vector<double> &vols, const Mat &corr) {
+    validate_inputs(b, spots, vols, corr);
+    Scenario s;
+    if (b.maturity <= 0.0) {
+        s.host_price = payoff_at_spots(b, spots);
+        return s;
+    }
+    const int n = b.n_assets;
+    if (n > ofg::BK_MAX) fail(OFG_INVALID_INPUT, "at most 16 assets per basket");
+    const Mat c = validate_or_repair(corr, n, s.repaired);
+    Mat l;
+    if (!cholesky_psd(c, n, l)) fail(OFG_INVALID_INPUT, "basket correlation matrix is not PSD");
+    ofg::BkDev &d = s.d;
+    d.n = n;
+    d.type = b.basket_type;
+    d.is_call = b.is_call != 0;
+    d.paths = b.num_paths;
+    d.seed = b.seed;
+    d.strike = b.strike;
+    const double t = b.maturity, sqrt_t = std::sqrt(t);
+    d.discount = std::exp(-b.rate * t);
+    for (int i = 0; i < n; ++i) {
+        const double v = vols[(size_t)i];
+        d.s0[i] = spots[(size_t)i];
+        d.drift[i] = (b.rate - b.dividends[i] - 0.5 * v * v) * t;
+        d.vol_t[i] = v * sqrt_t;
+        d.drift2[i] = d.drift[i] * OFG_INV_LN2_256;
+        d.vol_t2[i] = d.vol_t[i] * OFG_INV_LN2_256;
+        d.w[i] = b.basket_type == OFG_BASKET_AVERAGE ? b.weights[i] : 0.0;
+        for (int j = 0; j <= i; ++j) d.chol[i * (i + 1) / 2 + j] = l[(size_t)i * n + j];
+    }
+    s.on_device = true;
+    return s;
+}
+
+// bump_corr_matrix (basket.rs:1025-1037)
+Mat bump_corr(const Mat &c, int n, double bump) {
+    Mat out = c;
+    for (int i = 0; i < n; ++i) {
+        out[(size_t)i * n + i] = 1.0;
+        for (int j = i + 1; j < n; ++j) {
+            const double v = std::min(0.999, std::max(-0.999, c[(size_t)i * n + j] + bump));
+            out[(size_t)i * n + j] = out[(size_t)j * n + i] = v;
+        }
+    }
+    return out;
+}
+
+// average_off_diagonal (basket.rs:1039-1056)
+double average_off_diagonal(const Mat &c, int n) {
+    if (n < 2) return 0.0;
+    double sum = 0.0;
+    int64_t count = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            sum += c[(size_t)i * n + j];
+            ++count;
+        }
+    return sum / (double)count;
+}
+
+struct BasketPlan {
+    bool valid = false;
+    int32_t base = -1;
+    std::vector<int32_t> delta_up, delta_dn;
+    std::vector<double> delta_bump;
+    int32_t vega_up = -1, vega_dn = -1, cega_up = -1, cega_dn = -1;
+    double cega_bump = 0.0;
+    bool sens_ok = false;
+    double rho = 0.0;
+    bool repaired = false;
+};
+
+int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engine_cfg *cfg,
+                    int32_t with_sens, ofg_result *out, ofg_basket_sens *sens) {
+    if (!ctx || !cfg || (n && (!in || !out))) fail(OFG_INVALID_INPUT, "null argument");
+    const int32_t rng = cfg->rng_mode;
+    if (rng != OFG_RNG_EXTERNAL && rng != OFG_RNG_FAST && rng != OFG_RNG_REFERENCE)
+        fail(OFG_INVALID_INPUT, "unknown rng mode");
+    std::vector<Scenario> scen;
+    std::vector<BasketPlan> plans(n);
+    std::string last_msg;
+    int64_t normals_elems = 0, seed_words = 0, seed_threads = 0;
+    std::vector<ofg::LsmSeedJob> jobs;
+    std::vector<uint64_t> polys;
+    std::map<uint64_t, std::vector<ofg_jump::Poly>> poly_cache;
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_basket &b = in[i];
+        std::memset(&out[i], 0, sizeof out[i]);
+        out[i].price = NAN;
+        out[i].num_paths = out[i].num_steps = out[i].vol = NAN;
+        if (sens) std::memset(&sens[i], 0, sizeof sens[i]);
+        BasketPlan &pl = plans[i];
+        // a basket's external normals: [asset][path], n_assets * num_paths
+        const int64_t my_normals = normals_elems;
+        if (b.n_assets > 0 && b.num_paths > 0) normals_elems += (int64_t)b.n_assets * b.num_paths;
+        try {
+            if (b.n_assets <= 0 || !b.spots || !b.vols || !b.dividends || !b.corr)
+                fail(OFG_INVALID_INPUT, "basket spots cannot be empty");
+            if (b.n_weights < 0 || (b.n_weights > 0 && !b.weights))
+                fail(OFG_INVALID_INPUT, "basket weights must be finite");
+            const int na = b.n_assets;
+            std::vector<double> spots(b.spots, b.spots + na), vols(b.vols, b.vols + na);
+            const Mat corr(b.corr, b.corr + (size_t)na * na);
+            int64_t seed_off = -1;
+            auto push = [&](Scenario s) {
+                if (s.on_device) {
+                    s.d.normals_offset = my_normals;
+                    if (rng == OFG_RNG_REFERENCE) {
+                        if (seed_off < 0) {   // one stream per basket, shared by its bumps
+                            const ofg::LsmSeedJob j = ofg_jump::make_seed_job(
+                                b.seed, b.num_paths, na, seed_words, seed_threads, polys, poly_cache);
+                            jobs.push_back(j);
+                            seed_threads += ofg_jump::seed_job_threads(j);
+                            seed_off = seed_words;
+                            seed_words += 4 * b.num_paths;
+                        }
+                        s.d.seed_offset = seed_off;
+                    }
+                }
+                scen.push_back(s);
+                return (int32_t)scen.size() - 1;
+            };
+            Scenario base = prepare(b, spots, vols, corr);
+            pl.repaired = base.repaired;
+            pl.base = push(base);
+            pl.valid = true;
+            pl.rho = average_off_diagonal(corr, na);
+            if (with_sens) {
+                // basket_sensitivities (basket.rs:456-539)
+                try {
+                    for (int k = 0; k < na; ++k) {
+                        const double bump = std::max(std::fabs(spots[(size_t)k]) * SPOT_BUMP_REL, 1.0e-4);
+                        std::vector<double> up = spots, dn = spots;
+                        up[(size_t)k] += bump;
+                        dn[(size_t)k] = std::max(dn[(size_t)k] - bump, 1.0e-8);
+                        Scenario su = prepare(b, up, vols, corr), sd = prepare(b, dn, vols, corr);
+                        pl.delta_up.push_back(push(su));
+                        pl.delta_dn.push_back(push(sd));
+                        pl.delta_bump.push_back(bump);
+                    }
+                    std::vector<double> vu = vols, vd = vols;
+                    for (int k = 0; k < na; ++k) {
+                        vu[(size_t)k] += VOL_BUMP_ABS;
+                        vd[(size_t)k] = std::max(vd[(size_t)k] - VOL_BUMP_ABS, 1.0e-6);
+                    }
+                    Scenario su = prepare(b, spots, vu, corr), sd = prepare(b, spots, vd, corr);
+                    pl.vega_up = push(su);
+                    pl.vega_dn = push(sd);
+                    // correlation_sensitivity (basket.rs:551-597): halve the bump
+                    // while either bumped matrix fails to prepare
+                    double bump = CORR_BUMP;
+                    for (int tries = 0; tries < 6; ++tries, bump *= 0.5) {
+                        try {
+                            Scenario cu = prepare(b, spots, vols, bump_corr(corr, na, bump));
+                            Scenario cd = prepare(b, spots, vols, bump_corr(corr, na, -bump));
+                            pl.cega_up = push(cu);
+                            pl.cega_dn = push(cd);
+                            pl.cega_bump = bump;
+                            break;
+                        } catch (const OfgError &) {
+                        }
+                    }
+                    pl.sens_ok = true;
+                } catch (const OfgError &) {
+                    pl.sens_ok = false;
+                }
+            }
+        } catch (const OfgError &e) {
+            last_msg = e.what();
+            pl.valid = false;
+        }
+    }
+    std::vector<int32_t> dev_index(scen.size(), -1);
+    std::vector<ofg::BkDev> dev;
+    std::vector<int32_t> tiles;
+    for (size_t s = 0; s < scen.size(); ++s) {
+        if (!scen[s].on_device) continue;
+        ofg::BkDev d = scen[s].d;
+        const int64_t blk = ofg::MA_BLOCK;
+        d.ppt = (int32_t)std::max<int64_t>(
+            1, (d.paths + blk * ofg::MA_MAX_TILES - 1) / (blk * ofg::MA_MAX_TILES));
+        d.n_tiles = (int32_t)((d.paths + blk * d.ppt - 1) / (blk * d.ppt));
+        d.tile_first = (int32_t)tiles.size();
+        for (int32_t t = 0; t < d.n_tiles; ++t) tiles.push_back((int32_t)dev.size());
+        dev_index[s] = (int32_t)dev.size();
+        dev.push_back(d);
+    }
+    if (rng == OFG_RNG_EXTERNAL && !dev.empty() && !cfg->normals1)
+        fail(OFG_INVALID_INPUT, "external rng mode needs normals1");
+    std::vector<ofg::AcOut> res(dev.size());
+    float ms = 0.f;
+    if (!dev.empty()) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!ctx->basket) ctx->basket = new BasketBuffers();
+        BasketBuffers &bb = *ctx->basket;
+        hipStream_t st = ctx->stream;
+        auto up = [&](DevBuf &buf, const void *src, size_t bytes, const char *what) {
+            buf.ensure(std::max<size_t>(bytes, 1));
+            if (bytes) hip_check(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, st), what);
+        };
+        up(bb.scen, dev.data(), dev.size() * sizeof(dev[0]), "H2D basket scenarios");
+        up(bb.tiles, tiles.data(), tiles.size() * sizeof(int32_t), "H2D basket tiles");
+        bb.partials.ensure(tiles.size() * sizeof(ofg::AcPartial));
+        bb.out.ensure(dev.size() * sizeof(ofg::AcOut));
+        ofg::BkArgs a{};
+        if (rng == OFG_RNG_EXTERNAL) {
+            up(bb.normals, cfg->normals1, (size_t)normals_elems * sizeof(double), "H2D normals1");
+            a.normals = bb.normals.as<double>();
+        }
+        if (rng == OFG_RNG_REFERENCE) {
+            up(bb.jobs, jobs.data(), jobs.size() * sizeof(jobs[0]), "H2D seed jobs");
+            up(bb.polys, polys.data(), polys.size() * sizeof(uint64_t), "H2D jump polynomials");
+            bb.seeds.ensure((size_t)seed_words * sizeof(uint64_t));
+            a.seeds = bb.seeds.as<uint64_t>();
+        }
+        a.scen = bb.scen.as<ofg::BkDev>();
+        a.tile_scen = bb.tiles.as<int32_t>();
+        a.partials = bb.partials.as<ofg::AcPartial>();
+        a.out = bb.out.as<ofg::AcOut>();
+        a.n_tiles = (int32_t)tiles.size();
+        a.n_scen = (int32_t)dev.size();
+        a.rng_mode = rng;
+        hip_check(hipEventRecord(ctx->ev0, st), "event");
+        if (rng == OFG_RNG_REFERENCE)
+            hip_check(ofg::launch_lsm_seeds(bb.jobs.as<ofg::LsmSeedJob>(), (int32_t)jobs.size(),
+                                            seed_threads, bb.polys.as<uint64_t>(),
+                                            bb.seeds.as<uint64_t>(), st),
+                      "basket seed kernel launch");
+        hip_check(ofg::launch_basket(a, st), "basket kernel launch");
+        hip_check(hipEventRecord(ctx->ev1, st), "event");
+        hip_check(hipMemcpyAsync(res.data(), a.out, res.size() * sizeof(res[0]),
+                                 hipMemcpyDeviceToHost, st),
+                  "D2H basket results");
+        hip_check(hipStreamSynchronize(st), "basket kernel execution");
+        hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
+    }
+    auto price_of = [&](int32_t s, double *se) {
+        if (dev_index[(size_t)s] < 0) {
+            if (se) *se = 0.0;
+            return scen[(size_t)s].host_price;
+        }
+        const ofg::AcOut &r = res[(size_t)dev_index[(size_t)s]];
+        if (se) *se = r.stderr_;
+        return r.price;
+    };
+    for (size_t i = 0; i < n; ++i) {
+        const BasketPlan &pl = plans[i];
+        out[i].kernel_ms = ms;
+        if (!pl.valid) continue;
+        double se = 0.0;
+        out[i].price = price_of(pl.base, &se);
+        out[i].stderr_ = se;
+        out[i].has_stderr = 1;
+        out[i].num_paths = (double)in[i].num_paths;   // diagnostics (basket.rs:149-151)
+        ofg_basket_sens s{};
+        s.rho = pl.rho;
+        s.repaired = pl.repaired ? 1 : 0;
+        if (with_sens && pl.sens_ok) {
+            s.n_delta = (int32_t)pl.delta_up.size();
+            for (size_t k = 0; k < pl.delta_up.size(); ++k)
+                s.delta[k] = (price_of(pl.delta_up[k], nullptr) - price_of(pl.delta_dn[k], nullptr)) /
+                             (2.0 * pl.delta_bump[k]);
+            s.vega = (price_of(pl.vega_up, nullptr) - price_of(pl.vega_dn, nullptr)) /
+                     (2.0 * VOL_BUMP_ABS);
+            s.cega = pl.cega_up >= 0 ? (price_of(pl.cega_up, nullptr) - price_of(pl.cega_dn, nullptr)) /
+                                           (2.0 * pl.cega_bump)
+                                     : NAN;
+            s.ok = 1;
+        }
+        if (sens) sens[i] = s;
+    }
+    if (!last_msg.empty()) g_last_error = last_msg;
+    return OFG_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t ofg_basket_price(ofg_ctx *ctx, const ofg_basket *baskets, size_t n,
+                                    const ofg_engine_cfg *cfg, int32_t with_sens, ofg_result *out,
+                                    ofg_basket_sens *sens) {
+    return guarded([&]() { return basket_impl(ctx, baskets, n, cfg, with_sens, out, sens); });
+}
+
+extern "C" int32_t ofg_repair_correlation(const double *corr, int32_t n, double *out,
+                                          int32_t *was_repaired) {
+    return guarded([&]() {
+        if (!corr || !out || n <= 0) fail(OFG_INVALID_INPUT, "null argument");
+        bool rep = false;
+        const Mat c = validate_or_repair(Mat(corr, corr + (size_t)n * n), n, rep);
+        std::copy(c.begin(), c.end(), out);
+        if (was_repaired) *was_repaired = rep ? 1 : 0;
+        return OFG_OK;
+    });
+}

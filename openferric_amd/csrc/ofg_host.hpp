@@ -95,6 +95,7 @@ inline int32_t guarded(F &&f) {
 struct LsmBuffers;   // ofg_lsm_api.cpp
 struct QmcBuffers;   // ofg_qmc_api.cpp
 struct MultiBuffers; // ofg_multi_api.cpp
+struct BasketBuffers; // ofg_basket_api.cpp
 
 // Instrument validation with the reference's messages (instruments/*.rs
 // validate()); throws OfgError.
@@ -113,6 +114,7 @@ struct ofg_ctx {
     ofg_host::LsmBuffers *lsm = nullptr;   // Longstaff-Schwartz work buffers
     ofg_host::QmcBuffers *qmc = nullptr;   // quasi-Monte Carlo tables and partials
     ofg_host::MultiBuffers *multi = nullptr;   // multi-asset scenarios and partials
+    ofg_host::BasketBuffers *basket = nullptr; // basket scenarios, seeds and partials
     int cus = 256;
 };
 
@@ -123,4 +125,6 @@ void lsm_release(ofg_ctx *ctx);
 void qmc_release(ofg_ctx *ctx);
 // Releases the context's multi-asset buffers (ofg_multi_api.cpp).
 void multi_release(ofg_ctx *ctx);
+// Releases the context's basket buffers (ofg_basket_api.cpp).
+void basket_release(ofg_ctx *ctx);
 }  // namespace ofg_host

@@ -17,6 +17,7 @@
 #include "ofg_internal.hpp"
 #include "ofg_lsm.hpp"
 #include "ofg_math.hpp"
+#include "ofg_jump.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -28,6 +29,7 @@
 #include <vector>
 
 using namespace ofg_host;
+using namespace ofg_jump;
 
 namespace ofg_host {
 
@@ -52,116 +54,6 @@ void lsm_release(ofg_ctx *ctx) {
 }  // namespace ofg_host
 
 namespace {
-
-// ---------------------------------------------------------------------------
-// GF(2) polynomials of degree < 256 (bit i = coefficient of x^i)
-// ---------------------------------------------------------------------------
-struct Poly {
-    uint64_t w[4] = {0, 0, 0, 0};
-    bool bit(int i) const { return (w[i >> 6] >> (i & 63)) & 1u; }
-};
-
-// Low part of the characteristic polynomial P(x) = x^256 + low(x) of the
-// xoshiro256 state transition.
-const Poly &xoshiro_charpoly() {
-    static const Poly P = [] {
-        // the sequence of bit 0 of s0 satisfies the minimal recurrence of T
-        ofg::Xoshiro256pp st;
-        st.seed(0x2545F4914F6CDD1Dull);
-        const int N = 512;
-        std::vector<uint8_t> s(N);
-        for (int j = 0; j < N; ++j) {
-            s[j] = (uint8_t)(st.s0 & 1u);
-            (void)st.next_u64();
-        }
-        // Berlekamp-Massey over GF(2): connection polynomial C, length L
-        std::vector<uint8_t> C(N + 1, 0), B(N + 1, 0), T;
-        C[0] = B[0] = 1;
-        int L = 0, m = 1;
-        for (int n = 0; n < N; ++n) {
-            uint8_t d = s[n];
-            for (int i = 1; i <= L; ++i) d ^= (uint8_t)(C[i] & s[n - i]);
-            if (!d) {
-                ++m;
-                continue;
-            }
-            T = C;
-            for (int i = 0; i + m <= N; ++i) C[i + m] ^= B[i];
-            if (2 * L <= n) {
-                L = n + 1 - L;
-                B = T;
-                m = 1;
-            } else {
-                ++m;
-            }
-        }
-        if (L != 256) fail(OFG_NUMERICAL_ERROR, "xoshiro256 recurrence has unexpected degree");
-        // P(x) = x^L C(1/x): coefficient of x^(L - i) is C[i]
-        Poly p;
-        for (int i = 1; i <= L; ++i)
-            if (C[i]) p.w[(L - i) >> 6] |= 1ull << ((L - i) & 63);
-        return p;
-    }();
-    return P;
-}
-
-Poly mulmod(const Poly &a, const Poly &b) {
-    uint64_t prod[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < 256; ++i) {
-        if (!a.bit(i)) continue;
-        const int ws = i >> 6, bs = i & 63;
-        for (int k = 0; k < 4; ++k) {
-            prod[k + ws] ^= b.w[k] << bs;
-            if (bs) prod[k + ws + 1] ^= b.w[k] >> (64 - bs);
-        }
-    }
-    const Poly &low = xoshiro_charpoly();
-    for (int j = 511; j >= 256; --j) {
-        if (!((prod[j >> 6] >> (j & 63)) & 1u)) continue;
-        prod[j >> 6] ^= 1ull << (j & 63);   // x^j = x^(j-256) x^256 = x^(j-256) low(x)
-        const int sh = j - 256, ws = sh >> 6, bs = sh & 63;
-        for (int k = 0; k < 4; ++k) {
-            prod[k + ws] ^= low.w[k] << bs;
-            if (bs && k + ws + 1 < 8) prod[k + ws + 1] ^= low.w[k] >> (64 - bs);
-        }
-    }
-    Poly r;
-    for (int k = 0; k < 4; ++k) r.w[k] = prod[k];
-    return r;
-}
-
-// x^k mod P(x)
-Poly x_pow(uint64_t k) {
-    Poly result;
-    result.w[0] = 1;
-    Poly base;
-    base.w[0] = 2;   // x
-    while (k) {
-        if (k & 1u) result = mulmod(result, base);
-        base = mulmod(base, base);
-        k >>= 1;
-    }
-    return result;
-}
-
-void apply_poly(const Poly &r, uint64_t s[4]) {
-    ofg::Xoshiro256pp cur;
-    cur.s0 = s[0];
-    cur.s1 = s[1];
-    cur.s2 = s[2];
-    cur.s3 = s[3];
-    uint64_t acc[4] = {0, 0, 0, 0};
-    for (int i = 0; i < 256; ++i) {
-        if (r.bit(i)) {
-            acc[0] ^= cur.s0;
-            acc[1] ^= cur.s1;
-            acc[2] ^= cur.s2;
-            acc[3] ^= cur.s3;
-        }
-        (void)cur.next_u64();
-    }
-    std::memcpy(s, acc, sizeof acc);
-}
 
 // ---------------------------------------------------------------------------
 // Planning
@@ -426,35 +318,11 @@ LsmPlan build_lsm_plan(const ofg_lsm_option *in, size_t n, int32_t rng_mode) {
         p.max_rounds = std::max(p.max_rounds, d.barrier ? 0 : d.n_ex);
 
         if (rng_mode == OFG_RNG_REFERENCE) {
-            ofg::LsmSeedJob j{};
-            ofg::Xoshiro256pp st;
-            st.seed(o.seed);   // Xoshiro256PlusPlus::seed_from_u64 (fast_rng.rs:35-48)
-            j.base[0] = st.s0;
-            j.base[1] = st.s1;
-            j.base[2] = st.s2;
-            j.base[3] = st.s3;
-            j.paths = paths;
-            j.draws = steps * (dynamics == ofg::LSM_HESTON ? 2 : 1);
-            // a thread jumps once (about 128 x 256 transition steps for a
-            // 2^17-path option) and then walks `chunk` paths' draws
-            j.chunk = 64;
-            const int64_t threads = (paths + j.chunk - 1) / j.chunk;
-            int32_t bits = 0;
-            while (bits < 63 && (int64_t(1) << bits) < threads) ++bits;
-            j.n_polys = bits;
-            j.seed_offset = p.seeds;
-            j.thread_first = p.seed_threads;
-            const uint64_t key = (uint64_t)j.chunk * (uint64_t)j.draws;
-            std::vector<Poly> &qs = poly_cache[key];
-            if ((int32_t)qs.size() < bits) {
-                if (qs.empty()) qs.push_back(x_pow(key));
-                while ((int32_t)qs.size() < bits) qs.push_back(mulmod(qs.back(), qs.back()));
-            }
-            j.poly_offset = (int32_t)(p.polys.size() / 4);
-            for (int32_t k = 0; k < bits; ++k)
-                p.polys.insert(p.polys.end(), qs[(size_t)k].w, qs[(size_t)k].w + 4);
+            const ofg::LsmSeedJob j =
+                make_seed_job(o.seed, paths, steps * (dynamics == ofg::LSM_HESTON ? 2 : 1),
+                              p.seeds, p.seed_threads, p.polys, poly_cache);
             p.jobs.push_back(j);
-            p.seed_threads += threads;
+            p.seed_threads += seed_job_threads(j);
             p.seeds += 4 * paths;
         }
         h.simulate = true;

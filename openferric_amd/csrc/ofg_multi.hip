@@ -1,6 +1,7 @@
 // Multi-asset path kernels for MI355X (gfx950), FP64 on the VALU: worst-of
 // autocallables and Phoenix notes (simulate_autocallable_paths,
-// src/pricing/autocallable.rs:441-579).
+// src/pricing/autocallable.rs:441-579) and basket options under a Gaussian
+// copula (simulate_basket_once_matrix, src/pricing/basket.rs:599-683).
 //
 // Per path, the underlyings' spots live in registers (up to MA_MAX); each
 // step draws one normal per underlying, correlates them with the scenario's
@@ -191,7 +192,161 @@ __global__ __launch_bounds__(MA_BLOCK) void autocall_close_kernel(AcArgs a) {
     a.out[sc].stderr_ = sqrt(var / n);
 }
 
+// ------------------------------------------------------------------------
+// Basket options: one exact terminal step per asset
+// ------------------------------------------------------------------------
+
+// One path's payoff (terminal_payoff, basket.rs:759-784). The correlated
+// normal of asset i accumulates L[i][j] z_j column by column as each z_j is
+// drawn, the same terms in the same order as correlate_normals' running sum.
+template <int RNG>
+__device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const double *normals,
+                                              const uint64_t *seeds, const double *tab) {
+    constexpr bool FAST = RNG == OFG_RNG_FAST;
+    const int n = o.n;
+    double c[BK_MAX];
+#pragma unroll
+    for (int i = 0; i < BK_MAX; ++i) c[i] = 0.0;
+    Xoshiro256pp ref;
+    Xoroshiro128p fast;
+    if constexpr (RNG == OFG_RNG_REFERENCE) {
+        const uint64_t *sd = seeds + o.seed_offset + 4 * p;
+        ref.s0 = sd[0];
+        ref.s1 = sd[1];
+        ref.s2 = sd[2];
+        ref.s3 = sd[3];
+    }
+    if constexpr (FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
+    double zpair = 0.0;
+#pragma unroll
+    for (int j = 0; j < BK_MAX; ++j) {
+        if (j < n) {
+            double z;
+            if constexpr (RNG == OFG_RNG_REFERENCE) {
+                z = reference_normal(ref);
+            } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
+                z = normals[o.normals_offset + (int64_t)j * o.paths + p];
+            } else {
+                if ((j & 1) == 0) {
+                    const uint64_t wu = fast.next_u64();
+                    const uint64_t wt = fast.next_u64();
+                    box_muller_pair(wu, (uint32_t)(wt >> 32), z, zpair, tab);
+                } else {
+                    z = zpair;
+                }
+            }
+#pragma unroll
+            for (int i = j; i < BK_MAX; ++i)
+                if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
+        }
+    }
+    double u = o.type == BK_AVERAGE ? -0.0 : (o.type == BK_BEST_OF ? -INFINITY : INFINITY);
+#pragma unroll
+    for (int i = 0; i < BK_MAX; ++i) {
+        if (i < n) {
+            double g;
+            if constexpr (FAST) {
+                double y = o.drift2[i] + o.vol_t2[i] * c[i];
+                y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
+                g = exp2_256(y, tab);
+            } else {
+                g = exp_fp64(o.drift[i] + o.vol_t[i] * c[i]);
+            }
+            const double st = o.s0[i] * g;
+            if (o.type == BK_AVERAGE) u += o.w[i] * st;
+            else if (o.type == BK_BEST_OF) u = fmax(u, st / o.s0[i]);
+            else u = fmin(u, st / o.s0[i]);
+        }
+    }
+    return o.is_call ? fmax(u - o.strike, 0.0) : fmax(o.strike - u, 0.0);
+}
+
+template <int RNG>
+__global__ __launch_bounds__(MA_BLOCK) void basket_kernel(BkArgs a) {
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    __shared__ double red[NWAVES][2];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += MA_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    }
+    const int tile = blockIdx.x;
+    const int sc = a.tile_scen[tile];
+    const BkDev &o = a.scen[sc];
+    const int64_t base = (int64_t)(tile - o.tile_first) * MA_BLOCK * o.ppt;
+    double sum = 0.0, sq = 0.0;
+    for (int j = 0; j < o.ppt; ++j) {
+        const int64_t p = base + (int64_t)j * MA_BLOCK + threadIdx.x;
+        if (p >= o.paths) break;
+        const double x = basket_path<RNG>(o, p, a.normals, a.seeds, tab);
+        sum += x;
+        sq += x * x;
+    }
+    sum = wave_sum(sum);
+    sq = wave_sum(sq);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = sum;
+        red[wave][1] = sq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = red[0][0], q = red[0][1];
+        for (int w = 1; w < NWAVES; ++w) {
+            s += red[w][0];
+            q += red[w][1];
+        }
+        a.partials[tile].sum = s;
+        a.partials[tile].sum_sq = q;
+    }
+}
+
+// One block per scenario: discounted mean and standard error of the payoffs
+// (basket.rs:676-682).
+__global__ __launch_bounds__(MA_BLOCK) void basket_close_kernel(BkArgs a) {
+    __shared__ double red[NWAVES][2];
+    const int sc = blockIdx.x;
+    const BkDev &o = a.scen[sc];
+    double s = 0.0, q = 0.0;
+    for (int t = threadIdx.x; t < o.n_tiles; t += MA_BLOCK) {
+        s += a.partials[o.tile_first + t].sum;
+        q += a.partials[o.tile_first + t].sum_sq;
+    }
+    s = wave_sum(s);
+    q = wave_sum(q);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = s;
+        red[wave][1] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double sum = red[0][0], sq = red[0][1];
+    for (int w = 1; w < NWAVES; ++w) {
+        sum += red[w][0];
+        sq += red[w][1];
+    }
+    const double n = (double)o.paths;
+    const double var = o.paths > 1 ? fmax((sq - sum * sum / n) / (n - 1.0), 0.0) : 0.0;
+    a.out[sc].price = o.discount * (sum / n);
+    a.out[sc].stderr_ = o.discount * sqrt(var / n);
+}
+
 }  // namespace
+
+hipError_t launch_basket(const BkArgs &a, hipStream_t stream) {
+    if (a.n_tiles <= 0) return hipSuccess;
+    switch (a.rng_mode) {
+    case OFG_RNG_FAST: basket_kernel<OFG_RNG_FAST><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a); break;
+    case OFG_RNG_EXTERNAL:
+        basket_kernel<OFG_RNG_EXTERNAL><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+        break;
+    default: basket_kernel<OFG_RNG_REFERENCE><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a); break;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    basket_close_kernel<<<a.n_scen, MA_BLOCK, 0, stream>>>(a);
+    return hipGetLastError();
+}
 
 hipError_t launch_autocall(const AcArgs &a, hipStream_t stream) {
     if (a.n_tiles <= 0) return hipSuccess;

@@ -72,4 +72,40 @@ struct AcArgs {
 
 hipError_t launch_autocall(const AcArgs &a, hipStream_t stream);
 
+// ---- basket options (src/pricing/basket.rs:599-683, Gaussian copula) ----
+constexpr int BK_MAX = 16;
+enum BasketType : int32_t { BK_AVERAGE = 0, BK_BEST_OF = 1, BK_WORST_OF = 2 };
+
+struct alignas(16) BkDev {
+    int32_t n;             // assets
+    int32_t type;          // BasketType
+    int32_t is_call;
+    int32_t tile_first;
+    int32_t n_tiles;
+    int32_t ppt;
+    int64_t paths;
+    int64_t normals_offset;   // external normals, [asset][path]
+    int64_t seed_offset;      // reference-stream start states, [path][word]
+    uint64_t seed;            // fast draws: stream_seed(seed, path)
+    double strike, discount;
+    double s0[BK_MAX], drift[BK_MAX], vol_t[BK_MAX], w[BK_MAX];
+    double drift2[BK_MAX], vol_t2[BK_MAX];     // fast mode: scaled by 256/ln2
+    double chol[BK_MAX * (BK_MAX + 1) / 2];   // lower factor, row i at i(i+1)/2
+};
+
+struct BkArgs {
+    const BkDev *scen;
+    const int32_t *tile_scen;
+    AcPartial *partials;
+    AcOut *out;
+    const double *normals;
+    const uint64_t *seeds;
+    int32_t n_tiles;
+    int32_t n_scen;
+    int32_t rng_mode;
+    int32_t pad;
+};
+
+hipError_t launch_basket(const BkArgs &a, hipStream_t stream);
+
 }  // namespace ofg

@@ -1,13 +1,18 @@
-"""Multi-asset contracts on the GPU: worst-of autocallables and Phoenix notes.
+"""Multi-asset contracts on the GPU: worst-of autocallables, Phoenix notes and
+basket options.
 
 Mirrors src/instruments/autocallable.rs (Autocallable, PhoenixAutocallable)
 and src/pricing/autocallable.rs (price_autocallable,
 autocallable_sensitivities, price_phoenix_autocallable,
-phoenix_autocallable_sensitivities) over the C ABI's ofg_autocall_price.
+phoenix_autocallable_sensitivities) over the C ABI's ofg_autocall_price, and
+src/instruments/basket.rs (BasketOption) with src/pricing/basket.rs
+(price_basket_mc, basket_sensitivities) over ofg_basket_price.
 Draws: the reference walks one xoshiro256++ stream whose position depends on
 when earlier paths were called, which no parallel simulator can reproduce;
 `rng_mode` is Fast (per-path streams from the reference's seed 94210) or
-External (caller normals, [path][step][underlying] per contract).
+External (caller normals, [path][step][underlying] per contract). Baskets
+draw a fixed count per path, so they also take the reference's own stream
+(Reference, the default), split across paths by jump-ahead.
 """
 from __future__ import annotations
 
@@ -194,3 +199,140 @@ def autocallable_sensitivities(autocall, spots, vols, corr_matrix, r, q, n_paths
 
 
 phoenix_autocallable_sensitivities = autocallable_sensitivities
+
+
+# ---------------------------------------------------------------------------
+# Basket options (src/instruments/basket.rs, src/pricing/basket.rs)
+# ---------------------------------------------------------------------------
+
+class BasketType:
+    """instruments/basket.rs:14-23"""
+    Average = _lib.BASKET_AVERAGE
+    BestOf = _lib.BASKET_BEST_OF
+    WorstOf = _lib.BASKET_WORST_OF
+
+
+BASKET_MC_SEED = 3_148_159   # pricing/basket.rs:27
+
+
+@dataclass
+class BasketOption:
+    """instruments/basket.rs:25-60"""
+    weights: Sequence[float]
+    strike: float
+    maturity: float
+    is_call: bool
+    basket_type: int = BasketType.Average
+
+    def validate(self) -> None:
+        if self.strike < 0.0:
+            raise InvalidInput("basket strike must be >= 0")
+        if self.maturity < 0.0:
+            raise InvalidInput("basket maturity must be >= 0")
+        if any(not np.isfinite(w) for w in self.weights):
+            raise InvalidInput("basket weights must be finite")
+        if self.basket_type == BasketType.Average and len(self.weights) == 0:
+            raise InvalidInput("average basket requires non-empty weights")
+
+    def instrument_type(self) -> str:
+        return "BasketOption"
+
+
+@dataclass
+class BasketSensitivities:
+    """pricing/basket.rs:57-65"""
+    delta: List[float]
+    vega: float
+    cega: float
+
+
+def basket_struct(basket: BasketOption, spots, vols, corr_matrix, r, dividends, n_paths,
+                  keep: _Keep, seed: int = BASKET_MC_SEED) -> _lib.OfgBasket:
+    """Flattens a basket and its market inputs into the C ABI's ofg_basket."""
+    b = _lib.OfgBasket()
+    b.basket_type = int(basket.basket_type)
+    b.is_call = int(bool(basket.is_call))
+    b.n_weights = len(basket.weights)
+    b.weights = keep.arr(list(basket.weights)) if len(basket.weights) else None
+    b.strike, b.maturity = basket.strike, basket.maturity
+    n = len(spots)
+    if len(vols) != n or len(dividends) != n:
+        raise InvalidInput("basket spots/vols/dividends lengths must match")
+    cm = np.asarray(corr_matrix, dtype=np.float64)
+    if cm.shape != (n, n):
+        raise InvalidInput("correlation matrix dimensions must match asset count")
+    b.n_assets = n
+    b.spots = keep.arr(list(spots)) if n else None
+    b.vols = keep.arr(list(vols)) if n else None
+    b.dividends = keep.arr(list(dividends)) if n else None
+    b.corr = keep.arr(cm.ravel()) if n else None
+    b.rate = r
+    b.num_paths = int(n_paths)
+    b.seed = int(seed)
+    b._keep = keep
+    return b
+
+
+def basket_price_raw(structs, rng_mode=RngMode.Reference, normals=None, with_sens=False,
+                     device: int = 0):
+    """Calls ofg_basket_price. normals: per basket [path][asset] arrays
+    (External mode). Returns (results, sensitivities)."""
+    from .engines import engine_cfg
+    L = _lib.load()
+    ctx = _lib.context(device)
+    n = len(structs)
+    z = None
+    if normals is not None:
+        # the C ABI's layout: [asset][path] per basket
+        z = np.concatenate([np.ascontiguousarray(np.asarray(x, dtype=np.float64).T).ravel()
+                            for x in normals])
+    cfg = engine_cfg(rng_mode=rng_mode, normals1=z)
+    arr = (_lib.OfgBasket * n)(*structs)
+    out = (_lib.OfgResult * n)()
+    sens = (_lib.OfgBasketSens * n)()
+    _raise(L.ofg_basket_price(ctx, arr, n, C.byref(cfg), 1 if with_sens else 0, out, sens))
+    return list(out), list(sens)
+
+
+def price_basket_mc(basket: BasketOption, spots, vols, corr_matrix, r, dividends, n_paths,
+                    rng_mode=RngMode.Reference, normals=None) -> PricingResult:
+    """price_basket_mc (pricing/basket.rs:99-160): Gaussian copula, one exact
+    terminal step, the reference's seed 3148159 and, by default, its draws."""
+    keep = _Keep()
+    s = basket_struct(basket, spots, vols, corr_matrix, r, dividends, n_paths, keep)
+    res, sens = basket_price_raw([s], rng_mode, None if normals is None else [normals])
+    r0 = res[0]
+    diag = Diagnostics()
+    if r0.has_stderr:
+        diag.insert("num_paths", r0.num_paths)
+        diag.insert("rho", sens[0].rho)
+    return PricingResult(price=r0.price, stderr=r0.stderr_ if r0.has_stderr else None,
+                         diagnostics=diag, kernel_ms=r0.kernel_ms)
+
+
+def basket_sensitivities(basket: BasketOption, spots, vols, corr_matrix, r, dividends, n_paths,
+                         rng_mode=RngMode.Reference, normals=None) -> BasketSensitivities:
+    """basket_sensitivities (pricing/basket.rs:456-539): every bump on the
+    base price's draws, all in one launch."""
+    keep = _Keep()
+    s = basket_struct(basket, spots, vols, corr_matrix, r, dividends, n_paths, keep)
+    res, sens = basket_price_raw([s], rng_mode, None if normals is None else [normals], True)
+    if not res[0].has_stderr or not sens[0].ok:
+        raise InvalidInput(_lib.last_error() or "basket sensitivities failed")
+    sv = sens[0]
+    return BasketSensitivities([sv.delta[k] for k in range(sv.n_delta)], sv.vega, sv.cega)
+
+
+def validate_or_repair_correlation_matrix(corr_matrix):
+    """validate_or_repair_correlation_matrix (math/correlation.rs:340-358)
+    with the default PsdProjectionConfig: (matrix, was_repaired)."""
+    L = _lib.load()
+    c = np.ascontiguousarray(np.asarray(corr_matrix, dtype=np.float64))
+    n = c.shape[0]
+    if c.shape != (n, n):
+        raise InvalidInput("correlation matrix dimensions must match asset count")
+    outm = np.zeros_like(c)
+    rep = C.c_int32(0)
+    _raise(L.ofg_repair_correlation(c.ctypes.data_as(C.POINTER(C.c_double)), n,
+                                    outm.ctypes.data_as(C.POINTER(C.c_double)), C.byref(rep)))
+    return outm, bool(rep.value)

@@ -39,6 +39,9 @@
  *                                      call dates), never/always called,
  *                                      certain knock-in vs e^{(r-q)T} and
  *                                      Margrabe, reference monotonicity tests
+ *   or_basket_price                    one asset / unit weight = Black-Scholes,
+ *                                      best/worst-of two = Margrabe, Levy
+ *                                      moment match, reference basket tests
  *   or_sobol_*, or_mc_european_qmc     reference QMC assertions, (0,m,1)-net
  *                                      stratification of every dimension,
  *                                      Gray-code identity, BS within 0.1 %
@@ -224,6 +227,19 @@ int32_t or_observation_schedule(const double *dates, int32_t n_dates, double mat
 int or_autocall_price(const or_autocall *a, const double *normals, double *price,
                       double *stderr_out);
 void or_bump_corr_matrix(const double *c, int32_t n, double bump, double *out);
+
+/* ---- multi-asset: basket Monte Carlo (oracle_multi.c) ----------------- */
+enum { OR_BASKET_AVERAGE = 0, OR_BASKET_BEST_OF = 1, OR_BASKET_WORST_OF = 2 };
+typedef struct {
+    int32_t basket_type, is_call, n_assets, pad;
+    const double *weights;                       /* n_assets (Average) */
+    double strike, maturity, rate;
+    const double *spots, *vols, *dividends;      /* n_assets */
+    const double *corr;                          /* n_assets^2, PSD */
+    int64_t num_paths;
+    uint64_t seed;
+} or_basket;
+int or_basket_price(const or_basket *b, const double *normals, double *price, double *stderr_out);
 
 /* ---- quasi-Monte Carlo (oracle_qmc.c) -------------------------------- */
 /* build_direction_numbers / scrambler of SobolSequence (math/sobol.rs) */

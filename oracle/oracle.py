@@ -87,6 +87,19 @@ class OrAutocall(C.Structure):
     ]
 
 
+class OrBasket(C.Structure):
+    _fields_ = [
+        ("basket_type", C.c_int32), ("is_call", C.c_int32), ("n_assets", C.c_int32),
+        ("pad", C.c_int32), ("weights", C.POINTER(C.c_double)), ("strike", C.c_double),
+        ("maturity", C.c_double), ("rate", C.c_double), ("spots", C.POINTER(C.c_double)),
+        ("vols", C.POINTER(C.c_double)), ("dividends", C.POINTER(C.c_double)),
+        ("corr", C.POINTER(C.c_double)), ("num_paths", C.c_int64), ("seed", C.c_uint64),
+    ]
+
+
+BASKET_AVERAGE, BASKET_BEST_OF, BASKET_WORST_OF = 0, 1, 2
+
+
 def build(force: bool = False) -> str:
     """Compile liboracle.so with make (gcc); returns the library path."""
     if force or not os.path.exists(_LIB_PATH):
@@ -146,6 +159,8 @@ def lib():
         L.or_autocall_price.restype = C.c_int
         L.or_cholesky_lower.argtypes = [pd, i32, pd]; L.or_cholesky_lower.restype = C.c_int
         L.or_bump_corr_matrix.argtypes = [pd, i32, d, pd]; L.or_bump_corr_matrix.restype = None
+        L.or_basket_price.argtypes = [C.POINTER(OrBasket), pd, pd, pd]
+        L.or_basket_price.restype = C.c_int
         L.or_sobol_directions.argtypes = [u64, u64, pu64]; L.or_sobol_directions.restype = None
         L.or_sobol_scrambler.argtypes = [u64, u64]; L.or_sobol_scrambler.restype = u64
         L.or_sobol_points.argtypes = [i64, u64, i64, pu64, pd]; L.or_sobol_points.restype = None
@@ -417,3 +432,28 @@ def cholesky_lower(a):
     l = np.zeros((n, n))
     ok = lib().or_cholesky_lower(_ptr(a.ravel()), n, _ptr(l))
     return l if ok else None
+
+
+def make_basket(spots, vols, dividends, corr, weights=None, basket_type=BASKET_AVERAGE,
+                is_call=True, strike=100.0, maturity=1.0, rate=0.02, num_paths=10_000,
+                seed=3_148_159):
+    b = OrBasket()
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in
+            (spots, vols, dividends, np.asarray(corr, dtype=np.float64).ravel(),
+             weights if weights is not None else np.zeros(len(spots)))]
+    b._keep = arrs
+    b.spots, b.vols, b.dividends, b.corr, b.weights = [_ptr(a) for a in arrs]
+    b.n_assets, b.basket_type, b.is_call = len(arrs[0]), basket_type, int(is_call)
+    b.strike, b.maturity, b.rate = strike, maturity, rate
+    b.num_paths, b.seed = num_paths, seed
+    return b
+
+
+def basket_price(b, normals=None):
+    """simulate_basket_once_matrix (Gaussian): (price, stderr). normals [path][asset]."""
+    z = np.ascontiguousarray(normals, dtype=np.float64).ravel() if normals is not None else None
+    p, se = C.c_double(), C.c_double()
+    rc = lib().or_basket_price(C.byref(b), _ptr(z), C.byref(p), C.byref(se))
+    if rc != 0:
+        raise ValueError(f"or_basket_price failed rc={rc}")
+    return p.value, se.value

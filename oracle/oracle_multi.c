@@ -16,12 +16,24 @@
  * as the reference; otherwise normals[path][step][asset] (a called path
  * leaves the rest of its slice unused).
  *
+ * Also restates src/pricing/basket.rs: terminal_payoff (:759-784) and
+ * simulate_basket_once_matrix under the Gaussian copula (:599-683: one exact
+ * terminal step per asset, correlate_normals, RunningStats' Welford mean and
+ * variance, discount exp(-rT), zero maturity = payoff of the spots) on a PSD
+ * correlation matrix (the reference repairs others first, which the tests do
+ * with an independent Higham projection).
+ *
  * Pinned by (tests/test_oracle_multi.py): zero-volatility paths with closed
  * form present values (call dates, standard and Phoenix coupons, memory);
  * never-called / always-called limits; certain knock-in, where the price is
  * e^{-rT} N E[min_i S_i(T)/S_i(0)] / K_ki with E[S(T)/S(0)] = e^{(r-q)T} for
  * one asset and Margrabe's exchange-option formula for two correlated
  * assets; the reference's own monotonicity and single-underlying tests.
+ * Baskets (tests/test_oracle_basket.py): zero volatility (forward basket
+ * intrinsic), one asset = Black-Scholes, a zero weight = the one-asset price
+ * on the same first column of normals, best-of / worst-of of two = Margrabe,
+ * the single stream = the same draws passed path-major, zero maturity, and
+ * the reference's tests (Gentle moment matching within 2% on five assets).
  */
 #include "oracle.h"
 
@@ -191,4 +203,74 @@ void or_bump_corr_matrix(const double *c, int32_t n, double bump, double *out) {
             out[j * n + i] = b;
         }
     }
+}
+
+/* ===================================================================== */
+/* Basket Monte Carlo: src/pricing/basket.rs                              */
+/* ===================================================================== */
+
+/* terminal_payoff (basket.rs:759-784) */
+static double basket_payoff(const or_basket *b, const double *s0, const double *st) {
+    double u;
+    if (b->basket_type == OR_BASKET_AVERAGE) {
+        u = -0.0;
+        for (int32_t i = 0; i < b->n_assets; ++i) u += b->weights[i] * st[i];
+    } else if (b->basket_type == OR_BASKET_BEST_OF) {
+        u = -INFINITY;
+        for (int32_t i = 0; i < b->n_assets; ++i) u = fmax(u, st[i] / s0[i]);
+    } else {
+        u = INFINITY;
+        for (int32_t i = 0; i < b->n_assets; ++i) u = fmin(u, st[i] / s0[i]);
+    }
+    return b->is_call ? fmax(u - b->strike, 0.0) : fmax(b->strike - u, 0.0);
+}
+
+/* simulate_basket_once_matrix, Gaussian copula (basket.rs:599-683), on a PSD
+ * correlation matrix (the reference repairs others first). normals NULL: the
+ * engine's single stream from `seed`; else [path][asset]. Returns 0, 2 when
+ * the matrix is not PSD. */
+int or_basket_price(const or_basket *b, const double *normals, double *price, double *stderr_out) {
+    const int32_t n = b->n_assets;
+    if (b->maturity <= 0.0) {
+        *price = basket_payoff(b, b->spots, b->spots);
+        *stderr_out = 0.0;
+        return 0;
+    }
+    const double t = b->maturity, sqrt_t = sqrt(t);
+    const double discount = exp(-b->rate * t);
+    double *l = (double *)malloc(sizeof(double) * (size_t)(n * n));
+    if (!or_cholesky_lower(b->corr, n, l)) {
+        free(l);
+        return 2;
+    }
+    double drift[64], vol_t[64], indep[64], term[64];
+    for (int32_t i = 0; i < n; ++i) {
+        drift[i] = (b->rate - b->dividends[i] - 0.5 * b->vols[i] * b->vols[i]) * t;
+        vol_t[i] = b->vols[i] * sqrt_t;
+    }
+    uint64_t st[4];
+    or_xoshiro_seed(b->seed, st);
+    /* RunningStats (basket.rs:66-97): Welford */
+    double mean = 0.0, m2 = 0.0;
+    int64_t cnt = 0;
+    for (int64_t p = 0; p < b->num_paths; ++p) {
+        for (int32_t i = 0; i < n; ++i)
+            indep[i] = normals ? normals[p * n + i]
+                               : or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(st)));
+        for (int32_t i = 0; i < n; ++i) {
+            double sum = 0.0;
+            for (int32_t j = 0; j <= i; ++j) sum += l[i * n + j] * indep[j];
+            term[i] = b->spots[i] * exp(drift[i] + vol_t[i] * sum);
+        }
+        const double x = basket_payoff(b, b->spots, term);
+        ++cnt;
+        const double delta = x - mean;
+        mean += delta / (double)cnt;
+        m2 += delta * (x - mean);
+    }
+    const double var = cnt > 1 ? m2 / ((double)cnt - 1.0) : 0.0;
+    *price = discount * mean;
+    *stderr_out = discount * sqrt(var / (double)b->num_paths);
+    free(l);
+    return 0;
 }

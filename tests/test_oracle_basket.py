@@ -1,0 +1,173 @@
+"""The oracle's restatement of the reference's basket Monte Carlo
+(simulate_basket_once_matrix, src/pricing/basket.rs:599-683) pinned by
+closed forms and the reference's own tests; the library's host-side
+correlation repair (ofg_repair_correlation) against an independent numpy
+Higham projection and Higham's published example. CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.basket_numpy import bs_price, higham_nearest, margrabe, moment_matching
+
+AVG, BEST, WORST = 0, 1, 2
+
+
+def _z(n_paths, n_assets, seed):
+    return np.random.default_rng(seed).standard_normal((n_paths, n_assets))
+
+
+def test_zero_vol_is_discounted_forward_intrinsic(oracle):
+    spots, divs, w = [100.0, 80.0, 120.0], [0.01, 0.03, 0.0], [0.5, 0.3, 0.2]
+    corr = [[1, 0.3, 0.1], [0.3, 1, 0.2], [0.1, 0.2, 1]]
+    r, t = 0.04, 1.5
+    fwd = sum(wi * s * math.exp((r - q) * t) for wi, s, q in zip(w, spots, divs))
+    for is_call, k in ((True, 90.0), (False, 110.0), (True, 130.0)):
+        b = oracle.make_basket(spots, [0.0] * 3, divs, corr, w, AVG, is_call, k, t, r, 1000)
+        p, se = oracle.basket_price(b, _z(1000, 3, 1))
+        want = math.exp(-r * t) * max((fwd - k) if is_call else (k - fwd), 0.0)
+        assert abs(p - want) < 1e-11 * max(1.0, want) and se < 1e-10
+
+
+def test_one_asset_is_black_scholes(oracle):
+    for is_call, k in ((True, 100.0), (False, 95.0)):
+        b = oracle.make_basket([100.0], [0.25], [0.02], [[1.0]], [1.0], AVG, is_call, k, 1.0, 0.03,
+                               400_000)
+        p, se = oracle.basket_price(b)
+        want = bs_price(is_call, 100.0, k, 0.03, 0.02, 0.25, 1.0)
+        assert abs(p - want) < 4 * se, (p, want, se)
+
+
+def test_zero_weight_asset_equals_one_asset_on_first_column(oracle):
+    z = _z(20_000, 2, 5)
+    corr = [[1.0, -0.4], [-0.4, 1.0]]
+    two = oracle.make_basket([100.0, 70.0], [0.3, 0.2], [0.0, 0.01], corr, [1.0, 0.0], AVG, True,
+                             100.0, 1.0, 0.02, 20_000)
+    one = oracle.make_basket([100.0], [0.3], [0.0], [[1.0]], [1.0], AVG, True, 100.0, 1.0, 0.02,
+                             20_000)
+    p2, s2 = oracle.basket_price(two, z)
+    p1, s1 = oracle.basket_price(one, z[:, :1])
+    assert p1 == p2 and s1 == s2
+
+
+@pytest.mark.parametrize("rho", [-0.5, 0.6])
+def test_best_and_worst_of_two_match_margrabe(oracle, rho):
+    s1, s2, q1, q2, r, t = 0.3, 0.2, 0.01, 0.03, 0.02, 1.0
+    corr = [[1.0, rho], [rho, 1.0]]
+    sig = math.sqrt(s1 * s1 + s2 * s2 - 2 * rho * s1 * s2)
+    a, b = math.exp(-q1 * t), math.exp(-q2 * t)   # discounted forwards of S_i(T)/S_i(0)
+    best_want = a + margrabe(b, a, sig, t)         # max(X1, X2) = X1 + (X2 - X1)^+
+    worst_want = a - margrabe(a, b, sig, t)        # min(X1, X2) = X1 - (X1 - X2)^+
+    for kind, want in ((BEST, best_want), (WORST, worst_want)):
+        bk = oracle.make_basket([100.0, 50.0], [s1, s2], [q1, q2], corr, None, kind, True, 0.0, t,
+                                r, 400_000)
+        p, se = oracle.basket_price(bk)
+        assert abs(p - want) < 4 * se, (kind, p, want, se)
+
+
+def test_single_stream_is_path_major(oracle):
+    n, m = 3_000, 4
+    corr = np.full((m, m), 0.25)
+    np.fill_diagonal(corr, 1.0)
+    b = oracle.make_basket([100, 90, 110, 105], [0.2, 0.3, 0.25, 0.15], [0.0] * 4, corr,
+                           [0.25] * 4, AVG, False, 100.0, 2.0, 0.01, n, seed=77)
+    z = oracle.lsm_stream_normals(77, n * m).reshape(n, m)
+    assert oracle.basket_price(b) == oracle.basket_price(b, z)
+
+
+def test_zero_maturity_prices_the_spots(oracle):
+    b = oracle.make_basket([100.0, 120.0], [0.2, 0.2], [0.0, 0.0], [[1, 0.2], [0.2, 1]],
+                           None, BEST, True, 1.1, 0.0, 0.05, 100)
+    assert oracle.basket_price(b) == (0.0, 0.0)   # max ratio is 1 < 1.1
+    b = oracle.make_basket([100.0, 120.0], [0.2, 0.2], [0.0, 0.0], [[1, 0.2], [0.2, 1]],
+                           [0.5, 0.5], AVG, True, 100.0, 0.0, 0.05, 100)
+    assert oracle.basket_price(b) == (10.0, 0.0)   # no discounting at zero maturity
+
+
+FIVE = dict(spots=[100.0, 98.0, 102.0, 95.0, 105.0], vols=[0.2, 0.21, 0.19, 0.23, 0.18],
+            divs=[0.01] * 5, weights=[0.25, 0.2, 0.2, 0.2, 0.15],
+            corr=[[1.0, 0.35, 0.30, 0.20, 0.15], [0.35, 1.0, 0.25, 0.30, 0.10],
+                  [0.30, 0.25, 1.0, 0.28, 0.22], [0.20, 0.30, 0.28, 1.0, 0.26],
+                  [0.15, 0.10, 0.22, 0.26, 1.0]])
+
+
+def test_reference_five_asset_matches_moment_matching(oracle):
+    """five_asset_mc_matches_moment_matching_within_two_percent (basket.rs:1155-1195)."""
+    f = FIVE
+    b = oracle.make_basket(f["spots"], f["vols"], f["divs"], f["corr"], f["weights"], AVG, True,
+                           100.0, 1.0, 0.02, 250_000)
+    mc, se = oracle.basket_price(b)
+    mm = moment_matching("gentle", True, 100.0, f["spots"], f["weights"], f["vols"], f["corr"],
+                         0.02, f["divs"], 1.0)
+    assert abs(mc - mm) / mm <= 0.02, (mc, mm)
+    levy = moment_matching("levy", True, 100.0, f["spots"], f["weights"], f["vols"], f["corr"],
+                           0.02, f["divs"], 1.0)
+    assert abs(mc - levy) / levy <= 0.02 and se < 0.05
+
+
+def test_reference_one_asset_and_three_asset_tests(oracle):
+    b = oracle.make_basket([100.0], [0.2], [0.0], [[1.0]], [1.0], AVG, True, 100.0, 1.0, 0.03,
+                           120_000)
+    p, se = oracle.basket_price(b)
+    assert abs(p - bs_price(True, 100, 100, 0.03, 0.0, 0.2, 1.0)) <= 3 * se + 0.15
+    corr = [[1.0, 0.4, 0.2], [0.4, 1.0, 0.3], [0.2, 0.3, 1.0]]
+    prices = {}
+    for kind in (WORST, BEST):
+        b = oracle.make_basket([100.0, 98.0, 102.0], [0.25, 0.22, 0.2], [0.0] * 3, corr, None,
+                               kind, True, 1.0, 1.0, 0.01, 120_000)
+        prices[kind] = oracle.basket_price(b)[0]
+        assert math.isfinite(prices[kind]) and prices[kind] >= 0.0
+    assert prices[BEST] > prices[WORST]
+
+
+def test_non_psd_matrix_is_refused(oracle):
+    bad = [[1.0, 0.95, 0.95], [0.95, 1.0, -0.95], [0.95, -0.95, 1.0]]
+    b = oracle.make_basket([100.0] * 3, [0.2] * 3, [0.0] * 3, bad, [1 / 3] * 3, AVG)
+    with pytest.raises(ValueError):
+        oracle.basket_price(b)
+
+
+# ---- the library's host-side correlation repair ---------------------------
+
+def test_higham_published_example():
+    """Higham (2002), section 4: A = [[1,1,0],[1,1,1],[0,1,1]] has nearest
+    correlation matrix with off-diagonals 0.7607 and 0.1573."""
+    from openferric_amd import validate_or_repair_correlation_matrix
+    a = np.array([[1.0, 1.0, 0.0], [1.0, 1.0, 1.0], [0.0, 1.0, 1.0]])
+    m, rep = validate_or_repair_correlation_matrix(a)
+    assert rep
+    assert abs(m[0, 1] - 0.7607) < 5e-5 and abs(m[1, 2] - 0.7607) < 5e-5
+    assert abs(m[0, 2] - 0.1573) < 5e-5
+    assert np.abs(m - higham_nearest(a)).max() < 1e-9
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_repair_matches_numpy_projection(seed):
+    from openferric_amd import validate_or_repair_correlation_matrix
+    rng = np.random.default_rng(seed)
+    n = 4 + 3 * seed
+    a = rng.uniform(-0.9, 0.9, size=(n, n))
+    a = (a + a.T) / 2
+    np.fill_diagonal(a, 1.0)
+    assert np.linalg.eigvalsh(a).min() < -1e-3
+    m, rep = validate_or_repair_correlation_matrix(a)
+    assert rep
+    want = higham_nearest(a)
+    assert np.abs(m - want).max() < 1e-8
+    assert np.linalg.eigvalsh(m).min() >= -1e-8 and np.all(np.diag(m) == 1.0)
+    assert np.array_equal(m, m.T)
+
+
+def test_reference_repair_case_and_psd_passthrough():
+    """psd_repair_validation_produces_valid_matrix (basket.rs:1197-1210)."""
+    from openferric_amd import InvalidInput, validate_or_repair_correlation_matrix
+    bad = [[1.0, 0.95, 0.95], [0.95, 1.0, -0.95], [0.95, -0.95, 1.0]]
+    m, rep = validate_or_repair_correlation_matrix(bad)
+    assert rep and np.linalg.eigvalsh(m).min() >= -1e-8
+    good = np.array([[1.0, 0.3], [0.3, 1.0]])
+    m, rep = validate_or_repair_correlation_matrix(good)
+    assert not rep and np.array_equal(m, good)
+    with pytest.raises(InvalidInput, match="symmetric"):
+        validate_or_repair_correlation_matrix([[1.0, 0.3], [0.2, 1.0]])
+    with pytest.raises(InvalidInput, match="diagonal"):
+        validate_or_repair_correlation_matrix([[1.0, 0.3], [0.3, 0.9]])
