@@ -386,6 +386,167 @@ def qmc_leg(reps=3, cpu=True):
     return out
 
 
+AC_CONTRACTS, AC_UNDERLYINGS, AC_PATHS, AC_STEPS = 16, 4, 1 << 18, 252
+BK_BASKETS, BK_ASSETS, BK_PATHS = 32, 8, 1 << 20
+
+
+def flop_multi_asset_step(n):
+    """Canonical FLOP per asset-step of a correlated path: the GBM step's 65
+    (normal, exp, move) + the average Cholesky row, (n + 1) / 2 FMAs."""
+    return FLOP_GBM_STEP + (n + 1)
+
+
+def _corr_matrix(n, rho, seed):
+    rng = np.random.default_rng(seed)
+    c = np.full((n, n), rho) + rng.uniform(-0.1, 0.1, (n, n))
+    c = (c + c.T) / 2
+    np.fill_diagonal(c, 1.0)
+    assert np.linalg.eigvalsh(c).min() > 0.05
+    return c
+
+
+def autocall_workload(seed=WORKLOAD_SEED + 7):
+    """Worst-of Phoenix notes with memory on 4 of 12 underlyings, quarterly
+    observations over 3 years, daily steps."""
+    from openferric_amd import PhoenixAutocallable
+    rng = np.random.default_rng(seed)
+    market = dict(spots=list(rng.uniform(50, 150, 12)), vols=list(rng.uniform(0.15, 0.4, 12)),
+                  corr=_corr_matrix(12, 0.45, seed), r=0.03, q=0.01)
+    contracts = []
+    for i in range(AC_CONTRACTS):
+        und = [int(x) for x in rng.choice(12, AC_UNDERLYINGS, replace=False)]
+        contracts.append(PhoenixAutocallable(
+            und, 100.0, [0.25 * k for k in range(1, 13)], float(rng.uniform(1.0, 1.1)),
+            float(rng.uniform(0.6, 0.8)), float(rng.uniform(0.06, 0.12)), True,
+            float(rng.uniform(0.5, 0.65)), 1.0, 3.0))
+    return market, contracts
+
+
+def basket_workload(seed=WORKLOAD_SEED + 8):
+    """Average / best-of / worst-of calls and puts on 8 assets each."""
+    from openferric_amd import BasketOption
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(BK_BASKETS):
+        kind = i % 3
+        w = list(rng.dirichlet(np.ones(BK_ASSETS))) if kind == 0 else []
+        strike = float(rng.uniform(90, 110)) if kind == 0 else float(rng.uniform(0.9, 1.1))
+        out.append(dict(basket=BasketOption(w, strike, float(rng.uniform(0.5, 3.0)),
+                                            bool(i % 2 == 0), kind),
+                        spots=list(rng.uniform(50, 150, BK_ASSETS)),
+                        vols=list(rng.uniform(0.15, 0.45, BK_ASSETS)),
+                        divs=list(rng.uniform(0.0, 0.03, BK_ASSETS)),
+                        corr=_corr_matrix(BK_ASSETS, 0.35, seed + i), r=0.02))
+    return out
+
+
+def multi_leg(reps=3, cpu=True):
+    """Multi-asset contracts, each priced with its bump-and-reprice
+    sensitivities (every bump a scenario on the base price's draws) in one
+    launch: worst-of Phoenix autocallables (fast draws) and baskets (the
+    reference's single stream, split by jump-ahead)."""
+    from openferric_amd import RngMode, autocall_price_raw, autocall_struct, basket_price_raw
+    from openferric_amd import basket_struct
+    from openferric_amd.engines import _Keep
+    out = {}
+    # ---- autocallables
+    m, contracts = autocall_workload()
+    keep = _Keep()
+    structs = [autocall_struct(c, m["spots"], m["vols"], m["corr"], m["r"], m["q"], AC_PATHS,
+                               AC_STEPS, keep) for c in contracts]
+    autocall_price_raw(structs, RngMode.Fast, None, True)
+    ms = [autocall_price_raw(structs, RngMode.Fast, None, True)[0][0].kernel_ms
+          for _ in range(reps)]
+    k = float(np.mean(ms))
+    scen = AC_CONTRACTS * (1 + 2 * AC_UNDERLYINGS + 4)
+    paths = scen * AC_PATHS
+    nominal = paths * AC_STEPS * AC_UNDERLYINGS
+    ac = {"workload": f"{AC_CONTRACTS} worst-of Phoenix notes with memory, {AC_UNDERLYINGS} "
+                      f"underlyings, 12 quarterly observations, {AC_STEPS} steps, {AC_PATHS} "
+                      f"paths, price + delta/vega/cega ({scen} scenarios), fast draws, one launch",
+          "kernel_ms": k, "scenario_paths_per_s": paths / (k * 1e-3),
+          "nominal_asset_steps_per_s": nominal / (k * 1e-3),
+          "note": "called paths stop early, so nominal asset-steps overstate the work"}
+    if cpu:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import oracle as O
+        O.build()
+        threads = cpu_threads()
+        cpaths = 1 << 16
+        sample = (contracts * 4)[:2 * threads]
+
+        def one(c):
+            u = list(c.underlyings)
+            a = O.make_autocall(spots=[m["spots"][i] for i in u], vols=[m["vols"][i] for i in u],
+                                corr=np.asarray(m["corr"])[np.ix_(u, u)],
+                                dates=list(c.autocall_dates), notional=c.notional,
+                                maturity=c.maturity, autocall_barrier=c.autocall_barrier,
+                                coupon_rate=c.coupon_rate, coupon_barrier=c.coupon_barrier,
+                                ki_barrier=c.ki_barrier, ki_strike=c.ki_strike, rate=m["r"],
+                                div_yield=m["q"], phoenix=True, memory=True,
+                                num_paths=cpaths, num_steps=AC_STEPS)
+            return O.autocall_price(a, None)
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            list(ex.map(one, sample))
+        dt = time.perf_counter() - t0
+        cps = len(sample) * cpaths / dt
+        ac["cpu_baseline"] = {"value": cps, "unit": "scenario-paths/s", "cores": threads,
+                              "kind": "port",
+                              "sample": f"{len(sample)} base prices at {cpaths} paths, one per "
+                                        f"thread, {dt:.2f} s, oracle C restatement (the "
+                                        "reference's single stream)"}
+        ac["speedup_vs_cpu"] = ac["scenario_paths_per_s"] / cps
+    out["autocallable"] = ac
+    # ---- baskets
+    bw = basket_workload()
+    keep = _Keep()
+    bstructs = [basket_struct(d["basket"], d["spots"], d["vols"], d["corr"], d["r"], d["divs"],
+                              BK_PATHS, keep) for d in bw]
+    bscen = BK_BASKETS * (1 + 2 * BK_ASSETS + 4)
+    bk = {"workload": f"{BK_BASKETS} baskets (average / best-of / worst-of) on {BK_ASSETS} "
+                      f"assets, {BK_PATHS} paths, price + basket_sensitivities ({bscen} "
+                      "scenarios), one launch"}
+    for mode, name in ((RngMode.Reference, "reference_draws"), (RngMode.Fast, "fast_draws")):
+        basket_price_raw(bstructs, mode, None, True)
+        ms = [basket_price_raw(bstructs, mode, None, True)[0][0].kernel_ms for _ in range(reps)]
+        k = float(np.mean(ms))
+        sp = bscen * BK_PATHS / (k * 1e-3)
+        bk[name] = {"kernel_ms": k, "scenario_paths_per_s": sp,
+                    "frac_fp64_peak": sp * BK_ASSETS * flop_multi_asset_step(BK_ASSETS) / 1e12
+                    / FP64_PEAK_TFLOPS}
+    if cpu:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import oracle as O
+        threads = cpu_threads()
+        cpaths = 1 << 22
+        sample = (bw * 4)[:2 * threads]
+
+        def oneb(d):
+            b = d["basket"]
+            ob = O.make_basket(d["spots"], d["vols"], d["divs"], d["corr"],
+                               b.weights if b.weights else None, b.basket_type, b.is_call,
+                               b.strike, b.maturity, d["r"], cpaths)
+            return O.basket_price(ob, None)
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            list(ex.map(oneb, sample))
+        dt = time.perf_counter() - t0
+        cps = len(sample) * cpaths / dt
+        bk["cpu_baseline"] = {"value": cps, "unit": "scenario-paths/s", "cores": threads,
+                              "kind": "port",
+                              "sample": f"{len(sample)} base prices at {cpaths} paths, one per "
+                                        f"thread, {dt:.2f} s, oracle C restatement (the "
+                                        "reference's single stream)"}
+        bk["speedup_vs_cpu"] = bk["reference_draws"]["scenario_paths_per_s"] / cps
+    bk["flop_model"] = (f"{flop_multi_asset_step(BK_ASSETS)} FLOP per asset of a path: GBM "
+                        "step 65 + average Cholesky row")
+    out["basket"] = bk
+    return out
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (a plain copy kernel of known
     size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -471,6 +632,10 @@ def main():
     ap.add_argument("--no-qmc", action="store_true", help="skip the quasi-Monte Carlo leg")
     ap.add_argument("--qmc-only", action="store_true",
                     help="run only the quasi-Monte Carlo leg (profiling)")
+    ap.add_argument("--no-multi", action="store_true",
+                    help="skip the autocallable / basket leg")
+    ap.add_argument("--multi-only", action="store_true",
+                    help="run only the autocallable / basket leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one D2D copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -490,6 +655,9 @@ def main():
         return
     if args.qmc_only:
         print(json.dumps({"qmc": qmc_leg(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.multi_only:
+        print(json.dumps({"multi": multi_leg(cpu=not args.no_cpu)}), flush=True)
         return
 
     opts = make_workload(path_scale=args.path_scale)
@@ -569,6 +737,9 @@ def main():
     if not args.no_qmc:
         print("[bench] quasi-monte-carlo leg", file=sys.stderr, flush=True)
         out["qmc"] = qmc_leg(cpu=not args.no_cpu)
+    if not args.no_multi:
+        print("[bench] multi-asset leg", file=sys.stderr, flush=True)
+        out["multi"] = multi_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(opts)
