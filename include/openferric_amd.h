@@ -242,6 +242,52 @@ typedef struct ofg_basket_sens {
     int32_t pad;
 } ofg_basket_sens;
 
+/* Structured-product language (src/dsl): a compiled product, owned by the
+ * caller (ofg_dsl_compile / ofg_dsl_free). */
+typedef struct ofg_dsl_product ofg_dsl_product;
+
+/* AssetMarketData (src/dsl/market.rs:4-30): the kind selects the stepper
+ * (Equity, Fx: GBM; Commodity: Schwartz one-factor; Rate: Vasicek). */
+typedef enum ofg_dsl_asset_kind {
+    OFG_DSL_EQUITY = 0,
+    OFG_DSL_FX = 1,
+    OFG_DSL_COMMODITY = 2,
+    OFG_DSL_RATE = 3
+} ofg_dsl_asset_kind;
+
+typedef struct ofg_dsl_asset {
+    int32_t kind;             /* ofg_dsl_asset_kind */
+    int32_t pad;
+    double spot;              /* Equity / Fx / Commodity spot; Rate initial_rate */
+    double vol;
+    double dividend_yield;    /* Equity */
+    double domestic_rate, foreign_rate;       /* Fx */
+    double convenience_yield, kappa, mu;      /* Commodity */
+    double mean_reversion, long_run_mean;     /* Rate */
+} ofg_dsl_asset;
+
+/* MultiAssetMarket (src/dsl/market.rs:158-168). */
+typedef struct ofg_dsl_market {
+    int32_t n_assets;         /* at most 16 */
+    int32_t pad;
+    const ofg_dsl_asset *assets;
+    const double *corr;       /* n_assets x n_assets, row-major */
+    double rate;
+} ofg_dsl_market;
+
+/* DslMonteCarloEngine (src/dsl/engine.rs:237-253). */
+typedef struct ofg_dsl_engine {
+    int64_t num_paths, num_steps;
+    uint64_t seed;
+} ofg_dsl_engine;
+
+/* CompiledProduct's header fields (src/dsl/ir.rs:141-151). */
+typedef struct ofg_dsl_info {
+    double notional, maturity;
+    int32_t num_underlyings, n_state, n_schedules, n_locals;
+    char name[128];           /* truncated, NUL-terminated */
+} ofg_dsl_info;
+
 typedef struct ofg_ctx ofg_ctx;
 typedef struct ofg_batch ofg_batch;
 
@@ -391,6 +437,33 @@ int32_t ofg_basket_price(ofg_ctx *ctx, const ofg_basket *baskets, size_t n,
  * else a copy. Writes out (n x n), *was_repaired. Host-only. */
 int32_t ofg_repair_correlation(const double *corr, int32_t n, double *out,
                                int32_t *was_repaired);
+
+/* Structured-product language: parse_and_compile (src/dsl/mod.rs:24-28,
+ * lexer.rs, parser.rs, compiler.rs). On failure *out is NULL, the status is
+ * OFG_INVALID_INPUT and ofg_last_error holds the DslError text. */
+int32_t ofg_dsl_compile(const char *source, ofg_dsl_product **out);
+void ofg_dsl_free(ofg_dsl_product *product);
+int32_t ofg_dsl_get_info(const ofg_dsl_product *product, ofg_dsl_info *info);
+/* Observation dates of schedule `schedule` (ScheduleFreq::generate_dates,
+ * src/dsl/ast.rs:64-80): writes min(count, cap) dates, *count = all. */
+int32_t ofg_dsl_schedule_dates(const ofg_dsl_product *product, int32_t schedule,
+                               double *dates, int32_t cap, int32_t *count);
+
+/* DslMonteCarloEngine::price_multi_asset (src/dsl/engine.rs:255-353) of one
+ * product under n_markets markets in one launch, all on the same draws (the
+ * reference reprices every bump of greeks_multi_asset /
+ * extended_greeks_multi_asset / cross_greeks_multi_asset, :785-1000, with the
+ * same seed): out[i] is the price under markets[i]. Correlated paths with
+ * the reference's steppers; the compiled product runs per path on the GPU.
+ * cfg->rng_mode: REFERENCE (the engine's single xoshiro256++ stream split
+ * across paths by jump-ahead; the default build's draws), EXTERNAL
+ * (normals1 [step][asset][path], shared by all markets) or FAST. Errors are
+ * the reference's (InvalidInput for the market and engine, NumericalError
+ * "eval error: ..." for the product); the markets must share their asset
+ * count. Diagnostics num_paths, num_steps. */
+int32_t ofg_dsl_price(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_dsl_market *markets,
+                      size_t n_markets, const ofg_dsl_engine *engine, const ofg_engine_cfg *cfg,
+                      ofg_result *out);
 
 /* Device time of the context's last ofg_lsm_price call, split into path
  * simulation (seed and path kernels) and the backward rounds, and the number

@@ -24,6 +24,8 @@ from .multi import (Autocallable, AutocallableSensitivities, BasketOption,
                     basket_sensitivities, basket_struct, phoenix_autocallable_sensitivities,
                     price_autocallable, price_basket_mc, price_phoenix_autocallable,
                     validate_or_repair_correlation_matrix)
+from .dsl import (AssetMarketData, CompiledProduct, CrossGreeks, DslMonteCarloEngine, DslProduct,
+                  ExtendedGreeks, MultiAssetMarket, parse_and_compile)
 from ._lib import DeviceError
 
 __all__ = [
@@ -43,6 +45,8 @@ __all__ = [
     "BasketOption", "BasketSensitivities", "BasketType", "basket_price_raw",
     "basket_sensitivities", "basket_struct", "price_basket_mc",
     "validate_or_repair_correlation_matrix",
+    "AssetMarketData", "CompiledProduct", "CrossGreeks", "DslMonteCarloEngine", "DslProduct",
+    "ExtendedGreeks", "MultiAssetMarket", "parse_and_compile",
     "autocall_struct", "autocallable_sensitivities", "phoenix_autocallable_sensitivities",
     "price_autocallable", "price_phoenix_autocallable",
 ]

@@ -39,6 +39,8 @@ EXPORTED_SYMBOLS = [
     "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
     "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump", "ofg_lsm_last_timing",
     "ofg_qmc_price", "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation",
+    "ofg_dsl_compile", "ofg_dsl_free", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
+    "ofg_dsl_price",
 ]
 
 
@@ -151,6 +153,37 @@ class OfgBasketSens(C.Structure):
     ]
 
 
+DSL_EQUITY, DSL_FX, DSL_COMMODITY, DSL_RATE = 0, 1, 2, 3
+
+
+class OfgDslAsset(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("pad", C.c_int32), ("spot", C.c_double), ("vol", C.c_double),
+        ("dividend_yield", C.c_double), ("domestic_rate", C.c_double),
+        ("foreign_rate", C.c_double), ("convenience_yield", C.c_double), ("kappa", C.c_double),
+        ("mu", C.c_double), ("mean_reversion", C.c_double), ("long_run_mean", C.c_double),
+    ]
+
+
+class OfgDslMarket(C.Structure):
+    _fields_ = [
+        ("n_assets", C.c_int32), ("pad", C.c_int32), ("assets", C.POINTER(OfgDslAsset)),
+        ("corr", C.POINTER(C.c_double)), ("rate", C.c_double),
+    ]
+
+
+class OfgDslEngine(C.Structure):
+    _fields_ = [("num_paths", C.c_int64), ("num_steps", C.c_int64), ("seed", C.c_uint64)]
+
+
+class OfgDslInfo(C.Structure):
+    _fields_ = [
+        ("notional", C.c_double), ("maturity", C.c_double), ("num_underlyings", C.c_int32),
+        ("n_state", C.c_int32), ("n_schedules", C.c_int32), ("n_locals", C.c_int32),
+        ("name", C.c_char * 128),
+    ]
+
+
 class DeviceError(RuntimeError):
     """HIP library missing or device failure (OFG_DEVICE_ERROR)."""
 
@@ -220,6 +253,19 @@ def load() -> C.CDLL:
         L.ofg_repair_correlation.argtypes = [C.POINTER(C.c_double), C.c_int32,
                                              C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         L.ofg_repair_correlation.restype = C.c_int32
+        L.ofg_dsl_compile.argtypes = [C.c_char_p, C.POINTER(pv)]
+        L.ofg_dsl_compile.restype = C.c_int32
+        L.ofg_dsl_free.argtypes = [pv]
+        L.ofg_dsl_free.restype = None
+        L.ofg_dsl_get_info.argtypes = [pv, C.POINTER(OfgDslInfo)]
+        L.ofg_dsl_get_info.restype = C.c_int32
+        L.ofg_dsl_schedule_dates.argtypes = [pv, C.c_int32, C.POINTER(C.c_double), C.c_int32,
+                                             C.POINTER(C.c_int32)]
+        L.ofg_dsl_schedule_dates.restype = C.c_int32
+        L.ofg_dsl_price.argtypes = [pv, pv, C.POINTER(OfgDslMarket), C.c_size_t,
+                                    C.POINTER(OfgDslEngine), C.POINTER(OfgEngineCfg),
+                                    C.POINTER(OfgResult)]
+        L.ofg_dsl_price.restype = C.c_int32
         _lib = L
         return L
 

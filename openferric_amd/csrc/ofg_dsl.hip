@@ -1,0 +1,343 @@
+// Structured-product Monte Carlo on MI355X (gfx950), FP64 on the VALU:
+// correlated multi-asset paths (DslMonteCarloEngine::price_path_chunk_scalar,
+// src/dsl/engine.rs:413-495) and the compiled product evaluated per path
+// (evaluate_product_with_plan_in_place and execute_program, src/dsl/
+// eval.rs:383-500, 1774-1980).
+//
+// One thread simulates one path at a time: per step it draws one normal per
+// asset (the reference's single stream positioned by jump-ahead, caller
+// normals [step][asset][path], or fast per-path xoroshiro128+ Box-Muller
+// draws), correlates them with the scenario's lower Cholesky factor (running
+// sum from 0 in column order, correlate_normals) and moves every asset with
+// its stepper (GBM / Schwartz one-factor / Vasicek, engine.rs:95-122). The
+// spots at snapshot steps go to the tile's snapshot buffer, [snap][asset]
+// [thread] (coalesced). The thread then runs the product: the schedules in
+// order, each observation with cleared locals, the schedule's register
+// program under predication (ofg_dsl.hpp). Per tile (sum, sum of squares) of
+// the path values and the first out-of-range price() index go to the tile's
+// slot; one block per scenario sums the slots in tile order.
+#include "ofg_dsl.hpp"
+#include "ofg_internal.hpp"
+#include "ofg_math.hpp"
+
+namespace ofg {
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int NWAVES = DSL_BLOCK / WAVE;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, WAVE));
+    return v;
+}
+
+// AssetStepper::step (engine.rs:95-122)
+__device__ __forceinline__ double step_asset(int kind, double prev, double p0, double p1, double p2,
+                                             double z) {
+    if (kind == DSL_GBM) return prev * exp_fp64(p0 + p1 * z);
+    if (kind == DSL_SCHWARTZ) {
+        const double log_prev = log(prev);
+        const double log_next = p0 + p1 * (log_prev - p0) + p2 * z;
+        return exp_fp64(log_next);
+    }
+    return p0 + p1 * (prev - p0) + p2 * z;
+}
+
+// `v as usize` of an f64 (saturating; NaN -> 0), compared with n; `shown`
+// is that index as a double for the error message.
+__device__ __forceinline__ bool index_in_range(double v, int n, int &idx, double &shown) {
+    if (!(v >= 1.0)) {   // NaN, negatives and [0, 1) truncate to 0
+        idx = 0;
+        shown = 0.0;
+        return n > 0;
+    }
+    if (v >= (double)n) {
+        idx = 0;
+        shown = v >= 1.8446744073709552e19 ? 1.8446744073709552e19 : trunc(v);
+        return false;
+    }
+    idx = (int)v;
+    shown = (double)idx;
+    return true;
+}
+
+template <int RNG>
+__device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o, int64_t p,
+                                              double *snap, const double *tab) {
+    const int n = o.n_assets;
+    double s[DSL_MAX_ASSETS];
+#pragma unroll
+    for (int i = 0; i < DSL_MAX_ASSETS; ++i) s[i] = o.s0[i];
+    const int k0 = a.step_snap[0];
+    if (k0 >= 0) {
+#pragma unroll
+        for (int i = 0; i < DSL_MAX_ASSETS; ++i)
+            if (i < n) snap[(k0 * n + i) * DSL_BLOCK] = s[i];
+    }
+    Xoshiro256pp ref;
+    Xoroshiro128p fast;
+    if constexpr (RNG == OFG_RNG_REFERENCE) {
+        const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
+        ref.s0 = sd[0];
+        ref.s1 = sd[1];
+        ref.s2 = sd[2];
+        ref.s3 = sd[3];
+    }
+    if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
+    const double *zp = a.normals + o.normals_offset + p;
+    for (int step = 1; step <= a.steps; ++step) {
+        double c[DSL_MAX_ASSETS];
+#pragma unroll
+        for (int i = 0; i < DSL_MAX_ASSETS; ++i) c[i] = 0.0;
+        double zpair = 0.0;
+#pragma unroll
+        for (int j = 0; j < DSL_MAX_ASSETS; ++j) {
+            if (j < n) {
+                double z;
+                if constexpr (RNG == OFG_RNG_REFERENCE) {
+                    z = reference_normal(ref);
+                } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
+                    z = zp[((int64_t)(step - 1) * n + j) * o.paths];
+                } else {
+                    if ((j & 1) == 0) {
+                        const uint64_t wu = fast.next_u64();
+                        const uint64_t wt = fast.next_u64();
+                        box_muller_pair(wu, (uint32_t)(wt >> 32), z, zpair, tab);
+                    } else {
+                        z = zpair;
+                    }
+                }
+#pragma unroll
+                for (int i = j; i < DSL_MAX_ASSETS; ++i)
+                    if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < DSL_MAX_ASSETS; ++i)
+            if (i < n) s[i] = step_asset(o.kind[i], s[i], o.p0[i], o.p1[i], o.p2[i], c[i]);
+        const int k = a.step_snap[step];
+        if (k >= 0) {
+#pragma unroll
+            for (int i = 0; i < DSL_MAX_ASSETS; ++i)
+                if (i < n) snap[(k * n + i) * DSL_BLOCK] = s[i];
+        }
+    }
+}
+
+// The product's value on one simulated path; err receives an out-of-range
+// price() index of an active lane.
+__device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double *snap,
+                                double &err) {
+    const int n = o.n_assets;
+    double r[DSL_MAX_REGS];
+    bool saved[DSL_MAX_MASKS], cond[DSL_MAX_MASKS];
+    for (int k = 0; k < a.n_state; ++k) r[a.n_locals + k] = a.state_init[k];
+    bool alive = true;
+    double pv = 0.0;
+    for (int si = 0; si < a.n_sched; ++si) {
+        const DslSched sc = a.sched[si];
+        for (int oi = 0; oi < sc.n_obs; ++oi) {
+            if (!__any(alive)) break;   // every path of the wave has ended
+            const DslObs ob = a.obs[sc.obs_first + oi];
+            const double df = a.df[o.df_off + sc.obs_first + oi];
+            const double *row = snap + ob.snap * n * DSL_BLOCK;
+            for (int k = 0; k < a.n_locals; ++k) r[k] = 0.0;
+            bool active = alive;
+            for (int pc = sc.code_off; pc < sc.code_off + sc.code_len; ++pc) {
+                const DslIns in = a.code[pc];
+                const int d = in.dst;
+                switch (in.op) {
+                case DOP_CONST: r[d] = a.consts[in.aux]; break;
+                case DOP_NOTIONAL: r[d] = a.notional; break;
+                case DOP_DATE: r[d] = ob.date; break;
+                case DOP_ISFINAL: r[d] = ob.is_final ? 1.0 : 0.0; break;
+                case DOP_SET:
+                    if (in.b || active) r[d] = r[in.a];
+                    break;
+                case DOP_ADD: r[d] = r[in.a] + r[in.b]; break;
+                case DOP_SUB: r[d] = r[in.a] - r[in.b]; break;
+                case DOP_MUL: r[d] = r[in.a] * r[in.b]; break;
+                case DOP_DIV: {
+                    const double den = r[in.b];
+                    r[d] = den == 0.0 ? NAN : r[in.a] / den;
+                    break;
+                }
+                case DOP_MIN: r[d] = fmin(r[in.a], r[in.b]); break;
+                case DOP_MAX: r[d] = fmax(r[in.a], r[in.b]); break;
+                case DOP_EQ: r[d] = fabs(r[in.a] - r[in.b]) < 2.220446049250313e-16 ? 1.0 : 0.0; break;
+                case DOP_NE: r[d] = fabs(r[in.a] - r[in.b]) >= 2.220446049250313e-16 ? 1.0 : 0.0; break;
+                case DOP_LT: r[d] = r[in.a] < r[in.b] ? 1.0 : 0.0; break;
+                case DOP_LE: r[d] = r[in.a] <= r[in.b] ? 1.0 : 0.0; break;
+                case DOP_GT: r[d] = r[in.a] > r[in.b] ? 1.0 : 0.0; break;
+                case DOP_GE: r[d] = r[in.a] >= r[in.b] ? 1.0 : 0.0; break;
+                case DOP_AND: r[d] = (r[in.a] != 0.0 && r[in.b] != 0.0) ? 1.0 : 0.0; break;
+                case DOP_OR: r[d] = (r[in.a] != 0.0 || r[in.b] != 0.0) ? 1.0 : 0.0; break;
+                case DOP_NEG: r[d] = -r[in.a]; break;
+                case DOP_ABS: r[d] = fabs(r[in.a]); break;
+                case DOP_EXP: r[d] = exp(r[in.a]); break;
+                case DOP_LOG: r[d] = log(r[in.a]); break;
+                case DOP_NOT: r[d] = r[in.a] != 0.0 ? 0.0 : 1.0; break;
+                case DOP_WOF_PERF:
+                case DOP_BOF_PERF: {
+                    const bool worst = in.op == DOP_WOF_PERF;
+                    double v = worst ? INFINITY : -INFINITY;
+                    for (int i = 0; i < n; ++i) {
+                        if (o.s0[i] > 0.0) {
+                            const double perf = row[i * DSL_BLOCK] / o.s0[i];
+                            if (worst ? perf < v : perf > v) v = perf;
+                        }
+                    }
+                    r[d] = v;
+                    break;
+                }
+                case DOP_WORST:
+                case DOP_BEST: {
+                    const bool worst = in.op == DOP_WORST;
+                    double v = worst ? INFINITY : -INFINITY;
+                    for (int k = (int)in.aux - 1; k >= 0; --k) {   // popped last first
+                        const double x = r[in.a + k];
+                        if (worst ? x < v : x > v) v = x;
+                    }
+                    r[d] = v;
+                    break;
+                }
+                case DOP_PRICE: {
+                    int idx;
+                    double shown;
+                    const bool ok = index_in_range(r[in.a], n, idx, shown);
+                    if (!ok && active) err = fmax(err, shown);
+                    r[d] = ok ? row[idx * DSL_BLOCK] : 0.0;
+                    break;
+                }
+                case DOP_PAY:
+                    if (active) pv += r[in.a] * df;
+                    break;
+                case DOP_REDEEM:
+                    if (active) {
+                        pv += r[in.a] * df;
+                        alive = false;
+                        active = false;
+                    }
+                    break;
+                case DOP_SKIP:
+                    if (active) {
+                        alive = false;
+                        active = false;
+                    }
+                    break;
+                case DOP_IF:
+                    saved[in.aux] = active;
+                    cond[in.aux] = r[in.a] != 0.0;
+                    active = active && cond[in.aux];
+                    break;
+                case DOP_ELSE: active = saved[in.aux] && !cond[in.aux] && alive; break;
+                case DOP_ENDIF: active = saved[in.aux] && alive; break;
+                default: break;
+                }
+            }
+        }
+    }
+    return pv;
+}
+
+template <int RNG>
+__global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    __shared__ double red[NWAVES][3];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += DSL_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    }
+    const int tile = blockIdx.x;
+    const DslScen &o = a.scen[a.tile_scen[tile]];
+    double *snap = a.snaps + (int64_t)tile * a.n_snaps * o.n_assets * DSL_BLOCK + threadIdx.x;
+    const int64_t base = (int64_t)(tile - o.tile_first) * DSL_BLOCK * o.ppt;
+    double sum = 0.0, sq = 0.0, err = -1.0;
+    for (int j = 0; j < o.ppt; ++j) {
+        const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
+        if (p >= o.paths) break;
+        simulate_path<RNG>(a, o, p, snap, tab);
+        const double x = evaluate_path(a, o, snap, err);
+        sum += x;
+        sq += x * x;
+    }
+    sum = wave_sum(sum);
+    sq = wave_sum(sq);
+    err = wave_max(err);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = sum;
+        red[wave][1] = sq;
+        red[wave][2] = err;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = red[0][0], q = red[0][1], e = red[0][2];
+        for (int w = 1; w < NWAVES; ++w) {
+            s += red[w][0];
+            q += red[w][1];
+            e = fmax(e, red[w][2]);
+        }
+        a.partials[tile].sum = s;
+        a.partials[tile].sum_sq = q;
+        a.partials[tile].err_idx = e;
+    }
+}
+
+// One block per scenario: the tile slots in tile order.
+__global__ __launch_bounds__(DSL_BLOCK) void dsl_close_kernel(DslArgs a) {
+    __shared__ double red[NWAVES][3];
+    const DslScen &o = a.scen[blockIdx.x];
+    double s = 0.0, q = 0.0, e = -1.0;
+    for (int t = threadIdx.x; t < o.n_tiles; t += DSL_BLOCK) {
+        const DslPartial pt = a.partials[o.tile_first + t];
+        s += pt.sum;
+        q += pt.sum_sq;
+        e = fmax(e, pt.err_idx);
+    }
+    s = wave_sum(s);
+    q = wave_sum(q);
+    e = wave_max(e);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = s;
+        red[wave][1] = q;
+        red[wave][2] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double sum = red[0][0], sq = red[0][1], err = red[0][2];
+    for (int w = 1; w < NWAVES; ++w) {
+        sum += red[w][0];
+        sq += red[w][1];
+        err = fmax(err, red[w][2]);
+    }
+    a.out[blockIdx.x].sum = sum;
+    a.out[blockIdx.x].sum_sq = sq;
+    a.out[blockIdx.x].err_idx = err;
+}
+
+}  // namespace
+
+hipError_t launch_dsl(const DslArgs &a, hipStream_t stream) {
+    if (a.n_tiles <= 0) return hipSuccess;
+    switch (a.rng_mode) {
+    case OFG_RNG_FAST: dsl_kernel<OFG_RNG_FAST><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+    case OFG_RNG_EXTERNAL: dsl_kernel<OFG_RNG_EXTERNAL><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+    default: dsl_kernel<OFG_RNG_REFERENCE><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dsl_close_kernel<<<a.n_scen, DSL_BLOCK, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace ofg

@@ -1,0 +1,129 @@
+// Device layouts and launch interface of the structured-product engine
+// (ofg_dsl.hip): correlated multi-asset paths and the compiled product's
+// program evaluated per path (src/dsl/engine.rs, src/dsl/eval.rs). Shared
+// with ofg_dsl_front.cpp and ofg_dsl_api.cpp.
+//
+// Program model. The reference compiles each schedule's statements to stack
+// bytecode with conditional jumps (eval.rs:1542-1772). Here the stack depth
+// of every instruction is known when compiling, so stack slots become
+// registers with fixed indices, and branches become predication: every lane
+// of a wave runs the same instruction stream (fetched with scalar loads)
+// under a per-lane `active` flag that if / else / end-if narrow and restore
+// from a mask stack of fixed depth. A lane that redeems or skips clears its
+// `alive` flag, which every restore re-applies; its later statements and
+// observations run masked, as the reference returns from the path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ofg {
+
+constexpr int DSL_MAX_ASSETS = 16;
+constexpr int DSL_MAX_REGS = 64;
+constexpr int DSL_MAX_MASKS = 16;
+constexpr int DSL_BLOCK = 256;
+
+enum DslOp : uint8_t {
+    DOP_CONST = 0,    // r[dst] = consts[aux]
+    DOP_NOTIONAL,     // r[dst] = notional
+    DOP_DATE,         // r[dst] = observation date
+    DOP_ISFINAL,      // r[dst] = is_final ? 1 : 0
+    DOP_SET,          // active: r[dst] = r[a]   (let, set)
+    DOP_ADD, DOP_SUB, DOP_MUL, DOP_DIV, DOP_MIN, DOP_MAX,
+    DOP_EQ, DOP_NE, DOP_LT, DOP_LE, DOP_GT, DOP_GE, DOP_AND, DOP_OR,
+    DOP_NEG, DOP_ABS, DOP_EXP, DOP_LOG, DOP_NOT,
+    DOP_WOF_PERF,     // r[dst] = min_i S_i(t)/S_i(0) over assets with S_i(0) > 0
+    DOP_BOF_PERF,     // r[dst] = max_i ...
+    DOP_WORST,        // r[dst] = min of r[a .. a+aux-1], last argument first
+    DOP_BEST,         // r[dst] = max of ...
+    DOP_PRICE,        // r[dst] = S_idx(t), idx = r[a] as usize (active lanes check the range)
+    DOP_PAY,          // active: pv += r[a] * df
+    DOP_REDEEM,       // active: pv += r[a] * df; the path ends
+    DOP_SKIP,         // active: the path ends
+    DOP_IF,           // saved[aux] = active; cond[aux] = r[a] != 0; active &= cond[aux]
+    DOP_ELSE,         // active = saved[aux] && !cond[aux] && alive
+    DOP_ENDIF,        // active = saved[aux] && alive
+};
+
+struct DslIns {
+    uint8_t op, dst, a, b;
+    uint32_t aux;
+};
+
+// One schedule: its program and its observations.
+struct DslSched {
+    int32_t code_off, code_len;
+    int32_t obs_first, n_obs;
+};
+
+struct DslObs {
+    int32_t snap;       // snapshot index (eval.rs:288-302)
+    int32_t is_final;
+    double date;
+};
+
+enum DslStepper : int32_t { DSL_GBM = 0, DSL_SCHWARTZ = 1, DSL_VASICEK = 2 };
+
+// One scenario: the product under one market (the base price or a bump).
+struct alignas(16) DslScen {
+    int32_t n_assets;
+    int32_t tile_first;
+    int32_t n_tiles;
+    int32_t ppt;            // paths per thread
+    int64_t paths;
+    int64_t normals_offset; // external normals [step][asset][path]
+    int64_t seed_offset;    // reference-stream start states [path][word]
+    uint64_t seed;          // fast draws: stream_seed(seed, path)
+    int32_t df_off;         // discount factors of the observations, exp(-r t)
+    int32_t pad;
+    int32_t kind[DSL_MAX_ASSETS];
+    double s0[DSL_MAX_ASSETS];
+    // GBM: p0 drift, p1 diffusion; Schwartz: p0 long-run log mean,
+    // p1 exp(-kappa dt), p2 vol sqrt(dt); Vasicek: p0 long-run mean,
+    // p1 exp(-a dt), p2 vol sqrt(dt)   (engine.rs:51-91)
+    double p0[DSL_MAX_ASSETS], p1[DSL_MAX_ASSETS], p2[DSL_MAX_ASSETS];
+    double chol[DSL_MAX_ASSETS * (DSL_MAX_ASSETS + 1) / 2];
+};
+
+struct DslPartial {
+    double sum, sum_sq;
+    double err_idx;   // first out-of-range price() index of the tile, or -1
+    double pad;
+};
+
+struct DslOut {
+    double sum, sum_sq, err_idx, pad;
+};
+
+struct DslArgs {
+    const DslScen *scen;
+    const int32_t *tile_scen;
+    const DslIns *code;
+    const double *consts;
+    const DslSched *sched;
+    const DslObs *obs;
+    const double *df;
+    const int32_t *step_snap;   // [steps + 1]: snapshot index or -1
+    double *snaps;              // [tile][snap][asset][DSL_BLOCK]
+    DslPartial *partials;
+    DslOut *out;
+    const double *normals;
+    const uint64_t *seeds;
+    double notional;
+    int32_t steps;
+    int32_t n_snaps;
+    int32_t n_sched;
+    int32_t n_locals;
+    int32_t n_state;
+    int32_t n_masks;
+    int32_t n_tiles;
+    int32_t n_scen;
+    int32_t rng_mode;
+    int32_t pad;
+    const double *state_init;   // [n_state]
+};
+
+hipError_t launch_dsl(const DslArgs &a, hipStream_t stream);
+
+}  // namespace ofg

@@ -1,0 +1,387 @@
+// Host side of the structured-product engine: ofg_dsl_compile and friends
+// (parse_and_compile, src/dsl/mod.rs:24-28) and ofg_dsl_price
+// (DslMonteCarloEngine::price_multi_asset, src/dsl/engine.rs:255-353) for
+// one product under several markets in one launch.
+//
+// Per market, in the reference's order: MultiAssetMarket::validate
+// (market.rs:170-215), the engine's path / step counts and the product's
+// maturity (engine.rs:260-277), cholesky_for_correlation (correlated_mc.rs:
+// 17-33: validation, Higham repair, PSD Cholesky), then the execution plan
+// (eval.rs:268-330: observation -> snapshot step map, exp(-r t) discount
+// factors, the schedules' programs).
+#include "../../include/openferric_amd.h"
+#include "ofg_corr.hpp"
+#include "ofg_dsl.hpp"
+#include "ofg_dsl_front.hpp"
+#include "ofg_host.hpp"
+#include "ofg_internal.hpp"
+#include "ofg_jump.hpp"
+#include "ofg_math.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace ofg_host;
+using namespace ofg_corr;
+
+struct ofg_dsl_product {
+    std::unique_ptr<ofg_dsl::Product> prod;
+};
+
+namespace ofg_host {
+
+struct DslBuffers {
+    DevBuf scen, tiles, code, consts, sched, obs, df, step_snap, snaps, partials, out, normals,
+        seeds, polys, jobs, state;
+};
+
+void dsl_release(ofg_ctx *ctx) {
+    if (!ctx || !ctx->dsl) return;
+    DslBuffers *b = ctx->dsl;
+    for (DevBuf *d : {&b->scen, &b->tiles, &b->code, &b->consts, &b->sched, &b->obs, &b->df,
+                      &b->step_snap, &b->snaps, &b->partials, &b->out, &b->normals, &b->seeds,
+                      &b->polys, &b->jobs, &b->state})
+        d->release();
+    delete b;
+    ctx->dsl = nullptr;
+}
+
+}  // namespace ofg_host
+
+namespace {
+
+constexpr size_t SNAP_BUDGET = size_t(4) << 30;   // snapshot buffer bytes per launch
+
+// MultiAssetMarket::validate (market.rs:170-215)
+void validate_market(const ofg_dsl_market &m) {
+    if (m.n_assets <= 0 || !m.assets)
+        fail(OFG_INVALID_INPUT, "multi-asset market requires at least one asset");
+    if (!m.corr) fail(OFG_INVALID_INPUT, "correlation matrix dimensions must match asset count");
+    for (int32_t i = 0; i < m.n_assets; ++i) {
+        const ofg_dsl_asset &a = m.assets[i];
+        if (a.kind < OFG_DSL_EQUITY || a.kind > OFG_DSL_RATE)
+            fail(OFG_INVALID_INPUT, "unknown asset kind");
+        if (a.kind == OFG_DSL_RATE) {
+            if (a.vol <= 0.0 || !std::isfinite(a.vol))
+                fail(OFG_INVALID_INPUT, "rate asset vol must be finite and > 0");
+        } else {
+            if (a.spot <= 0.0) fail(OFG_INVALID_INPUT, "asset spot must be > 0");
+            if (a.vol <= 0.0 || !std::isfinite(a.vol))
+                fail(OFG_INVALID_INPUT, "asset vol must be finite and > 0");
+        }
+    }
+}
+
+// `x.round() as usize` (saturating)
+int64_t round_to_step(double x) {
+    const double r = std::round(x);
+    if (!(r > 0.0)) return 0;
+    if (r >= 9.2e18) return INT64_MAX;
+    return (int64_t)r;
+}
+
+int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_dsl_market *mk,
+                       size_t n_mk, const ofg_dsl_engine *eng, const ofg_engine_cfg *cfg,
+                       ofg_result *out) {
+    if (!ctx || !product || !product->prod || !eng || !cfg || (n_mk && (!mk || !out)))
+        fail(OFG_INVALID_INPUT, "null argument");
+    const int32_t rng = cfg->rng_mode;
+    if (rng != OFG_RNG_EXTERNAL && rng != OFG_RNG_FAST && rng != OFG_RNG_REFERENCE)
+        fail(OFG_INVALID_INPUT, "unknown rng mode");
+    for (size_t i = 0; i < n_mk; ++i) {
+        std::memset(&out[i], 0, sizeof out[i]);
+        out[i].price = NAN;
+        out[i].num_paths = out[i].num_steps = out[i].vol = NAN;
+    }
+    if (n_mk == 0) return OFG_OK;
+    const ofg_dsl::Product &prod = *product->prod;
+    const int64_t paths = eng->num_paths, steps = eng->num_steps;
+    std::vector<ofg::DslScen> scen(n_mk);
+    std::vector<Mat> chols(n_mk);
+    const int na = mk[0].n_assets;
+    for (size_t s = 0; s < n_mk; ++s) {
+        const ofg_dsl_market &m = mk[s];
+        validate_market(m);
+        if (paths <= 0) fail(OFG_INVALID_INPUT, "num_paths must be > 0");
+        if (steps <= 0) fail(OFG_INVALID_INPUT, "num_steps must be > 0");
+        if (prod.maturity <= 0.0) fail(OFG_INVALID_INPUT, "product maturity must be > 0");
+        const int n = m.n_assets;
+        bool rep = false;
+        const Mat c = validate_or_repair(Mat(m.corr, m.corr + (size_t)n * n), n, rep);
+        if (!cholesky_psd(c, n, chols[s]))
+            fail(OFG_INVALID_INPUT, "correlation matrix could not be factorized");
+        if (n != na) fail(OFG_INVALID_INPUT, "all markets of one call must have the same asset count");
+    }
+    if (na > ofg::DSL_MAX_ASSETS) fail(OFG_INVALID_INPUT, "at most 16 assets per market");
+    // build_execution_plan (eval.rs:268-330)
+    int n_locals = 0;
+    std::vector<ofg_dsl::Program> progs;
+    try {
+        progs = ofg_dsl::build_programs(prod, n_locals);
+    } catch (const ofg_dsl::DslError &e) {
+        fail(OFG_NUMERICAL_ERROR, e.what());
+    }
+    const double maturity = prod.maturity;
+    const double step_scale = (double)steps / maturity;
+    std::vector<int32_t> step_snap((size_t)steps + 1, -1);
+    std::vector<ofg::DslSched> sched;
+    std::vector<ofg::DslObs> obs;
+    std::vector<ofg::DslIns> code;
+    std::vector<double> consts;
+    int32_t n_snaps = 0, n_regs = 0, n_masks = 0;
+    for (size_t k = 0; k < prod.schedules.size(); ++k) {
+        const ofg_dsl::Schedule &sc = prod.schedules[k];
+        ofg::DslSched d{};
+        d.obs_first = (int32_t)obs.size();
+        d.n_obs = (int32_t)sc.dates.size();
+        for (size_t j = 0; j < sc.dates.size(); ++j) {
+            const int64_t step = std::min<int64_t>(round_to_step(sc.dates[j] * step_scale), steps);
+            if (step_snap[(size_t)step] < 0) step_snap[(size_t)step] = n_snaps++;
+            ofg::DslObs o{};
+            o.snap = step_snap[(size_t)step];
+            o.is_final = j + 1 == sc.dates.size();
+            o.date = sc.dates[j];
+            obs.push_back(o);
+        }
+        ofg_dsl::Program &pg = progs[k];
+        const int32_t c0 = (int32_t)consts.size();
+        for (ofg::DslIns in : pg.code) {
+            if (in.op == ofg::DOP_CONST) in.aux += (uint32_t)c0;
+            code.push_back(in);
+        }
+        d.code_off = (int32_t)(code.size() - pg.code.size());
+        d.code_len = (int32_t)pg.code.size();
+        consts.insert(consts.end(), pg.consts.begin(), pg.consts.end());
+        n_regs = std::max(n_regs, pg.n_regs);
+        n_masks = std::max(n_masks, pg.n_masks);
+        sched.push_back(d);
+    }
+    n_regs = std::max<int32_t>(n_regs, n_locals + (int32_t)prod.state_vars.size());
+    if (n_regs > ofg::DSL_MAX_REGS || n_masks > ofg::DSL_MAX_MASKS)
+        fail(OFG_INVALID_INPUT, "product program exceeds the device limits (64 registers, 16 nested ifs)");
+    std::vector<double> state_init;
+    for (const auto &sv : prod.state_vars) state_init.push_back(sv.initial);
+
+    auto fill_result = [&](size_t i, double sum, double sq) {
+        const double nn = (double)paths;
+        const double mean = sum / nn;
+        const double var = paths > 1 ? (sq - sum * sum / nn) / (nn - 1.0) : 0.0;
+        out[i].price = mean;
+        out[i].stderr_ = std::sqrt(var / nn);
+        out[i].has_stderr = 1;
+        out[i].num_paths = nn;
+        out[i].num_steps = (double)steps;
+    };
+    if (n_snaps == 0) {   // no observation: every path is worth 0 (engine.rs:424-430)
+        for (size_t i = 0; i < n_mk; ++i) fill_result(i, 0.0, 0.0);
+        return OFG_OK;
+    }
+    const double dt = maturity / (double)steps, sqrt_dt = std::sqrt(dt);
+    std::vector<double> dfs;
+    std::vector<int32_t> tiles;
+    const size_t tile_bytes = (size_t)n_snaps * na * ofg::DSL_BLOCK * sizeof(double);
+    const int64_t tile_cap = std::max<int64_t>(
+        1, std::min<int64_t>(1024, (int64_t)(SNAP_BUDGET / tile_bytes) / (int64_t)n_mk));
+    std::vector<ofg::LsmSeedJob> jobs;
+    std::vector<uint64_t> polys;
+    std::map<uint64_t, std::vector<ofg_jump::Poly>> poly_cache;
+    int64_t seed_threads = 0;
+    if (rng == OFG_RNG_REFERENCE) {
+        const ofg::LsmSeedJob j =
+            ofg_jump::make_seed_job(eng->seed, paths, steps * na, 0, 0, polys, poly_cache);
+        jobs.push_back(j);
+        seed_threads = ofg_jump::seed_job_threads(j);
+    }
+    for (size_t s = 0; s < n_mk; ++s) {
+        const ofg_dsl_market &m = mk[s];
+        ofg::DslScen &d = scen[s];
+        d.n_assets = na;
+        d.paths = paths;
+        d.seed = eng->seed;
+        d.df_off = (int32_t)dfs.size();
+        for (const ofg::DslObs &o : obs) dfs.push_back(std::exp(-m.rate * o.date));
+        for (int i = 0; i < na; ++i) {   // AssetStepper::from_asset (engine.rs:51-93)
+            const ofg_dsl_asset &a = m.assets[i];
+            d.s0[i] = a.spot;
+            switch (a.kind) {
+            case OFG_DSL_EQUITY:
+            case OFG_DSL_FX: {
+                const double mu = a.kind == OFG_DSL_EQUITY ? m.rate - a.dividend_yield
+                                                           : a.domestic_rate - a.foreign_rate;
+                d.kind[i] = ofg::DSL_GBM;
+                d.p0[i] = (mu - 0.5 * a.vol * a.vol) * dt;
+                d.p1[i] = a.vol * sqrt_dt;
+                break;
+            }
+            case OFG_DSL_COMMODITY:
+                d.kind[i] = ofg::DSL_SCHWARTZ;
+                d.p0[i] = a.mu;
+                d.p1[i] = std::exp(-a.kappa * dt);
+                d.p2[i] = a.vol * sqrt_dt;
+                break;
+            default:
+                d.kind[i] = ofg::DSL_VASICEK;
+                d.p0[i] = a.long_run_mean;
+                d.p1[i] = std::exp(-a.mean_reversion * dt);
+                d.p2[i] = a.vol * sqrt_dt;
+            }
+            for (int j = 0; j <= i; ++j) d.chol[i * (i + 1) / 2 + j] = chols[s][(size_t)i * na + j];
+        }
+        const int64_t blk = ofg::DSL_BLOCK;
+        d.ppt = (int32_t)std::max<int64_t>(1, (paths + blk * tile_cap - 1) / (blk * tile_cap));
+        d.n_tiles = (int32_t)((paths + blk * d.ppt - 1) / (blk * d.ppt));
+        d.tile_first = (int32_t)tiles.size();
+        for (int32_t t = 0; t < d.n_tiles; ++t) tiles.push_back((int32_t)s);
+    }
+    if (rng == OFG_RNG_EXTERNAL && !cfg->normals1)
+        fail(OFG_INVALID_INPUT, "external rng mode needs normals1");
+
+    std::vector<ofg::DslOut> res(n_mk);
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!ctx->dsl) ctx->dsl = new DslBuffers();
+        DslBuffers &b = *ctx->dsl;
+        hipStream_t st = ctx->stream;
+        auto up = [&](DevBuf &buf, const void *src, size_t bytes, const char *what) {
+            buf.ensure(std::max<size_t>(bytes, 8));
+            if (bytes) hip_check(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, st), what);
+        };
+        up(b.scen, scen.data(), scen.size() * sizeof(scen[0]), "H2D dsl scenarios");
+        up(b.tiles, tiles.data(), tiles.size() * sizeof(int32_t), "H2D dsl tiles");
+        up(b.code, code.data(), code.size() * sizeof(code[0]), "H2D dsl program");
+        up(b.consts, consts.data(), consts.size() * sizeof(double), "H2D dsl constants");
+        up(b.sched, sched.data(), sched.size() * sizeof(sched[0]), "H2D dsl schedules");
+        up(b.obs, obs.data(), obs.size() * sizeof(obs[0]), "H2D dsl observations");
+        up(b.df, dfs.data(), dfs.size() * sizeof(double), "H2D dsl discount factors");
+        up(b.step_snap, step_snap.data(), step_snap.size() * sizeof(int32_t), "H2D dsl snapshot map");
+        up(b.state, state_init.data(), state_init.size() * sizeof(double), "H2D dsl state");
+        b.snaps.ensure(tiles.size() * tile_bytes);
+        b.partials.ensure(tiles.size() * sizeof(ofg::DslPartial));
+        b.out.ensure(n_mk * sizeof(ofg::DslOut));
+        ofg::DslArgs a{};
+        if (rng == OFG_RNG_EXTERNAL) {
+            up(b.normals, cfg->normals1, (size_t)(steps * na * paths) * sizeof(double), "H2D normals1");
+            a.normals = b.normals.as<double>();
+        }
+        if (rng == OFG_RNG_REFERENCE) {
+            up(b.jobs, jobs.data(), jobs.size() * sizeof(jobs[0]), "H2D seed jobs");
+            up(b.polys, polys.data(), polys.size() * sizeof(uint64_t), "H2D jump polynomials");
+            b.seeds.ensure((size_t)(4 * paths) * sizeof(uint64_t));
+            a.seeds = b.seeds.as<uint64_t>();
+        }
+        a.scen = b.scen.as<ofg::DslScen>();
+        a.tile_scen = b.tiles.as<int32_t>();
+        a.code = b.code.as<ofg::DslIns>();
+        a.consts = b.consts.as<double>();
+        a.sched = b.sched.as<ofg::DslSched>();
+        a.obs = b.obs.as<ofg::DslObs>();
+        a.df = b.df.as<double>();
+        a.step_snap = b.step_snap.as<int32_t>();
+        a.snaps = b.snaps.as<double>();
+        a.partials = b.partials.as<ofg::DslPartial>();
+        a.out = b.out.as<ofg::DslOut>();
+        a.state_init = b.state.as<double>();
+        a.notional = prod.notional;
+        a.steps = (int32_t)steps;
+        a.n_snaps = n_snaps;
+        a.n_sched = (int32_t)sched.size();
+        a.n_locals = n_locals;
+        a.n_state = (int32_t)state_init.size();
+        a.n_masks = n_masks;
+        a.n_tiles = (int32_t)tiles.size();
+        a.n_scen = (int32_t)n_mk;
+        a.rng_mode = rng;
+        if (steps > INT32_MAX) fail(OFG_INVALID_INPUT, "num_steps exceeds the device limit");
+        hip_check(hipEventRecord(ctx->ev0, st), "event");
+        if (rng == OFG_RNG_REFERENCE)
+            hip_check(ofg::launch_lsm_seeds(b.jobs.as<ofg::LsmSeedJob>(), 1, seed_threads,
+                                            b.polys.as<uint64_t>(), b.seeds.as<uint64_t>(), st),
+                      "dsl seed kernel launch");
+        hip_check(ofg::launch_dsl(a, st), "dsl kernel launch");
+        hip_check(hipEventRecord(ctx->ev1, st), "event");
+        hip_check(hipMemcpyAsync(res.data(), a.out, res.size() * sizeof(res[0]),
+                                 hipMemcpyDeviceToHost, st),
+                  "D2H dsl results");
+        hip_check(hipStreamSynchronize(st), "dsl kernel execution");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
+        for (size_t i = 0; i < n_mk; ++i) out[i].kernel_ms = ms;
+    }
+    for (size_t i = 0; i < n_mk; ++i) {
+        if (res[i].err_idx >= 0.0) {   // execute_program's PRICE range check (eval.rs:1936-1945)
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "eval error: asset index %.0f out of range (have %d assets)",
+                          res[i].err_idx, na);
+            for (size_t k = 0; k < n_mk; ++k) out[k].price = NAN, out[k].has_stderr = 0;
+            fail(OFG_NUMERICAL_ERROR, buf);
+        }
+        fill_result(i, res[i].sum, res[i].sum_sq);
+    }
+    return OFG_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t ofg_dsl_compile(const char *source, ofg_dsl_product **out) {
+    return guarded([&]() {
+        if (!out) fail(OFG_INVALID_INPUT, "null argument");
+        *out = nullptr;
+        if (!source) fail(OFG_INVALID_INPUT, "null argument");
+        auto p = std::make_unique<ofg_dsl_product>();
+        try {
+            p->prod = ofg_dsl::parse_and_compile(source);
+        } catch (const ofg_dsl::DslError &e) {
+            fail(OFG_INVALID_INPUT, e.what());   // From<DslError> for PricingError
+        }
+        *out = p.release();
+        return OFG_OK;
+    });
+}
+
+extern "C" void ofg_dsl_free(ofg_dsl_product *product) { delete product; }
+
+extern "C" int32_t ofg_dsl_get_info(const ofg_dsl_product *product, ofg_dsl_info *info) {
+    return guarded([&]() {
+        if (!product || !product->prod || !info) fail(OFG_INVALID_INPUT, "null argument");
+        const ofg_dsl::Product &p = *product->prod;
+        std::memset(info, 0, sizeof *info);
+        info->notional = p.notional;
+        info->maturity = p.maturity;
+        info->num_underlyings = (int32_t)p.num_underlyings;
+        info->n_state = (int32_t)p.state_vars.size();
+        info->n_schedules = (int32_t)p.schedules.size();
+        info->n_locals = p.max_local_slots();
+        std::strncpy(info->name, p.name.c_str(), sizeof info->name - 1);
+        return OFG_OK;
+    });
+}
+
+extern "C" int32_t ofg_dsl_schedule_dates(const ofg_dsl_product *product, int32_t schedule,
+                                          double *dates, int32_t cap, int32_t *count) {
+    return guarded([&]() {
+        if (!product || !product->prod || !count || (cap > 0 && !dates))
+            fail(OFG_INVALID_INPUT, "null argument");
+        const ofg_dsl::Product &p = *product->prod;
+        if (schedule < 0 || schedule >= (int32_t)p.schedules.size())
+            fail(OFG_INVALID_INPUT, "schedule index out of range");
+        const std::vector<double> &d = p.schedules[(size_t)schedule].dates;
+        *count = (int32_t)d.size();
+        for (int32_t i = 0; i < cap && i < (int32_t)d.size(); ++i) dates[i] = d[(size_t)i];
+        return OFG_OK;
+    });
+}
+
+extern "C" int32_t ofg_dsl_price(ofg_ctx *ctx, const ofg_dsl_product *product,
+                                 const ofg_dsl_market *markets, size_t n_markets,
+                                 const ofg_dsl_engine *engine, const ofg_engine_cfg *cfg,
+                                 ofg_result *out) {
+    return guarded(
+        [&]() { return dsl_price_impl(ctx, product, markets, n_markets, engine, cfg, out); });
+}

@@ -96,6 +96,7 @@ struct LsmBuffers;   // ofg_lsm_api.cpp
 struct QmcBuffers;   // ofg_qmc_api.cpp
 struct MultiBuffers; // ofg_multi_api.cpp
 struct BasketBuffers; // ofg_basket_api.cpp
+struct DslBuffers;    // ofg_dsl_api.cpp
 
 // Instrument validation with the reference's messages (instruments/*.rs
 // validate()); throws OfgError.
@@ -115,6 +116,7 @@ struct ofg_ctx {
     ofg_host::QmcBuffers *qmc = nullptr;   // quasi-Monte Carlo tables and partials
     ofg_host::MultiBuffers *multi = nullptr;   // multi-asset scenarios and partials
     ofg_host::BasketBuffers *basket = nullptr; // basket scenarios, seeds and partials
+    ofg_host::DslBuffers *dsl = nullptr;       // structured-product programs and snapshots
     int cus = 256;
 };
 
@@ -127,4 +129,6 @@ void qmc_release(ofg_ctx *ctx);
 void multi_release(ofg_ctx *ctx);
 // Releases the context's basket buffers (ofg_basket_api.cpp).
 void basket_release(ofg_ctx *ctx);
+// Releases the context's structured-product buffers (ofg_dsl_api.cpp).
+void dsl_release(ofg_ctx *ctx);
 }  // namespace ofg_host
