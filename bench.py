@@ -547,6 +547,84 @@ def multi_leg(reps=3, cpu=True):
     return out
 
 
+DSL_PATHS, DSL_STEPS = 1 << 20, 378
+DSL_PRODUCT = """\
+product "WoF Autocall 18m"
+    notional: 1_000_000
+    maturity: 1.5
+    underlyings
+        SPX = asset(0)
+        SX5E = asset(1)
+        NKY = asset(2)
+    state
+        ki_hit: bool = false
+    schedule quarterly from 0.25 to 1.5
+        let wof = worst_of(performances())
+        if wof <= 0.60 then
+            set ki_hit = true
+        if wof >= 1.0 and not is_final then
+            pay notional * 0.08 * observation_date
+            redeem notional
+        if is_final then
+            pay notional * 0.08 * 1.5
+            if ki_hit and wof < 1.0 then
+                redeem notional * wof
+            else
+                redeem notional
+"""
+
+
+def dsl_leg(reps=3, cpu=True):
+    """Structured-product language: a worst-of autocallable on three
+    correlated equities, 2^20 paths x 378 daily steps, the reference's
+    single stream (jump-ahead) and fast draws; price alone and with
+    greeks_multi_asset (5 markets in one launch)."""
+    from openferric_amd import (AssetMarketData as A, DslMonteCarloEngine, MultiAssetMarket,
+                                RngMode, parse_and_compile)
+    prod = parse_and_compile(DSL_PRODUCT)
+    corr = [[1.0, 0.7, 0.5], [0.7, 1.0, 0.6], [0.5, 0.6, 1.0]]
+    mkt = MultiAssetMarket([A.Equity(100.0, 0.20, 0.02), A.Equity(100.0, 0.22, 0.03),
+                            A.Equity(100.0, 0.25, 0.01)], corr, 0.03)
+    asset_steps = DSL_PATHS * DSL_STEPS * 3
+    out = {"workload": f"worst-of autocallable (3 correlated equities, quarterly over 1.5 y, "
+                       f"knock-in), {DSL_PATHS} paths x {DSL_STEPS} steps",
+           "asset_steps": asset_steps}
+    for mode, name in ((RngMode.Reference, "reference_draws"), (RngMode.Fast, "fast_draws")):
+        eng = DslMonteCarloEngine(DSL_PATHS, DSL_STEPS, 42, rng_mode=mode)
+        eng.price_multi_asset(prod, mkt)
+        ms = []
+        for _ in range(reps):
+            ms.append(eng.price_multi_asset(prod, mkt).kernel_ms)
+        k = float(np.mean(ms))
+        eng.greeks_multi_asset(prod, mkt, 0)
+        gms = []
+        for _ in range(reps):
+            eng.greeks_multi_asset(prod, mkt, 0)
+            gms.append(eng.last_kernel_ms)
+        g = float(np.mean(gms))
+        out[name] = {"price_ms": k, "asset_steps_per_s": asset_steps / (k * 1e-3),
+                     "greeks_5_markets_ms": g,
+                     "greeks_asset_steps_per_s": 5 * asset_steps / (g * 1e-3),
+                     "frac_fp64_peak": asset_steps / (k * 1e-3) * flop_multi_asset_step(3) / 1e12
+                     / FP64_PEAK_TFLOPS}
+    if cpu:
+        from oracle import dsl_oracle as D
+        cpaths = 20_000
+        z = np.random.default_rng(1).standard_normal((cpaths, DSL_STEPS, 3))
+        assets = [dict(kind="equity", spot=100.0, vol=v, dividend_yield=q)
+                  for v, q in ((0.2, 0.02), (0.22, 0.03), (0.25, 0.01))]
+        t0 = time.perf_counter()
+        D.price(DSL_PRODUCT, assets, corr, 0.03, cpaths, DSL_STEPS, z)
+        dt = time.perf_counter() - t0
+        cps = cpaths * DSL_STEPS * 3 / dt
+        out["cpu_baseline"] = {"value": cps, "unit": "asset-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{cpaths} paths x {DSL_STEPS} steps, {dt:.2f} s, the "
+                                         "oracle's numpy paths and pure-Python evaluator (not "
+                                         "representative of the reference's compiled engine)"}
+        out["speedup_vs_cpu"] = out["reference_draws"]["asset_steps_per_s"] / cps
+    return out
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (a plain copy kernel of known
     size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -636,6 +714,9 @@ def main():
                     help="skip the autocallable / basket leg")
     ap.add_argument("--multi-only", action="store_true",
                     help="run only the autocallable / basket leg (profiling)")
+    ap.add_argument("--no-dsl", action="store_true", help="skip the structured-product leg")
+    ap.add_argument("--dsl-only", action="store_true",
+                    help="run only the structured-product leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one D2D copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -658,6 +739,9 @@ def main():
         return
     if args.multi_only:
         print(json.dumps({"multi": multi_leg(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.dsl_only:
+        print(json.dumps({"dsl": dsl_leg(cpu=not args.no_cpu)}), flush=True)
         return
 
     opts = make_workload(path_scale=args.path_scale)
@@ -740,6 +824,9 @@ def main():
     if not args.no_multi:
         print("[bench] multi-asset leg", file=sys.stderr, flush=True)
         out["multi"] = multi_leg(cpu=not args.no_cpu)
+    if not args.no_dsl:
+        print("[bench] structured-product leg", file=sys.stderr, flush=True)
+        out["dsl"] = dsl_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(opts)

@@ -10,11 +10,15 @@
 // draws), correlates them with the scenario's lower Cholesky factor (running
 // sum from 0 in column order, correlate_normals) and moves every asset with
 // its stepper (GBM / Schwartz one-factor / Vasicek, engine.rs:95-122). The
-// spots at snapshot steps go to the tile's snapshot buffer, [snap][asset]
-// [thread] (coalesced). The thread then runs the product: the schedules in
-// order, each observation with cleared locals, the schedule's register
-// program under predication (ofg_dsl.hpp). Per tile (sum, sum of squares) of
-// the path values and the first out-of-range price() index go to the tile's
+// spots at snapshot steps are stored, then the product runs on them: the
+// schedules in order, each observation with cleared locals, the schedule's
+// register program under predication (ofg_dsl.hpp). Split form (the
+// default): dsl_sim_kernel writes every path's snapshots, [snap][asset][path]
+// per scenario, and dsl_eval_kernel runs the programs -- the simulation then
+// carries none of the interpreter's registers. Fused form (when all
+// snapshots would not fit the buffer budget): one kernel per tile with a
+// reused [snap][asset][thread] buffer. Per tile (sum, sum of squares) of the
+// path values and the first out-of-range price() index go to the tile's
 // slot; one block per scenario sums the slots in tile order.
 #include "ofg_dsl.hpp"
 #include "ofg_internal.hpp"
@@ -69,18 +73,20 @@ __device__ __forceinline__ bool index_in_range(double v, int n, int &idx, double
     return true;
 }
 
-template <int RNG>
+// MAXA: the asset count rounded up to 1, 2, 4, 8 or 16 (registers per path).
+// Snapshot k of asset i goes to snap[(k n + i) stride].
+template <int RNG, int MAXA>
 __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o, int64_t p,
-                                              double *snap, const double *tab) {
+                                              double *snap, int64_t stride, const double *tab) {
     const int n = o.n_assets;
-    double s[DSL_MAX_ASSETS];
+    double s[MAXA];
 #pragma unroll
-    for (int i = 0; i < DSL_MAX_ASSETS; ++i) s[i] = o.s0[i];
+    for (int i = 0; i < MAXA; ++i) s[i] = o.s0[i];
     const int k0 = a.step_snap[0];
     if (k0 >= 0) {
 #pragma unroll
-        for (int i = 0; i < DSL_MAX_ASSETS; ++i)
-            if (i < n) snap[(k0 * n + i) * DSL_BLOCK] = s[i];
+        for (int i = 0; i < MAXA; ++i)
+            if (i < n) snap[(int64_t)(k0 * n + i) * stride] = s[i];
     }
     Xoshiro256pp ref;
     Xoroshiro128p fast;
@@ -94,12 +100,12 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
     if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
     const double *zp = a.normals + o.normals_offset + p;
     for (int step = 1; step <= a.steps; ++step) {
-        double c[DSL_MAX_ASSETS];
+        double c[MAXA];
 #pragma unroll
-        for (int i = 0; i < DSL_MAX_ASSETS; ++i) c[i] = 0.0;
+        for (int i = 0; i < MAXA; ++i) c[i] = 0.0;
         double zpair = 0.0;
 #pragma unroll
-        for (int j = 0; j < DSL_MAX_ASSETS; ++j) {
+        for (int j = 0; j < MAXA; ++j) {
             if (j < n) {
                 double z;
                 if constexpr (RNG == OFG_RNG_REFERENCE) {
@@ -116,18 +122,18 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
                     }
                 }
 #pragma unroll
-                for (int i = j; i < DSL_MAX_ASSETS; ++i)
+                for (int i = j; i < MAXA; ++i)
                     if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
             }
         }
 #pragma unroll
-        for (int i = 0; i < DSL_MAX_ASSETS; ++i)
+        for (int i = 0; i < MAXA; ++i)
             if (i < n) s[i] = step_asset(o.kind[i], s[i], o.p0[i], o.p1[i], o.p2[i], c[i]);
         const int k = a.step_snap[step];
         if (k >= 0) {
 #pragma unroll
-            for (int i = 0; i < DSL_MAX_ASSETS; ++i)
-                if (i < n) snap[(k * n + i) * DSL_BLOCK] = s[i];
+            for (int i = 0; i < MAXA; ++i)
+                if (i < n) snap[(int64_t)(k * n + i) * stride] = s[i];
         }
     }
 }
@@ -135,7 +141,7 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
 // The product's value on one simulated path; err receives an out-of-range
 // price() index of an active lane.
 __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double *snap,
-                                double &err) {
+                                int64_t stride, double &err) {
     const int n = o.n_assets;
     double r[DSL_MAX_REGS];
     bool saved[DSL_MAX_MASKS], cond[DSL_MAX_MASKS];
@@ -148,7 +154,7 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
             if (!__any(alive)) break;   // every path of the wave has ended
             const DslObs ob = a.obs[sc.obs_first + oi];
             const double df = a.df[o.df_off + sc.obs_first + oi];
-            const double *row = snap + ob.snap * n * DSL_BLOCK;
+            const double *row = snap + (int64_t)ob.snap * n * stride;
             for (int k = 0; k < a.n_locals; ++k) r[k] = 0.0;
             bool active = alive;
             for (int pc = sc.code_off; pc < sc.code_off + sc.code_len; ++pc) {
@@ -191,7 +197,7 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
                     double v = worst ? INFINITY : -INFINITY;
                     for (int i = 0; i < n; ++i) {
                         if (o.s0[i] > 0.0) {
-                            const double perf = row[i * DSL_BLOCK] / o.s0[i];
+                            const double perf = row[i * stride] / o.s0[i];
                             if (worst ? perf < v : perf > v) v = perf;
                         }
                     }
@@ -214,7 +220,7 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
                     double shown;
                     const bool ok = index_in_range(r[in.a], n, idx, shown);
                     if (!ok && active) err = fmax(err, shown);
-                    r[d] = ok ? row[idx * DSL_BLOCK] : 0.0;
+                    r[d] = ok ? row[idx * stride] : 0.0;
                     break;
                 }
                 case DOP_PAY:
@@ -248,7 +254,7 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
     return pv;
 }
 
-template <int RNG>
+template <int RNG, int MAXA>
 __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     __shared__ double red[NWAVES][3];
@@ -264,8 +270,63 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        simulate_path<RNG>(a, o, p, snap, tab);
-        const double x = evaluate_path(a, o, snap, err);
+        simulate_path<RNG, MAXA>(a, o, p, snap, DSL_BLOCK, tab);
+        const double x = evaluate_path(a, o, snap, DSL_BLOCK, err);
+        sum += x;
+        sq += x * x;
+    }
+    sum = wave_sum(sum);
+    sq = wave_sum(sq);
+    err = wave_max(err);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = sum;
+        red[wave][1] = sq;
+        red[wave][2] = err;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = red[0][0], q = red[0][1], e = red[0][2];
+        for (int w = 1; w < NWAVES; ++w) {
+            s += red[w][0];
+            q += red[w][1];
+            e = fmax(e, red[w][2]);
+        }
+        a.partials[tile].sum = s;
+        a.partials[tile].sum_sq = q;
+        a.partials[tile].err_idx = e;
+    }
+}
+
+// Split form, pass 1: the paths' snapshots, [snap][asset][path] per scenario.
+template <int RNG, int MAXA>
+__global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += DSL_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    }
+    const int tile = blockIdx.x;
+    const DslScen &o = a.scen[a.tile_scen[tile]];
+    const int64_t base = (int64_t)(tile - o.tile_first) * DSL_BLOCK * o.ppt;
+    for (int j = 0; j < o.ppt; ++j) {
+        const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
+        if (p >= o.paths) break;
+        simulate_path<RNG, MAXA>(a, o, p, a.snaps + o.snap_off + p, o.paths, tab);
+    }
+}
+
+// Split form, pass 2: the product on every path; tile sums as the fused form.
+__global__ __launch_bounds__(DSL_BLOCK) void dsl_eval_kernel(DslArgs a) {
+    __shared__ double red[NWAVES][3];
+    const int tile = blockIdx.x;
+    const DslScen &o = a.scen[a.tile_scen[tile]];
+    const int64_t base = (int64_t)(tile - o.tile_first) * DSL_BLOCK * o.ppt;
+    double sum = 0.0, sq = 0.0, err = -1.0;
+    for (int j = 0; j < o.ppt; ++j) {
+        const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
+        if (p >= o.paths) break;
+        const double x = evaluate_path(a, o, a.snaps + o.snap_off + p, o.paths, err);
         sum += x;
         sq += x * x;
     }
@@ -327,13 +388,32 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_close_kernel(DslArgs a) {
 
 }  // namespace
 
+template <int RNG, int MAXA>
+void launch_rng(const DslArgs &a, hipStream_t stream) {
+    if (a.split) {
+        dsl_sim_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
+        dsl_eval_kernel<<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
+    } else {
+        dsl_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
+    }
+}
+
+template <int MAXA>
+void launch_tiles(const DslArgs &a, hipStream_t stream) {
+    switch (a.rng_mode) {
+    case OFG_RNG_FAST: launch_rng<OFG_RNG_FAST, MAXA>(a, stream); break;
+    case OFG_RNG_EXTERNAL: launch_rng<OFG_RNG_EXTERNAL, MAXA>(a, stream); break;
+    default: launch_rng<OFG_RNG_REFERENCE, MAXA>(a, stream); break;
+    }
+}
+
 hipError_t launch_dsl(const DslArgs &a, hipStream_t stream) {
     if (a.n_tiles <= 0) return hipSuccess;
-    switch (a.rng_mode) {
-    case OFG_RNG_FAST: dsl_kernel<OFG_RNG_FAST><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
-    case OFG_RNG_EXTERNAL: dsl_kernel<OFG_RNG_EXTERNAL><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
-    default: dsl_kernel<OFG_RNG_REFERENCE><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
-    }
+    if (a.n_assets <= 1) launch_tiles<1>(a, stream);
+    else if (a.n_assets <= 2) launch_tiles<2>(a, stream);
+    else if (a.n_assets <= 4) launch_tiles<4>(a, stream);
+    else if (a.n_assets <= 8) launch_tiles<8>(a, stream);
+    else launch_tiles<DSL_MAX_ASSETS>(a, stream);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     dsl_close_kernel<<<a.n_scen, DSL_BLOCK, 0, stream>>>(a);

@@ -77,6 +77,7 @@ struct alignas(16) DslScen {
     uint64_t seed;          // fast draws: stream_seed(seed, path)
     int32_t df_off;         // discount factors of the observations, exp(-r t)
     int32_t pad;
+    int64_t snap_off;       // split form: this scenario's [snap][asset][path] snapshots
     int32_t kind[DSL_MAX_ASSETS];
     double s0[DSL_MAX_ASSETS];
     // GBM: p0 drift, p1 diffusion; Schwartz: p0 long-run log mean,
@@ -105,7 +106,8 @@ struct DslArgs {
     const DslObs *obs;
     const double *df;
     const int32_t *step_snap;   // [steps + 1]: snapshot index or -1
-    double *snaps;              // [tile][snap][asset][DSL_BLOCK]
+    double *snaps;              // split: per scenario [snap][asset][path];
+                                // fused: [tile][snap][asset][DSL_BLOCK]
     DslPartial *partials;
     DslOut *out;
     const double *normals;
@@ -120,7 +122,9 @@ struct DslArgs {
     int32_t n_tiles;
     int32_t n_scen;
     int32_t rng_mode;
-    int32_t pad;
+    int32_t n_assets;           // of every scenario
+    int32_t split;              // 1: simulation and evaluation kernels, 0: fused
+    int32_t pad2;
     const double *state_init;   // [n_state]
 };
 

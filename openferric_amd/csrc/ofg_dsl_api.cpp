@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -56,7 +57,8 @@ void dsl_release(ofg_ctx *ctx) {
 
 namespace {
 
-constexpr size_t SNAP_BUDGET = size_t(4) << 30;   // snapshot buffer bytes per launch
+constexpr size_t SPLIT_BUDGET = size_t(16) << 30;  // all paths' snapshots (split form)
+constexpr size_t SNAP_BUDGET = size_t(4) << 30;   // per-tile snapshot buffers (fused form)
 
 // MultiAssetMarket::validate (market.rs:170-215)
 void validate_market(const ofg_dsl_market &m) {
@@ -186,8 +188,17 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
     std::vector<double> dfs;
     std::vector<int32_t> tiles;
     const size_t tile_bytes = (size_t)n_snaps * na * ofg::DSL_BLOCK * sizeof(double);
-    const int64_t tile_cap = std::max<int64_t>(
-        1, std::min<int64_t>(1024, (int64_t)(SNAP_BUDGET / tile_bytes) / (int64_t)n_mk));
+    const size_t path_bytes = (size_t)n_snaps * na * sizeof(double);
+    // OFG_DSL_SPLIT_BUDGET (bytes) overrides the split-form budget; 0 selects
+    // the fused form
+    double split_budget = (double)SPLIT_BUDGET;
+    if (const char *env = std::getenv("OFG_DSL_SPLIT_BUDGET")) split_budget = std::atof(env);
+    const bool split = (double)path_bytes * (double)paths * (double)n_mk <= split_budget &&
+                       split_budget > 0.0;
+    const int64_t tile_cap =
+        split ? 1024
+              : std::max<int64_t>(1, std::min<int64_t>(1024, (int64_t)(SNAP_BUDGET / tile_bytes) /
+                                                                 (int64_t)n_mk));
     std::vector<ofg::LsmSeedJob> jobs;
     std::vector<uint64_t> polys;
     std::map<uint64_t, std::vector<ofg_jump::Poly>> poly_cache;
@@ -205,6 +216,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         d.paths = paths;
         d.seed = eng->seed;
         d.df_off = (int32_t)dfs.size();
+        d.snap_off = split ? (int64_t)s * n_snaps * na * paths : 0;
         for (const ofg::DslObs &o : obs) dfs.push_back(std::exp(-m.rate * o.date));
         for (int i = 0; i < na; ++i) {   // AssetStepper::from_asset (engine.rs:51-93)
             const ofg_dsl_asset &a = m.assets[i];
@@ -262,7 +274,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         up(b.df, dfs.data(), dfs.size() * sizeof(double), "H2D dsl discount factors");
         up(b.step_snap, step_snap.data(), step_snap.size() * sizeof(int32_t), "H2D dsl snapshot map");
         up(b.state, state_init.data(), state_init.size() * sizeof(double), "H2D dsl state");
-        b.snaps.ensure(tiles.size() * tile_bytes);
+        b.snaps.ensure(split ? path_bytes * (size_t)paths * n_mk : tiles.size() * tile_bytes);
         b.partials.ensure(tiles.size() * sizeof(ofg::DslPartial));
         b.out.ensure(n_mk * sizeof(ofg::DslOut));
         ofg::DslArgs a{};
@@ -298,6 +310,8 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         a.n_tiles = (int32_t)tiles.size();
         a.n_scen = (int32_t)n_mk;
         a.rng_mode = rng;
+        a.n_assets = na;
+        a.split = split ? 1 : 0;
         if (steps > INT32_MAX) fail(OFG_INVALID_INPUT, "num_steps exceeds the device limit");
         hip_check(hipEventRecord(ctx->ev0, st), "event");
         if (rng == OFG_RNG_REFERENCE)

@@ -9,6 +9,7 @@
 #include "ofg_lsm.hpp"
 #include "ofg_math.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -130,7 +131,8 @@ inline void apply_poly(const Poly &r, uint64_t s[4]) {
 
 // Seed job of the seed kernel for one engine stream: path p starts at draw
 // p * draws of the stream seeded by seed_from_u64(seed). A thread walks
-// `chunk` paths after one jump of g * chunk * draws draws, composed from the
+// `chunk` paths (64, fewer for long per-path streams) after one jump of
+// g * chunk * draws draws, composed from the
 // polynomials x^(chunk * draws * 2^k) (cached per distance, appended to
 // polys).
 inline ofg::LsmSeedJob make_seed_job(uint64_t seed, int64_t paths, int64_t draws,
@@ -146,7 +148,9 @@ inline ofg::LsmSeedJob make_seed_job(uint64_t seed, int64_t paths, int64_t draws
     j.base[3] = st.s3;
     j.paths = paths;
     j.draws = draws;
-    j.chunk = 64;
+    // paths per thread: long enough to amortise the jump, short enough that
+    // long per-path streams still give many threads
+    j.chunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(64, 8192 / std::max<int64_t>(1, draws)));
     const int64_t threads = (paths + j.chunk - 1) / j.chunk;
     int32_t bits = 0;
     while (bits < 63 && (int64_t(1) << bits) < threads) ++bits;

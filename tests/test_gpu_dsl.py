@@ -196,3 +196,17 @@ def test_many_paths_per_thread(gpu):
         parse_and_compile(P.CALL), mkt)
     p, se = D.price(P.CALL, ora, corr, rate, paths, steps, z)
     assert _close(r.price, p) and abs(r.stderr - se) <= 1e-8 * se
+
+
+def test_fused_form_matches_split_form(gpu, monkeypatch):
+    """The single-kernel form (used when all snapshots exceed the buffer
+    budget) prices as the two-kernel form."""
+    from openferric_amd import RngMode, parse_and_compile
+    src, spec, paths, steps = CASES[0]
+    mkt, *_ = _mk(spec)
+    prod = parse_and_compile(src)
+    split = _engine(paths, steps, RngMode.Reference, seed=3).price_multi_asset(prod, mkt)
+    monkeypatch.setenv("OFG_DSL_SPLIT_BUDGET", "0")
+    fused = _engine(paths, steps, RngMode.Reference, seed=3).price_multi_asset(prod, mkt)
+    assert abs(fused.price - split.price) <= 1e-12 * abs(split.price)
+    assert abs(fused.stderr - split.stderr) <= 1e-9 * split.stderr
