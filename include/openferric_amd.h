@@ -228,6 +228,16 @@ typedef struct ofg_basket {
     double rate;
     int64_t num_paths;
     uint64_t seed;            /* the reference's MC_SEED is 3148159 */
+    /* The contract: 0 BasketOption (above), 1 OutperformanceBasketOption
+     * (instruments/basket.rs:70-118: weights = lagger_weights, is_call =
+     * option_type, basket_type ignored), 2 QuantoBasketOption (:120-165:
+     * the basket above plus the FX fields; `rate` is ignored). */
+    int32_t product;
+    int32_t leader_index;     /* outperformance */
+    double fx_rate, fx_vol, domestic_rate, foreign_rate;   /* quanto */
+    const double *asset_fx_corr;                             /* quanto */
+    int32_t n_asset_fx_corr;
+    int32_t pad;
 } ofg_basket;
 
 /* BasketSensitivities (pricing/basket.rs:57-65) and the diagnostics of
@@ -416,7 +426,9 @@ int32_t ofg_autocall_price(ofg_ctx *ctx, const ofg_autocall *contracts, size_t n
 
 /* Basket options under the Gaussian copula on one exact terminal step:
  * replaces price_basket_mc and, with with_sens, basket_sensitivities
- * (src/pricing/basket.rs:99-160, 456-597; simulation :599-683), n baskets in
+ * (src/pricing/basket.rs:99-160, 456-597; simulation :599-683),
+ * price_outperformance_basket_mc (:288-366) and price_quanto_basket_mc
+ * (:368-445) (sensitivities only for BasketOption), n contracts in
  * one launch, every bump a scenario on the base price's draws. A correlation
  * matrix that is valid but not PSD is projected to the nearest correlation
  * matrix (Higham 2002 alternating projections, as

@@ -19,10 +19,12 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       mc_european_qmc, mc_european_qmc_with_seed, qmc_price_raw,
                       option_struct, price_raw, sample_payoffs)
 from .multi import (Autocallable, AutocallableSensitivities, BasketOption,
-                    BasketSensitivities, BasketType, PhoenixAutocallable, autocall_price_raw,
+                    BasketSensitivities, BasketType, OutperformanceBasketOption,
+                    PhoenixAutocallable, QuantoBasketOption, autocall_price_raw,
                     autocall_struct, autocallable_sensitivities, basket_price_raw,
                     basket_sensitivities, basket_struct, phoenix_autocallable_sensitivities,
-                    price_autocallable, price_basket_mc, price_phoenix_autocallable,
+                    price_autocallable, price_basket_mc, price_outperformance_basket_mc,
+                    price_phoenix_autocallable, price_quanto_basket_mc,
                     validate_or_repair_correlation_matrix)
 from .dsl import (AssetMarketData, CompiledProduct, CrossGreeks, DslMonteCarloEngine, DslProduct,
                   ExtendedGreeks, MultiAssetMarket, parse_and_compile)
@@ -43,7 +45,8 @@ __all__ = [
     "mc_european_qmc", "mc_european_qmc_with_seed", "qmc_price_raw",
     "Autocallable", "AutocallableSensitivities", "PhoenixAutocallable", "autocall_price_raw",
     "BasketOption", "BasketSensitivities", "BasketType", "basket_price_raw",
-    "basket_sensitivities", "basket_struct", "price_basket_mc",
+    "basket_sensitivities", "basket_struct", "price_basket_mc", "OutperformanceBasketOption",
+    "QuantoBasketOption", "price_outperformance_basket_mc", "price_quanto_basket_mc",
     "validate_or_repair_correlation_matrix",
     "AssetMarketData", "CompiledProduct", "CrossGreeks", "DslMonteCarloEngine", "DslProduct",
     "ExtendedGreeks", "MultiAssetMarket", "parse_and_compile",

@@ -142,6 +142,10 @@ class OfgBasket(C.Structure):
         ("spots", C.POINTER(C.c_double)), ("vols", C.POINTER(C.c_double)),
         ("dividends", C.POINTER(C.c_double)), ("corr", C.POINTER(C.c_double)),
         ("rate", C.c_double), ("num_paths", C.c_int64), ("seed", C.c_uint64),
+        ("product", C.c_int32), ("leader_index", C.c_int32), ("fx_rate", C.c_double),
+        ("fx_vol", C.c_double), ("domestic_rate", C.c_double), ("foreign_rate", C.c_double),
+        ("asset_fx_corr", C.POINTER(C.c_double)), ("n_asset_fx_corr", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 

@@ -79,8 +79,75 @@ void validate_inputs(const ofg_basket &b, const std::vector<double> &spots,
     validate_corr(corr, b.n_assets);
 }
 
-// terminal_payoff (basket.rs:759-784) of the spots themselves.
+// validate_outperformance_inputs (basket.rs:891-915) with
+// OutperformanceBasketOption::validate (instruments/basket.rs:82-110).
+void validate_outperformance(const ofg_basket &b, const std::vector<double> &spots,
+                             const std::vector<double> &vols, const Mat &corr) {
+    if (b.n_weights <= 0) fail(OFG_INVALID_INPUT, "outperformance lagger weights cannot be empty");
+    if (b.strike < 0.0) fail(OFG_INVALID_INPUT, "outperformance strike must be >= 0");
+    if (b.maturity < 0.0) fail(OFG_INVALID_INPUT, "outperformance maturity must be >= 0");
+    for (int32_t k = 0; k < b.n_weights; ++k)
+        if (!std::isfinite(b.weights[k]))
+            fail(OFG_INVALID_INPUT, "outperformance lagger weights must be finite");
+    if (b.num_paths <= 0) fail(OFG_INVALID_INPUT, "basket n_paths must be > 0");
+    if (b.n_assets <= 0) fail(OFG_INVALID_INPUT, "basket spots cannot be empty");
+    for (double s : spots)
+        if (s <= 0.0 || !std::isfinite(s)) fail(OFG_INVALID_INPUT, "basket spots must be finite and > 0");
+    for (double v : vols)
+        if (v < 0.0 || !std::isfinite(v)) fail(OFG_INVALID_INPUT, "basket vols must be finite and >= 0");
+    for (int32_t i = 0; i < b.n_assets; ++i)
+        if (!std::isfinite(b.dividends[i])) fail(OFG_INVALID_INPUT, "basket dividends must be finite");
+    validate_corr(corr, b.n_assets);
+    if (b.n_weights != b.n_assets)
+        fail(OFG_INVALID_INPUT, "outperformance lagger weights length must match assets");
+    if (b.leader_index < 0 || b.leader_index >= b.n_assets)
+        fail(OFG_INVALID_INPUT, "outperformance leader_index out of bounds");
+}
+
+// validate_quanto_inputs (basket.rs:917-943) with QuantoBasketOption::validate
+// (instruments/basket.rs:131-166).
+void validate_quanto(const ofg_basket &b, const std::vector<double> &spots,
+                     const std::vector<double> &vols, const Mat &corr) {
+    if (b.strike < 0.0) fail(OFG_INVALID_INPUT, "basket strike must be >= 0");
+    if (b.maturity < 0.0) fail(OFG_INVALID_INPUT, "basket maturity must be >= 0");
+    for (int32_t k = 0; k < b.n_weights; ++k)
+        if (!std::isfinite(b.weights[k])) fail(OFG_INVALID_INPUT, "basket weights must be finite");
+    if (b.basket_type == OFG_BASKET_AVERAGE && b.n_weights <= 0)
+        fail(OFG_INVALID_INPUT, "average basket requires non-empty weights");
+    if (b.fx_rate <= 0.0 || !std::isfinite(b.fx_rate))
+        fail(OFG_INVALID_INPUT, "quanto fx_rate must be finite and > 0");
+    if (b.fx_vol < 0.0 || !std::isfinite(b.fx_vol))
+        fail(OFG_INVALID_INPUT, "quanto fx_vol must be finite and >= 0");
+    for (int32_t k = 0; k < b.n_asset_fx_corr; ++k) {
+        const double rho = b.asset_fx_corr[k];
+        if (!std::isfinite(rho) || !(rho >= -1.0 && rho <= 1.0))
+            fail(OFG_INVALID_INPUT, "quanto asset_fx_corr entries must be finite and in [-1, 1]");
+    }
+    if (b.num_paths <= 0) fail(OFG_INVALID_INPUT, "basket n_paths must be > 0");
+    if (b.n_assets <= 0) fail(OFG_INVALID_INPUT, "basket spots cannot be empty");
+    for (double s : spots)
+        if (s <= 0.0 || !std::isfinite(s)) fail(OFG_INVALID_INPUT, "basket spots must be finite and > 0");
+    for (double v : vols)
+        if (v < 0.0 || !std::isfinite(v)) fail(OFG_INVALID_INPUT, "basket vols must be finite and >= 0");
+    for (int32_t i = 0; i < b.n_assets; ++i)
+        if (!std::isfinite(b.dividends[i])) fail(OFG_INVALID_INPUT, "basket dividends must be finite");
+    validate_corr(corr, b.n_assets);
+    if (b.n_asset_fx_corr != b.n_assets)
+        fail(OFG_INVALID_INPUT, "quanto asset_fx_corr length must match assets");
+    if (b.basket_type == OFG_BASKET_AVERAGE && b.n_weights != b.n_assets)
+        fail(OFG_INVALID_INPUT, "quanto average basket weights length must match assets");
+}
+
+// terminal_payoff (basket.rs:759-784) of the spots themselves, times the
+// quanto FX rate; outperformance_payoff (:786-800).
 double payoff_at_spots(const ofg_basket &b, const std::vector<double> &spots) {
+    if (b.product == 1) {
+        double lag = -0.0;
+        for (int32_t i = 0; i < b.n_assets; ++i) lag += b.weights[i] * spots[(size_t)i];
+        const double ratio = spots[(size_t)b.leader_index] / std::fmax(lag, 1.0e-12);
+        return b.is_call ? std::fmax(ratio - b.strike, 0.0) : std::fmax(b.strike - ratio, 0.0);
+    }
+    const double scale = b.product == 2 ? b.fx_rate : 1.0;
     double u;
     if (b.basket_type == OFG_BASKET_AVERAGE) {
         u = -0.0;
@@ -92,7 +159,7 @@ double payoff_at_spots(const ofg_basket &b, const std::vector<double> &spots) {
         u = INFINITY;
         for (int32_t i = 0; i < b.n_assets; ++i) u = std::fmin(u, spots[(size_t)i] / spots[(size_t)i]);
     }
-    return b.is_call ? std::fmax(u - b.strike, 0.0) : std::fmax(b.strike - u, 0.0);
+    return scale * (b.is_call ? std::fmax(u - b.strike, 0.0) : std::fmax(b.strike - u, 0.0));
 }
 
 // One scenario: a device descriptor or a host value (zero maturity).
@@ -103,20 +170,35 @@ struct Scenario {
     ofg::BkDev d{};
 };
 
-// simulate_basket_once_matrix's preparation (basket.rs:610-641); throws.
+// simulate_basket_once_matrix's preparation (basket.rs:610-641) and those
+// of price_outperformance_basket_mc (:299-326) and price_quanto_basket_mc
+// (:376-403), which repair and factor before the zero-maturity shortcut;
+// throws.
 Scenario prepare(const ofg_basket &b, const std::vector<double> &spots,
                  const std::vector<double> &vols, const Mat &corr) {
-    validate_inputs(b, spots, vols, corr);
     Scenario s;
-    if (b.maturity <= 0.0) {
-        s.host_price = payoff_at_spots(b, spots);
-        return s;
-    }
     const int n = b.n_assets;
-    if (n > ofg::BK_MAX) fail(OFG_INVALID_INPUT, "at most 16 assets per basket");
-    const Mat c = validate_or_repair(corr, n, s.repaired);
     Mat l;
-    if (!cholesky_psd(c, n, l)) fail(OFG_INVALID_INPUT, "basket correlation matrix is not PSD");
+    if (b.product == 0) {
+        validate_inputs(b, spots, vols, corr);
+        if (b.maturity <= 0.0) {
+            s.host_price = payoff_at_spots(b, spots);
+            return s;
+        }
+        if (n > ofg::BK_MAX) fail(OFG_INVALID_INPUT, "at most 16 assets per basket");
+        const Mat c = validate_or_repair(corr, n, s.repaired);
+        if (!cholesky_psd(c, n, l)) fail(OFG_INVALID_INPUT, "basket correlation matrix is not PSD");
+    } else {
+        if (b.product == 1) validate_outperformance(b, spots, vols, corr);
+        else validate_quanto(b, spots, vols, corr);
+        const Mat c = validate_or_repair(corr, n, s.repaired);
+        if (!cholesky_psd(c, n, l)) fail(OFG_INVALID_INPUT, "correlation matrix is not PSD");
+        if (b.maturity <= 0.0) {
+            s.host_price = payoff_at_spots(b, spots);
+            return s;
+        }
+        if (n > ofg::BK_MAX) fail(OFG_INVALID_INPUT, "at most 16 assets per basket");
+    }
     ofg::BkDev &d = s.d;
     d.n = n;
     d.type = b.basket_type;
@@ -124,16 +206,25 @@ Scenario prepare(const ofg_basket &b, const std::vector<double> &spots,
     d.paths = b.num_paths;
     d.seed = b.seed;
     d.strike = b.strike;
+    d.payoff = b.product == 1 ? ofg::BK_OUTPERFORMANCE : ofg::BK_TERMINAL;
+    d.leader = b.product == 1 ? b.leader_index : -1;
+    d.scale = b.product == 2 ? b.fx_rate : 1.0;
     const double t = b.maturity, sqrt_t = std::sqrt(t);
-    d.discount = std::exp(-b.rate * t);
+    // discount exp(-r T); quanto: exp(-r_d T) (basket.rs:436)
+    d.discount = std::exp(-(b.product == 2 ? b.domestic_rate : b.rate) * t);
     for (int i = 0; i < n; ++i) {
         const double v = vols[(size_t)i];
         d.s0[i] = spots[(size_t)i];
-        d.drift[i] = (b.rate - b.dividends[i] - 0.5 * v * v) * t;
+        if (b.product == 2) {   // quanto drift (basket.rs:408-417)
+            const double quanto_adj = b.asset_fx_corr[i] * v * b.fx_vol;
+            d.drift[i] = (b.foreign_rate - b.dividends[i] - quanto_adj - 0.5 * v * v) * t;
+        } else {
+            d.drift[i] = (b.rate - b.dividends[i] - 0.5 * v * v) * t;
+        }
         d.vol_t[i] = v * sqrt_t;
         d.drift2[i] = d.drift[i] * OFG_INV_LN2_256;
         d.vol_t2[i] = d.vol_t[i] * OFG_INV_LN2_256;
-        d.w[i] = b.basket_type == OFG_BASKET_AVERAGE ? b.weights[i] : 0.0;
+        d.w[i] = (b.basket_type == OFG_BASKET_AVERAGE || b.product == 1) ? b.weights[i] : 0.0;
         for (int j = 0; j <= i; ++j) d.chol[i * (i + 1) / 2 + j] = l[(size_t)i * n + j];
     }
     s.on_device = true;
@@ -206,6 +297,9 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
                 fail(OFG_INVALID_INPUT, "basket spots cannot be empty");
             if (b.n_weights < 0 || (b.n_weights > 0 && !b.weights))
                 fail(OFG_INVALID_INPUT, "basket weights must be finite");
+            if (b.product < 0 || b.product > 2) fail(OFG_INVALID_INPUT, "unknown basket product");
+            if (b.product == 2 && b.n_asset_fx_corr > 0 && !b.asset_fx_corr)
+                fail(OFG_INVALID_INPUT, "quanto asset_fx_corr length must match assets");
             const int na = b.n_assets;
             std::vector<double> spots(b.spots, b.spots + na), vols(b.vols, b.vols + na);
             const Mat corr(b.corr, b.corr + (size_t)na * na);
@@ -233,7 +327,7 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
             pl.base = push(base);
             pl.valid = true;
             pl.rho = average_off_diagonal(corr, na);
-            if (with_sens) {
+            if (with_sens && b.product == 0) {
                 // basket_sensitivities (basket.rs:456-539)
                 try {
                     for (int k = 0; k < na; ++k) {

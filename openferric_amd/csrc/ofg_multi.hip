@@ -196,7 +196,8 @@ __global__ __launch_bounds__(MA_BLOCK) void autocall_close_kernel(AcArgs a) {
 // Basket options: one exact terminal step per asset
 // ------------------------------------------------------------------------
 
-// One path's payoff (terminal_payoff, basket.rs:759-784). The correlated
+// One path's payoff (terminal_payoff, basket.rs:759-784; outperformance_payoff,
+// :786-800; times the quanto FX rate, :429). The correlated
 // normal of asset i accumulates L[i][j] z_j column by column as each z_j is
 // drawn, the same terms in the same order as correlate_normals' running sum.
 template <int RNG>
@@ -240,7 +241,10 @@ __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const d
                 if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
         }
     }
-    double u = o.type == BK_AVERAGE ? -0.0 : (o.type == BK_BEST_OF ? -INFINITY : INFINITY);
+    const bool outperf = o.payoff == BK_OUTPERFORMANCE;
+    double u = (o.type == BK_AVERAGE || outperf) ? -0.0
+               : (o.type == BK_BEST_OF ? -INFINITY : INFINITY);
+    double leader = 0.0;
 #pragma unroll
     for (int i = 0; i < BK_MAX; ++i) {
         if (i < n) {
@@ -253,12 +257,15 @@ __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const d
                 g = exp_fp64(o.drift[i] + o.vol_t[i] * c[i]);
             }
             const double st = o.s0[i] * g;
-            if (o.type == BK_AVERAGE) u += o.w[i] * st;
+            if (i == o.leader) leader = st;
+            if (o.type == BK_AVERAGE || outperf) u += o.w[i] * st;
             else if (o.type == BK_BEST_OF) u = fmax(u, st / o.s0[i]);
             else u = fmin(u, st / o.s0[i]);
         }
     }
-    return o.is_call ? fmax(u - o.strike, 0.0) : fmax(o.strike - u, 0.0);
+    if (outperf) u = leader / fmax(u, 1.0e-12);   // outperformance_payoff (basket.rs:786-800)
+    const double x = o.is_call ? fmax(u - o.strike, 0.0) : fmax(o.strike - u, 0.0);
+    return o.scale * x;
 }
 
 template <int RNG>

@@ -76,6 +76,8 @@ hipError_t launch_autocall(const AcArgs &a, hipStream_t stream);
 constexpr int BK_MAX = 16;
 enum BasketType : int32_t { BK_AVERAGE = 0, BK_BEST_OF = 1, BK_WORST_OF = 2 };
 
+enum BasketPayoff : int32_t { BK_TERMINAL = 0, BK_OUTPERFORMANCE = 1 };
+
 struct alignas(16) BkDev {
     int32_t n;             // assets
     int32_t type;          // BasketType
@@ -83,12 +85,15 @@ struct alignas(16) BkDev {
     int32_t tile_first;
     int32_t n_tiles;
     int32_t ppt;
+    int32_t payoff;        // BasketPayoff
+    int32_t leader;        // outperformance: leader index
+    double scale;          // payoff multiplier (quanto: the fixed FX rate)
     int64_t paths;
     int64_t normals_offset;   // external normals, [asset][path]
     int64_t seed_offset;      // reference-stream start states, [path][word]
     uint64_t seed;            // fast draws: stream_seed(seed, path)
     double strike, discount;
-    double s0[BK_MAX], drift[BK_MAX], vol_t[BK_MAX], w[BK_MAX];
+    double s0[BK_MAX], drift[BK_MAX], vol_t[BK_MAX], w[BK_MAX];   // w: weights or lagger weights
     double drift2[BK_MAX], vol_t2[BK_MAX];     // fast mode: scaled by 256/ln2
     double chol[BK_MAX * (BK_MAX + 1) / 2];   // lower factor, row i at i(i+1)/2
 };

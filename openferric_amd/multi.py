@@ -5,8 +5,10 @@ Mirrors src/instruments/autocallable.rs (Autocallable, PhoenixAutocallable)
 and src/pricing/autocallable.rs (price_autocallable,
 autocallable_sensitivities, price_phoenix_autocallable,
 phoenix_autocallable_sensitivities) over the C ABI's ofg_autocall_price, and
-src/instruments/basket.rs (BasketOption) with src/pricing/basket.rs
-(price_basket_mc, basket_sensitivities) over ofg_basket_price.
+src/instruments/basket.rs (BasketOption, OutperformanceBasketOption,
+QuantoBasketOption) with src/pricing/basket.rs (price_basket_mc,
+basket_sensitivities, price_outperformance_basket_mc,
+price_quanto_basket_mc) over ofg_basket_price.
 Draws: the reference walks one xoshiro256++ stream whose position depends on
 when earlier paths were called, which no parallel simulator can reproduce;
 `rng_mode` is Fast (per-path streams from the reference's seed 94210) or
@@ -321,6 +323,66 @@ def basket_sensitivities(basket: BasketOption, spots, vols, corr_matrix, r, divi
         raise InvalidInput(_lib.last_error() or "basket sensitivities failed")
     sv = sens[0]
     return BasketSensitivities([sv.delta[k] for k in range(sv.n_delta)], sv.vega, sv.cega)
+
+
+@dataclass
+class OutperformanceBasketOption:
+    """instruments/basket.rs:70-118"""
+    leader_index: int
+    lagger_weights: Sequence[float]
+    strike: float
+    maturity: float
+    is_call: bool = True   # option_type
+
+    def instrument_type(self) -> str:
+        return "OutperformanceBasketOption"
+
+
+@dataclass
+class QuantoBasketOption:
+    """instruments/basket.rs:120-165"""
+    basket: BasketOption
+    fx_rate: float
+    fx_vol: float
+    asset_fx_corr: Sequence[float]
+    domestic_rate: float
+    foreign_rate: float
+
+    def instrument_type(self) -> str:
+        return "QuantoBasketOption"
+
+
+def _variant_result(r) -> PricingResult:
+    # the reference's outperformance / quanto results carry no diagnostics
+    return PricingResult(price=r.price, stderr=r.stderr_ if r.has_stderr else None,
+                         diagnostics=Diagnostics(), kernel_ms=r.kernel_ms)
+
+
+def price_outperformance_basket_mc(option: OutperformanceBasketOption, spots, vols, corr_matrix,
+                                   r, dividends, n_paths, rng_mode=RngMode.Reference,
+                                   normals=None) -> PricingResult:
+    """price_outperformance_basket_mc (pricing/basket.rs:288-366)."""
+    keep = _Keep()
+    b = basket_struct(BasketOption(list(option.lagger_weights), option.strike, option.maturity,
+                                   option.is_call, BasketType.Average), spots, vols, corr_matrix,
+                      r, dividends, n_paths, keep)
+    b.product, b.leader_index = 1, int(option.leader_index)
+    res, _ = basket_price_raw([b], rng_mode, None if normals is None else [normals])
+    return _variant_result(res[0])
+
+
+def price_quanto_basket_mc(option: QuantoBasketOption, spots, vols, corr_matrix, dividends,
+                           n_paths, rng_mode=RngMode.Reference, normals=None) -> PricingResult:
+    """price_quanto_basket_mc (pricing/basket.rs:368-445)."""
+    keep = _Keep()
+    b = basket_struct(option.basket, spots, vols, corr_matrix, 0.0, dividends, n_paths, keep)
+    b.product = 2
+    b.fx_rate, b.fx_vol = option.fx_rate, option.fx_vol
+    b.domestic_rate, b.foreign_rate = option.domestic_rate, option.foreign_rate
+    b.n_asset_fx_corr = len(option.asset_fx_corr)
+    b.asset_fx_corr = keep.arr(list(option.asset_fx_corr)) if len(option.asset_fx_corr) else None
+    res, _ = basket_price_raw([b], rng_mode, None if normals is None else [normals])
+    return _variant_result(res[0])
 
 
 def validate_or_repair_correlation_matrix(corr_matrix):

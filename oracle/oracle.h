@@ -241,6 +241,16 @@ typedef struct {
 } or_basket;
 int or_basket_price(const or_basket *b, const double *normals, double *price, double *stderr_out);
 
+/* price_outperformance_basket_mc (basket.rs:288-366): weights = lagger
+ * weights, ratio S_leader / max(sum w S, 1e-12). price_quanto_basket_mc
+ * (:368-445): quanto drift r_f - q - rho_fx vol fx_vol, payoff times fx_rate,
+ * discount exp(-r_d T). PSD matrices only; normals as or_basket_price. */
+int or_outperformance_price(const or_basket *b, int32_t leader, const double *normals,
+                            double *price, double *stderr_out);
+int or_quanto_price(const or_basket *b, double fx_rate, double fx_vol, const double *asset_fx_corr,
+                    double domestic_rate, double foreign_rate, const double *normals,
+                    double *price, double *stderr_out);
+
 /* ---- quasi-Monte Carlo (oracle_qmc.c) -------------------------------- */
 /* build_direction_numbers / scrambler of SobolSequence (math/sobol.rs) */
 void or_sobol_directions(uint64_t dim, uint64_t seed, uint64_t v[64]);

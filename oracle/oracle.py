@@ -161,6 +161,11 @@ def lib():
         L.or_bump_corr_matrix.argtypes = [pd, i32, d, pd]; L.or_bump_corr_matrix.restype = None
         L.or_basket_price.argtypes = [C.POINTER(OrBasket), pd, pd, pd]
         L.or_basket_price.restype = C.c_int
+        L.or_outperformance_price.argtypes = [C.POINTER(OrBasket), C.c_int32, pd, pd, pd]
+        L.or_outperformance_price.restype = C.c_int
+        L.or_quanto_price.argtypes = [C.POINTER(OrBasket), C.c_double, C.c_double, pd, C.c_double,
+                                      C.c_double, pd, pd, pd]
+        L.or_quanto_price.restype = C.c_int
         L.or_sobol_directions.argtypes = [u64, u64, pu64]; L.or_sobol_directions.restype = None
         L.or_sobol_scrambler.argtypes = [u64, u64]; L.or_sobol_scrambler.restype = u64
         L.or_sobol_points.argtypes = [i64, u64, i64, pu64, pd]; L.or_sobol_points.restype = None
@@ -456,4 +461,26 @@ def basket_price(b, normals=None):
     rc = lib().or_basket_price(C.byref(b), _ptr(z), C.byref(p), C.byref(se))
     if rc != 0:
         raise ValueError(f"or_basket_price failed rc={rc}")
+    return p.value, se.value
+
+
+def outperformance_price(b, leader, normals=None):
+    """price_outperformance_basket_mc: (price, stderr); b.weights = lagger weights."""
+    z = np.ascontiguousarray(normals, dtype=np.float64).ravel() if normals is not None else None
+    p, se = C.c_double(), C.c_double()
+    rc = lib().or_outperformance_price(C.byref(b), int(leader), _ptr(z), C.byref(p), C.byref(se))
+    if rc != 0:
+        raise ValueError(f"or_outperformance_price failed rc={rc}")
+    return p.value, se.value
+
+
+def quanto_price(b, fx_rate, fx_vol, asset_fx_corr, domestic_rate, foreign_rate, normals=None):
+    """price_quanto_basket_mc: (price, stderr)."""
+    z = np.ascontiguousarray(normals, dtype=np.float64).ravel() if normals is not None else None
+    rho = np.ascontiguousarray(asset_fx_corr, dtype=np.float64)
+    p, se = C.c_double(), C.c_double()
+    rc = lib().or_quanto_price(C.byref(b), fx_rate, fx_vol, _ptr(rho), domestic_rate,
+                               foreign_rate, _ptr(z), C.byref(p), C.byref(se))
+    if rc != 0:
+        raise ValueError(f"or_quanto_price failed rc={rc}")
     return p.value, se.value

@@ -29,6 +29,9 @@
  * e^{-rT} N E[min_i S_i(T)/S_i(0)] / K_ki with E[S(T)/S(0)] = e^{(r-q)T} for
  * one asset and Margrabe's exchange-option formula for two correlated
  * assets; the reference's own monotonicity and single-underlying tests.
+ * The outperformance and quanto variants (basket.rs:288-445) share the
+ * loop: pinned by the lognormal ratio's closed forms (E[S1/S2] and the
+ * Black formula on it) and the quanto-adjusted Black-Scholes call.
  * Baskets (tests/test_oracle_basket.py): zero volatility (forward basket
  * intrinsic), one asset = Black-Scholes, a zero weight = the one-asset price
  * on the same first column of normals, best-of / worst-of of two = Margrabe,
@@ -273,4 +276,88 @@ int or_basket_price(const or_basket *b, const double *normals, double *price, do
     *stderr_out = discount * sqrt(var / (double)b->num_paths);
     free(l);
     return 0;
+}
+
+/* The common loop of the two variants: terminal spots from `drift`, the
+ * payoff of each path by `kind` (1 outperformance, 2 quanto), Welford. */
+static int basket_variant(const or_basket *b, int kind, int32_t leader, double fx_rate,
+                          const double *drift, double discount, const double *normals,
+                          double *price, double *stderr_out) {
+    const int32_t n = b->n_assets;
+    const double t = b->maturity, sqrt_t = sqrt(t);
+    double *l = (double *)malloc(sizeof(double) * (size_t)(n * n));
+    if (!or_cholesky_lower(b->corr, n, l)) {
+        free(l);
+        return 2;
+    }
+    if (t <= 0.0) {   /* after the repair and factorisation, as the reference */
+        free(l);
+        if (kind == 1) {
+            double lag = -0.0;
+            for (int32_t i = 0; i < n; ++i) lag += b->weights[i] * b->spots[i];
+            const double ratio = b->spots[leader] / fmax(lag, 1.0e-12);
+            *price = b->is_call ? fmax(ratio - b->strike, 0.0) : fmax(b->strike - ratio, 0.0);
+        } else {
+            *price = fx_rate * basket_payoff(b, b->spots, b->spots);
+        }
+        *stderr_out = 0.0;
+        return 0;
+    }
+    double vol_t[64], indep[64], term[64];
+    for (int32_t i = 0; i < n; ++i) vol_t[i] = b->vols[i] * sqrt_t;
+    uint64_t st[4];
+    or_xoshiro_seed(b->seed, st);
+    double mean = 0.0, m2 = 0.0;
+    int64_t cnt = 0;
+    for (int64_t p = 0; p < b->num_paths; ++p) {
+        for (int32_t i = 0; i < n; ++i)
+            indep[i] = normals ? normals[p * n + i]
+                               : or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(st)));
+        for (int32_t i = 0; i < n; ++i) {
+            double sum = 0.0;
+            for (int32_t j = 0; j <= i; ++j) sum += l[i * n + j] * indep[j];
+            term[i] = b->spots[i] * exp(drift[i] + vol_t[i] * sum);
+        }
+        double x;
+        if (kind == 1) {   /* outperformance_payoff (basket.rs:786-800) */
+            double lag = -0.0;
+            for (int32_t i = 0; i < n; ++i) lag += b->weights[i] * term[i];
+            const double ratio = term[leader] / fmax(lag, 1.0e-12);
+            x = b->is_call ? fmax(ratio - b->strike, 0.0) : fmax(b->strike - ratio, 0.0);
+        } else {
+            x = fx_rate * basket_payoff(b, b->spots, term);
+        }
+        ++cnt;
+        const double delta = x - mean;
+        mean += delta / (double)cnt;
+        m2 += delta * (x - mean);
+    }
+    const double var = cnt > 1 ? m2 / ((double)cnt - 1.0) : 0.0;
+    *price = discount * mean;
+    *stderr_out = discount * sqrt(var / (double)b->num_paths);
+    free(l);
+    return 0;
+}
+
+int or_outperformance_price(const or_basket *b, int32_t leader, const double *normals,
+                            double *price, double *stderr_out) {
+    double drift[64];
+    const double t = b->maturity;
+    for (int32_t i = 0; i < b->n_assets; ++i)
+        drift[i] = (b->rate - b->dividends[i] - 0.5 * b->vols[i] * b->vols[i]) * t;
+    return basket_variant(b, 1, leader, 1.0, drift, exp(-b->rate * t), normals, price, stderr_out);
+}
+
+int or_quanto_price(const or_basket *b, double fx_rate, double fx_vol, const double *asset_fx_corr,
+                    double domestic_rate, double foreign_rate, const double *normals,
+                    double *price, double *stderr_out) {
+    double drift[64];
+    const double t = b->maturity;
+    for (int32_t i = 0; i < b->n_assets; ++i) {
+        const double v = b->vols[i];
+        const double quanto_adj = asset_fx_corr[i] * v * fx_vol;
+        drift[i] = (foreign_rate - b->dividends[i] - quanto_adj - 0.5 * v * v) * t;
+    }
+    return basket_variant(b, 2, 0, fx_rate, drift, exp(-domestic_rate * t), normals, price,
+                          stderr_out);
 }

@@ -191,3 +191,34 @@ def test_zero_maturity_invalid_inputs_and_fast_draws(gpu):
                            [0.01], 1 << 20, rng_mode=RngMode.Fast)
     want = bs_price(False, 100.0, 100.0, 0.02, 0.01, 0.3, 1.0)
     assert abs(fast.price - want) < 4 * fast.stderr, (fast.price, want, fast.stderr)
+
+
+def test_outperformance_and_quanto_match_oracle(oracle, gpu):
+    from openferric_amd import (BasketOption, BasketType, InvalidInput, OutperformanceBasketOption,
+                                QuantoBasketOption, RngMode, _lib, price_outperformance_basket_mc,
+                                price_quanto_basket_mc)
+    spots, vols, divs = [100.0, 95.0, 105.0], [0.2, 0.22, 0.21], [0.0, 0.01, 0.02]
+    corr = [[1.0, 0.4, 0.35], [0.4, 1.0, 0.3], [0.35, 0.3, 1.0]]
+    out = OutperformanceBasketOption(0, [0.0, 0.5, 0.5], 1.0, 1.0, True)
+    paths = 70_001
+    for mode, z in ((RngMode.Reference, None),
+                    (RngMode.External, np.random.default_rng(9).standard_normal((paths, 3)))):
+        r = price_outperformance_basket_mc(out, spots, vols, corr, 0.02, divs, paths, mode, z)
+        b = oracle.make_basket(spots, vols, divs, corr, [0.0, 0.5, 0.5], AVG, True, 1.0, 1.0, 0.02,
+                               paths)
+        p, se = oracle.outperformance_price(b, 0, z)
+        assert abs(r.price - p) <= 1e-10 * p and abs(r.stderr - se) <= 1e-8 * se, (r.price, p)
+        assert r.diagnostics.get("num_paths") is None
+        q = QuantoBasketOption(BasketOption([], 1.0, 1.5, False, BasketType.WorstOf), 1.2, 0.12,
+                               [0.3, -0.25, 0.1], 0.03, 0.02)
+        r = price_quanto_basket_mc(q, spots, vols, corr, divs, paths, mode, z)
+        b = oracle.make_basket(spots, vols, divs, corr, None, WORST, False, 1.0, 1.5, 0.0, paths)
+        p, se = oracle.quanto_price(b, 1.2, 0.12, [0.3, -0.25, 0.1], 0.03, 0.02, z)
+        assert abs(r.price - p) <= 1e-10 * p and abs(r.stderr - se) <= 1e-8 * se, (r.price, p)
+    bad = OutperformanceBasketOption(3, [0.0, 0.5, 0.5], 1.0, 1.0, True)
+    r = price_outperformance_basket_mc(bad, spots, vols, corr, 0.02, divs, 100)
+    assert math.isnan(r.price) and "leader_index out of bounds" in _lib.last_error()
+    q = QuantoBasketOption(BasketOption([], 1.0, 1.5, False, BasketType.WorstOf), 1.2, 0.12,
+                           [0.3], 0.03, 0.02)
+    r = price_quanto_basket_mc(q, spots, vols, corr, divs, 100)
+    assert math.isnan(r.price) and "asset_fx_corr length" in _lib.last_error()

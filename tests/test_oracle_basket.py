@@ -171,3 +171,53 @@ def test_reference_repair_case_and_psd_passthrough():
         validate_or_repair_correlation_matrix([[1.0, 0.3], [0.2, 1.0]])
     with pytest.raises(InvalidInput, match="diagonal"):
         validate_or_repair_correlation_matrix([[1.0, 0.3], [0.3, 0.9]])
+
+
+# ---- outperformance and quanto variants (basket.rs:288-445) ----------------
+
+def _black(f, k, sig, t):
+    d1 = (math.log(f / k) + 0.5 * sig * sig * t) / (sig * math.sqrt(t))
+    from scipy.stats import norm
+    return f * norm.cdf(d1) - k * norm.cdf(d1 - sig * math.sqrt(t))
+
+
+@pytest.mark.parametrize("strike", [0.0, 1.2])
+def test_outperformance_ratio_matches_lognormal_closed_forms(oracle, strike):
+    s1, s2, q1, q2, r, rho, t = 0.3, 0.2, 0.01, 0.03, 0.02, 0.4, 1.0
+    b = oracle.make_basket([100.0, 50.0], [s1, s2], [q1, q2], [[1, rho], [rho, 1]], [0.0, 1.0],
+                           AVG, True, strike, t, r, 400_000)
+    p, se = oracle.outperformance_price(b, 0)
+    f = 2.0 * math.exp((q2 - q1 + s2 * s2 - rho * s1 * s2) * t)   # E[S1(T)/S2(T)]
+    sig = math.sqrt(s1 * s1 + s2 * s2 - 2 * rho * s1 * s2)
+    want = math.exp(-r * t) * (f if strike == 0.0 else _black(f, strike, sig, t))
+    assert abs(p - want) < 4 * se, (p, want, se)
+
+
+def test_quanto_call_matches_adjusted_black_scholes(oracle):
+    s, v, q, rf, rd, fx, fxv, rho, k, t = 100.0, 0.25, 0.01, 0.02, 0.04, 1.3, 0.12, -0.35, 95.0, 1.5
+    b = oracle.make_basket([s], [v], [q], [[1.0]], [1.0], AVG, True, k, t, 0.0, 400_000)
+    p, se = oracle.quanto_price(b, fx, fxv, [rho], rd, rf)
+    f = s * math.exp((rf - q - rho * v * fxv) * t)
+    want = fx * math.exp(-rd * t) * _black(f, k, v, t)
+    assert abs(p - want) < 4 * se, (p, want, se)
+
+
+def test_variants_zero_maturity_and_reference_case(oracle):
+    b = oracle.make_basket([100.0, 95.0, 105.0], [0.2, 0.22, 0.21], [0.0] * 3,
+                           [[1.0, 0.4, 0.35], [0.4, 1.0, 0.3], [0.35, 0.3, 1.0]], [0.0, 0.5, 0.5],
+                           AVG, True, 1.0, 0.0, 0.02, 100)
+    assert oracle.outperformance_price(b, 0) == (max(100.0 / 100.0 - 1.0, 0.0), 0.0)
+    b = oracle.make_basket([100.0, 98.0], [0.2, 0.22], [0.01, 0.01], [[1.0, 0.4], [0.4, 1.0]],
+                           [0.6, 0.4], AVG, True, 90.0, 0.0, 0.0, 100)
+    assert oracle.quanto_price(b, 1.2, 0.12, [0.3, 0.25], 0.03, 0.02) == \
+        (1.2 * (0.6 * 100 + 0.4 * 98 - 90.0), 0.0)
+    # quanto_and_outperformance_are_priced (basket.rs:1298-1349)
+    b = oracle.make_basket([100.0, 95.0, 105.0], [0.2, 0.22, 0.21], [0.0] * 3,
+                           [[1.0, 0.4, 0.35], [0.4, 1.0, 0.3], [0.35, 0.3, 1.0]], [0.0, 0.5, 0.5],
+                           AVG, True, 1.0, 1.0, 0.02, 120_000)
+    p, _ = oracle.outperformance_price(b, 0)
+    assert math.isfinite(p) and p >= 0.0
+    b = oracle.make_basket([100.0, 98.0], [0.2, 0.22], [0.01, 0.01], [[1.0, 0.4], [0.4, 1.0]],
+                           [0.6, 0.4], AVG, True, 100.0, 1.0, 0.0, 120_000)
+    p, _ = oracle.quanto_price(b, 1.2, 0.12, [0.3, 0.25], 0.03, 0.02)
+    assert math.isfinite(p) and p >= 0.0
