@@ -237,7 +237,12 @@ typedef struct ofg_basket {
     double fx_rate, fx_vol, domestic_rate, foreign_rate;   /* quanto */
     const double *asset_fx_corr;                             /* quanto */
     int32_t n_asset_fx_corr;
-    int32_t pad;
+    /* BasketOption under a FactorCorrelationModel instead of `corr`
+     * (price_basket_mc_with_factor_model, pricing/basket.rs:162-205;
+     * math/correlation.rs:62-212): 0 none, 1 OneFactor, 2 MultiFactor. */
+    int32_t factor_kind;
+    int32_t n_loading_rows, n_factors;   /* rows = the model's asset count; factors <= 4 */
+    const double *loadings;              /* n_loading_rows x n_factors, row-major */
 } ofg_basket;
 
 /* BasketSensitivities (pricing/basket.rs:57-65) and the diagnostics of

@@ -18,7 +18,8 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       lsm_last_timing, lsm_option_struct, lsm_price_raw, stream_jump,
                       mc_european_qmc, mc_european_qmc_with_seed, qmc_price_raw,
                       option_struct, price_raw, sample_payoffs)
-from .multi import (Autocallable, AutocallableSensitivities, BasketOption,
+from .multi import (Autocallable, AutocallableSensitivities, BasketCopula, BasketOption,
+                    FactorCorrelationModel, price_basket_mc_with_factor_model,
                     BasketSensitivities, BasketType, OutperformanceBasketOption,
                     PhoenixAutocallable, QuantoBasketOption, autocall_price_raw,
                     autocall_struct, autocallable_sensitivities, basket_price_raw,
@@ -47,6 +48,7 @@ __all__ = [
     "BasketOption", "BasketSensitivities", "BasketType", "basket_price_raw",
     "basket_sensitivities", "basket_struct", "price_basket_mc", "OutperformanceBasketOption",
     "QuantoBasketOption", "price_outperformance_basket_mc", "price_quanto_basket_mc",
+    "BasketCopula", "FactorCorrelationModel", "price_basket_mc_with_factor_model",
     "validate_or_repair_correlation_matrix",
     "AssetMarketData", "CompiledProduct", "CrossGreeks", "DslMonteCarloEngine", "DslProduct",
     "ExtendedGreeks", "MultiAssetMarket", "parse_and_compile",

@@ -145,7 +145,8 @@ class OfgBasket(C.Structure):
         ("product", C.c_int32), ("leader_index", C.c_int32), ("fx_rate", C.c_double),
         ("fx_vol", C.c_double), ("domestic_rate", C.c_double), ("foreign_rate", C.c_double),
         ("asset_fx_corr", C.POINTER(C.c_double)), ("n_asset_fx_corr", C.c_int32),
-        ("pad", C.c_int32),
+        ("factor_kind", C.c_int32), ("n_loading_rows", C.c_int32), ("n_factors", C.c_int32),
+        ("loadings", C.POINTER(C.c_double)),
     ]
 
 

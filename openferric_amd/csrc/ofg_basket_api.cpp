@@ -138,6 +138,52 @@ void validate_quanto(const ofg_basket &b, const std::vector<double> &spots,
         fail(OFG_INVALID_INPUT, "quanto average basket weights length must match assets");
 }
 
+// validate_inputs_factor (basket.rs:862-889) with FactorCorrelationModel::
+// validate (math/correlation.rs:84-121).
+void validate_factor(const ofg_basket &b, const std::vector<double> &spots,
+                     const std::vector<double> &vols) {
+    if (b.strike < 0.0) fail(OFG_INVALID_INPUT, "basket strike must be >= 0");
+    if (b.maturity < 0.0) fail(OFG_INVALID_INPUT, "basket maturity must be >= 0");
+    for (int32_t k = 0; k < b.n_weights; ++k)
+        if (!std::isfinite(b.weights[k])) fail(OFG_INVALID_INPUT, "basket weights must be finite");
+    if (b.basket_type == OFG_BASKET_AVERAGE && b.n_weights <= 0)
+        fail(OFG_INVALID_INPUT, "average basket requires non-empty weights");
+    if (b.num_paths <= 0) fail(OFG_INVALID_INPUT, "basket n_paths must be > 0");
+    if (b.n_assets <= 0) fail(OFG_INVALID_INPUT, "basket spots cannot be empty");
+    for (double s : spots)
+        if (s <= 0.0 || !std::isfinite(s)) fail(OFG_INVALID_INPUT, "basket spots must be finite and > 0");
+    for (double v : vols)
+        if (v < 0.0 || !std::isfinite(v)) fail(OFG_INVALID_INPUT, "basket vols must be finite and >= 0");
+    for (int32_t i = 0; i < b.n_assets; ++i)
+        if (!std::isfinite(b.dividends[i])) fail(OFG_INVALID_INPUT, "basket dividends must be finite");
+    const int rows = b.n_loading_rows, k = b.n_factors;
+    if (b.factor_kind == 1) {
+        if (rows <= 0) fail(OFG_INVALID_INPUT, "one-factor model requires at least one loading");
+        for (int i = 0; i < rows; ++i) {
+            const double beta = b.loadings[(size_t)i * k];
+            if (!std::isfinite(beta) || std::fabs(beta) > 1.0 + 1.0e-12)
+                fail(OFG_INVALID_INPUT, "one-factor loadings must be finite and in [-1, 1]");
+        }
+    } else {
+        if (rows <= 0) fail(OFG_INVALID_INPUT, "multi-factor model requires at least one asset row");
+        if (k <= 0) fail(OFG_INVALID_INPUT, "multi-factor model requires at least one factor");
+        for (int i = 0; i < rows; ++i) {
+            double norm2 = -0.0;
+            for (int j = 0; j < k; ++j) {
+                const double x = b.loadings[(size_t)i * k + j];
+                if (!std::isfinite(x)) fail(OFG_INVALID_INPUT, "multi-factor loadings must be finite");
+            }
+            for (int j = 0; j < k; ++j) norm2 += b.loadings[(size_t)i * k + j] * b.loadings[(size_t)i * k + j];
+            if (norm2 > 1.0 + 1.0e-10)
+                fail(OFG_INVALID_INPUT, "multi-factor row norm must be <= 1 for unit-variance assets");
+        }
+    }
+    if (rows != b.n_assets)
+        fail(OFG_INVALID_INPUT, "factor-model asset count must match spots length");
+    if (b.basket_type == OFG_BASKET_AVERAGE && b.n_weights != b.n_assets)
+        fail(OFG_INVALID_INPUT, "average basket weights length must match number of assets");
+}
+
 // terminal_payoff (basket.rs:759-784) of the spots themselves, times the
 // quanto FX rate; outperformance_payoff (:786-800).
 double payoff_at_spots(const ofg_basket &b, const std::vector<double> &spots) {
@@ -179,7 +225,16 @@ Scenario prepare(const ofg_basket &b, const std::vector<double> &spots,
     Scenario s;
     const int n = b.n_assets;
     Mat l;
-    if (b.product == 0) {
+    if (b.product == 0 && b.factor_kind != 0) {
+        // simulate_basket_once_factor (basket.rs:686-757): no matrix to repair
+        validate_factor(b, spots, vols);
+        if (b.maturity <= 0.0) {
+            s.host_price = payoff_at_spots(b, spots);
+            return s;
+        }
+        if (n > ofg::BK_MAX) fail(OFG_INVALID_INPUT, "at most 16 assets per basket");
+        if (b.n_factors > ofg::BK_FMAX) fail(OFG_INVALID_INPUT, "at most 4 factors per basket");
+    } else if (b.product == 0) {
         validate_inputs(b, spots, vols, corr);
         if (b.maturity <= 0.0) {
             s.host_price = payoff_at_spots(b, spots);
@@ -225,7 +280,28 @@ Scenario prepare(const ofg_basket &b, const std::vector<double> &spots,
         d.drift2[i] = d.drift[i] * OFG_INV_LN2_256;
         d.vol_t2[i] = d.vol_t[i] * OFG_INV_LN2_256;
         d.w[i] = (b.basket_type == OFG_BASKET_AVERAGE || b.product == 1) ? b.weights[i] : 0.0;
-        for (int j = 0; j <= i; ++j) d.chol[i * (i + 1) / 2 + j] = l[(size_t)i * n + j];
+        if (b.factor_kind == 0)
+            for (int j = 0; j <= i; ++j) d.chol[i * (i + 1) / 2 + j] = l[(size_t)i * n + j];
+    }
+    if (b.factor_kind != 0) {
+        const int k = b.n_factors;
+        d.n_factors = b.factor_kind == 1 ? 1 : k;
+        d.one_factor = b.factor_kind == 1;
+        for (int i = 0; i < n; ++i) {
+            if (b.factor_kind == 1) {   // beta clamped to [-1, 1], idio sqrt(max(1 - beta^2, 0))
+                const double beta = std::min(1.0, std::max(-1.0, b.loadings[(size_t)i * k]));
+                d.load[i * ofg::BK_FMAX] = beta;
+                d.idio[i] = std::sqrt(std::max(1.0 - beta * beta, 0.0));
+            } else {
+                double norm2 = -0.0;
+                for (int j = 0; j < k; ++j) {
+                    const double x = b.loadings[(size_t)i * k + j];
+                    d.load[i * ofg::BK_FMAX + j] = x;
+                    norm2 += x * x;
+                }
+                d.idio[i] = std::sqrt(std::max(1.0 - norm2, 0.0));
+            }
+        }
     }
     s.on_device = true;
     return s;
@@ -290,19 +366,29 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
         if (sens) std::memset(&sens[i], 0, sizeof sens[i]);
         BasketPlan &pl = plans[i];
         // a basket's external normals: [asset][path], n_assets * num_paths
+        // draws per path: one per asset, plus the factors of a factor model
+        const int64_t draws = (int64_t)b.n_assets +
+                              (b.factor_kind == 1 ? 1 : b.factor_kind == 2 ? std::max(b.n_factors, 0) : 0);
         const int64_t my_normals = normals_elems;
-        if (b.n_assets > 0 && b.num_paths > 0) normals_elems += (int64_t)b.n_assets * b.num_paths;
+        if (b.n_assets > 0 && b.num_paths > 0) normals_elems += draws * b.num_paths;
         try {
-            if (b.n_assets <= 0 || !b.spots || !b.vols || !b.dividends || !b.corr)
+            if (b.n_assets <= 0 || !b.spots || !b.vols || !b.dividends || (!b.corr && !b.factor_kind))
                 fail(OFG_INVALID_INPUT, "basket spots cannot be empty");
             if (b.n_weights < 0 || (b.n_weights > 0 && !b.weights))
                 fail(OFG_INVALID_INPUT, "basket weights must be finite");
             if (b.product < 0 || b.product > 2) fail(OFG_INVALID_INPUT, "unknown basket product");
+            if (b.factor_kind < 0 || b.factor_kind > 2 || (b.factor_kind && b.product != 0))
+                fail(OFG_INVALID_INPUT, "factor models apply to BasketOption only");
+            if (b.factor_kind && (b.n_loading_rows < 0 || b.n_factors < 0 ||
+                                  (b.factor_kind == 1 && b.n_factors != 1) ||
+                                  (b.n_loading_rows * b.n_factors > 0 && !b.loadings)))
+                fail(OFG_INVALID_INPUT, "factor-model loadings must be n_loading_rows x n_factors "
+                                        "(one factor for OneFactor)");
             if (b.product == 2 && b.n_asset_fx_corr > 0 && !b.asset_fx_corr)
                 fail(OFG_INVALID_INPUT, "quanto asset_fx_corr length must match assets");
             const int na = b.n_assets;
             std::vector<double> spots(b.spots, b.spots + na), vols(b.vols, b.vols + na);
-            const Mat corr(b.corr, b.corr + (size_t)na * na);
+            const Mat corr = b.corr ? Mat(b.corr, b.corr + (size_t)na * na) : Mat((size_t)na * na, 0.0);
             int64_t seed_off = -1;
             auto push = [&](Scenario s) {
                 if (s.on_device) {
@@ -310,7 +396,7 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
                     if (rng == OFG_RNG_REFERENCE) {
                         if (seed_off < 0) {   // one stream per basket, shared by its bumps
                             const ofg::LsmSeedJob j = ofg_jump::make_seed_job(
-                                b.seed, b.num_paths, na, seed_words, seed_threads, polys, poly_cache);
+                                b.seed, b.num_paths, draws, seed_words, seed_threads, polys, poly_cache);
                             jobs.push_back(j);
                             seed_threads += ofg_jump::seed_job_threads(j);
                             seed_off = seed_words;
@@ -326,8 +412,10 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
             pl.repaired = base.repaired;
             pl.base = push(base);
             pl.valid = true;
-            pl.rho = average_off_diagonal(corr, na);
-            if (with_sens && b.product == 0) {
+            // diagnostics rho: the matrix's average off-diagonal, 0 under a
+            // factor model (basket.rs:151, 195)
+            pl.rho = b.factor_kind ? 0.0 : average_off_diagonal(corr, na);
+            if (with_sens && b.product == 0 && b.factor_kind == 0) {
                 // basket_sensitivities (basket.rs:456-539)
                 try {
                     for (int k = 0; k < na; ++k) {

@@ -219,26 +219,55 @@ __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const d
     }
     if constexpr (FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
     double zpair = 0.0;
-#pragma unroll
-    for (int j = 0; j < BK_MAX; ++j) {
-        if (j < n) {
-            double z;
-            if constexpr (RNG == OFG_RNG_REFERENCE) {
-                z = reference_normal(ref);
-            } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
-                z = normals[o.normals_offset + (int64_t)j * o.paths + p];
+    // draw k of the path (external normals: [draw][path])
+    auto draw = [&](int k) {
+        double z;
+        if constexpr (RNG == OFG_RNG_REFERENCE) {
+            z = reference_normal(ref);
+        } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
+            z = normals[o.normals_offset + (int64_t)k * o.paths + p];
+        } else {
+            if ((k & 1) == 0) {
+                const uint64_t wu = fast.next_u64();
+                const uint64_t wt = fast.next_u64();
+                box_muller_pair(wu, (uint32_t)(wt >> 32), z, zpair, tab);
             } else {
-                if ((j & 1) == 0) {
-                    const uint64_t wu = fast.next_u64();
-                    const uint64_t wt = fast.next_u64();
-                    box_muller_pair(wu, (uint32_t)(wt >> 32), z, zpair, tab);
+                z = zpair;
+            }
+        }
+        return z;
+    };
+    if (o.n_factors == 0) {
+#pragma unroll
+        for (int j = 0; j < BK_MAX; ++j) {
+            if (j < n) {
+                const double z = draw(j);
+#pragma unroll
+                for (int i = j; i < BK_MAX; ++i)
+                    if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
+            }
+        }
+    } else {
+        // FactorCorrelationModel::sample_correlated_normals (correlation.rs:
+        // 173-212): the factors first, then one idiosyncratic draw per asset
+        double f[BK_FMAX];
+#pragma unroll
+        for (int k = 0; k < BK_FMAX; ++k)
+            if (k < o.n_factors) f[k] = draw(k);
+#pragma unroll
+        for (int i = 0; i < BK_MAX; ++i) {
+            if (i < n) {
+                const double eps = draw(o.n_factors + i);
+                if (o.one_factor) {
+                    c[i] = fma(o.load[i * BK_FMAX], f[0], o.idio[i] * eps);
                 } else {
-                    z = zpair;
+                    double sys = -0.0;
+#pragma unroll
+                    for (int k = 0; k < BK_FMAX; ++k)
+                        if (k < o.n_factors) sys += o.load[i * BK_FMAX + k] * f[k];
+                    c[i] = sys + o.idio[i] * eps;
                 }
             }
-#pragma unroll
-            for (int i = j; i < BK_MAX; ++i)
-                if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
         }
     }
     const bool outperf = o.payoff == BK_OUTPERFORMANCE;

@@ -74,6 +74,7 @@ hipError_t launch_autocall(const AcArgs &a, hipStream_t stream);
 
 // ---- basket options (src/pricing/basket.rs:599-683, Gaussian copula) ----
 constexpr int BK_MAX = 16;
+constexpr int BK_FMAX = 4;   // factors of a factor-model basket
 enum BasketType : int32_t { BK_AVERAGE = 0, BK_BEST_OF = 1, BK_WORST_OF = 2 };
 
 enum BasketPayoff : int32_t { BK_TERMINAL = 0, BK_OUTPERFORMANCE = 1 };
@@ -96,6 +97,11 @@ struct alignas(16) BkDev {
     double s0[BK_MAX], drift[BK_MAX], vol_t[BK_MAX], w[BK_MAX];   // w: weights or lagger weights
     double drift2[BK_MAX], vol_t2[BK_MAX];     // fast mode: scaled by 256/ln2
     double chol[BK_MAX * (BK_MAX + 1) / 2];   // lower factor, row i at i(i+1)/2
+    // factor model (n_factors > 0): normals from loadings instead of chol
+    int32_t n_factors;
+    int32_t one_factor;        // OneFactor: fma(beta, m, idio eps)
+    double load[BK_MAX * BK_FMAX];   // [asset][factor]; OneFactor: the clamped beta
+    double idio[BK_MAX];
 };
 
 struct BkArgs {

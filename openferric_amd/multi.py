@@ -277,8 +277,9 @@ def basket_struct(basket: BasketOption, spots, vols, corr_matrix, r, dividends, 
 
 def basket_price_raw(structs, rng_mode=RngMode.Reference, normals=None, with_sens=False,
                      device: int = 0):
-    """Calls ofg_basket_price. normals: per basket [path][asset] arrays
-    (External mode). Returns (results, sensitivities)."""
+    """Calls ofg_basket_price. normals: per basket [path][draw] arrays
+    (External mode; draws = assets, or factors then assets under a factor
+    model). Returns (results, sensitivities)."""
     from .engines import engine_cfg
     L = _lib.load()
     ctx = _lib.context(device)
@@ -323,6 +324,67 @@ def basket_sensitivities(basket: BasketOption, spots, vols, corr_matrix, r, divi
         raise InvalidInput(_lib.last_error() or "basket sensitivities failed")
     sv = sens[0]
     return BasketSensitivities([sv.delta[k] for k in range(sv.n_delta)], sv.vega, sv.cega)
+
+
+@dataclass
+class FactorCorrelationModel:
+    """math/correlation.rs:62-66: OneFactor (loadings = one beta per asset)
+    or MultiFactor (loadings = one row of factor loadings per asset)."""
+    loadings: Sequence
+    one_factor: bool = True
+
+    @staticmethod
+    def OneFactor(loadings) -> "FactorCorrelationModel":
+        return FactorCorrelationModel(list(loadings), True)
+
+    @staticmethod
+    def MultiFactor(loadings) -> "FactorCorrelationModel":
+        return FactorCorrelationModel([list(r) for r in loadings], False)
+
+    def n_assets(self) -> int:
+        return len(self.loadings)
+
+    def correlation_matrix(self):
+        """correlation_matrix (correlation.rs:123-161): implied correlations."""
+        L = np.asarray(self.loadings, dtype=np.float64)
+        if self.one_factor:
+            L = L.reshape(-1, 1)
+        c = np.clip(L @ L.T, -1.0, 1.0)
+        np.fill_diagonal(c, 1.0)
+        return c.tolist()
+
+
+class BasketCopula:
+    """pricing/basket.rs:32-46 (Gaussian only here: the Student-t copula's
+    CDF comes from a crate that is not part of the reference tree)."""
+    Gaussian = "gaussian"
+
+
+def price_basket_mc_with_factor_model(basket: BasketOption, spots, vols,
+                                      factor_model: FactorCorrelationModel, r, dividends, n_paths,
+                                      copula=BasketCopula.Gaussian, rng_mode=RngMode.Reference,
+                                      normals=None) -> PricingResult:
+    """price_basket_mc_with_factor_model (pricing/basket.rs:162-205).
+    normals (External): [path][draw], draws = factors then one per asset."""
+    if copula != BasketCopula.Gaussian:
+        raise InvalidInput("only the Gaussian copula is supported")
+    keep = _Keep()
+    n = len(spots)
+    b = basket_struct(basket, spots, vols, np.eye(n) if n else [], r, dividends, n_paths, keep)
+    b.corr = None
+    rows = np.asarray(factor_model.loadings, dtype=np.float64)
+    k = 1 if factor_model.one_factor else (rows.shape[1] if rows.ndim == 2 and rows.size else 0)
+    b.factor_kind = 1 if factor_model.one_factor else 2
+    b.n_loading_rows, b.n_factors = len(factor_model.loadings), k
+    b.loadings = keep.arr(rows.ravel()) if rows.size else None
+    res, sens = basket_price_raw([b], rng_mode, None if normals is None else [normals])
+    r0 = res[0]
+    diag = Diagnostics()
+    if r0.has_stderr:
+        diag.insert("num_paths", r0.num_paths)
+        diag.insert("rho", 0.0)
+    return PricingResult(price=r0.price, stderr=r0.stderr_ if r0.has_stderr else None,
+                         diagnostics=diag, kernel_ms=r0.kernel_ms)
 
 
 @dataclass

@@ -241,6 +241,11 @@ typedef struct {
 } or_basket;
 int or_basket_price(const or_basket *b, const double *normals, double *price, double *stderr_out);
 
+/* simulate_basket_once_factor (basket.rs:686-757): OneFactor (loadings =
+ * betas) or MultiFactor (loadings [n][k]); normals [path][k + n] or NULL. */
+int or_basket_factor_price(const or_basket *b, int one_factor, int32_t k, const double *loadings,
+                           const double *normals, double *price, double *stderr_out);
+
 /* price_outperformance_basket_mc (basket.rs:288-366): weights = lagger
  * weights, ratio S_leader / max(sum w S, 1e-12). price_quanto_basket_mc
  * (:368-445): quanto drift r_f - q - rho_fx vol fx_vol, payoff times fx_rate,

@@ -161,6 +161,9 @@ def lib():
         L.or_bump_corr_matrix.argtypes = [pd, i32, d, pd]; L.or_bump_corr_matrix.restype = None
         L.or_basket_price.argtypes = [C.POINTER(OrBasket), pd, pd, pd]
         L.or_basket_price.restype = C.c_int
+        L.or_basket_factor_price.argtypes = [C.POINTER(OrBasket), C.c_int, C.c_int32, pd, pd, pd,
+                                             pd]
+        L.or_basket_factor_price.restype = C.c_int
         L.or_outperformance_price.argtypes = [C.POINTER(OrBasket), C.c_int32, pd, pd, pd]
         L.or_outperformance_price.restype = C.c_int
         L.or_quanto_price.argtypes = [C.POINTER(OrBasket), C.c_double, C.c_double, pd, C.c_double,
@@ -483,4 +486,17 @@ def quanto_price(b, fx_rate, fx_vol, asset_fx_corr, domestic_rate, foreign_rate,
                                foreign_rate, _ptr(z), C.byref(p), C.byref(se))
     if rc != 0:
         raise ValueError(f"or_quanto_price failed rc={rc}")
+    return p.value, se.value
+
+
+def basket_factor_price(b, loadings, one_factor, normals=None):
+    """simulate_basket_once_factor: (price, stderr). normals [path][k + n]."""
+    L = np.ascontiguousarray(loadings, dtype=np.float64)
+    k = 1 if one_factor else L.shape[1]
+    z = np.ascontiguousarray(normals, dtype=np.float64).ravel() if normals is not None else None
+    p, se = C.c_double(), C.c_double()
+    rc = lib().or_basket_factor_price(C.byref(b), int(one_factor), k, _ptr(L.ravel()), _ptr(z),
+                                      C.byref(p), C.byref(se))
+    if rc != 0:
+        raise ValueError(f"or_basket_factor_price failed rc={rc}")
     return p.value, se.value

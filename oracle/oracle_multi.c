@@ -29,6 +29,10 @@
  * e^{-rT} N E[min_i S_i(T)/S_i(0)] / K_ki with E[S(T)/S(0)] = e^{(r-q)T} for
  * one asset and Margrabe's exchange-option formula for two correlated
  * assets; the reference's own monotonicity and single-underlying tests.
+ * Factor-model baskets (or_basket_factor_price): pinned by the model's
+ * implied correlation -- a two-asset best-of under a one-factor model
+ * equals Margrabe with rho = beta1 beta2, a multi-factor basket matches
+ * the Cholesky basket on the implied matrix -- and the reference's test.
  * The outperformance and quanto variants (basket.rs:288-445) share the
  * loop: pinned by the lognormal ratio's closed forms (E[S1/S2] and the
  * Black formula on it) and the quanto-adjusted Black-Scholes call.
@@ -360,4 +364,69 @@ int or_quanto_price(const or_basket *b, double fx_rate, double fx_vol, const dou
     }
     return basket_variant(b, 2, 0, fx_rate, drift, exp(-domestic_rate * t), normals, price,
                           stderr_out);
+}
+
+/* simulate_basket_once_factor (basket.rs:686-757) with
+ * FactorCorrelationModel::sample_correlated_normals (correlation.rs:173-212):
+ * one_factor: loadings[i] = beta_i (clamped), out = fma(beta, m, idio eps);
+ * else loadings [n][k], out = sum_j l_ij f_j + idio eps. Draws per path: the
+ * k factors, then one per asset; normals [path][k + n] or NULL. */
+int or_basket_factor_price(const or_basket *b, int one_factor, int32_t k, const double *loadings,
+                           const double *normals, double *price, double *stderr_out) {
+    const int32_t n = b->n_assets;
+    if (b->maturity <= 0.0) {
+        *price = basket_payoff(b, b->spots, b->spots);
+        *stderr_out = 0.0;
+        return 0;
+    }
+    if (one_factor) k = 1;
+    const double t = b->maturity, sqrt_t = sqrt(t);
+    const double discount = exp(-b->rate * t);
+    double drift[64], vol_t[64], term[64], f[64], idio[64];
+    for (int32_t i = 0; i < n; ++i) {
+        drift[i] = (b->rate - b->dividends[i] - 0.5 * b->vols[i] * b->vols[i]) * t;
+        vol_t[i] = b->vols[i] * sqrt_t;
+        if (one_factor) {
+            const double beta = fmin(1.0, fmax(-1.0, loadings[i]));
+            idio[i] = sqrt(fmax(1.0 - beta * beta, 0.0));
+        } else {
+            double norm2 = -0.0;
+            for (int32_t j = 0; j < k; ++j) norm2 += loadings[i * k + j] * loadings[i * k + j];
+            idio[i] = sqrt(fmax(1.0 - norm2, 0.0));
+        }
+    }
+    uint64_t st[4];
+    or_xoshiro_seed(b->seed, st);
+    double mean = 0.0, m2 = 0.0;
+    int64_t cnt = 0, d;
+    for (int64_t p = 0; p < b->num_paths; ++p) {
+        d = 0;
+#define OR_DRAW() (normals ? normals[p * (k + n) + (d++)] \
+                           : (d++, or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(st)))))
+        for (int32_t j = 0; j < k; ++j) f[j] = OR_DRAW();
+        for (int32_t i = 0; i < n; ++i) {
+            double z;
+            if (one_factor) {
+                const double beta = fmin(1.0, fmax(-1.0, loadings[i]));
+                const double eps = OR_DRAW();
+                z = fma(beta, f[0], idio[i] * eps);
+            } else {
+                double sys = -0.0;
+                for (int32_t j = 0; j < k; ++j) sys += loadings[i * k + j] * f[j];
+                const double eps = OR_DRAW();
+                z = sys + idio[i] * eps;
+            }
+            term[i] = b->spots[i] * exp(drift[i] + vol_t[i] * z);
+        }
+#undef OR_DRAW
+        const double x = basket_payoff(b, b->spots, term);
+        ++cnt;
+        const double delta = x - mean;
+        mean += delta / (double)cnt;
+        m2 += delta * (x - mean);
+    }
+    const double var = cnt > 1 ? m2 / ((double)cnt - 1.0) : 0.0;
+    *price = discount * mean;
+    *stderr_out = discount * sqrt(var / (double)b->num_paths);
+    return 0;
 }

@@ -222,3 +222,35 @@ def test_outperformance_and_quanto_match_oracle(oracle, gpu):
                            [0.3], 0.03, 0.02)
     r = price_quanto_basket_mc(q, spots, vols, corr, divs, 100)
     assert math.isnan(r.price) and "asset_fx_corr length" in _lib.last_error()
+
+
+def test_factor_model_baskets_match_oracle(oracle, gpu):
+    from openferric_amd import (BasketOption, BasketType, FactorCorrelationModel, RngMode, _lib,
+                                price_basket_mc_with_factor_model)
+    spots, vols, divs = [100.0, 101.0, 99.0, 97.0, 103.0], [0.2, 0.22, 0.18, 0.21, 0.19], [0.0] * 5
+    w = [0.3, 0.25, 0.2, 0.15, 0.1]
+    multi = [[0.6, 0.1], [0.5, -0.1], [0.4, 0.2], [0.35, -0.2], [0.25, 0.15]]
+    one = [0.7, -0.3, 0.5, 0.9, 0.2]
+    paths = 90_001
+    for model, kind, loads, k in ((FactorCorrelationModel.MultiFactor(multi), AVG, multi, 2),
+                                  (FactorCorrelationModel.OneFactor(one), WORST, one, 1)):
+        basket = BasketOption(w if kind == AVG else [], 100.0 if kind == AVG else 0.95, 1.0,
+                              kind == AVG, kind)
+        for mode, z in ((RngMode.Reference, None),
+                        (RngMode.External,
+                         np.random.default_rng(k).standard_normal((paths, k + 5)))):
+            r = price_basket_mc_with_factor_model(basket, spots, vols, model, 0.02, divs, paths,
+                                                  rng_mode=mode, normals=z)
+            b = oracle.make_basket(spots, vols, divs, np.eye(5), w if kind == AVG else None, kind,
+                                   kind == AVG, 100.0 if kind == AVG else 0.95, 1.0, 0.02, paths)
+            p, se = oracle.basket_factor_price(b, loads, k == 1, z)
+            assert abs(r.price - p) <= 1e-10 * p and abs(r.stderr - se) <= 1e-8 * se, (r.price, p)
+            assert r.diagnostics.get("rho") == 0.0
+    bad = FactorCorrelationModel.MultiFactor([[0.9, 0.9]] * 5)
+    r = price_basket_mc_with_factor_model(BasketOption(w, 100.0, 1.0, True), spots, vols, bad,
+                                          0.02, divs, 100)
+    assert math.isnan(r.price) and "row norm must be <= 1" in _lib.last_error()
+    short = FactorCorrelationModel.OneFactor([0.5, 0.5])
+    r = price_basket_mc_with_factor_model(BasketOption(w, 100.0, 1.0, True), spots, vols, short,
+                                          0.02, divs, 100)
+    assert math.isnan(r.price) and "asset count must match" in _lib.last_error()

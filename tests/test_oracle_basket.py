@@ -221,3 +221,36 @@ def test_variants_zero_maturity_and_reference_case(oracle):
                            [0.6, 0.4], AVG, True, 100.0, 1.0, 0.0, 120_000)
     p, _ = oracle.quanto_price(b, 1.2, 0.12, [0.3, 0.25], 0.03, 0.02)
     assert math.isfinite(p) and p >= 0.0
+
+
+# ---- factor-model baskets (basket.rs:162-205, 686-757) ----------------------
+
+@pytest.mark.parametrize("betas", [(0.8, 0.5), (-0.6, 0.7)])
+def test_one_factor_best_of_matches_margrabe(oracle, betas):
+    s1, s2, q1, q2, r, t = 0.3, 0.2, 0.01, 0.03, 0.02, 1.0
+    rho = betas[0] * betas[1]
+    b = oracle.make_basket([100.0, 50.0], [s1, s2], [q1, q2], np.eye(2), None, BEST, True, 0.0,
+                           t, r, 400_000)
+    p, se = oracle.basket_factor_price(b, list(betas), True)
+    sig = math.sqrt(s1 * s1 + s2 * s2 - 2 * rho * s1 * s2)
+    a, bb = math.exp(-q1 * t), math.exp(-q2 * t)
+    assert abs(p - (a + margrabe(bb, a, sig, t))) < 4 * se
+
+
+def test_multi_factor_matches_cholesky_basket_on_implied_matrix(oracle):
+    load = np.array([[0.6, 0.1], [0.5, -0.1], [0.4, 0.2], [0.35, -0.2], [0.25, 0.15]])
+    implied = np.clip(load @ load.T, -1, 1)
+    np.fill_diagonal(implied, 1.0)
+    args = ([100.0, 101.0, 99.0, 97.0, 103.0], [0.2, 0.22, 0.18, 0.21, 0.19], [0.0] * 5)
+    w = [0.3, 0.25, 0.2, 0.15, 0.1]
+    bf = oracle.make_basket(*args, np.eye(5), w, AVG, True, 100.0, 1.0, 0.02, 200_000)
+    bc = oracle.make_basket(*args, implied, w, AVG, True, 100.0, 1.0, 0.02, 200_000, seed=99)
+    pf, sf = oracle.basket_factor_price(bf, load, False)
+    pc, sc = oracle.basket_price(bc)
+    assert abs(pf - pc) < 4 * math.hypot(sf, sc), (pf, pc)
+    # factor_model_mc_prices_finite (basket.rs:1228-1260)
+    assert math.isfinite(pf) and pf >= 0.0
+    z = np.random.default_rng(3).standard_normal((1000, 7))
+    bz = oracle.make_basket(*args, np.eye(5), w, AVG, True, 100.0, 1.0, 0.02, 1000)
+    assert oracle.basket_factor_price(bz, load, False, z) != oracle.basket_factor_price(
+        bz, load, False, z[:, ::-1].copy())   # the draw order matters: factors first
