@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -36,14 +37,14 @@ using namespace ofg_corr;
 namespace ofg_host {
 
 struct BasketBuffers {
-    DevBuf scen, tiles, partials, out, normals, seeds, polys, jobs;
+    DevBuf scen, tiles, partials, out, normals, seeds, polys, jobs, gbk, gtiles, gpartials, gout;
 };
 
 void basket_release(ofg_ctx *ctx) {
     if (!ctx || !ctx->basket) return;
     BasketBuffers *b = ctx->basket;
     for (DevBuf *d : {&b->scen, &b->tiles, &b->partials, &b->out, &b->normals, &b->seeds,
-                      &b->polys, &b->jobs})
+                      &b->polys, &b->jobs, &b->gbk, &b->gtiles, &b->gpartials, &b->gout})
         d->release();
     delete b;
     ctx->basket = nullptr;
@@ -460,11 +461,86 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
             pl.valid = false;
         }
     }
+    // basket_sensitivities fused per path: every scenario of a basket on the
+    // same draws in one pass (OFG_BASKET_FUSED=0 prices them one by one)
+    std::vector<int32_t> fused_slot(scen.size(), -1);
+    std::vector<ofg::BkGreeksDev> gdev;
+    std::vector<int32_t> gtiles;
+    int32_t g_out = 0, max_n = 0;
+    const char *fenv = std::getenv("OFG_BASKET_FUSED");
+    const bool fuse = with_sens && !(fenv && std::atoi(fenv) == 0);
+    for (size_t i = 0; fuse && i < n; ++i) {
+        const BasketPlan &pl = plans[i];
+        if (!pl.valid || !pl.sens_ok || in[i].product != 0 || in[i].factor_kind != 0) continue;
+        std::vector<int32_t> ids{pl.base, pl.vega_up, pl.vega_dn};
+        for (size_t k = 0; k < pl.delta_up.size(); ++k) {
+            ids.push_back(pl.delta_up[k]);
+            ids.push_back(pl.delta_dn[k]);
+        }
+        if (pl.cega_up >= 0) {
+            ids.push_back(pl.cega_up);
+            ids.push_back(pl.cega_dn);
+        }
+        bool all_dev = true;
+        for (int32_t id : ids) all_dev = all_dev && scen[(size_t)id].on_device;
+        if (!all_dev) continue;
+        const ofg::BkDev &b0 = scen[(size_t)pl.base].d;
+        ofg::BkGreeksDev g{};
+        g.n = b0.n;
+        g.type = b0.type;
+        g.is_call = b0.is_call;
+        g.has_cega = pl.cega_up >= 0;
+        g.n_scen = 1 + 2 * g.n + 2 + 2 * g.has_cega;
+        g.paths = b0.paths;
+        g.normals_offset = b0.normals_offset;
+        g.seed_offset = b0.seed_offset;
+        g.seed = b0.seed;
+        g.strike = b0.strike;
+        g.discount = b0.discount;
+        const ofg::BkDev *sets[3] = {&b0, &scen[(size_t)pl.vega_up].d, &scen[(size_t)pl.vega_dn].d};
+        const ofg::BkDev *csets[3] = {&b0, g.has_cega ? &scen[(size_t)pl.cega_up].d : &b0,
+                                      g.has_cega ? &scen[(size_t)pl.cega_dn].d : &b0};
+        for (int a = 0; a < g.n; ++a) {
+            g.s0[a] = b0.s0[a];
+            g.w[a] = b0.w[a];
+            g.s0_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].d.s0[a];
+            g.s0_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].d.s0[a];
+            for (int v = 0; v < 3; ++v) {
+                g.drift[v][a] = sets[v]->drift[a];
+                g.vol_t[v][a] = sets[v]->vol_t[a];
+                g.drift2[v][a] = sets[v]->drift2[a];
+                g.vol_t2[v][a] = sets[v]->vol_t2[a];
+            }
+        }
+        for (int v = 0; v < 3; ++v)
+            std::memcpy(g.chol[v], csets[v]->chol, sizeof g.chol[v]);
+        const int64_t blk = ofg::MA_BLOCK;
+        g.ppt = (int32_t)std::max<int64_t>(
+            1, (g.paths + blk * ofg::MA_MAX_TILES - 1) / (blk * ofg::MA_MAX_TILES));
+        g.n_tiles = (int32_t)((g.paths + blk * g.ppt - 1) / (blk * g.ppt));
+        g.tile_first = (int32_t)gtiles.size();
+        for (int32_t t = 0; t < g.n_tiles; ++t) gtiles.push_back((int32_t)gdev.size());
+        g.out_first = g_out;
+        g_out += ofg::BKG_MAXS;
+        fused_slot[(size_t)pl.base] = g.out_first;
+        for (int k = 0; k < g.n; ++k) {
+            fused_slot[(size_t)pl.delta_up[(size_t)k]] = g.out_first + 1 + 2 * k;
+            fused_slot[(size_t)pl.delta_dn[(size_t)k]] = g.out_first + 2 + 2 * k;
+        }
+        fused_slot[(size_t)pl.vega_up] = g.out_first + 1 + 2 * g.n;
+        fused_slot[(size_t)pl.vega_dn] = g.out_first + 2 + 2 * g.n;
+        if (g.has_cega) {
+            fused_slot[(size_t)pl.cega_up] = g.out_first + 3 + 2 * g.n;
+            fused_slot[(size_t)pl.cega_dn] = g.out_first + 4 + 2 * g.n;
+        }
+        max_n = std::max(max_n, g.n);
+        gdev.push_back(g);
+    }
     std::vector<int32_t> dev_index(scen.size(), -1);
     std::vector<ofg::BkDev> dev;
     std::vector<int32_t> tiles;
     for (size_t s = 0; s < scen.size(); ++s) {
-        if (!scen[s].on_device) continue;
+        if (!scen[s].on_device || fused_slot[s] >= 0) continue;
         ofg::BkDev d = scen[s].d;
         const int64_t blk = ofg::MA_BLOCK;
         d.ppt = (int32_t)std::max<int64_t>(
@@ -475,11 +551,11 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
         dev_index[s] = (int32_t)dev.size();
         dev.push_back(d);
     }
-    if (rng == OFG_RNG_EXTERNAL && !dev.empty() && !cfg->normals1)
+    if (rng == OFG_RNG_EXTERNAL && (!dev.empty() || !gdev.empty()) && !cfg->normals1)
         fail(OFG_INVALID_INPUT, "external rng mode needs normals1");
-    std::vector<ofg::AcOut> res(dev.size());
+    std::vector<ofg::AcOut> res(dev.size()), gres((size_t)g_out);
     float ms = 0.f;
-    if (!dev.empty()) {
+    if (!dev.empty() || !gdev.empty()) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         if (!ctx->basket) ctx->basket = new BasketBuffers();
@@ -517,15 +593,43 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
                                             seed_threads, bb.polys.as<uint64_t>(),
                                             bb.seeds.as<uint64_t>(), st),
                       "basket seed kernel launch");
-        hip_check(ofg::launch_basket(a, st), "basket kernel launch");
+        if (!dev.empty()) hip_check(ofg::launch_basket(a, st), "basket kernel launch");
+        ofg::BkgArgs ga{};
+        if (!gdev.empty()) {
+            up(bb.gbk, gdev.data(), gdev.size() * sizeof(gdev[0]), "H2D basket sensitivities");
+            up(bb.gtiles, gtiles.data(), gtiles.size() * sizeof(int32_t), "H2D basket tiles");
+            bb.gpartials.ensure(gtiles.size() * ofg::BKG_MAXS * sizeof(ofg::AcPartial));
+            bb.gout.ensure((size_t)g_out * sizeof(ofg::AcOut));
+            ga.bk = bb.gbk.as<ofg::BkGreeksDev>();
+            ga.tile_bk = bb.gtiles.as<int32_t>();
+            ga.partials = bb.gpartials.as<ofg::AcPartial>();
+            ga.out = bb.gout.as<ofg::AcOut>();
+            ga.normals = a.normals;
+            ga.seeds = a.seeds;
+            ga.n_tiles = (int32_t)gtiles.size();
+            ga.n_bk = (int32_t)gdev.size();
+            ga.rng_mode = rng;
+            ga.max_n = max_n;
+            hip_check(ofg::launch_basket_greeks(ga, st), "basket sensitivity kernel launch");
+        }
         hip_check(hipEventRecord(ctx->ev1, st), "event");
-        hip_check(hipMemcpyAsync(res.data(), a.out, res.size() * sizeof(res[0]),
-                                 hipMemcpyDeviceToHost, st),
-                  "D2H basket results");
+        if (!dev.empty())
+            hip_check(hipMemcpyAsync(res.data(), a.out, res.size() * sizeof(res[0]),
+                                     hipMemcpyDeviceToHost, st),
+                      "D2H basket results");
+        if (!gdev.empty())
+            hip_check(hipMemcpyAsync(gres.data(), ga.out, gres.size() * sizeof(gres[0]),
+                                     hipMemcpyDeviceToHost, st),
+                      "D2H basket sensitivity results");
         hip_check(hipStreamSynchronize(st), "basket kernel execution");
         hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
     }
     auto price_of = [&](int32_t s, double *se) {
+        if (fused_slot[(size_t)s] >= 0) {
+            const ofg::AcOut &r = gres[(size_t)fused_slot[(size_t)s]];
+            if (se) *se = r.stderr_;
+            return r.price;
+        }
         if (dev_index[(size_t)s] < 0) {
             if (se) *se = 0.0;
             return scen[(size_t)s].host_price;

@@ -367,7 +367,226 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_close_kernel(BkArgs a) {
     a.out[sc].stderr_ = o.discount * sqrt(var / n);
 }
 
+// ------------------------------------------------------------------------
+// basket_sensitivities fused per path
+// ------------------------------------------------------------------------
+
+// terminal_payoff with asset k's terminal spot and initial spot replaced
+// (k = -1: none), the reference's fold order.
+template <int MAXN>
+__device__ __forceinline__ double bkg_payoff(const BkGreeksDev &o, const double *st, int k,
+                                             double stk, double s0k) {
+    const int n = o.n;
+    double u = o.type == BK_AVERAGE ? -0.0 : (o.type == BK_BEST_OF ? -INFINITY : INFINITY);
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        if (i < n) {
+            const double sti = i == k ? stk : st[i];
+            const double s0i = i == k ? s0k : o.s0[i];
+            if (o.type == BK_AVERAGE) u += o.w[i] * sti;
+            else if (o.type == BK_BEST_OF) u = fmax(u, sti / s0i);
+            else u = fmin(u, sti / s0i);
+        }
+    }
+    return o.is_call ? fmax(u - o.strike, 0.0) : fmax(o.strike - u, 0.0);
+}
+
+// exp(drift + vol_t c) per asset for parameter set `set`.
+template <int RNG, int MAXN>
+__device__ __forceinline__ void bkg_factors(const BkGreeksDev &o, int set, const double *c,
+                                            double *g, const double *tab) {
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        if (i < o.n) {
+            if constexpr (RNG == OFG_RNG_FAST) {
+                double y = o.drift2[set][i] + o.vol_t2[set][i] * c[i];
+                y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
+                g[i] = exp2_256(y, tab);
+            } else {
+                g[i] = exp_fp64(o.drift[set][i] + o.vol_t[set][i] * c[i]);
+            }
+        }
+    }
+}
+
+template <int MAXN>
+__device__ __forceinline__ void bkg_correlate(const BkGreeksDev &o, int set, const double *z,
+                                              double *c) {
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) c[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAXN; ++j)
+#pragma unroll
+        for (int i = j; i < MAXN; ++i)
+            if (i < o.n && j < o.n) c[i] += o.chol[set][i * (i + 1) / 2 + j] * z[j];
+}
+
+template <int RNG, int MAXN>
+__global__ __launch_bounds__(MA_BLOCK) void basket_greeks_kernel(BkgArgs a) {
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    __shared__ double red[NWAVES][BKG_MAXS][2];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += MA_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    }
+    const int tile = blockIdx.x;
+    const BkGreeksDev &o = a.bk[a.tile_bk[tile]];
+    const int n = o.n;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    const int64_t base = (int64_t)(tile - o.tile_first) * MA_BLOCK * o.ppt;
+    // lane s accumulates scenario s's wave sums
+    double acc = 0.0, acc_sq = 0.0;
+    auto add = [&](int s, double x) {
+        const double sx = wave_sum(x), sq = wave_sum(x * x);
+        if (lane == s) {
+            acc += sx;
+            acc_sq += sq;
+        }
+    };
+    for (int j = 0; j < o.ppt; ++j) {
+        const int64_t p = base + (int64_t)j * MA_BLOCK + threadIdx.x;
+        const bool valid = p < o.paths;
+        if (!__any(valid)) break;
+        double z[MAXN], c[MAXN], g[MAXN], st[MAXN];
+        if (valid) {
+            Xoshiro256pp ref;
+            Xoroshiro128p fast;
+            if constexpr (RNG == OFG_RNG_REFERENCE) {
+                const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
+                ref.s0 = sd[0];
+                ref.s1 = sd[1];
+                ref.s2 = sd[2];
+                ref.s3 = sd[3];
+            }
+            if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
+            double zpair = 0.0;
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i) {
+                if (i < n) {
+                    if constexpr (RNG == OFG_RNG_REFERENCE) {
+                        z[i] = reference_normal(ref);
+                    } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
+                        z[i] = a.normals[o.normals_offset + (int64_t)i * o.paths + p];
+                    } else {
+                        if ((i & 1) == 0) {
+                            const uint64_t wu = fast.next_u64();
+                            const uint64_t wt = fast.next_u64();
+                            box_muller_pair(wu, (uint32_t)(wt >> 32), z[i], zpair, tab);
+                        } else {
+                            z[i] = zpair;
+                        }
+                    }
+                } else {
+                    z[i] = 0.0;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i) z[i] = 0.0;
+        }
+        const double keep = valid ? 1.0 : 0.0;   // invalid lanes add zeros
+        bkg_correlate<MAXN>(o, 0, z, c);
+        bkg_factors<RNG, MAXN>(o, 0, c, g, tab);
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) st[i] = i < n ? o.s0[i] * g[i] : 0.0;
+        add(0, keep * bkg_payoff<MAXN>(o, st, -1, 0.0, 0.0));
+        for (int k = 0; k < n; ++k) {
+            double gk = g[0];
+#pragma unroll
+            for (int i = 1; i < MAXN; ++i)
+                if (i == k) gk = g[i];
+            add(1 + 2 * k, keep * bkg_payoff<MAXN>(o, st, k, o.s0_up[k] * gk, o.s0_up[k]));
+            add(2 + 2 * k, keep * bkg_payoff<MAXN>(o, st, k, o.s0_dn[k] * gk, o.s0_dn[k]));
+        }
+        for (int v = 1; v <= 2; ++v) {   // vol up, vol down: the base correlated normals
+            bkg_factors<RNG, MAXN>(o, v, c, g, tab);
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i) st[i] = i < n ? o.s0[i] * g[i] : 0.0;
+            add(2 * n + v, keep * bkg_payoff<MAXN>(o, st, -1, 0.0, 0.0));
+        }
+        if (o.has_cega) {
+            for (int v = 1; v <= 2; ++v) {   // correlation up, down: the same draws
+                bkg_correlate<MAXN>(o, v, z, c);
+                bkg_factors<RNG, MAXN>(o, 0, c, g, tab);
+#pragma unroll
+                for (int i = 0; i < MAXN; ++i) st[i] = i < n ? o.s0[i] * g[i] : 0.0;
+                add(2 * n + 2 + v, keep * bkg_payoff<MAXN>(o, st, -1, 0.0, 0.0));
+            }
+        }
+    }
+    if (lane < BKG_MAXS) {
+        red[wave][lane][0] = acc;
+        red[wave][lane][1] = acc_sq;
+    }
+    __syncthreads();
+    if (threadIdx.x < o.n_scen) {
+        double sm = red[0][threadIdx.x][0], q = red[0][threadIdx.x][1];
+        for (int w = 1; w < NWAVES; ++w) {
+            sm += red[w][threadIdx.x][0];
+            q += red[w][threadIdx.x][1];
+        }
+        a.partials[(int64_t)tile * BKG_MAXS + threadIdx.x].sum = sm;
+        a.partials[(int64_t)tile * BKG_MAXS + threadIdx.x].sum_sq = q;
+    }
+}
+
+// One block per (basket, scenario): the tile slots in tile order.
+__global__ __launch_bounds__(MA_BLOCK) void basket_greeks_close_kernel(BkgArgs a) {
+    __shared__ double red[NWAVES][2];
+    const BkGreeksDev &o = a.bk[blockIdx.x];
+    const int sc = blockIdx.y;
+    if (sc >= o.n_scen) return;
+    double s = 0.0, q = 0.0;
+    for (int t = threadIdx.x; t < o.n_tiles; t += MA_BLOCK) {
+        const AcPartial pt = a.partials[(int64_t)(o.tile_first + t) * BKG_MAXS + sc];
+        s += pt.sum;
+        q += pt.sum_sq;
+    }
+    s = wave_sum(s);
+    q = wave_sum(q);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = s;
+        red[wave][1] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double sum = red[0][0], sq = red[0][1];
+    for (int w = 1; w < NWAVES; ++w) {
+        sum += red[w][0];
+        sq += red[w][1];
+    }
+    const double nn = (double)o.paths;
+    const double var = o.paths > 1 ? fmax((sq - sum * sum / nn) / (nn - 1.0), 0.0) : 0.0;
+    a.out[o.out_first + sc].price = o.discount * (sum / nn);
+    a.out[o.out_first + sc].stderr_ = o.discount * sqrt(var / nn);
+}
+
+template <int MAXN>
+void launch_bkg(const BkgArgs &a, hipStream_t stream) {
+    switch (a.rng_mode) {
+    case OFG_RNG_FAST:
+        basket_greeks_kernel<OFG_RNG_FAST, MAXN><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+        break;
+    case OFG_RNG_EXTERNAL:
+        basket_greeks_kernel<OFG_RNG_EXTERNAL, MAXN><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+        break;
+    default: basket_greeks_kernel<OFG_RNG_REFERENCE, MAXN><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_basket_greeks(const BkgArgs &a, hipStream_t stream) {
+    if (a.n_tiles <= 0) return hipSuccess;
+    if (a.max_n <= 4) launch_bkg<4>(a, stream);
+    else if (a.max_n <= 8) launch_bkg<8>(a, stream);
+    else launch_bkg<BK_MAX>(a, stream);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    basket_greeks_close_kernel<<<dim3(a.n_bk, BKG_MAXS), MA_BLOCK, 0, stream>>>(a);
+    return hipGetLastError();
+}
 
 hipError_t launch_basket(const BkArgs &a, hipStream_t stream) {
     if (a.n_tiles <= 0) return hipSuccess;

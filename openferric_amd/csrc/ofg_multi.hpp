@@ -119,4 +119,44 @@ struct BkArgs {
 
 hipError_t launch_basket(const BkArgs &a, hipStream_t stream);
 
+// ---- basket_sensitivities fused per path ----
+// All scenarios of one basket's sensitivities (base, spot up / down per
+// asset, vol up / down, correlation up / down) evaluated on each path's
+// shared draws: the spot bumps reuse the base terminal factors (a spot bump
+// scales S_i(T) = S_i(0) exp(...) and leaves the exponent unchanged), the
+// vol bumps reuse the base correlated normals, the correlation bumps reuse
+// the independent draws. Scenario order: 0 base; 1 + 2k / 2 + 2k spot up /
+// down of asset k; 1 + 2n / 2 + 2n vol up / down; 3 + 2n / 4 + 2n
+// correlation up / down (when present).
+constexpr int BKG_MAXS = 1 + 2 * BK_MAX + 4;
+
+struct alignas(16) BkGreeksDev {
+    int32_t n, type, is_call, has_cega;
+    int32_t tile_first, n_tiles, ppt, n_scen;
+    int32_t out_first;     // first result slot
+    int32_t pad;
+    int64_t paths, normals_offset, seed_offset;
+    uint64_t seed;
+    double strike, discount;
+    double s0[BK_MAX], s0_up[BK_MAX], s0_dn[BK_MAX], w[BK_MAX];
+    double drift[3][BK_MAX], vol_t[3][BK_MAX];     // base, vol up, vol down
+    double drift2[3][BK_MAX], vol_t2[3][BK_MAX];   // fast mode: scaled by 256/ln2
+    double chol[3][BK_MAX * (BK_MAX + 1) / 2];    // base, correlation up, down
+};
+
+struct BkgArgs {
+    const BkGreeksDev *bk;
+    const int32_t *tile_bk;
+    AcPartial *partials;     // [tile][BKG_MAXS]
+    AcOut *out;
+    const double *normals;
+    const uint64_t *seeds;
+    int32_t n_tiles;
+    int32_t n_bk;
+    int32_t rng_mode;
+    int32_t max_n;           // largest asset count of the launch
+};
+
+hipError_t launch_basket_greeks(const BkgArgs &a, hipStream_t stream);
+
 }  // namespace ofg

@@ -254,3 +254,24 @@ def test_factor_model_baskets_match_oracle(oracle, gpu):
     r = price_basket_mc_with_factor_model(BasketOption(w, 100.0, 1.0, True), spots, vols, short,
                                           0.02, divs, 100)
     assert math.isnan(r.price) and "asset count must match" in _lib.last_error()
+
+
+@pytest.mark.parametrize("mode", ["reference", "fast"])
+def test_fused_sensitivities_match_scenario_by_scenario(gpu, monkeypatch, mode):
+    """The fused pass (every bump on each path's shared draws) prices each
+    scenario as pricing it on its own does."""
+    from openferric_amd import RngMode, basket_price_raw
+    from openferric_amd.engines import _Keep
+    rng_mode = RngMode.Reference if mode == "reference" else RngMode.Fast
+    cases = [CASES[1], CASES[3], CASES[4], CASES[6]]
+    fused, fsens = basket_price_raw(_structs(cases, _Keep()), rng_mode, None, True)
+    monkeypatch.setenv("OFG_BASKET_FUSED", "0")
+    alone, asens = basket_price_raw(_structs(cases, _Keep()), rng_mode, None, True)
+    for f, a, fs, as_ in zip(fused, alone, fsens, asens):
+        assert abs(f.price - a.price) <= 1e-12 * abs(a.price) + 1e-15
+        assert abs(f.stderr_ - a.stderr_) <= 1e-9 * a.stderr_
+        assert fs.ok and as_.ok and fs.n_delta == as_.n_delta
+        for k in range(fs.n_delta):
+            assert abs(fs.delta[k] - as_.delta[k]) <= 1e-9 * max(1.0, abs(as_.delta[k]))
+        assert abs(fs.vega - as_.vega) <= 1e-9 * max(1.0, abs(as_.vega))
+        assert abs(fs.cega - as_.cega) <= 1e-9 * max(1.0, abs(as_.cega))
