@@ -139,13 +139,16 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
 }
 
 // The product's value on one simulated path; err receives an out-of-range
-// price() index of an active lane.
+// price() index of an active lane. The register file lives in LDS, register
+// i of thread t at rf[i * DSL_BLOCK] with rf = base + t (uniform register
+// indices: conflict-free 8-byte rows); the if / else mask stack is two bit
+// fields.
+#define R(i) rf[(int)(i) * DSL_BLOCK]
 __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double *snap,
-                                int64_t stride, double &err) {
+                                int64_t stride, double *rf, double &err) {
     const int n = o.n_assets;
-    double r[DSL_MAX_REGS];
-    bool saved[DSL_MAX_MASKS], cond[DSL_MAX_MASKS];
-    for (int k = 0; k < a.n_state; ++k) r[a.n_locals + k] = a.state_init[k];
+    uint32_t saved = 0, cond = 0;
+    for (int k = 0; k < a.n_state; ++k) R(a.n_locals + k) = a.state_init[k];
     bool alive = true;
     double pv = 0.0;
     for (int si = 0; si < a.n_sched; ++si) {
@@ -155,42 +158,42 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
             const DslObs ob = a.obs[sc.obs_first + oi];
             const double df = a.df[o.df_off + sc.obs_first + oi];
             const double *row = snap + (int64_t)ob.snap * n * stride;
-            for (int k = 0; k < a.n_locals; ++k) r[k] = 0.0;
+            for (int k = 0; k < a.n_locals; ++k) R(k) = 0.0;
             bool active = alive;
             for (int pc = sc.code_off; pc < sc.code_off + sc.code_len; ++pc) {
                 const DslIns in = a.code[pc];
                 const int d = in.dst;
                 switch (in.op) {
-                case DOP_CONST: r[d] = a.consts[in.aux]; break;
-                case DOP_NOTIONAL: r[d] = a.notional; break;
-                case DOP_DATE: r[d] = ob.date; break;
-                case DOP_ISFINAL: r[d] = ob.is_final ? 1.0 : 0.0; break;
+                case DOP_CONST: R(d) = a.consts[in.aux]; break;
+                case DOP_NOTIONAL: R(d) = a.notional; break;
+                case DOP_DATE: R(d) = ob.date; break;
+                case DOP_ISFINAL: R(d) = ob.is_final ? 1.0 : 0.0; break;
                 case DOP_SET:
-                    if (in.b || active) r[d] = r[in.a];
+                    if (in.b || active) R(d) = R(in.a);
                     break;
-                case DOP_ADD: r[d] = r[in.a] + r[in.b]; break;
-                case DOP_SUB: r[d] = r[in.a] - r[in.b]; break;
-                case DOP_MUL: r[d] = r[in.a] * r[in.b]; break;
+                case DOP_ADD: R(d) = R(in.a) + R(in.b); break;
+                case DOP_SUB: R(d) = R(in.a) - R(in.b); break;
+                case DOP_MUL: R(d) = R(in.a) * R(in.b); break;
                 case DOP_DIV: {
-                    const double den = r[in.b];
-                    r[d] = den == 0.0 ? NAN : r[in.a] / den;
+                    const double den = R(in.b);
+                    R(d) = den == 0.0 ? NAN : R(in.a) / den;
                     break;
                 }
-                case DOP_MIN: r[d] = fmin(r[in.a], r[in.b]); break;
-                case DOP_MAX: r[d] = fmax(r[in.a], r[in.b]); break;
-                case DOP_EQ: r[d] = fabs(r[in.a] - r[in.b]) < 2.220446049250313e-16 ? 1.0 : 0.0; break;
-                case DOP_NE: r[d] = fabs(r[in.a] - r[in.b]) >= 2.220446049250313e-16 ? 1.0 : 0.0; break;
-                case DOP_LT: r[d] = r[in.a] < r[in.b] ? 1.0 : 0.0; break;
-                case DOP_LE: r[d] = r[in.a] <= r[in.b] ? 1.0 : 0.0; break;
-                case DOP_GT: r[d] = r[in.a] > r[in.b] ? 1.0 : 0.0; break;
-                case DOP_GE: r[d] = r[in.a] >= r[in.b] ? 1.0 : 0.0; break;
-                case DOP_AND: r[d] = (r[in.a] != 0.0 && r[in.b] != 0.0) ? 1.0 : 0.0; break;
-                case DOP_OR: r[d] = (r[in.a] != 0.0 || r[in.b] != 0.0) ? 1.0 : 0.0; break;
-                case DOP_NEG: r[d] = -r[in.a]; break;
-                case DOP_ABS: r[d] = fabs(r[in.a]); break;
-                case DOP_EXP: r[d] = exp(r[in.a]); break;
-                case DOP_LOG: r[d] = log(r[in.a]); break;
-                case DOP_NOT: r[d] = r[in.a] != 0.0 ? 0.0 : 1.0; break;
+                case DOP_MIN: R(d) = fmin(R(in.a), R(in.b)); break;
+                case DOP_MAX: R(d) = fmax(R(in.a), R(in.b)); break;
+                case DOP_EQ: R(d) = fabs(R(in.a) - R(in.b)) < 2.220446049250313e-16 ? 1.0 : 0.0; break;
+                case DOP_NE: R(d) = fabs(R(in.a) - R(in.b)) >= 2.220446049250313e-16 ? 1.0 : 0.0; break;
+                case DOP_LT: R(d) = R(in.a) < R(in.b) ? 1.0 : 0.0; break;
+                case DOP_LE: R(d) = R(in.a) <= R(in.b) ? 1.0 : 0.0; break;
+                case DOP_GT: R(d) = R(in.a) > R(in.b) ? 1.0 : 0.0; break;
+                case DOP_GE: R(d) = R(in.a) >= R(in.b) ? 1.0 : 0.0; break;
+                case DOP_AND: R(d) = (R(in.a) != 0.0 && R(in.b) != 0.0) ? 1.0 : 0.0; break;
+                case DOP_OR: R(d) = (R(in.a) != 0.0 || R(in.b) != 0.0) ? 1.0 : 0.0; break;
+                case DOP_NEG: R(d) = -R(in.a); break;
+                case DOP_ABS: R(d) = fabs(R(in.a)); break;
+                case DOP_EXP: R(d) = exp(R(in.a)); break;
+                case DOP_LOG: R(d) = log(R(in.a)); break;
+                case DOP_NOT: R(d) = R(in.a) != 0.0 ? 0.0 : 1.0; break;
                 case DOP_WOF_PERF:
                 case DOP_BOF_PERF: {
                     const bool worst = in.op == DOP_WOF_PERF;
@@ -201,7 +204,7 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
                             if (worst ? perf < v : perf > v) v = perf;
                         }
                     }
-                    r[d] = v;
+                    R(d) = v;
                     break;
                 }
                 case DOP_WORST:
@@ -209,26 +212,26 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
                     const bool worst = in.op == DOP_WORST;
                     double v = worst ? INFINITY : -INFINITY;
                     for (int k = (int)in.aux - 1; k >= 0; --k) {   // popped last first
-                        const double x = r[in.a + k];
+                        const double x = R(in.a + k);
                         if (worst ? x < v : x > v) v = x;
                     }
-                    r[d] = v;
+                    R(d) = v;
                     break;
                 }
                 case DOP_PRICE: {
                     int idx;
                     double shown;
-                    const bool ok = index_in_range(r[in.a], n, idx, shown);
+                    const bool ok = index_in_range(R(in.a), n, idx, shown);
                     if (!ok && active) err = fmax(err, shown);
-                    r[d] = ok ? row[idx * stride] : 0.0;
+                    R(d) = ok ? row[idx * stride] : 0.0;
                     break;
                 }
                 case DOP_PAY:
-                    if (active) pv += r[in.a] * df;
+                    if (active) pv += R(in.a) * df;
                     break;
                 case DOP_REDEEM:
                     if (active) {
-                        pv += r[in.a] * df;
+                        pv += R(in.a) * df;
                         alive = false;
                         active = false;
                     }
@@ -239,13 +242,18 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
                         active = false;
                     }
                     break;
-                case DOP_IF:
-                    saved[in.aux] = active;
-                    cond[in.aux] = r[in.a] != 0.0;
-                    active = active && cond[in.aux];
+                case DOP_IF: {
+                    const uint32_t bit = 1u << in.aux;
+                    const bool c = R(in.a) != 0.0;
+                    saved = active ? (saved | bit) : (saved & ~bit);
+                    cond = c ? (cond | bit) : (cond & ~bit);
+                    active = active && c;
                     break;
-                case DOP_ELSE: active = saved[in.aux] && !cond[in.aux] && alive; break;
-                case DOP_ENDIF: active = saved[in.aux] && alive; break;
+                }
+                case DOP_ELSE:
+                    active = ((saved >> in.aux) & 1u) && !((cond >> in.aux) & 1u) && alive;
+                    break;
+                case DOP_ENDIF: active = ((saved >> in.aux) & 1u) && alive; break;
                 default: break;
                 }
             }
@@ -253,9 +261,11 @@ __device__ double evaluate_path(const DslArgs &a, const DslScen &o, const double
     }
     return pv;
 }
+#undef R
 
 template <int RNG, int MAXA>
 __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
+    extern __shared__ double regs[];   // the programs' register files, [reg][thread]
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     __shared__ double red[NWAVES][3];
     if constexpr (RNG == OFG_RNG_FAST) {
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
         simulate_path<RNG, MAXA>(a, o, p, snap, DSL_BLOCK, tab);
-        const double x = evaluate_path(a, o, snap, DSL_BLOCK, err);
+        const double x = evaluate_path(a, o, snap, DSL_BLOCK, regs + threadIdx.x, err);
         sum += x;
         sq += x * x;
     }
@@ -316,8 +326,123 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
     }
 }
 
+// Split form, pass 1 for several markets at once (their Greeks bumps): one
+// thread per path draws each step's normals once and moves every market's
+// spots; a market's GBM asset whose step equals market 0's (share_mask)
+// multiplies by market 0's factor -- the reference forms the same product --
+// and a market with its own Cholesky factor correlates the same draws.
+// Launched over market 0's tiles; writes every market's snapshots.
+template <int RNG, int MAXA>
+__global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += DSL_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    }
+    const DslScen &o = a.scen[0];
+    const int n = o.n_assets, ns = a.n_scen;
+    const int64_t base = (int64_t)blockIdx.x * DSL_BLOCK * o.ppt;
+    for (int j = 0; j < o.ppt; ++j) {
+        const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
+        if (p >= o.paths) break;
+        double s[DSL_FUSE_MAXS][MAXA];
+#pragma unroll
+        for (int m = 0; m < DSL_FUSE_MAXS; ++m)
+#pragma unroll
+            for (int i = 0; i < MAXA; ++i) s[m][i] = m < ns ? a.scen[m].s0[i] : 0.0;
+        const int k0 = a.step_snap[0];
+        if (k0 >= 0) {
+#pragma unroll
+            for (int m = 0; m < DSL_FUSE_MAXS; ++m)
+                if (m < ns)
+#pragma unroll
+                    for (int i = 0; i < MAXA; ++i)
+                        if (i < n)
+                            a.snaps[a.scen[m].snap_off + (int64_t)(k0 * n + i) * o.paths + p] = s[m][i];
+        }
+        Xoshiro256pp ref;
+        Xoroshiro128p fast;
+        if constexpr (RNG == OFG_RNG_REFERENCE) {
+            const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
+            ref.s0 = sd[0];
+            ref.s1 = sd[1];
+            ref.s2 = sd[2];
+            ref.s3 = sd[3];
+        }
+        if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
+        const double *zp = a.normals + o.normals_offset + p;
+        for (int step = 1; step <= a.steps; ++step) {
+            double z[MAXA], c0[MAXA], g0[MAXA];
+            double zpair = 0.0;
+#pragma unroll
+            for (int j2 = 0; j2 < MAXA; ++j2) {
+                if (j2 < n) {
+                    if constexpr (RNG == OFG_RNG_REFERENCE) {
+                        z[j2] = reference_normal(ref);
+                    } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
+                        z[j2] = zp[((int64_t)(step - 1) * n + j2) * o.paths];
+                    } else {
+                        if ((j2 & 1) == 0) {
+                            const uint64_t wu = fast.next_u64();
+                            const uint64_t wt = fast.next_u64();
+                            box_muller_pair(wu, (uint32_t)(wt >> 32), z[j2], zpair, tab);
+                        } else {
+                            z[j2] = zpair;
+                        }
+                    }
+                } else {
+                    z[j2] = 0.0;
+                }
+            }
+            // market 0: correlated normals and GBM factors
+#pragma unroll
+            for (int i = 0; i < MAXA; ++i) c0[i] = 0.0;
+#pragma unroll
+            for (int j2 = 0; j2 < MAXA; ++j2)
+#pragma unroll
+                for (int i = j2; i < MAXA; ++i)
+                    if (i < n && j2 < n) c0[i] += o.chol[i * (i + 1) / 2 + j2] * z[j2];
+#pragma unroll
+            for (int i = 0; i < MAXA; ++i)
+                g0[i] = (i < n && o.kind[i] == DSL_GBM) ? exp_fp64(o.p0[i] + o.p1[i] * c0[i]) : 1.0;
+            const int ks = a.step_snap[step];
+#pragma unroll
+            for (int m = 0; m < DSL_FUSE_MAXS; ++m) {
+                if (m >= ns) continue;
+                const DslScen &q = a.scen[m];
+                double c[MAXA];
+                if (q.own_corr) {
+#pragma unroll
+                    for (int i = 0; i < MAXA; ++i) c[i] = 0.0;
+#pragma unroll
+                    for (int j2 = 0; j2 < MAXA; ++j2)
+#pragma unroll
+                        for (int i = j2; i < MAXA; ++i)
+                            if (i < n && j2 < n) c[i] += q.chol[i * (i + 1) / 2 + j2] * z[j2];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < MAXA; ++i) c[i] = c0[i];
+                }
+#pragma unroll
+                for (int i = 0; i < MAXA; ++i) {
+                    if (i < n) {
+                        if ((q.share_mask >> i) & 1u) s[m][i] = s[m][i] * g0[i];
+                        else s[m][i] = step_asset(q.kind[i], s[m][i], q.p0[i], q.p1[i], q.p2[i], c[i]);
+                    }
+                }
+                if (ks >= 0) {
+#pragma unroll
+                    for (int i = 0; i < MAXA; ++i)
+                        if (i < n) a.snaps[q.snap_off + (int64_t)(ks * n + i) * o.paths + p] = s[m][i];
+                }
+            }
+        }
+    }
+}
+
 // Split form, pass 2: the product on every path; tile sums as the fused form.
 __global__ __launch_bounds__(DSL_BLOCK) void dsl_eval_kernel(DslArgs a) {
+    extern __shared__ double regs[];   // the programs' register files, [reg][thread]
     __shared__ double red[NWAVES][3];
     const int tile = blockIdx.x;
     const DslScen &o = a.scen[a.tile_scen[tile]];
@@ -326,7 +451,7 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_eval_kernel(DslArgs a) {
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        const double x = evaluate_path(a, o, a.snaps + o.snap_off + p, o.paths, err);
+        const double x = evaluate_path(a, o, a.snaps + o.snap_off + p, o.paths, regs + threadIdx.x, err);
         sum += x;
         sq += x * x;
     }
@@ -388,13 +513,24 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_close_kernel(DslArgs a) {
 
 }  // namespace
 
+size_t lds_bytes(const DslArgs &a) {
+    return (size_t)(a.n_regs > 0 ? a.n_regs : 1) * DSL_BLOCK * sizeof(double);
+}
+
 template <int RNG, int MAXA>
 void launch_rng(const DslArgs &a, hipStream_t stream) {
     if (a.split) {
-        dsl_sim_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
-        dsl_eval_kernel<<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
+        if constexpr (MAXA <= DSL_FUSE_MAXA) {
+            if (a.fuse_markets)
+                dsl_sim_markets_kernel<RNG, MAXA><<<a.fuse_tiles, DSL_BLOCK, 0, stream>>>(a);
+            else
+                dsl_sim_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
+        } else {
+            dsl_sim_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
+        }
+        dsl_eval_kernel<<<a.n_tiles, DSL_BLOCK, lds_bytes(a), stream>>>(a);
     } else {
-        dsl_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
+        dsl_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, lds_bytes(a), stream>>>(a);
     }
 }
 

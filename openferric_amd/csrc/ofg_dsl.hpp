@@ -20,9 +20,11 @@
 namespace ofg {
 
 constexpr int DSL_MAX_ASSETS = 16;
-constexpr int DSL_MAX_REGS = 64;
+constexpr int DSL_MAX_REGS = 32;   // 64 KB of LDS per 256-thread block at the limit
 constexpr int DSL_MAX_MASKS = 16;
 constexpr int DSL_BLOCK = 256;
+constexpr int DSL_FUSE_MAXS = 9;    // markets simulated together per path
+constexpr int DSL_FUSE_MAXA = 4;    // assets per market in the fused simulation
 
 enum DslOp : uint8_t {
     DOP_CONST = 0,    // r[dst] = consts[aux]
@@ -76,8 +78,12 @@ struct alignas(16) DslScen {
     int64_t seed_offset;    // reference-stream start states [path][word]
     uint64_t seed;          // fast draws: stream_seed(seed, path)
     int32_t df_off;         // discount factors of the observations, exp(-r t)
-    int32_t pad;
+    int32_t own_corr;       // fused simulation: 1 when the Cholesky factor differs from market 0's
     int64_t snap_off;       // split form: this scenario's [snap][asset][path] snapshots
+    // fused simulation: bit i set when asset i's step is market 0's GBM step
+    // (same parameters and correlated normal), so its factor is shared
+    uint32_t share_mask;
+    uint32_t pad2;
     int32_t kind[DSL_MAX_ASSETS];
     double s0[DSL_MAX_ASSETS];
     // GBM: p0 drift, p1 diffusion; Schwartz: p0 long-run log mean,
@@ -124,7 +130,9 @@ struct DslArgs {
     int32_t rng_mode;
     int32_t n_assets;           // of every scenario
     int32_t split;              // 1: simulation and evaluation kernels, 0: fused
-    int32_t pad2;
+    int32_t fuse_markets;       // split form: 1 = one simulation pass for all markets
+    int32_t fuse_tiles;         // its grid: market 0's tiles
+    int32_t n_regs;             // program registers (LDS per thread)
     const double *state_init;   // [n_state]
 };
 

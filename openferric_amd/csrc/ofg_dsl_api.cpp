@@ -166,7 +166,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
     }
     n_regs = std::max<int32_t>(n_regs, n_locals + (int32_t)prod.state_vars.size());
     if (n_regs > ofg::DSL_MAX_REGS || n_masks > ofg::DSL_MAX_MASKS)
-        fail(OFG_INVALID_INPUT, "product program exceeds the device limits (64 registers, 16 nested ifs)");
+        fail(OFG_INVALID_INPUT, "product program exceeds the device limits (32 registers, 16 nested ifs)");
     std::vector<double> state_init;
     for (const auto &sv : prod.state_vars) state_init.push_back(sv.initial);
 
@@ -251,6 +251,24 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         d.tile_first = (int32_t)tiles.size();
         for (int32_t t = 0; t < d.n_tiles; ++t) tiles.push_back((int32_t)s);
     }
+    // several markets (Greeks bumps) in one simulation pass when they fit the
+    // fused kernel: shared draws, shared correlation and GBM factors where a
+    // market's step equals market 0's (OFG_DSL_FUSE=0 simulates them apart)
+    const char *fenv = std::getenv("OFG_DSL_FUSE");
+    const bool fuse = split && n_mk >= 2 && n_mk <= (size_t)ofg::DSL_FUSE_MAXS &&
+                      na <= ofg::DSL_FUSE_MAXA && !(fenv && std::atoi(fenv) == 0);
+    for (size_t sidx = 0; sidx < n_mk; ++sidx) {
+        ofg::DslScen &d = scen[sidx];
+        const ofg::DslScen &d0 = scen[0];
+        d.own_corr = std::memcmp(d.chol, d0.chol, sizeof d.chol) != 0;
+        d.share_mask = 0;
+        for (int i = 0; i < na; ++i) {
+            const bool same = !d.own_corr && d.kind[i] == ofg::DSL_GBM && d0.kind[i] == ofg::DSL_GBM &&
+                              std::memcmp(&d.p0[i], &d0.p0[i], sizeof(double)) == 0 &&
+                              std::memcmp(&d.p1[i], &d0.p1[i], sizeof(double)) == 0;
+            if (same) d.share_mask |= 1u << i;
+        }
+    }
     if (rng == OFG_RNG_EXTERNAL && !cfg->normals1)
         fail(OFG_INVALID_INPUT, "external rng mode needs normals1");
 
@@ -312,6 +330,9 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         a.rng_mode = rng;
         a.n_assets = na;
         a.split = split ? 1 : 0;
+        a.fuse_markets = fuse ? 1 : 0;
+        a.fuse_tiles = scen[0].n_tiles;
+        a.n_regs = n_regs;
         if (steps > INT32_MAX) fail(OFG_INVALID_INPUT, "num_steps exceeds the device limit");
         hip_check(hipEventRecord(ctx->ev0, st), "event");
         if (rng == OFG_RNG_REFERENCE)

@@ -176,6 +176,10 @@ def test_errors_and_masked_index(gpu):
         _engine(0, 4, RngMode.Reference).price_multi_asset(parse_and_compile(P.CALL), one)
     with pytest.raises(InvalidInput, match="correlation matrix diagonal must be 1"):
         eng.price_multi_asset(parse_and_compile(P.CALL), MultiAssetMarket(one.assets, [[0.9]], 0.0))
+    deep = "(1 + " * 40 + "1" + ")" * 40   # 41 stack slots: beyond the device's 32 registers
+    with pytest.raises(InvalidInput, match="exceeds the device limits"):
+        eng.price_multi_asset(parse_and_compile(P.CALL.replace("max(price(0) - 100, 0)", deep)),
+                              one)
 
 
 def test_fast_draws_call_matches_black_scholes(gpu):
@@ -210,3 +214,37 @@ def test_fused_form_matches_split_form(gpu, monkeypatch):
     fused = _engine(paths, steps, RngMode.Reference, seed=3).price_multi_asset(prod, mkt)
     assert abs(fused.price - split.price) <= 1e-12 * abs(split.price)
     assert abs(fused.stderr - split.stderr) <= 1e-9 * split.stderr
+
+
+@pytest.mark.parametrize("case", [0, 2])
+def test_markets_fused_simulation_matches_separate(gpu, monkeypatch, case):
+    """Greeks bumps simulated in one pass (shared draws and factors) price as
+    markets simulated one by one -- equity, FX, Schwartz and Vasicek assets,
+    spot / vol / rate / correlation bumps."""
+    from openferric_amd import MultiAssetMarket, RngMode, parse_and_compile
+    from openferric_amd.dsl import with_spot_bump, with_vol_bump
+    src, spec, paths, steps = CASES[case]
+    mkt, *_ = _mk(spec)
+    prod = parse_and_compile(src)
+    n = len(mkt.assets)
+    corr_up = [list(r) for r in mkt.correlation]
+    corr_up[0][1] = corr_up[1][0] = min(corr_up[0][1] + 0.01, 1.0)
+    bumps = [mkt]
+    for i in range(n):
+        a = list(mkt.assets)
+        a[i] = with_spot_bump(a[i], 0.01 * a[i].initial_value())
+        bumps.append(MultiAssetMarket(a, mkt.correlation, mkt.rate))
+    a = list(mkt.assets)
+    a[0] = with_vol_bump(a[0], 0.01)
+    bumps.append(MultiAssetMarket(a, mkt.correlation, mkt.rate))
+    bumps.append(MultiAssetMarket(mkt.assets, mkt.correlation, mkt.rate + 0.0001))
+    bumps.append(MultiAssetMarket(mkt.assets, corr_up, mkt.rate))
+    bumps = bumps[:9]
+    for mode in (RngMode.Reference, RngMode.Fast):
+        eng = _engine(paths, steps, mode, seed=5)
+        fused = [r.price for r in eng.price_markets(prod, bumps)]
+        monkeypatch.setenv("OFG_DSL_FUSE", "0")
+        apart = [r.price for r in eng.price_markets(prod, bumps)]
+        monkeypatch.delenv("OFG_DSL_FUSE")
+        for f, s_ in zip(fused, apart):
+            assert abs(f - s_) <= 1e-12 * abs(s_) + 1e-12, (f, s_)
