@@ -368,6 +368,305 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_close_kernel(BkArgs a) {
 }
 
 // ------------------------------------------------------------------------
+// autocallable sensitivities fused per path
+// ------------------------------------------------------------------------
+
+template <int RNG, int MAXN>
+__device__ __forceinline__ void acg_move(const AcGreeksDev &o, int set, const double *c, double *sp,
+                                         const double *tab) {
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        if (i < o.n) {
+            double g;
+            if constexpr (RNG == OFG_RNG_FAST) {
+                double y = o.drift2[set][i] + o.vol_dt2[set][i] * c[i];
+                y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
+                g = exp2_256(y, tab);
+            } else {
+                g = exp_fp64(o.drift[set][i] + o.vol_dt[set][i] * c[i]);
+            }
+            sp[i] = fmax(sp[i] * g, 1.0e-12);
+        }
+    }
+}
+
+template <int MAXN>
+__device__ __forceinline__ void acg_correlate(const AcGreeksDev &o, int set, const double *z,
+                                              double *c) {
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j <= i; ++j)
+            if (i < o.n) acc += o.chol[set][i * (i + 1) / 2 + j] * z[j];
+        c[i] = acc;
+    }
+}
+
+template <int MAXN>
+__device__ __forceinline__ double acg_worst(const AcGreeksDev &o, const double *sp) {
+    double w = INFINITY;
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i)
+        if (i < o.n) w = fmin(w, sp[i] / o.s0[i]);
+    return w;
+}
+
+template <int RNG, int MAXN>
+__global__ __launch_bounds__(MA_BLOCK) void autocall_greeks_kernel(AcgArgs a) {
+    constexpr int MAXS = 5 + 2 * MAXN;
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    __shared__ double red[NWAVES][ACG_MAXS][2];
+    if constexpr (RNG == OFG_RNG_FAST) {
+        for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += MA_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    }
+    const int tile = blockIdx.x;
+    const AcGreeksDev &o = a.ct[a.tile_ct[tile]];
+    const AcObs *obs = a.obs + o.obs_offset;
+    const int n = o.n, ns = o.n_scen;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    const int64_t base = (int64_t)(tile - o.tile_first) * MA_BLOCK * o.ppt;
+    // scenarios of this contract: 0..4 always present in the layout, 3 and 4
+    // only with correlation bumps, 5 + 2q / 6 + 2q for q < n
+    uint32_t present = 0x7u | (o.has_cega ? 0x18u : 0u);
+    for (int q = 0; q < n; ++q) present |= 3u << (5 + 2 * q);
+    double acc = 0.0, acc_sq = 0.0;
+    for (int j = 0; j < o.ppt; ++j) {
+        const int64_t p = base + (int64_t)j * MA_BLOCK + threadIdx.x;
+        const bool valid = p < o.paths;
+        if (!__any(valid)) break;
+        double pv[MAXS], pend[MAXS];
+#pragma unroll
+        for (int s = 0; s < MAXS; ++s) {
+            pv[s] = 0.0;
+            pend[s] = 0.0;
+        }
+        uint32_t called = valid ? ~present : 0xffffffffu, ki = 0u;
+        double sb[MAXN], sv[2][MAXN], sc[2][MAXN], su[MAXN], sd[MAXN];
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) {
+            sb[i] = sv[0][i] = sv[1][i] = sc[0][i] = sc[1][i] = o.s0[i];
+            su[i] = o.s0_up[i];
+            sd[i] = o.s0_dn[i];
+        }
+        Xoroshiro128p rng;
+        if constexpr (RNG == OFG_RNG_FAST) rng.seed(stream_seed(o.seed, (uint64_t)p));
+        const double *zp = a.normals + o.normals_offset + p;
+        int k = 0;
+        int next_step = o.n_obs > 0 ? obs[0].step : 0x7fffffff;
+        double prev_t = 0.0;
+        for (int step = 1; step <= o.steps && called != 0xffffffffu; ++step) {
+            double z[MAXN];
+            if constexpr (RNG == OFG_RNG_FAST) {
+#pragma unroll
+                for (int i = 0; i < MAXN; i += 2) {
+                    if (i < n) {
+                        const uint64_t wu = rng.next_u64();
+                        const uint64_t wt = rng.next_u64();
+                        double z1;
+                        box_muller_pair(wu, (uint32_t)(wt >> 32), z[i], z1, tab);
+                        if (i + 1 < MAXN) z[i + 1] = z1;
+                    }
+                }
+            } else {
+                const int64_t row = (int64_t)(step - 1) * n;
+#pragma unroll
+                for (int i = 0; i < MAXN; ++i)
+                    z[i] = i < n ? zp[(row + i) * o.paths] : 0.0;
+            }
+            double c[MAXN];
+            acg_correlate<MAXN>(o, 0, z, c);
+            // base factors, shared by the base spots and the spot-bumped ones
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i) {
+                if (i < n) {
+                    double f;
+                    if constexpr (RNG == OFG_RNG_FAST) {
+                        double y = o.drift2[0][i] + o.vol_dt2[0][i] * c[i];
+                        y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
+                        f = exp2_256(y, tab);
+                    } else {
+                        f = exp_fp64(o.drift[0][i] + o.vol_dt[0][i] * c[i]);
+                    }
+                    sb[i] = fmax(sb[i] * f, 1.0e-12);
+                    su[i] = fmax(su[i] * f, 1.0e-12);
+                    sd[i] = fmax(sd[i] * f, 1.0e-12);
+                }
+            }
+            acg_move<RNG, MAXN>(o, 1, c, sv[0], tab);
+            acg_move<RNG, MAXN>(o, 2, c, sv[1], tab);
+            if (o.has_cega) {
+                acg_correlate<MAXN>(o, 1, z, c);
+                acg_move<RNG, MAXN>(o, 0, c, sc[0], tab);
+                acg_correlate<MAXN>(o, 2, z, c);
+                acg_move<RNG, MAXN>(o, 0, c, sc[1], tab);
+            }
+            const bool obs_now = step == next_step;
+            AcObs ob{};
+            double accrual = 0.0;
+            if (obs_now) {
+                ob = obs[k];
+                accrual = o.notional * o.coupon_rate * fmax(ob.time - prev_t, 0.0);
+            }
+            // knock-in and observation of scenario s with worst performance w
+            auto visit = [&](int s, double w) {
+                const uint32_t bit = 1u << s;
+                if (called & bit) return;
+                if (w <= o.ki_barrier) ki |= bit;
+                if (!obs_now) return;
+                if (o.phoenix) {
+                    if (w >= o.coupon_barrier) {
+                        const double due = o.memory ? pend[s] + accrual : accrual;
+                        pv[s] += ob.df * due;
+                        pend[s] = 0.0;
+                    } else if (o.memory) {
+                        pend[s] += accrual;
+                    }
+                }
+                if (w >= o.autocall_barrier) {
+                    if (o.phoenix) pv[s] += ob.df * o.notional;
+                    else pv[s] = ob.call_pv;
+                    called |= bit;
+                }
+            };
+            double rb[MAXN];
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i) rb[i] = i < n ? sb[i] / o.s0[i] : INFINITY;
+            double w0 = INFINITY;
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i) w0 = fmin(w0, rb[i]);
+            visit(0, w0);
+            visit(1, acg_worst<MAXN>(o, sv[0]));
+            visit(2, acg_worst<MAXN>(o, sv[1]));
+            if (o.has_cega) {
+                visit(3, acg_worst<MAXN>(o, sc[0]));
+                visit(4, acg_worst<MAXN>(o, sc[1]));
+            }
+#pragma unroll
+            for (int q = 0; q < MAXN; ++q) {
+                if (q < n) {
+                    double wu = INFINITY, wd = INFINITY;
+#pragma unroll
+                    for (int i = 0; i < MAXN; ++i) {
+                        wu = fmin(wu, i == q ? su[i] / o.s0_up[i] : rb[i]);
+                        wd = fmin(wd, i == q ? sd[i] / o.s0_dn[i] : rb[i]);
+                    }
+                    visit(5 + 2 * q, wu);
+                    visit(6 + 2 * q, wd);
+                }
+            }
+            if (obs_now) {
+                if (o.phoenix) prev_t = ob.time;
+                ++k;
+                next_step = k < o.n_obs ? obs[k].step : 0x7fffffff;
+            }
+        }
+        // each scenario's present value (redemption of those never called, on
+        // the last step's worst) into lane s's sums
+        auto finish = [&](int s, double wf) {
+            double x = 0.0;
+            if (valid) {
+                double v = pv[s];
+                if (!((called >> s) & 1u)) {
+                    double redemption;
+                    if ((ki >> s) & 1u) redemption = o.notional * (wf / o.ki_strike);
+                    else if (!o.phoenix) redemption = o.redeem_plain;
+                    else redemption = o.notional;
+                    v += o.df_maturity * redemption;
+                }
+                x = v - o.shift;
+            }
+            const double sx = wave_sum(x), sq = wave_sum(x * x);
+            if (lane == s) {
+                acc += sx;
+                acc_sq += sq;
+            }
+        };
+        double rb[MAXN];
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) rb[i] = i < n ? sb[i] / o.s0[i] : INFINITY;
+        double w0 = INFINITY;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) w0 = fmin(w0, rb[i]);
+        finish(0, w0);
+        finish(1, acg_worst<MAXN>(o, sv[0]));
+        finish(2, acg_worst<MAXN>(o, sv[1]));
+        if (o.has_cega) {
+            finish(3, acg_worst<MAXN>(o, sc[0]));
+            finish(4, acg_worst<MAXN>(o, sc[1]));
+        }
+#pragma unroll
+        for (int q = 0; q < MAXN; ++q) {
+            if (q < n) {
+                double wu = INFINITY, wd = INFINITY;
+#pragma unroll
+                for (int i = 0; i < MAXN; ++i) {
+                    wu = fmin(wu, i == q ? su[i] / o.s0_up[i] : rb[i]);
+                    wd = fmin(wd, i == q ? sd[i] / o.s0_dn[i] : rb[i]);
+                }
+                finish(5 + 2 * q, wu);
+                finish(6 + 2 * q, wd);
+            }
+        }
+    }
+    if (lane < ACG_MAXS) {
+        red[wave][lane][0] = acc;
+        red[wave][lane][1] = acc_sq;
+    }
+    __syncthreads();
+    if (threadIdx.x < ns) {
+        double sm = red[0][threadIdx.x][0], q = red[0][threadIdx.x][1];
+        for (int w2 = 1; w2 < NWAVES; ++w2) {
+            sm += red[w2][threadIdx.x][0];
+            q += red[w2][threadIdx.x][1];
+        }
+        a.partials[(int64_t)tile * ACG_MAXS + threadIdx.x].sum = sm;
+        a.partials[(int64_t)tile * ACG_MAXS + threadIdx.x].sum_sq = q;
+    }
+}
+
+// One block per (contract, scenario): the tile slots in tile order.
+__global__ __launch_bounds__(MA_BLOCK) void autocall_greeks_close_kernel(AcgArgs a) {
+    __shared__ double red[NWAVES][2];
+    const AcGreeksDev &o = a.ct[blockIdx.x];
+    const int sc = blockIdx.y;
+    if (sc >= o.n_scen) return;
+    double s = 0.0, q = 0.0;
+    for (int t = threadIdx.x; t < o.n_tiles; t += MA_BLOCK) {
+        const AcPartial pt = a.partials[(int64_t)(o.tile_first + t) * ACG_MAXS + sc];
+        s += pt.sum;
+        q += pt.sum_sq;
+    }
+    s = wave_sum(s);
+    q = wave_sum(q);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[wave][0] = s;
+        red[wave][1] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double sum = red[0][0], sq = red[0][1];
+    for (int w = 1; w < NWAVES; ++w) {
+        sum += red[w][0];
+        sq += red[w][1];
+    }
+    const double nn = (double)o.paths;
+    const double var = o.paths > 1 ? fmax((sq - sum * sum / nn) / (nn - 1.0), 0.0) : 0.0;
+    a.out[o.out_first + sc].price = o.shift + sum / nn;
+    a.out[o.out_first + sc].stderr_ = sqrt(var / nn);
+}
+
+template <int MAXN>
+void launch_acg(const AcgArgs &a, hipStream_t stream) {
+    if (a.rng_mode == OFG_RNG_FAST)
+        autocall_greeks_kernel<OFG_RNG_FAST, MAXN><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+    else
+        autocall_greeks_kernel<OFG_RNG_EXTERNAL, MAXN><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+}
+
+// ------------------------------------------------------------------------
 // basket_sensitivities fused per path
 // ------------------------------------------------------------------------
 
@@ -576,6 +875,17 @@ void launch_bkg(const BkgArgs &a, hipStream_t stream) {
 }
 
 }  // namespace
+
+hipError_t launch_autocall_greeks(const AcgArgs &a, hipStream_t stream) {
+    if (a.n_tiles <= 0) return hipSuccess;
+    if (a.max_n <= 2) launch_acg<2>(a, stream);
+    else if (a.max_n <= 4) launch_acg<4>(a, stream);
+    else launch_acg<MA_MAX>(a, stream);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    autocall_greeks_close_kernel<<<dim3(a.n_ct, ACG_MAXS), MA_BLOCK, 0, stream>>>(a);
+    return hipGetLastError();
+}
 
 hipError_t launch_basket_greeks(const BkgArgs &a, hipStream_t stream) {
     if (a.n_tiles <= 0) return hipSuccess;

@@ -72,6 +72,45 @@ struct AcArgs {
 
 hipError_t launch_autocall(const AcArgs &a, hipStream_t stream);
 
+// ---- autocallable sensitivities fused per path ----
+// The 1 + 2n + 2 (+ 2) scenarios of a contract's bump-and-reprice
+// sensitivities on each path's shared draws: the spot bumps reuse the base
+// spots of the other underlyings and the base factors (a spot bump scales
+// one underlying's spot and leaves every exponent unchanged), the vol bumps
+// reuse the base correlated normals, the correlation bumps the independent
+// draws. Scenario order: 0 base; 1 / 2 vol up / down; 3 / 4 correlation up
+// / down (when present); 5 + 2k / 6 + 2k spot up / down of underlying k.
+constexpr int ACG_MAXS = 5 + 2 * MA_MAX;
+
+struct alignas(16) AcGreeksDev {
+    int32_t n, phoenix, memory, n_obs;
+    int32_t steps, obs_offset, tile_first, n_tiles;
+    int32_t ppt, n_scen, has_cega, out_first;
+    int64_t paths, normals_offset;
+    uint64_t seed;
+    double s0[MA_MAX], s0_up[MA_MAX], s0_dn[MA_MAX];
+    double drift[3][MA_MAX], vol_dt[3][MA_MAX];     // base, vol up, vol down
+    double drift2[3][MA_MAX], vol_dt2[3][MA_MAX];   // fast mode: scaled by 256/ln2
+    double chol[3][MA_MAX * (MA_MAX + 1) / 2];     // base, correlation up, down
+    double notional, autocall_barrier, coupon_rate, coupon_barrier;
+    double ki_barrier, ki_strike, df_maturity, redeem_plain, shift;
+};
+
+struct AcgArgs {
+    const AcGreeksDev *ct;
+    const int32_t *tile_ct;
+    const AcObs *obs;
+    AcPartial *partials;     // [tile][ACG_MAXS]
+    AcOut *out;
+    const double *normals;
+    int32_t n_tiles;
+    int32_t n_ct;
+    int32_t rng_mode;
+    int32_t max_n;
+};
+
+hipError_t launch_autocall_greeks(const AcgArgs &a, hipStream_t stream);
+
 // ---- basket options (src/pricing/basket.rs:599-683, Gaussian copula) ----
 constexpr int BK_MAX = 16;
 constexpr int BK_FMAX = 4;   // factors of a factor-model basket

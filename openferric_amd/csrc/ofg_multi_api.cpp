@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -29,13 +30,14 @@ using namespace ofg_host;
 namespace ofg_host {
 
 struct MultiBuffers {
-    DevBuf scen, tiles, obs, partials, out, normals;
+    DevBuf scen, tiles, obs, partials, out, normals, gct, gtiles, gpartials, gout;
 };
 
 void multi_release(ofg_ctx *ctx) {
     if (!ctx || !ctx->multi) return;
     MultiBuffers *b = ctx->multi;
-    for (DevBuf *d : {&b->scen, &b->tiles, &b->obs, &b->partials, &b->out, &b->normals})
+    for (DevBuf *d : {&b->scen, &b->tiles, &b->obs, &b->partials, &b->out, &b->normals, &b->gct,
+                      &b->gtiles, &b->gpartials, &b->gout})
         d->release();
     delete b;
     ctx->multi = nullptr;
@@ -309,20 +311,97 @@ int32_t autocall_impl(ofg_ctx *ctx, const ofg_autocall *in, size_t n, const ofg_
     if (rng == OFG_RNG_EXTERNAL && !scen.empty() && !cfg->normals1)
         fail(OFG_INVALID_INPUT, "external rng mode needs normals1");
 
-    // tiles: the size depends on the path count alone
+    // sensitivities fused per path: every scenario of a contract on the same
+    // draws in one pass (OFG_AUTOCALL_FUSED=0 prices them one by one)
+    std::vector<int32_t> fused_slot(scen.size(), -1);
+    std::vector<ofg::AcGreeksDev> gdev;
+    std::vector<int32_t> gtiles;
+    int32_t g_out = 0, max_n = 0;
+    const char *fenv = std::getenv("OFG_AUTOCALL_FUSED");
+    const bool fuse = with_greeks && !(fenv && std::atoi(fenv) == 0);
+    for (size_t i = 0; fuse && i < n; ++i) {
+        const ContractPlan &pl = plans[i];
+        if (!pl.valid || !pl.sens_ok) continue;
+        const ofg::AcDev &b0 = scen[(size_t)pl.base];
+        const ofg::AcDev &vu = scen[(size_t)pl.vega_up], &vd = scen[(size_t)pl.vega_dn];
+        ofg::AcGreeksDev g{};
+        g.n = b0.n;
+        g.phoenix = b0.phoenix;
+        g.memory = b0.memory;
+        g.n_obs = b0.n_obs;
+        g.steps = b0.steps;
+        g.obs_offset = b0.obs_offset;
+        g.has_cega = pl.cega_up >= 0;
+        g.n_scen = 5 + 2 * g.n;
+        g.paths = b0.paths;
+        g.normals_offset = b0.normals_offset;
+        g.seed = b0.seed;
+        const ofg::AcDev *sets[3] = {&b0, &vu, &vd};
+        const ofg::AcDev *csets[3] = {&b0, g.has_cega ? &scen[(size_t)pl.cega_up] : &b0,
+                                      g.has_cega ? &scen[(size_t)pl.cega_dn] : &b0};
+        for (int a = 0; a < g.n; ++a) {
+            g.s0[a] = b0.s0[a];
+            g.s0_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].s0[a];
+            g.s0_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].s0[a];
+            for (int v = 0; v < 3; ++v) {
+                g.drift[v][a] = sets[v]->drift[a];
+                g.vol_dt[v][a] = sets[v]->vol_dt[a];
+                g.drift2[v][a] = sets[v]->drift2[a];
+                g.vol_dt2[v][a] = sets[v]->vol_dt2[a];
+            }
+        }
+        for (int v = 0; v < 3; ++v) std::memcpy(g.chol[v], csets[v]->chol, sizeof g.chol[v]);
+        g.notional = b0.notional;
+        g.autocall_barrier = b0.autocall_barrier;
+        g.coupon_rate = b0.coupon_rate;
+        g.coupon_barrier = b0.coupon_barrier;
+        g.ki_barrier = b0.ki_barrier;
+        g.ki_strike = b0.ki_strike;
+        g.df_maturity = b0.df_maturity;
+        g.redeem_plain = b0.redeem_plain;
+        g.shift = b0.shift;
+        const int64_t blk = ofg::MA_BLOCK;
+        g.ppt = (int32_t)std::max<int64_t>(
+            1, (g.paths + blk * ofg::MA_MAX_TILES - 1) / (blk * ofg::MA_MAX_TILES));
+        g.n_tiles = (int32_t)((g.paths + blk * g.ppt - 1) / (blk * g.ppt));
+        g.tile_first = (int32_t)gtiles.size();
+        for (int32_t t = 0; t < g.n_tiles; ++t) gtiles.push_back((int32_t)gdev.size());
+        g.out_first = g_out;
+        g_out += ofg::ACG_MAXS;
+        fused_slot[(size_t)pl.base] = g.out_first;
+        fused_slot[(size_t)pl.vega_up] = g.out_first + 1;
+        fused_slot[(size_t)pl.vega_dn] = g.out_first + 2;
+        if (g.has_cega) {
+            fused_slot[(size_t)pl.cega_up] = g.out_first + 3;
+            fused_slot[(size_t)pl.cega_dn] = g.out_first + 4;
+        }
+        for (int k = 0; k < g.n; ++k) {
+            fused_slot[(size_t)pl.delta_up[(size_t)k]] = g.out_first + 5 + 2 * k;
+            fused_slot[(size_t)pl.delta_dn[(size_t)k]] = g.out_first + 6 + 2 * k;
+        }
+        max_n = std::max(max_n, g.n);
+        gdev.push_back(g);
+    }
+    // the remaining scenarios one by one; tiles: the size depends on the
+    // path count alone
+    std::vector<int32_t> reg_index(scen.size(), -1);
+    std::vector<ofg::AcDev> rscen;
     std::vector<int32_t> tiles;
     for (size_t s = 0; s < scen.size(); ++s) {
-        ofg::AcDev &d = scen[s];
+        if (fused_slot[s] >= 0) continue;
+        ofg::AcDev d = scen[s];
         const int64_t blk = ofg::MA_BLOCK;
         d.ppt = (int32_t)std::max<int64_t>(1, (d.paths + blk * ofg::MA_MAX_TILES - 1) /
                                                   (blk * ofg::MA_MAX_TILES));
         d.n_tiles = (int32_t)((d.paths + blk * d.ppt - 1) / (blk * d.ppt));
         d.tile_first = (int32_t)tiles.size();
-        for (int32_t t = 0; t < d.n_tiles; ++t) tiles.push_back((int32_t)s);
+        for (int32_t t = 0; t < d.n_tiles; ++t) tiles.push_back((int32_t)rscen.size());
+        reg_index[s] = (int32_t)rscen.size();
+        rscen.push_back(d);
     }
-    std::vector<ofg::AcOut> res(scen.size());
+    std::vector<ofg::AcOut> rres(rscen.size()), gres((size_t)g_out);
     float ms = 0.f;
-    if (!scen.empty()) {
+    if (!rscen.empty() || !gdev.empty()) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         if (!ctx->multi) ctx->multi = new MultiBuffers();
@@ -332,11 +411,11 @@ int32_t autocall_impl(ofg_ctx *ctx, const ofg_autocall *in, size_t n, const ofg_
             buf.ensure(std::max<size_t>(bytes, 1));
             if (bytes) hip_check(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, st), what);
         };
-        up(b.scen, scen.data(), scen.size() * sizeof(scen[0]), "H2D autocall scenarios");
+        up(b.scen, rscen.data(), rscen.size() * sizeof(rscen[0]), "H2D autocall scenarios");
         up(b.tiles, tiles.data(), tiles.size() * sizeof(int32_t), "H2D autocall tiles");
         up(b.obs, obs.data(), obs.size() * sizeof(obs[0]), "H2D autocall schedule");
         b.partials.ensure(tiles.size() * sizeof(ofg::AcPartial));
-        b.out.ensure(scen.size() * sizeof(ofg::AcOut));
+        b.out.ensure(rscen.size() * sizeof(ofg::AcOut));
         ofg::AcArgs a{};
         if (rng == OFG_RNG_EXTERNAL) {
             up(b.normals, cfg->normals1, (size_t)normals_elems * sizeof(double), "H2D normals1");
@@ -348,17 +427,44 @@ int32_t autocall_impl(ofg_ctx *ctx, const ofg_autocall *in, size_t n, const ofg_
         a.partials = b.partials.as<ofg::AcPartial>();
         a.out = b.out.as<ofg::AcOut>();
         a.n_tiles = (int32_t)tiles.size();
-        a.n_scen = (int32_t)scen.size();
+        a.n_scen = (int32_t)rscen.size();
         a.rng_mode = rng;
+        ofg::AcgArgs ga{};
+        if (!gdev.empty()) {
+            up(b.gct, gdev.data(), gdev.size() * sizeof(gdev[0]), "H2D autocall sensitivities");
+            up(b.gtiles, gtiles.data(), gtiles.size() * sizeof(int32_t), "H2D autocall tiles");
+            b.gpartials.ensure(gtiles.size() * ofg::ACG_MAXS * sizeof(ofg::AcPartial));
+            b.gout.ensure((size_t)g_out * sizeof(ofg::AcOut));
+            ga.ct = b.gct.as<ofg::AcGreeksDev>();
+            ga.tile_ct = b.gtiles.as<int32_t>();
+            ga.obs = a.obs;
+            ga.partials = b.gpartials.as<ofg::AcPartial>();
+            ga.out = b.gout.as<ofg::AcOut>();
+            ga.normals = a.normals;
+            ga.n_tiles = (int32_t)gtiles.size();
+            ga.n_ct = (int32_t)gdev.size();
+            ga.rng_mode = rng;
+            ga.max_n = max_n;
+        }
         hip_check(hipEventRecord(ctx->ev0, st), "event");
-        hip_check(ofg::launch_autocall(a, st), "autocall kernel launch");
+        if (!rscen.empty()) hip_check(ofg::launch_autocall(a, st), "autocall kernel launch");
+        if (!gdev.empty())
+            hip_check(ofg::launch_autocall_greeks(ga, st), "autocall sensitivity kernel launch");
         hip_check(hipEventRecord(ctx->ev1, st), "event");
-        hip_check(hipMemcpyAsync(res.data(), a.out, res.size() * sizeof(res[0]),
-                                 hipMemcpyDeviceToHost, st),
-                  "D2H autocall results");
+        if (!rscen.empty())
+            hip_check(hipMemcpyAsync(rres.data(), a.out, rres.size() * sizeof(rres[0]),
+                                     hipMemcpyDeviceToHost, st),
+                      "D2H autocall results");
+        if (!gdev.empty())
+            hip_check(hipMemcpyAsync(gres.data(), ga.out, gres.size() * sizeof(gres[0]),
+                                     hipMemcpyDeviceToHost, st),
+                      "D2H autocall sensitivity results");
         hip_check(hipStreamSynchronize(st), "autocall kernel execution");
         hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
     }
+    std::vector<ofg::AcOut> res(scen.size());
+    for (size_t sidx = 0; sidx < scen.size(); ++sidx)
+        res[sidx] = fused_slot[sidx] >= 0 ? gres[(size_t)fused_slot[sidx]] : rres[(size_t)reg_index[sidx]];
     for (size_t i = 0; i < n; ++i) {
         const ContractPlan &pl = plans[i];
         out[i].kernel_ms = ms;

@@ -187,3 +187,31 @@ def test_invalid_contract_and_reference_draws(gpu):
     s = autocall_struct(good, [100.0], [0.2], [[1.0]], 0.01, 0.0, 100, 16, _Keep())
     with pytest.raises(InvalidInput, match="single stream"):
         autocall_price_raw([s], RngMode.Reference)
+
+
+@pytest.mark.parametrize("mode", ["external", "fast"])
+def test_fused_sensitivities_match_scenario_by_scenario(gpu, monkeypatch, mode):
+    """The fused pass (every bump of a contract on each path's shared draws)
+    prices each scenario as pricing it on its own does."""
+    from openferric_amd import RngMode, autocall_price_raw, autocall_struct
+    from openferric_amd.engines import _Keep
+    rng_mode = RngMode.External if mode == "external" else RngMode.Fast
+    zs = [_normals(kw, 300 + i) for i, kw in enumerate(CONTRACTS)] if mode == "external" else None
+
+    def run():
+        keep = _Keep()
+        structs = [autocall_struct(_contract(kw), MARKET["spots"], MARKET["vols"], CORR, 0.02, 0.01,
+                                   kw["num_paths"], kw["num_steps"], keep) for kw in CONTRACTS]
+        return autocall_price_raw(structs, rng_mode, zs, with_greeks=True)
+
+    fused, fs = run()
+    monkeypatch.setenv("OFG_AUTOCALL_FUSED", "0")
+    alone, as_ = run()
+    for f, a, x, y in zip(fused, alone, fs, as_):
+        assert abs(f.price - a.price) <= 1e-12 * abs(a.price)
+        assert abs(f.stderr_ - a.stderr_) <= 1e-9 * a.stderr_
+        assert x.ok and y.ok and x.n_delta == y.n_delta
+        for k in range(x.n_delta):
+            assert abs(x.delta[k] - y.delta[k]) <= 1e-9 * max(1.0, abs(y.delta[k]))
+        assert abs(x.vega - y.vega) <= 1e-9 * max(1.0, abs(y.vega))
+        assert abs(x.cega - y.cega) <= 1e-9 * max(1.0, abs(y.cega))
