@@ -10,6 +10,12 @@
 //   mc_european_qmc(_with_seed)                  src/engines/monte_carlo/mc_qmc.rs
 //   Market (flat vol)                            src/market/market.rs:202-414
 //   Heston                                       src/models/stochastic.rs:37-72
+//   Autocallable / PhoenixAutocallable, price_autocallable, ...
+//                                                src/pricing/autocallable.rs
+//   BasketOption, price_basket_mc, basket_sensitivities, ...
+//                                                src/pricing/basket.rs
+//   dsl::parse_and_compile, DslMonteCarloEngine, MultiAssetMarket
+//                                                src/dsl/{mod,engine,market}.rs
 //   PricingEngine<I>::price / price_with_greeks_aad (engine.rs:37-52)
 // Errors are C++ exceptions carrying the PricingError variant (Rust returns
 // Result<_, PricingError>); every call goes through libopenferric_amd.so.
@@ -735,5 +741,352 @@ inline std::pair<double, double> barrier_price_mc(OptionType t, BarrierStyle sty
     o.rebate = 0.0;
     return detail::legacy_price(o);
 }
+
+// ------------------------------------------------------- multi-asset --
+
+struct AutocallableSensitivities {   // pricing/autocallable.rs:25-33
+    std::vector<double> delta;
+    double vega = 0.0, cega = 0.0;
+};
+
+// instruments/autocallable.rs:18-157 (phoenix / memory select the Phoenix note)
+struct Autocallable {
+    std::vector<int32_t> underlyings;
+    double notional = 100.0;
+    std::vector<double> autocall_dates;
+    double autocall_barrier = 1.0, coupon_rate = 0.0;
+    double ki_barrier = 0.6, ki_strike = 1.0, maturity = 1.0;
+    bool phoenix = false, memory = false;
+    double coupon_barrier = 0.0;
+};
+
+namespace detail {
+inline ofg_engine_cfg cfg_for(int32_t rng) {
+    ofg_engine_cfg c{};
+    c.rng_mode = rng;
+    return c;
+}
+inline ofg_autocall autocall_struct(const Autocallable &a, const std::vector<double> &spots,
+                                    const std::vector<double> &vols,
+                                    const std::vector<double> &corr, double r, double q,
+                                    int64_t paths, int64_t steps) {
+    ofg_autocall c{};
+    c.phoenix = a.phoenix;
+    c.memory = a.memory;
+    c.n_underlyings = (int32_t)a.underlyings.size();
+    c.underlyings = a.underlyings.data();
+    c.n_dates = (int32_t)a.autocall_dates.size();
+    c.autocall_dates = a.autocall_dates.data();
+    c.notional = a.notional;
+    c.maturity = a.maturity;
+    c.autocall_barrier = a.autocall_barrier;
+    c.coupon_rate = a.coupon_rate;
+    c.coupon_barrier = a.coupon_barrier;
+    c.ki_barrier = a.ki_barrier;
+    c.ki_strike = a.ki_strike;
+    c.n_assets = (int32_t)spots.size();
+    c.spots = spots.data();
+    c.vols = vols.data();
+    c.corr = corr.data();
+    c.rate = r;
+    c.div_yield = q;
+    c.num_paths = paths;
+    c.num_steps = steps;
+    return c;
+}
+}  // namespace detail
+
+// price_autocallable / price_phoenix_autocallable (pricing/autocallable.rs:52-190):
+// price, stderr and bump-and-reprice Greeks (delta = sum of deltas, vega,
+// rho = cega); fast draws (the reference's single stream cannot be split:
+// a called path stops drawing). corr: row-major n x n over the market.
+inline PricingResult price_autocallable(const Autocallable &a, const std::vector<double> &spots,
+                                        const std::vector<double> &vols,
+                                        const std::vector<double> &corr, double r, double q,
+                                        int64_t n_paths, int64_t n_steps,
+                                        int32_t rng_mode = OFG_RNG_FAST) {
+    const ofg_autocall c = detail::autocall_struct(a, spots, vols, corr, r, q, n_paths, n_steps);
+    const ofg_engine_cfg cfg = detail::cfg_for(rng_mode);
+    ofg_result res{};
+    ofg_autocall_sens sens{};
+    check(ofg_autocall_price(device_context(), &c, 1, &cfg, 1, &res, &sens));
+    PricingResult out;
+    out.price = res.price;
+    if (res.has_stderr) {
+        out.stderr_ = res.stderr_;
+        out.diagnostics.insert("num_paths", res.num_paths);
+        out.diagnostics.insert("num_steps", res.num_steps);
+        out.diagnostics.insert("observation_count", (double)a.autocall_dates.size());
+    }
+    if (res.has_greeks) out.greeks = Greeks{res.delta, 0.0, res.vega, 0.0, res.rho};
+    return out;
+}
+
+// autocallable_sensitivities (pricing/autocallable.rs:101-133, 192-223)
+inline AutocallableSensitivities autocallable_sensitivities(
+    const Autocallable &a, const std::vector<double> &spots, const std::vector<double> &vols,
+    const std::vector<double> &corr, double r, double q, int64_t n_paths, int64_t n_steps,
+    int32_t rng_mode = OFG_RNG_FAST) {
+    const ofg_autocall c = detail::autocall_struct(a, spots, vols, corr, r, q, n_paths, n_steps);
+    const ofg_engine_cfg cfg = detail::cfg_for(rng_mode);
+    ofg_result res{};
+    ofg_autocall_sens sens{};
+    check(ofg_autocall_price(device_context(), &c, 1, &cfg, 1, &res, &sens));
+    if (!res.has_stderr || !sens.ok)
+        throw PricingError(PricingError::Kind::InvalidInput, ofg_last_error());
+    return AutocallableSensitivities{std::vector<double>(sens.delta, sens.delta + sens.n_delta),
+                                     sens.vega, sens.cega};
+}
+
+enum class BasketType { Average = OFG_BASKET_AVERAGE, BestOf = OFG_BASKET_BEST_OF,
+                        WorstOf = OFG_BASKET_WORST_OF };
+
+struct BasketOption {   // instruments/basket.rs:25-34
+    std::vector<double> weights;
+    double strike = 0.0, maturity = 1.0;
+    bool is_call = true;
+    BasketType basket_type = BasketType::Average;
+};
+
+struct BasketSensitivities {   // pricing/basket.rs:57-65
+    std::vector<double> delta;
+    double vega = 0.0, cega = 0.0;
+};
+
+namespace detail {
+inline ofg_basket basket_struct(const BasketOption &b, const std::vector<double> &spots,
+                                const std::vector<double> &vols, const std::vector<double> &corr,
+                                double r, const std::vector<double> &divs, int64_t paths) {
+    ofg_basket o{};
+    o.basket_type = (int32_t)b.basket_type;
+    o.is_call = b.is_call;
+    o.n_weights = (int32_t)b.weights.size();
+    o.weights = b.weights.data();
+    o.n_assets = (int32_t)spots.size();
+    o.strike = b.strike;
+    o.maturity = b.maturity;
+    o.spots = spots.data();
+    o.vols = vols.data();
+    o.dividends = divs.data();
+    o.corr = corr.data();
+    o.rate = r;
+    o.num_paths = paths;
+    o.seed = 3148159;   // MC_SEED (pricing/basket.rs:27)
+    return o;
+}
+}  // namespace detail
+
+// price_basket_mc (pricing/basket.rs:99-160): the reference's draws; an
+// invalid basket gives price NaN and no stderr, as the reference.
+inline PricingResult price_basket_mc(const BasketOption &b, const std::vector<double> &spots,
+                                     const std::vector<double> &vols,
+                                     const std::vector<double> &corr, double r,
+                                     const std::vector<double> &divs, int64_t n_paths) {
+    const ofg_basket o = detail::basket_struct(b, spots, vols, corr, r, divs, n_paths);
+    const ofg_engine_cfg cfg = detail::cfg_for(OFG_RNG_REFERENCE);
+    ofg_result res{};
+    ofg_basket_sens sens{};
+    check(ofg_basket_price(device_context(), &o, 1, &cfg, 0, &res, &sens));
+    PricingResult out;
+    out.price = res.price;
+    if (res.has_stderr) {
+        out.stderr_ = res.stderr_;
+        out.diagnostics.insert("num_paths", res.num_paths);
+        out.diagnostics.insert("rho", sens.rho);
+    }
+    return out;
+}
+
+// basket_sensitivities (pricing/basket.rs:456-539): every bump on the base
+// price's draws, in one pass per path.
+inline BasketSensitivities basket_sensitivities(const BasketOption &b,
+                                                const std::vector<double> &spots,
+                                                const std::vector<double> &vols,
+                                                const std::vector<double> &corr, double r,
+                                                const std::vector<double> &divs, int64_t n_paths) {
+    const ofg_basket o = detail::basket_struct(b, spots, vols, corr, r, divs, n_paths);
+    const ofg_engine_cfg cfg = detail::cfg_for(OFG_RNG_REFERENCE);
+    ofg_result res{};
+    ofg_basket_sens sens{};
+    check(ofg_basket_price(device_context(), &o, 1, &cfg, 1, &res, &sens));
+    if (!res.has_stderr || !sens.ok)
+        throw PricingError(PricingError::Kind::InvalidInput, ofg_last_error());
+    return BasketSensitivities{std::vector<double>(sens.delta, sens.delta + sens.n_delta),
+                               sens.vega, sens.cega};
+}
+
+// -------------------------------------------------- product language --
+namespace dsl {
+
+// CompiledProduct (src/dsl/ir.rs:141-151), owned.
+class CompiledProduct {
+public:
+    explicit CompiledProduct(ofg_dsl_product *p) : p_(p, &ofg_dsl_free) {
+        ofg_dsl_info info{};
+        check(ofg_dsl_get_info(p, &info));
+        name = info.name;
+        notional = info.notional;
+        maturity = info.maturity;
+        num_underlyings = info.num_underlyings;
+        num_state_vars = info.n_state;
+        num_schedules = info.n_schedules;
+    }
+    std::vector<double> schedule_dates(int32_t k) const {
+        int32_t n = 0;
+        check(ofg_dsl_schedule_dates(p_.get(), k, nullptr, 0, &n));
+        std::vector<double> d((size_t)n);
+        check(ofg_dsl_schedule_dates(p_.get(), k, d.data(), n, &n));
+        return d;
+    }
+    const ofg_dsl_product *handle() const { return p_.get(); }
+    std::string name;
+    double notional = 0.0, maturity = 0.0;
+    int32_t num_underlyings = 0, num_state_vars = 0, num_schedules = 0;
+
+private:
+    std::shared_ptr<ofg_dsl_product> p_;
+};
+
+// parse_and_compile (src/dsl/mod.rs:24-28): throws InvalidInput with the
+// DslError text.
+inline CompiledProduct parse_and_compile(const std::string &source) {
+    ofg_dsl_product *p = nullptr;
+    check(ofg_dsl_compile(source.c_str(), &p));
+    return CompiledProduct(p);
+}
+
+// AssetMarketData (src/dsl/market.rs:4-30)
+struct AssetMarketData {
+    ofg_dsl_asset a{};
+    static AssetMarketData equity(double spot, double vol, double dividend_yield) {
+        AssetMarketData d;
+        d.a.kind = OFG_DSL_EQUITY;
+        d.a.spot = spot;
+        d.a.vol = vol;
+        d.a.dividend_yield = dividend_yield;
+        return d;
+    }
+    static AssetMarketData fx(double spot, double vol, double domestic_rate, double foreign_rate) {
+        AssetMarketData d;
+        d.a.kind = OFG_DSL_FX;
+        d.a.spot = spot;
+        d.a.vol = vol;
+        d.a.domestic_rate = domestic_rate;
+        d.a.foreign_rate = foreign_rate;
+        return d;
+    }
+    static AssetMarketData commodity(double spot, double vol, double convenience_yield, double kappa,
+                                     double mu) {
+        AssetMarketData d;
+        d.a.kind = OFG_DSL_COMMODITY;
+        d.a.spot = spot;
+        d.a.vol = vol;
+        d.a.convenience_yield = convenience_yield;
+        d.a.kappa = kappa;
+        d.a.mu = mu;
+        return d;
+    }
+    static AssetMarketData rate(double initial_rate, double vol, double mean_reversion,
+                                double long_run_mean) {
+        AssetMarketData d;
+        d.a.kind = OFG_DSL_RATE;
+        d.a.spot = initial_rate;
+        d.a.vol = vol;
+        d.a.mean_reversion = mean_reversion;
+        d.a.long_run_mean = long_run_mean;
+        return d;
+    }
+    double initial_value() const { return a.spot; }
+    AssetMarketData with_spot_bump(double x) const {
+        AssetMarketData d = *this;
+        d.a.spot += x;
+        return d;
+    }
+    AssetMarketData with_vol_bump(double x) const {
+        AssetMarketData d = *this;
+        d.a.vol += x;
+        return d;
+    }
+};
+
+// MultiAssetMarket (src/dsl/market.rs:158-168); correlation row-major.
+struct MultiAssetMarket {
+    std::vector<AssetMarketData> assets;
+    std::vector<double> correlation;
+    double rate = 0.0;
+    static MultiAssetMarket single(double spot, double vol, double rate, double q) {
+        return MultiAssetMarket{{AssetMarketData::equity(spot, vol, q)}, {1.0}, rate};
+    }
+};
+
+// DslMonteCarloEngine (src/dsl/engine.rs:237-837)
+class DslMonteCarloEngine {
+public:
+    DslMonteCarloEngine(int64_t num_paths, int64_t num_steps, uint64_t seed)
+        : num_paths(num_paths), num_steps(num_steps), seed(seed) {}
+    DslMonteCarloEngine &with_rng_mode(int32_t mode) {
+        rng_mode = mode;
+        return *this;
+    }
+
+    // one product under several markets, one launch, the same draws
+    std::vector<PricingResult> price_markets(const CompiledProduct &p,
+                                             const std::vector<MultiAssetMarket> &markets) const {
+        std::vector<std::vector<ofg_dsl_asset>> keep(markets.size());
+        std::vector<ofg_dsl_market> ms(markets.size());
+        for (size_t i = 0; i < markets.size(); ++i) {
+            for (const auto &a : markets[i].assets) keep[i].push_back(a.a);
+            ms[i].n_assets = (int32_t)keep[i].size();
+            ms[i].assets = keep[i].data();
+            ms[i].corr = markets[i].correlation.data();
+            ms[i].rate = markets[i].rate;
+        }
+        const ofg_dsl_engine eng{num_paths, num_steps, seed};
+        const ofg_engine_cfg cfg = detail::cfg_for(rng_mode);
+        std::vector<ofg_result> out(markets.size());
+        check(ofg_dsl_price(device_context(), p.handle(), ms.data(), ms.size(), &eng, &cfg,
+                            out.data()));
+        std::vector<PricingResult> res;
+        for (const auto &r : out) {
+            PricingResult x;
+            x.price = r.price;
+            x.stderr_ = r.stderr_;
+            x.diagnostics.insert("num_paths", r.num_paths);
+            x.diagnostics.insert("num_steps", r.num_steps);
+            res.push_back(x);
+        }
+        return res;
+    }
+
+    PricingResult price_multi_asset(const CompiledProduct &p, const MultiAssetMarket &m) const {
+        return price_markets(p, {m})[0];
+    }
+
+    // greeks_multi_asset (engine.rs:785-837): the five markets in one launch
+    Greeks greeks_multi_asset(const CompiledProduct &p, const MultiAssetMarket &m,
+                              size_t asset_index) const {
+        if (asset_index >= m.assets.size())
+            throw PricingError(PricingError::Kind::InvalidInput,
+                               "asset_index " + std::to_string(asset_index) +
+                                   " out of range (have " + std::to_string(m.assets.size()) +
+                                   " assets)");
+        const double h = 0.01 * m.assets[asset_index].initial_value();
+        std::vector<MultiAssetMarket> s(5, m);
+        s[1].assets[asset_index] = m.assets[asset_index].with_spot_bump(h);
+        s[2].assets[asset_index] = m.assets[asset_index].with_spot_bump(-h);
+        s[3].assets[asset_index] = m.assets[asset_index].with_vol_bump(0.01);
+        s[4].rate = m.rate + 0.0001;
+        const auto r = price_markets(p, s);
+        const double base = r[0].price, up = r[1].price, dn = r[2].price;
+        return Greeks{(up - dn) / (2.0 * h), (up - 2.0 * base + dn) / (h * h),
+                      (r[3].price - base) / 0.01, 0.0, (r[4].price - base) / 0.0001};
+    }
+
+    int64_t num_paths, num_steps;
+    uint64_t seed;
+    int32_t rng_mode = OFG_RNG_REFERENCE;
+};
+
+}  // namespace dsl
 
 }  // namespace openferric_amd

@@ -243,6 +243,55 @@ int main() {
         const auto am = mc_european_qmc(VanillaOption::american_put(100.0, 1.0), m, 100, 4);
         expect(std::isnan(am.price) && !am.stderr_, "qmc_non_european_is_nan");
     }
+    {  // basket.rs:1067-1099 and the multi-asset mirrors
+        const auto one = price_basket_mc(BasketOption{{1.0}, 100.0, 1.0, true, BasketType::Average},
+                                         {100.0}, {0.2}, {1.0}, 0.03, {0.0}, 120000);
+        auto N = [](double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); };
+        const double d1 = (0.03 + 0.5 * 0.04) / 0.2, d2 = d1 - 0.2;
+        const double bs = 100.0 * N(d1) - 100.0 * std::exp(-0.03) * N(d2);
+        expect(std::fabs(one.price - bs) <= 3.0 * *one.stderr_ + 0.15,
+               "basket_with_one_asset_reduces_to_vanilla_option", std::to_string(one.price));
+        const auto bad = price_basket_mc(BasketOption{{0.5}, 100.0, 1.0, true, BasketType::Average},
+                                         {100.0, 90.0}, {0.2, 0.2}, {1, 0.3, 0.3, 1}, 0.03,
+                                         {0.0, 0.0}, 1000);
+        expect(std::isnan(bad.price) && !bad.stderr_, "basket_invalid_weights_is_nan");
+        const auto sens = basket_sensitivities(
+            BasketOption{{0.5, 0.5}, 100.0, 1.0, true, BasketType::Average}, {100.0, 90.0},
+            {0.2, 0.25}, {1, 0.3, 0.3, 1}, 0.03, {0.0, 0.0}, 50000);
+        expect(sens.delta.size() == 2 && sens.delta[0] > 0.0 && sens.delta[1] > 0.0 &&
+                   sens.vega > 0.0,
+               "basket_sensitivities_signs");
+        Autocallable note;
+        note.underlyings = {0, 1};
+        note.autocall_dates = {0.25, 0.5, 0.75, 1.0};
+        note.coupon_rate = 0.08;
+        const auto ac = price_autocallable(note, {100.0, 100.0}, {0.2, 0.25}, {1, 0.5, 0.5, 1},
+                                           0.02, 0.0, 20000, 64);
+        expect(std::isfinite(ac.price) && ac.greeks.has_value() &&
+                   ac.diagnostics.get("observation_count").value_or(0) == 4.0,
+               "price_autocallable_with_greeks");
+    }
+    {  // dsl/mod.rs end_to_end_simple_forward, engine.rs greeks_produce_sensible_delta
+        const auto prod = dsl::parse_and_compile(
+            "product \"Forward\"\n    notional: 100\n    maturity: 1.0\n    underlyings\n"
+            "        SPX = asset(0)\n    schedule annual from 1.0 to 1.0\n"
+            "        redeem notional * price(0) / 100\n");
+        const auto mk = dsl::MultiAssetMarket::single(100.0, 0.20, 0.05, 0.0);
+        const auto r = dsl::DslMonteCarloEngine(10000, 100, 42).price_multi_asset(prod, mk);
+        expect(std::fabs(r.price - 100.0) < 1.0, "dsl_forward_product_prices_near_forward",
+               std::to_string(r.price));
+        const auto g = dsl::DslMonteCarloEngine(20000, 50, 42).greeks_multi_asset(prod, mk, 0);
+        expect(std::fabs(g.delta - 1.0) < 0.05, "dsl_greeks_produce_sensible_delta",
+               std::to_string(g.delta));
+        bool threw = false;
+        try {
+            dsl::parse_and_compile("product \"x\"\n    notional: 1\n");
+        } catch (const PricingError &e) {
+            threw = e.kind() == PricingError::Kind::InvalidInput &&
+                    std::string(e.what()).find("missing maturity") != std::string::npos;
+        }
+        expect(threw, "dsl_compile_error_is_invalid_input");
+    }
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
 }
