@@ -243,7 +243,15 @@ typedef struct ofg_basket {
     int32_t factor_kind;
     int32_t n_loading_rows, n_factors;   /* rows = the model's asset count; factors <= 4 */
     const double *loadings;              /* n_loading_rows x n_factors, row-major */
+    /* BasketCopula (pricing/basket.rs:32-46) of a BasketOption, matrix or
+     * factor model (price_basket_mc_with_copula, :120-160): 0 Gaussian,
+     * 1 StudentT with copula_dof degrees of freedom (at least 2, checked when
+     * the first path is drawn, as math/correlation.rs:559-562). */
+    int32_t copula;
+    int32_t copula_dof;
 } ofg_basket;
+
+enum { OFG_COPULA_GAUSSIAN = 0, OFG_COPULA_STUDENT_T = 1 };
 
 /* BasketSensitivities (pricing/basket.rs:57-65) and the diagnostics of
  * price_basket_mc (:149-151). */

@@ -12,7 +12,7 @@
 //   Heston                                       src/models/stochastic.rs:37-72
 //   Autocallable / PhoenixAutocallable, price_autocallable, ...
 //                                                src/pricing/autocallable.rs
-//   BasketOption, price_basket_mc, basket_sensitivities, ...
+//   BasketOption, price_basket_mc(_with_copula), basket_sensitivities, ...
 //                                                src/pricing/basket.rs
 //   dsl::parse_and_compile, DslMonteCarloEngine, MultiAssetMarket
 //                                                src/dsl/{mod,engine,market}.rs
@@ -883,6 +883,40 @@ inline PricingResult price_basket_mc(const BasketOption &b, const std::vector<do
                                      const std::vector<double> &corr, double r,
                                      const std::vector<double> &divs, int64_t n_paths) {
     const ofg_basket o = detail::basket_struct(b, spots, vols, corr, r, divs, n_paths);
+    const ofg_engine_cfg cfg = detail::cfg_for(OFG_RNG_REFERENCE);
+    ofg_result res{};
+    ofg_basket_sens sens{};
+    check(ofg_basket_price(device_context(), &o, 1, &cfg, 0, &res, &sens));
+    PricingResult out;
+    out.price = res.price;
+    if (res.has_stderr) {
+        out.stderr_ = res.stderr_;
+        out.diagnostics.insert("num_paths", res.num_paths);
+        out.diagnostics.insert("rho", sens.rho);
+    }
+    return out;
+}
+
+// BasketCopula (pricing/basket.rs:32-46): Gaussian, or StudentT with
+// degrees_of_freedom >= 2.
+struct BasketCopula {
+    bool student_t = false;
+    int32_t degrees_of_freedom = 0;
+    static BasketCopula Gaussian() { return {}; }
+    static BasketCopula StudentT(int32_t dof) { return {true, dof}; }
+};
+
+// price_basket_mc_with_copula (pricing/basket.rs:120-160), the reference's
+// draws: under StudentT each path draws its asset normals, then dof more.
+inline PricingResult price_basket_mc_with_copula(const BasketOption &b,
+                                                 const std::vector<double> &spots,
+                                                 const std::vector<double> &vols,
+                                                 const std::vector<double> &corr, double r,
+                                                 const std::vector<double> &divs, int64_t n_paths,
+                                                 BasketCopula copula) {
+    ofg_basket o = detail::basket_struct(b, spots, vols, corr, r, divs, n_paths);
+    o.copula = copula.student_t ? OFG_COPULA_STUDENT_T : OFG_COPULA_GAUSSIAN;
+    o.copula_dof = copula.degrees_of_freedom;
     const ofg_engine_cfg cfg = detail::cfg_for(OFG_RNG_REFERENCE);
     ofg_result res{};
     ofg_basket_sens sens{};

@@ -146,8 +146,11 @@ class OfgBasket(C.Structure):
         ("fx_vol", C.c_double), ("domestic_rate", C.c_double), ("foreign_rate", C.c_double),
         ("asset_fx_corr", C.POINTER(C.c_double)), ("n_asset_fx_corr", C.c_int32),
         ("factor_kind", C.c_int32), ("n_loading_rows", C.c_int32), ("n_factors", C.c_int32),
-        ("loadings", C.POINTER(C.c_double)),
+        ("loadings", C.POINTER(C.c_double)), ("copula", C.c_int32), ("copula_dof", C.c_int32),
     ]
+
+
+COPULA_GAUSSIAN, COPULA_STUDENT_T = 0, 1
 
 
 class OfgBasketSens(C.Structure):

@@ -185,6 +185,22 @@ void validate_factor(const ofg_basket &b, const std::vector<double> &spots,
         fail(OFG_INVALID_INPUT, "average basket weights length must match number of assets");
 }
 
+// ln Gamma(x) for x >= 0.5 as statrs 0.18's ln_gamma (the crate of
+// StudentsT::cdf): the Lanczos sum with Pugh's r and d_k, the table of
+// src/math/gamma.rs:17-31.
+double ln_gamma_lanczos(double x) {
+    static const double R = 10.900511;
+    static const double DK[11] = {
+        2.4857408913875355e-5, 1.0514237858172197, -3.4568709722201624, 4.512277094668948,
+        -2.9828522532357666, 1.056397115771267, -1.9542877319164587e-1, 1.709705434044412e-2,
+        -5.719261174043057e-4, 4.633994733599057e-6, -2.7199490848860772e-9};
+    const double LN_2_SQRT_E_OVER_PI = 0.6207822376352452223455184457816472122518527279025978;
+    const double E = 2.7182818284590452353602874713526624977572470937;
+    double s = DK[0];
+    for (int i = 1; i < 11; ++i) s += DK[i] / (x + (double)i - 1.0);
+    return std::log(s) + LN_2_SQRT_E_OVER_PI + (x - 0.5) * std::log((x - 0.5 + R) / E);
+}
+
 // terminal_payoff (basket.rs:759-784) of the spots themselves, times the
 // quanto FX rate; outperformance_payoff (:786-800).
 double payoff_at_spots(const ofg_basket &b, const std::vector<double> &spots) {
@@ -255,7 +271,19 @@ Scenario prepare(const ofg_basket &b, const std::vector<double> &spots,
         }
         if (n > ofg::BK_MAX) fail(OFG_INVALID_INPUT, "at most 16 assets per basket");
     }
+    // the Student-t copula's dof check runs when the first path is drawn
+    // (map_copula_normals_to_uniforms, math/correlation.rs:559-562), after
+    // the zero-maturity shortcut, the repair and the factorisation
+    if (b.copula == OFG_COPULA_STUDENT_T && b.copula_dof < 2)
+        fail(OFG_INVALID_INPUT, "student-t copula requires degrees_of_freedom >= 2");
     ofg::BkDev &d = s.d;
+    if (b.copula == OFG_COPULA_STUDENT_T) {
+        const double nu = (double)b.copula_dof, a = nu / 2.0;
+        d.t_dof = b.copula_dof;
+        d.t_nu = nu;
+        d.t_a = a;
+        d.t_lbeta = ln_gamma_lanczos(a + 0.5) - ln_gamma_lanczos(a) - ln_gamma_lanczos(0.5);
+    }
     d.n = n;
     d.type = b.basket_type;
     d.is_call = b.is_call != 0;
@@ -368,8 +396,11 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
         BasketPlan &pl = plans[i];
         // a basket's external normals: [asset][path], n_assets * num_paths
         // draws per path: one per asset, plus the factors of a factor model
-        const int64_t draws = (int64_t)b.n_assets +
-                              (b.factor_kind == 1 ? 1 : b.factor_kind == 2 ? std::max(b.n_factors, 0) : 0);
+        // and the chi-square draws of a Student-t copula
+        const int64_t draws =
+            (int64_t)b.n_assets +
+            (b.factor_kind == 1 ? 1 : b.factor_kind == 2 ? std::max(b.n_factors, 0) : 0) +
+            (b.copula == OFG_COPULA_STUDENT_T ? std::max(b.copula_dof, 0) : 0);
         const int64_t my_normals = normals_elems;
         if (b.n_assets > 0 && b.num_paths > 0) normals_elems += draws * b.num_paths;
         try {
@@ -385,6 +416,10 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
                                   (b.n_loading_rows * b.n_factors > 0 && !b.loadings)))
                 fail(OFG_INVALID_INPUT, "factor-model loadings must be n_loading_rows x n_factors "
                                         "(one factor for OneFactor)");
+            if (b.copula != OFG_COPULA_GAUSSIAN && b.copula != OFG_COPULA_STUDENT_T)
+                fail(OFG_INVALID_INPUT, "unknown basket copula");
+            if (b.copula != OFG_COPULA_GAUSSIAN && b.product != 0)
+                fail(OFG_INVALID_INPUT, "the Student-t copula applies to BasketOption only");
             if (b.product == 2 && b.n_asset_fx_corr > 0 && !b.asset_fx_corr)
                 fail(OFG_INVALID_INPUT, "quanto asset_fx_corr length must match assets");
             const int na = b.n_assets;
@@ -416,7 +451,8 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
             // diagnostics rho: the matrix's average off-diagonal, 0 under a
             // factor model (basket.rs:151, 195)
             pl.rho = b.factor_kind ? 0.0 : average_off_diagonal(corr, na);
-            if (with_sens && b.product == 0 && b.factor_kind == 0) {
+            if (with_sens && b.product == 0 && b.factor_kind == 0 &&
+                b.copula == OFG_COPULA_GAUSSIAN) {
                 // basket_sensitivities (basket.rs:456-539)
                 try {
                     for (int k = 0; k < na; ++k) {
@@ -471,7 +507,9 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
     const bool fuse = with_sens && !(fenv && std::atoi(fenv) == 0);
     for (size_t i = 0; fuse && i < n; ++i) {
         const BasketPlan &pl = plans[i];
-        if (!pl.valid || !pl.sens_ok || in[i].product != 0 || in[i].factor_kind != 0) continue;
+        if (!pl.valid || !pl.sens_ok || in[i].product != 0 || in[i].factor_kind != 0 ||
+            in[i].copula != OFG_COPULA_GAUSSIAN)
+            continue;
         std::vector<int32_t> ids{pl.base, pl.vega_up, pl.vega_dn};
         for (size_t k = 0; k < pl.delta_up.size(); ++k) {
             ids.push_back(pl.delta_up[k]);
@@ -587,6 +625,7 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
         a.n_tiles = (int32_t)tiles.size();
         a.n_scen = (int32_t)dev.size();
         a.rng_mode = rng;
+        for (const ofg::BkDev &d : dev) a.any_t |= d.t_dof > 0;
         hip_check(hipEventRecord(ctx->ev0, st), "event");
         if (rng == OFG_RNG_REFERENCE)
             hip_check(ofg::launch_lsm_seeds(bb.jobs.as<ofg::LsmSeedJob>(), (int32_t)jobs.size(),

@@ -196,11 +196,68 @@ __global__ __launch_bounds__(MA_BLOCK) void autocall_close_kernel(AcArgs a) {
 // Basket options: one exact terminal step per asset
 // ------------------------------------------------------------------------
 
+// I_x(a, b), statrs 0.18's beta_reg (the crate behind StudentsT::cdf in
+// math/correlation.rs:564-568): the modified Lentz continued fraction with
+// the symmetry transform when x >= (a + 1) / (a + b + 2), at most 140 terms,
+// tolerance 2^-53, and a zero prefactor when x == 0 or x lies within 4 ulps
+// of 1. lbeta carries the prefactor's log-gamma terms for the caller's (a, b).
+__device__ __noinline__ double beta_reg_dev(double a, double b, double x, double lbeta) {
+    const double bt = (x == 0.0 || x >= 1.0 - 4.0 * 0x1p-53)
+                          ? 0.0
+                          : exp_fp64(lbeta + a * log(x) + b * log(1.0 - x));
+    const bool symm = x >= (a + 1.0) / (a + b + 2.0);
+    const double eps = 0x1p-53, fpmin = 0x1p-1022 / eps;
+    if (symm) {
+        const double t = a;
+        x = 1.0 - x;
+        a = b;
+        b = t;
+    }
+    const double qab = a + b, qap = a + 1.0, qam = a - 1.0;
+    double c = 1.0, d = 1.0 - qab * x / qap;
+    if (fabs(d) < fpmin) d = fpmin;
+    d = 1.0 / d;
+    double h = d;
+    for (int mi = 1; mi < 141; ++mi) {
+        const double m = (double)mi, m2 = m * 2.0;
+        double aa = m * (b - m) * x / ((qam + m2) * (a + m2));
+        d = 1.0 + aa * d;
+        if (fabs(d) < fpmin) d = fpmin;
+        c = 1.0 + aa / c;
+        if (fabs(c) < fpmin) c = fpmin;
+        d = 1.0 / d;
+        h = h * d * c;
+        aa = -(a + m) * (qab + m) * x / ((a + m2) * (qap + m2));
+        d = 1.0 + aa * d;
+        if (fabs(d) < fpmin) d = fpmin;
+        c = 1.0 + aa / c;
+        if (fabs(c) < fpmin) c = fpmin;
+        d = 1.0 / d;
+        const double del = d * c;
+        h *= del;
+        if (fabs(del - 1.0) <= eps) break;
+    }
+    return symm ? 1.0 - bt * h / a : bt * h / a;
+}
+
+// StudentsT::cdf at location 0, scale 1: h = nu / (nu + x^2),
+// ib = I_h(nu/2, 1/2) / 2, x <= 0 ? ib : 1 - ib.
+__device__ __forceinline__ double student_t_cdf_dev(double x, double nu, double a, double lbeta) {
+    const double h = nu / (nu + x * x);
+    const double ib = 0.5 * beta_reg_dev(a, 0.5, h, lbeta);
+    return x <= 0.0 ? ib : 1.0 - ib;
+}
+
+__device__ __forceinline__ double clamp_copula_uniform(double u) {   // clamp(1e-12, 1 - 1e-12)
+    const double lo = 1.0e-12, hi = 1.0 - 1.0e-12;
+    return u < lo ? lo : (u > hi ? hi : u);
+}
+
 // One path's payoff (terminal_payoff, basket.rs:759-784; outperformance_payoff,
 // :786-800; times the quanto FX rate, :429). The correlated
 // normal of asset i accumulates L[i][j] z_j column by column as each z_j is
 // drawn, the same terms in the same order as correlate_normals' running sum.
-template <int RNG>
+template <int RNG, bool TCOP>
 __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const double *normals,
                                               const uint64_t *seeds, const double *tab) {
     constexpr bool FAST = RNG == OFG_RNG_FAST;
@@ -270,6 +327,26 @@ __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const d
             }
         }
     }
+    if (TCOP && o.t_dof > 0) {
+        // Student-t copula: sample_chi_square_integer_dof, then
+        // map_copula_normals_to_uniforms and copula_uniforms_to_normals
+        // (math/correlation.rs:545-582, 624-633)
+        const int k0 = n + o.n_factors;
+        double chi2 = 0.0;
+        for (int k = 0; k < o.t_dof; ++k) {
+            const double z = draw(k0 + k);
+            chi2 += z * z;
+        }
+        const double scale = sqrt(fmax(chi2 / o.t_nu, 1.0e-16));
+#pragma unroll
+        for (int i = 0; i < BK_MAX; ++i) {
+            if (i < n) {
+                const double u =
+                    clamp_copula_uniform(student_t_cdf_dev(c[i] / scale, o.t_nu, o.t_a, o.t_lbeta));
+                c[i] = acklam_inv_cdf(clamp_copula_uniform(u));
+            }
+        }
+    }
     const bool outperf = o.payoff == BK_OUTPERFORMANCE;
     double u = (o.type == BK_AVERAGE || outperf) ? -0.0
                : (o.type == BK_BEST_OF ? -INFINITY : INFINITY);
@@ -297,7 +374,9 @@ __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const d
     return o.scale * x;
 }
 
-template <int RNG>
+// TCOP: the batch holds Student-t copula baskets (a separate instantiation
+// keeps the Gaussian kernel's register budget).
+template <int RNG, bool TCOP>
 __global__ __launch_bounds__(MA_BLOCK) void basket_kernel(BkArgs a) {
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     __shared__ double red[NWAVES][2];
@@ -313,7 +392,7 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_kernel(BkArgs a) {
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * MA_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        const double x = basket_path<RNG>(o, p, a.normals, a.seeds, tab);
+        const double x = basket_path<RNG, TCOP>(o, p, a.normals, a.seeds, tab);
         sum += x;
         sq += x * x;
     }
@@ -900,13 +979,18 @@ hipError_t launch_basket_greeks(const BkgArgs &a, hipStream_t stream) {
 
 hipError_t launch_basket(const BkArgs &a, hipStream_t stream) {
     if (a.n_tiles <= 0) return hipSuccess;
-    switch (a.rng_mode) {
-    case OFG_RNG_FAST: basket_kernel<OFG_RNG_FAST><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a); break;
-    case OFG_RNG_EXTERNAL:
-        basket_kernel<OFG_RNG_EXTERNAL><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
-        break;
-    default: basket_kernel<OFG_RNG_REFERENCE><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a); break;
-    }
+    auto go = [&](auto tcop) {
+        constexpr bool T = decltype(tcop)::value;
+        switch (a.rng_mode) {
+        case OFG_RNG_FAST: basket_kernel<OFG_RNG_FAST, T><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a); break;
+        case OFG_RNG_EXTERNAL:
+            basket_kernel<OFG_RNG_EXTERNAL, T><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a);
+            break;
+        default: basket_kernel<OFG_RNG_REFERENCE, T><<<a.n_tiles, MA_BLOCK, 0, stream>>>(a); break;
+        }
+    };
+    if (a.any_t) go(std::true_type{});
+    else go(std::false_type{});
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     basket_close_kernel<<<a.n_scen, MA_BLOCK, 0, stream>>>(a);

@@ -111,7 +111,7 @@ struct AcgArgs {
 
 hipError_t launch_autocall_greeks(const AcgArgs &a, hipStream_t stream);
 
-// ---- basket options (src/pricing/basket.rs:599-683, Gaussian copula) ----
+// ---- basket options (src/pricing/basket.rs:599-757, Gaussian and Student-t copulas) ----
 constexpr int BK_MAX = 16;
 constexpr int BK_FMAX = 4;   // factors of a factor-model basket
 enum BasketType : int32_t { BK_AVERAGE = 0, BK_BEST_OF = 1, BK_WORST_OF = 2 };
@@ -141,6 +141,13 @@ struct alignas(16) BkDev {
     int32_t one_factor;        // OneFactor: fma(beta, m, idio eps)
     double load[BK_MAX * BK_FMAX];   // [asset][factor]; OneFactor: the clamped beta
     double idio[BK_MAX];
+    // Student-t copula (t_dof > 0): t_dof more draws per path after the
+    // asset and factor draws; t_nu = dof, t_a = dof / 2 and t_lbeta =
+    // lnG(t_a + 1/2) - lnG(t_a) - lnG(1/2), the log-gamma terms of the
+    // incomplete beta's prefactor, evaluated on the host
+    int32_t t_dof;
+    int32_t pad_t;
+    double t_nu, t_a, t_lbeta;
 };
 
 struct BkArgs {
@@ -153,7 +160,7 @@ struct BkArgs {
     int32_t n_tiles;
     int32_t n_scen;
     int32_t rng_mode;
-    int32_t pad;
+    int32_t any_t;    // some scenario uses the Student-t copula
 };
 
 hipError_t launch_basket(const BkArgs &a, hipStream_t stream);

@@ -354,10 +354,48 @@ class FactorCorrelationModel:
         return c.tolist()
 
 
+@dataclass(frozen=True)
+class _StudentTCopula:
+    degrees_of_freedom: int
+
+
 class BasketCopula:
-    """pricing/basket.rs:32-46 (Gaussian only here: the Student-t copula's
-    CDF comes from a crate that is not part of the reference tree)."""
+    """pricing/basket.rs:32-46: BasketCopula.Gaussian or
+    BasketCopula.StudentT(degrees_of_freedom) (at least 2, checked when the
+    first path is drawn, as math/correlation.rs:559-562)."""
     Gaussian = "gaussian"
+
+    @staticmethod
+    def StudentT(degrees_of_freedom: int) -> _StudentTCopula:
+        return _StudentTCopula(int(degrees_of_freedom))
+
+
+def _apply_copula(b, copula) -> None:
+    if copula == BasketCopula.Gaussian:
+        b.copula, b.copula_dof = _lib.COPULA_GAUSSIAN, 0
+    elif isinstance(copula, _StudentTCopula):
+        b.copula, b.copula_dof = _lib.COPULA_STUDENT_T, copula.degrees_of_freedom
+    else:
+        raise InvalidInput("copula must be BasketCopula.Gaussian or BasketCopula.StudentT(dof)")
+
+
+def price_basket_mc_with_copula(basket: BasketOption, spots, vols, corr_matrix, r, dividends,
+                                n_paths, copula=BasketCopula.Gaussian,
+                                rng_mode=RngMode.Reference, normals=None) -> PricingResult:
+    """price_basket_mc_with_copula (pricing/basket.rs:120-160). Under
+    StudentT(dof) each path draws its asset normals, then dof more for the
+    chi-square (normals, External mode: [path][asset + dof])."""
+    keep = _Keep()
+    s = basket_struct(basket, spots, vols, corr_matrix, r, dividends, n_paths, keep)
+    _apply_copula(s, copula)
+    res, sens = basket_price_raw([s], rng_mode, None if normals is None else [normals])
+    r0 = res[0]
+    diag = Diagnostics()
+    if r0.has_stderr:
+        diag.insert("num_paths", r0.num_paths)
+        diag.insert("rho", sens[0].rho)
+    return PricingResult(price=r0.price, stderr=r0.stderr_ if r0.has_stderr else None,
+                         diagnostics=diag, kernel_ms=r0.kernel_ms)
 
 
 def price_basket_mc_with_factor_model(basket: BasketOption, spots, vols,
@@ -365,9 +403,8 @@ def price_basket_mc_with_factor_model(basket: BasketOption, spots, vols,
                                       copula=BasketCopula.Gaussian, rng_mode=RngMode.Reference,
                                       normals=None) -> PricingResult:
     """price_basket_mc_with_factor_model (pricing/basket.rs:162-205).
-    normals (External): [path][draw], draws = factors then one per asset."""
-    if copula != BasketCopula.Gaussian:
-        raise InvalidInput("only the Gaussian copula is supported")
+    normals (External): [path][draw], draws = factors, one per asset, then
+    the Student-t copula's dof."""
     keep = _Keep()
     n = len(spots)
     b = basket_struct(basket, spots, vols, np.eye(n) if n else [], r, dividends, n_paths, keep)
@@ -377,6 +414,7 @@ def price_basket_mc_with_factor_model(basket: BasketOption, spots, vols,
     b.factor_kind = 1 if factor_model.one_factor else 2
     b.n_loading_rows, b.n_factors = len(factor_model.loadings), k
     b.loadings = keep.arr(rows.ravel()) if rows.size else None
+    _apply_copula(b, copula)
     res, sens = basket_price_raw([b], rng_mode, None if normals is None else [normals])
     r0 = res[0]
     diag = Diagnostics()

@@ -231,7 +231,8 @@ void or_bump_corr_matrix(const double *c, int32_t n, double bump, double *out);
 /* ---- multi-asset: basket Monte Carlo (oracle_multi.c) ----------------- */
 enum { OR_BASKET_AVERAGE = 0, OR_BASKET_BEST_OF = 1, OR_BASKET_WORST_OF = 2 };
 typedef struct {
-    int32_t basket_type, is_call, n_assets, pad;
+    int32_t basket_type, is_call, n_assets;
+    int32_t t_dof;   /* 0: Gaussian copula; >= 2: Student-t copula, t_dof degrees of freedom */
     const double *weights;                       /* n_assets (Average) */
     double strike, maturity, rate;
     const double *spots, *vols, *dividends;      /* n_assets */
@@ -240,6 +241,13 @@ typedef struct {
     uint64_t seed;
 } or_basket;
 int or_basket_price(const or_basket *b, const double *normals, double *price, double *stderr_out);
+
+/* Student-t copula pieces (math/correlation.rs:559-582): statrs 0.18's
+ * ln_gamma (Lanczos, x >= 0.5), beta_reg (Lentz continued fraction) and
+ * StudentsT::cdf at location 0, scale 1. */
+double or_ln_gamma(double x);
+double or_beta_reg(double a, double b, double x);
+double or_student_t_cdf(double x, double nu);
 
 /* simulate_basket_once_factor (basket.rs:686-757): OneFactor (loadings =
  * betas) or MultiFactor (loadings [n][k]); normals [path][k + n] or NULL. */

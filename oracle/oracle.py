@@ -90,7 +90,7 @@ class OrAutocall(C.Structure):
 class OrBasket(C.Structure):
     _fields_ = [
         ("basket_type", C.c_int32), ("is_call", C.c_int32), ("n_assets", C.c_int32),
-        ("pad", C.c_int32), ("weights", C.POINTER(C.c_double)), ("strike", C.c_double),
+        ("t_dof", C.c_int32), ("weights", C.POINTER(C.c_double)), ("strike", C.c_double),
         ("maturity", C.c_double), ("rate", C.c_double), ("spots", C.POINTER(C.c_double)),
         ("vols", C.POINTER(C.c_double)), ("dividends", C.POINTER(C.c_double)),
         ("corr", C.POINTER(C.c_double)), ("num_paths", C.c_int64), ("seed", C.c_uint64),
@@ -164,6 +164,9 @@ def lib():
         L.or_basket_factor_price.argtypes = [C.POINTER(OrBasket), C.c_int, C.c_int32, pd, pd, pd,
                                              pd]
         L.or_basket_factor_price.restype = C.c_int
+        L.or_ln_gamma.argtypes = [C.c_double]; L.or_ln_gamma.restype = C.c_double
+        L.or_beta_reg.argtypes = [C.c_double] * 3; L.or_beta_reg.restype = C.c_double
+        L.or_student_t_cdf.argtypes = [C.c_double] * 2; L.or_student_t_cdf.restype = C.c_double
         L.or_outperformance_price.argtypes = [C.POINTER(OrBasket), C.c_int32, pd, pd, pd]
         L.or_outperformance_price.restype = C.c_int
         L.or_quanto_price.argtypes = [C.POINTER(OrBasket), C.c_double, C.c_double, pd, C.c_double,
@@ -444,7 +447,7 @@ def cholesky_lower(a):
 
 def make_basket(spots, vols, dividends, corr, weights=None, basket_type=BASKET_AVERAGE,
                 is_call=True, strike=100.0, maturity=1.0, rate=0.02, num_paths=10_000,
-                seed=3_148_159):
+                seed=3_148_159, t_dof=0):
     b = OrBasket()
     arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in
             (spots, vols, dividends, np.asarray(corr, dtype=np.float64).ravel(),
@@ -453,12 +456,26 @@ def make_basket(spots, vols, dividends, corr, weights=None, basket_type=BASKET_A
     b.spots, b.vols, b.dividends, b.corr, b.weights = [_ptr(a) for a in arrs]
     b.n_assets, b.basket_type, b.is_call = len(arrs[0]), basket_type, int(is_call)
     b.strike, b.maturity, b.rate = strike, maturity, rate
-    b.num_paths, b.seed = num_paths, seed
+    b.num_paths, b.seed, b.t_dof = num_paths, seed, t_dof
     return b
 
 
+def ln_gamma(x):
+    return lib().or_ln_gamma(float(x))
+
+
+def beta_reg(a, b, x):
+    return lib().or_beta_reg(float(a), float(b), float(x))
+
+
+def student_t_cdf(x, nu):
+    """statrs 0.18 StudentsT::new(0, 1, nu).cdf(x)."""
+    return lib().or_student_t_cdf(float(x), float(nu))
+
+
 def basket_price(b, normals=None):
-    """simulate_basket_once_matrix (Gaussian): (price, stderr). normals [path][asset]."""
+    """simulate_basket_once_matrix: (price, stderr). normals [path][asset]
+    (Student-t copula: [path][asset + dof])."""
     z = np.ascontiguousarray(normals, dtype=np.float64).ravel() if normals is not None else None
     p, se = C.c_double(), C.c_double()
     rc = lib().or_basket_price(C.byref(b), _ptr(z), C.byref(p), C.byref(se))
@@ -490,7 +507,8 @@ def quanto_price(b, fx_rate, fx_vol, asset_fx_corr, domestic_rate, foreign_rate,
 
 
 def basket_factor_price(b, loadings, one_factor, normals=None):
-    """simulate_basket_once_factor: (price, stderr). normals [path][k + n]."""
+    """simulate_basket_once_factor: (price, stderr). normals [path][k + n]
+    (Student-t copula: [path][k + n + dof])."""
     L = np.ascontiguousarray(loadings, dtype=np.float64)
     k = 1 if one_factor else L.shape[1]
     z = np.ascontiguousarray(normals, dtype=np.float64).ravel() if normals is not None else None

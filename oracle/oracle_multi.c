@@ -36,6 +36,18 @@
  * The outperformance and quanto variants (basket.rs:288-445) share the
  * loop: pinned by the lognormal ratio's closed forms (E[S1/S2] and the
  * Black formula on it) and the quanto-adjusted Black-Scholes call.
+ * The Student-t copula (t_dof >= 2; map_copula_normals_to_uniforms and
+ * copula_uniforms_to_normals, math/correlation.rs:545-582, 624-633) draws
+ * t_dof more normals per path after the asset (and factor) draws, scales
+ * the correlated normals by sqrt(max(chi2 / dof, 1e-16)) and maps them
+ * through StudentsT::cdf and the Acklam inverse. The t CDF restates
+ * statrs 0.18 (a dependency absent from the reference tree): beta_reg's
+ * Lentz continued fraction and ln_gamma's Lanczos sum, whose coefficients
+ * are the reference's own (src/math/gamma.rs:17-31). Pinned by scipy's
+ * t CDF and regularized incomplete beta, the closed forms at 2 and 4
+ * degrees of freedom, symmetry, scipy's gammaln; the copula by a one-asset
+ * price equal to Black-Scholes (the t-copula marginal is exactly normal),
+ * the Gaussian limit at large dof, and an independent numpy simulation.
  * Baskets (tests/test_oracle_basket.py): zero volatility (forward basket
  * intrinsic), one asset = Black-Scholes, a zero weight = the one-asset price
  * on the same first column of normals, best-of / worst-of of two = Margrabe,
@@ -216,6 +228,99 @@ void or_bump_corr_matrix(const double *c, int32_t n, double bump, double *out) {
 /* Basket Monte Carlo: src/pricing/basket.rs                              */
 /* ===================================================================== */
 
+/* statrs 0.18 function::gamma::ln_gamma: Lanczos with Pugh's r and d_k
+ * (the table of src/math/gamma.rs:17-31, which the reference ports from the
+ * same crate). */
+static const double OR_GAMMA_R = 10.900511;
+static const double OR_GAMMA_DK[11] = {
+    2.4857408913875355e-5, 1.0514237858172197, -3.4568709722201624, 4.512277094668948,
+    -2.9828522532357666, 1.056397115771267, -1.9542877319164587e-1, 1.709705434044412e-2,
+    -5.719261174043057e-4, 4.633994733599057e-6, -2.7199490848860772e-9};
+static const double OR_LN_2_SQRT_E_OVER_PI = 0.6207822376352452223455184457816472122518527279025978;
+static const double OR_LN_PI = 1.1447298858494001741434273513530587116472948129153;
+static const double OR_E = 2.7182818284590452353602874713526624977572470937;
+static const double OR_PI = 3.1415926535897932384626433832795028841971693993751;
+
+double or_ln_gamma(double x) {
+    double s = OR_GAMMA_DK[0];
+    if (x < 0.5) {
+        for (int i = 1; i < 11; ++i) s += OR_GAMMA_DK[i] / ((double)i - x);
+        return OR_LN_PI - log(sin(OR_PI * x)) - log(s) - OR_LN_2_SQRT_E_OVER_PI -
+               (0.5 - x) * log((0.5 - x + OR_GAMMA_R) / OR_E);
+    }
+    for (int i = 1; i < 11; ++i) s += OR_GAMMA_DK[i] / (x + (double)i - 1.0);
+    return log(s) + OR_LN_2_SQRT_E_OVER_PI + (x - 0.5) * log((x - 0.5 + OR_GAMMA_R) / OR_E);
+}
+
+/* statrs 0.18 function::beta::beta_reg: I_x(a, b) by the modified Lentz
+ * continued fraction (Numerical Recipes' betacf), the symmetry transform
+ * when x >= (a + 1) / (a + b + 2), at most 140 terms, tolerance 2^-53,
+ * prefactor 0 when x == 0 or x is within 4 ulps of 1. NaN outside the
+ * domain (the crate panics there). */
+double or_beta_reg(double a, double b, double x) {
+    if (!(a > 0.0) || !(b > 0.0) || !(x >= 0.0 && x <= 1.0)) return NAN;
+    const double bt = (x == 0.0 || x >= 1.0 - 4.0 * 0x1p-53)
+                          ? 0.0
+                          : exp(or_ln_gamma(a + b) - or_ln_gamma(a) - or_ln_gamma(b) + a * log(x) +
+                                b * log(1.0 - x));
+    const int symm = x >= (a + 1.0) / (a + b + 2.0);
+    const double eps = 0x1p-53, fpmin = 0x1p-1022 / eps;
+    if (symm) {
+        const double t = a;
+        x = 1.0 - x;
+        a = b;
+        b = t;
+    }
+    const double qab = a + b, qap = a + 1.0, qam = a - 1.0;
+    double c = 1.0, d = 1.0 - qab * x / qap;
+    if (fabs(d) < fpmin) d = fpmin;
+    d = 1.0 / d;
+    double h = d;
+    for (int mi = 1; mi < 141; ++mi) {
+        const double m = (double)mi, m2 = m * 2.0;
+        double aa = m * (b - m) * x / ((qam + m2) * (a + m2));
+        d = 1.0 + aa * d;
+        if (fabs(d) < fpmin) d = fpmin;
+        c = 1.0 + aa / c;
+        if (fabs(c) < fpmin) c = fpmin;
+        d = 1.0 / d;
+        h = h * d * c;
+        aa = -(a + m) * (qab + m) * x / ((a + m2) * (qap + m2));
+        d = 1.0 + aa * d;
+        if (fabs(d) < fpmin) d = fpmin;
+        c = 1.0 + aa / c;
+        if (fabs(c) < fpmin) c = fpmin;
+        d = 1.0 / d;
+        const double del = d * c;
+        h *= del;
+        if (fabs(del - 1.0) <= eps) break;
+    }
+    return symm ? 1.0 - bt * h / a : bt * h / a;
+}
+
+/* statrs 0.18 StudentsT::cdf at location 0, scale 1 (finite freedom) */
+double or_student_t_cdf(double x, double nu) {
+    const double h = nu / (nu + x * x);
+    const double ib = 0.5 * or_beta_reg(nu / 2.0, 0.5, h);
+    return x <= 0.0 ? ib : 1.0 - ib;
+}
+
+static double clamp_unit(double u) {   /* f64::clamp(1e-12, 1 - 1e-12) */
+    const double lo = 1.0e-12, hi = 1.0 - 1.0e-12;
+    return u < lo ? lo : (u > hi ? hi : u);
+}
+
+/* map_copula_normals_to_uniforms, Student-t (correlation.rs:559-570), then
+ * copula_uniforms_to_normals (:624-633) on the n correlated normals. */
+static void student_t_remap(int32_t dof, double chi2, double *z, int32_t n) {
+    const double nu = (double)dof;
+    const double scale = sqrt(fmax(chi2 / nu, 1.0e-16));
+    for (int32_t i = 0; i < n; ++i) {
+        const double u = clamp_unit(or_student_t_cdf(z[i] / scale, nu));
+        z[i] = or_acklam_inv_cdf(clamp_unit(u));
+    }
+}
+
 /* terminal_payoff (basket.rs:759-784) */
 static double basket_payoff(const or_basket *b, const double *s0, const double *st) {
     double u;
@@ -232,10 +337,10 @@ static double basket_payoff(const or_basket *b, const double *s0, const double *
     return b->is_call ? fmax(u - b->strike, 0.0) : fmax(b->strike - u, 0.0);
 }
 
-/* simulate_basket_once_matrix, Gaussian copula (basket.rs:599-683), on a PSD
- * correlation matrix (the reference repairs others first). normals NULL: the
- * engine's single stream from `seed`; else [path][asset]. Returns 0, 2 when
- * the matrix is not PSD. */
+/* simulate_basket_once_matrix (basket.rs:599-683), Gaussian or Student-t
+ * copula, on a PSD correlation matrix (the reference repairs others first).
+ * normals NULL: the engine's single stream from `seed`; else
+ * [path][asset (+ t_dof)]. Returns 0, 2 when the matrix is not PSD. */
 int or_basket_price(const or_basket *b, const double *normals, double *price, double *stderr_out) {
     const int32_t n = b->n_assets;
     if (b->maturity <= 0.0) {
@@ -250,25 +355,37 @@ int or_basket_price(const or_basket *b, const double *normals, double *price, do
         free(l);
         return 2;
     }
-    double drift[64], vol_t[64], indep[64], term[64];
+    double drift[64], vol_t[64], indep[64], corr[64], term[64];
     for (int32_t i = 0; i < n; ++i) {
         drift[i] = (b->rate - b->dividends[i] - 0.5 * b->vols[i] * b->vols[i]) * t;
         vol_t[i] = b->vols[i] * sqrt_t;
     }
+    const int64_t stride = n + (b->t_dof > 0 ? b->t_dof : 0);
     uint64_t st[4];
     or_xoshiro_seed(b->seed, st);
     /* RunningStats (basket.rs:66-97): Welford */
     double mean = 0.0, m2 = 0.0;
     int64_t cnt = 0;
     for (int64_t p = 0; p < b->num_paths; ++p) {
-        for (int32_t i = 0; i < n; ++i)
-            indep[i] = normals ? normals[p * n + i]
-                               : or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(st)));
-        for (int32_t i = 0; i < n; ++i) {
+        int64_t d = 0;
+#define OR_DRAW() (normals ? normals[p * stride + (d++)] \
+                           : (d++, or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(st)))))
+        for (int32_t i = 0; i < n; ++i) indep[i] = OR_DRAW();
+        for (int32_t i = 0; i < n; ++i) {   /* correlate_normals (correlation.rs:374-388) */
             double sum = 0.0;
             for (int32_t j = 0; j <= i; ++j) sum += l[i * n + j] * indep[j];
-            term[i] = b->spots[i] * exp(drift[i] + vol_t[i] * sum);
+            corr[i] = sum;
         }
+        if (b->t_dof > 0) {   /* sample_chi_square_integer_dof (correlation.rs:575-582) */
+            double chi2 = 0.0;
+            for (int32_t k = 0; k < b->t_dof; ++k) {
+                const double z = OR_DRAW();
+                chi2 += z * z;
+            }
+            student_t_remap(b->t_dof, chi2, corr, n);
+        }
+#undef OR_DRAW
+        for (int32_t i = 0; i < n; ++i) term[i] = b->spots[i] * exp(drift[i] + vol_t[i] * corr[i]);
         const double x = basket_payoff(b, b->spots, term);
         ++cnt;
         const double delta = x - mean;
@@ -395,13 +512,15 @@ int or_basket_factor_price(const or_basket *b, int one_factor, int32_t k, const 
             idio[i] = sqrt(fmax(1.0 - norm2, 0.0));
         }
     }
+    const int64_t stride = k + n + (b->t_dof > 0 ? b->t_dof : 0);
+    double zc[64];
     uint64_t st[4];
     or_xoshiro_seed(b->seed, st);
     double mean = 0.0, m2 = 0.0;
     int64_t cnt = 0, d;
     for (int64_t p = 0; p < b->num_paths; ++p) {
         d = 0;
-#define OR_DRAW() (normals ? normals[p * (k + n) + (d++)] \
+#define OR_DRAW() (normals ? normals[p * stride + (d++)] \
                            : (d++, or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(st)))))
         for (int32_t j = 0; j < k; ++j) f[j] = OR_DRAW();
         for (int32_t i = 0; i < n; ++i) {
@@ -416,9 +535,18 @@ int or_basket_factor_price(const or_basket *b, int one_factor, int32_t k, const 
                 const double eps = OR_DRAW();
                 z = sys + idio[i] * eps;
             }
-            term[i] = b->spots[i] * exp(drift[i] + vol_t[i] * z);
+            zc[i] = z;
+        }
+        if (b->t_dof > 0) {
+            double chi2 = 0.0;
+            for (int32_t j = 0; j < b->t_dof; ++j) {
+                const double z = OR_DRAW();
+                chi2 += z * z;
+            }
+            student_t_remap(b->t_dof, chi2, zc, n);
         }
 #undef OR_DRAW
+        for (int32_t i = 0; i < n; ++i) term[i] = b->spots[i] * exp(drift[i] + vol_t[i] * zc[i]);
         const double x = basket_payoff(b, b->spots, term);
         ++cnt;
         const double delta = x - mean;

@@ -255,6 +255,15 @@ int main() {
                                          {100.0, 90.0}, {0.2, 0.2}, {1, 0.3, 0.3, 1}, 0.03,
                                          {0.0, 0.0}, 1000);
         expect(std::isnan(bad.price) && !bad.stderr_, "basket_invalid_weights_is_nan");
+        const auto t1 = price_basket_mc_with_copula(
+            BasketOption{{1.0}, 100.0, 1.0, true, BasketType::Average}, {100.0}, {0.2}, {1.0},
+            0.03, {0.0}, 120000, BasketCopula::StudentT(4));
+        expect(std::fabs(t1.price - bs) <= 4.0 * *t1.stderr_,
+               "student_t_copula_one_asset_marginal_is_black_scholes", std::to_string(t1.price));
+        const auto tbad = price_basket_mc_with_copula(
+            BasketOption{{1.0}, 100.0, 1.0, true, BasketType::Average}, {100.0}, {0.2}, {1.0},
+            0.03, {0.0}, 100, BasketCopula::StudentT(1));
+        expect(std::isnan(tbad.price), "student_t_copula_requires_two_degrees_of_freedom");
         const auto sens = basket_sensitivities(
             BasketOption{{0.5, 0.5}, 100.0, 1.0, true, BasketType::Average}, {100.0, 90.0},
             {0.2, 0.25}, {1, 0.3, 0.3, 1}, 0.03, {0.0, 0.0}, 50000);

@@ -13,7 +13,13 @@ oracle's restatement of simulate_basket_once_matrix.
   the PSD Cholesky (an independent numpy projection fails it too), zero
   maturity prices the spots, invalid baskets give NaN and the reference's
   message;
-* fast draws: one asset against Black-Scholes within 4 standard errors.
+* fast draws: one asset against Black-Scholes within 4 standard errors;
+* the Student-t copula (price_basket_mc_with_copula, matrix and factor
+  model): a batch mixing Gaussian and t baskets matches the oracle on the
+  reference's stream and on external normals, the Gaussian baskets of the
+  batch equal their Gaussian-only prices, a one-asset t basket on fast draws
+  is Black-Scholes (the t-copula marginal is normal), and dof < 2 gives NaN
+  with the reference's message after the zero-maturity shortcut.
 """
 import math
 
@@ -61,12 +67,12 @@ def _basket(c):
     return BasketOption(c["weights"], c["strike"], c["maturity"], c["is_call"], c["kind"])
 
 
-def _oracle(oracle, c, normals=None, spots=None, vols=None, corr=None, seed=3_148_159):
+def _oracle(oracle, c, normals=None, spots=None, vols=None, corr=None, seed=3_148_159, t_dof=0):
     b = oracle.make_basket(spots if spots is not None else c["spots"],
                            vols if vols is not None else c["vols"], c["divs"],
                            corr if corr is not None else c["corr"],
                            c["weights"] if c["weights"] else None, c["kind"], c["is_call"],
-                           c["strike"], c["maturity"], c["rate"], c["paths"], seed)
+                           c["strike"], c["maturity"], c["rate"], c["paths"], seed, t_dof)
     return oracle.basket_price(b, normals)
 
 
@@ -275,3 +281,58 @@ def test_fused_sensitivities_match_scenario_by_scenario(gpu, monkeypatch, mode):
             assert abs(fs.delta[k] - as_.delta[k]) <= 1e-9 * max(1.0, abs(as_.delta[k]))
         assert abs(fs.vega - as_.vega) <= 1e-9 * max(1.0, abs(as_.vega))
         assert abs(fs.cega - as_.cega) <= 1e-9 * max(1.0, abs(as_.cega))
+
+
+def test_student_t_copula_matches_oracle(oracle, gpu):
+    from openferric_amd import (BasketCopula, BasketOption, FactorCorrelationModel, RngMode, _lib,
+                                basket_price_raw, price_basket_mc_with_copula,
+                                price_basket_mc_with_factor_model)
+    from openferric_amd.engines import _Keep
+    from openferric_amd.multi import _apply_copula
+    cases = [CASES[1], CASES[2], CASES[4], CASES[5], CASES[6]]
+    dofs = [3, 0, 7, 2, 5]
+    structs = _structs(cases, _Keep())
+    for st, d in zip(structs, dofs):
+        _apply_copula(st, BasketCopula.StudentT(d) if d else BasketCopula.Gaussian)
+    gauss_only, _ = basket_price_raw([structs[1]], RngMode.Reference)
+    for mode in (RngMode.Reference, RngMode.External):
+        zs = None
+        if mode == RngMode.External:
+            zs = [np.random.default_rng(70 + i).standard_normal((c["paths"], len(c["spots"]) + d))
+                  for i, (c, d) in enumerate(zip(cases, dofs))]
+        print("student-t batch", mode, flush=True)
+        res, _ = basket_price_raw(structs, mode, zs)
+        for i, (c, d, r) in enumerate(zip(cases, dofs, res)):
+            p, se = _oracle(oracle, c, None if zs is None else zs[i], t_dof=d)
+            assert r.has_stderr and abs(r.price - p) <= 1e-10 * abs(p), (d, r.price, p)
+            assert abs(r.stderr_ - se) <= 1e-8 * se, (d, r.stderr_, se)
+        if mode == RngMode.Reference:
+            assert abs(res[1].price - gauss_only[0].price) <= 1e-14 * abs(res[1].price)
+    # factor model under the t copula
+    spots, vols, divs = [100.0, 101.0, 99.0, 97.0], [0.2, 0.22, 0.18, 0.21], [0.0] * 4
+    load = [[0.6, 0.1], [0.5, -0.1], [0.4, 0.2], [0.35, -0.2]]
+    w = [0.4, 0.3, 0.2, 0.1]
+    paths, dof = 70_001, 4
+    for mode, z in ((RngMode.Reference, None),
+                    (RngMode.External, np.random.default_rng(9).standard_normal((paths, 6 + dof)))):
+        r = price_basket_mc_with_factor_model(
+            BasketOption(w, 100.0, 1.0, False), spots, vols, FactorCorrelationModel.MultiFactor(load),
+            0.02, divs, paths, BasketCopula.StudentT(dof), rng_mode=mode, normals=z)
+        b = oracle.make_basket(spots, vols, divs, np.eye(4), w, AVG, False, 100.0, 1.0, 0.02, paths,
+                               t_dof=dof)
+        p, se = oracle.basket_factor_price(b, load, False, z)
+        assert abs(r.price - p) <= 1e-10 * p and abs(r.stderr - se) <= 1e-8 * se, (r.price, p)
+    # fast draws: one asset is Black-Scholes for any dof
+    r = price_basket_mc_with_copula(BasketOption([1.0], 100.0, 1.0, True), [100.0], [0.3], [[1.0]],
+                                    0.03, [0.01], 400_000, BasketCopula.StudentT(3),
+                                    rng_mode=RngMode.Fast)
+    want = bs_price(True, 100.0, 100.0, 0.03, 0.01, 0.3, 1.0)
+    assert abs(r.price - want) < 4 * r.stderr, (r.price, want, r.stderr)
+    # dof < 2: NaN and the reference's message, except under the zero-maturity shortcut
+    r = price_basket_mc_with_copula(BasketOption([1.0], 100.0, 1.0, True), [100.0], [0.3], [[1.0]],
+                                    0.03, [0.0], 1000, BasketCopula.StudentT(1))
+    assert math.isnan(r.price) and r.stderr is None
+    assert "degrees_of_freedom >= 2" in _lib.last_error()
+    r = price_basket_mc_with_copula(BasketOption([1.0], 90.0, 0.0, True), [100.0], [0.3], [[1.0]],
+                                    0.03, [0.0], 1000, BasketCopula.StudentT(1))
+    assert r.price == 10.0 and r.stderr == 0.0

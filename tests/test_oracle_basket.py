@@ -254,3 +254,124 @@ def test_multi_factor_matches_cholesky_basket_on_implied_matrix(oracle):
     bz = oracle.make_basket(*args, np.eye(5), w, AVG, True, 100.0, 1.0, 0.02, 1000)
     assert oracle.basket_factor_price(bz, load, False, z) != oracle.basket_factor_price(
         bz, load, False, z[:, ::-1].copy())   # the draw order matters: factors first
+
+
+# ---- Student-t copula (basket.rs:120-160; math/correlation.rs:545-633) ------
+
+def test_student_t_cdf_matches_scipy_and_closed_forms(oracle):
+    """The oracle's restatement of statrs 0.18's StudentsT::cdf, beta_reg and
+    ln_gamma against scipy's independent implementations, mpmath's
+    regularized incomplete beta and the closed forms at 2 and 4 degrees of
+    freedom."""
+    from scipy import special, stats
+    import mpmath as mp
+    for nu in (2, 3, 4, 5, 7, 10, 30, 100, 1000):
+        xs = np.concatenate([np.linspace(-30, 30, 241), np.geomspace(1e-2, 1e4, 40),
+                             -np.geomspace(1e-2, 1e4, 40)])
+        for x in xs:
+            got, want = oracle.student_t_cdf(x, nu), stats.t.cdf(x, nu)
+            if want > 1e-250:
+                assert abs(got - want) <= 1e-11 * want, (nu, x, got, want)
+            assert abs(got + oracle.student_t_cdf(-x, nu) - 1.0) <= 2e-16
+    for x in (-7.0, -1.3, -0.2, 0.05, 0.9, 3.0, 25.0):
+        f2 = 0.5 + x / (2.0 * math.sqrt(x * x + 2.0))
+        t = x * x / 4.0
+        f4 = 0.5 + 0.375 * (x / math.sqrt(1.0 + t)) * (1.0 - t / (3.0 * (1.0 + t)))
+        assert abs(oracle.student_t_cdf(x, 2) - f2) <= 1e-14
+        assert abs(oracle.student_t_cdf(x, 4) - f4) <= 1e-14
+    # statrs evaluates I_h at h = nu / (nu + x^2), which rounds to 1 near 0
+    assert oracle.student_t_cdf(1e-9, 100) == 0.5 and oracle.student_t_cdf(0.0, 5) == 0.5
+    mp.mp.dps = 40
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        a, b, x = rng.uniform(0.5, 60.0), rng.choice([0.5, rng.uniform(0.2, 8.0)]), rng.uniform()
+        want = float(mp.betainc(a, b, 0, x, regularized=True))
+        if want > 1e-250:
+            assert abs(oracle.beta_reg(a, b, x) - want) <= 5e-12 * want, (a, b, x)
+    for x in (0.5, 1.0, 1.5, 2.5, 7.0, 33.5, 150.0, 0.2):
+        assert abs(oracle.ln_gamma(x) - special.gammaln(x)) <= 1e-12 * max(1.0, abs(special.gammaln(x)))
+
+
+def _numpy_t_copula_price(spots, vols, divs, corr_or_none, load, one_factor, dof, weights, kind,
+                          is_call, strike, t, r, z):
+    """An independent restatement with scipy's t CDF and exact normal
+    quantile, on the same draws: rows [factors..., assets..., chi-square...]."""
+    from scipy import stats
+    n = len(spots)
+    if corr_or_none is not None:
+        c = np.linalg.cholesky(np.asarray(corr_or_none)) @ z[:, :n].T
+        k = 0
+    else:
+        L = np.asarray(load, dtype=np.float64).reshape(n, -1)
+        k = L.shape[1]
+        idio = np.sqrt(np.maximum(1.0 - (L * L).sum(axis=1), 0.0))
+        c = L @ z[:, :k].T + idio[:, None] * z[:, k:k + n].T
+    chi2 = (z[:, k + n:k + n + dof] ** 2).sum(axis=1)
+    scale = np.sqrt(np.maximum(chi2 / dof, 1e-16))
+    u = np.clip(stats.t.cdf(c / scale, dof), 1e-12, 1 - 1e-12)
+    zz = stats.norm.ppf(u)
+    st = np.asarray(spots)[:, None] * np.exp(
+        ((r - np.asarray(divs) - 0.5 * np.asarray(vols) ** 2) * t)[:, None]
+        + (np.asarray(vols) * math.sqrt(t))[:, None] * zz)
+    if kind == AVG:
+        u = (np.asarray(weights)[:, None] * st).sum(axis=0)
+    else:
+        rel = st / np.asarray(spots)[:, None]
+        u = rel.max(axis=0) if kind == BEST else rel.min(axis=0)
+    x = np.maximum(u - strike, 0.0) if is_call else np.maximum(strike - u, 0.0)
+    return math.exp(-r * t) * x.mean()
+
+
+@pytest.mark.parametrize("dof", [3, 8])
+def test_t_copula_matches_independent_numpy_on_same_draws(oracle, dof):
+    """Same draws, independent arithmetic (numpy Cholesky, scipy t CDF and
+    exact normal quantile): agreement to the Acklam inverse's 1.2e-9."""
+    spots, vols, divs = [100.0, 90.0, 110.0], [0.25, 0.3, 0.2], [0.01, 0.0, 0.02]
+    corr = [[1, 0.4, 0.2], [0.4, 1, -0.1], [0.2, -0.1, 1]]
+    w = [0.5, 0.3, 0.2]
+    n = 4000
+    z = np.random.default_rng(dof).standard_normal((n, 3 + dof))
+    for kind, strike, is_call in ((AVG, 100.0, True), (WORST, 0.9, False), (BEST, 1.05, True)):
+        b = oracle.make_basket(spots, vols, divs, corr, w if kind == AVG else None, kind, is_call,
+                               strike, 1.25, 0.03, n, t_dof=dof)
+        p, _ = oracle.basket_price(b, z)
+        want = _numpy_t_copula_price(spots, vols, divs, corr, None, False, dof, w, kind, is_call,
+                                     strike, 1.25, 0.03, z)
+        assert abs(p - want) <= 1e-7 * max(want, 1e-3), (kind, p, want)
+    load = [[0.6, 0.2], [0.5, -0.3], [0.4, 0.4]]
+    zf = np.random.default_rng(40 + dof).standard_normal((n, 2 + 3 + dof))
+    b = oracle.make_basket(spots, vols, divs, np.eye(3), w, AVG, True, 100.0, 1.0, 0.02, n,
+                           t_dof=dof)
+    p, _ = oracle.basket_factor_price(b, load, False, zf)
+    want = _numpy_t_copula_price(spots, vols, divs, None, load, False, dof, w, AVG, True, 100.0,
+                                 1.0, 0.02, zf)
+    assert abs(p - want) <= 1e-7 * want, (p, want)
+
+
+def test_t_copula_one_asset_marginal_is_black_scholes(oracle):
+    """T = z / sqrt(chi2_nu / nu) is exactly t_nu, so Phi^-1(F_nu(T)) is
+    exactly normal: one asset prices at Black-Scholes whatever the dof."""
+    for dof, is_call, k in ((2, True, 100.0), (5, False, 105.0)):
+        b = oracle.make_basket([100.0], [0.3], [0.01], [[1.0]], [1.0], AVG, is_call, k, 1.0, 0.03,
+                               300_000, t_dof=dof)
+        p, se = oracle.basket_price(b)
+        want = bs_price(is_call, 100.0, k, 0.03, 0.01, 0.3, 1.0)
+        assert abs(p - want) < 4 * se, (dof, p, want, se)
+
+
+def test_t_copula_stream_order_and_tail_dependence(oracle):
+    """The single stream draws each path's asset normals, then its dof
+    chi-square normals; with zero correlation the t copula still couples the
+    tails, so a best-of put (paid when every asset falls) is dearer than under
+    the Gaussian copula and a worst-of put (paid when any falls) cheaper."""
+    n, m, dof = 2_000, 3, 4
+    b = oracle.make_basket([100, 90, 110], [0.2, 0.3, 0.25], [0.0] * 3, np.eye(3), None, WORST,
+                           False, 1.0, 1.0, 0.01, n, seed=77, t_dof=dof)
+    z = oracle.lsm_stream_normals(77, n * (m + dof)).reshape(n, m + dof)
+    assert oracle.basket_price(b) == oracle.basket_price(b, z)
+    for kind, sign in ((BEST, 1.0), (WORST, -1.0)):
+        args = ([100.0, 100.0], [0.3, 0.3], [0.0, 0.0], np.eye(2), None, kind, False, 0.8, 1.0,
+                0.02, 400_000)
+        pt, st = oracle.basket_price(oracle.make_basket(*args, t_dof=3))
+        pg, sg = oracle.basket_price(oracle.make_basket(*args))
+        assert sign * (pt - pg) > 6 * math.hypot(st, sg), (kind, pt, pg)
