@@ -716,6 +716,8 @@ def main():
     ap.add_argument("--no-dsl", action="store_true", help="skip the structured-product leg")
     ap.add_argument("--dsl-only", action="store_true",
                     help="run only the structured-product leg (profiling)")
+    ap.add_argument("--heston-only", action="store_true",
+                    help="run only the two Heston legs (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one D2D copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -741,6 +743,13 @@ def main():
         return
     if args.dsl_only:
         print(json.dumps({"dsl": dsl_leg(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.heston_only:
+        print("[bench] heston barrier leg", file=sys.stderr, flush=True)
+        h = heston_leg(RngMode.Fast)
+        print("[bench] heston adjoint leg", file=sys.stderr, flush=True)
+        print(json.dumps({"heston": h, "heston_greeks": heston_greeks_leg(RngMode.Fast)}),
+              flush=True)
         return
 
     opts = make_workload(path_scale=args.path_scale)
