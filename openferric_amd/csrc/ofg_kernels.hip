@@ -31,6 +31,7 @@ namespace ofg {
 namespace {
 
 constexpr int WAVE = 64;
+constexpr int REF_AK = 8;   // reference draws per deferred-tail inverse-CDF block
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -93,13 +94,21 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
                 if (j + 1 < steps) advance(z1, j + 1);
                 if (j + 2 < steps) advance(z2, j + 2);
             }
-        } else {
-            for (int j = 0; j < steps; ++j) {
-                double z;
-                if constexpr (RNG == OFG_RNG_REFERENCE) z = reference_normal(rng);
-                else z = ext.z1(j);
-                advance(z, j);
+        } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+            // the reference's draws REF_AK at a time (the inverse CDF with
+            // its tail region deferred), applied in order
+            for (int j = 0; j < steps; j += REF_AK) {
+                double p[REF_AK], z[REF_AK];
+#pragma unroll
+                for (int i = 0; i < REF_AK; ++i)
+                    p[i] = j + i < steps ? uniform_open01(rng.next_f64()) : 0.5;
+                acklam_block<REF_AK>(p, z);
+#pragma unroll
+                for (int i = 0; i < REF_AK; ++i)
+                    if (j + i < steps) advance(z[i], j + i);
             }
+        } else {
+            for (int j = 0; j < steps; ++j) advance(ext.z1(j), j);
         }
     } else {
         double va = o.v0, vb = o.v0;
@@ -123,18 +132,20 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
                 normals4(rng, z0, z1, z2, z3, tab);
                 advance(z0, z1, j);
             }
-        } else {
-            for (int j = 0; j < steps; ++j) {
-                double z1, z2;
-                if constexpr (RNG == OFG_RNG_REFERENCE) {
-                    z1 = reference_normal(rng);
-                    z2 = reference_normal(rng);
-                } else {
-                    z1 = ext.z1(j);
-                    z2 = ext.z2(j);
-                }
-                advance(z1, z2, j);
+        } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+            // z1, z2 of REF_AK / 2 steps per block, in the reference's order
+            for (int j = 0; j < steps; j += REF_AK / 2) {
+                double p[REF_AK], z[REF_AK];
+#pragma unroll
+                for (int i = 0; i < REF_AK; ++i)
+                    p[i] = j + i / 2 < steps ? uniform_open01(rng.next_f64()) : 0.5;
+                acklam_block<REF_AK>(p, z);
+#pragma unroll
+                for (int i = 0; i < REF_AK / 2; ++i)
+                    if (j + i < steps) advance(z[2 * i], z[2 * i + 1], j + i);
             }
+        } else {
+            for (int j = 0; j < steps; ++j) advance(ext.z1(j), ext.z2(j), j);
         }
     }
 

@@ -18,6 +18,7 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int NWAVES = QMC_BLOCK / WAVE;
+constexpr int QMC_AK = 8;   // inverse-CDF values per deferred-tail block
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -61,12 +62,21 @@ __global__ __launch_bounds__(QMC_BLOCK) void qmc_kernel(QmcArgs a) {
                 xhi[wave][dd] = x;
             }
             __syncthreads();
-            for (int dd = 0; dd < seg; ++dd) {
-                const int d = d0 + dd;
-                const uint64_t x = xhi[wave][dd] ^ lows[(int64_t)d * 64 + low] ^ scr[d];
-                const double u = fma((double)x, inv, half);
-                const double z = acklam_inv_cdf(uniform_open01(u));
-                spot *= exp_fp64(fma(o.dt_vol, z, o.dt_drift));
+            // QMC_AK dimensions at a time: their uniforms, the inverse CDF
+            // with the tail region deferred, then the steps in order
+            for (int db = 0; db < seg; db += QMC_AK) {
+                double p[QMC_AK], z[QMC_AK];
+#pragma unroll
+                for (int i = 0; i < QMC_AK; ++i) {
+                    const int dd = min(db + i, seg - 1);
+                    const int d = d0 + dd;
+                    const uint64_t x = xhi[wave][dd] ^ lows[(int64_t)d * 64 + low] ^ scr[d];
+                    p[i] = uniform_open01(fma((double)x, inv, half));
+                }
+                acklam_block<QMC_AK>(p, z);
+#pragma unroll
+                for (int i = 0; i < QMC_AK; ++i)
+                    if (db + i < seg) spot *= exp_fp64(fma(o.dt_vol, z[i], o.dt_drift));
             }
             __syncthreads();
         }
