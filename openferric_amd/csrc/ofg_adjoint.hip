@@ -81,11 +81,21 @@ __device__ __forceinline__ void seg_forward(RngOf<RNG> &rng, const ExternalNorma
             }
         }
     } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+        // the segment's uniforms, then the inverse CDF with its tail region
+        // deferred over the whole segment (acklam_block)
+        constexpr int PER = MODEL == OFG_MODEL_HESTON ? 2 : 1;
+        double p[PER * K], z[PER * K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int e = 0; e < PER; ++e)
+                p[PER * k + e] = k < len ? uniform_open01(rng.next_f64()) : 0.5;
+        acklam_block<PER * K>(p, z);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (k < len) {
-                z1[k] = sign * reference_normal(rng);
-                if constexpr (MODEL == OFG_MODEL_HESTON) z2[k] = sign * reference_normal(rng);
+                z1[k] = sign * z[PER * k];
+                if constexpr (MODEL == OFG_MODEL_HESTON) z2[k] = sign * z[PER * k + 1];
             }
         }
     } else {
@@ -131,15 +141,22 @@ __device__ __forceinline__ void seg_reverse(RngOf<RNG> &rng, const ExternalNorma
             }
         }
     } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+        constexpr int PER = MODEL == OFG_MODEL_HESTON ? 2 : 1;
+        double p[PER * K], z[PER * K];
 #pragma unroll
-        for (int k = K - 1; k >= 0; --k) {
+        for (int k = K - 1; k >= 0; --k)
+#pragma unroll
+            for (int e = PER - 1; e >= 0; --e)
+                p[PER * k + e] =
+                    k < len ? uniform_open01((double)(rng.prev_u64() >> 11) *
+                                             (1.0 / 9007199254740992.0))
+                            : 0.5;
+        acklam_block<PER * K>(p, z);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
             if (k < len) {
-                if constexpr (MODEL == OFG_MODEL_HESTON) {
-                    const double u2 = (double)(rng.prev_u64() >> 11) * (1.0 / 9007199254740992.0);
-                    z2[k] = sign * acklam_inv_cdf(uniform_open01(u2));
-                }
-                const double u1 = (double)(rng.prev_u64() >> 11) * (1.0 / 9007199254740992.0);
-                z1[k] = sign * acklam_inv_cdf(uniform_open01(u1));
+                z1[k] = sign * z[PER * k];
+                if constexpr (MODEL == OFG_MODEL_HESTON) z2[k] = sign * z[PER * k + 1];
             }
         }
     } else {

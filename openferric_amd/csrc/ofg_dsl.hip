@@ -104,12 +104,19 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
 #pragma unroll
         for (int i = 0; i < MAXA; ++i) c[i] = 0.0;
         double zpair = 0.0;
+        double zr[MAXA];   // reference draws of the step, tail region deferred
+        if constexpr (RNG == OFG_RNG_REFERENCE) {
+            double pr[MAXA];
+#pragma unroll
+            for (int j = 0; j < MAXA; ++j) pr[j] = j < n ? uniform_open01(ref.next_f64()) : 0.5;
+            acklam_block<MAXA>(pr, zr);
+        }
 #pragma unroll
         for (int j = 0; j < MAXA; ++j) {
             if (j < n) {
                 double z;
                 if constexpr (RNG == OFG_RNG_REFERENCE) {
-                    z = reference_normal(ref);
+                    z = zr[j];
                 } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
                     z = zp[((int64_t)(step - 1) * n + j) * o.paths];
                 } else {
@@ -374,11 +381,19 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
         for (int step = 1; step <= a.steps; ++step) {
             double z[MAXA], c0[MAXA], g0[MAXA];
             double zpair = 0.0;
+            double zr[MAXA];   // reference draws of the step, tail region deferred
+            if constexpr (RNG == OFG_RNG_REFERENCE) {
+                double pr[MAXA];
+#pragma unroll
+                for (int j2 = 0; j2 < MAXA; ++j2)
+                    pr[j2] = j2 < n ? uniform_open01(ref.next_f64()) : 0.5;
+                acklam_block<MAXA>(pr, zr);
+            }
 #pragma unroll
             for (int j2 = 0; j2 < MAXA; ++j2) {
                 if (j2 < n) {
                     if constexpr (RNG == OFG_RNG_REFERENCE) {
-                        z[j2] = reference_normal(ref);
+                        z[j2] = zr[j2];
                     } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
                         z[j2] = zp[((int64_t)(step - 1) * n + j2) * o.paths];
                     } else {

@@ -273,11 +273,8 @@ __device__ __forceinline__ void drive(const LsmDev &o, const LsmArgs &a, int64_t
         rng.s1 = sd[1];
         rng.s2 = sd[2];
         rng.s3 = sd[3];
-        for (int t = 1; t <= n; ++t) {
-            const double z1 = reference_normal(rng);
-            const double z2 = TWO ? reference_normal(rng) : 0.0;
-            step(z1, z2, t);
-        }
+        reference_steps<8, TWO ? 2 : 1>(rng, n,
+                                        [&](int j, double z1, double z2) { step(z1, z2, j + 1); });
     } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
         const double *e1 = a.normals1 + o.normals_offset + p;
         const double *e2 = TWO ? a.normals2 + o.normals_offset + p : nullptr;

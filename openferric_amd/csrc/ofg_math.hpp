@@ -248,6 +248,30 @@ __device__ __forceinline__ void acklam_block(const double (&p)[K], double (&z)[K
     }
 }
 
+// n_steps path steps of PER reference normals each (Heston: z1 then z2),
+// drawn K at a time from the stream and inverted by acklam_block; f(j, z1,
+// z2) runs for j = 0 .. n_steps - 1 in order (z2 = 0 when PER is 1). The
+// stream advances by exactly PER * n_steps draws.
+template <int K, int PER, class RNG, class F>
+__device__ __forceinline__ void reference_steps(RNG &rng, int n_steps, F &&f) {
+    static_assert(K % PER == 0, "whole steps per block");
+    constexpr int SPB = K / PER;
+    for (int j = 0; j < n_steps; j += SPB) {
+        double p[K], z[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            p[i] = j + i / PER < n_steps ? uniform_open01(rng.next_f64()) : 0.5;
+        acklam_block<K>(p, z);
+#pragma unroll
+        for (int i = 0; i < SPB; ++i) {
+            if (j + i < n_steps) {
+                if constexpr (PER == 2) f(j + i, z[2 * i], z[2 * i + 1]);
+                else f(j + i, z[i], 0.0);
+            }
+        }
+    }
+}
+
 // sample_standard_normal (fast_rng.rs:223-226) on a xoshiro stream.
 OFG_HD double reference_normal(Xoshiro256pp &rng) {
     return acklam_inv_cdf(uniform_open01(rng.next_f64()));

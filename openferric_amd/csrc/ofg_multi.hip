@@ -295,10 +295,19 @@ __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const d
         return z;
     };
     if (o.n_factors == 0) {
+        double zr[BK_MAX];   // reference draws of the assets, tail region deferred
+        if constexpr (RNG == OFG_RNG_REFERENCE) {
+            double pr[BK_MAX];
+#pragma unroll
+            for (int j = 0; j < BK_MAX; ++j) pr[j] = j < n ? uniform_open01(ref.next_f64()) : 0.5;
+            acklam_block<BK_MAX>(pr, zr);
+        }
 #pragma unroll
         for (int j = 0; j < BK_MAX; ++j) {
             if (j < n) {
-                const double z = draw(j);
+                double z;
+                if constexpr (RNG == OFG_RNG_REFERENCE) z = zr[j];
+                else z = draw(j);
 #pragma unroll
                 for (int i = j; i < BK_MAX; ++i)
                     if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
@@ -838,14 +847,19 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_greeks_kernel(BkgArgs a) {
             }
             if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
             double zpair = 0.0;
+            if constexpr (RNG == OFG_RNG_REFERENCE) {
+                // the path's n draws, tail region deferred (acklam_block)
+                double pr[MAXN];
+#pragma unroll
+                for (int i = 0; i < MAXN; ++i) pr[i] = i < n ? uniform_open01(ref.next_f64()) : 0.5;
+                acklam_block<MAXN>(pr, z);
+            }
 #pragma unroll
             for (int i = 0; i < MAXN; ++i) {
                 if (i < n) {
-                    if constexpr (RNG == OFG_RNG_REFERENCE) {
-                        z[i] = reference_normal(ref);
-                    } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
+                    if constexpr (RNG == OFG_RNG_EXTERNAL) {
                         z[i] = a.normals[o.normals_offset + (int64_t)i * o.paths + p];
-                    } else {
+                    } else if constexpr (RNG == OFG_RNG_FAST) {
                         if ((i & 1) == 0) {
                             const uint64_t wu = fast.next_u64();
                             const uint64_t wt = fast.next_u64();

@@ -137,13 +137,10 @@ __device__ __forceinline__ double pathwise_sample(const OptDev &o, int64_t sampl
             if (j + 1 < steps) advance(z1, j + 1);
             if (j + 2 < steps) advance(z2, j + 2);
         }
+    } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+        reference_steps<8, 1>(rng, steps, [&](int j, double z, double) { advance(z, j); });
     } else {
-        for (int j = 0; j < steps; ++j) {
-            double z;
-            if constexpr (RNG == OFG_RNG_REFERENCE) z = reference_normal(rng);
-            else z = ext.z1(j);
-            advance(z, j);
-        }
+        for (int j = 0; j < steps; ++j) advance(ext.z1(j), j);
     }
 
     double pv = leg_gradient<PK>(ga, o, Z, sa, G2, gr);
@@ -319,18 +316,10 @@ __device__ __forceinline__ double heston_tangent_sample(const OptDev &o, int64_t
             normals4(rng, z0, z1, z2, z3, tab);
             advance(z0, z1, j);
         }
+    } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+        reference_steps<8, 2>(rng, steps, [&](int j, double z1, double z2) { advance(z1, z2, j); });
     } else {
-        for (int j = 0; j < steps; ++j) {
-            double z1, z2;
-            if constexpr (RNG == OFG_RNG_REFERENCE) {
-                z1 = reference_normal(rng);
-                z2 = reference_normal(rng);
-            } else {
-                z1 = ext.z1(j);
-                z2 = ext.z2(j);
-            }
-            advance(z1, z2, j);
-        }
+        for (int j = 0; j < steps; ++j) advance(ext.z1(j), ext.z2(j), j);
     }
     double pv = heston_leg_gradient<PK>(ga, o, ta, gr);
     if constexpr (ANTI) {
