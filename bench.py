@@ -810,7 +810,9 @@ def main():
         "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": PRICE_KERNEL_HBM_BYTES if args.path_scale == 1.0 else None,
-                     "kernel": "price_kernel<RNG_FAST,256>",
+                     # the instantiation launched: threads per block from the autotuner
+                     # (the library's default shape, 256, without it)
+                     "kernel": f"price_kernel<RNG_FAST, GBM, {256 if tuned is None else tuned[0]}>",
                      "kernel_ms_avg": k_ms,
                      "flop_model": "65 FLOP per GBM path-step, 66 with Asian running sum"},
     }
