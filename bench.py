@@ -31,10 +31,12 @@ FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector peak
 FLOP_GBM_STEP = 65            # canonical FLOP per GBM path-step
 FLOP_ASIAN_STEP = 66          # + arithmetic running sum
 WORKLOAD_SEED = 0x0FE22C1C
-# HBM bytes per launch of the pricing kernel on the default workload, from
-# rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) scaled by a 1 GiB
-# copy kernel (profiles/price_kernel_fast_gbm_batch4096_counters.md).
-PRICE_KERNEL_HBM_BYTES = 91.3e6 + 38.7e6
+# HBM bytes per launch of the pricing kernel on the default workload, by
+# threads per block of the instantiation, from rocprofv3 FETCH_SIZE (x2.000)
+# and WRITE_SIZE (x1.000) scaled by a 1 GiB copy kernel
+# (profiles/price_kernel_fast_gbm_batch4096_counters.md); other shapes are
+# unmeasured.
+PRICE_KERNEL_HBM_BYTES = {64: 77.3e6 + 37.4e6, 256: 91.3e6 + 38.7e6}
 
 
 def make_workload(n_opts=4096, seed=WORKLOAD_SEED, path_scale=1.0):
@@ -809,7 +811,8 @@ def main():
                    {"block": tuned[0], "item_lane_steps": tuned[1]}},
         "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": PRICE_KERNEL_HBM_BYTES if args.path_scale == 1.0 else None,
+                     "traffic": PRICE_KERNEL_HBM_BYTES.get(256 if tuned is None else tuned[0])
+                     if args.path_scale == 1.0 else None,
                      # the instantiation launched: threads per block from the autotuner
                      # (the library's default shape, 256, without it)
                      "kernel": f"price_kernel<RNG_FAST, GBM, {256 if tuned is None else tuned[0]}>",
