@@ -43,10 +43,25 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// A GBM step's factor exp(p0 + p1 z): the reference's exp on its own and
+// external draws; the table-driven 2^(y/256) of the fast mode otherwise
+// (fast draws are checked statistically, like every fast-mode kernel).
+template <bool FAST>
+__device__ __forceinline__ double gbm_factor(double p0, double p1, double z, const double *tab) {
+    if constexpr (FAST) {
+        double y = (p0 + p1 * z) * OFG_INV_LN2_256;
+        y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
+        return exp2_256(y, tab);
+    } else {
+        return exp_fp64(p0 + p1 * z);
+    }
+}
+
 // AssetStepper::step (engine.rs:95-122)
+template <bool FAST>
 __device__ __forceinline__ double step_asset(int kind, double prev, double p0, double p1, double p2,
-                                             double z) {
-    if (kind == DSL_GBM) return prev * exp_fp64(p0 + p1 * z);
+                                             double z, const double *tab) {
+    if (kind == DSL_GBM) return prev * gbm_factor<FAST>(p0, p1, z, tab);
     if (kind == DSL_SCHWARTZ) {
         const double log_prev = log(prev);
         const double log_next = p0 + p1 * (log_prev - p0) + p2 * z;
@@ -135,7 +150,9 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
         }
 #pragma unroll
         for (int i = 0; i < MAXA; ++i)
-            if (i < n) s[i] = step_asset(o.kind[i], s[i], o.p0[i], o.p1[i], o.p2[i], c[i]);
+            if (i < n)
+                s[i] = step_asset<RNG == OFG_RNG_FAST>(o.kind[i], s[i], o.p0[i], o.p1[i], o.p2[i],
+                                                       c[i], tab);
         const int k = a.step_snap[step];
         if (k >= 0) {
 #pragma unroll
@@ -419,7 +436,9 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
                     if (i < n && j2 < n) c0[i] += o.chol[i * (i + 1) / 2 + j2] * z[j2];
 #pragma unroll
             for (int i = 0; i < MAXA; ++i)
-                g0[i] = (i < n && o.kind[i] == DSL_GBM) ? exp_fp64(o.p0[i] + o.p1[i] * c0[i]) : 1.0;
+                g0[i] = (i < n && o.kind[i] == DSL_GBM)
+                            ? gbm_factor<RNG == OFG_RNG_FAST>(o.p0[i], o.p1[i], c0[i], tab)
+                            : 1.0;
             const int ks = a.step_snap[step];
 #pragma unroll
             for (int m = 0; m < DSL_FUSE_MAXS; ++m) {
@@ -442,7 +461,9 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
                 for (int i = 0; i < MAXA; ++i) {
                     if (i < n) {
                         if ((q.share_mask >> i) & 1u) s[m][i] = s[m][i] * g0[i];
-                        else s[m][i] = step_asset(q.kind[i], s[m][i], q.p0[i], q.p1[i], q.p2[i], c[i]);
+                        else
+                            s[m][i] = step_asset<RNG == OFG_RNG_FAST>(q.kind[i], s[m][i], q.p0[i],
+                                                                      q.p1[i], q.p2[i], c[i], tab);
                     }
                 }
                 if (ks >= 0) {
