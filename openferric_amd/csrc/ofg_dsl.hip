@@ -88,10 +88,53 @@ __device__ __forceinline__ bool index_in_range(double v, int n, int &idx, double
     return true;
 }
 
+// One market's step parameters as the simulation reads them. Up to four
+// assets they are copied out of the scenario once per kernel, before any
+// snapshot store: the stores could alias the scenario as far as the compiler
+// knows, so a read inside the step loop would be a vector reload with a
+// memory wait every step, where the copies stay in scalar registers. Wider
+// markets read the scenario in place (their copies would not fit).
+template <int MAXA, bool COPY = (MAXA <= 4)>
+struct StepParams;
+
+template <int MAXA>
+struct StepParams<MAXA, true> {
+    int kind_[MAXA];
+    double p0_[MAXA], p1_[MAXA], p2_[MAXA], ch_[MAXA * (MAXA + 1) / 2];
+    __device__ explicit StepParams(const DslScen &o) {
+#pragma unroll
+        for (int i = 0; i < MAXA; ++i) {
+            kind_[i] = o.kind[i];
+            p0_[i] = o.p0[i];
+            p1_[i] = o.p1[i];
+            p2_[i] = o.p2[i];
+        }
+#pragma unroll
+        for (int q = 0; q < MAXA * (MAXA + 1) / 2; ++q) ch_[q] = o.chol[q];
+    }
+    __device__ int kind(int i) const { return kind_[i]; }
+    __device__ double p0(int i) const { return p0_[i]; }
+    __device__ double p1(int i) const { return p1_[i]; }
+    __device__ double p2(int i) const { return p2_[i]; }
+    __device__ double chol(int q) const { return ch_[q]; }
+};
+
+template <int MAXA>
+struct StepParams<MAXA, false> {
+    const DslScen &o;
+    __device__ explicit StepParams(const DslScen &scen) : o(scen) {}
+    __device__ int kind(int i) const { return o.kind[i]; }
+    __device__ double p0(int i) const { return o.p0[i]; }
+    __device__ double p1(int i) const { return o.p1[i]; }
+    __device__ double p2(int i) const { return o.p2[i]; }
+    __device__ double chol(int q) const { return o.chol[q]; }
+};
+
 // MAXA: the asset count rounded up to 1, 2, 4, 8 or 16 (registers per path).
 // Snapshot k of asset i goes to snap[(k n + i) stride].
 template <int RNG, int MAXA>
-__device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o, int64_t p,
+__device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o,
+                                              const StepParams<MAXA> &sp, int64_t p,
                                               double *snap, int64_t stride, const double *tab) {
     const int n = o.n_assets;
     double s[MAXA];
@@ -145,14 +188,14 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
                 }
 #pragma unroll
                 for (int i = j; i < MAXA; ++i)
-                    if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
+                    if (i < n) c[i] += sp.chol(i * (i + 1) / 2 + j) * z;
             }
         }
 #pragma unroll
         for (int i = 0; i < MAXA; ++i)
             if (i < n)
-                s[i] = step_asset<RNG == OFG_RNG_FAST>(o.kind[i], s[i], o.p0[i], o.p1[i], o.p2[i],
-                                                       c[i], tab);
+                s[i] = step_asset<RNG == OFG_RNG_FAST>(sp.kind(i), s[i], sp.p0(i), sp.p1(i),
+                                                       sp.p2(i), c[i], tab);
         const int k = a.step_snap[step];
         if (k >= 0) {
 #pragma unroll
@@ -298,13 +341,14 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
     }
     const int tile = blockIdx.x;
     const DslScen &o = a.scen[a.tile_scen[tile]];
+    const StepParams<MAXA> sp(o);   // read once, before any snapshot store
     double *snap = a.snaps + (int64_t)tile * a.n_snaps * o.n_assets * DSL_BLOCK + threadIdx.x;
     const int64_t base = (int64_t)(tile - o.tile_first) * DSL_BLOCK * o.ppt;
     double sum = 0.0, sq = 0.0, err = -1.0;
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        simulate_path<RNG, MAXA>(a, o, p, snap, DSL_BLOCK, tab);
+        simulate_path<RNG, MAXA>(a, o, sp, p, snap, DSL_BLOCK, tab);
         const double x = evaluate_path(a, o, snap, DSL_BLOCK, regs + threadIdx.x, err);
         sum += x;
         sq += x * x;
@@ -342,11 +386,12 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
     }
     const int tile = blockIdx.x;
     const DslScen &o = a.scen[a.tile_scen[tile]];
+    const StepParams<MAXA> sp(o);   // read once, before any snapshot store
     const int64_t base = (int64_t)(tile - o.tile_first) * DSL_BLOCK * o.ppt;
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        simulate_path<RNG, MAXA>(a, o, p, a.snaps + o.snap_off + p, o.paths, tab);
+        simulate_path<RNG, MAXA>(a, o, sp, p, a.snaps + o.snap_off + p, o.paths, tab);
     }
 }
 
