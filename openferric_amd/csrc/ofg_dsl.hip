@@ -409,7 +409,15 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
         __syncthreads();
     }
     const DslScen &o = a.scen[0];
+    const StepParams<MAXA> sp0(o);   // market 0's, read once, before any snapshot store
     const int n = o.n_assets, ns = a.n_scen;
+    uint32_t share[DSL_FUSE_MAXS];     // every market's share mask and own-factor flag, likewise
+    bool own[DSL_FUSE_MAXS];
+#pragma unroll
+    for (int m = 0; m < DSL_FUSE_MAXS; ++m) {
+        share[m] = m < ns ? a.scen[m].share_mask : 0u;
+        own[m] = m < ns && a.scen[m].own_corr;
+    }
     const int64_t base = (int64_t)blockIdx.x * DSL_BLOCK * o.ppt;
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
@@ -478,11 +486,11 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
             for (int j2 = 0; j2 < MAXA; ++j2)
 #pragma unroll
                 for (int i = j2; i < MAXA; ++i)
-                    if (i < n && j2 < n) c0[i] += o.chol[i * (i + 1) / 2 + j2] * z[j2];
+                    if (i < n && j2 < n) c0[i] += sp0.chol(i * (i + 1) / 2 + j2) * z[j2];
 #pragma unroll
             for (int i = 0; i < MAXA; ++i)
-                g0[i] = (i < n && o.kind[i] == DSL_GBM)
-                            ? gbm_factor<RNG == OFG_RNG_FAST>(o.p0[i], o.p1[i], c0[i], tab)
+                g0[i] = (i < n && sp0.kind(i) == DSL_GBM)
+                            ? gbm_factor<RNG == OFG_RNG_FAST>(sp0.p0(i), sp0.p1(i), c0[i], tab)
                             : 1.0;
             const int ks = a.step_snap[step];
 #pragma unroll
@@ -490,7 +498,7 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
                 if (m >= ns) continue;
                 const DslScen &q = a.scen[m];
                 double c[MAXA];
-                if (q.own_corr) {
+                if (own[m]) {
 #pragma unroll
                     for (int i = 0; i < MAXA; ++i) c[i] = 0.0;
 #pragma unroll
@@ -505,7 +513,7 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
 #pragma unroll
                 for (int i = 0; i < MAXA; ++i) {
                     if (i < n) {
-                        if ((q.share_mask >> i) & 1u) s[m][i] = s[m][i] * g0[i];
+                        if ((share[m] >> i) & 1u) s[m][i] = s[m][i] * g0[i];
                         else
                             s[m][i] = step_asset<RNG == OFG_RNG_FAST>(q.kind[i], s[m][i], q.p0[i],
                                                                       q.p1[i], q.p2[i], c[i], tab);
