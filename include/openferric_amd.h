@@ -463,6 +463,32 @@ int32_t ofg_basket_price(ofg_ctx *ctx, const ofg_basket *baskets, size_t n,
 int32_t ofg_repair_correlation(const double *corr, int32_t n, double *out,
                                int32_t *was_repaired);
 
+/* CorrelationStressScenario (src/math/correlation.rs:49-60). value: the
+ * factor, shift, floor, cap or pair value; i, j: OverridePair only. */
+typedef enum ofg_corr_stress_kind {
+    OFG_STRESS_SCALE_OFF_DIAGONAL = 0,
+    OFG_STRESS_ADDITIVE_SHIFT = 1,
+    OFG_STRESS_FLOOR_OFF_DIAGONAL = 2,
+    OFG_STRESS_CAP_OFF_DIAGONAL = 3,
+    OFG_STRESS_OVERRIDE_PAIR = 4
+} ofg_corr_stress_kind;
+
+typedef struct ofg_corr_stress {
+    int32_t kind;             /* ofg_corr_stress_kind */
+    int32_t i, j;
+    int32_t pad;
+    double value;
+} ofg_corr_stress;
+
+/* stressed_correlation_matrix (src/pricing/basket.rs:447-454, i.e.
+ * apply_correlation_stress with repair, math/correlation.rs:442-543): the
+ * scenarios in order, off-diagonal entries clamped to +-0.999999 and the
+ * diagonal reset after each, then the Higham repair when the minimum
+ * eigenvalue is below -1e-10, then validation. Writes out (n x n). Host
+ * only; errors carry the reference's messages. */
+int32_t ofg_stress_correlation(const double *corr, int32_t n, const ofg_corr_stress *scenarios,
+                               int32_t n_scenarios, double *out);
+
 /* Structured-product language: parse_and_compile (src/dsl/mod.rs:24-28,
  * lexer.rs, parser.rs, compiler.rs). On failure *out is NULL, the status is
  * OFG_INVALID_INPUT and ofg_last_error holds the DslError text. */

@@ -949,6 +949,14 @@ inline BasketSensitivities basket_sensitivities(const BasketOption &b,
                                sens.vega, sens.cega};
 }
 
+// stressed_correlation_matrix (pricing/basket.rs:447-454); corr row-major.
+inline std::vector<double> stressed_correlation_matrix(const std::vector<double> &corr, int32_t n,
+                                                       const std::vector<ofg_corr_stress> &sc) {
+    std::vector<double> out((size_t)n * n);
+    check(ofg_stress_correlation(corr.data(), n, sc.data(), (int32_t)sc.size(), out.data()));
+    return out;
+}
+
 // -------------------------------------------------- product language --
 namespace dsl {
 

@@ -20,7 +20,8 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       option_struct, price_raw, sample_payoffs)
 from .multi import (Autocallable, AutocallableSensitivities, BasketCopula, BasketOption,
                     FactorCorrelationModel, price_basket_mc_with_factor_model,
-                    price_basket_mc_with_copula,
+                    price_basket_mc_with_copula, CorrelationStressScenario,
+                    stressed_correlation_matrix,
                     BasketSensitivities, BasketType, OutperformanceBasketOption,
                     PhoenixAutocallable, QuantoBasketOption, autocall_price_raw,
                     autocall_struct, autocallable_sensitivities, basket_price_raw,
@@ -50,7 +51,7 @@ __all__ = [
     "basket_sensitivities", "basket_struct", "price_basket_mc", "OutperformanceBasketOption",
     "QuantoBasketOption", "price_outperformance_basket_mc", "price_quanto_basket_mc",
     "BasketCopula", "FactorCorrelationModel", "price_basket_mc_with_factor_model",
-    "price_basket_mc_with_copula",
+    "price_basket_mc_with_copula", "CorrelationStressScenario", "stressed_correlation_matrix",
     "validate_or_repair_correlation_matrix",
     "AssetMarketData", "CompiledProduct", "CrossGreeks", "DslMonteCarloEngine", "DslProduct",
     "ExtendedGreeks", "MultiAssetMarket", "parse_and_compile",

@@ -38,7 +38,7 @@ EXPORTED_SYMBOLS = [
     "ofg_sample_payoffs", "ofg_price_greeks", "ofg_batch_create", "ofg_batch_run",
     "ofg_batch_results", "ofg_batch_pricing_kernel_ms", "ofg_batch_destroy", "ofg_autotune",
     "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump", "ofg_lsm_last_timing",
-    "ofg_qmc_price", "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation",
+    "ofg_qmc_price", "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation", "ofg_stress_correlation",
     "ofg_dsl_compile", "ofg_dsl_free", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
     "ofg_dsl_price",
 ]
@@ -153,6 +153,15 @@ class OfgBasket(C.Structure):
 COPULA_GAUSSIAN, COPULA_STUDENT_T = 0, 1
 
 
+class OfgCorrStress(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("i", C.c_int32), ("j", C.c_int32), ("pad", C.c_int32),
+                ("value", C.c_double)]
+
+
+(STRESS_SCALE_OFF_DIAGONAL, STRESS_ADDITIVE_SHIFT, STRESS_FLOOR_OFF_DIAGONAL,
+ STRESS_CAP_OFF_DIAGONAL, STRESS_OVERRIDE_PAIR) = range(5)
+
+
 class OfgBasketSens(C.Structure):
     _fields_ = [
         ("delta", C.c_double * 16), ("vega", C.c_double), ("cega", C.c_double),
@@ -261,6 +270,10 @@ def load() -> C.CDLL:
         L.ofg_repair_correlation.argtypes = [C.POINTER(C.c_double), C.c_int32,
                                              C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         L.ofg_repair_correlation.restype = C.c_int32
+        L.ofg_stress_correlation.argtypes = [C.POINTER(C.c_double), C.c_int32,
+                                             C.POINTER(OfgCorrStress), C.c_int32,
+                                             C.POINTER(C.c_double)]
+        L.ofg_stress_correlation.restype = C.c_int32
         L.ofg_dsl_compile.argtypes = [C.c_char_p, C.POINTER(pv)]
         L.ofg_dsl_compile.restype = C.c_int32
         L.ofg_dsl_free.argtypes = [pv]

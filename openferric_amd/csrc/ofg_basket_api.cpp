@@ -726,3 +726,64 @@ extern "C" int32_t ofg_repair_correlation(const double *corr, int32_t n, double 
         return OFG_OK;
     });
 }
+
+// apply_correlation_stress with repair_to_psd (math/correlation.rs:442-543),
+// as stressed_correlation_matrix calls it (pricing/basket.rs:447-454).
+extern "C" int32_t ofg_stress_correlation(const double *corr, int32_t n,
+                                          const ofg_corr_stress *scenarios, int32_t n_scenarios,
+                                          double *out) {
+    return guarded([&]() {
+        if (!corr || !out || n <= 0 || n_scenarios < 0 || (n_scenarios > 0 && !scenarios))
+            fail(OFG_INVALID_INPUT, "null argument");
+        Mat c(corr, corr + (size_t)n * n);
+        validate_corr(c, n);
+        const double lim = 0.999999;
+        auto clamp = [lim](double x) { return std::min(lim, std::max(-lim, x)); };
+        auto each_pair = [&](auto &&f) {
+            for (int i = 0; i < n; ++i)
+                for (int j = i + 1; j < n; ++j) {
+                    const double v = clamp(f(c[(size_t)i * n + j]));
+                    c[(size_t)i * n + j] = c[(size_t)j * n + i] = v;
+                }
+        };
+        for (int32_t k = 0; k < n_scenarios; ++k) {
+            const ofg_corr_stress &sc = scenarios[k];
+            const double x = sc.value;
+            switch (sc.kind) {
+            case OFG_STRESS_SCALE_OFF_DIAGONAL:
+                if (!std::isfinite(x)) fail(OFG_INVALID_INPUT, "scale factor must be finite");
+                each_pair([x](double r) { return r * x; });
+                break;
+            case OFG_STRESS_ADDITIVE_SHIFT:
+                if (!std::isfinite(x)) fail(OFG_INVALID_INPUT, "additive shift must be finite");
+                each_pair([x](double r) { return r + x; });
+                break;
+            case OFG_STRESS_FLOOR_OFF_DIAGONAL:
+                if (!std::isfinite(x) || x < -1.0 || x > 1.0)
+                    fail(OFG_INVALID_INPUT, "floor must be finite and in [-1, 1]");
+                each_pair([x](double r) { return std::fmax(r, x); });
+                break;
+            case OFG_STRESS_CAP_OFF_DIAGONAL:
+                if (!std::isfinite(x) || x < -1.0 || x > 1.0)
+                    fail(OFG_INVALID_INPUT, "cap must be finite and in [-1, 1]");
+                each_pair([x](double r) { return std::fmin(r, x); });
+                break;
+            case OFG_STRESS_OVERRIDE_PAIR: {
+                if (sc.i < 0 || sc.j < 0 || sc.i >= n || sc.j >= n || sc.i == sc.j)
+                    fail(OFG_INVALID_INPUT, "override pair indices must be distinct and in-range");
+                if (!std::isfinite(x) || !(x >= -1.0 && x <= 1.0))
+                    fail(OFG_INVALID_INPUT, "override pair value must be finite and in [-1, 1]");
+                const double v = clamp(x);
+                c[(size_t)sc.i * n + sc.j] = c[(size_t)sc.j * n + sc.i] = v;
+                break;
+            }
+            default: fail(OFG_INVALID_INPUT, "unknown correlation stress scenario");
+            }
+            for (int i = 0; i < n; ++i) c[(size_t)i * n + i] = 1.0;
+        }
+        if (!(min_eigenvalue(c, n) >= -PSD_TOL)) c = nearest_correlation(c, n);
+        validate_corr(c, n);
+        std::copy(c.begin(), c.end(), out);
+        return OFG_OK;
+    });
+}

@@ -498,3 +498,50 @@ def validate_or_repair_correlation_matrix(corr_matrix):
     _raise(L.ofg_repair_correlation(c.ctypes.data_as(C.POINTER(C.c_double)), n,
                                     outm.ctypes.data_as(C.POINTER(C.c_double)), C.byref(rep)))
     return outm, bool(rep.value)
+
+
+@dataclass(frozen=True)
+class CorrelationStressScenario:
+    """CorrelationStressScenario (math/correlation.rs:49-60); build with the
+    constructors below."""
+    kind: int
+    value: float
+    i: int = 0
+    j: int = 0
+
+    @staticmethod
+    def ScaleOffDiagonal(factor: float) -> "CorrelationStressScenario":
+        return CorrelationStressScenario(_lib.STRESS_SCALE_OFF_DIAGONAL, float(factor))
+
+    @staticmethod
+    def AdditiveShift(shift: float) -> "CorrelationStressScenario":
+        return CorrelationStressScenario(_lib.STRESS_ADDITIVE_SHIFT, float(shift))
+
+    @staticmethod
+    def FloorOffDiagonal(floor: float) -> "CorrelationStressScenario":
+        return CorrelationStressScenario(_lib.STRESS_FLOOR_OFF_DIAGONAL, float(floor))
+
+    @staticmethod
+    def CapOffDiagonal(cap: float) -> "CorrelationStressScenario":
+        return CorrelationStressScenario(_lib.STRESS_CAP_OFF_DIAGONAL, float(cap))
+
+    @staticmethod
+    def OverridePair(i: int, j: int, value: float) -> "CorrelationStressScenario":
+        return CorrelationStressScenario(_lib.STRESS_OVERRIDE_PAIR, float(value), int(i), int(j))
+
+
+def stressed_correlation_matrix(corr_matrix, scenarios):
+    """stressed_correlation_matrix (pricing/basket.rs:447-454): the scenarios
+    applied in order, then the Higham repair when the result is not PSD."""
+    L = _lib.load()
+    c = np.ascontiguousarray(np.asarray(corr_matrix, dtype=np.float64))
+    n = c.shape[0] if c.ndim == 2 else 0
+    if c.shape != (n, n) or n == 0:
+        raise InvalidInput("correlation matrix dimensions must match asset count")
+    arr = (_lib.OfgCorrStress * max(len(scenarios), 1))()
+    for k, sc in enumerate(scenarios):
+        arr[k].kind, arr[k].i, arr[k].j, arr[k].value = sc.kind, sc.i, sc.j, sc.value
+    out = np.zeros_like(c)
+    _raise(L.ofg_stress_correlation(c.ctypes.data_as(C.POINTER(C.c_double)), n, arr,
+                                    len(scenarios), out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out

@@ -264,6 +264,13 @@ int main() {
             BasketOption{{1.0}, 100.0, 1.0, true, BasketType::Average}, {100.0}, {0.2}, {1.0},
             0.03, {0.0}, 100, BasketCopula::StudentT(1));
         expect(std::isnan(tbad.price), "student_t_copula_requires_two_degrees_of_freedom");
+        ofg_corr_stress shift{};
+        shift.kind = OFG_STRESS_ADDITIVE_SHIFT;
+        shift.value = 0.35;
+        const auto stressed = stressed_correlation_matrix(
+            {1.0, 0.7, 0.6, 0.7, 1.0, 0.65, 0.6, 0.65, 1.0}, 3, {shift});
+        expect(stressed[0] == 1.0 && stressed[1] > 0.9 && stressed[1] == stressed[3],
+               "stressed_correlation_is_repaired_to_psd");
         const auto sens = basket_sensitivities(
             BasketOption{{0.5, 0.5}, 100.0, 1.0, true, BasketType::Average}, {100.0, 90.0},
             {0.2, 0.25}, {1, 0.3, 0.3, 1}, 0.03, {0.0, 0.0}, 50000);

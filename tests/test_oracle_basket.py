@@ -375,3 +375,68 @@ def test_t_copula_stream_order_and_tail_dependence(oracle):
         pt, st = oracle.basket_price(oracle.make_basket(*args, t_dof=3))
         pg, sg = oracle.basket_price(oracle.make_basket(*args))
         assert sign * (pt - pg) > 6 * math.hypot(st, sg), (kind, pt, pg)
+
+
+# ---- correlation stress (basket.rs:447-454; correlation.rs:442-543) --------
+
+def _numpy_stress(c, scen):
+    """The scenario arithmetic restated in numpy: off-diagonal entries
+    transformed, clamped to +-0.999999, the diagonal reset after each."""
+    c = np.array(c, dtype=np.float64)
+    n = c.shape[0]
+    iu = np.triu_indices(n, 1)
+    for kind, value, i, j in scen:
+        if kind == "override":
+            v = np.clip(value, -0.999999, 0.999999)
+            c[i, j] = c[j, i] = v
+        else:
+            x = c[iu]
+            x = {"scale": x * value, "shift": x + value, "floor": np.maximum(x, value),
+                 "cap": np.minimum(x, value)}[kind]
+            x = np.clip(x, -0.999999, 0.999999)
+            c[iu] = x
+            c[(iu[1], iu[0])] = x
+        np.fill_diagonal(c, 1.0)
+    return c
+
+
+def test_stressed_correlation_matches_numpy_and_reference_test():
+    from openferric_amd import CorrelationStressScenario as S, stressed_correlation_matrix
+    from openferric_amd.core import InvalidInput
+    base = [[1.0, 0.7, 0.6], [0.7, 1.0, 0.65], [0.6, 0.65, 1.0]]
+    # stressed_correlation_is_repaired_to_psd (basket.rs:1212-1226)
+    m = stressed_correlation_matrix(base, [S.AdditiveShift(0.35)])
+    assert np.linalg.eigvalsh(m).min() >= -1e-8
+    assert np.abs(m - higham_nearest(_numpy_stress(base, [("shift", 0.35, 0, 0)]))).max() < 1e-8
+    rng = np.random.default_rng(4)
+    a = rng.normal(size=(5, 8))
+    c = a @ a.T
+    d = np.sqrt(np.diag(c))
+    c = c / np.outer(d, d)
+    np.fill_diagonal(c, 1.0)
+    c = (c + c.T) / 2
+    cases = [[("scale", 0.5, 0, 0)], [("shift", -0.1, 0, 0)], [("floor", 0.2, 0, 0)],
+             [("cap", 0.1, 0, 0)], [("override", 0.3, 1, 3)],
+             [("scale", 1.3, 0, 0), ("cap", 0.6, 0, 0), ("override", -0.05, 4, 0)]]
+    ctor = {"scale": S.ScaleOffDiagonal, "shift": S.AdditiveShift, "floor": S.FloorOffDiagonal,
+            "cap": S.CapOffDiagonal}
+    for sc in cases:
+        want = _numpy_stress(c, sc)
+        if np.linalg.eigvalsh(want).min() < -1e-10:
+            want = higham_nearest(want)
+            tol = 1e-8
+        else:
+            tol = 0.0
+        objs = [S.OverridePair(i, j, v) if k == "override" else ctor[k](v) for k, v, i, j in sc]
+        got = stressed_correlation_matrix(c, objs)
+        assert np.abs(got - want).max() <= tol, (sc, np.abs(got - want).max())
+    for bad, msg in (([S.ScaleOffDiagonal(float("nan"))], "scale factor must be finite"),
+                     ([S.FloorOffDiagonal(1.5)], "floor must be finite"),
+                     ([S.CapOffDiagonal(-2.0)], "cap must be finite"),
+                     ([S.OverridePair(0, 0, 0.1)], "distinct and in-range"),
+                     ([S.OverridePair(0, 3, 0.1)], "distinct and in-range"),
+                     ([S.OverridePair(0, 1, 1.2)], "override pair value")):
+        with pytest.raises(InvalidInput, match=msg):
+            stressed_correlation_matrix(base, bad)
+    with pytest.raises(InvalidInput, match="symmetric"):
+        stressed_correlation_matrix([[1.0, 0.2], [0.3, 1.0]], [S.AdditiveShift(0.1)])
