@@ -157,11 +157,14 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
     }
     if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
     const double *zp = a.normals + o.normals_offset + p;
+    // fast draws: Box-Muller pairs run across steps, so an odd asset count
+    // wastes no normal
+    double zpair = 0.0;
+    bool have_pair = false;
     for (int step = 1; step <= a.steps; ++step) {
         double c[MAXA];
 #pragma unroll
         for (int i = 0; i < MAXA; ++i) c[i] = 0.0;
-        double zpair = 0.0;
         double zr[MAXA];   // reference draws of the step, tail region deferred
         if constexpr (RNG == OFG_RNG_REFERENCE) {
             double pr[MAXA];
@@ -178,13 +181,14 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
                 } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
                     z = zp[((int64_t)(step - 1) * n + j) * o.paths];
                 } else {
-                    if ((j & 1) == 0) {
+                    if (!have_pair) {
                         const uint64_t wu = fast.next_u64();
                         const uint64_t wt = fast.next_u64();
                         box_muller_pair(wu, (uint32_t)(wt >> 32), z, zpair, tab);
                     } else {
                         z = zpair;
                     }
+                    have_pair = !have_pair;
                 }
 #pragma unroll
                 for (int i = j; i < MAXA; ++i)
@@ -448,9 +452,10 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
         }
         if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
         const double *zp = a.normals + o.normals_offset + p;
+        double zpair = 0.0;   // fast draws: pairs run across steps, as simulate_path
+        bool have_pair = false;
         for (int step = 1; step <= a.steps; ++step) {
             double z[MAXA], c0[MAXA], g0[MAXA];
-            double zpair = 0.0;
             double zr[MAXA];   // reference draws of the step, tail region deferred
             if constexpr (RNG == OFG_RNG_REFERENCE) {
                 double pr[MAXA];
@@ -467,13 +472,14 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
                     } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
                         z[j2] = zp[((int64_t)(step - 1) * n + j2) * o.paths];
                     } else {
-                        if ((j2 & 1) == 0) {
+                        if (!have_pair) {
                             const uint64_t wu = fast.next_u64();
                             const uint64_t wt = fast.next_u64();
                             box_muller_pair(wu, (uint32_t)(wt >> 32), z[j2], zpair, tab);
                         } else {
                             z[j2] = zpair;
                         }
+                        have_pair = !have_pair;
                     }
                 } else {
                     z[j2] = 0.0;
