@@ -295,22 +295,33 @@ __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const d
         return z;
     };
     if (o.n_factors == 0) {
-        double zr[BK_MAX];   // reference draws of the assets, tail region deferred
-        if constexpr (RNG == OFG_RNG_REFERENCE) {
-            double pr[BK_MAX];
+        // the assets' draws in blocks of BK_AK (reference draws: the inverse
+        // CDF's tail region deferred over the block), each column added to
+        // the correlated normals as it is drawn
+        constexpr int BK_AK = 8;
 #pragma unroll
-            for (int j = 0; j < BK_MAX; ++j) pr[j] = j < n ? uniform_open01(ref.next_f64()) : 0.5;
-            acklam_block<BK_MAX>(pr, zr);
-        }
+        for (int h = 0; h < BK_MAX; h += BK_AK) {
+            if (h < n) {
+                double zr[BK_AK];
+                if constexpr (RNG == OFG_RNG_REFERENCE) {
+                    double pr[BK_AK];
 #pragma unroll
-        for (int j = 0; j < BK_MAX; ++j) {
-            if (j < n) {
-                double z;
-                if constexpr (RNG == OFG_RNG_REFERENCE) z = zr[j];
-                else z = draw(j);
+                    for (int k = 0; k < BK_AK; ++k)
+                        pr[k] = h + k < n ? uniform_open01(ref.next_f64()) : 0.5;
+                    acklam_block<BK_AK>(pr, zr);
+                }
 #pragma unroll
-                for (int i = j; i < BK_MAX; ++i)
-                    if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
+                for (int k = 0; k < BK_AK; ++k) {
+                    const int j = h + k;
+                    if (j < n) {
+                        double z;
+                        if constexpr (RNG == OFG_RNG_REFERENCE) z = zr[k];
+                        else z = draw(j);
+#pragma unroll
+                        for (int i = j; i < BK_MAX; ++i)
+                            if (i < n) c[i] += o.chol[i * (i + 1) / 2 + j] * z;
+                    }
+                }
             }
         }
     } else {
