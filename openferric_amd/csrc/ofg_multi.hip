@@ -511,8 +511,18 @@ __device__ __forceinline__ double acg_worst(const AcGreeksDev &o, const double *
     return w;
 }
 
+// Waves per SIMD the register allocator is held to: up to four underlyings
+// the kernel fits 3 (168 VGPRs); left free it takes a few more and drops to
+// 2, where the per-step scalar reloads of the scenario parameters are no
+// longer hidden.
+template <int MAXN>
+constexpr int acg_waves_target() {
+    return MAXN <= 4 ? 3 : 1;
+}
+
 template <int RNG, int MAXN>
-__global__ __launch_bounds__(MA_BLOCK) void autocall_greeks_kernel(AcgArgs a) {
+__global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
+    acg_waves_target<MAXN>()))) void autocall_greeks_kernel(AcgArgs a) {
     constexpr int MAXS = 5 + 2 * MAXN;
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     __shared__ double red[NWAVES][ACG_MAXS][2];
