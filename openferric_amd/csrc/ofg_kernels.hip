@@ -195,10 +195,11 @@ __device__ __forceinline__ void dispatch_kind(const OptDev &o, const WorkItem &w
 }
 
 // Minimum waves per SIMD asked of the register allocator: the plain fast GBM
-// loop fits 128 VGPRs (4 waves) without spilling.
+// and Heston loops fit 128 VGPRs (4 waves) without spilling (Heston takes 129
+// left free).
 template <int RNG, int MODEL, bool ANTI>
 constexpr int price_waves_per_eu() {
-    return (RNG == OFG_RNG_FAST && MODEL == OFG_MODEL_GBM && !ANTI) ? 4 : 1;
+    return (RNG == OFG_RNG_FAST && !ANTI) ? 4 : 1;
 }
 template <int RNG, int MODEL, bool ANTI, int BLOCK>
 constexpr int price_waves_target() {

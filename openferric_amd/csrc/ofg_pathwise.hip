@@ -351,12 +351,20 @@ __device__ void pathwise_item(const OptDev &o, const WorkItem &wi, int lane, con
     }
 }
 
-// The Heston tangent holds 16 (Asian: 32) extra doubles per leg: with one
-// leg, ask the register allocator for two waves per SIMD (it fits 256 VGPRs
-// without spilling; two antithetic legs do not).
+// Waves per SIMD asked of the register allocator. The Heston tangent holds
+// 16 (Asian: 32) extra doubles per leg: with one leg it fits two waves (256
+// VGPRs) without spilling; two antithetic legs do not. The fast-draw GBM
+// tangent takes 169-171 VGPRs left free, just over the 168 of three waves,
+// and fits them.
+template <int RNG, int MODEL, bool ANTI>
+constexpr int pathwise_waves_target() {
+    if (MODEL == OFG_MODEL_HESTON) return ANTI ? 1 : 2;
+    return RNG == OFG_RNG_FAST ? 3 : 1;
+}
+
 template <int RNG, int MODEL, bool ANTI, int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
-    (MODEL == OFG_MODEL_HESTON && !ANTI) ? 2 : 1))) void pathwise_kernel(AdjArgs a) {
+    pathwise_waves_target<RNG, MODEL, ANTI>()))) void pathwise_kernel(AdjArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
