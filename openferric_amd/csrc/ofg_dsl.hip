@@ -405,8 +405,12 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
 // multiplies by market 0's factor -- the reference forms the same product --
 // and a market with its own Cholesky factor correlates the same draws.
 // Launched over market 0's tiles; writes every market's snapshots.
+// Held to 3 waves per SIMD (168 VGPRs): left free, four-asset markets take
+// about 195 and run at 2; the few registers spilled cost less than the lost
+// wave (five-market Greeks 16.5 -> 15.7 ms on reference draws).
 template <int RNG, int MAXA>
-__global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_markets_kernel(DslArgs a) {
+__global__ __launch_bounds__(DSL_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
+dsl_sim_markets_kernel(DslArgs a) {
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
         for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += DSL_BLOCK) tab[i] = OFG_FAST_TABLE[i];
