@@ -337,8 +337,16 @@ __device__ void adjoint_item(const OptDev &o, const WorkItem &wi, int lane, cons
     }
 }
 
+// Waves per SIMD asked of the register allocator: the fast-draw GBM sweep
+// takes about 191 VGPRs left free (2 waves) and is held to 3.
+template <int RNG, int MODEL, bool ANTI>
+constexpr int adjoint_waves_target() {
+    return (RNG == OFG_RNG_FAST && MODEL == OFG_MODEL_GBM && !ANTI) ? 3 : 1;
+}
+
 template <int RNG, int MODEL, bool ANTI, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
+    adjoint_waves_target<RNG, MODEL, ANTI>()))) void adjoint_kernel(AdjArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
