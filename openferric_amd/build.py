@@ -8,6 +8,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -56,15 +57,18 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
     (incremental: only sources older than their object or any header)."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
-        if incremental and not _stale(src, obj):
-            objs.append(obj)
-            continue
+    objs = [os.path.join(objdir, src.rsplit(".", 1)[0] + ".o") for src in SOURCES]
+    todo = [(s, o) for s, o in zip(SOURCES, objs) if not incremental or _stale(s, o)]
+
+    def one(job):
         if verbose:
-            print("hipcc", src, flush=True)
-        objs.append(_compile(src, obj))
+            print("hipcc", job[0], flush=True)
+        return _compile(*job)
+
+    # one hipcc per source, as many at once as the host has cores (at most 16)
+    jobs = max(1, min(16, os.cpu_count() or 1, len(todo) or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        list(pool.map(one, todo))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

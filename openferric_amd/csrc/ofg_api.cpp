@@ -568,6 +568,7 @@ int32_t mc_greeks_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg
                        double spot_bump_rel, ofg_mc_greeks *out) {
     if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
     if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+    check_tuning(*cfg);
     std::vector<ofg_option> sim;
     std::vector<size_t> sim_of;
     std::vector<double> ups, dns;
@@ -914,6 +915,7 @@ int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_eng
                    ofg_result *out) {
     if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
     if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+    check_tuning(*cfg);
     if (cfg->greek_mode == OFG_GREEKS_FD_CRN) return price_fd(ctx, opts, n, cfg, out);
     if (cfg->greek_mode == OFG_GREEKS_ADJOINT || cfg->greek_mode == OFG_GREEKS_FORWARD)
         return adjoint_impl(ctx, opts, n, cfg, out);
@@ -1013,6 +1015,8 @@ int32_t ofg_sample_payoffs(ofg_ctx *ctx, const ofg_option *opt, const ofg_engine
                            double *xs, double *ys) {
     return guarded([&]() -> int32_t {
         if (!ctx || !opt || !cfg || !xs) fail(OFG_INVALID_INPUT, "null argument");
+        if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+        check_tuning(*cfg);
         std::lock_guard<std::mutex> lock(ctx->mu);
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         if (opt->expiry == 0.0) fail(OFG_INVALID_INPUT, "maturity-zero option has no paths");
@@ -1029,6 +1033,8 @@ ofg_batch *ofg_batch_create(ofg_ctx *ctx, const ofg_option *opts, size_t n,
     ofg_batch *out = nullptr;
     guarded([&]() -> int32_t {
         if (!ctx || !opts || !cfg || n == 0) fail(OFG_INVALID_INPUT, "null argument");
+        if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+        check_tuning(*cfg);
         if (cfg->rng_mode == OFG_RNG_EXTERNAL)
             fail(OFG_INVALID_INPUT, "batches use generated normals");
         if (cfg->greek_mode != OFG_GREEKS_NONE)
@@ -1181,6 +1187,8 @@ int32_t ofg_autotune(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
                      const char *cache_path, int32_t *best_block, int32_t *best_item_steps) {
     return guarded([&]() -> int32_t {
         if (!ctx || !opts || !cfg || n == 0) fail(OFG_INVALID_INPUT, "null argument");
+        if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+        check_tuning(*cfg);
         if (cfg->rng_mode == OFG_RNG_EXTERNAL) fail(OFG_INVALID_INPUT, "autotune needs generated normals");
         // Scaled-down copy: same options, path counts scaled so the sample
         // workload is ~2e10 path-steps (or the full workload if smaller):

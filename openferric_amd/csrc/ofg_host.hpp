@@ -102,6 +102,17 @@ struct DslBuffers;    // ofg_dsl_api.cpp
 // validate()); throws OfgError.
 void validate_instrument(const ofg_option &o);
 
+// Launch-shape overrides of an engine configuration. The kernels exist for
+// 64, 128, 256 and 512 threads per block (the Greeks kernels for 64, 128 and
+// 256: a 512 request runs them at 256), so any other block size is refused
+// before a grid or scratch size is derived from it.
+inline void check_tuning(const ofg_engine_cfg &cfg) {
+    const int32_t b = cfg.tune_block;
+    if (!(b == 0 || b == 64 || b == 128 || b == 256 || b == 512))
+        fail(OFG_INVALID_INPUT, "tune_block must be 0, 64, 128, 256 or 512");
+    if (cfg.tune_item_steps < 0) fail(OFG_INVALID_INPUT, "tune_item_steps must be >= 0");
+}
+
 }  // namespace ofg_host
 
 struct ofg_ctx {

@@ -156,6 +156,17 @@ int or_mc_european_pathwise_aad(const or_option *o, int64_t num_paths,
 int or_heston_price_delta_aad(const or_option *o, int64_t num_paths,
                               int64_t num_steps, uint64_t seed, double out[2]);
 
+/* Pathwise sensitivities of MonteCarloEngine::run's estimator for every
+ * payoff and both models, by forward tangents (oracle_pathwise.c). out:
+ * price, stderr, then d price / d {S0, sigma | v0, r, T, kappa, theta, xi,
+ * rho}. normals1/2 (may be NULL: the reference draws) are [sample][step].
+ * per_sample (may be NULL): samples x 9 (pv, then the 8 derivatives).
+ * mutate != 0 drops one term of the tangent recursion (tests only).
+ * Returns 0 on success. */
+int or_pathwise_greeks(const or_option *o, int64_t num_paths, int64_t num_steps, uint64_t seed,
+                       int32_t antithetic, const double *normals1, const double *normals2,
+                       int32_t nthreads, int32_t mutate, double out[10], double *per_sample);
+
 /* MonteCarloGreeksEngine::run_estimators (mc_greeks.rs:131-255), the
  * sequential fold. out: pathwise delta, pathwise gamma, LR delta, LR gamma,
  * LR vega (discounted means). o->div_yield is the market's effective yield.

@@ -101,8 +101,10 @@ BASKET_AVERAGE, BASKET_BEST_OF, BASKET_WORST_OF = 0, 1, 2
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with make (gcc); returns the library path."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile liboracle.so with make (gcc); returns the library path. Without
+    `force`, make rebuilds only when a source is newer than the library (the
+    sources are absent where only the built library was shipped)."""
+    if force or not os.path.exists(_LIB_PATH) or os.path.exists(os.path.join(_HERE, "oracle.c")):
         subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"],
                        check=True, capture_output=True)
     return _LIB_PATH
@@ -152,6 +154,9 @@ def lib():
         L.or_mc_european_pathwise_aad.restype = C.c_int
         L.or_heston_price_delta_aad.argtypes = [C.POINTER(OrOption), i64, i64, u64, pd]
         L.or_heston_price_delta_aad.restype = C.c_int
+        L.or_pathwise_greeks.argtypes = [C.POINTER(OrOption), i64, i64, u64, i32, pd, pd, i32,
+                                         i32, pd, pd]
+        L.or_pathwise_greeks.restype = C.c_int
         L.or_mc_greeks_estimate.argtypes = [C.POINTER(OrOption), i64, u64, i32, d, pd]
         L.or_mc_greeks_estimate.restype = C.c_int
         L.or_lu3_solve.argtypes = [pd, pd, pd]; L.or_lu3_solve.restype = C.c_int
@@ -297,6 +302,30 @@ def heston_price_delta_aad(o, num_paths, num_steps, seed):
     if rc:
         raise ValueError("heston aad failed")
     return float(out[0]), float(out[1])
+
+
+PATHWISE_SLOTS = ["s0", "sigma_or_v0", "r", "T", "kappa", "theta", "xi", "rho"]
+
+
+def pathwise_greeks(o, num_paths, num_steps, seed, antithetic=False, normals1=None,
+                    normals2=None, nthreads=1, mutate=0, want_samples=False):
+    """Pathwise estimator of MonteCarloEngine::run's price and its derivatives
+    (forward tangents, oracle_pathwise.c). Returns dict(price, stderr,
+    g=array of 8: d/d{S0, sigma|v0, r, T, kappa, theta, xi, rho}) and, with
+    want_samples, `samples`: [sample][pv, g0..g7]."""
+    out = np.zeros(10)
+    samples = (num_paths + 1) // 2 if antithetic else num_paths
+    per = np.zeros((samples, 9)) if want_samples else None
+    n1 = np.ascontiguousarray(normals1, dtype=np.float64) if normals1 is not None else None
+    n2 = np.ascontiguousarray(normals2, dtype=np.float64) if normals2 is not None else None
+    rc = lib().or_pathwise_greeks(C.byref(o), num_paths, num_steps, seed, 1 if antithetic else 0,
+                                  _ptr(n1), _ptr(n2), nthreads, mutate, _ptr(out), _ptr(per))
+    if rc:
+        raise ValueError(f"or_pathwise_greeks failed rc={rc}")
+    r = dict(price=float(out[0]), stderr=float(out[1]), g=out[2:].copy())
+    if want_samples:
+        r["samples"] = per
+    return r
 
 
 def mc_greeks_estimate(o, num_paths, seed, antithetic=True, spot_bump_rel=1e-2):

@@ -191,3 +191,35 @@ def test_torch_autograd_gbm_and_heston(gpu):
     ph.sum().backward()
     assert all(x.grad is not None and torch.isfinite(x.grad).all() for x in hs)
     assert hs[0].grad.item() > 0  # call delta
+
+
+@pytest.mark.parametrize("block", [32, 100, 192, 1024, -64])
+def test_unsupported_block_sizes_are_refused_before_any_launch(gpu, block):
+    """Every entry point taking an engine configuration refuses a block size
+    with no kernel instantiation (a grid or checkpoint scratch derived from it
+    would not fit the kernel launched) with InvalidInput."""
+    from openferric_amd import InvalidInput, RngMode, engine_cfg, price_raw, _lib
+    import ctypes as C
+    _, g, keep = make_pair(num_paths=4096, num_steps=32)
+    for greek in (_lib.GREEKS_NONE, _lib.GREEKS_ADJOINT, _lib.GREEKS_FORWARD,
+                  _lib.GREEKS_FD_CRN):
+        cfg = engine_cfg(rng_mode=RngMode.Fast, greek_mode=greek, tune_block=block)
+        with pytest.raises(InvalidInput, match="tune_block"):
+            price_raw([g], cfg)
+    cfg = engine_cfg(rng_mode=RngMode.Fast, tune_block=block)
+    with pytest.raises(InvalidInput, match="tune_block"):
+        price_raw([g], cfg, greeks=True)
+    L, ctx = _lib.load(), _lib.context(0)
+    out = (_lib.OfgMcGreeks * 1)()
+    rc = L.ofg_mc_greeks_estimate(ctx, C.byref(g), 1, C.byref(cfg), C.c_double(0.01), out)
+    assert rc == _lib.OFG_INVALID_INPUT and "tune_block" in _lib.last_error()
+    xs = np.zeros(4096)
+    rc = L.ofg_sample_payoffs(ctx, C.byref(g), C.byref(cfg),
+                              xs.ctypes.data_as(C.POINTER(C.c_double)), None)
+    assert rc == _lib.OFG_INVALID_INPUT
+    arr = (_lib.OfgOption * 1)(g)
+    assert not L.ofg_batch_create(ctx, arr, 1, C.byref(cfg), 1)
+    assert "tune_block" in _lib.last_error()
+    # the context still prices afterwards
+    r = price_raw([g], engine_cfg(rng_mode=RngMode.Fast))[0]
+    assert np.isfinite(r.price) and r.price > 0
