@@ -195,45 +195,82 @@ def greeks_leg(rng_mode):
             "forward_vs_adjoint_max_rel_diff": max(rel_fwd)}
 
 
-def heston_leg(rng_mode):
-    """Heston up-and-out pricing: 256 options x 2^20 paths x 365 steps."""
-    from openferric_amd import BatchPricer, engine_cfg
+def heston_leg(rng_mode, cpu=True):
+    """Heston up-and-out pricing: 256 options x 2^20 paths x 365 steps, one
+    persistent launch replayed from a hipGraph; fast draws, and the
+    reference's own draws."""
+    from openferric_amd import BatchPricer, RngMode, engine_cfg
     opts = heston_workload()
     structs, keeps = zip(*[to_ofg(d) for d in opts])
-    b = BatchPricer(list(structs), engine_cfg(rng_mode=rng_mode), use_graph=True)
-    b.run()
-    ms = [b.run() for _ in range(2)]
-    b.close()
     ps = sum(d["num_paths"] * d["num_steps"] for d in opts)
-    k_ms = float(np.mean(ms))
-    v = ps / (k_ms * 1e-3)
-    return {"workload": "256 Heston up-and-out calls, 2^20 paths x 365 steps", "path_steps": ps,
-            "kernel_ms": k_ms, "path_steps_per_s": v,
-            "frac_fp64_peak": v * FLOP_HESTON_STEP / 1e12 / FP64_PEAK_TFLOPS}
+    out = {"workload": "256 Heston up-and-out calls, 2^20 paths x 365 steps", "path_steps": ps}
+    for name, mode in (("fast", rng_mode), ("reference_draws", RngMode.Reference)):
+        b = BatchPricer(list(structs), engine_cfg(rng_mode=mode), use_graph=True)
+        b.run()
+        ms = [b.run() for _ in range(2)]
+        b.close()
+        k_ms = float(np.mean(ms))
+        v = ps / (k_ms * 1e-3)
+        leg = {"kernel_ms": k_ms, "path_steps_per_s": v,
+               "frac_fp64_peak": v * FLOP_HESTON_STEP / 1e12 / FP64_PEAK_TFLOPS}
+        if name == "fast":
+            out.update(leg)
+        else:
+            out[name] = leg
+    if cpu:
+        out["cpu_baseline"] = heston_cpu_baseline()
+        out["speedup_vs_cpu_reference_draws"] = (out["reference_draws"]["path_steps_per_s"]
+                                                 / out["cpu_baseline"]["value"])
+    return out
 
 
 def heston_greeks_leg(rng_mode):
-    """Batched Heston adjoint: 1024 options, 2^18 paths, steps ceil(252 T)."""
+    """Batched Heston sensitivities: 1024 options, 2^18 paths, steps
+    ceil(252 T); d price / d {S0, v0, kappa, theta, xi, rho} by the adjoint
+    kernel, by the forward tangent and by bump-and-revalue (central
+    differences with common random numbers, 2k+1 = 13 pricing passes of the
+    whole batch in one launch, relative bump 1e-4)."""
     from openferric_amd import _lib, engine_cfg, price_raw
     opts = heston_greeks_workload()
     structs, keeps = zip(*[to_ofg(d) for d in opts])
     structs = list(structs)
     cfg = engine_cfg(rng_mode=rng_mode)
+    price_raw(structs, cfg)  # warm-up
     p = price_raw(structs, cfg)
     a = price_raw(structs, cfg, greeks=True)
-    f = price_raw(structs, engine_cfg(rng_mode=rng_mode, greek_mode=_lib.GREEKS_FORWARD))
+    fwd_cfg = engine_cfg(rng_mode=rng_mode, greek_mode=_lib.GREEKS_FORWARD)
+    f = price_raw(structs, fwd_cfg)
+    fd = price_raw(structs, engine_cfg(rng_mode=rng_mode, greek_mode=_lib.GREEKS_FD_CRN,
+                                       bump_rel=1e-4))
     rel = [abs(y - x) / max(abs(x), 1e-12) for ra, rf in zip(a, f)
            for x, y in zip(ra.heston_grad, rf.heston_grad)]
+    # adjoint vs bump on the Europeans (a barrier's pathwise estimate omits
+    # the crossing term by construction), per sensitivity, relative to the
+    # option's price scale
+    rel_fd = {}
+    names = ["s0", "v0", "kappa", "theta", "xi", "rho"]
+    for k, name in enumerate(names):
+        rel_fd[name] = float(np.median([
+            abs(ra.heston_grad[k] - rb.heston_grad[k]) / max(abs(rb.heston_grad[k]), 1e-3)
+            for d, ra, rb in zip(opts, a, fd) if d["payoff"] == "vanilla"]))
+    t_p, t_a, t_f, t_fd = p[0].kernel_ms, a[0].kernel_ms, f[0].kernel_ms, fd[0].kernel_ms
     ps = sum(d["num_paths"] * d["num_steps"] for d in opts)
+    n_greeks = 6 * len(opts)
     return {"workload": "1024 Heston European/up-and-out calls, 2^18 paths, steps ceil(252T)",
-            "path_steps": ps, "pricing_ms": p[0].kernel_ms, "adjoint_ms": a[0].kernel_ms,
-            "adjoint_cost_in_pricing_passes": a[0].kernel_ms / p[0].kernel_ms,
-            "adjoint_greeks_per_s": 6 * len(opts) / (a[0].kernel_ms * 1e-3),
+            "path_steps": ps, "pricing_ms": t_p, "adjoint_ms": t_a,
+            "pricing_path_steps_per_s": ps / (t_p * 1e-3),
+            "pricing_frac_fp64_peak": ps / (t_p * 1e-3) * FLOP_HESTON_STEP / 1e12
+            / FP64_PEAK_TFLOPS,
+            "adjoint_cost_in_pricing_passes": t_a / t_p,
+            "adjoint_greeks_per_s": n_greeks / (t_a * 1e-3),
             # the same sensitivities by a forward tangent of the Euler step
-            "forward_ms": f[0].kernel_ms,
-            "forward_cost_in_pricing_passes": f[0].kernel_ms / p[0].kernel_ms,
-            "forward_greeks_per_s": 6 * len(opts) / (f[0].kernel_ms * 1e-3),
-            "forward_vs_adjoint_max_rel_diff": max(rel)}
+            "forward_ms": t_f,
+            "forward_cost_in_pricing_passes": t_f / t_p,
+            "forward_greeks_per_s": n_greeks / (t_f * 1e-3),
+            "forward_vs_adjoint_max_rel_diff": max(rel),
+            "bump_ms": t_fd, "bump_passes": 13, "bump_greeks_per_s": n_greeks / (t_fd * 1e-3),
+            "adjoint_speedup_vs_bump": t_fd / t_a, "forward_speedup_vs_bump": t_fd / t_f,
+            "adjoint_vs_bump_median_rel_diff_europeans": rel_fd}
 
 
 # Longstaff & Schwartz (2001) Table 1: American puts, K = 40, r = 6 %, 50
@@ -626,6 +663,107 @@ def dsl_leg(reps=3, cpu=True):
     return out
 
 
+def reference_draws_leg(opts, reps=2):
+    """The batched GBM workload on the reference's own draws
+    (OFG_RNG_REFERENCE: per-sample xoshiro256++ streams and the Acklam
+    inverse CDF, per-path parity with MonteCarloEngine::run): one persistent
+    launch replayed from a hipGraph, as the headline."""
+    from openferric_amd import BatchPricer, RngMode, engine_cfg
+    structs, keeps = zip(*[to_ofg(d) for d in opts])
+    b = BatchPricer(list(structs), engine_cfg(rng_mode=RngMode.Reference), use_graph=True)
+    b.run()
+    ms = [b.run() for _ in range(reps)]
+    b.close()
+    ps, flops = path_steps_and_flops(opts)
+    k = float(np.mean(ms))
+    return {"workload": "the batched GBM workload (4096 options) on the reference's draws",
+            "kernel_ms": k, "path_steps_per_s": ps / (k * 1e-3),
+            "frac_fp64_peak": flops / (k * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+
+
+CONFIG1 = dict(payoff="vanilla", option_type="call", spot=100.0, strike=100.0, expiry=1.0,
+               rate=0.05, q=0.0, vol=0.2, num_steps=252, num_paths=100_000, obs_times=None)
+
+
+def config1_leg(cpu=True, reps=5):
+    """The reference's own CPU-runnable case: a European call under GBM,
+    S0 = K = 100, r = 5 %, sigma = 20 %, T = 1, 100,000 antithetic paths x 252
+    steps, seed 42, on the reference's draws (OFG_RNG_REFERENCE), checked
+    against the oracle on the same draws and against Black-Scholes; the CPU
+    baseline is the oracle on the whole case."""
+    from openferric_amd import RngMode, engine_cfg, price_raw
+    o, keep = to_ofg(CONFIG1, seed=42)
+    out = {"workload": "European call under GBM, 100000 antithetic paths x 252 steps, "
+                       "seed 42, reference draws"}
+    ps = CONFIG1["num_paths"] * CONFIG1["num_steps"]
+    for name, mode in (("reference_draws", RngMode.Reference), ("fast_draws", RngMode.Fast)):
+        cfg = engine_cfg(antithetic=True, rng_mode=mode)
+        price_raw([o], cfg)
+        rs = [price_raw([o], cfg)[0] for _ in range(reps)]
+        k = float(np.mean([r.kernel_ms for r in rs]))
+        out[name] = {"price": rs[0].price, "stderr": rs[0].stderr_, "kernel_ms": k,
+                     "path_steps_per_s": ps / (k * 1e-3)}
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        threads = cpu_threads()
+        oo = O.make_option(spot=100.0, strike=100.0, expiry=1.0, rate=0.05, vol=0.2)
+        O.mc_run(oo, 1000, 252, 42, antithetic=True, nthreads=threads)
+        t0 = time.perf_counter()
+        n_rep = 0
+        while True:
+            p_or, se_or = O.mc_run(oo, 100_000, 252, 42, antithetic=True, nthreads=threads)
+            n_rep += 1
+            dt = time.perf_counter() - t0
+            if dt >= 2.0:
+                break
+        v = n_rep * ps / dt
+        out["cpu_baseline"] = {
+            "value": v, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"the whole case, {n_rep} repetitions in {dt:.2f} s, oracle C "
+                      f"restatement of MonteCarloEngine::run, OpenMP {threads} threads",
+            "gflops_canonical": v * FLOP_GBM_STEP / 1e9}
+        out["oracle_price"] = p_or
+        out["gpu_vs_oracle_rel_diff"] = abs(out["reference_draws"]["price"] - p_or) / p_or
+        out["speedup_vs_cpu_reference_draws"] = out["reference_draws"]["path_steps_per_s"] / v
+    out["black_scholes"] = 10.450583572185565
+    return out
+
+
+def heston_cpu_baseline(target_s=10.0):
+    """The oracle (C restatement of MonteCarloEngine::run with
+    HestonPathGenerator, all host threads) on a fixed subset of the Heston
+    barrier workload: every 32nd option, paths scaled to ~target_s."""
+    from oracle import oracle as O
+    O.build()
+    sub = heston_workload()[::32]
+    threads = cpu_threads()
+
+    def run(paths):
+        t0 = time.perf_counter()
+        for d in sub:
+            oo = O.make_option(model=O.HESTON, payoff=O.BARRIER, spot=d["spot"],
+                               strike=d["strike"], expiry=d["expiry"], rate=d["rate"],
+                               kappa=d["kappa"], theta=d["theta"], xi=d["xi"], rho=d["rho"],
+                               v0=d["v0"], barrier_dir=O.UP, barrier_style=O.OUT,
+                               barrier_level=d["barrier_level"])
+            O.mc_run(oo, paths, d["num_steps"], 42, nthreads=threads)
+        return len(sub) * paths * 365, time.perf_counter() - t0
+
+    paths = 1 << 12
+    for _ in range(6):
+        ps, dt = run(paths)
+        if dt >= 0.5 * target_s or paths >= (1 << 20):
+            break
+        paths = min(1 << 20, int(paths * target_s / max(dt, 1e-3)))
+    v = ps / dt
+    return {"value": v, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sub)} Heston up-and-out calls (every 32nd of the workload) at "
+                      f"{paths} paths x 365 steps, {ps:.3e} path-steps in {dt:.1f} s, oracle C "
+                      f"restatement, OpenMP {threads} threads",
+            "gflops_canonical": v * FLOP_HESTON_STEP / 1e9}
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (a plain copy kernel of known
     size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -702,9 +840,13 @@ def main():
     ap.add_argument("--path-scale", type=float, default=1.0,
                     help="scale per-option path counts (profiling runs only)")
     ap.add_argument("--no-greeks", action="store_true", help="skip the adjoint Greeks leg")
-    ap.add_argument("--heston", action="store_true", help="add the Heston barrier pricing leg")
-    ap.add_argument("--heston-greeks", action="store_true",
-                    help="add the batched Heston adjoint leg")
+    ap.add_argument("--no-heston", action="store_true", help="skip the Heston barrier pricing leg")
+    ap.add_argument("--no-heston-greeks", action="store_true",
+                    help="skip the batched Heston sensitivities leg")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the single European (100000 antithetic paths x 252 steps) leg")
+    ap.add_argument("--no-reference-draws", action="store_true",
+                    help="skip the batch on the reference's draws")
     ap.add_argument("--no-lsm", action="store_true", help="skip the Longstaff-Schwartz leg")
     ap.add_argument("--lsm-only", action="store_true",
                     help="run only the Longstaff-Schwartz leg (profiling)")
@@ -748,8 +890,8 @@ def main():
         return
     if args.heston_only:
         print("[bench] heston barrier leg", file=sys.stderr, flush=True)
-        h = heston_leg(RngMode.Fast)
-        print("[bench] heston adjoint leg", file=sys.stderr, flush=True)
+        h = heston_leg(RngMode.Fast, cpu=not args.no_cpu)
+        print("[bench] heston sensitivities leg", file=sys.stderr, flush=True)
         print(json.dumps({"heston": h, "heston_greeks": heston_greeks_leg(RngMode.Fast)}),
               flush=True)
         return
@@ -819,14 +961,20 @@ def main():
                      "kernel_ms_avg": k_ms,
                      "flop_model": "65 FLOP per GBM path-step, 66 with Asian running sum"},
     }
+    if not args.no_reference_draws:
+        print("[bench] batch on the reference's draws", file=sys.stderr, flush=True)
+        out["reference_draws"] = reference_draws_leg(opts)
+    if not args.no_config1:
+        print("[bench] single European leg", file=sys.stderr, flush=True)
+        out["config1"] = config1_leg(cpu=not args.no_cpu)
     if not args.no_greeks:
         print("[bench] adjoint greeks leg", file=sys.stderr, flush=True)
         out["greeks"] = greeks_leg(RngMode.Fast)
-    if args.heston:
+    if not args.no_heston:
         print("[bench] heston leg", file=sys.stderr, flush=True)
-        out["heston"] = heston_leg(RngMode.Fast)
-    if args.heston_greeks:
-        print("[bench] heston greeks leg", file=sys.stderr, flush=True)
+        out["heston"] = heston_leg(RngMode.Fast, cpu=not args.no_cpu)
+    if not args.no_heston_greeks:
+        print("[bench] heston sensitivities leg", file=sys.stderr, flush=True)
         out["heston_greeks"] = heston_greeks_leg(RngMode.Fast)
     if not args.no_lsm:
         print("[bench] longstaff-schwartz leg", file=sys.stderr, flush=True)
