@@ -76,11 +76,11 @@ __device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z, cons
 //                per-option constants (fast mode): kappa dt, kappa theta dt,
 //                xi sqrt(dt), and the log-spot increment's mu dt, dt / 2,
 //                sqrt(dt) pre-scaled by 256/ln2 for exp2_256.
-// Intermediates of one Heston step that its tangent needs: sqrt of the
-// positive variance, the correlated normal, and the unclamped spot and
-// variance.
+// Intermediates of one Heston step that its tangent needs: the positive
+// part of the variance and its sqrt, the correlated normal, and the
+// unclamped spot and variance.
 struct HestonParts {
-    double sv, zs, sn, vn;
+    double vp, sv, zs, sn, vn;
 };
 
 template <bool REF>
@@ -110,6 +110,7 @@ __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o
     const double sn = s * growth;
     s_next = fmax(sn, 1e-12);   // f64::max(sn, 1e-12): maxNum, NaN -> 1e-12
     v_next = fmax(vn, 0.0);
+    hp.vp = v_pos;
     hp.sv = sv;
     hp.zs = zs;
     hp.sn = sn;

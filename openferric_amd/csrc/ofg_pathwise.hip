@@ -156,34 +156,75 @@ __device__ __forceinline__ double pathwise_sample(const OptDev &o, int64_t sampl
 
 // ---------------------------------------------------------------------------
 // Heston: forward tangent of the Euler step (no closed form for the variance
-// path). Each leg carries d ln S and d v for the gradient slots 0 S0, 1 v0,
-// 2 r, 3 T, 4 kappa, 5 theta, 6 xi, 7 rho, with the adjoint kernel's
-// conventions: the clamp indicators 1{sn >= 1e-12} and 1{vn > 0}, d v_pos/dv =
-// 1{v > 0}, d sqrt(v_pos)/d v_pos = 1/(2 sqrt) taken as 0 at v_pos = 0, and
-// d/dT through dt = T/n. Asian legs also carry sum_j w_j S_j d ln S_j (a) and
-// sum_j w_j d ln S_j (gq).
+// path), with the adjoint kernel's conventions: the clamp indicators
+// 1{sn >= 1e-12} and 1{vn > 0}, d v_pos/dv = 1{v > 0}, d sqrt(v_pos)/d v_pos =
+// 1/(2 sqrt) taken as 0 at v_pos = 0.
+//
+// Only v0, kappa, theta and xi move the variance, so only they carry a
+// variance tangent dv and a log-spot tangent dl through each step:
+//   dl' = dl + B dv,              B = d e / d v,
+//   dv' = 1{vn > 0} (A dv + c),   A = d vn / d v, c = d vn / d parameter.
+// The recursions run in rescaled units that keep constant factors out of the
+// step (each factor is applied once, after the last step): B' = B / sqrt(dt)
+// = zs / (2 sqrt(v+)) - sqrt(dt) / 2, and four variance tangents
+//   v0 (c = 0), Y (c' = v+), U (c' = 1), X (c' = sqrt(v+) z1)
+// with dv_v0 = v0's, dv_kappa = dt (theta U - Y), dv_theta = kappa dt U and
+// dv_xi = sqrt(dt) X; m_k = sum B' tangent_k gives d ln S (v0: sqrt(dt) m_v0,
+// kappa: dt sqrt(dt) (theta m_U - m_Y), theta: kappa dt sqrt(dt) m_U,
+// xi: dt m_X).
+// The other slots need no per-step tangent:
+//   S0   d ln S_j / d S0 = 1/S0 until the spot floor is first hit, then 0;
+//   mu   d ln S_j / d mu = dt x (steps since the last spot floor);
+//   rho  d ln S_j / d rho = sqrt(dt) (P1 - (rho / rho_c) P2) with P1, P2 the
+//        sums of sqrt(v+) z1 and sqrt(v+) z2 over the steps since the last floor;
+//   T    the Euler path depends on T only through kappa dt, theta dt, xi dt,
+//        mu dt and v0 dt (in the per-step variance w = v dt), so path by path
+//        T dP/dT = kappa P_kappa + theta P_theta + xi P_xi + mu P_mu + v0 P_v0
+//        and the T slot follows from the others after the last step.
+// A spot-floor hit (rare) resets the log-spot tangents in a divergent branch.
+// Asian legs also carry every slot's observation sum: sum_j w_j S_j d ln S_j
+// (arithmetic average) or sum_j w_j d ln S_j (geometric).
 // ---------------------------------------------------------------------------
 struct HTan {
-    double ls[ADJ_NG], dv[ADJ_NG], a[ADJ_NG], gq[ADJ_NG];
+    double dv[4], dl[4];          // v0, Y, U, X (rescaled as above): tangent and m
+    double p1, p2;                // rho running sums
+    double nf;                    // steps since the last spot floor (mu; Asian only)
+    int jf;                       // path index of the last spot floor (0: none)
+    bool s0_live;                 // the spot floor has not been hit (S0)
+    double obs_s0, obs_s0_frozen; // S0 observation sum, running and at the first floor
+    double a[4], ap1, ap2, amu;   // observation sums of the other slots (Asian)
 };
+
+template <int PK>
+__device__ __forceinline__ bool htan_geometric(const OptDev &o) {
+    if constexpr (PK == PK_ASIAN_LOG) return (o.flags & F_GEOMETRIC) != 0;
+    return false;
+}
 
 template <int PK>
 __device__ __forceinline__ void htan_init(HTan &t, const OptDev &o, double w0) {
 #pragma unroll
-    for (int k = 0; k < ADJ_NG; ++k) t.ls[k] = t.dv[k] = t.a[k] = t.gq[k] = 0.0;
-    t.ls[0] = 1.0 / o.s0;
-    t.dv[1] = 1.0;
-    if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) t.a[0] = w0;
-    if constexpr (PK == PK_ASIAN_LOG) t.gq[0] = w0 * t.ls[0];
+    for (int k = 0; k < 4; ++k) t.dv[k] = t.dl[k] = t.a[k] = 0.0;
+    t.dv[0] = 1.0;
+    t.p1 = t.p2 = t.nf = t.ap1 = t.ap2 = t.amu = 0.0;
+    t.jf = 0;
+    t.s0_live = true;
+    t.obs_s0_frozen = 0.0;
+    // path index 0 is S0 itself: only the S0 slot has a tangent there
+    t.obs_s0 = htan_geometric<PK>(o) ? w0 : w0 * o.s0;
 }
 
+// zero_start: this is the first step of a path with v0 = 0 (the only step
+// at which v = 0 while a variance tangent is nonzero: after a clamp to 0
+// every variance tangent is 0 as well, so elsewhere d v_pos / d v = 1{v > 0}
+// may be taken as 1).
 template <int PK, bool REF>
 __device__ __forceinline__ void heston_tangent_step(Leg &g, double &v, HTan &t, const OptDev &o,
                                                     double z1, double z2, const double *tab,
-                                                    double half_isdt, double inv_n) {
+                                                    int j, bool zero_start) {
+    constexpr bool ASIAN = PK == PK_ASIAN || PK == PK_ASIAN_LOG;
     double s_next, v_next, growth, e;
     HestonParts hp;
-    const double v_old = v;
     heston_euler<REF>(g.s, v, o, z1, z2, s_next, v_next, growth, e, tab, hp);
     if constexpr (PK == PK_ASIAN_LOG) {
         const bool floor_hit = !(g.s * growth >= 1e-12);
@@ -192,73 +233,109 @@ __device__ __forceinline__ void heston_tangent_step(Leg &g, double &v, HTan &t, 
     g.s = s_next;
     v = v_next;
 
-    const double v_pos = fmax(v_old, 0.0);
-    const double av = v_old > 0.0 ? 1.0 : 0.0;
-    const double hinv = hp.sv > 0.0 ? 0.5 / hp.sv : 0.0;
-    const double ind_s = hp.sn >= 1e-12 ? 1.0 : 0.0;
-    const double ind_v = hp.vn > 0.0 ? 1.0 : 0.0;
-    const double svd = av * hinv;   // d sqrt(v_pos) / d v
-    const double A = fma(o.xi * o.sqrt_dt * z1, svd, fma(-o.kappa * o.dt, av, 1.0));  // d vn / d v
-    const double B = fma(o.sqrt_dt * hp.zs, svd, -0.5 * o.dt * av);                 // d e / d v
-    const double sq = hp.sv * o.sqrt_dt;
-    const double dv_kappa = (o.theta - v_pos) * o.dt;
-    const double dv_theta = o.kappa * o.dt;
-    const double dv_xi = sq * z1;
-    const double de_rho = sq * fma(-o.rho / o.rho_c, z2, z1);
-    const double de_t = fma(hp.sv * hp.zs, half_isdt, o.mu - 0.5 * v_pos) * inv_n;
-    const double dv_t = fma(o.xi * hp.sv * z1, half_isdt, o.kappa * (o.theta - v_pos)) * inv_n;
-
-    // S0 and r never move v
-    t.ls[0] = t.ls[0] * ind_s;
-    t.ls[2] = (t.ls[2] + o.dt) * ind_s;
-    // v0
-    t.ls[1] = fma(t.dv[1], B, t.ls[1]) * ind_s;
-    t.dv[1] = t.dv[1] * A * ind_v;
-    // T
-    t.ls[3] = (t.ls[3] + fma(t.dv[3], B, de_t)) * ind_s;
-    t.dv[3] = fma(t.dv[3], A, dv_t) * ind_v;
-    // kappa, theta, xi (variance direct terms)
-    t.ls[4] = fma(t.dv[4], B, t.ls[4]) * ind_s;
-    t.dv[4] = fma(t.dv[4], A, dv_kappa) * ind_v;
-    t.ls[5] = fma(t.dv[5], B, t.ls[5]) * ind_s;
-    t.dv[5] = fma(t.dv[5], A, dv_theta) * ind_v;
-    t.ls[6] = fma(t.dv[6], B, t.ls[6]) * ind_s;
-    t.dv[6] = fma(t.dv[6], A, dv_xi) * ind_v;
-    // rho (log-spot direct term)
-    t.ls[7] = (t.ls[7] + fma(t.dv[7], B, de_rho)) * ind_s;
-    t.dv[7] = t.dv[7] * A * ind_v;
+    const double v_pos = hp.vp;
+    const bool live = v_pos > 0.0;  // v > 0
+    double hinv;  // d sqrt(v_pos) / d v_pos
+    if constexpr (REF) {
+        hinv = live ? 0.5 / hp.sv : 0.0;
+    } else {
+        // 1 / (2 sv) by one Newton step from the rsqrt seed that sqrt_pos
+        // starts from (relative error 2^-23 -> 2^-46)
+        const double y = rsq_seed(v_pos);
+        hinv = live ? (0.5 * y) * fma(-hp.sv, y, 2.0) : 0.0;
+    }
+    double A, Bs;  // d vn / d v and (d e / d v) / sqrt(dt)
+    if (zero_start) {
+        A = 1.0;
+        Bs = 0.0;
+    } else {
+        A = fma(o.xsdt * z1, hinv, 1.0 - o.kdt);
+        Bs = fma(hp.zs, hinv, -0.5 * o.sqrt_dt);
+    }
+    // the variance clamp: 1{vn > 0} (0 for NaN)
+    const double iv = hp.vn > 0.0 ? 1.0 : 0.0;
+    const double Ai = A * iv;
+    const double svz1 = hp.sv * z1;
+    // log-spot tangents through e, at the variance tangents of v_old
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t.dl[k] = fma(Bs, t.dv[k], t.dl[k]);
+    t.p1 += svz1;
+    t.p2 = fma(hp.sv, z2, t.p2);
+    if constexpr (ASIAN) t.nf += 1.0;
+    // variance tangents: v0 (no direct term), Y (v+), U (1), X (sqrt(v+) z1)
+    t.dv[0] *= Ai;
+    t.dv[1] = fma(t.dv[1], Ai, v_pos * iv);
+    t.dv[2] = fma(t.dv[2], Ai, iv);
+    t.dv[3] = fma(t.dv[3], Ai, svz1 * iv);
+    // spot floor (or NaN): d ln S = 0 for every slot from here. Rare, so the
+    // wave branches on whether any lane hit it and pays nothing otherwise.
+    const bool floored = !(hp.sn >= 1e-12);
+    if (__builtin_expect(__ballot(floored) != 0, 0)) {
+        if (floored) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) t.dl[k] = 0.0;
+            t.p1 = t.p2 = t.nf = 0.0;
+            t.jf = j + 1;
+            if (t.s0_live) t.obs_s0_frozen = t.obs_s0;
+            t.s0_live = false;
+        }
+    }
 }
 
 template <int PK>
-__device__ __forceinline__ void htan_observe(HTan &t, const Leg &g, double wj) {
+__device__ __forceinline__ void htan_observe(HTan &t, const Leg &g, const OptDev &o, double wj) {
     if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) {
-        const double ws = wj * g.s;
+        const double q = htan_geometric<PK>(o) ? wj : wj * g.s;
 #pragma unroll
-        for (int k = 0; k < ADJ_NG; ++k) t.a[k] = fma(ws, t.ls[k], t.a[k]);
-    }
-    if constexpr (PK == PK_ASIAN_LOG) {
-#pragma unroll
-        for (int k = 0; k < ADJ_NG; ++k) t.gq[k] = fma(wj, t.ls[k], t.gq[k]);
+        for (int k = 0; k < 4; ++k) t.a[k] = fma(q, t.dl[k], t.a[k]);
+        t.ap1 = fma(q, t.p1, t.ap1);
+        t.ap2 = fma(q, t.p2, t.ap2);
+        t.amu = fma(q, t.nf, t.amu);
+        t.obs_s0 += q;
     }
 }
 
 template <int PK>
 __device__ __forceinline__ double heston_leg_gradient(const Leg &g, const OptDev &o,
                                                       const HTan &t, double (&gr)[ADJ_NG]) {
+    constexpr bool ASIAN = PK == PK_ASIAN || PK == PK_ASIAN_LOG;
     double ct, ca, cg;
     const double x = leg_payoff_seeds<PK>(g, o, ct, ca, cg);
     const double pv = o.df * x;
     const double T = o.dt * (double)o.steps;
     const double tS = ct * g.s;
-#pragma unroll
-    for (int k = 0; k < ADJ_NG; ++k) {
-        double d = tS * t.ls[k];
-        if constexpr (PK == PK_ASIAN || PK == PK_ASIAN_LOG) d = fma(ca, t.a[k], d);
-        if constexpr (PK == PK_ASIAN_LOG) d = fma(cg, t.gq[k], d);
-        gr[k] += o.df * d;
-    }
-    gr[2] -= T * pv;        // mu = r - q and the discount exp(-r T)
-    gr[3] -= o.rate * pv;   // the discount's T
+    const double cobs = htan_geometric<PK>(o) ? cg : ca;
+    // d pv / d parameter from the terminal log-spot tangent and the
+    // observation sum
+    auto dpv = [&](double dln, double obs) {
+        double d = tS * dln;
+        if constexpr (ASIAN) d = fma(cobs, obs, d);
+        return o.df * d;
+    };
+    const double rr = o.rho / o.rho_c;
+    const double inv_s0 = 1.0 / o.s0;
+    const double sdt = o.sqrt_dt, dt = o.dt;
+    const double dt15 = dt * sdt;
+    const double g_s0 = dpv(t.s0_live ? inv_s0 : 0.0,
+                            (t.s0_live ? t.obs_s0 : t.obs_s0_frozen) * inv_s0);
+    const double g_v0 = dpv(sdt * t.dl[0], sdt * t.a[0]);
+    const double g_kappa = dpv(dt15 * fma(o.theta, t.dl[2], -t.dl[1]),
+                               dt15 * fma(o.theta, t.a[2], -t.a[1]));
+    const double g_theta = dpv(o.kappa * dt15 * t.dl[2], o.kappa * dt15 * t.a[2]);
+    const double g_xi = dpv(dt * t.dl[3], dt * t.a[3]);
+    const double nf = ASIAN ? t.nf : (double)(o.steps - t.jf);
+    const double g_mu = dpv(nf * dt, t.amu * dt);
+    const double g_rho = dpv(sdt * fma(-rr, t.p2, t.p1), sdt * fma(-rr, t.ap2, t.ap1));
+    gr[0] += g_s0;
+    gr[1] += g_v0;
+    gr[2] += g_mu - T * pv;   // mu = r - q and the discount exp(-r T)
+    const double tdp = o.kappa * g_kappa + o.theta * g_theta + o.xi * g_xi + o.mu * g_mu +
+                       o.v0 * g_v0;
+    gr[3] += tdp / T - o.rate * pv;
+    gr[4] += g_kappa;
+    gr[5] += g_theta;
+    gr[6] += g_xi;
+    gr[7] += g_rho;
     return pv;
 }
 
@@ -271,8 +348,6 @@ __device__ __forceinline__ double heston_tangent_sample(const OptDev &o, int64_t
     constexpr bool REF = RNG != OFG_RNG_FAST;
     const double barrier = o.barrier;
     const int steps = o.steps;
-    const double half_isdt = 0.5 / o.sqrt_dt;
-    const double inv_n = 1.0 / (double)steps;
     Leg ga, gb;
     HTan ta, tb;
     double va = o.v0, vb = o.v0;
@@ -292,15 +367,17 @@ __device__ __forceinline__ double heston_tangent_sample(const OptDev &o, int64_t
     } else {
         rng.seed(stream_seed(o.seed, (uint64_t)sample));
     }
+    const bool v0_zero = o.v0 == 0.0;
     auto advance = [&](double z1, double z2, int j) {
-        heston_tangent_step<PK, REF>(ga, va, ta, o, z1, z2, tab, half_isdt, inv_n);
-        if constexpr (ANTI) heston_tangent_step<PK, REF>(gb, vb, tb, o, -z1, -z2, tab, half_isdt, inv_n);
+        const bool zs = v0_zero && j == 0;
+        heston_tangent_step<PK, REF>(ga, va, ta, o, z1, z2, tab, j, zs);
+        if constexpr (ANTI) heston_tangent_step<PK, REF>(gb, vb, tb, o, -z1, -z2, tab, j, zs);
         const double wj = ASIAN ? w[j + 1] : 0.0;
         leg_observe<PK>(ga, wj, barrier);
-        htan_observe<PK>(ta, ga, wj);
+        htan_observe<PK>(ta, ga, o, wj);
         if constexpr (ANTI) {
             leg_observe<PK>(gb, wj, barrier);
-            htan_observe<PK>(tb, gb, wj);
+            htan_observe<PK>(tb, gb, o, wj);
         }
     };
     if constexpr (RNG == OFG_RNG_FAST) {
@@ -351,14 +428,16 @@ __device__ void pathwise_item(const OptDev &o, const WorkItem &wi, int lane, con
     }
 }
 
-// Waves per SIMD asked of the register allocator. The Heston tangent holds
-// 16 (Asian: 32) extra doubles per leg: with one leg it fits two waves (256
-// VGPRs) without spilling; two antithetic legs do not. The fast-draw GBM
+// Waves per SIMD asked of the register allocator. The single-leg fast-draw
+// Heston tangent keeps its vanilla and barrier step loops free of spills at
+// three waves (168 VGPRs; only the Asian paths spill) and runs 4 % faster
+// than at two; at four the step loop itself spills. The reference-draw
+// variants stay at two, two antithetic legs at one. The fast-draw GBM
 // tangent takes 169-171 VGPRs left free, just over the 168 of three waves,
 // and fits them.
 template <int RNG, int MODEL, bool ANTI>
 constexpr int pathwise_waves_target() {
-    if (MODEL == OFG_MODEL_HESTON) return ANTI ? 1 : 2;
+    if (MODEL == OFG_MODEL_HESTON) return ANTI ? 1 : (RNG == OFG_RNG_FAST ? 3 : 2);
     return RNG == OFG_RNG_FAST ? 3 : 1;
 }
 
@@ -380,6 +459,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
         const int32_t it = __builtin_amdgcn_readfirstlane(a.order[a.order_begin + (int32_t)q]);
         const WorkItem wi = a.items[it];
         const OptDev o = a.opts[wi.option];
+        // per-lane sums in the adjoint kernel's order, so both return the
+        // same price bit for bit
         double spv = 0.0, spv2 = 0.0;
         double acc[ADJ_NG] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         switch (o.kind) {

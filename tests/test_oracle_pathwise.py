@@ -182,3 +182,25 @@ def test_planted_mistakes_are_caught(mutate, case, field):
     fd = _per_path_fd(kw, field, paths, STEPS, seed, False)
     bad = _mismatch(fd, r["samples"][:, 1 + SLOTS[field]])
     assert bad.mean() > 0.3, (mutate, case, field, bad.mean())
+
+
+@pytest.mark.parametrize("name,kw,fields", CASES, ids=[c[0] for c in CASES])
+def test_maturity_slot_from_scaling_identity(name, kw, fields):
+    """The Euler path depends on T only through kappa dt, theta dt, xi dt,
+    mu dt and v0 dt (GBM: mu dt and sigma sqrt(dt)) in the per-step variance
+    w = v dt, so path by path
+      dP/dT = (kappa P_kappa + theta P_theta + xi P_xi + mu P_mu + v0 P_v0) / T - r pv
+    (GBM: (mu P_mu + sigma P_sigma / 2) / T - r pv), P_mu = P_r + T pv.
+    The oracle's directly propagated T slot satisfies it to rounding."""
+    o = _opt(**kw)
+    r = O.pathwise_greeks(o, 2000, STEPS, 31, antithetic=True, want_samples=True)
+    pv, g = r["samples"][:, 0], r["samples"][:, 1:]
+    T, mu, rate = kw["expiry"], kw["rate"] - kw["div_yield"], kw["rate"]
+    g_mu = g[:, 2] + T * pv
+    if kw["model"] == O.HESTON:
+        ident = (kw["kappa"] * g[:, 4] + kw["theta"] * g[:, 5] + kw["xi"] * g[:, 6] +
+                 mu * g_mu + kw["v0"] * g[:, 1]) / T - rate * pv
+    else:
+        ident = (mu * g_mu + 0.5 * kw["vol"] * g[:, 1]) / T - rate * pv
+    scale = np.abs(g[:, 3]).mean() + np.abs(ident).mean()
+    assert np.max(np.abs(ident - g[:, 3])) <= 1e-11 * scale
