@@ -343,6 +343,29 @@ int32_t ofg_price(ofg_ctx *ctx, const ofg_option *opts, size_t n,
 int32_t ofg_sample_payoffs(ofg_ctx *ctx, const ofg_option *opt,
                            const ofg_engine_cfg *cfg, double *xs, double *ys);
 
+/* Limits of the fixed device region ofg_price_noalloc prices in. */
+#define OFG_NOALLOC_MAX_OPTIONS 4096
+#define OFG_NOALLOC_MAX_WORK_ITEMS 262144
+#define OFG_NOALLOC_MAX_OBS_WEIGHTS 1048576
+
+/* ofg_price without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The staged
+ * contracts, the work queue and its counters, the partial sums and the
+ * results live in one fixed region of the library per GPU (33 MiB, resolved
+ * by ofg_create, held until the process ends); the call makes no device
+ * allocation or free. Limits: at most OFG_NOALLOC_MAX_OPTIONS options and
+ * OFG_NOALLOC_MAX_OBS_WEIGHTS observation weights (num_steps + 1 per Asian
+ * option, 1 per other option); the work-item size is doubled until the queue
+ * holds at most OFG_NOALLOC_MAX_WORK_ITEMS items. Beyond them, or with
+ * cfg->rng_mode OFG_RNG_EXTERNAL or greek_mode other than OFG_GREEKS_NONE,
+ * the call returns OFG_INVALID_INPUT before anything runs. The region keeps
+ * the last call's contents; calls on one GPU share it, so the library
+ * serializes them (a call from another context on the same GPU waits).
+ * Results equal ofg_price's for the same launch shape. out[n]. */
+int32_t ofg_price_noalloc(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                          const ofg_engine_cfg *cfg, ofg_result *out);
+
 /* Pathwise adjoint Greeks. Replaces mc_european_pathwise_aad
  * (src/engines/monte_carlo/mc_aad.rs:75-219) and heston_price_delta_aad
  * (mc_aad.rs:279-342), generalised to Asian/barrier payoffs and the full

@@ -40,7 +40,7 @@ EXPORTED_SYMBOLS = [
     "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump", "ofg_lsm_last_timing",
     "ofg_qmc_price", "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation", "ofg_stress_correlation",
     "ofg_dsl_compile", "ofg_dsl_free", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
-    "ofg_dsl_price",
+    "ofg_dsl_price", "ofg_price_noalloc",
 ]
 
 
@@ -229,6 +229,8 @@ def load() -> C.CDLL:
         L.ofg_price.restype = C.c_int32
         L.ofg_price_greeks.argtypes = L.ofg_price.argtypes
         L.ofg_price_greeks.restype = C.c_int32
+        L.ofg_price_noalloc.argtypes = L.ofg_price.argtypes
+        L.ofg_price_noalloc.restype = C.c_int32
         L.ofg_sample_payoffs.argtypes = [pv, C.POINTER(OfgOption), C.POINTER(OfgEngineCfg),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ofg_sample_payoffs.restype = C.c_int32

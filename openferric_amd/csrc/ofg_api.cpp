@@ -377,6 +377,9 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
         int64_t spl = (int64_t)std::llround((double)tune.item_steps / per_sample);
         spl = std::max<int64_t>(1, std::min<int64_t>(spl, 1024));
         const int64_t item_samples = 64 * spl;
+        // refuse before building the queue: 2^27 items is already 4 GiB of host plan
+        if ((samples + item_samples - 1) / item_samples > ((int64_t)1 << 27) - (int64_t)p.items.size())
+            fail(OFG_INVALID_INPUT, "too many work items");
         p.item_first.push_back((int64_t)p.items.size());
         int32_t count = 0;
         for (int64_t b = 0; b < samples; b += item_samples) {
@@ -935,6 +938,134 @@ int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_eng
     return OFG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Pricing without device allocation (ofg_price_noalloc): every device buffer
+// of the pricing launch -- the staged plan, the work queue and its counters,
+// the per-item partial sums and the per-option results -- lives in one fixed
+// region per GPU, the file-scope array `scratch`. Host code never uses the
+// array's name as an address: ofg_create resolves each GPU's copy with
+// hipGetSymbolAddress and the context keeps that pointer.
+// ---------------------------------------------------------------------------
+constexpr size_t NA_MAX_OPTIONS = OFG_NOALLOC_MAX_OPTIONS;
+constexpr size_t NA_MAX_ITEMS = OFG_NOALLOC_MAX_WORK_ITEMS;
+constexpr size_t NA_MAX_OBS = OFG_NOALLOC_MAX_OBS_WEIGHTS;
+constexpr size_t na_align(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NA_OPTS = 0;
+constexpr size_t NA_OBS = NA_OPTS + na_align(NA_MAX_OPTIONS * sizeof(ofg::OptDev));
+constexpr size_t NA_ITEMS = NA_OBS + na_align(NA_MAX_OBS * sizeof(double));
+constexpr size_t NA_ORDER = NA_ITEMS + na_align(NA_MAX_ITEMS * sizeof(ofg::WorkItem));
+constexpr size_t NA_FIRST = NA_ORDER + na_align(NA_MAX_ITEMS * sizeof(int32_t));
+constexpr size_t NA_COUNT = NA_FIRST + na_align(NA_MAX_OPTIONS * sizeof(int64_t));
+constexpr size_t NA_PARTIALS = NA_COUNT + na_align(NA_MAX_OPTIONS * sizeof(int32_t));
+constexpr size_t NA_OUTS = NA_PARTIALS + na_align(NA_MAX_ITEMS * sizeof(ofg::Partial));
+constexpr size_t NA_HEAD = NA_OUTS + na_align(NA_MAX_OPTIONS * sizeof(ofg::OptOut));
+constexpr size_t NA_BYTES = NA_HEAD + 256;
+static_assert(NA_BYTES <= ((size_t)34 << 20), "no-allocation region larger than stated");
+
+// One copy per GPU the library runs on, held until the process ends.
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_init_mu;
+void *g_na_ptr[NA_MAX_DEVICES] = {};
+// Calls on one GPU share its region: they are serialized per GPU.
+std::mutex g_na_mu[NA_MAX_DEVICES];
+
+// This GPU's copy of `scratch` (the device must be current); nullptr if the
+// address cannot be resolved.
+void *noalloc_region(int device) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_na_init_mu);
+    if (!g_na_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_na_ptr[device] = p;
+    }
+    return g_na_ptr[device];
+}
+
+template <class T>
+void stage(unsigned char *base, size_t offset, const std::vector<T> &v, hipStream_t s) {
+    if (!v.empty())
+        hip_check(hipMemcpyAsync(base + offset, v.data(), v.size() * sizeof(T),
+                                 hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync H2D");
+}
+
+int32_t price_noalloc_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                           const ofg_engine_cfg *cfg, ofg_result *out) {
+    if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
+    if (cfg->rng_mode != OFG_RNG_REFERENCE && cfg->rng_mode != OFG_RNG_FAST)
+        fail(OFG_INVALID_INPUT,
+             "ofg_price_noalloc takes generated draws (OFG_RNG_REFERENCE or OFG_RNG_FAST)");
+    if (cfg->greek_mode != OFG_GREEKS_NONE)
+        fail(OFG_INVALID_INPUT, "ofg_price_noalloc prices only (greek_mode OFG_GREEKS_NONE)");
+    check_tuning(*cfg);
+    if (n > NA_MAX_OPTIONS)
+        fail(OFG_INVALID_INPUT, "ofg_price_noalloc takes at most 4096 options per call");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    unsigned char *base = static_cast<unsigned char *>(ctx->noalloc);
+    if (!base) fail(OFG_DEVICE_ERROR, "the no-allocation region is not available on this device");
+    Plan probe = build_plan(opts, n, *cfg, Tuning{});
+    Tuning t = tuning_for(ctx, probe.opts, *cfg);
+    Plan p = (t.item_steps == Tuning{}.item_steps) ? std::move(probe) : build_plan(opts, n, *cfg, t);
+    // larger work items until the queue fits its part of the region
+    while (p.items.size() > NA_MAX_ITEMS) {
+        if (t.item_steps >= (1 << 26))
+            fail(OFG_INVALID_INPUT, "ofg_price_noalloc: the work needs more than 262144 work items");
+        t.item_steps *= 2;
+        p = build_plan(opts, n, *cfg, t);
+    }
+    if (p.obs_w.size() > NA_MAX_OBS)
+        fail(OFG_INVALID_INPUT,
+             "ofg_price_noalloc takes at most 1048576 observation weights per call");
+    const size_t nopt = p.opts.size();
+    std::vector<ofg::OptOut> outs(nopt);
+    double ms = 0.0;
+    if (nopt) {
+        std::lock_guard<std::mutex> region_lock(g_na_mu[ctx->device]);
+        hipStream_t s = ctx->stream;
+        stage(base, NA_OPTS, p.opts, s);
+        stage(base, NA_OBS, p.obs_w, s);
+        stage(base, NA_ITEMS, p.items, s);
+        stage(base, NA_ORDER, p.order, s);
+        stage(base, NA_FIRST, p.item_first, s);
+        stage(base, NA_COUNT, p.item_count, s);
+        ofg::LaunchArgs a{};
+        a.opts = reinterpret_cast<const ofg::OptDev *>(base + NA_OPTS);
+        a.items = reinterpret_cast<const ofg::WorkItem *>(base + NA_ITEMS);
+        a.order = reinterpret_cast<const int32_t *>(base + NA_ORDER);
+        a.partials = reinterpret_cast<ofg::Partial *>(base + NA_PARTIALS);
+        a.obs_w = reinterpret_cast<const double *>(base + NA_OBS);
+        a.rng_mode = cfg->rng_mode;
+        auto *d_outs = reinterpret_cast<ofg::OptOut *>(base + NA_OUTS);
+        hip_check(hipEventRecord(ctx->ev0, s), "event");
+        // the queue counters are zeroed by a kernel; every item's partial and
+        // every option's result is written before it is read
+        enqueue_pricing(a, p, t, ctx->device, reinterpret_cast<uint32_t *>(base + NA_HEAD), s);
+        hip_check(ofg::launch_reduce(a.opts, reinterpret_cast<const int64_t *>(base + NA_FIRST),
+                                     reinterpret_cast<const int32_t *>(base + NA_COUNT),
+                                     a.partials, d_outs, (int32_t)nopt, s),
+                  "reduce kernel launch");
+        hip_check(hipEventRecord(ctx->ev2, s), "event");
+        hip_check(hipMemcpyAsync(outs.data(), d_outs, nopt * sizeof(ofg::OptOut),
+                                 hipMemcpyDeviceToHost, s),
+                  "D2H results");
+        hip_check(hipStreamSynchronize(s), "kernel execution");
+        float f = 0.f;
+        hip_check(hipEventElapsedTime(&f, ctx->ev0, ctx->ev2), "event time");
+        ms = f;
+    }
+    std::vector<const ofg::OptOut *> map(n, nullptr);
+    for (size_t k = 0; k < p.active.size(); ++k) map[(size_t)p.active[k]] = &outs[k];
+    for (size_t i = 0; i < n; ++i) fill_result(out[i], opts[i], map[i], ms);
+    return OFG_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -963,6 +1094,9 @@ ofg_ctx *ofg_create(int32_t device) {
         hip_check(hipEventCreate(&c->ev2), "hipEventCreate");
         hip_check(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device),
                   "hipDeviceGetAttribute");
+        // ofg_price_noalloc's region on this GPU (nullptr: that entry point
+        // reports a device error; the others are unaffected)
+        c->noalloc = noalloc_region(device);
         ctx = c.release();
         trace("create: done");
         return OFG_OK;
@@ -993,6 +1127,11 @@ void ofg_destroy(ofg_ctx *ctx) {
 int32_t ofg_price(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
                   ofg_result *out) {
     return guarded([&]() { return price_impl(ctx, opts, n, cfg, out); });
+}
+
+int32_t ofg_price_noalloc(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                          const ofg_engine_cfg *cfg, ofg_result *out) {
+    return guarded([&]() { return price_noalloc_impl(ctx, opts, n, cfg, out); });
 }
 
 int32_t ofg_price_greeks(ofg_ctx *ctx, const ofg_option *opts, size_t n,

@@ -128,6 +128,7 @@ struct ofg_ctx {
     ofg_host::MultiBuffers *multi = nullptr;   // multi-asset scenarios and partials
     ofg_host::BasketBuffers *basket = nullptr; // basket scenarios, seeds and partials
     ofg_host::DslBuffers *dsl = nullptr;       // structured-product programs and snapshots
+    void *noalloc = nullptr;   // this GPU's fixed region of ofg_price_noalloc (not owned)
     int cus = 256;
 };
 

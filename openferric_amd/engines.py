@@ -145,6 +145,18 @@ def price_raw(opts: Sequence[_lib.OfgOption], cfg: _lib.OfgEngineCfg, device: in
     return list(out)
 
 
+def price_noalloc_raw(opts: Sequence[_lib.OfgOption], cfg: _lib.OfgEngineCfg,
+                      device: int = 0) -> List[_lib.OfgResult]:
+    """Calls ofg_price_noalloc: ofg_price with every device buffer in the
+    library's fixed per-GPU region (no device allocation in the call)."""
+    L = _lib.load()
+    ctx = _lib.context(device)
+    arr = (_lib.OfgOption * len(opts))(*opts)
+    out = (_lib.OfgResult * len(opts))()
+    _raise(L.ofg_price_noalloc(ctx, arr, len(opts), C.byref(cfg), out))
+    return list(out)
+
+
 def sample_payoffs(opt: _lib.OfgOption, cfg: _lib.OfgEngineCfg, antithetic: bool,
                    device: int = 0, want_y: bool = True) -> Tuple[np.ndarray, Optional[np.ndarray]]:
     """Per-sample (x, y) of one option via ofg_sample_payoffs."""
