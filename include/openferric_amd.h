@@ -336,6 +336,42 @@ const char *ofg_version(void);
 int32_t ofg_price(ofg_ctx *ctx, const ofg_option *opts, size_t n,
                   const ofg_engine_cfg *cfg, ofg_result *out);
 
+/* Absolute bumps of finite_difference_greeks / bump_and_reprice
+ * (src/greeks/bsm.rs:130-212): spot, rate, vol, maturity. */
+typedef struct ofg_fd_bumps {
+    double ds, dr, dv, dt;
+} ofg_fd_bumps;
+
+/* FiniteDifferenceGreeks (src/greeks/bsm.rs:16-24), plus the device time. */
+typedef struct ofg_fd_greeks {
+    double delta, gamma, vega, theta, rho, vanna, volga;
+    double kernel_ms;
+} ofg_fd_greeks;
+
+/* finite_difference_greeks (src/greeks/bsm.rs:165-212) with the Monte Carlo
+ * engine as the GenericPricer (bsm.rs:44-55): price(s, k, r, sigma, t) is
+ * the price of opts[i] with spot s, strike k, rate r, vol sigma and expiry t
+ * (num_steps unchanged; an Asian option's fixings scaled by t / expiry, so
+ * each stays at the same path index). GBM options only (sigma is the
+ * market vol). The reference's 13 revaluations -- s +- ds (s - ds floored at
+ * 1e-8), r +- dr, sigma +- dv (sigma - dv floored at 1e-8), t +- dt (t - dt
+ * floored at 1e-8), the four (s, sigma) corners -- run on the option's own
+ * seed, so every bump shares the base price's draws, and all n options'
+ * revaluations are priced in one batched launch. The Greeks combine them
+ * with the reference's formulas (theta = -dV/dT; vanna from the four
+ * corners; volga). cfg: rng_mode, antithetic, control_variate as ofg_price
+ * (greek_mode ignored). out[n]. */
+int32_t ofg_finite_difference_greeks(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                                     const ofg_engine_cfg *cfg, const ofg_fd_bumps *bumps,
+                                     ofg_fd_greeks *out);
+
+/* bump_and_reprice (src/greeks/bsm.rs:130-163) with the same pricer:
+ * out[6 i .. 6 i + 5] = (base, s + ds, r + dr, sigma + dv, expiry
+ * max(t + dt, 1e-8), s - ds) prices of opts[i] (the spot-down bump is not
+ * floored, as in the reference), all in one batched launch. out[6 n]. */
+int32_t ofg_bump_and_reprice(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                             const ofg_engine_cfg *cfg, const ofg_fd_bumps *bumps, double *out);
+
 /* Per-sample payoffs of ONE option: x[i] (and control variate y[i]) exactly
  * as the fold body of MonteCarloEngine::run computes them
  * (simulation.rs:212-252), antithetic legs averaged. xs/ys hold

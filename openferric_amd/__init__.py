@@ -33,7 +33,11 @@ from .dsl import (AssetMarketData, CompiledProduct, CrossGreeks, DslMonteCarloEn
                   ExtendedGreeks, MultiAssetMarket, parse_and_compile)
 from ._lib import DeviceError
 
+from .greeks import (FiniteDifferenceGreeks, McPricer, bump_and_reprice, bump_and_reprice_raw,
+                     fd_greeks_raw, finite_difference_greeks)
 __all__ = [
+    "FiniteDifferenceGreeks", "McPricer", "bump_and_reprice", "bump_and_reprice_raw",
+    "fd_greeks_raw", "finite_difference_greeks",
     "AsianOption", "AsianSpec", "Averaging", "BarrierDirection", "BarrierOption", "BarrierSpec",
     "BarrierStyle", "ConvergenceFailure", "Diagnostics", "ExerciseStyle", "Gbm", "Greeks",
     "Heston", "InvalidInput", "Market", "MarketDataMissing", "NumericalError", "OptionType",

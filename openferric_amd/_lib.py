@@ -40,7 +40,8 @@ EXPORTED_SYMBOLS = [
     "ofg_mc_greeks_estimate", "ofg_lsm_price", "ofg_stream_jump", "ofg_lsm_last_timing",
     "ofg_qmc_price", "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation", "ofg_stress_correlation",
     "ofg_dsl_compile", "ofg_dsl_free", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
-    "ofg_dsl_price", "ofg_price_noalloc",
+    "ofg_dsl_price", "ofg_price_noalloc", "ofg_finite_difference_greeks",
+    "ofg_bump_and_reprice",
 ]
 
 
@@ -79,6 +80,15 @@ class OfgResult(C.Structure):
         ("num_paths", C.c_double), ("num_steps", C.c_double), ("vol", C.c_double),
         ("kernel_ms", C.c_double),
     ]
+
+
+class OfgFdBumps(C.Structure):
+    _fields_ = [("ds", C.c_double), ("dr", C.c_double), ("dv", C.c_double), ("dt", C.c_double)]
+
+
+class OfgFdGreeks(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("delta", "gamma", "vega", "theta", "rho", "vanna",
+                                          "volga", "kernel_ms")]
 
 
 class OfgMcGreeks(C.Structure):
@@ -231,6 +241,14 @@ def load() -> C.CDLL:
         L.ofg_price_greeks.restype = C.c_int32
         L.ofg_price_noalloc.argtypes = L.ofg_price.argtypes
         L.ofg_price_noalloc.restype = C.c_int32
+        L.ofg_finite_difference_greeks.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t,
+                                                   C.POINTER(OfgEngineCfg), C.POINTER(OfgFdBumps),
+                                                   C.POINTER(OfgFdGreeks)]
+        L.ofg_finite_difference_greeks.restype = C.c_int32
+        L.ofg_bump_and_reprice.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t,
+                                           C.POINTER(OfgEngineCfg), C.POINTER(OfgFdBumps),
+                                           C.POINTER(C.c_double)]
+        L.ofg_bump_and_reprice.restype = C.c_int32
         L.ofg_sample_payoffs.argtypes = [pv, C.POINTER(OfgOption), C.POINTER(OfgEngineCfg),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ofg_sample_payoffs.restype = C.c_int32

@@ -771,6 +771,142 @@ int32_t price_fd(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engin
     return OFG_OK;
 }
 
+// The GenericPricer revaluations of finite_difference_greeks and
+// bump_and_reprice (greeks/bsm.rs:130-212): `specs` holds, per option, the
+// (s, r, sigma, t) of each revaluation; all are priced in one batched launch
+// on the option's own seed. prices[i * m + j] receives revaluation j of
+// option i. Returns the device time.
+struct Reval {
+    double s, r, v, t;
+};
+
+double price_revaluations(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                          const ofg_engine_cfg *cfg, const std::vector<Reval> &specs, size_t m,
+                          std::vector<double> &prices) {
+    std::vector<ofg_option> all;
+    all.reserve(n * m);
+    std::vector<std::vector<double>> scaled;
+    scaled.reserve(n * m);
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = opts[i];
+        if (o.model != OFG_MODEL_GBM)
+            fail(OFG_INVALID_INPUT,
+                 "finite-difference Greeks take GBM options (the pricer's sigma is the vol)");
+        for (size_t j = 0; j < m; ++j) {
+            const Reval &rv = specs[i * m + j];
+            ofg_option b = o;
+            b.spot = rv.s;
+            b.rate = rv.r;
+            b.vol = rv.v;
+            b.expiry = rv.t;
+            if (o.payoff == OFG_PAYOFF_ASIAN && o.n_obs > 0 && o.obs_times && o.expiry > 0.0 &&
+                rv.t != o.expiry) {
+                scaled.emplace_back(o.obs_times, o.obs_times + o.n_obs);
+                for (double &x : scaled.back()) x = std::fmin(x * (rv.t / o.expiry), rv.t);
+                b.obs_times = scaled.back().data();
+            }
+            all.push_back(b);
+        }
+    }
+    ofg_engine_cfg c = *cfg;
+    c.greek_mode = OFG_GREEKS_NONE;
+    std::vector<ofg_result> res(all.size());
+    const int32_t rc = price_impl(ctx, all.data(), all.size(), &c, res.data());
+    if (rc != OFG_OK) fail(rc, g_last_error);
+    prices.resize(all.size());
+    double ms = 0.0;
+    for (size_t k = 0; k < all.size(); ++k) {
+        prices[k] = res[k].price;
+        ms = std::max(ms, res[k].kernel_ms);
+    }
+    return ms;
+}
+
+void check_bumps(const ofg_option *opts, const ofg_fd_bumps *bumps, size_t n) {
+    if (n && (!opts || !bumps)) fail(OFG_INVALID_INPUT, "null argument");
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_fd_bumps &b = bumps[i];
+        if (!(std::isfinite(b.ds) && std::isfinite(b.dr) && std::isfinite(b.dv) &&
+              std::isfinite(b.dt)))
+            fail(OFG_INVALID_INPUT, "finite-difference bumps must be finite");
+    }
+}
+
+// finite_difference_greeks (greeks/bsm.rs:165-212).
+int32_t fd_greeks_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
+                       const ofg_fd_bumps *bumps, ofg_fd_greeks *out) {
+    if (!ctx || !cfg || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
+    check_bumps(opts, bumps, n);
+    constexpr size_t M = 13;
+    std::vector<Reval> specs(n * M);
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = opts[i];
+        const ofg_fd_bumps &b = bumps[i];
+        const double s = o.spot, r = o.rate, v = o.vol, t = o.expiry;
+        const double s_dn = std::fmax(s - b.ds, 1e-8);
+        const double v_dn = std::fmax(v - b.dv, 1e-8);
+        Reval *x = &specs[i * M];
+        x[0] = {s, r, v, t};
+        x[1] = {s + b.ds, r, v, t};
+        x[2] = {s_dn, r, v, t};
+        x[3] = {s, r + b.dr, v, t};
+        x[4] = {s, r - b.dr, v, t};
+        x[5] = {s, r, v + b.dv, t};
+        x[6] = {s, r, v_dn, t};
+        x[7] = {s, r, v, t + b.dt};
+        x[8] = {s, r, v, std::fmax(t - b.dt, 1e-8)};
+        x[9] = {s + b.ds, r, v + b.dv, t};
+        x[10] = {s + b.ds, r, v_dn, t};
+        x[11] = {s_dn, r, v + b.dv, t};
+        x[12] = {s_dn, r, v_dn, t};
+    }
+    std::vector<double> p;
+    const double ms = price_revaluations(ctx, opts, n, cfg, specs, M, p);
+    for (size_t i = 0; i < n; ++i) {
+        const double *q = &p[i * M];
+        const ofg_fd_bumps &b = bumps[i];
+        const double p0 = q[0], p_s_up = q[1], p_s_dn = q[2], p_r_up = q[3], p_r_dn = q[4];
+        const double p_v_up = q[5], p_v_dn = q[6], p_t_up = q[7], p_t_dn = q[8];
+        const double p_uu = q[9], p_ud = q[10], p_du = q[11], p_dd = q[12];
+        ofg_fd_greeks &g = out[i];
+        g.delta = (p_s_up - p_s_dn) / (2.0 * b.ds);
+        g.gamma = (p_s_up - 2.0 * p0 + p_s_dn) / (b.ds * b.ds);
+        g.rho = (p_r_up - p_r_dn) / (2.0 * b.dr);
+        g.vega = (p_v_up - p_v_dn) / (2.0 * b.dv);
+        // market convention: theta is dV/d(calendar time), -dV/d(maturity)
+        g.theta = -(p_t_up - p_t_dn) / (2.0 * b.dt);
+        g.vanna = (p_uu - p_ud - p_du + p_dd) / (4.0 * b.ds * b.dv);
+        g.volga = (p_v_up - 2.0 * p0 + p_v_dn) / (b.dv * b.dv);
+        g.kernel_ms = ms;
+    }
+    return OFG_OK;
+}
+
+// bump_and_reprice (greeks/bsm.rs:130-163).
+int32_t bump_reprice_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                          const ofg_engine_cfg *cfg, const ofg_fd_bumps *bumps, double *out) {
+    if (!ctx || !cfg || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
+    check_bumps(opts, bumps, n);
+    constexpr size_t M = 6;
+    std::vector<Reval> specs(n * M);
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = opts[i];
+        const ofg_fd_bumps &b = bumps[i];
+        const double s = o.spot, r = o.rate, v = o.vol, t = o.expiry;
+        Reval *x = &specs[i * M];
+        x[0] = {s, r, v, t};
+        x[1] = {s + b.ds, r, v, t};
+        x[2] = {s, r + b.dr, v, t};
+        x[3] = {s, r, v + b.dv, t};
+        x[4] = {s, r, v, std::fmax(t + b.dt, 1e-8)};
+        x[5] = {s - b.ds, r, v, t};
+    }
+    std::vector<double> p;
+    price_revaluations(ctx, opts, n, cfg, specs, M, p);
+    std::copy(p.begin(), p.end(), out);
+    return OFG_OK;
+}
+
 // Pathwise adjoint Greeks: forward + backward sweeps per sample in one
 // persistent launch per model (ofg_adjoint.hip), then a segmented reduce.
 int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
@@ -1127,6 +1263,17 @@ void ofg_destroy(ofg_ctx *ctx) {
 int32_t ofg_price(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
                   ofg_result *out) {
     return guarded([&]() { return price_impl(ctx, opts, n, cfg, out); });
+}
+
+int32_t ofg_finite_difference_greeks(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                                     const ofg_engine_cfg *cfg, const ofg_fd_bumps *bumps,
+                                     ofg_fd_greeks *out) {
+    return guarded([&]() { return fd_greeks_impl(ctx, opts, n, cfg, bumps, out); });
+}
+
+int32_t ofg_bump_and_reprice(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                             const ofg_engine_cfg *cfg, const ofg_fd_bumps *bumps, double *out) {
+    return guarded([&]() { return bump_reprice_impl(ctx, opts, n, cfg, bumps, out); });
 }
 
 int32_t ofg_price_noalloc(ofg_ctx *ctx, const ofg_option *opts, size_t n,

@@ -1,4 +1,5 @@
-"""ctypes binding of the CPU oracle (liboracle.so).
+"""ctypes binding of the CPU oracle (liboracle.so), and the oracle's Python
+restatements of the finite-difference Greeks harness (src/greeks/bsm.rs).
 
 TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the timed
 CPU baseline. Only tests/, the smoke check and the benchmark's cpu_baseline
@@ -547,3 +548,80 @@ def basket_factor_price(b, loadings, one_factor, normals=None):
     if rc != 0:
         raise ValueError(f"or_basket_factor_price failed rc={rc}")
     return p.value, se.value
+
+
+# ---- finite_difference_greeks / bump_and_reprice (src/greeks/bsm.rs) ------
+# The reference's harness restated over a GenericPricer callable
+# price(s, k, r, sigma, t) (bsm.rs:44-55); Python evaluates each expression
+# left to right, as Rust does.
+FD_FIELDS = ("delta", "gamma", "vega", "theta", "rho", "vanna", "volga")
+
+
+def bump_and_reprice(pricer, s, k, r, sigma, t, bump_s, bump_r, bump_sigma, bump_t):
+    """bump_and_reprice, bsm.rs:130-163: (base, s_up, r_up, v_up, t_up,
+    s_dn); t_up at max(t + bump_t, 1e-8), s_dn not floored."""
+    base = pricer(s, k, r, sigma, t)
+    s_up = pricer(s + bump_s, k, r, sigma, t)
+    r_up = pricer(s, k, r + bump_r, sigma, t)
+    v_up = pricer(s, k, r, sigma + bump_sigma, t)
+    t_up = pricer(s, k, r, sigma, max(t + bump_t, 1e-8))
+    return (base, s_up, r_up, v_up, t_up, pricer(s - bump_s, k, r, sigma, t))
+
+
+def finite_difference_greeks(pricer, s, k, r, sigma, t, ds, dr, dv, dt):
+    """finite_difference_greeks, bsm.rs:165-212: dict of FD_FIELDS."""
+    s_dn = max(s - ds, 1e-8)
+    v_dn = max(sigma - dv, 1e-8)
+    p0 = pricer(s, k, r, sigma, t)
+    p_s_up = pricer(s + ds, k, r, sigma, t)
+    p_s_dn = pricer(s_dn, k, r, sigma, t)
+    p_r_up = pricer(s, k, r + dr, sigma, t)
+    p_r_dn = pricer(s, k, r - dr, sigma, t)
+    p_v_up = pricer(s, k, r, sigma + dv, t)
+    p_v_dn = pricer(s, k, r, v_dn, t)
+    p_t_up = pricer(s, k, r, sigma, t + dt)
+    p_t_dn = pricer(s, k, r, sigma, max(t - dt, 1e-8))
+    p_s_up_v_up = pricer(s + ds, k, r, sigma + dv, t)
+    p_s_up_v_dn = pricer(s + ds, k, r, v_dn, t)
+    p_s_dn_v_up = pricer(s_dn, k, r, sigma + dv, t)
+    p_s_dn_v_dn = pricer(s_dn, k, r, v_dn, t)
+    return dict(
+        delta=(p_s_up - p_s_dn) / (2.0 * ds),
+        gamma=(p_s_up - 2.0 * p0 + p_s_dn) / (ds * ds),
+        rho=(p_r_up - p_r_dn) / (2.0 * dr),
+        vega=(p_v_up - p_v_dn) / (2.0 * dv),
+        theta=-(p_t_up - p_t_dn) / (2.0 * dt),
+        vanna=(p_s_up_v_up - p_s_up_v_dn - p_s_dn_v_up + p_s_dn_v_dn) / (4.0 * ds * dv),
+        volga=(p_v_up - 2.0 * p0 + p_v_dn) / (dv * dv),
+    )
+
+
+def bsm_greeks_closed_form(call, s, k, r, q, sigma, t):
+    """black_scholes_merton_greeks (bsm.rs:63-128) with the exact normal
+    pdf/cdf: delta, gamma, vega, theta (-dV/dT), rho, vanna (d2V/dS dsigma)
+    and volga (d2V/dsigma2)."""
+    import math
+    if s <= 0.0 or k <= 0.0 or t <= 0.0 or sigma <= 0.0:
+        return dict.fromkeys(FD_FIELDS, 0.0)
+    pdf = lambda x: math.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+    cdf = lambda x: 0.5 * math.erfc(-x / math.sqrt(2.0))
+    vt = sigma * math.sqrt(t)
+    d1 = (math.log(s / k) + (r - q + 0.5 * sigma * sigma) * t) / vt
+    d2 = d1 - vt
+    sqrt_t = math.sqrt(t)
+    df_q, df_r = math.exp(-q * t), math.exp(-r * t)
+    delta = df_q * cdf(d1) if call else df_q * (cdf(d1) - 1.0)
+    gamma = df_q * pdf(d1) / (s * sigma * sqrt_t)
+    vega = s * df_q * pdf(d1) * sqrt_t
+    if call:
+        theta = (-s * df_q * pdf(d1) * sigma / (2.0 * sqrt_t) - r * k * df_r * cdf(d2)
+                 + q * s * df_q * cdf(d1))
+        rho = k * t * df_r * cdf(d2)
+    else:
+        theta = (-s * df_q * pdf(d1) * sigma / (2.0 * sqrt_t) + r * k * df_r * cdf(-d2)
+                 - q * s * df_q * cdf(-d1))
+        rho = -k * t * df_r * cdf(-d2)
+    vanna = -df_q * pdf(d1) * d2 / sigma
+    volga = vega * d1 * d2 / sigma
+    return dict(delta=delta, gamma=gamma, vega=vega, theta=theta, rho=rho, vanna=vanna,
+                volga=volga)
