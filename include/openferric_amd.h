@@ -372,6 +372,35 @@ int32_t ofg_finite_difference_greeks(ofg_ctx *ctx, const ofg_option *opts, size_
 int32_t ofg_bump_and_reprice(ofg_ctx *ctx, const ofg_option *opts, size_t n,
                              const ofg_engine_cfg *cfg, const ofg_fd_bumps *bumps, double *out);
 
+/* The exact-terminal European pricers of the reference's default build
+ * (Cargo feature "simd" off). */
+enum ofg_exact_kind {
+    OFG_EXACT_SEQUENTIAL = 0,   /* mc_european_sequential (mc_parallel.rs:365-434) */
+    OFG_EXACT_PARALLEL = 1,     /* mc_european_parallel (mc_parallel.rs:264-362) */
+    OFG_EXACT_ARENA = 2         /* mc_european_with_arena (mc_engine.rs:186-343) */
+};
+
+/* Replaces mc_european_sequential, mc_european_parallel and
+ * mc_european_with_arena (src/engines/monte_carlo/mc_parallel.rs:98-148,
+ * 264-434; mc_engine.rs:186-343) on the reference's own draws: one standard
+ * normal per path, drawn in path order from a single xoshiro256++ stream --
+ * seed 0xDEAD_BEEF_CAFE_BABE (sequential), 42 (arena), or per chunk of
+ * split_paths(num_paths, n_chunks) seed 0xDEAD_BEEF_CAFE_BABE + i *
+ * 6364136223846793005 (parallel; n_chunks is the reference run's
+ * rayon::current_num_threads()) -- split across GPU threads by jump-ahead.
+ * opts[i]: European GBM vanillas; spot, strike, expiry, rate, vol,
+ * num_paths; div_yield = the market's effective dividend yield (sequential,
+ * parallel) or dividend_yield (arena); num_steps and seed are ignored (the
+ * reference ignores n_steps and fixes the seeds). As the reference, degenerate
+ * inputs give results, not errors: zero paths, a non-positive or non-finite
+ * vol, or (arena) a strike <= 0 or expiry < 0 give price NaN and
+ * has_stderr 0; expiry <= 0 gives the payoff at the spot with stderr 0.
+ * Diagnostics num_paths, num_steps = 1, vol. payoffs (may be NULL): every
+ * path's undiscounted payoff, options concatenated (num_paths each, in path
+ * order). out[n]. */
+int32_t ofg_european_exact(ofg_ctx *ctx, const ofg_option *opts, size_t n, int32_t kind,
+                           int32_t n_chunks, ofg_result *out, double *payoffs);
+
 /* Per-sample payoffs of ONE option: x[i] (and control variate y[i]) exactly
  * as the fold body of MonteCarloEngine::run computes them
  * (simulation.rs:212-252), antithetic legs averaged. xs/ys hold

@@ -23,6 +23,7 @@
 
 #include "../openferric_amd.h"
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <memory>
@@ -31,6 +32,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <thread>
 #include <vector>
 
 namespace openferric_amd {
@@ -468,48 +470,76 @@ private:
     RngMode rng_ = RngMode::Reference;
 };
 
-// mc_european_parallel / mc_european_sequential (mc_parallel.rs:264-434):
-// exact terminal sampling, one normal per path, fast draws; invalid inputs
-// give a NaN price (the reference returns, it does not fail).
-inline PricingResult mc_european_exact(const VanillaOption &i, const Market &m, int64_t n_paths) {
+// mc_european_parallel / mc_european_sequential (mc_parallel.rs:264-434) and
+// mc_european_with_arena (mc_engine.rs:186-343): exact terminal sampling on
+// the reference's own single streams (ofg_european_exact), one normal per
+// path; invalid inputs give a NaN price (the reference returns, it does not
+// fail). The early returns follow each function's own order.
+namespace detail {
+inline PricingResult exact_intrinsic(const VanillaOption &i, const Market &m) {
+    PricingResult r;
+    r.price = i.option_type == OptionType::Call ? std::fmax(m.spot - i.strike, 0.0)
+                                                : std::fmax(i.strike - m.spot, 0.0);
+    r.stderr_ = 0.0;
+    return r;
+}
+inline PricingResult exact_simulate(const VanillaOption &i, const Market &m, int64_t n_paths,
+                                    int32_t kind, int32_t n_chunks) {
+    Market eff = m;
+    if (kind != OFG_EXACT_ARENA) eff.dividend_yield = m.effective_dividend_yield(i.expiry);
+    ofg_option o = detail::to_ofg(i, eff, n_paths, 1, 0);
+    ofg_result res{};
+    check(ofg_european_exact(device_context(), &o, 1, kind, n_chunks, &res, nullptr));
+    PricingResult r;
+    r.price = res.price;
+    if (!res.has_stderr) return r;
+    r.stderr_ = res.stderr_;
+    r.diagnostics.insert("num_paths", (double)n_paths);
+    r.diagnostics.insert("num_steps", 1.0);
+    if (kind == OFG_EXACT_PARALLEL) r.diagnostics.insert("num_threads", (double)n_chunks);
+    r.diagnostics.insert("vol", m.vol_for(i.strike, i.expiry));
+    r.kernel_ms = res.kernel_ms;
+    return r;
+}
+}  // namespace detail
+
+// n_chunks: the reference run's rayon::current_num_threads() (default: the
+// host's hardware threads, rayon's default pool)
+inline PricingResult mc_european_parallel(const VanillaOption &i, const Market &m, int64_t n_paths,
+                                          int64_t /*n_steps*/ = 1, int32_t n_chunks = 0) {
     PricingResult r;
     if (n_paths == 0 || i.exercise != ExerciseStyle::European) {
         r.price = std::nan("");
         return r;
     }
-    if (i.expiry <= 0.0) {
-        r.price = i.option_type == OptionType::Call ? std::fmax(m.spot - i.strike, 0.0)
-                                                    : std::fmax(i.strike - m.spot, 0.0);
-        r.stderr_ = 0.0;
-        return r;
-    }
-    const double vol = m.vol_for(i.strike, i.expiry);
-    if (!(vol > 0.0) || !std::isfinite(vol)) {
-        r.price = std::nan("");
-        return r;
-    }
-    Market eff = m;
-    eff.dividend_yield = m.effective_dividend_yield(i.expiry);
-    ofg_option o = detail::to_ofg(i, eff, n_paths, 1, 0xDEADBEEFCAFEBABEull);
-    ofg_engine_cfg c{};
-    c.rng_mode = OFG_RNG_FAST;
-    ofg_result res{};
-    check(ofg_price(device_context(), &o, 1, &c, &res));
-    r.price = res.price;
-    r.stderr_ = res.stderr_;
-    r.diagnostics.insert("num_paths", (double)n_paths);
-    r.diagnostics.insert("num_steps", 1.0);
-    r.diagnostics.insert("vol", vol);
-    r.kernel_ms = res.kernel_ms;
-    return r;
-}
-inline PricingResult mc_european_parallel(const VanillaOption &i, const Market &m, int64_t n_paths,
-                                          int64_t /*n_steps*/ = 1) {
-    return mc_european_exact(i, m, n_paths);
+    if (i.expiry <= 0.0) return detail::exact_intrinsic(i, m);
+    if (n_chunks <= 0) n_chunks = (int32_t)std::max(1u, std::thread::hardware_concurrency());
+    return detail::exact_simulate(i, m, n_paths, OFG_EXACT_PARALLEL, n_chunks);
 }
 inline PricingResult mc_european_sequential(const VanillaOption &i, const Market &m,
                                             int64_t n_paths, int64_t /*n_steps*/ = 1) {
-    return mc_european_exact(i, m, n_paths);
+    PricingResult r;
+    if (n_paths == 0) {
+        r.price = std::nan("");
+        return r;
+    }
+    if (i.expiry <= 0.0) return detail::exact_intrinsic(i, m);
+    if (i.exercise != ExerciseStyle::European) {
+        r.price = std::nan("");
+        return r;
+    }
+    return detail::exact_simulate(i, m, n_paths, OFG_EXACT_SEQUENTIAL, 1);
+}
+inline PricingResult mc_european_with_arena(const VanillaOption &i, const Market &m,
+                                            int64_t n_paths, int64_t /*n_steps*/ = 1) {
+    PricingResult r;
+    if (n_paths == 0 || !(i.strike > 0.0) || i.expiry < 0.0 ||
+        i.exercise != ExerciseStyle::European) {
+        r.price = std::nan("");
+        return r;
+    }
+    if (i.expiry <= 0.0) return detail::exact_intrinsic(i, m);
+    return detail::exact_simulate(i, m, n_paths, OFG_EXACT_ARENA, 1);
 }
 
 // mc_european_qmc_with_seed / mc_european_qmc (mc_qmc.rs:29-144): the

@@ -17,7 +17,8 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       mc_european_with_arena, LongstaffSchwartzEngine, LsmDynamics,
                       lsm_last_timing, lsm_option_struct, lsm_price_raw, stream_jump,
                       mc_european_qmc, mc_european_qmc_with_seed, qmc_price_raw,
-                      option_struct, price_noalloc_raw, price_raw, sample_payoffs)
+                      option_struct, price_noalloc_raw, price_raw, sample_payoffs,
+                      european_exact_raw)
 from .multi import (Autocallable, AutocallableSensitivities, BasketCopula, BasketOption,
                     FactorCorrelationModel, price_basket_mc_with_factor_model,
                     price_basket_mc_with_copula, CorrelationStressScenario,
@@ -44,7 +45,7 @@ __all__ = [
     "PricingError", "PricingResult", "StrikeType", "VanillaOption", "BatchPricer",
     "GpuArithmeticAsianMC", "GpuMonteCarloPricingEngine", "HestonAadConfig",
     "HestonMonteCarloPricingEngine", "RngMode", "VarianceReduction", "autotune", "engine_cfg",
-    "heston_price_delta_aad", "mc_european_pathwise_aad", "option_struct", "price_raw", "price_noalloc_raw",
+    "heston_price_delta_aad", "mc_european_pathwise_aad", "option_struct", "price_raw", "price_noalloc_raw", "european_exact_raw",
     "sample_payoffs", "DeviceError", "MonteCarloGreeksEngine", "mc_european_parallel",
     "mc_european_sequential", "mc_european_with_arena", "GreekMethod", "BermudanOption",
     "BermudanLsmOutput", "ExerciseBoundaryPoint", "LongstaffSchwartzEngine", "LsmDynamics",

@@ -28,6 +28,7 @@ UP, DOWN = 0, 1
 IN, OUT = 0, 1
 RNG_REFERENCE, RNG_EXTERNAL, RNG_FAST = 0, 1, 2
 GREEKS_NONE, GREEKS_ADJOINT, GREEKS_FD_CRN, GREEKS_FORWARD = 0, 1, 2, 3
+EXACT_SEQUENTIAL, EXACT_PARALLEL, EXACT_ARENA = 0, 1, 2
 LSM_VANILLA, LSM_BERMUDAN, LSM_BARRIER = 0, 1, 2
 EUROPEAN, AMERICAN, BERMUDAN = 0, 1, 2
 LSM_GBM, LSM_LOCAL_VOL, LSM_HESTON = 0, 1, 2
@@ -41,7 +42,7 @@ EXPORTED_SYMBOLS = [
     "ofg_qmc_price", "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation", "ofg_stress_correlation",
     "ofg_dsl_compile", "ofg_dsl_free", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
     "ofg_dsl_price", "ofg_price_noalloc", "ofg_finite_difference_greeks",
-    "ofg_bump_and_reprice",
+    "ofg_bump_and_reprice", "ofg_european_exact",
 ]
 
 
@@ -249,6 +250,9 @@ def load() -> C.CDLL:
                                            C.POINTER(OfgEngineCfg), C.POINTER(OfgFdBumps),
                                            C.POINTER(C.c_double)]
         L.ofg_bump_and_reprice.restype = C.c_int32
+        L.ofg_european_exact.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t, C.c_int32,
+                                         C.c_int32, C.POINTER(OfgResult), C.POINTER(C.c_double)]
+        L.ofg_european_exact.restype = C.c_int32
         L.ofg_sample_payoffs.argtypes = [pv, C.POINTER(OfgOption), C.POINTER(OfgEngineCfg),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ofg_sample_payoffs.restype = C.c_int32

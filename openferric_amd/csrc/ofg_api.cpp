@@ -1249,6 +1249,7 @@ void ofg_destroy(ofg_ctx *ctx) {
     multi_release(ctx);
     basket_release(ctx);
     dsl_release(ctx);
+    exact_release(ctx);
     for (DevBuf *b : {&ctx->opts, &ctx->obs_w, &ctx->items, &ctx->order, &ctx->item_first,
                       &ctx->item_count, &ctx->partials, &ctx->outs, &ctx->head, &ctx->xs, &ctx->ys,
                       &ctx->n1, &ctx->n2, &ctx->adj})

@@ -97,6 +97,7 @@ struct QmcBuffers;   // ofg_qmc_api.cpp
 struct MultiBuffers; // ofg_multi_api.cpp
 struct BasketBuffers; // ofg_basket_api.cpp
 struct DslBuffers;    // ofg_dsl_api.cpp
+struct ExactBuffers;  // ofg_exact_api.cpp
 
 // Instrument validation with the reference's messages (instruments/*.rs
 // validate()); throws OfgError.
@@ -128,6 +129,7 @@ struct ofg_ctx {
     ofg_host::MultiBuffers *multi = nullptr;   // multi-asset scenarios and partials
     ofg_host::BasketBuffers *basket = nullptr; // basket scenarios, seeds and partials
     ofg_host::DslBuffers *dsl = nullptr;       // structured-product programs and snapshots
+    ofg_host::ExactBuffers *exact = nullptr;   // exact-terminal pricers' streams and sums
     void *noalloc = nullptr;   // this GPU's fixed region of ofg_price_noalloc (not owned)
     int cus = 256;
 };
@@ -143,4 +145,6 @@ void multi_release(ofg_ctx *ctx);
 void basket_release(ofg_ctx *ctx);
 // Releases the context's structured-product buffers (ofg_dsl_api.cpp).
 void dsl_release(ofg_ctx *ctx);
+// Releases the context's exact-terminal pricer buffers (ofg_exact_api.cpp).
+void exact_release(ofg_ctx *ctx);
 }  // namespace ofg_host

@@ -493,27 +493,6 @@ __global__ __launch_bounds__(LSM_BLOCK) void lsm_round_kernel(LsmArgs a) {
 // Reference-stream start states
 // ------------------------------------------------------------------------
 
-// state <- r(T) state for the polynomial r (bit i = coefficient of x^i):
-// sum over set bits of T^i state, T the xoshiro256 transition.
-__device__ __forceinline__ void apply_poly(const uint64_t *r, Xoshiro256pp &st) {
-    Xoshiro256pp acc;
-    acc.s0 = acc.s1 = acc.s2 = acc.s3 = 0;
-    Xoshiro256pp cur = st;
-    for (int w = 0; w < 4; ++w) {
-        const uint64_t word = r[w];
-        for (int b = 0; b < 64; ++b) {
-            if ((word >> b) & 1u) {
-                acc.s0 ^= cur.s0;
-                acc.s1 ^= cur.s1;
-                acc.s2 ^= cur.s2;
-                acc.s3 ^= cur.s3;
-            }
-            (void)cur.next_u64();
-        }
-    }
-    st = acc;
-}
-
 __global__ __launch_bounds__(256) void lsm_seed_kernel(const LsmSeedJob *__restrict__ jobs,
                                                        int32_t n_jobs, int64_t total,
                                                        const uint64_t *__restrict__ polys,
@@ -536,7 +515,7 @@ __global__ __launch_bounds__(256) void lsm_seed_kernel(const LsmSeedJob *__restr
     st.s2 = j.base[2];
     st.s3 = j.base[3];
     for (int k = 0; k < j.n_polys; ++k)
-        if ((g >> k) & 1) apply_poly(polys + 4 * (j.poly_offset + k), st);
+        if ((g >> k) & 1) xoshiro_apply_poly(polys + 4 * (j.poly_offset + k), st);
     uint64_t *out = seeds + j.seed_offset;
     for (int q = 0; q < j.chunk; ++q) {
         const int64_t p = first + q;

@@ -272,6 +272,29 @@ __device__ __forceinline__ void reference_steps(RNG &rng, int n_steps, F &&f) {
     }
 }
 
+// state <- r(T) state for a GF(2) polynomial r (bit i = coefficient of x^i,
+// four words): the XOR over the set bits i of T^i state, T the xoshiro256
+// transition. With r = x^k mod P(x), P the transition's characteristic
+// polynomial, this jumps the stream k draws ahead.
+OFG_HD void xoshiro_apply_poly(const uint64_t *r, Xoshiro256pp &st) {
+    Xoshiro256pp acc;
+    acc.s0 = acc.s1 = acc.s2 = acc.s3 = 0;
+    Xoshiro256pp cur = st;
+    for (int w = 0; w < 4; ++w) {
+        const uint64_t word = r[w];
+        for (int b = 0; b < 64; ++b) {
+            if ((word >> b) & 1u) {
+                acc.s0 ^= cur.s0;
+                acc.s1 ^= cur.s1;
+                acc.s2 ^= cur.s2;
+                acc.s3 ^= cur.s3;
+            }
+            (void)cur.next_u64();
+        }
+    }
+    st = acc;
+}
+
 // sample_standard_normal (fast_rng.rs:223-226) on a xoshiro stream.
 OFG_HD double reference_normal(Xoshiro256pp &rng) {
     return acklam_inv_cdf(uniform_open01(rng.next_f64()));

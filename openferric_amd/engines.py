@@ -436,31 +436,72 @@ class MonteCarloGreeksEngine:
 _NAN_RESULT = float("nan")
 
 
-def _mc_european_exact(instrument: VanillaOption, market: Market, n_paths: int,
-                       seed: int = 0xDEADBEEFCAFEBABE, effective_q: bool = True) -> PricingResult:
-    """mc_european_parallel / mc_european_sequential (mc_parallel.rs:264-434)
-    and mc_european_with_arena (mc_engine.rs:186-343): the terminal spot
-    sampled exactly (one step over T, one normal per path, no variance
-    reduction); invalid inputs give a NaN price, not an error."""
-    if n_paths == 0 or instrument.exercise is not ExerciseStyle.European:
-        return PricingResult(_NAN_RESULT, None)
-    if instrument.expiry <= 0.0:
-        intrinsic = max(market.spot - instrument.strike, 0.0) \
-            if instrument.option_type is OptionType.Call else max(instrument.strike - market.spot, 0.0)
-        return PricingResult(intrinsic, 0.0)
+def european_exact_raw(opts: Sequence[_lib.OfgOption], kind: int, n_chunks: int = 1,
+                       want_payoffs: bool = False, device: int = 0):
+    """ofg_european_exact: the reference's exact-terminal European pricers on
+    its own single streams. Returns the results (and, with want_payoffs,
+    every path's payoff, options concatenated)."""
+    L = _lib.load()
+    ctx = _lib.context(device)
+    arr = (_lib.OfgOption * len(opts))(*opts)
+    out = (_lib.OfgResult * len(opts))()
+    pay = np.full(sum(int(o.num_paths) for o in opts), np.nan) if want_payoffs else None
+    _raise(L.ofg_european_exact(ctx, arr, len(opts), kind, n_chunks, out,
+                                pay.ctypes.data_as(C.POINTER(C.c_double)) if want_payoffs
+                                else None))
+    return (list(out), pay) if want_payoffs else list(out)
+
+
+def _nan_result() -> PricingResult:
+    return PricingResult(_NAN_RESULT, None)
+
+
+def _reference_threads() -> int:
+    """rayon::current_num_threads() of a default pool: the available
+    parallelism of this process."""
+    import os
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        return max(1, os.cpu_count() or 1)
+
+
+def _exact(instrument: VanillaOption, market: Market, n_paths: int, kind: int, n_chunks: int,
+           rng_mode: "RngMode", device: int) -> PricingResult:
+    """The simulation of the three exact-terminal pricers once their early
+    returns are past: expiry > 0 and a European instrument."""
     vol = market.vol_for(instrument.strike, instrument.expiry)
-    if not (vol > 0.0) or vol == float("inf"):
-        return PricingResult(_NAN_RESULT, None)
     keep = _Keep()
-    q = market.effective_dividend_yield(instrument.expiry) if effective_q else market.dividend_yield
+    # mu: rate - effective dividend yield (mc_parallel.rs:317, 397), rate -
+    # dividend_yield for the arena (mc_engine.rs:230)
+    q = market.dividend_yield if kind == _lib.EXACT_ARENA else \
+        market.effective_dividend_yield(instrument.expiry)
     eff = Market(market.spot, market.rate, q, market.flat_vol)
+    seed = 42 if kind == _lib.EXACT_ARENA else 0xDEADBEEFCAFEBABE
     o = option_struct(instrument, eff, n_paths, 1, seed, keep)
-    r = price_raw([o], engine_cfg(rng_mode=RngMode.Fast, keep=keep))[0]
+    if rng_mode is RngMode.Reference:
+        r = european_exact_raw([o], kind, n_chunks, device=device)[0]
+    else:
+        # throughput streams: per-path generators instead of the single stream
+        if not (vol > 0.0) or vol == float("inf"):
+            return _nan_result()
+        r = price_raw([o], engine_cfg(rng_mode=rng_mode, keep=keep), device)[0]
+    if not r.has_stderr:
+        return _nan_result()
     diag = Diagnostics()
     diag.insert("num_paths", float(n_paths))
     diag.insert("num_steps", 1.0)
+    if kind == _lib.EXACT_PARALLEL:
+        diag.insert("num_threads", float(n_chunks))
     diag.insert("vol", vol)
     return PricingResult(r.price, r.stderr_, diagnostics=diag, kernel_ms=r.kernel_ms)
+
+
+def _intrinsic(instrument: VanillaOption, market: Market) -> PricingResult:
+    call = instrument.option_type is OptionType.Call
+    v = max(market.spot - instrument.strike, 0.0) if call else \
+        max(instrument.strike - market.spot, 0.0)
+    return PricingResult(v, 0.0)
 
 
 def qmc_price_raw(opts: Sequence[_lib.OfgOption], device: int = 0) -> List[_lib.OfgResult]:
@@ -504,33 +545,57 @@ def mc_european_qmc(instrument: VanillaOption, market: Market, n_paths: int,
 
 
 def mc_european_parallel(instrument: VanillaOption, market: Market, n_paths: int,
-                         n_steps: int = 1) -> PricingResult:
-    """GPU counterpart of mc_european_parallel (mc_parallel.rs:264-362).
-    n_steps is ignored, as in the reference. The reference's draws depend on
-    its thread count and SIMD path, so parity is statistical; the reference's
-    `num_threads` diagnostic has no GPU counterpart and is not set."""
-    return _mc_european_exact(instrument, market, n_paths)
+                         n_steps: int = 1, n_chunks: Optional[int] = None,
+                         rng_mode: "RngMode" = None, device: int = 0) -> PricingResult:
+    """mc_european_parallel (mc_parallel.rs:264-362) on the GPU. n_steps is
+    ignored, as in the reference. The reference splits the paths into
+    n_chunks = rayon::current_num_threads() chunks, each its own stream
+    (default: this process's available parallelism, as rayon's default pool);
+    with the reference's thread count the draws are the reference's, path by
+    path. rng_mode=RngMode.Fast uses per-path throughput streams instead."""
+    rng_mode = RngMode.Reference if rng_mode is None else rng_mode
+    if n_paths == 0 or instrument.exercise is not ExerciseStyle.European:
+        return _nan_result()
+    if instrument.expiry <= 0.0:
+        return _intrinsic(instrument, market)
+    chunks = _reference_threads() if n_chunks is None else int(n_chunks)
+    return _exact(instrument, market, n_paths, _lib.EXACT_PARALLEL, chunks, rng_mode, device)
 
 
 def mc_european_sequential(instrument: VanillaOption, market: Market, n_paths: int,
-                           n_steps: int = 1) -> PricingResult:
-    """GPU counterpart of mc_european_sequential (mc_parallel.rs:365-434)."""
-    return _mc_european_exact(instrument, market, n_paths)
+                           n_steps: int = 1, rng_mode: "RngMode" = None,
+                           device: int = 0) -> PricingResult:
+    """mc_european_sequential (mc_parallel.rs:365-434) on the GPU: the
+    reference's single stream (seed 0xDEAD_BEEF_CAFE_BABE), path by path."""
+    rng_mode = RngMode.Reference if rng_mode is None else rng_mode
+    if n_paths == 0:
+        return _nan_result()
+    if instrument.expiry <= 0.0:
+        return _intrinsic(instrument, market)
+    if instrument.exercise is not ExerciseStyle.European:
+        return _nan_result()
+    return _exact(instrument, market, n_paths, _lib.EXACT_SEQUENTIAL, 1, rng_mode, device)
 
 
 def mc_european_with_arena(instrument: VanillaOption, market: Market, n_paths: int,
-                           n_steps: int = 1, arena=None) -> PricingResult:
-    """GPU counterpart of mc_european_with_arena (mc_engine.rs:186-343):
-    exact terminal sampling with mu = r - q and seed 42. The arena (a host
-    payoff buffer in the reference) is accepted and unused: payoffs are
-    reduced on the device. An instrument that fails validation gives NaN."""
+                           n_steps: int = 1, arena=None, rng_mode: "RngMode" = None,
+                           device: int = 0) -> PricingResult:
+    """mc_european_with_arena (mc_engine.rs:186-343) on the GPU: the
+    reference's single stream (seed 42), path by path, mu = r - q. The arena
+    (a host payoff buffer in the reference) is accepted and unused: payoffs
+    are reduced on the device. An instrument that fails validation gives NaN."""
+    rng_mode = RngMode.Reference if rng_mode is None else rng_mode
+    if n_paths == 0:
+        return _nan_result()
     try:
         instrument.validate()
     except InvalidInput:
-        return PricingResult(_NAN_RESULT, None)
-    if n_paths == 0:
-        return PricingResult(_NAN_RESULT, None)
-    return _mc_european_exact(instrument, market, n_paths, seed=42, effective_q=False)
+        return _nan_result()
+    if instrument.exercise is not ExerciseStyle.European:
+        return _nan_result()
+    if instrument.expiry <= 0.0:
+        return _intrinsic(instrument, market)
+    return _exact(instrument, market, n_paths, _lib.EXACT_ARENA, 1, rng_mode, device)
 
 
 class BatchPricer:

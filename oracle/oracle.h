@@ -156,6 +156,13 @@ int or_mc_european_pathwise_aad(const or_option *o, int64_t num_paths,
 int or_heston_price_delta_aad(const or_option *o, int64_t num_paths,
                               int64_t num_steps, uint64_t seed, double out[2]);
 
+/* Exact-terminal European pricers (oracle_exact.c): kind 0
+ * mc_european_sequential, 1 mc_european_parallel with n_chunks chunks,
+ * 2 mc_european_with_arena. out: price, stderr, has_stderr. */
+int or_mc_european_exact(int32_t kind, int32_t type, double spot, double strike, double rate,
+                         double q, double vol, double expiry, int64_t n_paths, int32_t n_chunks,
+                         double out[3], double *payoffs);
+
 /* Pathwise sensitivities of MonteCarloEngine::run's estimator for every
  * payoff and both models, by forward tangents (oracle_pathwise.c). out:
  * price, stderr, then d price / d {S0, sigma | v0, r, T, kappa, theta, xi,

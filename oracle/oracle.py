@@ -155,6 +155,8 @@ def lib():
         L.or_mc_european_pathwise_aad.restype = C.c_int
         L.or_heston_price_delta_aad.argtypes = [C.POINTER(OrOption), i64, i64, u64, pd]
         L.or_heston_price_delta_aad.restype = C.c_int
+        L.or_mc_european_exact.argtypes = [i32, i32, d, d, d, d, d, d, i64, i32, pd, pd]
+        L.or_mc_european_exact.restype = C.c_int
         L.or_pathwise_greeks.argtypes = [C.POINTER(OrOption), i64, i64, u64, i32, pd, pd, i32,
                                          i32, pd, pd]
         L.or_pathwise_greeks.restype = C.c_int
@@ -303,6 +305,23 @@ def heston_price_delta_aad(o, num_paths, num_steps, seed):
     if rc:
         raise ValueError("heston aad failed")
     return float(out[0]), float(out[1])
+
+
+EXACT_SEQUENTIAL, EXACT_PARALLEL, EXACT_ARENA = 0, 1, 2
+
+
+def mc_european_exact(kind, option_type, spot, strike, rate, q, vol, expiry, n_paths,
+                      n_chunks=1, want_payoffs=False):
+    """mc_european_sequential / _parallel / _with_arena of the reference's
+    default build (oracle_exact.c). Returns (price, stderr or None[, payoffs])."""
+    out = np.zeros(3)
+    pay = np.full(n_paths, np.nan) if want_payoffs else None
+    rc = lib().or_mc_european_exact(kind, option_type, spot, strike, rate, q, vol, expiry,
+                                    n_paths, n_chunks, _ptr(out), _ptr(pay))
+    if rc:
+        raise ValueError(f"or_mc_european_exact failed rc={rc}")
+    se = float(out[1]) if out[2] else None
+    return (float(out[0]), se, pay) if want_payoffs else (float(out[0]), se)
 
 
 PATHWISE_SLOTS = ["s0", "sigma_or_v0", "r", "T", "kappa", "theta", "xi", "rho"]
