@@ -51,7 +51,14 @@ enum ofg_rng_mode {
     /* Throughput mode: per-sample xoroshiro128+ streams (same SplitMix64
        seeding) feeding a branch-free, table-driven Box-Muller transform; path
        exponentials as 2^(y/256). Statistically equivalent draws. */
-    OFG_RNG_FAST = 2
+    OFG_RNG_FAST = 2,
+    /* The reference's Pcg64 kind (FastRngKind::Pcg64, fast_rng.rs:93-137):
+       per-sample PCG XSL-RR 128/64 streams seeded from SplitMix64 of
+       seed + 7919*i (one draw discarded), Acklam inverse CDF as REFERENCE.
+       Per-path parity with the reference's Pcg64 engines. Taken by
+       ofg_price (greek modes NONE and FD_CRN), ofg_mc_greeks_estimate and
+       ofg_spread_price. */
+    OFG_RNG_PCG64 = 3
 };
 
 /* How ofg_price fills the Greeks.
@@ -400,6 +407,31 @@ enum ofg_exact_kind {
  * order). out[n]. */
 int32_t ofg_european_exact(ofg_ctx *ctx, const ofg_option *opts, size_t n, int32_t kind,
                            int32_t n_chunks, ofg_result *out, double *payoffs);
+
+/* SpreadOption (src/instruments/spread.rs:15-26) with the path count and
+ * seed of the SpreadMonteCarloEngine pricing it (spread_mc.rs:21-33). */
+typedef struct ofg_spread_option {
+    double s1, s2, k, vol1, vol2, rho, q1, q2, r, t;
+    int64_t num_paths;
+    uint64_t seed;
+} ofg_spread_option;
+
+/* Replaces SpreadMonteCarloEngine::price (src/engines/monte_carlo/
+ * spread_mc.rs:83-175) with reproducible streams: sample i draws z1 then z2
+ * from its own stream seeded with seed + 7919 i -- xoshiro256++
+ * (OFG_RNG_REFERENCE, the reference's default kind) or Pcg64
+ * (OFG_RNG_PCG64) per path as the reference, or the fast generator
+ * (OFG_RNG_FAST); antithetic averages each sample with (-z1, -z2) over
+ * ceil(num_paths / 2) samples. Errors, in the reference's order per option:
+ * SpreadOption::validate's messages, then "spread MC num_paths must be > 0";
+ * any failing option fails the call. t <= 0 gives max(s1 - s2 - k, 0) with
+ * stderr 0 and num_paths 0 (no diagnostics). Otherwise price and stderr
+ * (discounted sample mean, n - 1 variance over the samples, unclamped) and
+ * the num_paths diagnostic. payoffs (may be NULL): each simulated option's
+ * sample payoffs, options concatenated, samples slots per option in call
+ * order. out[n]. */
+int32_t ofg_spread_price(ofg_ctx *ctx, const ofg_spread_option *opts, size_t n,
+                         int32_t antithetic, int32_t rng_mode, ofg_result *out, double *payoffs);
 
 /* Per-sample payoffs of ONE option: x[i] (and control variate y[i]) exactly
  * as the fold body of MonteCarloEngine::run computes them

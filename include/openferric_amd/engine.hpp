@@ -29,6 +29,7 @@
 #include <memory>
 #include <mutex>
 #include <optional>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -226,7 +227,25 @@ enum class VarianceReduction { None, Antithetic, ControlVariate };
 // the same pathwise estimator in one forward sweep (GBM; Heston uses the
 // adjoint).
 enum class GreekMethod { Adjoint = OFG_GREEKS_ADJOINT, Forward = OFG_GREEKS_FORWARD };
-enum class RngMode { Reference = OFG_RNG_REFERENCE, Fast = OFG_RNG_FAST };
+enum class RngMode { Reference = OFG_RNG_REFERENCE, Fast = OFG_RNG_FAST, Pcg64 = OFG_RNG_PCG64 };
+// FastRngKind (fast_rng.rs:20-27). Xoshiro256PlusPlus and Pcg64 draw the
+// reference's per-path streams; ThreadRng is served by xoshiro256++ streams on
+// a fresh random base seed (non-reproducible, as there); StdRng (rand's
+// ChaCha12) has no GPU stream and is refused.
+enum class FastRngKind { Xoshiro256PlusPlus, Pcg64, ThreadRng, StdRng };
+
+namespace detail {
+inline RngMode kind_mode(FastRngKind k) {
+    if (k == FastRngKind::StdRng)
+        throw PricingError(PricingError::Kind::InvalidInput,
+                           "FastRngKind::StdRng streams are not available on the GPU");
+    return k == FastRngKind::Pcg64 ? RngMode::Pcg64 : RngMode::Reference;
+}
+inline uint64_t random_base_seed() {
+    std::random_device rd;
+    return ((uint64_t)rd() << 32) ^ (uint64_t)rd();
+}
+}  // namespace detail
 
 // ------------------------------------------------------------- context --
 // One process-wide context per device (the engine's OnceLock analogue).
@@ -329,12 +348,24 @@ public:
     }
     GpuMonteCarloPricingEngine &with_seed(uint64_t s) {
         seed = s;
+        reproducible = true;
         return *this;
     }
     GpuMonteCarloPricingEngine &with_rng_mode(RngMode m) {
         rng_mode = m;
         return *this;
     }
+    // mc_engine.rs:626-656
+    GpuMonteCarloPricingEngine &with_rng_kind(FastRngKind k) {
+        rng_mode = detail::kind_mode(k);
+        if (k == FastRngKind::ThreadRng) reproducible = false;
+        return *this;
+    }
+    GpuMonteCarloPricingEngine &with_randomized_streams() {
+        reproducible = false;
+        return *this;
+    }
+    GpuMonteCarloPricingEngine &with_thread_rng() { return with_rng_kind(FastRngKind::ThreadRng); }
     GpuMonteCarloPricingEngine &with_greek_method(GreekMethod g) {
         greek_method = g;
         return *this;
@@ -358,12 +389,14 @@ public:
     VarianceReduction variance_reduction = VarianceReduction::None;
     RngMode rng_mode = RngMode::Reference;
     GreekMethod greek_method = GreekMethod::Adjoint;
+    bool reproducible = true;
     int device = 0;
 
 private:
     template <class I>
     PricingResult run(const I &i, const Market &m, bool greeks) const {
-        ofg_option o = detail::to_ofg(i, m, num_paths, num_steps, seed);
+        ofg_option o = detail::to_ofg(i, m, num_paths, num_steps,
+                                      reproducible ? seed : detail::random_base_seed());
         ofg_engine_cfg c{};
         c.antithetic = variance_reduction == VarianceReduction::Antithetic;
         c.control_variate = variance_reduction == VarianceReduction::ControlVariate && !greeks;
@@ -1160,5 +1193,79 @@ public:
 };
 
 }  // namespace dsl
+
+// SpreadOption (src/instruments/spread.rs:15-78) and SpreadMonteCarloEngine
+// (src/engines/monte_carlo/spread_mc.rs:21-175) over ofg_spread_price.
+struct SpreadOption {
+    double s1, s2, k, vol1, vol2, rho, q1, q2, r, t;
+
+    // spread.rs:31-58 (the same comparisons: NaN fields pass where they pass there)
+    void validate() const {
+        auto bad = [](const char *m) { throw PricingError(PricingError::Kind::InvalidInput, m); };
+        if (s1 <= 0.0 || s2 <= 0.0) bad("spread spots s1 and s2 must be > 0");
+        if (k < 0.0) bad("spread strike k must be >= 0");
+        if (vol1 < 0.0 || vol2 < 0.0) bad("spread volatilities vol1 and vol2 must be >= 0");
+        if (!(rho >= -1.0 && rho <= 1.0)) bad("spread correlation rho must be in [-1, 1]");
+        if (t < 0.0) bad("spread maturity t must be >= 0");
+    }
+    // spread.rs:61-72
+    double effective_volatility() const {
+        const double v = vol1 * vol1 - 2.0 * rho * vol1 * vol2 + vol2 * vol2;
+        if (v < -1.0e-14)
+            throw PricingError(PricingError::Kind::InvalidInput,
+                               "spread effective variance is negative");
+        return std::sqrt(std::fmax(v, 0.0));
+    }
+};
+
+class SpreadMonteCarloEngine {
+public:
+    SpreadMonteCarloEngine(int64_t num_paths, uint64_t seed) : paths_(num_paths), seed_(seed) {}
+    SpreadMonteCarloEngine &with_antithetic(bool a) { anti_ = a; return *this; }
+    SpreadMonteCarloEngine &with_rng_kind(FastRngKind k) {
+        kind_ = k;
+        if (k == FastRngKind::ThreadRng) reproducible_ = false;
+        return *this;
+    }
+    SpreadMonteCarloEngine &with_seed(uint64_t seed) {
+        seed_ = seed;
+        reproducible_ = true;
+        return *this;
+    }
+    SpreadMonteCarloEngine &with_randomized_streams() { reproducible_ = false; return *this; }
+    SpreadMonteCarloEngine &with_thread_rng() { return with_rng_kind(FastRngKind::ThreadRng); }
+    // Throughput generator instead of the reference's kinds.
+    SpreadMonteCarloEngine &with_fast_rng(bool fast) { fast_ = fast; return *this; }
+
+    // spread_mc.rs:83-175 (the market is not used there either)
+    PricingResult price(const SpreadOption &i) const {
+        i.validate();
+        if (paths_ == 0)
+            throw PricingError(PricingError::Kind::InvalidInput, "spread MC num_paths must be > 0");
+        const RngMode mode = fast_ ? RngMode::Fast : detail::kind_mode(kind_);
+        ofg_spread_option o{i.s1, i.s2, i.k, i.vol1, i.vol2, i.rho, i.q1, i.q2, i.r, i.t,
+                            paths_, reproducible_ ? seed_ : detail::random_base_seed()};
+        ofg_result r{};
+        check(ofg_spread_price(device_context(), &o, 1, anti_ ? 1 : 0, (int32_t)mode, &r,
+                               nullptr));
+        PricingResult out;
+        out.price = r.price;
+        if (r.has_stderr) out.stderr_ = r.stderr_;
+        if (r.num_paths > 0) {   // simulated (t > 0)
+            out.diagnostics.insert("num_paths", r.num_paths);
+            out.diagnostics.insert("rho", i.rho);
+        }
+        out.kernel_ms = r.kernel_ms;
+        return out;
+    }
+
+private:
+    int64_t paths_;
+    uint64_t seed_;
+    bool anti_ = true;
+    FastRngKind kind_ = FastRngKind::Xoshiro256PlusPlus;
+    bool reproducible_ = true;
+    bool fast_ = false;
+};
 
 }  // namespace openferric_amd

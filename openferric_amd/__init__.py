@@ -36,7 +36,12 @@ from ._lib import DeviceError
 
 from .greeks import (FiniteDifferenceGreeks, McPricer, bump_and_reprice, bump_and_reprice_raw,
                      fd_greeks_raw, finite_difference_greeks)
+from .engines import FastRngKind, kind_rng_mode, rng_kind_from_str
+from .spread import (SpreadMonteCarloEngine, SpreadOption, spread_price, spread_price_raw,
+                     spread_struct)
 __all__ = [
+    "FastRngKind", "kind_rng_mode", "SpreadMonteCarloEngine", "SpreadOption", "rng_kind_from_str", "spread_price",
+    "spread_price_raw", "spread_struct",
     "FiniteDifferenceGreeks", "McPricer", "bump_and_reprice", "bump_and_reprice_raw",
     "fd_greeks_raw", "finite_difference_greeks",
     "AsianOption", "AsianSpec", "Averaging", "BarrierDirection", "BarrierOption", "BarrierSpec",

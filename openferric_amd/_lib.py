@@ -26,7 +26,7 @@ ARITHMETIC, GEOMETRIC = 0, 1
 FIXED, FLOATING = 0, 1
 UP, DOWN = 0, 1
 IN, OUT = 0, 1
-RNG_REFERENCE, RNG_EXTERNAL, RNG_FAST = 0, 1, 2
+RNG_REFERENCE, RNG_EXTERNAL, RNG_FAST, RNG_PCG64 = 0, 1, 2, 3
 GREEKS_NONE, GREEKS_ADJOINT, GREEKS_FD_CRN, GREEKS_FORWARD = 0, 1, 2, 3
 EXACT_SEQUENTIAL, EXACT_PARALLEL, EXACT_ARENA = 0, 1, 2
 LSM_VANILLA, LSM_BERMUDAN, LSM_BARRIER = 0, 1, 2
@@ -42,7 +42,7 @@ EXPORTED_SYMBOLS = [
     "ofg_qmc_price", "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation", "ofg_stress_correlation",
     "ofg_dsl_compile", "ofg_dsl_free", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
     "ofg_dsl_price", "ofg_price_noalloc", "ofg_finite_difference_greeks",
-    "ofg_bump_and_reprice", "ofg_european_exact",
+    "ofg_bump_and_reprice", "ofg_european_exact", "ofg_spread_price",
 ]
 
 
@@ -90,6 +90,12 @@ class OfgFdBumps(C.Structure):
 class OfgFdGreeks(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("delta", "gamma", "vega", "theta", "rho", "vanna",
                                           "volga", "kernel_ms")]
+
+
+class OfgSpreadOption(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("s1", "s2", "k", "vol1", "vol2", "rho", "q1", "q2",
+                                          "r", "t")] + [("num_paths", C.c_int64),
+                                                        ("seed", C.c_uint64)]
 
 
 class OfgMcGreeks(C.Structure):
@@ -253,6 +259,9 @@ def load() -> C.CDLL:
         L.ofg_european_exact.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t, C.c_int32,
                                          C.c_int32, C.POINTER(OfgResult), C.POINTER(C.c_double)]
         L.ofg_european_exact.restype = C.c_int32
+        L.ofg_spread_price.argtypes = [pv, C.POINTER(OfgSpreadOption), C.c_size_t, C.c_int32,
+                                       C.c_int32, C.POINTER(OfgResult), C.POINTER(C.c_double)]
+        L.ofg_spread_price.restype = C.c_int32
         L.ofg_sample_payoffs.argtypes = [pv, C.POINTER(OfgOption), C.POINTER(OfgEngineCfg),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ofg_sample_payoffs.restype = C.c_int32

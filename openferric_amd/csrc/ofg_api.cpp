@@ -570,7 +570,7 @@ int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_eng
 int32_t mc_greeks_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
                        double spot_bump_rel, ofg_mc_greeks *out) {
     if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
-    if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+    if (cfg->rng_mode < 0 || cfg->rng_mode > 3) fail(OFG_INVALID_INPUT, "unknown rng mode");
     check_tuning(*cfg);
     std::vector<ofg_option> sim;
     std::vector<size_t> sim_of;
@@ -911,6 +911,9 @@ int32_t bump_reprice_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n,
 // persistent launch per model (ofg_adjoint.hip), then a segmented reduce.
 int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
                      ofg_result *out) {
+    if (cfg->rng_mode == OFG_RNG_PCG64)
+        fail(OFG_INVALID_INPUT,
+             "pathwise Greeks draw OFG_RNG_REFERENCE, OFG_RNG_EXTERNAL or OFG_RNG_FAST normals");
     std::lock_guard<std::mutex> lock(ctx->mu);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     ofg_engine_cfg c = *cfg;
@@ -1053,7 +1056,7 @@ int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
 int32_t price_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
                    ofg_result *out) {
     if (!ctx || !cfg || (!opts && n) || (!out && n)) fail(OFG_INVALID_INPUT, "null argument");
-    if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+    if (cfg->rng_mode < 0 || cfg->rng_mode > 3) fail(OFG_INVALID_INPUT, "unknown rng mode");
     check_tuning(*cfg);
     if (cfg->greek_mode == OFG_GREEKS_FD_CRN) return price_fd(ctx, opts, n, cfg, out);
     if (cfg->greek_mode == OFG_GREEKS_ADJOINT || cfg->greek_mode == OFG_GREEKS_FORWARD)
@@ -1250,6 +1253,7 @@ void ofg_destroy(ofg_ctx *ctx) {
     basket_release(ctx);
     dsl_release(ctx);
     exact_release(ctx);
+    spread_release(ctx);
     for (DevBuf *b : {&ctx->opts, &ctx->obs_w, &ctx->items, &ctx->order, &ctx->item_first,
                       &ctx->item_count, &ctx->partials, &ctx->outs, &ctx->head, &ctx->xs, &ctx->ys,
                       &ctx->n1, &ctx->n2, &ctx->adj})
@@ -1302,7 +1306,7 @@ int32_t ofg_sample_payoffs(ofg_ctx *ctx, const ofg_option *opt, const ofg_engine
                            double *xs, double *ys) {
     return guarded([&]() -> int32_t {
         if (!ctx || !opt || !cfg || !xs) fail(OFG_INVALID_INPUT, "null argument");
-        if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+        if (cfg->rng_mode < 0 || cfg->rng_mode > 3) fail(OFG_INVALID_INPUT, "unknown rng mode");
         check_tuning(*cfg);
         std::lock_guard<std::mutex> lock(ctx->mu);
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -1320,7 +1324,7 @@ ofg_batch *ofg_batch_create(ofg_ctx *ctx, const ofg_option *opts, size_t n,
     ofg_batch *out = nullptr;
     guarded([&]() -> int32_t {
         if (!ctx || !opts || !cfg || n == 0) fail(OFG_INVALID_INPUT, "null argument");
-        if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+        if (cfg->rng_mode < 0 || cfg->rng_mode > 3) fail(OFG_INVALID_INPUT, "unknown rng mode");
         check_tuning(*cfg);
         if (cfg->rng_mode == OFG_RNG_EXTERNAL)
             fail(OFG_INVALID_INPUT, "batches use generated normals");
@@ -1474,7 +1478,7 @@ int32_t ofg_autotune(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
                      const char *cache_path, int32_t *best_block, int32_t *best_item_steps) {
     return guarded([&]() -> int32_t {
         if (!ctx || !opts || !cfg || n == 0) fail(OFG_INVALID_INPUT, "null argument");
-        if (cfg->rng_mode < 0 || cfg->rng_mode > 2) fail(OFG_INVALID_INPUT, "unknown rng mode");
+        if (cfg->rng_mode < 0 || cfg->rng_mode > 3) fail(OFG_INVALID_INPUT, "unknown rng mode");
         check_tuning(*cfg);
         if (cfg->rng_mode == OFG_RNG_EXTERNAL) fail(OFG_INVALID_INPUT, "autotune needs generated normals");
         // Scaled-down copy: same options, path counts scaled so the sample

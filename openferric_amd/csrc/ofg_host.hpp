@@ -98,6 +98,7 @@ struct MultiBuffers; // ofg_multi_api.cpp
 struct BasketBuffers; // ofg_basket_api.cpp
 struct DslBuffers;    // ofg_dsl_api.cpp
 struct ExactBuffers;  // ofg_exact_api.cpp
+struct SpreadBuffers; // ofg_spread_api.cpp
 
 // Instrument validation with the reference's messages (instruments/*.rs
 // validate()); throws OfgError.
@@ -130,6 +131,7 @@ struct ofg_ctx {
     ofg_host::BasketBuffers *basket = nullptr; // basket scenarios, seeds and partials
     ofg_host::DslBuffers *dsl = nullptr;       // structured-product programs and snapshots
     ofg_host::ExactBuffers *exact = nullptr;   // exact-terminal pricers' streams and sums
+    ofg_host::SpreadBuffers *spread = nullptr; // spread options' jobs and sums
     void *noalloc = nullptr;   // this GPU's fixed region of ofg_price_noalloc (not owned)
     int cus = 256;
 };
@@ -147,4 +149,6 @@ void basket_release(ofg_ctx *ctx);
 void dsl_release(ofg_ctx *ctx);
 // Releases the context's exact-terminal pricer buffers (ofg_exact_api.cpp).
 void exact_release(ofg_ctx *ctx);
+// Releases the context's spread-option buffers (ofg_spread_api.cpp).
+void spread_release(ofg_ctx *ctx);
 }  // namespace ofg_host

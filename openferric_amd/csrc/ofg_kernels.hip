@@ -54,12 +54,16 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
     leg_init<PK>(ga, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0);
     if constexpr (ANTI) leg_init<PK>(gb, o, (PK == PK_ASIAN || PK == PK_ASIAN_LOG) ? w[0] : 0.0);
 
-    // Reference draws use the reference's xoshiro256++; the fast mode uses
-    // xoroshiro128+ with Box-Muller (four normals per three 64-bit draws).
-    using Rng = typename std::conditional<RNG == OFG_RNG_FAST, Xoroshiro128p, Xoshiro256pp>::type;
+    // Reference draws use the reference's xoshiro256++ (or its Pcg64 kind);
+    // the fast mode uses xoroshiro128+ with Box-Muller (four normals per three
+    // 64-bit draws).
+    constexpr bool REF = RNG == OFG_RNG_REFERENCE || RNG == OFG_RNG_PCG64;
+    using Rng = typename std::conditional<
+        RNG == OFG_RNG_FAST, Xoroshiro128p,
+        typename std::conditional<RNG == OFG_RNG_PCG64, Pcg64, Xoshiro256pp>::type>::type;
     Rng rng;
     ExternalNormals ext;
-    if constexpr (RNG == OFG_RNG_REFERENCE || RNG == OFG_RNG_FAST) {
+    if constexpr (REF || RNG == OFG_RNG_FAST) {
         rng.seed(stream_seed(o.seed, (uint64_t)sample));
     } else {
         ext.p1 = a.normals1 + o.normals_offset + sample;
@@ -94,7 +98,7 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
                 if (j + 1 < steps) advance(z1, j + 1);
                 if (j + 2 < steps) advance(z2, j + 2);
             }
-        } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+        } else if constexpr (REF) {
             // the reference's draws REF_AK at a time (the inverse CDF with
             // its tail region deferred), applied in order
             for (int j = 0; j < steps; j += REF_AK) {
@@ -132,7 +136,7 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
                 normals4(rng, z0, z1, z2, z3, tab);
                 advance(z0, z1, j);
             }
-        } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+        } else if constexpr (REF) {
             // z1, z2 of REF_AK / 2 steps per block, in the reference's order
             for (int j = 0; j < steps; j += REF_AK / 2) {
                 double p[REF_AK], z[REF_AK];
@@ -346,6 +350,7 @@ hipError_t launch_pricing(const LaunchArgs &a, hipStream_t stream) {
     switch (a.rng_mode) {
     case OFG_RNG_EXTERNAL: return launch_rng<OFG_RNG_EXTERNAL>(a, stream);
     case OFG_RNG_FAST: return launch_rng<OFG_RNG_FAST>(a, stream);
+    case OFG_RNG_PCG64: return launch_rng<OFG_RNG_PCG64>(a, stream);
     default: return launch_rng<OFG_RNG_REFERENCE>(a, stream);
     }
 }
@@ -363,6 +368,7 @@ hipError_t launch_reduce(const OptDev *opts, const int64_t *item_first, const in
 int persistent_grid(int block, int rng_mode, int model, int antithetic, int device) {
     const void *k = rng_mode == OFG_RNG_EXTERNAL ? kernel_ptr<OFG_RNG_EXTERNAL>(block, model, antithetic)
                     : rng_mode == OFG_RNG_FAST   ? kernel_ptr<OFG_RNG_FAST>(block, model, antithetic)
+                    : rng_mode == OFG_RNG_PCG64  ? kernel_ptr<OFG_RNG_PCG64>(block, model, antithetic)
                                                  : kernel_ptr<OFG_RNG_REFERENCE>(block, model, antithetic);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess ||

@@ -342,6 +342,51 @@ OFG_HD uint64_t Xoshiro256pp::prev_u64() {
     return rotl64(s0 + s3, 23) + s0;
 }
 
+// ------------------------------------------------ reference Pcg64 stream ---
+
+OFG_HD uint64_t mul_hi_u64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// Pcg64 of fast_rng.rs:93-137 (PCG XSL-RR 128/64): the 128-bit state held as
+// two words, advanced by state * M + inc mod 2^128 (one 64x64 full product
+// and two low products), each draw the XSL-RR permutation of the state
+// BEFORE the step: (hi ^ lo) rotated right by hi >> 58. Seeded from three
+// SplitMix64 outputs (state high, state low, stream; inc = stream << 1 | 1
+// as a 128-bit value), then one draw discarded.
+struct Pcg64 {
+    static constexpr uint64_t M_HI = 0x2360ED051FC65DA4ull;
+    static constexpr uint64_t M_LO = 0x4385DF649FCCF645ull;
+    uint64_t lo, hi, inc_lo, inc_hi;
+
+    OFG_HD void seed(uint64_t seed) {
+        uint64_t sm = seed;
+        hi = splitmix64_next(sm);
+        lo = splitmix64_next(sm);
+        const uint64_t stream = splitmix64_next(sm);
+        inc_lo = (stream << 1) | 1ull;
+        inc_hi = stream >> 63;
+        (void)next_u64();
+    }
+
+    OFG_HD uint64_t next_u64() {
+        const uint64_t olo = lo, ohi = hi;
+        const uint64_t plo = olo * M_LO;
+        const uint64_t phi = mul_hi_u64(olo, M_LO) + olo * M_HI + ohi * M_LO;
+        lo = plo + inc_lo;
+        hi = phi + inc_hi + (lo < plo ? 1ull : 0ull);
+        const uint64_t x = ohi ^ olo;
+        const uint32_t r = (uint32_t)(ohi >> 58);
+        return (x >> r) | (x << ((64u - r) & 63u));
+    }
+
+    OFG_HD double next_f64() { return (double)(next_u64() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
 OFG_HD double bits_as_double(uint64_t b) {
     double d;
     __builtin_memcpy(&d, &b, sizeof d);

@@ -93,6 +93,10 @@ __device__ void mcg_item(const OptDev &o, const WorkItem &wi, int lane, const Ad
             Xoshiro256pp rng;
             rng.seed(stream_seed(o.seed, (uint64_t)sample));
             z = reference_normal(rng);
+        } else if constexpr (RNG == OFG_RNG_PCG64) {
+            Pcg64 rng;
+            rng.seed(stream_seed(o.seed, (uint64_t)sample));
+            z = acklam_inv_cdf(uniform_open01(rng.next_f64()));
         } else if constexpr (RNG == OFG_RNG_FAST) {
             Xoroshiro128p rng;
             rng.seed(stream_seed(o.seed, (uint64_t)sample));
@@ -210,6 +214,7 @@ hipError_t launch_mc_greeks(const AdjArgs &a, hipStream_t s) {
     switch (a.rng_mode) {
     case OFG_RNG_EXTERNAL: return launch_anti<OFG_RNG_EXTERNAL>(a, s);
     case OFG_RNG_FAST: return launch_anti<OFG_RNG_FAST>(a, s);
+    case OFG_RNG_PCG64: return launch_anti<OFG_RNG_PCG64>(a, s);
     default: return launch_anti<OFG_RNG_REFERENCE>(a, s);
     }
 }
@@ -226,6 +231,7 @@ hipError_t launch_mc_greeks_reduce(const int64_t *item_first, const int32_t *ite
 int mc_greeks_grid(int block, int rng_mode, int antithetic, int device) {
     const void *k = rng_mode == OFG_RNG_EXTERNAL ? anti_ptr<OFG_RNG_EXTERNAL>(block, antithetic)
                     : rng_mode == OFG_RNG_FAST   ? anti_ptr<OFG_RNG_FAST>(block, antithetic)
+                    : rng_mode == OFG_RNG_PCG64  ? anti_ptr<OFG_RNG_PCG64>(block, antithetic)
                                                  : anti_ptr<OFG_RNG_REFERENCE>(block, antithetic);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess ||

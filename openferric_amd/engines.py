@@ -43,6 +43,76 @@ class RngMode(enum.Enum):
     Reference = _lib.RNG_REFERENCE
     External = _lib.RNG_EXTERNAL
     Fast = _lib.RNG_FAST
+    Pcg64 = _lib.RNG_PCG64
+
+
+class FastRngKind(enum.Enum):
+    """fast_rng.rs:20-27: the reference's per-path stream generators."""
+    Xoshiro256PlusPlus = 0
+    Pcg64 = 1
+    ThreadRng = 2
+    StdRng = 3
+
+
+def rng_kind_from_str(value: Optional[str]) -> FastRngKind:
+    """python/src/engines.rs:92-108"""
+    v = (value if value is not None else "xoshiro256plusplus").lower()
+    if v in ("xoshiro256plusplus", "xoshiro256_plus_plus", "xoshiro"):
+        return FastRngKind.Xoshiro256PlusPlus
+    if v in ("pcg64", "pcg"):
+        return FastRngKind.Pcg64
+    if v in ("threadrng", "thread_rng", "thread"):
+        return FastRngKind.ThreadRng
+    if v in ("stdrng", "std_rng", "std"):
+        return FastRngKind.StdRng
+    raise ValueError(f"unsupported rng kind '{value}'")
+
+
+def kind_rng_mode(kind: FastRngKind) -> RngMode:
+    """The library's draws for a reference stream kind: xoshiro256++ or
+    Pcg64 per path as the reference. ThreadRng streams are non-reproducible
+    there; here they are xoshiro256++ streams on a fresh random base seed.
+    StdRng (rand's ChaCha12) has no GPU stream."""
+    if kind is FastRngKind.StdRng:
+        raise InvalidInput("FastRngKind::StdRng streams are not available on the GPU")
+    return RngMode.Pcg64 if kind is FastRngKind.Pcg64 else RngMode.Reference
+
+
+class _StreamOptions:
+    """rng_kind / reproducible builders shared by the engines whose
+    reference counterparts carry them (MonteCarloPricingEngine,
+    mc_engine.rs:626-656; MonteCarloGreeksEngine, mc_greeks.rs:44-75;
+    SpreadMonteCarloEngine, spread_mc.rs:53-80). Randomized streams draw a
+    fresh base seed per call (the reference a fresh seed per path: both are
+    independent non-reproducible streams)."""
+
+    rng_kind = FastRngKind.Xoshiro256PlusPlus
+    reproducible = True
+
+    def with_rng_kind(self, rng_kind: FastRngKind):
+        self.rng_kind = rng_kind
+        if rng_kind is FastRngKind.ThreadRng:
+            self.reproducible = False
+        if rng_kind is not FastRngKind.StdRng:
+            self.rng_mode = kind_rng_mode(rng_kind)
+        return self
+
+    def with_randomized_streams(self):
+        self.reproducible = False
+        return self
+
+    def with_thread_rng(self):
+        return self.with_rng_kind(FastRngKind.ThreadRng)
+
+    def _stream_mode(self) -> RngMode:
+        if self.rng_kind is FastRngKind.StdRng and self.rng_mode in (RngMode.Reference,
+                                                                     RngMode.Pcg64):
+            kind_rng_mode(self.rng_kind)   # raises
+        return self.rng_mode
+
+    def _stream_seed(self) -> int:
+        import secrets
+        return self.seed if self.reproducible else secrets.randbits(64)
 
 
 _ERRORS = {
@@ -187,7 +257,7 @@ def _to_result(r: _lib.OfgResult, with_greeks: bool, heston: bool) -> PricingRes
     return res
 
 
-class GpuMonteCarloPricingEngine:
+class GpuMonteCarloPricingEngine(_StreamOptions):
     """Drop-in for MonteCarloPricingEngine (mc_engine.rs:587-670, 871-988)."""
 
     def __init__(self, num_paths: int, num_steps: int, seed: int):
@@ -209,6 +279,7 @@ class GpuMonteCarloPricingEngine:
 
     def with_seed(self, seed: int) -> "GpuMonteCarloPricingEngine":
         self.seed = seed
+        self.reproducible = True
         return self
 
     def with_rng_mode(self, mode: RngMode) -> "GpuMonteCarloPricingEngine":
@@ -229,10 +300,11 @@ class GpuMonteCarloPricingEngine:
     def price(self, instrument, market: Market) -> PricingResult:
         self._validate(instrument)
         keep = _Keep()
-        o = option_struct(instrument, market, self.num_paths, self.num_steps, self.seed, keep)
+        o = option_struct(instrument, market, self.num_paths, self.num_steps, self._stream_seed(),
+                          keep)
         cfg = engine_cfg(antithetic=self.variance_reduction is VarianceReduction.Antithetic,
                          control_variate=self.variance_reduction is VarianceReduction.ControlVariate,
-                         rng_mode=self.rng_mode, keep=keep)
+                         rng_mode=self._stream_mode(), keep=keep)
         r = price_raw([o], cfg, self.device)[0]
         return _to_result(r, False, False)
 
@@ -241,9 +313,11 @@ class GpuMonteCarloPricingEngine:
         them for VanillaOption, mc_engine.rs:977-987)."""
         self._validate(instrument)
         keep = _Keep()
-        o = option_struct(instrument, market, self.num_paths, self.num_steps, self.seed, keep)
+        o = option_struct(instrument, market, self.num_paths, self.num_steps, self._stream_seed(),
+                          keep)
         cfg = engine_cfg(antithetic=self.variance_reduction is VarianceReduction.Antithetic,
-                         rng_mode=self.rng_mode, greek_mode=self.greek_method.value, keep=keep)
+                         rng_mode=self._stream_mode(), greek_mode=self.greek_method.value,
+                         keep=keep)
         r = price_raw([o], cfg, self.device)[0]
         return _to_result(r, True, False)
 
@@ -373,7 +447,7 @@ def mc_european_pathwise_aad(engine: GpuMonteCarloPricingEngine, instrument: Van
     return engine.price_with_greeks_aad(instrument, eff)
 
 
-class MonteCarloGreeksEngine:
+class MonteCarloGreeksEngine(_StreamOptions):
     """Drop-in for MonteCarloGreeksEngine (mc_greeks.rs:20-129): pathwise and
     likelihood-ratio Greeks of European GBM vanillas from one exact-terminal
     normal per sample, computed by the GPU estimator kernel
@@ -397,6 +471,7 @@ class MonteCarloGreeksEngine:
 
     def with_seed(self, seed: int) -> "MonteCarloGreeksEngine":
         self.seed = seed
+        self.reproducible = True
         return self
 
     def with_rng_mode(self, mode: RngMode) -> "MonteCarloGreeksEngine":
@@ -411,8 +486,8 @@ class MonteCarloGreeksEngine:
         keep = _Keep()
         eff = Market(market.spot, market.rate, market.effective_dividend_yield(instrument.expiry),
                      market.flat_vol)
-        o = option_struct(instrument, eff, self.num_paths, 1, self.seed, keep)
-        cfg = engine_cfg(antithetic=self.antithetic, rng_mode=self.rng_mode, keep=keep)
+        o = option_struct(instrument, eff, self.num_paths, 1, self._stream_seed(), keep)
+        cfg = engine_cfg(antithetic=self.antithetic, rng_mode=self._stream_mode(), keep=keep)
         L = _lib.load()
         ctx = _lib.context(self.device)
         out = _lib.OfgMcGreeks()

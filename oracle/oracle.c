@@ -395,11 +395,11 @@ double or_builtin_cv_expected(const or_option *o, int32_t *cv_kind) {
 
 void or_sample_normals(uint64_t base_seed, int64_t sample, int64_t steps,
                        int32_t streams, double *z1, double *z2) {
-    uint64_t s[4];
-    or_xoshiro_seed(or_stream_seed(base_seed, (uint64_t)sample), s);
+    or_fast_rng g;
+    or_fast_rng_seed(&g, or_rng_kind(), or_stream_seed(base_seed, (uint64_t)sample));
     for (int64_t j = 0; j < steps; ++j) {
-        z1[j] = or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(s)));
-        if (streams >= 2) z2[j] = or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(s)));
+        z1[j] = or_fast_rng_normal(&g);
+        if (streams >= 2) z2[j] = or_fast_rng_normal(&g);
     }
 }
 
@@ -787,9 +787,9 @@ int or_mc_greeks_estimate(const or_option *o, int64_t num_paths, uint64_t seed,
     const int64_t samples = antithetic ? (num_paths + 1) / 2 : num_paths;
     double sums[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int64_t i = 0; i < samples; ++i) {
-        uint64_t s[4];
-        or_xoshiro_seed(or_stream_seed(seed, (uint64_t)i), s);
-        const double z = or_acklam_inv_cdf(or_uniform_open01(or_xoshiro_next_f64(s)));
+        or_fast_rng g;
+        or_fast_rng_seed(&g, or_rng_kind(), or_stream_seed(seed, (uint64_t)i));
+        const double z = or_fast_rng_normal(&g);
         double b[5], a[5];
         mcg_single_path(o->option_type, z, spot, strike, spot_up, spot_dn, spot_span, vol,
                         maturity, drift, sig_sqrt_t, b);

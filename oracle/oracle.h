@@ -45,6 +45,10 @@
  *   or_sobol_*, or_mc_european_qmc     reference QMC assertions, (0,m,1)-net
  *                                      stratification of every dimension,
  *                                      Gray-code identity, BS within 0.1 %
+ *   or_pcg64_*                         numpy's PCG64 XSL-RR 128/64 streams
+ *   or_spread_mc                       Margrabe at k = 0, zero-vol legs as
+ *                                      Black-Scholes, Kirk (reference test),
+ *                                      per-sample payoffs from pinned streams
  */
 #ifndef OPENFERRIC_ORACLE_H
 #define OPENFERRIC_ORACLE_H
@@ -292,6 +296,40 @@ void or_sobol_points(int64_t dims, uint64_t seed, int64_t n, uint64_t *words, do
 /* mc_european_qmc_with_seed (engines/monte_carlo/mc_qmc.rs:40-144). */
 void or_mc_european_qmc(const or_option *o, int64_t n_paths, int64_t n_steps, uint64_t seed,
                         double *price, double *stderr_out, int32_t *has_stderr);
+
+/* ---- Pcg64 and the spread engine (oracle_spread.c) -------------------- */
+/* Pcg64 of fast_rng.rs:93-137; st = {state lo, state hi, inc lo, inc hi}. */
+void or_pcg64_seed(uint64_t seed, uint64_t st[4]);
+uint64_t or_pcg64_next_u64(uint64_t st[4]);
+double or_pcg64_next_f64(uint64_t st[4]);
+enum { OR_RNG_XOSHIRO = 0, OR_RNG_PCG64 = 1 };
+/* A seeded per-sample stream of either kind (FastRng, fast_rng.rs:138-181). */
+typedef struct {
+    int32_t kind;
+    uint64_t s[4];
+} or_fast_rng;
+void or_fast_rng_seed(or_fast_rng *g, int32_t kind, uint64_t seed);
+double or_fast_rng_normal(or_fast_rng *g);   /* sample_standard_normal */
+/* Stream kind drawn by or_sample_normals (so or_mc_run) and
+ * or_mc_greeks_estimate: OR_RNG_XOSHIRO (default) or OR_RNG_PCG64. */
+void or_set_rng_kind(int32_t kind);
+int32_t or_rng_kind(void);
+enum {
+    OR_SPREAD_BAD_SPOTS = -1,      /* "spread spots s1 and s2 must be > 0" */
+    OR_SPREAD_BAD_STRIKE = -2,     /* "spread strike k must be >= 0" */
+    OR_SPREAD_BAD_VOLS = -3,       /* "spread volatilities vol1 and vol2 must be >= 0" */
+    OR_SPREAD_BAD_RHO = -4,        /* "spread correlation rho must be in [-1, 1]" */
+    OR_SPREAD_BAD_MATURITY = -5,   /* "spread maturity t must be >= 0" */
+    OR_SPREAD_NO_PATHS = -6,       /* "spread MC num_paths must be > 0" */
+    OR_SPREAD_BAD_RNG = -7
+};
+/* SpreadOption::validate (instruments/spread.rs:31-58); p = {s1, s2, k,
+ * vol1, vol2, rho, q1, q2, r, t}. */
+int32_t or_spread_validate(const double p[10]);
+/* SpreadMonteCarloEngine::price (engines/monte_carlo/spread_mc.rs:83-175),
+ * reproducible streams of rng_kind OR_RNG_*. */
+int32_t or_spread_mc(const double p[10], int64_t num_paths, uint64_t seed, int32_t antithetic,
+                     int32_t rng_kind, double out[2], double *payoffs);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@ leg import this module. The product package never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -157,6 +158,13 @@ def lib():
         L.or_heston_price_delta_aad.restype = C.c_int
         L.or_mc_european_exact.argtypes = [i32, i32, d, d, d, d, d, d, i64, i32, pd, pd]
         L.or_mc_european_exact.restype = C.c_int
+        L.or_pcg64_seed.argtypes = [u64, pu64]; L.or_pcg64_seed.restype = None
+        L.or_pcg64_next_u64.argtypes = [pu64]; L.or_pcg64_next_u64.restype = u64
+        L.or_pcg64_next_f64.argtypes = [pu64]; L.or_pcg64_next_f64.restype = d
+        L.or_set_rng_kind.argtypes = [i32]; L.or_set_rng_kind.restype = None
+        L.or_rng_kind.argtypes = []; L.or_rng_kind.restype = i32
+        L.or_spread_validate.argtypes = [pd]; L.or_spread_validate.restype = i32
+        L.or_spread_mc.argtypes = [pd, i64, u64, i32, i32, pd, pd]; L.or_spread_mc.restype = i32
         L.or_pathwise_greeks.argtypes = [C.POINTER(OrOption), i64, i64, u64, i32, pd, pd, i32,
                                          i32, pd, pd]
         L.or_pathwise_greeks.restype = C.c_int
@@ -322,6 +330,100 @@ def mc_european_exact(kind, option_type, spot, strike, rate, q, vol, expiry, n_p
         raise ValueError(f"or_mc_european_exact failed rc={rc}")
     se = float(out[1]) if out[2] else None
     return (float(out[0]), se, pay) if want_payoffs else (float(out[0]), se)
+
+
+RNG_XOSHIRO, RNG_PCG64 = 0, 1
+SPREAD_MESSAGES = {
+    -1: "spread spots s1 and s2 must be > 0",
+    -2: "spread strike k must be >= 0",
+    -3: "spread volatilities vol1 and vol2 must be >= 0",
+    -4: "spread correlation rho must be in [-1, 1]",
+    -5: "spread maturity t must be >= 0",
+    -6: "spread MC num_paths must be > 0",
+}
+SPREAD_FIELDS = ("s1", "s2", "k", "vol1", "vol2", "rho", "q1", "q2", "r", "t")
+
+
+class rng_kind:
+    """Context manager: the oracle's MonteCarloEngine streams (or_mc_run,
+    or_sample_normals, or_mc_greeks_estimate) draw Pcg64 or xoshiro256++."""
+
+    def __init__(self, kind):
+        self.kind = kind
+
+    def __enter__(self):
+        self.prev = lib().or_rng_kind()
+        lib().or_set_rng_kind(self.kind)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_rng_kind(self.prev)
+        return False
+
+
+def pcg64_stream(seed, n):
+    """The first n u64 draws of Pcg64::seed_from_u64(seed) (fast_rng.rs:93-137)."""
+    st = (C.c_uint64 * 4)()
+    lib().or_pcg64_seed(seed, st)
+    return [lib().or_pcg64_next_u64(st) for _ in range(n)]
+
+
+def spread_mc(option, num_paths, seed, antithetic=True, rng_kind=RNG_XOSHIRO,
+              want_payoffs=False):
+    """SpreadMonteCarloEngine::price (spread_mc.rs:83-175) with reproducible
+    streams; option is a dict of SPREAD_FIELDS. Returns (price, stderr[,
+    payoffs]); raises ValueError with the reference's message on its errors."""
+    p = np.array([float(option[f]) for f in SPREAD_FIELDS])
+    out = np.zeros(2)
+    samples = (num_paths + 1) // 2 if antithetic else num_paths
+    pay = np.full(max(samples, 0), np.nan) if want_payoffs else None
+    rc = lib().or_spread_mc(_ptr(p), num_paths, seed, 1 if antithetic else 0, rng_kind,
+                            _ptr(out), _ptr(pay))
+    if rc:
+        raise ValueError(SPREAD_MESSAGES.get(rc, f"or_spread_mc failed rc={rc}"))
+    return (float(out[0]), float(out[1]), pay) if want_payoffs else (float(out[0]), float(out[1]))
+
+
+def _ncdf(x):
+    return 0.5 * math.erfc(-x / math.sqrt(2.0))
+
+
+def margrabe_exchange_price(o):
+    """margrabe_exchange_price (src/engines/analytic/spread.rs:69-98), k = 0,
+    with the exact normal CDF."""
+    t = o["t"]
+    if t <= 0.0:
+        return max(o["s1"] - o["s2"], 0.0)
+    var = o["vol1"] ** 2 - 2.0 * o["rho"] * o["vol1"] * o["vol2"] + o["vol2"] ** 2
+    sigma = math.sqrt(max(var, 0.0))
+    s1_df = o["s1"] * math.exp(-o["q1"] * t)
+    s2_df = o["s2"] * math.exp(-o["q2"] * t)
+    if sigma <= 0.0:
+        return max(s1_df - s2_df, 0.0)
+    sst = sigma * math.sqrt(t)
+    d1 = (math.log(o["s1"] / o["s2"]) + (o["q2"] - o["q1"] + 0.5 * sigma * sigma) * t) / sst
+    return s1_df * _ncdf(d1) - s2_df * _ncdf(d1 - sst)
+
+
+def kirk_spread_price(o):
+    """kirk_spread_price (src/engines/analytic/spread.rs:100-147) with the
+    exact normal CDF."""
+    t = o["t"]
+    if t <= 0.0:
+        return max(o["s1"] - o["s2"] - o["k"], 0.0)
+    f1 = o["s1"] * math.exp((o["r"] - o["q1"]) * t)
+    f2 = o["s2"] * math.exp((o["r"] - o["q2"]) * t)
+    den = f2 + o["k"]
+    beta = f2 / den
+    v1, v2, rho = o["vol1"], o["vol2"], o["rho"]
+    sk2 = v1 * v1 - 2.0 * beta * rho * v1 * v2 + beta * beta * v2 * v2
+    sk = math.sqrt(max(sk2, 0.0))
+    disc = math.exp(-o["r"] * t)
+    if sk <= 0.0:
+        return disc * max(f1 - den, 0.0)
+    sst = sk * math.sqrt(t)
+    d1 = (math.log(f1 / den) + 0.5 * sk2 * t) / sst
+    return disc * (f1 * _ncdf(d1) - den * _ncdf(d1 - sst))
 
 
 PATHWISE_SLOTS = ["s0", "sigma_or_v0", "r", "T", "kappa", "theta", "xi", "rho"]

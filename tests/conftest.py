@@ -44,6 +44,7 @@ def _load_math_shim():
         "shim_xoshiro_roundtrip": ([u64, C.c_int, i64, C.POINTER(u64), C.POINTER(u64)],
                                    C.c_int),
         "shim_xoroshiro_from_state": ([u64, u64, i64, C.POINTER(u64)], None),
+        "shim_pcg64": ([u64, i64, C.POINTER(u64), pd], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -308,6 +308,28 @@ int main() {
         }
         expect(threw, "dsl_compile_error_is_invalid_input");
     }
+        {  // spread_mc.rs:206-241: spread MC within 1 % of Kirk (300k paths, seed 13)
+        const SpreadOption sp{100.0, 96.0, 3.0, 0.20, 0.15, 0.5, 0.0, 0.0, 0.05, 0.5};
+        auto r = SpreadMonteCarloEngine(300000, 13).price(sp);
+        const double kirk = 5.57702191;   // kirk_reference_case, spread.rs:193-209
+        expect(std::fabs((r.price - kirk) / kirk) <= 0.01,
+               "spread_mc_matches_kirk_within_one_percent_for_near_zero_strike_case",
+               std::to_string(r.price));
+        expect(r.diagnostics.get("num_paths").value_or(0) == 300000.0 &&
+                   r.diagnostics.get("rho").value_or(0) == 0.5,
+               "spread_diagnostics_num_paths_rho");
+        auto p = SpreadMonteCarloEngine(300000, 13).with_rng_kind(FastRngKind::Pcg64).price(sp);
+        expect(std::fabs((p.price - kirk) / kirk) <= 0.01 && p.price != r.price,
+               "spread_mc_pcg64_kind_within_one_percent_of_kirk", std::to_string(p.price));
+        bool threw = false;
+        try {
+            SpreadMonteCarloEngine(10, 1).price(SpreadOption{100.0, 96.0, -1.0, 0.2, 0.15, 0.5,
+                                                             0.0, 0.0, 0.05, 0.5});
+        } catch (const PricingError &e) {
+            threw = std::string(e.what()) == "invalid input: spread strike k must be >= 0";
+        }
+        expect(threw, "spread_negative_strike_is_invalid_input");
+    }
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
 }

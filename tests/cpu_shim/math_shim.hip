@@ -86,6 +86,17 @@ double shim_fast_heston_terminal(uint64_t seed, uint64_t sample, double s0, doub
 }
 
 extern "C" {
+// n outputs of the device Pcg64 seeded with seed, and their next_f64 values
+// from a second stream of the same seed.
+void shim_pcg64(uint64_t seed, int64_t n, uint64_t *out, double *u) {
+    ofg::Pcg64 a, b;
+    a.seed(seed);
+    b.seed(seed);
+    for (int64_t i = 0; i < n; ++i) {
+        out[i] = a.next_u64();
+        u[i] = b.next_f64();
+    }
+}
 // n outputs of xoroshiro128+ from an explicit state (no seeding).
 void shim_xoroshiro_from_state(uint64_t s0, uint64_t s1, int64_t n, uint64_t *out) {
     ofg::Xoroshiro128p r;

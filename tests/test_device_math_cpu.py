@@ -10,6 +10,7 @@ import ctypes as C
 
 import mpmath as mp
 import numpy as np
+import pytest
 
 
 def test_exp_fp64_accuracy(shim):
@@ -183,3 +184,17 @@ def test_xoroshiro128plus_unit_state_by_hand(shim):
     # s1' = rotl(2^63, 37) = 2^36
     shim.shim_xoroshiro_from_state(1 << 63, 0, 2, out)
     assert list(out) == [1 << 63, ((1 << 23) ^ (1 << 63)) + (1 << 36)]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 13, 7919, 0xDEADBEEFCAFEBABE, 2**64 - 1])
+def test_device_pcg64_equals_oracle(shim, oracle, seed):
+    """The device Pcg64 (two-word state, mul_hi) against the oracle's 128-bit
+    restatement, itself pinned by numpy's PCG64."""
+    import ctypes as C
+    n = 3000
+    out = (C.c_uint64 * n)()
+    u = np.zeros(n)
+    shim.shim_pcg64(seed, n, out, u.ctypes.data_as(C.POINTER(C.c_double)))
+    want = oracle.pcg64_stream(seed, n)
+    assert list(out) == want
+    assert np.array_equal(u, np.array([(x >> 11) * 2.0 ** -53 for x in want]))
