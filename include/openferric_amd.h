@@ -651,6 +651,14 @@ int32_t ofg_lsm_last_timing(ofg_ctx *ctx, double *paths_ms, double *rounds_ms,
  * Host-only, needs no GPU. */
 int32_t ofg_stream_jump(const uint64_t state[4], uint64_t k, uint64_t out[4]);
 
+/* Fault injection for callers' error-path tests. No counterpart in the
+ * reference. The after-th device allocation made by any context from now on
+ * (1 = the next one) fails as an out-of-memory hipMalloc would: the entry
+ * point that made it returns OFG_DEVICE_ERROR with a message naming the
+ * allocation, and the context stays usable (the buffer is allocated again
+ * by the next call). 0 disarms. Process-wide; needs no GPU. */
+void ofg_inject_alloc_failure(int32_t after);
+
 #ifdef __cplusplus
 }
 #endif

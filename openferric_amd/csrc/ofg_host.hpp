@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -48,6 +49,10 @@ inline void hip_check(hipError_t e, const char *what) {
     }
 }
 
+// Armed by ofg_inject_alloc_failure: the k-th device allocation from then on
+// (1 = the next) fails as an out-of-memory hipMalloc would; 0 = disarmed.
+inline std::atomic<int32_t> g_alloc_fault{0};
+
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
@@ -57,7 +62,17 @@ struct DevBuf {
         p = nullptr;
         cap = 0;
         const size_t want = std::max<size_t>(bytes, 256);
-        hip_check(hipMalloc(&p, want), "hipMalloc");
+        int32_t k = g_alloc_fault.load();
+        while (k > 0 && !g_alloc_fault.compare_exchange_weak(k, k - 1)) {
+        }
+        const hipError_t e = k == 1 ? hipErrorOutOfMemory : hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            // the caller unwinds and frees the host arrays of copies it has
+            // already enqueued: let those finish first
+            p = nullptr;
+            (void)hipDeviceSynchronize();
+            hip_check(e, k == 1 ? "hipMalloc (injected failure)" : "hipMalloc");
+        }
         cap = want;
     }
     void release() {
