@@ -723,7 +723,7 @@ constexpr size_t NA_MAX_OBS = OFG_NOALLOC_MAX_OBS_WEIGHTS;
 constexpr size_t na_align(size_t b) { return (b + 255) & ~(size_t)255; }
 constexpr size_t NA_OPTS = 0;
 constexpr size_t NA_OBS = NA_OPTS + na_align(NA_MAX_OPTIONS * sizeof(ofg::OptDev));
-constexpr size_t NA_ITEMS = NA_OBS + na_align(NA_MAX_OBS * sizeof(double));
+constexpr size_t NA_ITEMS = NA_OBS + na_align((NA_MAX_OBS + OBS_PAD) * sizeof(double));
 constexpr size_t NA_ORDER = NA_ITEMS + na_align(NA_MAX_ITEMS * sizeof(ofg::WorkItem));
 constexpr size_t NA_FIRST = NA_ORDER + na_align(NA_MAX_ITEMS * sizeof(int32_t));
 constexpr size_t NA_COUNT = NA_FIRST + na_align(NA_MAX_OPTIONS * sizeof(int64_t));
@@ -791,7 +791,7 @@ int32_t price_noalloc_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n,
         t.item_steps *= 2;
         p = build_plan(opts, n, *cfg, t);
     }
-    if (p.obs_w.size() > NA_MAX_OBS)
+    if (p.obs_w.size() > NA_MAX_OBS + OBS_PAD)
         fail(OFG_INVALID_INPUT,
              "ofg_price_noalloc takes at most 1048576 observation weights per call");
     const size_t nopt = p.opts.size();

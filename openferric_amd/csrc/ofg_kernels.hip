@@ -31,7 +31,10 @@ namespace ofg {
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int REF_AK = 8;   // reference draws per deferred-tail inverse-CDF block
+// Reference draws per inverse-CDF block (acklam_block_wave). 16 draws per
+// block halve the tail rounds per draw but take the GBM loop to 190 VGPRs
+// (2 waves per SIMD), which measured slower.
+constexpr int REF_AK = 8;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -43,11 +46,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Sample simulation: one (possibly antithetic) sample -> (x, y)
 // ------------------------------------------------------------------------
 
+// Observation weights: read-only for the kernel's lifetime and indexed by the
+// wave-uniform step, so they are read through the constant address space,
+// which makes them scalar loads into SGPRs instead of per-lane vector loads.
+using cdouble = __attribute__((address_space(4))) const double;
+
 template <int MODEL, int PK, int RNG, bool ANTI>
 __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
-                                                const double *__restrict__ w,
-                                                const LaunchArgs &a, const double *tab, double &x,
-                                                double &y) {
+                                                const cdouble *w,
+                                                const LaunchArgs &a, const double *tab,
+                                                double *slots, double &x, double &y) {
     const double barrier = o.barrier;
     const int steps = o.steps;
     Leg ga, gb;
@@ -74,13 +82,13 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
 
     if constexpr (MODEL == OFG_MODEL_GBM) {
         // one GBM step on both legs, then the observation of path index j+1
-        auto advance = [&](double z, int j) {
-            gbm_step<PK, RNG == OFG_RNG_FAST>(ga, o, z, tab);
-            if constexpr (ANTI) gbm_step<PK, RNG == OFG_RNG_FAST>(gb, o, -z, tab);
-            const double wj = ASIAN ? w[j + 1] : 0.0;
+        auto advance_w = [&](double z, double wj) {
+            gbm_step<PK, RNG == OFG_RNG_FAST, REF>(ga, o, z, tab);
+            if constexpr (ANTI) gbm_step<PK, RNG == OFG_RNG_FAST, REF>(gb, o, -z, tab);
             leg_observe<PK>(ga, wj, barrier);
             if constexpr (ANTI) leg_observe<PK>(gb, wj, barrier);
         };
+        auto advance = [&](double z, int j) { advance_w(z, ASIAN ? w[j + 1] : 0.0); };
         if constexpr (RNG == OFG_RNG_FAST) {
             int j = 0;
             for (; j + 3 < steps; j += 4) {
@@ -99,29 +107,35 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
                 if (j + 2 < steps) advance(z2, j + 2);
             }
         } else if constexpr (REF) {
-            // the reference's draws REF_AK at a time (the inverse CDF with
-            // its tail region deferred), applied in order
-            for (int j = 0; j < steps; j += REF_AK) {
-                double p[REF_AK], z[REF_AK];
+            // the reference's draws AK at a time (the inverse CDF with
+            // its tail region evaluated once per wave), applied in order; a
+            // block's observation weights are read together (the weights
+            // array is padded past its last option)
+            constexpr int AK = REF_AK;
+            for (int j = 0; j < steps; j += AK) {
+                double z[AK], wv[AK];
+                acklam_block_wave<AK>(
+                    [&](int i) { return j + i < steps ? uniform_open01(rng.next_f64()) : 0.5; },
+                    z, slots);
 #pragma unroll
-                for (int i = 0; i < REF_AK; ++i)
-                    p[i] = j + i < steps ? uniform_open01(rng.next_f64()) : 0.5;
-                acklam_block<REF_AK>(p, z);
+                for (int i = 0; i < AK; ++i) wv[i] = ASIAN ? w[j + i + 1] : 0.0;
 #pragma unroll
-                for (int i = 0; i < REF_AK; ++i)
-                    if (j + i < steps) advance(z[i], j + i);
+                for (int i = 0; i < AK; ++i)
+                    if (j + i < steps) advance_w(z[i], wv[i]);
             }
         } else {
             for (int j = 0; j < steps; ++j) advance(ext.z1(j), j);
         }
     } else {
         double va = o.v0, vb = o.v0;
-        auto advance = [&](double z1, double z2, int j) {
-            heston_step<PK, RNG != OFG_RNG_FAST>(ga, va, o, z1, z2, tab);
-            if constexpr (ANTI) heston_step<PK, RNG != OFG_RNG_FAST>(gb, vb, o, -z1, -z2, tab);
-            const double wj = ASIAN ? w[j + 1] : 0.0;
+        auto advance_w = [&](double z1, double z2, double wj) {
+            heston_step<PK, RNG != OFG_RNG_FAST, REF>(ga, va, o, z1, z2, tab);
+            if constexpr (ANTI) heston_step<PK, RNG != OFG_RNG_FAST, REF>(gb, vb, o, -z1, -z2, tab);
             leg_observe<PK>(ga, wj, barrier);
             if constexpr (ANTI) leg_observe<PK>(gb, wj, barrier);
+        };
+        auto advance = [&](double z1, double z2, int j) {
+            advance_w(z1, z2, ASIAN ? w[j + 1] : 0.0);
         };
         if constexpr (RNG == OFG_RNG_FAST) {
             int j = 0;
@@ -137,16 +151,20 @@ __device__ __forceinline__ void simulate_sample(const OptDev &o, int64_t sample,
                 advance(z0, z1, j);
             }
         } else if constexpr (REF) {
-            // z1, z2 of REF_AK / 2 steps per block, in the reference's order
-            for (int j = 0; j < steps; j += REF_AK / 2) {
-                double p[REF_AK], z[REF_AK];
+            // z1, z2 of AK / 2 steps per block, in the reference's order
+            constexpr int AK = REF_AK;
+            for (int j = 0; j < steps; j += AK / 2) {
+                double z[AK], wv[AK / 2];
+                acklam_block_wave<AK>(
+                    [&](int i) {
+                        return j + i / 2 < steps ? uniform_open01(rng.next_f64()) : 0.5;
+                    },
+                    z, slots);
 #pragma unroll
-                for (int i = 0; i < REF_AK; ++i)
-                    p[i] = j + i / 2 < steps ? uniform_open01(rng.next_f64()) : 0.5;
-                acklam_block<REF_AK>(p, z);
+                for (int i = 0; i < AK / 2; ++i) wv[i] = ASIAN ? w[j + i + 1] : 0.0;
 #pragma unroll
-                for (int i = 0; i < REF_AK / 2; ++i)
-                    if (j + i < steps) advance(z[2 * i], z[2 * i + 1], j + i);
+                for (int i = 0; i < AK / 2; ++i)
+                    if (j + i < steps) advance_w(z[2 * i], z[2 * i + 1], wv[i]);
             }
         } else {
             for (int j = 0; j < steps; ++j) advance(ext.z1(j), ext.z2(j), j);
@@ -168,13 +186,13 @@ struct Acc {
 
 template <int MODEL, int PK, int RNG, bool ANTI>
 __device__ void run_item(const OptDev &o, const WorkItem &wi, int lane, const LaunchArgs &a,
-                         const double *tab, Acc &acc) {
-    const double *__restrict__ w = a.obs_w + o.obs_offset;
+                         const double *tab, double *slots, Acc &acc) {
+    const cdouble *w = (const cdouble *)(a.obs_w + o.obs_offset);
     for (int64_t base = wi.begin; base < wi.end; base += WAVE) {
         const int64_t sample = base + lane;
         if (sample < wi.end) {
             double x, y;
-            simulate_sample<MODEL, PK, RNG, ANTI>(o, sample, w, a, tab, x, y);
+            simulate_sample<MODEL, PK, RNG, ANTI>(o, sample, w, a, tab, slots, x, y);
             acc.sx += x;
             acc.sx2 += x * x;
             acc.sy += y;
@@ -188,22 +206,29 @@ __device__ void run_item(const OptDev &o, const WorkItem &wi, int lane, const La
 
 template <int MODEL, int RNG, bool ANTI>
 __device__ __forceinline__ void dispatch_kind(const OptDev &o, const WorkItem &wi, int lane,
-                                              const LaunchArgs &a, const double *tab, Acc &acc) {
+                                              const LaunchArgs &a, const double *tab,
+                                              double *slots, Acc &acc) {
     switch (o.kind) {
-    case PK_VANILLA: run_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
-    case PK_ASIAN: run_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
-    case PK_ASIAN_LOG: run_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
-    case PK_BARRIER: run_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
-    default: run_item<MODEL, PK_BARRIER_DOWN, RNG, ANTI>(o, wi, lane, a, tab, acc); break;
+    case PK_VANILLA: run_item<MODEL, PK_VANILLA, RNG, ANTI>(o, wi, lane, a, tab, slots, acc); break;
+    case PK_ASIAN: run_item<MODEL, PK_ASIAN, RNG, ANTI>(o, wi, lane, a, tab, slots, acc); break;
+    case PK_ASIAN_LOG:
+        run_item<MODEL, PK_ASIAN_LOG, RNG, ANTI>(o, wi, lane, a, tab, slots, acc);
+        break;
+    case PK_BARRIER: run_item<MODEL, PK_BARRIER, RNG, ANTI>(o, wi, lane, a, tab, slots, acc); break;
+    default:
+        run_item<MODEL, PK_BARRIER_DOWN, RNG, ANTI>(o, wi, lane, a, tab, slots, acc);
+        break;
     }
 }
 
 // Minimum waves per SIMD asked of the register allocator: the plain fast GBM
 // and Heston loops fit 128 VGPRs (4 waves) without spilling (Heston takes 129
-// left free).
+// left free); the reference-draw Heston loop is held to 168 (3 waves), which
+// measured faster than its unconstrained 177 (2 waves).
 template <int RNG, int MODEL, bool ANTI>
 constexpr int price_waves_per_eu() {
-    return (RNG == OFG_RNG_FAST && !ANTI) ? 4 : 1;
+    constexpr bool REF = RNG == OFG_RNG_REFERENCE || RNG == OFG_RNG_PCG64;
+    return (RNG == OFG_RNG_FAST && !ANTI) ? 4 : (REF && MODEL == OFG_MODEL_HESTON) ? 3 : 1;
 }
 template <int RNG, int MODEL, bool ANTI, int BLOCK>
 constexpr int price_waves_target() {
@@ -215,12 +240,22 @@ template <int RNG, int MODEL, bool ANTI, int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
     price_waves_target<RNG, MODEL, ANTI, BLOCK>()))) void price_kernel(LaunchArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
-    // log table of the fast-mode Box-Muller, staged once per block in LDS
-    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    // the fast mode's tables (log, exp2, sin/cos), or the reference draws'
+    // 2^(j/256) table of the path exponentials, staged once per block in LDS
+    constexpr bool REF = RNG == OFG_RNG_REFERENCE || RNG == OFG_RNG_PCG64;
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : REF ? 256 : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
         for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += BLOCK) tab[i] = OFG_FAST_TABLE[i];
         __syncthreads();
+    } else if constexpr (REF) {
+        for (int i = threadIdx.x; i < 256; i += BLOCK) tab[i] = OFG_FAST_TABLE[OFG_TAB_EXP2 + i];
+        __syncthreads();
     }
+    // reference draws: each wave's slots for the tail values of one
+    // inverse-CDF block (acklam_block_wave)
+    constexpr int AK = REF_AK;
+    __shared__ double tail_slots[REF ? BLOCK * AK : 1];
+    double *slots = REF ? tail_slots + (threadIdx.x / WAVE) * WAVE * AK : tail_slots;
     const uint32_t span = (uint32_t)(a.order_end - a.order_begin);
     for (;;) {
         // Every lane executes the atomic (lane 0 adds 1, the others 0) so the
@@ -234,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
         const WorkItem wi = a.items[it];
         const OptDev o = a.opts[wi.option];
         Acc acc{0.0, 0.0, 0.0, 0.0, 0.0};
-        dispatch_kind<MODEL, RNG, ANTI>(o, wi, lane, a, tab, acc);
+        dispatch_kind<MODEL, RNG, ANTI>(o, wi, lane, a, tab, slots, acc);
         const double sx = wave_sum(acc.sx);
         const double sx2 = wave_sum(acc.sx2);
         const double sy = wave_sum(acc.sy);

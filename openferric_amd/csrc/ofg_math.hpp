@@ -115,12 +115,13 @@ OFG_HD void xoshiro256_unstep(uint64_t &s0, uint64_t &s1, uint64_t &s2, uint64_t
 // stream_seed (fast_rng.rs:202-205), wrapping.
 OFG_HD uint64_t stream_seed(uint64_t base, uint64_t index) { return base + index * 7919ull; }
 
-// uniform_open01 (fast_rng.rs:218-221): clamp to [eps, 1 - eps].
+// uniform_open01 (fast_rng.rs:218-221): clamp to [eps, 1 - eps]. u is a
+// next_f64 draw (finite, in [0, 1)), so the reference's two compare-selects
+// are exactly a max and a min.
 OFG_HD double uniform_open01(double u) {
     const double lo = 2.220446049250313e-16;
     const double hi = 1.0 - 2.220446049250313e-16;
-    u = u < lo ? lo : u;
-    return u > hi ? hi : u;
+    return fmin(fmax(u, lo), hi);
 }
 
 // ------------------------------------------------------- elementary math --
@@ -183,6 +184,26 @@ constexpr double OFG_AK_P_HIGH = 1.0 - OFG_AK_P_LOW;
 
 OFG_HD bool acklam_is_tail(double p) { return p < OFG_AK_P_LOW || p > OFG_AK_P_HIGH; }
 
+// n / d, correctly rounded, for the operands of the Acklam rationals: d in
+// [2.6e-3, 1] (central) or [36, 447] (tail), n finite with |n| in
+// [2^-56, 2^12] or n = +0. On the device this is the Newton-Markstein
+// sequence of the compiler's IEEE f64 division (reciprocal, two Newton
+// refinements, quotient and one residual correction) without its
+// v_div_scale range scaling, v_div_fmas rescale and v_div_fixup special
+// cases, which leave operands in these ranges unchanged: the same
+// instructions on the same values, so the same bits as n / d.
+OFG_HD double acklam_div(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    const double q = n * r;
+    return fma(fma(-d, q, n), r, q);
+#else
+    return n / d;
+#endif
+}
+
 OFG_HD double acklam_tail(double p) {
     const bool lower = p < OFG_AK_P_LOW;
     const double pt = lower ? p : 1.0 - p;
@@ -196,7 +217,7 @@ OFG_HD double acklam_tail(double p) {
     den = fma(den, q, OFG_AK_D2);
     den = fma(den, q, OFG_AK_D3);
     den = fma(den, q, 1.0);
-    const double r = num / den;
+    const double r = acklam_div(num, den);
     return lower ? r : -r;
 }
 
@@ -213,7 +234,7 @@ OFG_HD double acklam_central(double p) {
     den = fma(den, r, OFG_AK_B3);
     den = fma(den, r, OFG_AK_B4);
     den = fma(den, r, 1.0);
-    return num * q / den;
+    return acklam_div(num * q, den);
 }
 
 OFG_HD double acklam_inv_cdf(double p) {
@@ -247,6 +268,58 @@ __device__ __forceinline__ void acklam_block(const double (&p)[K], double (&z)[K
         for (int k = 0; k < K; ++k) z[k] = j == k ? t : z[k];
     }
 }
+
+#if defined(__HIP__)  // wave-level code: HIP compilations only (not the host tests' g++ builds)
+// Orders this wave's LDS accesses: its writes before the barrier are seen by
+// its reads after it (LDS executes one wave's instructions in order; this
+// keeps the compiler from moving accesses across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// acklam_block with the tail region evaluated once per wave instead of once
+// per lane: every lane evaluates the central rational for its K values; the
+// p of the whole wave's tail values go to consecutive slots of the wave's
+// LDS buffer (value k of every lane, lane order, then k + 1, ...), lanes
+// 0..63 invert them 64 at a time, and each lane reads its values back from
+// their slots. A block of K = 8 holds about 25 tail values per wave, so one
+// round of tail evaluations nearly always serves the block, against about
+// 2.3 rounds when each lane inverts its own tail values in turn. Bit for bit
+// acklam_inv_cdf of each p[k]. slots: this wave's 64 K doubles of LDS.
+// draw(k) supplies p[k], k = 0 .. K-1 in order.
+template <int K, class Draw>
+__device__ __forceinline__ void acklam_block_wave(Draw &&draw, double (&z)[K], double *slots) {
+    uint32_t slot[K];     // this lane's slot of value k, ~0u for a central value
+    uint32_t total = 0;   // wave-uniform count of tail values
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double p = draw(k);
+        const bool t = acklam_is_tail(p);
+        z[k] = acklam_central(p);
+        const uint64_t m = __ballot(t);
+        // tail lanes below this one at k, after the earlier positions' values
+        const uint32_t s = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, total));
+        slot[k] = t ? s : ~0u;
+        if (t) slots[s] = p;
+        total += (uint32_t)__popcll(m);
+    }
+    if (total == 0) return;
+    wave_lds_sync();
+    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    for (uint32_t r = 0; r < total; r += 64) {
+        const uint32_t i = r + lane;
+        if (i < total) slots[i] = acklam_tail(slots[i]);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (slot[k] != ~0u) z[k] = slots[slot[k]];
+    wave_lds_sync();   // the buffer is reused by the next block
+}
+#endif  // __HIP__
 
 // n_steps path steps of PER reference normals each (Heston: z1 then z2),
 // drawn K at a time from the stream and inverted by acklam_block; f(j, z1,
@@ -504,20 +577,23 @@ OFG_HD void normals4(Rng &rng, double &z0, double &z1, double &z2, double &z3, c
 // j = k mod 256 and the binary exponent floor(k/256), and f = y - k is exact
 // (|f| <= 1/2); 2^(f/256) - 1 is its Taylor series through f^4 (truncation
 // 3.8e-17 relative).
-OFG_HD double exp2_256(double y, const double *tab) {
+// t256: the 256 entries 2^(j/256).
+OFG_HD double exp2_256t(double y, const double *t256) {
     const double kd = y + 6755399441055744.0;
     const double k = kd - 6755399441055744.0;
     const double f = y - k;
     uint64_t kb;
     __builtin_memcpy(&kb, &kd, sizeof kb);
     const int ki = (int)(uint32_t)kb;
-    const double t = tab[OFG_TAB_EXP2 + (ki & 255)];
+    const double t = t256[ki & 255];
     // (ln2/256)^n / n!, n = 4, 3, 2, 1
     double q = fma(f, 2.239395190875157e-12, 3.3083026805413713e-09);
     q = fma(q, f, 3.6655655969101062e-06);
     q = fma(q, f, 0.0027076061740622863);
     return ldexp(fma(t, f * q, t), ki >> 8);
 }
+
+OFG_HD double exp2_256(double y, const double *tab) { return exp2_256t(y, tab + OFG_TAB_EXP2); }
 
 // ln2 / 256 and its inverse: the fast mode's exponent scale; the bound on
 // |y| that exp2_256 accepts.

@@ -46,14 +46,22 @@ __device__ __forceinline__ void leg_observe(Leg &g, double w, double barrier) {
 
 // Growth factor exp(x) of one GBM step (simulation.rs:67-70), x = diffusion z
 // + drift, and x itself. FAST evaluates it as 2^(y/256) with y = x 256/ln2
-// formed from the pre-scaled constants (one rounding, as for x).
-template <bool FAST>
+// formed from the pre-scaled constants (one rounding, as for x). Otherwise x
+// is the reference's fma and exp(x) is exp_fp64(x), or with TABLE
+// 2^(y/256) of y = x 256/ln2 (tab: the 256 entries 2^(j/256)); both are
+// within about an ulp of exp(x).
+template <bool FAST, bool TABLE = false>
 __device__ __forceinline__ double gbm_growth(const OptDev &o, double z, const double *tab,
                                              double &x) {
     if constexpr (FAST) {
         const double y = fma(o.diffusion2, z, o.drift2);
         x = y * OFG_LN2_256;
         return exp2_256(y, tab);
+    } else if constexpr (TABLE) {
+        x = fma(o.diffusion, z, o.drift);
+        // past |y| = 2^30 the growth is 0 or inf either way
+        return exp2_256t(fmin(fmax(x * OFG_INV_LN2_256, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX),
+                         tab);
     } else {
         x = fma(o.diffusion, z, o.drift);
         return exp_fp64(x);
@@ -61,10 +69,10 @@ __device__ __forceinline__ double gbm_growth(const OptDev &o, double z, const do
 }
 
 // One GBM step: s *= exp(fma(diffusion, z, drift)).
-template <int PK, bool FAST>
+template <int PK, bool FAST, bool TABLE = false>
 __device__ __forceinline__ void gbm_step(Leg &g, const OptDev &o, double z, const double *tab) {
     double x;
-    g.s *= gbm_growth<FAST>(o, z, tab, x);
+    g.s *= gbm_growth<FAST, TABLE>(o, z, tab, x);
     if constexpr (PK == PK_ASIAN_LOG) g.ls += x;
 }
 
@@ -83,7 +91,7 @@ struct HestonParts {
     double vp, sv, zs, sn, vn;
 };
 
-template <bool REF>
+template <bool REF, bool TABLE = false>
 __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o, double z1,
                                              double z2, double &s_next, double &v_next,
                                              double &growth, double &e, const double *tab,
@@ -95,7 +103,14 @@ __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o
         zs = o.rho * z1 + o.rho_c * z2;
         vn = v + o.kappa * (o.theta - v_pos) * o.dt + o.xi * sv * o.sqrt_dt * z1;
         e = (o.mu - 0.5 * v_pos) * o.dt + sv * o.sqrt_dt * zs;
-        growth = exp_fp64(e);
+        // e is bounded by the clamp below only through the growth: keep the
+        // table exp's argument inside its domain (past |y| = 2^30 the growth
+        // is 0 or inf either way)
+        if constexpr (TABLE)
+            growth = exp2_256t(fmin(fmax(e * OFG_INV_LN2_256, -OFG_EXP2_256_YMAX),
+                                    OFG_EXP2_256_YMAX), tab);
+        else
+            growth = exp_fp64(e);
     } else {
         sv = v_pos > 0.0 ? sqrt_pos(v_pos) : 0.0;
         zs = fma(o.rho, z1, o.rho_c * z2);
@@ -117,19 +132,19 @@ __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o
     hp.vn = vn;
 }
 
-template <bool REF>
+template <bool REF, bool TABLE = false>
 __device__ __forceinline__ void heston_euler(double s, double v, const OptDev &o, double z1,
                                              double z2, double &s_next, double &v_next,
                                              double &growth, double &e, const double *tab) {
     HestonParts hp;
-    heston_euler<REF>(s, v, o, z1, z2, s_next, v_next, growth, e, tab, hp);
+    heston_euler<REF, TABLE>(s, v, o, z1, z2, s_next, v_next, growth, e, tab, hp);
 }
 
-template <int PK, bool REF>
+template <int PK, bool REF, bool TABLE = false>
 __device__ __forceinline__ void heston_step(Leg &g, double &v, const OptDev &o, double z1,
                                             double z2, const double *tab) {
     double s_next, v_next, growth, e;
-    heston_euler<REF>(g.s, v, o, z1, z2, s_next, v_next, growth, e, tab);
+    heston_euler<REF, TABLE>(g.s, v, o, z1, z2, s_next, v_next, growth, e, tab);
     if constexpr (PK == PK_ASIAN_LOG) {
         const bool floor_hit = !(g.s * growth >= 1e-12);
         g.ls = floor_hit ? -27.631021115928547 : g.ls + e;   // ln(1e-12) at the floor

@@ -306,6 +306,7 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
         p.model_begin[m] = (int32_t)(lo - p.order.begin());
         p.model_end[m] = (int32_t)(hi - p.order.begin());
     }
+    p.obs_w.insert(p.obs_w.end(), OBS_PAD, 0.0);
     return p;
 }
 

@@ -17,6 +17,12 @@
 
 namespace ofg_plan {
 
+// Zero observation weights after the last option's: the reference-draw
+// kernels read a block's weights (up to 16 steps) together, past the end of
+// an option's own weights at its last block (the values read there are
+// never applied).
+constexpr size_t OBS_PAD = 16;
+
 // Host-side plan of one pricing launch.
 struct Plan {
     std::vector<ofg::OptDev> opts;
