@@ -282,13 +282,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 // acklam_block with the tail region evaluated once per wave instead of once
 // per lane: every lane evaluates the central rational for its K values; the
 // p of the whole wave's tail values go to consecutive slots of the wave's
-// LDS buffer (value k of every lane, lane order, then k + 1, ...), lanes
-// 0..63 invert them 64 at a time, and each lane reads its values back from
-// their slots. A block of K = 8 holds about 25 tail values per wave, so one
-// round of tail evaluations nearly always serves the block, against about
-// 2.3 rounds when each lane inverts its own tail values in turn. Bit for bit
-// acklam_inv_cdf of each p[k]. slots: this wave's 64 K doubles of LDS.
-// draw(k) supplies p[k], k = 0 .. K-1 in order.
+// LDS buffer (value k of every lane, lane order, then k + 1, ...), the
+// calling lanes invert them one each per round, and each lane reads its
+// values back from their slots. A block of K = 8 holds about 25 tail values
+// per wave, so one round of tail evaluations nearly always serves the block,
+// against about 2.3 rounds when each lane inverts its own tail values in
+// turn. Bit for bit acklam_inv_cdf of each p[k]. draw(k) supplies p[k],
+// k = 0 .. K-1 in order; slots is this wave's 64 K doubles of LDS. Any
+// subset of the wave's lanes may make the call, all of them together.
 template <int K, class Draw>
 __device__ __forceinline__ void acklam_block_wave(Draw &&draw, double (&z)[K], double *slots) {
     uint32_t slot[K];     // this lane's slot of value k, ~0u for a central value
@@ -308,9 +309,14 @@ __device__ __forceinline__ void acklam_block_wave(Draw &&draw, double (&z)[K], d
     }
     if (total == 0) return;
     wave_lds_sync();
-    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-    for (uint32_t r = 0; r < total; r += 64) {
-        const uint32_t i = r + lane;
+    // the lanes executing this call (the tail of a sample range leaves a
+    // wave partly idle) share the rounds by their rank among themselves
+    const uint64_t act = __ballot(true);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+    const uint32_t n_act = (uint32_t)__popcll(act);
+    for (uint32_t r = 0; r < total; r += n_act) {
+        const uint32_t i = r + rank;
         if (i < total) slots[i] = acklam_tail(slots[i]);
     }
     wave_lds_sync();

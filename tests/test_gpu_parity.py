@@ -229,3 +229,17 @@ def test_fast_mode_heston_paths_equal_host_build(gpu, shim, steps):
                          for i in range(n)])
         host = np.maximum(host - 1e-9, 0.0)
         np.testing.assert_allclose(xs, host, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("model", ["gbm", "heston"])
+@pytest.mark.parametrize("num_paths", [1, 2, 3, 7, 65, 130])
+def test_ragged_waves_reference_draws_per_path(oracle, gpu, model, num_paths):
+    """A sample range that leaves a wave mostly idle (one to seven busy
+    lanes, or a last chunk of one or two): the tail values of a draw block
+    are inverted by the busy lanes only, and with 400 steps a block often
+    holds more tail values than busy lanes."""
+    kw = dict(GBM_EURO, num_paths=num_paths, num_steps=400, seed=1234 + num_paths)
+    if model == "heston":
+        kw.update(model="heston", kappa=1.5, theta=0.04, xi=0.5, rho=-0.7, v0=0.04)
+    ref, xs, ys, _ = _per_path(oracle, kw)
+    _check(ref, xs, ys, 100.0)
