@@ -440,6 +440,22 @@ int32_t ofg_spread_price(ofg_ctx *ctx, const ofg_spread_option *opts, size_t n,
 int32_t ofg_sample_payoffs(ofg_ctx *ctx, const ofg_option *opt,
                            const ofg_engine_cfg *cfg, double *xs, double *ys);
 
+/* The reference's per-sample normal streams, drawn on the GPU. Replaces the
+ * draw loop of MonteCarloEngine::run's fold body (src/mc/simulation.rs:
+ * 214-225: FastRng::from_seed(kind, resolve_stream_seed(seed, i)), then
+ * sample_standard_normal per value; fill_standard_normals,
+ * src/math/fast_rng.rs:228-240, over a whole stream): stream i = 0 ..
+ * n_streams-1 is seeded with stream_seed(seed, first_stream + i) and yields
+ * n_per_stream standard normals, Acklam's inverse CDF of the clamped uniform
+ * (a Heston path's z1, z2 interleave in one stream). rng_mode
+ * OFG_RNG_REFERENCE (xoshiro256++) or OFG_RNG_PCG64. out: host array of
+ * n_streams * n_per_stream doubles, out[i * n_per_stream + j], or with
+ * step_major out[j * n_streams + i] (the layout OFG_RNG_EXTERNAL reads).
+ * kernel_ms (may be NULL) receives the device time. */
+int32_t ofg_fill_normals(ofg_ctx *ctx, int32_t rng_mode, uint64_t seed, uint64_t first_stream,
+                         int64_t n_streams, int64_t n_per_stream, int32_t step_major,
+                         double *out, double *kernel_ms);
+
 /* Limits of the fixed device region ofg_price_noalloc prices in. */
 #define OFG_NOALLOC_MAX_OPTIONS 4096
 #define OFG_NOALLOC_MAX_WORK_ITEMS 262144

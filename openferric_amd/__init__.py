@@ -17,7 +17,7 @@ from .engines import (BatchPricer, GpuArithmeticAsianMC, GpuMonteCarloPricingEng
                       mc_european_with_arena, LongstaffSchwartzEngine, LsmDynamics,
                       lsm_last_timing, lsm_option_struct, lsm_price_raw, stream_jump,
                       mc_european_qmc, mc_european_qmc_with_seed, qmc_price_raw,
-                      option_struct, price_noalloc_raw, price_raw, sample_payoffs,
+                      option_struct, price_noalloc_raw, price_raw, sample_payoffs, fill_normals,
                       european_exact_raw)
 from .multi import (Autocallable, AutocallableSensitivities, BasketCopula, BasketOption,
                     FactorCorrelationModel, price_basket_mc_with_factor_model,
@@ -51,7 +51,7 @@ __all__ = [
     "GpuArithmeticAsianMC", "GpuMonteCarloPricingEngine", "HestonAadConfig",
     "HestonMonteCarloPricingEngine", "RngMode", "VarianceReduction", "autotune", "engine_cfg",
     "heston_price_delta_aad", "mc_european_pathwise_aad", "option_struct", "price_raw", "price_noalloc_raw", "european_exact_raw",
-    "sample_payoffs", "DeviceError", "MonteCarloGreeksEngine", "mc_european_parallel",
+    "sample_payoffs", "fill_normals", "DeviceError", "MonteCarloGreeksEngine", "mc_european_parallel",
     "mc_european_sequential", "mc_european_with_arena", "GreekMethod", "BermudanOption",
     "BermudanLsmOutput", "ExerciseBoundaryPoint", "LongstaffSchwartzEngine", "LsmDynamics",
     "lsm_last_timing", "lsm_option_struct", "lsm_price_raw", "stream_jump",

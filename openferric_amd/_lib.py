@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = [
     "ofg_dsl_compile", "ofg_dsl_free", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
     "ofg_dsl_price", "ofg_price_noalloc", "ofg_finite_difference_greeks",
     "ofg_bump_and_reprice", "ofg_european_exact", "ofg_spread_price",
-    "ofg_inject_alloc_failure",
+    "ofg_inject_alloc_failure", "ofg_fill_normals",
 ]
 
 
@@ -288,6 +288,10 @@ def load() -> C.CDLL:
         L.ofg_lsm_price.restype = C.c_int32
         L.ofg_stream_jump.argtypes = [C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(C.c_uint64)]
         L.ofg_stream_jump.restype = C.c_int32
+        L.ofg_fill_normals.argtypes = [pv, C.c_int32, C.c_uint64, C.c_uint64, C.c_int64,
+                                       C.c_int64, C.c_int32, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        L.ofg_fill_normals.restype = C.c_int32
         L.ofg_inject_alloc_failure.argtypes = [C.c_int32]
         L.ofg_inject_alloc_failure.restype = None
         L.ofg_lsm_last_timing.argtypes = [pv, C.POINTER(C.c_double), C.POINTER(C.c_double),

@@ -15,6 +15,7 @@
 #include "ofg_pathwise.hpp"
 #include "ofg_host.hpp"
 #include "ofg_plan.hpp"
+#include "ofg_normals.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -890,7 +891,7 @@ void ofg_destroy(ofg_ctx *ctx) {
     spread_release(ctx);
     for (DevBuf *b : {&ctx->opts, &ctx->obs_w, &ctx->items, &ctx->order, &ctx->item_first,
                       &ctx->item_count, &ctx->partials, &ctx->outs, &ctx->head, &ctx->xs, &ctx->ys,
-                      &ctx->n1, &ctx->n2, &ctx->adj})
+                      &ctx->n1, &ctx->n2, &ctx->adj, &ctx->fill})
         b->release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
@@ -913,6 +914,44 @@ int32_t ofg_finite_difference_greeks(ofg_ctx *ctx, const ofg_option *opts, size_
 int32_t ofg_bump_and_reprice(ofg_ctx *ctx, const ofg_option *opts, size_t n,
                              const ofg_engine_cfg *cfg, const ofg_fd_bumps *bumps, double *out) {
     return guarded([&]() { return bump_reprice_impl(ctx, opts, n, cfg, bumps, out); });
+}
+
+int32_t ofg_fill_normals(ofg_ctx *ctx, int32_t rng_mode, uint64_t seed, uint64_t first_stream,
+                         int64_t n_streams, int64_t n_per_stream, int32_t step_major,
+                         double *out, double *kernel_ms) {
+    return guarded([&]() -> int32_t {
+        if (!ctx || (!out && n_streams > 0 && n_per_stream > 0))
+            fail(OFG_INVALID_INPUT, "null argument");
+        if (rng_mode != OFG_RNG_REFERENCE && rng_mode != OFG_RNG_PCG64)
+            fail(OFG_INVALID_INPUT, "ofg_fill_normals draws OFG_RNG_REFERENCE or OFG_RNG_PCG64 streams");
+        if (n_streams < 0 || n_per_stream < 0)
+            fail(OFG_INVALID_INPUT, "stream counts must be >= 0");
+        if (kernel_ms) *kernel_ms = 0.0;
+        if (n_streams == 0 || n_per_stream == 0) return OFG_OK;
+        if (n_streams > ((int64_t)1 << 40) || n_per_stream > ((int64_t)1 << 40) / n_streams)
+            fail(OFG_INVALID_INPUT, "at most 2^40 normals per call");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        const size_t bytes = (size_t)n_streams * (size_t)n_per_stream * sizeof(double);
+        ctx->fill.ensure(bytes);
+        ofg::NormalsArgs a{};
+        a.out = ctx->fill.as<double>();
+        a.seed = seed;
+        a.first = first_stream;
+        a.n_streams = n_streams;
+        a.n_per_stream = n_per_stream;
+        a.step_major = step_major ? 1 : 0;
+        hipStream_t s = ctx->stream;
+        hip_check(hipEventRecord(ctx->ev0, s), "event");
+        hip_check(ofg::launch_normals(a, rng_mode, s), "normals kernel launch");
+        hip_check(hipEventRecord(ctx->ev1, s), "event");
+        hip_check(hipMemcpyAsync(out, a.out, bytes, hipMemcpyDeviceToHost, s), "D2H normals");
+        hip_check(hipStreamSynchronize(s), "normals kernel execution");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
+        if (kernel_ms) *kernel_ms = ms;
+        return OFG_OK;
+    });
 }
 
 int32_t ofg_price_noalloc(ofg_ctx *ctx, const ofg_option *opts, size_t n,

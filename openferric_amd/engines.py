@@ -227,6 +227,26 @@ def price_noalloc_raw(opts: Sequence[_lib.OfgOption], cfg: _lib.OfgEngineCfg,
     return list(out)
 
 
+def fill_normals(seed: int, n_streams: int, n_per_stream: int, first_stream: int = 0,
+                 rng_mode: "RngMode" = None, step_major: bool = False,
+                 device: int = 0) -> np.ndarray:
+    """The reference's per-sample normal streams drawn on the GPU
+    (ofg_fill_normals): row i holds the n_per_stream normals of the stream
+    seeded stream_seed(seed, first_stream + i), as fill_standard_normals
+    draws them (src/math/fast_rng.rs:228-240). Shape [n_streams,
+    n_per_stream], or [n_per_stream, n_streams] with step_major."""
+    L = _lib.load()
+    ctx = _lib.context(device)
+    mode = (rng_mode or RngMode.Reference).value
+    shape = (n_per_stream, n_streams) if step_major else (n_streams, n_per_stream)
+    out = np.zeros(shape)
+    ms = C.c_double()
+    _raise(L.ofg_fill_normals(ctx, mode, seed % (1 << 64), first_stream, n_streams, n_per_stream,
+                              1 if step_major else 0,
+                              out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ms)))
+    return out
+
+
 def sample_payoffs(opt: _lib.OfgOption, cfg: _lib.OfgEngineCfg, antithetic: bool,
                    device: int = 0, want_y: bool = True) -> Tuple[np.ndarray, Optional[np.ndarray]]:
     """Per-sample (x, y) of one option via ofg_sample_payoffs."""
