@@ -36,12 +36,14 @@ def _build():
     deps = SOURCES + [os.path.join(CSRC, h) for h in HEADERS]
     if os.path.exists(EXE) and all(os.path.getmtime(d) <= os.path.getmtime(EXE) for d in deps):
         return EXE
+    # per-process output name: parallel test workers may build at once
+    tmp = f"{EXE}.tmp{os.getpid()}"
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer",
            "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
-           "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", *SOURCES, "-o", EXE + ".tmp"]
+           "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", *SOURCES, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-4000:]
-    os.replace(EXE + ".tmp", EXE)
+    os.replace(tmp, EXE)
     return EXE
 
 
