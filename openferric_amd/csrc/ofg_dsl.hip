@@ -46,22 +46,28 @@ __device__ __forceinline__ double wave_max(double v) {
 // A GBM step's factor exp(p0 + p1 z): the reference's exp on its own and
 // external draws; the table-driven 2^(y/256) of the fast mode otherwise
 // (fast draws are checked statistically, like every fast-mode kernel).
-template <bool FAST>
+// With TABLE the reference's exponent p0 + p1 z goes through the same
+// 2^(j/256) table (tab: its 256 entries), within about an ulp of exp.
+template <bool FAST, bool TABLE = false>
 __device__ __forceinline__ double gbm_factor(double p0, double p1, double z, const double *tab) {
     if constexpr (FAST) {
         double y = (p0 + p1 * z) * OFG_INV_LN2_256;
         y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
         return exp2_256(y, tab);
+    } else if constexpr (TABLE) {
+        double y = (p0 + p1 * z) * OFG_INV_LN2_256;
+        y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
+        return exp2_256t(y, tab);
     } else {
         return exp_fp64(p0 + p1 * z);
     }
 }
 
 // AssetStepper::step (engine.rs:95-122)
-template <bool FAST>
+template <bool FAST, bool TABLE = false>
 __device__ __forceinline__ double step_asset(int kind, double prev, double p0, double p1, double p2,
                                              double z, const double *tab) {
-    if (kind == DSL_GBM) return prev * gbm_factor<FAST>(p0, p1, z, tab);
+    if (kind == DSL_GBM) return prev * gbm_factor<FAST, TABLE>(p0, p1, z, tab);
     if (kind == DSL_SCHWARTZ) {
         const double log_prev = log(prev);
         const double log_next = p0 + p1 * (log_prev - p0) + p2 * z;
@@ -132,10 +138,16 @@ struct StepParams<MAXA, false> {
 
 // MAXA: the asset count rounded up to 1, 2, 4, 8 or 16 (registers per path).
 // Snapshot k of asset i goes to snap[(k n + i) stride].
-template <int RNG, int MAXA>
+// With WTAIL (reference draws) slots is this wave's 64 max(8, MAXA) doubles
+// of LDS for the draws' tail values (acklam_block_wave) and tab the 256
+// entries 2^(j/256) of the GBM factors; without (a kernel whose LDS holds
+// the programs' register files) each lane inverts its own tail values and
+// GBM factors take exp_fp64.
+template <int RNG, int MAXA, bool WTAIL = true>
 __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o,
                                               const StepParams<MAXA> &sp, int64_t p,
-                                              double *snap, int64_t stride, const double *tab) {
+                                              double *snap, int64_t stride, const double *tab,
+                                              double *slots) {
     const int n = o.n_assets;
     double s[MAXA];
 #pragma unroll
@@ -161,17 +173,11 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
     // wastes no normal
     double zpair = 0.0;
     bool have_pair = false;
-    for (int step = 1; step <= a.steps; ++step) {
+    // one step from its normals (zr: the reference draws of the step)
+    auto advance = [&](int step, const double *zr) {
         double c[MAXA];
 #pragma unroll
         for (int i = 0; i < MAXA; ++i) c[i] = 0.0;
-        double zr[MAXA];   // reference draws of the step, tail region deferred
-        if constexpr (RNG == OFG_RNG_REFERENCE) {
-            double pr[MAXA];
-#pragma unroll
-            for (int j = 0; j < MAXA; ++j) pr[j] = j < n ? uniform_open01(ref.next_f64()) : 0.5;
-            acklam_block<MAXA>(pr, zr);
-        }
 #pragma unroll
         for (int j = 0; j < MAXA; ++j) {
             if (j < n) {
@@ -198,14 +204,42 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
 #pragma unroll
         for (int i = 0; i < MAXA; ++i)
             if (i < n)
-                s[i] = step_asset<RNG == OFG_RNG_FAST>(sp.kind(i), s[i], sp.p0(i), sp.p1(i),
-                                                       sp.p2(i), c[i], tab);
+                s[i] = step_asset<RNG == OFG_RNG_FAST, WTAIL && RNG == OFG_RNG_REFERENCE>(
+                    sp.kind(i), s[i], sp.p0(i), sp.p1(i), sp.p2(i), c[i], tab);
         const int k = a.step_snap[step];
         if (k >= 0) {
 #pragma unroll
             for (int i = 0; i < MAXA; ++i)
                 if (i < n) snap[(int64_t)(k * n + i) * stride] = s[i];
         }
+    };
+    if constexpr (RNG == OFG_RNG_REFERENCE && WTAIL) {
+        // the draws of SPB steps per inverse-CDF block (in the reference's
+        // order: step by step, asset by asset), tail values inverted per wave
+        constexpr int SPB = MAXA <= 4 ? 8 / MAXA : 1;
+        for (int step = 1; step <= a.steps; step += SPB) {
+            double zr[SPB * MAXA];
+            acklam_block_wave<SPB * MAXA>(
+                [&](int j) {
+                    return (j % MAXA < n && step + j / MAXA <= a.steps)
+                               ? uniform_open01(ref.next_f64())
+                               : 0.5;
+                },
+                zr, slots);
+#pragma unroll
+            for (int q = 0; q < SPB; ++q)
+                if (step + q <= a.steps) advance(step + q, zr + q * MAXA);
+        }
+    } else if constexpr (RNG == OFG_RNG_REFERENCE) {
+        for (int step = 1; step <= a.steps; ++step) {
+            double pr[MAXA], zr[MAXA];   // each lane inverts its own tail values
+#pragma unroll
+            for (int j = 0; j < MAXA; ++j) pr[j] = j < n ? uniform_open01(ref.next_f64()) : 0.5;
+            acklam_block<MAXA>(pr, zr);
+            advance(step, zr);
+        }
+    } else {
+        for (int step = 1; step <= a.steps; ++step) advance(step, nullptr);
     }
 }
 
@@ -352,7 +386,7 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        simulate_path<RNG, MAXA>(a, o, sp, p, snap, DSL_BLOCK, tab);
+        simulate_path<RNG, MAXA, false>(a, o, sp, p, snap, DSL_BLOCK, tab, nullptr);
         const double x = evaluate_path(a, o, snap, DSL_BLOCK, regs + threadIdx.x, err);
         sum += x;
         sq += x * x;
@@ -383,9 +417,18 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
 // Split form, pass 1: the paths' snapshots, [snap][asset][path] per scenario.
 template <int RNG, int MAXA>
 __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
-    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    // the fast mode's tables, or the reference draws' 2^(j/256) table and
+    // each wave's slots for the tail values of an inverse-CDF block
+    constexpr bool REF = RNG == OFG_RNG_REFERENCE;
+    constexpr int AK = MAXA <= 4 ? 8 : MAXA;
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : REF ? 256 : 1];
+    __shared__ double tail_slots[REF ? DSL_BLOCK * AK : 1];
+    double *slots = tail_slots + (REF ? (threadIdx.x / WAVE) * WAVE * AK : 0);
     if constexpr (RNG == OFG_RNG_FAST) {
         for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += DSL_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    } else if constexpr (REF) {
+        for (int i = threadIdx.x; i < 256; i += DSL_BLOCK) tab[i] = OFG_FAST_TABLE[OFG_TAB_EXP2 + i];
         __syncthreads();
     }
     const int tile = blockIdx.x;
@@ -395,7 +438,7 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        simulate_path<RNG, MAXA>(a, o, sp, p, a.snaps + o.snap_off + p, o.paths, tab);
+        simulate_path<RNG, MAXA>(a, o, sp, p, a.snaps + o.snap_off + p, o.paths, tab, slots);
     }
 }
 
@@ -460,7 +503,10 @@ dsl_sim_markets_kernel(DslArgs a) {
         bool have_pair = false;
         for (int step = 1; step <= a.steps; ++step) {
             double z[MAXA], c0[MAXA], g0[MAXA];
-            double zr[MAXA];   // reference draws of the step, tail region deferred
+            // reference draws of the step, each lane inverting its own tail
+            // values (this kernel's registers are spoken for; the wave form's
+            // slot indices spill)
+            double zr[MAXA];
             if constexpr (RNG == OFG_RNG_REFERENCE) {
                 double pr[MAXA];
 #pragma unroll

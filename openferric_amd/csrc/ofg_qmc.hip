@@ -4,7 +4,8 @@
 // formed directly from its index (Gray-code identity, ofg_qmc.hpp) instead of
 // the reference's sequential next_into. The uniform map fma(x, 2^-64,
 // 2^-65), uniform_open01, the Acklam inverse CDF and the step
-// spot *= exp(fma(dt_vol, z, dt_drift)) follow the reference's operations.
+// spot *= exp(fma(dt_vol, z, dt_drift)) follow the reference's operations
+// (exp within about an ulp of the reference's).
 //
 // Work: a tile is one block of 256 threads over 256 * ppt consecutive point
 // indices of one option; tile partial sums (sum, sum of squares) go to the
@@ -18,7 +19,7 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int NWAVES = QMC_BLOCK / WAVE;
-constexpr int QMC_AK = 8;   // inverse-CDF values per deferred-tail block
+constexpr int QMC_AK = 8;   // inverse-CDF values per wave-tail block
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -29,11 +30,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 __global__ __launch_bounds__(QMC_BLOCK) void qmc_kernel(QmcArgs a) {
     __shared__ uint64_t xhi[NWAVES][QMC_SEG];
     __shared__ double red[NWAVES][2];
+    // each wave's slots for the tail values of an inverse-CDF block, and the
+    // 2^(j/256) table of the path exponentials
+    __shared__ double tail_slots[QMC_BLOCK * QMC_AK];
+    __shared__ double exp2_tab[256];
+    for (int i = threadIdx.x; i < 256; i += QMC_BLOCK) exp2_tab[i] = OFG_FAST_TABLE[OFG_TAB_EXP2 + i];
+    __syncthreads();
     const int tile = blockIdx.x;
     const int opt = a.tile_opt[tile];
     const QmcDev o = a.opts[opt];
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
+    double *slots = tail_slots + wave * WAVE * QMC_AK;
     const bool put = o.put != 0;
     const uint64_t *dirs = a.dirs + o.dim_offset * 64;
     const uint64_t *lows = a.lows + o.dim_offset * 64;
@@ -63,20 +71,27 @@ __global__ __launch_bounds__(QMC_BLOCK) void qmc_kernel(QmcArgs a) {
             }
             __syncthreads();
             // QMC_AK dimensions at a time: their uniforms, the inverse CDF
-            // with the tail region deferred, then the steps in order
+            // with the tail values inverted per wave, then the steps in
+            // order; exp by the 2^(j/256) table (within about an ulp, as
+            // exp_fp64), its argument clamped to the table's domain
             for (int db = 0; db < seg; db += QMC_AK) {
-                double p[QMC_AK], z[QMC_AK];
+                double z[QMC_AK];
+                acklam_block_wave<QMC_AK>(
+                    [&](int i) {
+                        const int dd = min(db + i, seg - 1);
+                        const int d = d0 + dd;
+                        const uint64_t x = xhi[wave][dd] ^ lows[(int64_t)d * 64 + low] ^ scr[d];
+                        return uniform_open01(fma((double)x, inv, half));
+                    },
+                    z, slots);
 #pragma unroll
                 for (int i = 0; i < QMC_AK; ++i) {
-                    const int dd = min(db + i, seg - 1);
-                    const int d = d0 + dd;
-                    const uint64_t x = xhi[wave][dd] ^ lows[(int64_t)d * 64 + low] ^ scr[d];
-                    p[i] = uniform_open01(fma((double)x, inv, half));
+                    if (db + i < seg) {
+                        const double y = fma(o.dt_vol, z[i], o.dt_drift) * OFG_INV_LN2_256;
+                        spot *= exp2_256t(fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX),
+                                          exp2_tab);
+                    }
                 }
-                acklam_block<QMC_AK>(p, z);
-#pragma unroll
-                for (int i = 0; i < QMC_AK; ++i)
-                    if (db + i < seg) spot *= exp_fp64(fma(o.dt_vol, z[i], o.dt_drift));
             }
             __syncthreads();
         }
