@@ -456,6 +456,43 @@ int32_t ofg_fill_normals(ofg_ctx *ctx, int32_t rng_mode, uint64_t seed, uint64_t
                          int64_t n_streams, int64_t n_per_stream, int32_t step_major,
                          double *out, double *kernel_ms);
 
+/* Device-resident parameters and results of ofg_price_device. Each input is
+ * a device array of n doubles (option i at [i]) that replaces the
+ * corresponding ofg_option field, or NULL to keep the field; the outputs are
+ * device arrays the call writes. */
+typedef struct ofg_device_io {
+    const double *spot, *strike, *expiry, *rate, *vol;   /* vol: GBM sigma */
+    const double *v0, *kappa, *theta, *xi, *rho;         /* Heston */
+    double *price;      /* n values; required */
+    double *stderr_;    /* n values; may be NULL */
+    /* n x 8, row-major: d price / d (spot, vol | v0, rate, expiry, kappa,
+       theta, xi, rho) with greek_mode OFG_GREEKS_ADJOINT or FORWARD (the
+       pathwise estimator of ofg_price_greeks; GBM rows fill the first four);
+       may be NULL, ignored with OFG_GREEKS_NONE */
+    double *grads;
+} ofg_device_io;
+
+/* ofg_price with the market and model parameters read from, and the prices
+ * and gradients written to, device memory: the zero-copy form of
+ * PricingEngine::price / price_with_greeks_aad (mc_engine.rs:875-987) for
+ * callers whose data already live on the GPU (the torch autograd path).
+ * opts supplies each contract (payoff, type, barrier, observation times,
+ * dividend yield, paths, steps, seed) and the fields io leaves NULL; the
+ * descriptors are formed and validated on the device (the reference's
+ * rules: strike > 0, volatility > 0 under GBM, Heston::validate; every
+ * maturity must be > 0 here) and a violation returns OFG_INVALID_INPUT
+ * naming the offending option, with no output written. With device
+ * maturities an Asian option observes at every step (obs_times NULL, as
+ * its default schedule).
+ * greek_mode OFG_GREEKS_NONE, ADJOINT or FORWARD; control variates,
+ * OFG_RNG_EXTERNAL (host normals) and FD_CRN are refused: use ofg_price.
+ * The call reads its inputs on the library's own stream: work on other
+ * streams that produces them must be complete; the outputs are complete
+ * when it returns. Results equal ofg_price's on the same values to the
+ * rounding of the derived constants (exp, log and sqrt of the device). */
+int32_t ofg_price_device(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                         const ofg_engine_cfg *cfg, const ofg_device_io *io);
+
 /* Limits of the fixed device region ofg_price_noalloc prices in. */
 #define OFG_NOALLOC_MAX_OPTIONS 4096
 #define OFG_NOALLOC_MAX_WORK_ITEMS 262144

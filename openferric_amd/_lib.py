@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = [
     "ofg_dsl_compile", "ofg_dsl_free", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
     "ofg_dsl_price", "ofg_price_noalloc", "ofg_finite_difference_greeks",
     "ofg_bump_and_reprice", "ofg_european_exact", "ofg_spread_price",
-    "ofg_inject_alloc_failure", "ofg_fill_normals",
+    "ofg_inject_alloc_failure", "ofg_fill_normals", "ofg_price_device",
 ]
 
 
@@ -82,6 +82,12 @@ class OfgResult(C.Structure):
         ("num_paths", C.c_double), ("num_steps", C.c_double), ("vol", C.c_double),
         ("kernel_ms", C.c_double),
     ]
+
+
+class OfgDeviceIo(C.Structure):
+    """ofg_device_io: device addresses (int or None) of the inputs and outputs."""
+    _fields_ = [(k, C.c_void_p) for k in ("spot", "strike", "expiry", "rate", "vol", "v0", "kappa",
+                                          "theta", "xi", "rho", "price", "stderr_", "grads")]
 
 
 class OfgFdBumps(C.Structure):
@@ -292,6 +298,9 @@ def load() -> C.CDLL:
                                        C.c_int64, C.c_int32, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
         L.ofg_fill_normals.restype = C.c_int32
+        L.ofg_price_device.argtypes = [pv, C.POINTER(OfgOption), C.c_size_t,
+                                       C.POINTER(OfgEngineCfg), C.POINTER(OfgDeviceIo)]
+        L.ofg_price_device.restype = C.c_int32
         L.ofg_inject_alloc_failure.argtypes = [C.c_int32]
         L.ofg_inject_alloc_failure.restype = None
         L.ofg_lsm_last_timing.argtypes = [pv, C.POINTER(C.c_double), C.POINTER(C.c_double),

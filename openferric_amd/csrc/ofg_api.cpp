@@ -16,6 +16,7 @@
 #include "ofg_host.hpp"
 #include "ofg_plan.hpp"
 #include "ofg_normals.hpp"
+#include "ofg_device_io.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -26,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -540,6 +542,105 @@ int32_t bump_reprice_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n,
     return OFG_OK;
 }
 
+// Uploads a plan and enqueues the adjoint (or, with `forward`, the forward
+// pathwise) kernels of both models and their segmented reduce on the context
+// stream; ctx->outs then holds one AdjOut per plan option. `staged` runs on
+// the stream after the uploads (device-side staging of the descriptors).
+// Returns nothing until the caller synchronizes; ev0 / ev2 bracket the work.
+void enqueue_adjoint(ofg_ctx *ctx, const Plan &p, const ofg_engine_cfg &c, bool forward,
+                     const Tuning &t, const std::function<void(hipStream_t)> &staged) {
+    hipStream_t s = ctx->stream;
+    const size_t nopt = p.opts.size();
+    upload(ctx->opts, p.opts, s);
+    upload(ctx->obs_w, p.obs_w, s);
+    upload(ctx->items, p.items, s);
+    upload(ctx->order, p.order, s);
+    upload(ctx->item_first, p.item_first, s);
+    upload(ctx->item_count, p.item_count, s);
+    ctx->partials.ensure(p.items.size() * sizeof(ofg::AdjPartial));
+    ctx->outs.ensure(nopt * sizeof(ofg::AdjOut));
+    ctx->head.ensure(256);
+    const double *n1_dev = nullptr, *n2_dev = nullptr;
+    if (c.rng_mode == OFG_RNG_EXTERNAL) {
+        if (!c.normals1) fail(OFG_INVALID_INPUT, "external normals required for OFG_RNG_EXTERNAL");
+        if (p.any_heston && !c.normals2)
+            fail(OFG_INVALID_INPUT, "Heston needs a second external normal stream");
+        const size_t bytes = (size_t)p.normals_elems * sizeof(double);
+        ctx->n1.ensure(bytes);
+        hip_check(hipMemcpyAsync(ctx->n1.p, c.normals1, bytes, hipMemcpyHostToDevice, s), "H2D normals");
+        n1_dev = ctx->n1.as<double>();
+        if (c.normals2) {
+            ctx->n2.ensure(bytes);
+            hip_check(hipMemcpyAsync(ctx->n2.p, c.normals2, bytes, hipMemcpyHostToDevice, s), "H2D normals");
+            n2_dev = ctx->n2.as<double>();
+        }
+    }
+    // per-model launch shapes and checkpoint scratch
+    ofg::AdjArgs la[2];
+    size_t scratch_doubles = 0;
+    for (int m = 0; m < 2; ++m) {
+        const int32_t cnt = p.model_end[m] - p.model_begin[m];
+        if (cnt <= 0) continue;
+        int max_steps = 1;
+        for (const auto &d : p.opts)
+            if (d.model == m) max_steps = std::max(max_steps, (int)d.steps);
+        const int nv = m == OFG_MODEL_HESTON ? 2 : 1;
+        const int seg = ofg::adj_segment(m);
+        const int64_t nseg = (max_steps + seg - 1) / seg;
+        ofg::AdjArgs &a = la[m];
+        a = ofg::AdjArgs{};
+        a.opts = ctx->opts.as<ofg::OptDev>();
+        a.items = ctx->items.as<ofg::WorkItem>();
+        a.order = ctx->order.as<int32_t>();
+        a.order_begin = p.model_begin[m];
+        a.order_end = p.model_end[m];
+        a.queue_head = ctx->head.as<uint32_t>() + m;
+        a.partials = ctx->partials.as<ofg::AdjPartial>();
+        a.obs_w = ctx->obs_w.as<double>();
+        a.normals1 = n1_dev;
+        a.normals2 = n2_dev;
+        a.wave_stride = nseg * nv * 64;
+        a.rng_mode = c.rng_mode;
+        a.model = m;
+        a.antithetic = p.antithetic;
+        a.block = t.block;
+        const int wpb = t.block / 64;
+        const int full = forward ? ofg::pathwise_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device)
+                                 : ofg::adjoint_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device);
+        a.grid = std::max(1, (int)std::min<int64_t>(full, ((int64_t)cnt + wpb - 1) / wpb));
+        if (!forward)   // no checkpoints in the forward sweep
+            scratch_doubles =
+                std::max(scratch_doubles, (size_t)a.grid * wpb * (size_t)a.wave_stride);
+    }
+    ctx->adj.ensure(scratch_doubles * sizeof(double));
+    if (staged) staged(s);
+    hip_check(ofg::launch_reset_queue(ctx->head.as<uint32_t>(), 2, s), "reset queue");
+    hip_check(hipEventRecord(ctx->ev0, s), "event");
+    for (int m = 0; m < 2; ++m) {
+        if (p.model_end[m] - p.model_begin[m] <= 0) continue;
+        la[m].scratch = ctx->adj.as<double>();
+        if (forward)
+            hip_check(ofg::launch_pathwise(la[m], s), "pathwise kernel launch");
+        else
+            hip_check(ofg::launch_adjoint(la[m], s), "adjoint kernel launch");
+    }
+    hip_check(hipEventRecord(ctx->ev1, s), "event");
+    hip_check(ofg::launch_adjoint_reduce(ctx->opts.as<ofg::OptDev>(), ctx->item_first.as<int64_t>(),
+                                         ctx->item_count.as<int32_t>(),
+                                         ctx->partials.as<ofg::AdjPartial>(),
+                                         ctx->outs.as<ofg::AdjOut>(), (int32_t)nopt, s),
+              "adjoint reduce launch");
+    hip_check(hipEventRecord(ctx->ev2, s), "event");
+}
+
+// Launch shape of the Greeks kernels (built for 64, 128 and 256 threads).
+Tuning adjoint_tuning(const ofg_engine_cfg &cfg) {
+    Tuning t;
+    t.block = cfg.tune_block > 0 ? std::min(cfg.tune_block, 256) : 256;
+    t.item_steps = cfg.tune_item_steps > 0 ? cfg.tune_item_steps : 4096;
+    return t;
+}
+
 // Pathwise adjoint Greeks: forward + backward sweeps per sample in one
 // persistent launch per model (ofg_adjoint.hip), then a segmented reduce.
 int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_engine_cfg *cfg,
@@ -553,98 +654,16 @@ int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
     c.control_variate = 0;  // mc_aad.rs: plain estimator of price and Greeks
     const bool forward = cfg->greek_mode == OFG_GREEKS_FORWARD;
     c.greek_mode = OFG_GREEKS_NONE;
-    Tuning t;
-    t.block = cfg->tune_block > 0 ? std::min(cfg->tune_block, 256) : 256;
-    t.item_steps = cfg->tune_item_steps > 0 ? cfg->tune_item_steps : 4096;
+    const Tuning t = adjoint_tuning(*cfg);
     Plan p = build_plan(opts, n, c, t);
-    hipStream_t s = ctx->stream;
     const size_t nopt = p.opts.size();
     std::vector<ofg::AdjOut> outs(nopt);
     double ms = 0.0;
     if (nopt) {
-        upload(ctx->opts, p.opts, s);
-        upload(ctx->obs_w, p.obs_w, s);
-        upload(ctx->items, p.items, s);
-        upload(ctx->order, p.order, s);
-        upload(ctx->item_first, p.item_first, s);
-        upload(ctx->item_count, p.item_count, s);
-        ctx->partials.ensure(p.items.size() * sizeof(ofg::AdjPartial));
-        ctx->outs.ensure(nopt * sizeof(ofg::AdjOut));
-        ctx->head.ensure(256);
-        const double *n1_dev = nullptr, *n2_dev = nullptr;
-        if (c.rng_mode == OFG_RNG_EXTERNAL) {
-            if (!c.normals1) fail(OFG_INVALID_INPUT, "external normals required for OFG_RNG_EXTERNAL");
-            if (p.any_heston && !c.normals2)
-                fail(OFG_INVALID_INPUT, "Heston needs a second external normal stream");
-            const size_t bytes = (size_t)p.normals_elems * sizeof(double);
-            ctx->n1.ensure(bytes);
-            hip_check(hipMemcpyAsync(ctx->n1.p, c.normals1, bytes, hipMemcpyHostToDevice, s), "H2D normals");
-            n1_dev = ctx->n1.as<double>();
-            if (c.normals2) {
-                ctx->n2.ensure(bytes);
-                hip_check(hipMemcpyAsync(ctx->n2.p, c.normals2, bytes, hipMemcpyHostToDevice, s), "H2D normals");
-                n2_dev = ctx->n2.as<double>();
-            }
-        }
-        // per-model launch shapes and checkpoint scratch
-        ofg::AdjArgs la[2];
-        size_t scratch_doubles = 0;
-        for (int m = 0; m < 2; ++m) {
-            const int32_t cnt = p.model_end[m] - p.model_begin[m];
-            if (cnt <= 0) continue;
-            int max_steps = 1;
-            for (const auto &d : p.opts)
-                if (d.model == m) max_steps = std::max(max_steps, (int)d.steps);
-            const int nv = m == OFG_MODEL_HESTON ? 2 : 1;
-            const int seg = ofg::adj_segment(m);
-            const int64_t nseg = (max_steps + seg - 1) / seg;
-            ofg::AdjArgs &a = la[m];
-            a = ofg::AdjArgs{};
-            a.opts = ctx->opts.as<ofg::OptDev>();
-            a.items = ctx->items.as<ofg::WorkItem>();
-            a.order = ctx->order.as<int32_t>();
-            a.order_begin = p.model_begin[m];
-            a.order_end = p.model_end[m];
-            a.queue_head = ctx->head.as<uint32_t>() + m;
-            a.partials = ctx->partials.as<ofg::AdjPartial>();
-            a.obs_w = ctx->obs_w.as<double>();
-            a.normals1 = n1_dev;
-            a.normals2 = n2_dev;
-            a.wave_stride = nseg * nv * 64;
-            a.rng_mode = c.rng_mode;
-            a.model = m;
-            a.antithetic = p.antithetic;
-            a.block = t.block;
-            const int wpb = t.block / 64;
-            const bool fwd = forward;  // no checkpoints
-            const int full = fwd ? ofg::pathwise_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device)
-                                 : ofg::adjoint_grid(t.block, c.rng_mode, m, p.antithetic, ctx->device);
-            a.grid = std::max(1, (int)std::min<int64_t>(full, ((int64_t)cnt + wpb - 1) / wpb));
-            if (!fwd)
-                scratch_doubles =
-                    std::max(scratch_doubles, (size_t)a.grid * wpb * (size_t)a.wave_stride);
-        }
-        ctx->adj.ensure(scratch_doubles * sizeof(double));
-        hip_check(ofg::launch_reset_queue(ctx->head.as<uint32_t>(), 2, s), "reset queue");
-        hip_check(hipEventRecord(ctx->ev0, s), "event");
-        for (int m = 0; m < 2; ++m) {
-            if (p.model_end[m] - p.model_begin[m] <= 0) continue;
-            la[m].scratch = ctx->adj.as<double>();
-            if (forward)
-                hip_check(ofg::launch_pathwise(la[m], s), "pathwise kernel launch");
-            else
-                hip_check(ofg::launch_adjoint(la[m], s), "adjoint kernel launch");
-        }
-        hip_check(hipEventRecord(ctx->ev1, s), "event");
-        hip_check(ofg::launch_adjoint_reduce(ctx->opts.as<ofg::OptDev>(), ctx->item_first.as<int64_t>(),
-                                             ctx->item_count.as<int32_t>(),
-                                             ctx->partials.as<ofg::AdjPartial>(),
-                                             ctx->outs.as<ofg::AdjOut>(), (int32_t)nopt, s),
-                  "adjoint reduce launch");
-        hip_check(hipEventRecord(ctx->ev2, s), "event");
+        enqueue_adjoint(ctx, p, c, forward, t, nullptr);
         hip_check(hipMemcpyAsync(outs.data(), ctx->outs.p, nopt * sizeof(ofg::AdjOut),
-                                 hipMemcpyDeviceToHost, s), "D2H adjoint results");
-        hip_check(hipStreamSynchronize(s), "adjoint execution");
+                                 hipMemcpyDeviceToHost, ctx->stream), "D2H adjoint results");
+        hip_check(hipStreamSynchronize(ctx->stream), "adjoint execution");
         float f = 0.f;
         hip_check(hipEventElapsedTime(&f, ctx->ev0, ctx->ev2), "event time");
         ms = f;
@@ -682,6 +701,154 @@ int32_t adjoint_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg_e
             r.heston_grad[4] = a->g[6];
             r.heston_grad[5] = a->g[7];
         }
+    }
+    return OFG_OK;
+}
+
+// ofg_price_device: the plan is built on the host from placeholder values of
+// the fields the caller's device arrays replace (they only have to pass the
+// host checks: the work queue depends on paths, steps, model and payoff
+// alone), then the staging kernel overwrites every market- and model-
+// dependent descriptor field from the device arrays and validates them.
+int32_t price_device_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                          const ofg_engine_cfg *cfg, const ofg_device_io *io) {
+    if (!ctx || !cfg || !io || (!opts && n)) fail(OFG_INVALID_INPUT, "null argument");
+    if (n && !io->price) fail(OFG_INVALID_INPUT, "null argument");
+    if (cfg->rng_mode < 0 || cfg->rng_mode > 3) fail(OFG_INVALID_INPUT, "unknown rng mode");
+    check_tuning(*cfg);
+    if (cfg->control_variate)
+        fail(OFG_INVALID_INPUT,
+             "ofg_price_device: control variates take host closed forms; use ofg_price");
+    if (cfg->rng_mode == OFG_RNG_EXTERNAL)
+        fail(OFG_INVALID_INPUT, "ofg_price_device: external normals are host arrays; use ofg_price");
+    const int32_t gm = cfg->greek_mode;
+    if (gm != OFG_GREEKS_NONE && gm != OFG_GREEKS_ADJOINT && gm != OFG_GREEKS_FORWARD)
+        fail(OFG_INVALID_INPUT, "ofg_price_device takes greek_mode NONE, ADJOINT or FORWARD");
+    const bool greeks = gm != OFG_GREEKS_NONE;
+    if (greeks && cfg->rng_mode == OFG_RNG_PCG64)
+        fail(OFG_INVALID_INPUT,
+             "pathwise Greeks draw OFG_RNG_REFERENCE, OFG_RNG_EXTERNAL or OFG_RNG_FAST normals");
+    if (n > (size_t)INT32_MAX) fail(OFG_INVALID_INPUT, "too many options");
+    std::vector<ofg_option> tpl(opts, opts + n);
+    std::vector<std::vector<double>> every_step(n);
+    for (size_t i = 0; i < n; ++i) {
+        ofg_option &o = tpl[i];
+        if (io->expiry) {
+            if (o.payoff == OFG_PAYOFF_ASIAN) {
+                if (o.obs_times)
+                    fail(OFG_INVALID_INPUT,
+                         "ofg_price_device: with device maturities an Asian option observes at "
+                         "every step (obs_times NULL)");
+                if (o.num_steps <= 0 || o.num_steps > 1000000)
+                    fail(OFG_INVALID_INPUT, "num_steps must be > 0");
+                for (int64_t j = 1; j <= o.num_steps; ++j)
+                    every_step[i].push_back((double)j / (double)o.num_steps);
+                o.obs_times = every_step[i].data();
+                o.n_obs = (int32_t)o.num_steps;
+            }
+            o.expiry = 1.0;
+        } else if (!(o.expiry > 0.0)) {
+            fail(OFG_INVALID_INPUT, "ofg_price_device prices maturities > 0");
+        }
+        if (io->spot) o.spot = 100.0;
+        if (io->strike) o.strike = 100.0;
+        if (io->rate) o.rate = 0.0;
+        if (io->vol) o.vol = 0.2;
+        if (io->v0) o.v0 = 0.04;
+        if (io->kappa) o.kappa = 1.0;
+        if (io->theta) o.theta = 0.04;
+        if (io->xi) o.xi = 0.5;
+        if (io->rho) o.rho = 0.0;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ofg_engine_cfg c = *cfg;
+    c.greek_mode = OFG_GREEKS_NONE;
+    Plan p;
+    Tuning t;
+    if (greeks) {
+        t = adjoint_tuning(*cfg);
+        p = build_plan(tpl.data(), n, c, t);
+    } else {
+        Plan probe = build_plan(tpl.data(), n, c, Tuning{});
+        t = tuning_for(ctx, probe.opts, c);
+        p = (t.item_steps == Tuning{}.item_steps) ? std::move(probe) : build_plan(tpl.data(), n, c, t);
+    }
+    const int32_t nplan = (int32_t)p.opts.size();
+    if (nplan == 0) return OFG_OK;
+    std::vector<double> host(2 * (size_t)nplan);
+    for (int32_t k = 0; k < nplan; ++k) {
+        const ofg_option &o = opts[(size_t)p.active[(size_t)k]];
+        host[2 * (size_t)k] = o.div_yield;
+        host[2 * (size_t)k + 1] = o.expiry;
+    }
+    hipStream_t s = ctx->stream;
+    upload(ctx->stage, host, s);
+    upload(ctx->active, p.active, s);
+    ctx->err.ensure(2 * sizeof(int32_t));
+    ofg::StageArgs sa{};
+    sa.active = ctx->active.as<int32_t>();
+    sa.n_plan = nplan;
+    sa.rng_mode = c.rng_mode;
+    sa.host = ctx->stage.as<double>();
+    sa.err = ctx->err.as<int32_t>();
+    auto staged = [&](hipStream_t st) {
+        sa.opts = ctx->opts.as<ofg::OptDev>();
+        hip_check(hipMemsetAsync(sa.err, 0, 2 * sizeof(int32_t), st), "clear error flag");
+        hip_check(ofg::launch_stage(sa, *io, st), "staging kernel launch");
+    };
+    if (greeks) {
+        enqueue_adjoint(ctx, p, c, gm == OFG_GREEKS_FORWARD, t, staged);
+        hip_check(ofg::launch_scatter_adjoint(ctx->outs.as<ofg::AdjOut>(), sa.active, nplan, *io,
+                                              sa.err, s),
+                  "scatter launch");
+    } else {
+        upload(ctx->opts, p.opts, s);
+        upload(ctx->obs_w, p.obs_w, s);
+        upload(ctx->items, p.items, s);
+        upload(ctx->order, p.order, s);
+        upload(ctx->item_first, p.item_first, s);
+        upload(ctx->item_count, p.item_count, s);
+        ctx->partials.ensure(p.items.size() * sizeof(ofg::Partial));
+        ctx->outs.ensure((size_t)nplan * sizeof(ofg::OptOut));
+        ctx->head.ensure(256);
+        staged(s);
+        ofg::LaunchArgs a{};
+        a.opts = ctx->opts.as<ofg::OptDev>();
+        a.items = ctx->items.as<ofg::WorkItem>();
+        a.order = ctx->order.as<int32_t>();
+        a.partials = ctx->partials.as<ofg::Partial>();
+        a.obs_w = ctx->obs_w.as<double>();
+        a.rng_mode = c.rng_mode;
+        enqueue_pricing(a, p, t, ctx->device, ctx->head.as<uint32_t>(), s);
+        hip_check(ofg::launch_reduce(a.opts, ctx->item_first.as<int64_t>(),
+                                     ctx->item_count.as<int32_t>(), a.partials,
+                                     ctx->outs.as<ofg::OptOut>(), nplan, s),
+                  "reduce kernel launch");
+        hip_check(ofg::launch_scatter_price(ctx->outs.as<ofg::OptOut>(), sa.active, nplan, *io,
+                                            sa.err, s),
+                  "scatter launch");
+    }
+    int32_t err[2] = {0, 0};
+    hip_check(hipMemcpyAsync(err, sa.err, sizeof err, hipMemcpyDeviceToHost, s), "D2H status");
+    hip_check(hipStreamSynchronize(s), "device pricing");
+    if (err[0] != ofg::DIE_NONE) {
+        const ofg_option &o = opts[(size_t)err[1]];
+        const char *kind = o.payoff == OFG_PAYOFF_ASIAN     ? "asian"
+                           : o.payoff == OFG_PAYOFF_BARRIER ? "barrier"
+                                                            : "vanilla";
+        std::string m;
+        switch (err[0]) {
+        case ofg::DIE_STRIKE: m = std::string(kind) + " strike must be > 0"; break;
+        case ofg::DIE_EXPIRY: m = "ofg_price_device prices maturities > 0"; break;
+        case ofg::DIE_VOL: m = "market volatility must be > 0"; break;
+        case ofg::DIE_HESTON: m = "invalid Heston parameters"; break;
+        case ofg::DIE_FAST_RANGE:
+            m = "fast rng mode needs per-step log-returns below 2.9e6 in magnitude";
+            break;
+        default: m = "device inputs must be finite"; break;
+        }
+        fail(OFG_INVALID_INPUT, m + " (option " + std::to_string(err[1]) + ")");
     }
     return OFG_OK;
 }
@@ -891,7 +1058,8 @@ void ofg_destroy(ofg_ctx *ctx) {
     spread_release(ctx);
     for (DevBuf *b : {&ctx->opts, &ctx->obs_w, &ctx->items, &ctx->order, &ctx->item_first,
                       &ctx->item_count, &ctx->partials, &ctx->outs, &ctx->head, &ctx->xs, &ctx->ys,
-                      &ctx->n1, &ctx->n2, &ctx->adj, &ctx->fill})
+                      &ctx->n1, &ctx->n2, &ctx->adj, &ctx->fill, &ctx->stage, &ctx->active,
+                      &ctx->err})
         b->release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
@@ -952,6 +1120,11 @@ int32_t ofg_fill_normals(ofg_ctx *ctx, int32_t rng_mode, uint64_t seed, uint64_t
         if (kernel_ms) *kernel_ms = ms;
         return OFG_OK;
     });
+}
+
+int32_t ofg_price_device(ofg_ctx *ctx, const ofg_option *opts, size_t n,
+                         const ofg_engine_cfg *cfg, const ofg_device_io *io) {
+    return guarded([&]() { return price_device_impl(ctx, opts, n, cfg, io); });
 }
 
 int32_t ofg_price_noalloc(ofg_ctx *ctx, const ofg_option *opts, size_t n,

@@ -138,7 +138,7 @@ struct ofg_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     std::mutex mu;
     ofg_host::DevBuf opts, obs_w, items, order, item_first, item_count, partials, outs, head, xs, ys, n1,
-        n2, adj, fill;
+        n2, adj, fill, stage, active, err;
     std::map<std::string, ofg_host::Tuning> tuned;
     ofg_host::LsmBuffers *lsm = nullptr;   // Longstaff-Schwartz work buffers
     ofg_host::QmcBuffers *qmc = nullptr;   // quasi-Monte Carlo tables and partials
