@@ -90,7 +90,10 @@ Tuning tuning_for(ofg_ctx *ctx, const std::vector<ofg::OptDev> &opts, const ofg_
     auto it = ctx->tuned.find(workload_class(opts, cfg.rng_mode));
     if (it != ctx->tuned.end()) t = it->second;
     if (cfg.tune_block > 0) t.block = cfg.tune_block;
-    if (cfg.tune_item_steps > 0) t.item_steps = cfg.tune_item_steps;
+    if (cfg.tune_item_steps > 0) {
+        t.item_steps = cfg.tune_item_steps;
+        t.auto_items = false;
+    }
     return t;
 }
 
@@ -240,6 +243,7 @@ int32_t mc_greeks_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n, const ofg
     Tuning t;
     t.block = cfg->tune_block > 0 ? std::min(cfg->tune_block, 256) : 256;
     t.item_steps = cfg->tune_item_steps > 0 ? cfg->tune_item_steps : 4096;
+    t.auto_items = cfg->tune_item_steps <= 0;
     Plan p = build_plan(sim.data(), sim.size(), c, t);
     const size_t nopt = p.opts.size();
     for (size_t k = 0; k < nopt; ++k) {
@@ -638,6 +642,7 @@ Tuning adjoint_tuning(const ofg_engine_cfg &cfg) {
     Tuning t;
     t.block = cfg.tune_block > 0 ? std::min(cfg.tune_block, 256) : 256;
     t.item_steps = cfg.tune_item_steps > 0 ? cfg.tune_item_steps : 4096;
+    t.auto_items = cfg.tune_item_steps <= 0;
     return t;
 }
 
@@ -957,6 +962,7 @@ int32_t price_noalloc_impl(ofg_ctx *ctx, const ofg_option *opts, size_t n,
         if (t.item_steps >= (1 << 26))
             fail(OFG_INVALID_INPUT, "ofg_price_noalloc: the work needs more than 262144 work items");
         t.item_steps *= 2;
+        t.auto_items = false;
         p = build_plan(opts, n, *cfg, t);
     }
     if (p.obs_w.size() > NA_MAX_OBS + OBS_PAD)

@@ -88,6 +88,9 @@ struct DevBuf {
 struct Tuning {
     int32_t block = 256;
     int32_t item_steps = 8192;
+    // item_steps may shrink for a call too small to give the queue
+    // ofg_plan::MIN_ITEMS items (not when the caller fixed the size)
+    bool auto_items = true;
 };
 
 template <class F>

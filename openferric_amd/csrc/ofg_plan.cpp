@@ -124,6 +124,18 @@ std::string workload_class(const std::vector<ofg::OptDev> &opts, int32_t rng_mod
 // `vols` is Market::vol_for (flat) per option; for Heston options ignored.
 Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const Tuning &tune) {
     Plan p;
+    // the call's whole work in lane-steps, weighted as the items below
+    double work = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const ofg_option &o = in[i];
+        if (o.num_paths <= 0 || o.num_steps <= 0 || !(o.expiry > 0.0)) continue;
+        const double samples = cfg.antithetic ? (double)((o.num_paths + 1) / 2) : (double)o.num_paths;
+        work += samples * (double)o.num_steps * (o.model == OFG_MODEL_HESTON ? 1.9 : 1.0) *
+                (cfg.antithetic ? 1.6 : 1.0);
+    }
+    const double item_steps = tune.auto_items
+                                  ? std::min<double>(tune.item_steps, work / (64.0 * MIN_ITEMS))
+                                  : (double)tune.item_steps;
     p.vols.resize(n);
     int64_t out_offset = 0, normals_offset = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -255,11 +267,11 @@ Plan build_plan(const ofg_option *in, size_t n, const ofg_engine_cfg &cfg, const
         p.opts.push_back(d);
         p.active.push_back((int32_t)i);
 
-        // work items: samples_per_lane chosen so an item is ~tune.item_steps
+        // work items: samples_per_lane chosen so an item is ~item_steps
         // lane-steps of work (Heston and antithetic legs weigh more).
         const double leg_w = (o.model == OFG_MODEL_HESTON ? 1.9 : 1.0) * (cfg.antithetic ? 1.6 : 1.0);
         const double per_sample = (double)o.num_steps * leg_w;
-        int64_t spl = (int64_t)std::llround((double)tune.item_steps / per_sample);
+        int64_t spl = (int64_t)std::llround(item_steps / per_sample);
         spl = std::max<int64_t>(1, std::min<int64_t>(spl, 1024));
         const int64_t item_samples = 64 * spl;
         // refuse before building the queue: 2^27 items is already 4 GiB of host plan

@@ -23,6 +23,11 @@ namespace ofg_plan {
 // never applied).
 constexpr size_t OBS_PAD = 16;
 
+// A call whose whole work would make fewer work items than this at the
+// tuned item size gets smaller items (down to one sample per lane): about
+// eight per resident wave of the 256 CUs, so the queue keeps every SIMD fed.
+constexpr double MIN_ITEMS = 8192.0;
+
 // Host-side plan of one pricing launch.
 struct Plan {
     std::vector<ofg::OptDev> opts;
