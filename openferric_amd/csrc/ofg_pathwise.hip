@@ -240,9 +240,13 @@ __device__ __forceinline__ void heston_tangent_step(Leg &g, double &v, HTan &t, 
         hinv = live ? 0.5 / hp.sv : 0.0;
     } else {
         // 1 / (2 sv) by one Newton step from the rsqrt seed that sqrt_pos
-        // starts from (relative error 2^-23 -> 2^-46)
-        const double y = rsq_seed(v_pos);
-        hinv = live ? (0.5 * y) * fma(-hp.sv, y, 2.0) : 0.0;
+        // starts from (relative error 2^-23 -> 2^-46). At v_pos = 0 the seed
+        // is +inf: it is capped so that hinv stays finite, which is all that
+        // step needs -- every variance tangent is 0 there (the clamp that
+        // put v at 0 zeroed them; a path starting at v0 = 0 takes the
+        // zero_start branch), so hinv only ever multiplies zeros.
+        const double y = fmin(rsq_seed(v_pos), 1e150);
+        hinv = (0.5 * y) * fma(-hp.sv, y, 2.0);
     }
     double A, Bs;  // d vn / d v and (d e / d v) / sqrt(dt)
     if (zero_start) {
