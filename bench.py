@@ -31,12 +31,14 @@ FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector peak
 FLOP_GBM_STEP = 65            # canonical FLOP per GBM path-step
 FLOP_ASIAN_STEP = 66          # + arithmetic running sum
 WORKLOAD_SEED = 0x0FE22C1C
-# HBM bytes per launch of the pricing kernel on the default workload, by
-# threads per block of the instantiation, from rocprofv3 FETCH_SIZE (x2.000)
-# and WRITE_SIZE (x1.000) scaled by a 1 GiB copy kernel
-# (profiles/price_kernel_fast_gbm_batch4096_counters.md); other shapes are
-# unmeasured.
-PRICE_KERNEL_HBM_BYTES = {64: 77.3e6 + 37.4e6, 256: 91.3e6 + 38.7e6}
+# HBM bytes per launch of the pricing kernel on the default workload, for
+# each launch shape the autotuner chooses from, from rocprofv3 FETCH_SIZE
+# (x2.000) and WRITE_SIZE (x1.000) scaled by a 1 GiB copy kernel
+# (profiles/price_kernel_fast_gbm_batch4096_counters.md).
+PRICE_KERNEL_HBM_BYTES = {   # (threads per block, lane-steps per item) -> bytes
+    (512, 8192): 156.7e6, (256, 8192): 155.1e6, (128, 8192): 158.4e6, (64, 8192): 141.5e6,
+    (512, 2048): 415.3e6, (256, 2048): 417.7e6, (128, 2048): 417.5e6, (64, 2048): 352.9e6,
+    (512, 32768): 73.2e6, (256, 32768): 74.0e6, (128, 32768): 75.2e6, (64, 32768): 71.6e6}
 
 
 def make_workload(n_opts=4096, seed=WORKLOAD_SEED, path_scale=1.0):
@@ -884,6 +886,9 @@ def main():
                     help="run only the structured-product leg (profiling)")
     ap.add_argument("--heston-only", action="store_true",
                     help="run only the two Heston legs (profiling)")
+    ap.add_argument("--shape", default="",
+                    help="BLOCK,ITEM: launch shape of the batch instead of the autotuner's "
+                         "(counter runs of one instantiation)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one D2D copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -922,7 +927,11 @@ def main():
     structs, keeps = zip(*[to_ofg(d) for d in opts])
     cfg = engine_cfg(antithetic=False, control_variate=False, rng_mode=RngMode.Fast)
     tuned = None
-    if not args.no_autotune:
+    if args.shape:
+        tuned = tuple(int(x) for x in args.shape.split(","))
+        cfg = engine_cfg(antithetic=False, control_variate=False, rng_mode=RngMode.Fast,
+                         tune_block=tuned[0], tune_item_steps=tuned[1])
+    elif not args.no_autotune:
         # block size x work-item size, timed on a scaled copy of the batch; the
         # choice is remembered by the context and used by the batch below
         print("[bench] autotuning launch shape", file=sys.stderr, flush=True)
@@ -975,7 +984,8 @@ def main():
                    {"block": tuned[0], "item_lane_steps": tuned[1]}},
         "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": PRICE_KERNEL_HBM_BYTES.get(256 if tuned is None else tuned[0])
+                     "traffic": PRICE_KERNEL_HBM_BYTES.get((256, 8192) if tuned is None
+                                                           else tuple(tuned))
                      if args.path_scale == 1.0 else None,
                      # the instantiation launched: threads per block from the autotuner
                      # (the library's default shape, 256, without it)
