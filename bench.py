@@ -896,8 +896,10 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "
+                                   "(a batch would shard by work item across GPUs, each "
+                                   "GPU running the persistent kernel on its share; not "
+                                   "measured here)"}))
         sys.exit(2)
 
     from openferric_amd import BatchPricer, autotune, engine_cfg, RngMode

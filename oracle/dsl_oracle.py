@@ -1,5 +1,5 @@
 """CPU oracle, structured-product language -- TEST INFRASTRUCTURE ONLY (the
-checker of the GPU structured-product engine; never imported by the
+reference results the GPU structured-product engine is tested against; never imported by the
 product).
 
 Restates src/dsl of rosssaunders/openferric in plain Python:

@@ -1,6 +1,6 @@
 /*
  * CPU oracle for the OpenFerric Monte Carlo pricing path -- TEST
- * INFRASTRUCTURE ONLY (the checker and the timed CPU baseline; never part of
+ * INFRASTRUCTURE ONLY (the tests' reference results and the timed CPU baseline; never part of
  * the product path). See oracle.h for the list of functions and how each one
  * is pinned. Reference file:line citations point into rosssaunders/openferric.
  *

@@ -2,7 +2,7 @@
  * CPU oracle for the OpenFerric Monte Carlo pricing path.
  *
  * TEST INFRASTRUCTURE ONLY. This library is a plain-C restatement of the
- * reference Rust algorithm (rosssaunders/openferric) used as the checker for
+ * reference Rust algorithm (rosssaunders/openferric) used as the tests' reference for
  * the HIP product path and as the timed CPU baseline. Only tests/, the smoke
  * check and the benchmark's cpu_baseline leg may load it; the product library
  * (openferric_amd) never links, calls or falls back to it.

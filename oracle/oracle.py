@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (liboracle.so), and the oracle's Python
 restatements of the finite-difference Greeks harness (src/greeks/bsm.rs).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the timed
+TEST INFRASTRUCTURE ONLY: the tests' reference for the HIP product path and the timed
 CPU baseline. Only tests/, the smoke check and the benchmark's cpu_baseline
 leg import this module. The product package never imports it.
 """
@@ -107,8 +107,12 @@ def build(force: bool = False) -> str:
     `force`, make rebuilds only when a source is newer than the library (the
     sources are absent where only the built library was shipped)."""
     if force or not os.path.exists(_LIB_PATH) or os.path.exists(os.path.join(_HERE, "oracle.c")):
-        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"],
-                       check=True, capture_output=True)
+        import fcntl
+        # one build at a time: parallel test workers may call this together
+        with open(os.path.join(_HERE, ".build.lock"), "w") as lock:
+            fcntl.flock(lock, fcntl.LOCK_EX)
+            subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"],
+                           check=True, capture_output=True)
     return _LIB_PATH
 
 

@@ -1,6 +1,6 @@
 /*
  * Oracle: the exact-terminal European pricers of the reference's default
- * build (Cargo feature "simd" off), restated for the checker.
+ * build (Cargo feature "simd" off), restated as the tests' reference.
  *
  * TEST INFRASTRUCTURE ONLY; the product never links or loads it.
  *

@@ -1,6 +1,6 @@
 /*
- * CPU oracle, Longstaff-Schwartz part -- TEST INFRASTRUCTURE ONLY (the checker
- * of the HIP least-squares Monte Carlo path and its timed CPU baseline).
+ * CPU oracle, Longstaff-Schwartz part -- TEST INFRASTRUCTURE ONLY (the tests' reference
+ * for the HIP least-squares Monte Carlo path and its timed CPU baseline).
  *
  * Restates src/engines/lsm/longstaff_schwartz.rs of rosssaunders/openferric:
  *   - path simulation from ONE xoshiro256++ stream drawn path-major

@@ -1,5 +1,5 @@
 /*
- * CPU oracle, multi-asset part -- TEST INFRASTRUCTURE ONLY (the checker of
+ * CPU oracle, multi-asset part -- TEST INFRASTRUCTURE ONLY (the tests' reference for
  * the HIP multi-asset path and its timed CPU baseline).
  *
  * Restates src/pricing/autocallable.rs of rosssaunders/openferric:

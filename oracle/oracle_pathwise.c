@@ -1,7 +1,7 @@
 /*
  * Oracle: pathwise sensitivities of MonteCarloEngine::run's estimator.
  *
- * TEST INFRASTRUCTURE ONLY (the checker of the GPU adjoint and forward-mode
+ * TEST INFRASTRUCTURE ONLY (the tests' reference for the GPU adjoint and forward-mode
  * Greeks kernels); the product never links or loads it.
  *
  * The reference differentiates two cases itself: the European vanilla under

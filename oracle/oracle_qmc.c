@@ -1,6 +1,6 @@
 /*
- * CPU oracle, quasi-Monte Carlo part -- TEST INFRASTRUCTURE ONLY (the checker
- * of the HIP Sobol path and its timed CPU baseline).
+ * CPU oracle, quasi-Monte Carlo part -- TEST INFRASTRUCTURE ONLY (the tests' reference
+ * for the HIP Sobol path and its timed CPU baseline).
  *
  * Restates src/math/sobol.rs (SobolSequence: direction numbers, scramblers,
  * Gray-code stepping, the u64 -> (0,1) map) and

@@ -3,7 +3,7 @@
  * (src/engines/monte_carlo/spread_mc.rs:83-200) and of the reference's Pcg64
  * stream (src/math/fast_rng.rs:93-137).
  *
- * TEST INFRASTRUCTURE ONLY: the checker for ofg_spread_price; the product
+ * TEST INFRASTRUCTURE ONLY: the tests' reference for ofg_spread_price; the product
  * library never links, calls or falls back to it.
  *
  * Pins (tests/test_oracle_spread.py):
