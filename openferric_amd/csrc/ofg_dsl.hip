@@ -143,20 +143,29 @@ struct StepParams<MAXA, false> {
 // entries 2^(j/256) of the GBM factors; without (a kernel whose LDS holds
 // the programs' register files) each lane inverts its own tail values and
 // GBM factors take exp_fp64.
-template <int RNG, int MAXA, bool WTAIL = true>
+// NA > 0: exactly NA assets (NA = MAXA), every one stepping as GBM, so the
+// per-asset kind tests and asset-count guards are gone from the step.
+// The snapshot steps are walked with a cursor over a.snap_list (a scalar
+// load per snapshot, not per step).
+template <int RNG, int MAXA, bool WTAIL = true, int NA = 0>
 __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o,
                                               const StepParams<MAXA> &sp, int64_t p,
                                               double *snap, int64_t stride, const double *tab,
                                               double *slots) {
-    const int n = o.n_assets;
+    static_assert(NA == 0 || NA == MAXA, "exact asset count");
+    const int n = NA > 0 ? NA : o.n_assets;
     double s[MAXA];
 #pragma unroll
     for (int i = 0; i < MAXA; ++i) s[i] = o.s0[i];
-    const int k0 = a.step_snap[0];
-    if (k0 >= 0) {
+    const int32_t *sl = a.snap_list;
+    int next_step = sl[0], next_k = sl[1];
+    if (next_step == 0) {
 #pragma unroll
         for (int i = 0; i < MAXA; ++i)
-            if (i < n) snap[(int64_t)(k0 * n + i) * stride] = s[i];
+            if (i < n) snap[(int64_t)(next_k * n + i) * stride] = s[i];
+        sl += 2;
+        next_step = sl[0];
+        next_k = sl[1];
     }
     Xoshiro256pp ref;
     Xoroshiro128p fast;
@@ -202,15 +211,23 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
             }
         }
 #pragma unroll
-        for (int i = 0; i < MAXA; ++i)
-            if (i < n)
-                s[i] = step_asset<RNG == OFG_RNG_FAST, WTAIL && RNG == OFG_RNG_REFERENCE>(
-                    sp.kind(i), s[i], sp.p0(i), sp.p1(i), sp.p2(i), c[i], tab);
-        const int k = a.step_snap[step];
-        if (k >= 0) {
+        for (int i = 0; i < MAXA; ++i) {
+            if (i < n) {
+                if constexpr (NA > 0)
+                    s[i] = s[i] * gbm_factor<RNG == OFG_RNG_FAST, WTAIL && RNG == OFG_RNG_REFERENCE>(
+                                      sp.p0(i), sp.p1(i), c[i], tab);
+                else
+                    s[i] = step_asset<RNG == OFG_RNG_FAST, WTAIL && RNG == OFG_RNG_REFERENCE>(
+                        sp.kind(i), s[i], sp.p0(i), sp.p1(i), sp.p2(i), c[i], tab);
+            }
+        }
+        if (step == next_step) {
 #pragma unroll
             for (int i = 0; i < MAXA; ++i)
-                if (i < n) snap[(int64_t)(k * n + i) * stride] = s[i];
+                if (i < n) snap[(int64_t)(next_k * n + i) * stride] = s[i];
+            sl += 2;
+            next_step = sl[0];
+            next_k = sl[1];
         }
     };
     if constexpr (RNG == OFG_RNG_REFERENCE && WTAIL) {
@@ -415,7 +432,7 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
 }
 
 // Split form, pass 1: the paths' snapshots, [snap][asset][path] per scenario.
-template <int RNG, int MAXA>
+template <int RNG, int MAXA, int NA = 0>
 __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
     // the fast mode's tables, or the reference draws' 2^(j/256) table and
     // each wave's slots for the tail values of an inverse-CDF block
@@ -438,7 +455,8 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        simulate_path<RNG, MAXA>(a, o, sp, p, a.snaps + o.snap_off + p, o.paths, tab, slots);
+        simulate_path<RNG, MAXA, true, NA>(a, o, sp, p, a.snaps + o.snap_off + p, o.paths, tab,
+                                           slots);
     }
 }
 
@@ -662,12 +680,26 @@ size_t lds_bytes(const DslArgs &a) {
     return (size_t)(a.n_regs > 0 ? a.n_regs : 1) * DSL_BLOCK * sizeof(double);
 }
 
+// The split form's simulation of markets whose assets all step as GBM, at
+// their exact asset count (up to four).
+template <int RNG>
+void launch_sim_gbm(const DslArgs &a, hipStream_t stream) {
+    switch (a.n_assets) {
+    case 1: dsl_sim_kernel<RNG, 1, 1><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+    case 2: dsl_sim_kernel<RNG, 2, 2><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+    case 3: dsl_sim_kernel<RNG, 3, 3><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+    default: dsl_sim_kernel<RNG, 4, 4><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+    }
+}
+
 template <int RNG, int MAXA>
 void launch_rng(const DslArgs &a, hipStream_t stream) {
     if (a.split) {
         if constexpr (MAXA <= DSL_FUSE_MAXA) {
             if (a.fuse_markets)
                 dsl_sim_markets_kernel<RNG, MAXA><<<a.fuse_tiles, DSL_BLOCK, 0, stream>>>(a);
+            else if (a.gbm_only && a.n_assets >= 1 && a.n_assets <= 4)
+                launch_sim_gbm<RNG>(a, stream);
             else
                 dsl_sim_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
         } else {

@@ -112,6 +112,8 @@ struct DslArgs {
     const DslObs *obs;
     const double *df;
     const int32_t *step_snap;   // [steps + 1]: snapshot index or -1
+    const int32_t *snap_list;   // (step, snapshot index) pairs in step order,
+                                // then the sentinel (steps + 1, -1)
     double *snaps;              // split: per scenario [snap][asset][path];
                                 // fused: [tile][snap][asset][DSL_BLOCK]
     DslPartial *partials;
@@ -133,6 +135,7 @@ struct DslArgs {
     int32_t fuse_markets;       // split form: 1 = one simulation pass for all markets
     int32_t fuse_tiles;         // its grid: market 0's tiles
     int32_t n_regs;             // program registers (LDS per thread)
+    int32_t gbm_only;           // every asset of every scenario steps as GBM
     const double *state_init;   // [n_state]
 };
 

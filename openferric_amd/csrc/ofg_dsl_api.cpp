@@ -38,15 +38,16 @@ struct ofg_dsl_product {
 namespace ofg_host {
 
 struct DslBuffers {
-    DevBuf scen, tiles, code, consts, sched, obs, df, step_snap, snaps, partials, out, normals,
-        seeds, polys, jobs, state;
+    DevBuf scen, tiles, code, consts, sched, obs, df, step_snap, snap_list, snaps, partials, out,
+        normals, seeds, polys, jobs, state;
 };
 
 void dsl_release(ofg_ctx *ctx) {
     if (!ctx || !ctx->dsl) return;
     DslBuffers *b = ctx->dsl;
     for (DevBuf *d : {&b->scen, &b->tiles, &b->code, &b->consts, &b->sched, &b->obs, &b->df,
-                      &b->step_snap, &b->snaps, &b->partials, &b->out, &b->normals, &b->seeds,
+                      &b->step_snap, &b->snap_list, &b->snaps, &b->partials, &b->out, &b->normals,
+                      &b->seeds,
                       &b->polys, &b->jobs, &b->state})
         d->release();
     delete b;
@@ -121,6 +122,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         if (n != na) fail(OFG_INVALID_INPUT, "all markets of one call must have the same asset count");
     }
     if (na > ofg::DSL_MAX_ASSETS) fail(OFG_INVALID_INPUT, "at most 16 assets per market");
+    if (steps > INT32_MAX) fail(OFG_INVALID_INPUT, "num_steps exceeds the device limit");
     // build_execution_plan (eval.rs:268-330)
     int n_locals = 0;
     std::vector<ofg_dsl::Program> progs;
@@ -291,6 +293,15 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         up(b.obs, obs.data(), obs.size() * sizeof(obs[0]), "H2D dsl observations");
         up(b.df, dfs.data(), dfs.size() * sizeof(double), "H2D dsl discount factors");
         up(b.step_snap, step_snap.data(), step_snap.size() * sizeof(int32_t), "H2D dsl snapshot map");
+        std::vector<int32_t> snap_list;   // the snapshot steps in order, then a sentinel
+        for (size_t st = 0; st < step_snap.size(); ++st) {
+            if (step_snap[st] < 0) continue;
+            snap_list.push_back((int32_t)st);
+            snap_list.push_back(step_snap[st]);
+        }
+        snap_list.push_back((int32_t)steps + 1);
+        snap_list.push_back(-1);
+        up(b.snap_list, snap_list.data(), snap_list.size() * sizeof(int32_t), "H2D dsl snapshot list");
         up(b.state, state_init.data(), state_init.size() * sizeof(double), "H2D dsl state");
         b.snaps.ensure(split ? path_bytes * (size_t)paths * n_mk : tiles.size() * tile_bytes);
         b.partials.ensure(tiles.size() * sizeof(ofg::DslPartial));
@@ -314,6 +325,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         a.obs = b.obs.as<ofg::DslObs>();
         a.df = b.df.as<double>();
         a.step_snap = b.step_snap.as<int32_t>();
+        a.snap_list = b.snap_list.as<int32_t>();
         a.snaps = b.snaps.as<double>();
         a.partials = b.partials.as<ofg::DslPartial>();
         a.out = b.out.as<ofg::DslOut>();
@@ -333,7 +345,10 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         a.fuse_markets = fuse ? 1 : 0;
         a.fuse_tiles = scen[0].n_tiles;
         a.n_regs = n_regs;
-        if (steps > INT32_MAX) fail(OFG_INVALID_INPUT, "num_steps exceeds the device limit");
+        a.gbm_only = 1;
+        for (size_t sidx = 0; sidx < n_mk; ++sidx)
+            for (int i = 0; i < na; ++i)
+                if (scen[sidx].kind[i] != ofg::DSL_GBM) a.gbm_only = 0;
         hip_check(hipEventRecord(ctx->ev0, st), "event");
         if (rng == OFG_RNG_REFERENCE)
             hip_check(ofg::launch_lsm_seeds(b.jobs.as<ofg::LsmSeedJob>(), 1, seed_threads,

@@ -68,6 +68,10 @@ THREE = dict(assets=[EQ(100.0, 0.2, 0.02), EQ(95.0, 0.25, 0.01), EQ(105.0, 0.3, 
 TWO = dict(assets=[EQ(100.0, 0.3), EQ(90.0, 0.25, 0.02)], corr=[[1.0, 0.3], [0.3, 1.0]],
            rate=0.02)
 ONE = dict(assets=[EQ(100.0, 0.2, 0.01)], corr=[[1.0]], rate=0.03)
+# four assets that all step as GBM (three equities and an FX rate)
+FOUR = dict(assets=[EQ(100.0, 0.2, 0.02), EQ(95.0, 0.25, 0.01), EQ(105.0, 0.3, 0.0),
+                    dict(kind="fx", spot=1.2, vol=0.12, domestic_rate=0.03, foreign_rate=0.015)],
+            corr=_corr(4, 4), rate=0.03)
 
 CASES = [
     (P.KITCHEN_SINK, MIXED, 1_500, 48),
@@ -76,6 +80,7 @@ CASES = [
     (P.phoenix_memory(2, 1.05, 0.75, 0.1, 2.0), TWO, 2_001, 16),
     (P.WORST_DIGITAL, TWO, 3_333, 5),
     (P.CALL, ONE, 4_097, 3),
+    (P.standard_autocall(4, 1.0, 0.07, 1.0), FOUR, 2_222, 20),
 ]
 
 
@@ -105,7 +110,9 @@ def test_external_normals_match_oracle(gpu, case):
     assert r.diagnostics.get("num_paths") == paths and r.diagnostics.get("num_steps") == steps
 
 
-@pytest.mark.parametrize("case", [0, 2])
+# MIXED steps Schwartz and Vasicek assets too; the others are GBM-only
+# markets of 3, 2, 1 and 4 assets
+@pytest.mark.parametrize("case", [0, 2, 3, 5, 6])
 def test_reference_draws_match_oracle(gpu, case):
     from openferric_amd import RngMode, parse_and_compile
     src, spec, paths, steps = CASES[case]
@@ -187,6 +194,31 @@ def test_fast_draws_call_matches_black_scholes(gpu):
     mkt, *_ = _mk(ONE)
     r = _engine(1 << 20, 4, RngMode.Fast).price_multi_asset(parse_and_compile(P.CALL), mkt)
     want = bs_price(True, 100.0, 100.0, 0.03, 0.01, 0.2, 1.0)
+    assert abs(r.price - want) < 4 * r.stderr, (r.price, want, r.stderr)
+
+
+def _forward_sum(n):
+    names = "\n".join(f"        A{i} = asset({i})" for i in range(n))
+    total = " + ".join(f"price({i})" for i in range(n))
+    return (f'product "Forward sum"\n    notional: 1\n    maturity: 1.5\n    underlyings\n'
+            f'{names}\n    schedule 1.5 from 1.5 to 1.5\n        redeem {total}\n')
+
+
+@pytest.mark.parametrize("spec", [ONE, TWO, THREE, FOUR], ids=["1", "2", "3", "4"])
+def test_fast_draws_forwards_match_closed_form(gpu, spec):
+    """Fast draws on GBM-only markets of 1 to 4 assets: the discounted sum of
+    the terminal prices equals the sum of the forwards discounted at the
+    market rate, within 4 standard errors."""
+    from openferric_amd import RngMode, parse_and_compile
+    mkt, ora, corr, rate = _mk(spec)
+    T = 1.5
+    want = 0.0
+    for a in ora:
+        mu = (a["domestic_rate"] - a["foreign_rate"]) if a["kind"] == "fx" else \
+            rate - a.get("dividend_yield", 0.0)
+        want += a["spot"] * math.exp((mu - rate) * T)
+    r = _engine(1 << 20, 9, RngMode.Fast, seed=11).price_multi_asset(
+        parse_and_compile(_forward_sum(len(ora))), mkt)
     assert abs(r.price - want) < 4 * r.stderr, (r.price, want, r.stderr)
 
 
