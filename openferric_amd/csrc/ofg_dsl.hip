@@ -469,9 +469,11 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
 // Held to 3 waves per SIMD (168 VGPRs): left free, four-asset markets take
 // about 195 and run at 2; the few registers spilled cost less than the lost
 // wave (five-market Greeks 16.5 -> 15.7 ms on reference draws).
-template <int RNG, int MAXA>
+// NA > 0: every market has exactly NA = MAXA assets, all stepping as GBM.
+template <int RNG, int MAXA, int NA = 0>
 __global__ __launch_bounds__(DSL_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
 dsl_sim_markets_kernel(DslArgs a) {
+    static_assert(NA == 0 || NA == MAXA, "exact asset count");
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
         for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += DSL_BLOCK) tab[i] = OFG_FAST_TABLE[i];
@@ -479,7 +481,7 @@ dsl_sim_markets_kernel(DslArgs a) {
     }
     const DslScen &o = a.scen[0];
     const StepParams<MAXA> sp0(o);   // market 0's, read once, before any snapshot store
-    const int n = o.n_assets, ns = a.n_scen;
+    const int n = NA > 0 ? NA : o.n_assets, ns = a.n_scen;
     uint32_t share[DSL_FUSE_MAXS];     // every market's share mask and own-factor flag, likewise
     bool own[DSL_FUSE_MAXS];
 #pragma unroll
@@ -496,15 +498,21 @@ dsl_sim_markets_kernel(DslArgs a) {
         for (int m = 0; m < DSL_FUSE_MAXS; ++m)
 #pragma unroll
             for (int i = 0; i < MAXA; ++i) s[m][i] = m < ns ? a.scen[m].s0[i] : 0.0;
-        const int k0 = a.step_snap[0];
-        if (k0 >= 0) {
+        // snapshot steps by a cursor over a.snap_list
+        const int32_t *sl = a.snap_list;
+        int next_step = sl[0], next_k = sl[1];
+        if (next_step == 0) {
 #pragma unroll
             for (int m = 0; m < DSL_FUSE_MAXS; ++m)
                 if (m < ns)
 #pragma unroll
                     for (int i = 0; i < MAXA; ++i)
                         if (i < n)
-                            a.snaps[a.scen[m].snap_off + (int64_t)(k0 * n + i) * o.paths + p] = s[m][i];
+                            a.snaps[a.scen[m].snap_off + (int64_t)(next_k * n + i) * o.paths + p] =
+                                s[m][i];
+            sl += 2;
+            next_step = sl[0];
+            next_k = sl[1];
         }
         Xoshiro256pp ref;
         Xoroshiro128p fast;
@@ -563,10 +571,11 @@ dsl_sim_markets_kernel(DslArgs a) {
                     if (i < n && j2 < n) c0[i] += sp0.chol(i * (i + 1) / 2 + j2) * z[j2];
 #pragma unroll
             for (int i = 0; i < MAXA; ++i)
-                g0[i] = (i < n && sp0.kind(i) == DSL_GBM)
+                g0[i] = (i < n && (NA > 0 || sp0.kind(i) == DSL_GBM))
                             ? gbm_factor<RNG == OFG_RNG_FAST>(sp0.p0(i), sp0.p1(i), c0[i], tab)
                             : 1.0;
-            const int ks = a.step_snap[step];
+            const bool snap_now = step == next_step;
+            const int ks = next_k;
 #pragma unroll
             for (int m = 0; m < DSL_FUSE_MAXS; ++m) {
                 if (m >= ns) continue;
@@ -588,16 +597,24 @@ dsl_sim_markets_kernel(DslArgs a) {
                 for (int i = 0; i < MAXA; ++i) {
                     if (i < n) {
                         if ((share[m] >> i) & 1u) s[m][i] = s[m][i] * g0[i];
+                        else if constexpr (NA > 0)
+                            s[m][i] = s[m][i] * gbm_factor<RNG == OFG_RNG_FAST>(q.p0[i], q.p1[i],
+                                                                                c[i], tab);
                         else
                             s[m][i] = step_asset<RNG == OFG_RNG_FAST>(q.kind[i], s[m][i], q.p0[i],
                                                                       q.p1[i], q.p2[i], c[i], tab);
                     }
                 }
-                if (ks >= 0) {
+                if (snap_now) {
 #pragma unroll
                     for (int i = 0; i < MAXA; ++i)
                         if (i < n) a.snaps[q.snap_off + (int64_t)(ks * n + i) * o.paths + p] = s[m][i];
                 }
+            }
+            if (snap_now) {
+                sl += 2;
+                next_step = sl[0];
+                next_k = sl[1];
             }
         }
     }
@@ -680,10 +697,20 @@ size_t lds_bytes(const DslArgs &a) {
     return (size_t)(a.n_regs > 0 ? a.n_regs : 1) * DSL_BLOCK * sizeof(double);
 }
 
-// The split form's simulation of markets whose assets all step as GBM, at
-// their exact asset count (up to four).
+// The split form's simulation (one market, or several in one pass) of
+// markets whose assets all step as GBM, at their exact asset count (up to
+// four).
 template <int RNG>
 void launch_sim_gbm(const DslArgs &a, hipStream_t stream) {
+    if (a.fuse_markets) {
+        switch (a.n_assets) {
+        case 1: dsl_sim_markets_kernel<RNG, 1, 1><<<a.fuse_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+        case 2: dsl_sim_markets_kernel<RNG, 2, 2><<<a.fuse_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+        case 3: dsl_sim_markets_kernel<RNG, 3, 3><<<a.fuse_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+        default: dsl_sim_markets_kernel<RNG, 4, 4><<<a.fuse_tiles, DSL_BLOCK, 0, stream>>>(a); break;
+        }
+        return;
+    }
     switch (a.n_assets) {
     case 1: dsl_sim_kernel<RNG, 1, 1><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
     case 2: dsl_sim_kernel<RNG, 2, 2><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a); break;
@@ -696,10 +723,10 @@ template <int RNG, int MAXA>
 void launch_rng(const DslArgs &a, hipStream_t stream) {
     if (a.split) {
         if constexpr (MAXA <= DSL_FUSE_MAXA) {
-            if (a.fuse_markets)
-                dsl_sim_markets_kernel<RNG, MAXA><<<a.fuse_tiles, DSL_BLOCK, 0, stream>>>(a);
-            else if (a.gbm_only && a.n_assets >= 1 && a.n_assets <= 4)
+            if (a.gbm_only && a.n_assets >= 1 && a.n_assets <= 4)
                 launch_sim_gbm<RNG>(a, stream);
+            else if (a.fuse_markets)
+                dsl_sim_markets_kernel<RNG, MAXA><<<a.fuse_tiles, DSL_BLOCK, 0, stream>>>(a);
             else
                 dsl_sim_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
         } else {

@@ -248,7 +248,7 @@ def test_fused_form_matches_split_form(gpu, monkeypatch):
     assert abs(fused.stderr - split.stderr) <= 1e-9 * split.stderr
 
 
-@pytest.mark.parametrize("case", [0, 2])
+@pytest.mark.parametrize("case", [0, 2, 3, 6])
 def test_markets_fused_simulation_matches_separate(gpu, monkeypatch, case):
     """Greeks bumps simulated in one pass (shared draws and factors) price as
     markets simulated one by one -- equity, FX, Schwartz and Vasicek assets,
