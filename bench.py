@@ -629,7 +629,7 @@ def dsl_leg(reps=3, cpu=True):
     out = {"workload": f"worst-of autocallable (3 correlated equities, quarterly over 1.5 y, "
                        f"knock-in), {DSL_PATHS} paths x {DSL_STEPS} steps",
            "asset_steps": asset_steps}
-    for mode, name in ((RngMode.Reference, "reference_draws"), (RngMode.Fast, "fast_draws")):
+    def timed(mode):
         eng = DslMonteCarloEngine(DSL_PATHS, DSL_STEPS, 42, rng_mode=mode)
         eng.price_multi_asset(prod, mkt)
         ms = []
@@ -642,11 +642,21 @@ def dsl_leg(reps=3, cpu=True):
             eng.greeks_multi_asset(prod, mkt, 0)
             gms.append(eng.last_kernel_ms)
         g = float(np.mean(gms))
-        out[name] = {"price_ms": k, "asset_steps_per_s": asset_steps / (k * 1e-3),
-                     "greeks_5_markets_ms": g,
-                     "greeks_asset_steps_per_s": 5 * asset_steps / (g * 1e-3),
-                     "frac_fp64_peak": asset_steps / (k * 1e-3) * flop_multi_asset_step(3) / 1e12
-                     / FP64_PEAK_TFLOPS}
+        return {"price_ms": k, "asset_steps_per_s": asset_steps / (k * 1e-3),
+                "greeks_5_markets_ms": g,
+                "greeks_asset_steps_per_s": 5 * asset_steps / (g * 1e-3),
+                "frac_fp64_peak": asset_steps / (k * 1e-3) * flop_multi_asset_step(3) / 1e12
+                / FP64_PEAK_TFLOPS}
+
+    modes = ((RngMode.Reference, "reference_draws"), (RngMode.Fast, "fast_draws"))
+    # the program interpreter (the default), then the product's native
+    # evaluation kernel (compiled at run time, the same prices bit for bit)
+    for mode, name in modes:
+        out[name] = timed(mode)
+    native = {"compile_ms": prod.jit_compile(3)}
+    for mode, name in modes:
+        native[name] = timed(mode)
+    out["native_kernel"] = native
     if cpu:
         # the oracle's compiled correlated-path pricer (or_autocall_price: the
         # same three correlated GBM assets over 378 steps drawn path after

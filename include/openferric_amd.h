@@ -689,6 +689,25 @@ int32_t ofg_dsl_price(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_ds
                       size_t n_markets, const ofg_dsl_engine *engine, const ofg_engine_cfg *cfg,
                       ofg_result *out);
 
+/* Compiles the product's programs, for markets of n_assets assets, into a
+ * native gfx950 evaluation kernel at run time (hipRTC) and attaches it to the
+ * context: later ofg_dsl_price calls of this product with that asset count
+ * run it in place of the predicated-program interpreter (split form, which
+ * the path and market counts of a call select as before), with prices equal
+ * bit for bit. The reference's counterpart is the cranelift JIT of the
+ * product bytecode (src/dsl/jit.rs:168-200, JitCompiledProgram::compile,
+ * feature "jit"). Compiling takes a few hundred milliseconds; a product is
+ * compiled once per context and asset count (later calls return at once).
+ * compile_ms (may be NULL) receives the compile and load time.
+ * OFG_DEVICE_ERROR with hipRTC's log when compiling fails. */
+int32_t ofg_dsl_jit_compile(ofg_ctx *ctx, const ofg_dsl_product *product, int32_t n_assets,
+                            double *compile_ms);
+
+/* Native evaluation kernels attached to the context and how many times
+ * ofg_dsl_price has launched one. No counterpart in the reference; for tests
+ * and measurement. Either output may be NULL. */
+int32_t ofg_dsl_jit_stats(ofg_ctx *ctx, int64_t *kernels, int64_t *launches);
+
 /* Device time of the context's last ofg_lsm_price call, split into path
  * simulation (seed and path kernels) and the backward rounds, and the number
  * of round launches. No counterpart in the reference; for measurement.

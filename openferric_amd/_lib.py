@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = [
     "ofg_dsl_price", "ofg_price_noalloc", "ofg_finite_difference_greeks",
     "ofg_bump_and_reprice", "ofg_european_exact", "ofg_spread_price",
     "ofg_inject_alloc_failure", "ofg_fill_normals", "ofg_price_device",
+    "ofg_dsl_jit_compile", "ofg_dsl_jit_stats",
 ]
 
 
@@ -336,6 +337,10 @@ def load() -> C.CDLL:
                                     C.POINTER(OfgDslEngine), C.POINTER(OfgEngineCfg),
                                     C.POINTER(OfgResult)]
         L.ofg_dsl_price.restype = C.c_int32
+        L.ofg_dsl_jit_compile.argtypes = [pv, pv, C.c_int32, C.POINTER(C.c_double)]
+        L.ofg_dsl_jit_compile.restype = C.c_int32
+        L.ofg_dsl_jit_stats.argtypes = [pv, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.ofg_dsl_jit_stats.restype = C.c_int32
         _lib = L
         return L
 

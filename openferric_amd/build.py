@@ -17,11 +17,13 @@ SOURCES = ["ofg_kernels.hip", "ofg_adjoint.hip", "ofg_mcgreeks.hip", "ofg_pathwi
            "ofg_lsm.hip", "ofg_qmc.hip", "ofg_multi.hip", "ofg_dsl.hip", "ofg_api.cpp",
            "ofg_lsm_api.cpp", "ofg_qmc_api.cpp", "ofg_multi_api.cpp", "ofg_basket_api.cpp",
            "ofg_dsl_front.cpp", "ofg_dsl_api.cpp", "ofg_exact.hip", "ofg_exact_api.cpp",
-           "ofg_spread.hip", "ofg_spread_api.cpp", "ofg_plan.cpp", "ofg_normals.hip", "ofg_device_io.hip"]
+           "ofg_spread.hip", "ofg_spread_api.cpp", "ofg_plan.cpp", "ofg_normals.hip", "ofg_device_io.hip",
+           "ofg_dsl_jit.cpp"]
 HEADERS = ["ofg_math.hpp", "ofg_internal.hpp", "ofg_adjoint.hpp", "ofg_paths.hpp",
            "ofg_mcgreeks.hpp", "ofg_pathwise.hpp", "ofg_lsm.hpp", "ofg_qmc.hpp", "ofg_multi.hpp",
            "ofg_host.hpp", "ofg_jump.hpp", "ofg_corr.hpp", "ofg_dsl.hpp", "ofg_dsl_front.hpp",
-           "ofg_fast_tables.inc", "ofg_exact.hpp", "ofg_spread.hpp", "ofg_plan.hpp", "ofg_normals.hpp", "ofg_device_io.hpp"]
+           "ofg_fast_tables.inc", "ofg_exact.hpp", "ofg_spread.hpp", "ofg_plan.hpp", "ofg_normals.hpp", "ofg_device_io.hpp",
+           "ofg_dsl_jit.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -77,7 +79,8 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as pool:
         list(pool.map(one, todo))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    # hipRTC: the structured-product engine compiles native product kernels at run time
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -720,7 +720,8 @@ void launch_sim_gbm(const DslArgs &a, hipStream_t stream) {
 }
 
 template <int RNG, int MAXA>
-void launch_rng(const DslArgs &a, hipStream_t stream) {
+hipError_t launch_rng(const DslArgs &a, hipStream_t stream, hipFunction_t jit_eval,
+                      const DslJitArgs *jit_args) {
     if (a.split) {
         if constexpr (MAXA <= DSL_FUSE_MAXA) {
             if (a.gbm_only && a.n_assets >= 1 && a.n_assets <= 4)
@@ -732,28 +733,40 @@ void launch_rng(const DslArgs &a, hipStream_t stream) {
         } else {
             dsl_sim_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, 0, stream>>>(a);
         }
+        if (jit_eval) {
+            DslJitArgs ja = *jit_args;
+            void *params[] = {&ja};
+            return hipModuleLaunchKernel(jit_eval, (unsigned)a.n_tiles, 1, 1, DSL_BLOCK, 1, 1, 0,
+                                         stream, params, nullptr);
+        }
         dsl_eval_kernel<<<a.n_tiles, DSL_BLOCK, lds_bytes(a), stream>>>(a);
     } else {
         dsl_kernel<RNG, MAXA><<<a.n_tiles, DSL_BLOCK, lds_bytes(a), stream>>>(a);
     }
+    return hipSuccess;
 }
 
 template <int MAXA>
-void launch_tiles(const DslArgs &a, hipStream_t stream) {
+hipError_t launch_tiles(const DslArgs &a, hipStream_t stream, hipFunction_t jit_eval,
+                        const DslJitArgs *jit_args) {
     switch (a.rng_mode) {
-    case OFG_RNG_FAST: launch_rng<OFG_RNG_FAST, MAXA>(a, stream); break;
-    case OFG_RNG_EXTERNAL: launch_rng<OFG_RNG_EXTERNAL, MAXA>(a, stream); break;
-    default: launch_rng<OFG_RNG_REFERENCE, MAXA>(a, stream); break;
+    case OFG_RNG_FAST: return launch_rng<OFG_RNG_FAST, MAXA>(a, stream, jit_eval, jit_args);
+    case OFG_RNG_EXTERNAL: return launch_rng<OFG_RNG_EXTERNAL, MAXA>(a, stream, jit_eval, jit_args);
+    default: return launch_rng<OFG_RNG_REFERENCE, MAXA>(a, stream, jit_eval, jit_args);
     }
 }
 
-hipError_t launch_dsl(const DslArgs &a, hipStream_t stream) {
+hipError_t launch_dsl(const DslArgs &a, hipStream_t stream, hipFunction_t jit_eval,
+                      const DslJitArgs *jit_args) {
     if (a.n_tiles <= 0) return hipSuccess;
-    if (a.n_assets <= 1) launch_tiles<1>(a, stream);
-    else if (a.n_assets <= 2) launch_tiles<2>(a, stream);
-    else if (a.n_assets <= 4) launch_tiles<4>(a, stream);
-    else if (a.n_assets <= 8) launch_tiles<8>(a, stream);
-    else launch_tiles<DSL_MAX_ASSETS>(a, stream);
+    if (jit_eval && (!jit_args || !a.split)) return hipErrorInvalidValue;
+    hipError_t le;
+    if (a.n_assets <= 1) le = launch_tiles<1>(a, stream, jit_eval, jit_args);
+    else if (a.n_assets <= 2) le = launch_tiles<2>(a, stream, jit_eval, jit_args);
+    else if (a.n_assets <= 4) le = launch_tiles<4>(a, stream, jit_eval, jit_args);
+    else if (a.n_assets <= 8) le = launch_tiles<8>(a, stream, jit_eval, jit_args);
+    else le = launch_tiles<DSL_MAX_ASSETS>(a, stream, jit_eval, jit_args);
+    if (le != hipSuccess) return le;
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     dsl_close_kernel<<<a.n_scen, DSL_BLOCK, 0, stream>>>(a);

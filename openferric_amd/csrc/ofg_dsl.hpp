@@ -15,6 +15,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace ofg {
@@ -139,6 +140,51 @@ struct DslArgs {
     const double *state_init;   // [n_state]
 };
 
-hipError_t launch_dsl(const DslArgs &a, hipStream_t stream);
+// Native evaluation kernel of one product (compiled at run time by
+// ofg_dsl_jit_compile, ofg_dsl_jit.cpp): the layouts below are pasted into
+// its generated source as text, so host and kernel agree on them by
+// construction (one declaration per field, no commas).
+#define OFG_DSL_JIT_LAYOUTS                                                           \
+    struct DslJitScen {                                                               \
+        long long paths;                                                              \
+        long long snap_off;                                                           \
+        int tile_first;                                                               \
+        int ppt;                                                                      \
+        int df_off;                                                                   \
+        int pad;                                                                      \
+        double s0[16];                                                                \
+    };                                                                                \
+    struct DslJitObs {                                                                \
+        int snap;                                                                     \
+        int is_final;                                                                 \
+        double date;                                                                  \
+    };                                                                                \
+    struct DslJitPartial {                                                            \
+        double sum;                                                                   \
+        double sum_sq;                                                                \
+        double err_idx;                                                               \
+        double pad;                                                                   \
+    };                                                                                \
+    struct DslJitArgs {                                                               \
+        const int *tile_scen;                                                         \
+        const DslJitScen *scen;                                                       \
+        const double *snaps;                                                          \
+        const DslJitObs *obs;                                                         \
+        const double *df;                                                             \
+        DslJitPartial *partials;                                                      \
+    };
+OFG_DSL_JIT_LAYOUTS
+#define OFG_DSL_STR_(...) #__VA_ARGS__
+#define OFG_DSL_STR(...) OFG_DSL_STR_(__VA_ARGS__)
+static_assert(sizeof(DslJitObs) == sizeof(DslObs) && offsetof(DslJitObs, date) == offsetof(DslObs, date),
+              "observation layout");
+static_assert(sizeof(DslJitPartial) == sizeof(DslPartial), "partial-sum layout");
+static_assert(sizeof(((DslJitScen *)nullptr)->s0) == sizeof(((DslScen *)nullptr)->s0), "spots");
+
+// Launches the split or fused form and the close kernel. jit_eval, when not
+// null, is the product's native evaluation kernel, run with jit_args in place
+// of the program interpreter (split form only).
+hipError_t launch_dsl(const DslArgs &a, hipStream_t stream, hipFunction_t jit_eval = nullptr,
+                      const DslJitArgs *jit_args = nullptr);
 
 }  // namespace ofg

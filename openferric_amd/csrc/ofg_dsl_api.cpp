@@ -13,6 +13,7 @@
 #include "ofg_corr.hpp"
 #include "ofg_dsl.hpp"
 #include "ofg_dsl_front.hpp"
+#include "ofg_dsl_jit.hpp"
 #include "ofg_host.hpp"
 #include "ofg_internal.hpp"
 #include "ofg_jump.hpp"
@@ -21,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -33,13 +35,17 @@ using namespace ofg_corr;
 
 struct ofg_dsl_product {
     std::unique_ptr<ofg_dsl::Product> prod;
+    uint64_t uid = 0;   // names the product's native kernels in a context
 };
 
 namespace ofg_host {
 
 struct DslBuffers {
     DevBuf scen, tiles, code, consts, sched, obs, df, step_snap, snap_list, snaps, partials, out,
-        normals, seeds, polys, jobs, state;
+        normals, seeds, polys, jobs, state, jit_scen;
+    // native evaluation kernels by (product uid, asset count)
+    std::map<std::pair<uint64_t, int32_t>, ofg_dsl_jit::Kernel> jit;
+    int64_t jit_launches = 0;
 };
 
 void dsl_release(ofg_ctx *ctx) {
@@ -48,8 +54,10 @@ void dsl_release(ofg_ctx *ctx) {
     for (DevBuf *d : {&b->scen, &b->tiles, &b->code, &b->consts, &b->sched, &b->obs, &b->df,
                       &b->step_snap, &b->snap_list, &b->snaps, &b->partials, &b->out, &b->normals,
                       &b->seeds,
-                      &b->polys, &b->jobs, &b->state})
+                      &b->polys, &b->jobs, &b->state, &b->jit_scen})
         d->release();
+    for (auto &kv : b->jit)
+        if (kv.second.module) (void)hipModuleUnload(kv.second.module);
     delete b;
     ctx->dsl = nullptr;
 }
@@ -123,27 +131,20 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
     }
     if (na > ofg::DSL_MAX_ASSETS) fail(OFG_INVALID_INPUT, "at most 16 assets per market");
     if (steps > INT32_MAX) fail(OFG_INVALID_INPUT, "num_steps exceeds the device limit");
-    // build_execution_plan (eval.rs:268-330)
-    int n_locals = 0;
-    std::vector<ofg_dsl::Program> progs;
-    try {
-        progs = ofg_dsl::build_programs(prod, n_locals);
-    } catch (const ofg_dsl::DslError &e) {
-        fail(OFG_NUMERICAL_ERROR, e.what());
-    }
+    // build_execution_plan (eval.rs:268-330): the programs, then the
+    // observation -> snapshot step map
+    const ofg_dsl_jit::ProgramImage im = ofg_dsl_jit::assemble(prod);
+    const std::vector<ofg::DslSched> &sched = im.sched;
+    const std::vector<ofg::DslIns> &code = im.code;
+    const std::vector<double> &consts = im.consts;
+    const std::vector<double> &state_init = im.state_init;
+    const int32_t n_locals = im.n_locals, n_regs = im.n_regs, n_masks = im.n_masks;
     const double maturity = prod.maturity;
     const double step_scale = (double)steps / maturity;
     std::vector<int32_t> step_snap((size_t)steps + 1, -1);
-    std::vector<ofg::DslSched> sched;
     std::vector<ofg::DslObs> obs;
-    std::vector<ofg::DslIns> code;
-    std::vector<double> consts;
-    int32_t n_snaps = 0, n_regs = 0, n_masks = 0;
-    for (size_t k = 0; k < prod.schedules.size(); ++k) {
-        const ofg_dsl::Schedule &sc = prod.schedules[k];
-        ofg::DslSched d{};
-        d.obs_first = (int32_t)obs.size();
-        d.n_obs = (int32_t)sc.dates.size();
+    int32_t n_snaps = 0;
+    for (const ofg_dsl::Schedule &sc : prod.schedules) {
         for (size_t j = 0; j < sc.dates.size(); ++j) {
             const int64_t step = std::min<int64_t>(round_to_step(sc.dates[j] * step_scale), steps);
             if (step_snap[(size_t)step] < 0) step_snap[(size_t)step] = n_snaps++;
@@ -153,24 +154,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
             o.date = sc.dates[j];
             obs.push_back(o);
         }
-        ofg_dsl::Program &pg = progs[k];
-        const int32_t c0 = (int32_t)consts.size();
-        for (ofg::DslIns in : pg.code) {
-            if (in.op == ofg::DOP_CONST) in.aux += (uint32_t)c0;
-            code.push_back(in);
-        }
-        d.code_off = (int32_t)(code.size() - pg.code.size());
-        d.code_len = (int32_t)pg.code.size();
-        consts.insert(consts.end(), pg.consts.begin(), pg.consts.end());
-        n_regs = std::max(n_regs, pg.n_regs);
-        n_masks = std::max(n_masks, pg.n_masks);
-        sched.push_back(d);
     }
-    n_regs = std::max<int32_t>(n_regs, n_locals + (int32_t)prod.state_vars.size());
-    if (n_regs > ofg::DSL_MAX_REGS || n_masks > ofg::DSL_MAX_MASKS)
-        fail(OFG_INVALID_INPUT, "product program exceeds the device limits (32 registers, 16 nested ifs)");
-    std::vector<double> state_init;
-    for (const auto &sv : prod.state_vars) state_init.push_back(sv.initial);
 
     auto fill_result = [&](size_t i, double sum, double sq) {
         const double nn = (double)paths;
@@ -294,10 +278,10 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         up(b.df, dfs.data(), dfs.size() * sizeof(double), "H2D dsl discount factors");
         up(b.step_snap, step_snap.data(), step_snap.size() * sizeof(int32_t), "H2D dsl snapshot map");
         std::vector<int32_t> snap_list;   // the snapshot steps in order, then a sentinel
-        for (size_t st = 0; st < step_snap.size(); ++st) {
-            if (step_snap[st] < 0) continue;
-            snap_list.push_back((int32_t)st);
-            snap_list.push_back(step_snap[st]);
+        for (size_t k = 0; k < step_snap.size(); ++k) {
+            if (step_snap[k] < 0) continue;
+            snap_list.push_back((int32_t)k);
+            snap_list.push_back(step_snap[k]);
         }
         snap_list.push_back((int32_t)steps + 1);
         snap_list.push_back(-1);
@@ -354,7 +338,32 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
             hip_check(ofg::launch_lsm_seeds(b.jobs.as<ofg::LsmSeedJob>(), 1, seed_threads,
                                             b.polys.as<uint64_t>(), b.seeds.as<uint64_t>(), st),
                       "dsl seed kernel launch");
-        hip_check(ofg::launch_dsl(a, st), "dsl kernel launch");
+        // the product's native evaluation kernel, when one is attached
+        hipFunction_t jit_fn = nullptr;
+        ofg::DslJitArgs ja{};
+        const auto jk = b.jit.find({product->uid, (int32_t)na});
+        if (split && jk != b.jit.end()) {
+            std::vector<ofg::DslJitScen> js(n_mk);
+            for (size_t i = 0; i < n_mk; ++i) {
+                const ofg::DslScen &d = scen[i];
+                js[i].paths = d.paths;
+                js[i].snap_off = d.snap_off;
+                js[i].tile_first = d.tile_first;
+                js[i].ppt = d.ppt;
+                js[i].df_off = d.df_off;
+                std::memcpy(js[i].s0, d.s0, sizeof js[i].s0);
+            }
+            up(b.jit_scen, js.data(), js.size() * sizeof(js[0]), "H2D dsl native-kernel scenarios");
+            ja.tile_scen = a.tile_scen;
+            ja.scen = b.jit_scen.as<ofg::DslJitScen>();
+            ja.snaps = a.snaps;
+            ja.obs = reinterpret_cast<const ofg::DslJitObs *>(a.obs);
+            ja.df = a.df;
+            ja.partials = reinterpret_cast<ofg::DslJitPartial *>(a.partials);
+            jit_fn = jk->second.fn;
+            ++b.jit_launches;
+        }
+        hip_check(ofg::launch_dsl(a, st, jit_fn, jit_fn ? &ja : nullptr), "dsl kernel launch");
         hip_check(hipEventRecord(ctx->ev1, st), "event");
         hip_check(hipMemcpyAsync(res.data(), a.out, res.size() * sizeof(res[0]),
                                  hipMemcpyDeviceToHost, st),
@@ -390,12 +399,45 @@ extern "C" int32_t ofg_dsl_compile(const char *source, ofg_dsl_product **out) {
         } catch (const ofg_dsl::DslError &e) {
             fail(OFG_INVALID_INPUT, e.what());   // From<DslError> for PricingError
         }
+        static std::atomic<uint64_t> next_uid{1};
+        p->uid = next_uid.fetch_add(1);
         *out = p.release();
         return OFG_OK;
     });
 }
 
 extern "C" void ofg_dsl_free(ofg_dsl_product *product) { delete product; }
+
+extern "C" int32_t ofg_dsl_jit_compile(ofg_ctx *ctx, const ofg_dsl_product *product,
+                                       int32_t n_assets, double *compile_ms) {
+    return guarded([&]() {
+        if (compile_ms) *compile_ms = 0.0;
+        if (!ctx || !product || !product->prod) fail(OFG_INVALID_INPUT, "null argument");
+        if (n_assets < 1 || n_assets > ofg::DSL_MAX_ASSETS)
+            fail(OFG_INVALID_INPUT, "n_assets must be in 1..16");
+        const ofg_dsl::Product &prod = *product->prod;
+        const ofg_dsl_jit::ProgramImage im = ofg_dsl_jit::assemble(prod);
+        const std::string src = ofg_dsl_jit::kernel_source(im, prod.notional, n_assets);
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!ctx->dsl) ctx->dsl = new DslBuffers();
+        DslBuffers &b = *ctx->dsl;
+        const std::pair<uint64_t, int32_t> key{product->uid, n_assets};
+        if (b.jit.count(key)) return OFG_OK;   // already attached
+        b.jit[key] = ofg_dsl_jit::compile(src, compile_ms);
+        return OFG_OK;
+    });
+}
+
+extern "C" int32_t ofg_dsl_jit_stats(ofg_ctx *ctx, int64_t *kernels, int64_t *launches) {
+    return guarded([&]() {
+        if (!ctx) fail(OFG_INVALID_INPUT, "null argument");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        if (kernels) *kernels = ctx->dsl ? (int64_t)ctx->dsl->jit.size() : 0;
+        if (launches) *launches = ctx->dsl ? ctx->dsl->jit_launches : 0;
+        return OFG_OK;
+    });
+}
 
 extern "C" int32_t ofg_dsl_get_info(const ofg_dsl_product *product, ofg_dsl_info *info) {
     return guarded([&]() {

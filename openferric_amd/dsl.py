@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field, replace
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -46,6 +46,19 @@ class CompiledProduct:
         _raise(L.ofg_dsl_schedule_dates(self._h, k, buf, n.value, C.byref(n)))
         return list(buf[:n.value])
 
+    def jit_compile(self, n_assets: Optional[int] = None, device: int = 0) -> float:
+        """Compiles this product's programs into a native evaluation kernel
+        for markets of n_assets assets (default: the product's underlyings)
+        on `device`'s context: later prices of the product there run it, with
+        the interpreter's results bit for bit (ofg_dsl_jit_compile; the
+        counterpart of JitCompiledProgram::compile, jit.rs:168-200). Returns
+        the compile time in ms (0 when already compiled)."""
+        L = _lib.load()
+        ms = C.c_double(0.0)
+        n = self.num_underlyings if n_assets is None else int(n_assets)
+        _raise(L.ofg_dsl_jit_compile(_lib.context(device), self._h, n, C.byref(ms)))
+        return ms.value
+
     def max_local_slots(self) -> int:
         info = _lib.OfgDslInfo()
         _raise(_lib.load().ofg_dsl_get_info(self._h, C.byref(info)))
@@ -59,6 +72,14 @@ class CompiledProduct:
             except Exception:
                 pass
             self._h = C.c_void_p(0)
+
+
+def jit_stats(device: int = 0) -> Tuple[int, int]:
+    """(native evaluation kernels attached to `device`'s context, launches of
+    them so far)."""
+    k, n = C.c_int64(0), C.c_int64(0)
+    _raise(_lib.load().ofg_dsl_jit_stats(_lib.context(device), C.byref(k), C.byref(n)))
+    return k.value, n.value
 
 
 def parse_and_compile(source: str) -> CompiledProduct:

@@ -17,6 +17,7 @@ against the oracle's restatement (oracle/dsl_oracle.py).
 * fast draws: a call against Black-Scholes within 4 standard errors.
 """
 import math
+import re
 
 import numpy as np
 import pytest
@@ -280,3 +281,108 @@ def test_markets_fused_simulation_matches_separate(gpu, monkeypatch, case):
         monkeypatch.delenv("OFG_DSL_FUSE")
         for f, s_ in zip(fused, apart):
             assert abs(f - s_) <= 1e-12 * abs(s_) + 1e-12, (f, s_)
+
+
+# ---- native evaluation kernels (run-time compiled product programs) ----
+
+def _all_products():
+    """Every product of the test corpus that the library compiles, with a
+    market of its underlyings' count."""
+    from openferric_amd import parse_and_compile
+    out = []
+    for name, src in sorted(vars(P).items()):
+        if not (isinstance(src, str) and src.startswith("product")):
+            continue
+        try:
+            prod = parse_and_compile(src)
+        except Exception:
+            continue
+        n = max(1, prod.num_underlyings)
+        spec = {1: ONE, 2: TWO, 3: THREE, 4: MIXED}.get(n)
+        if spec is not None:
+            out.append((name, src, spec))
+    for n, spec in ((1, ONE), (2, TWO), (3, THREE), (4, FOUR)):
+        out.append((f"standard_autocall_{n}", P.standard_autocall(n, 1.0, 0.07, 1.0), spec))
+        out.append((f"phoenix_memory_{n}", P.phoenix_memory(n, 1.05, 0.75, 0.1, 2.0), spec))
+        out.append((f"forward_sum_{n}", _forward_sum(n), spec))
+    import bench
+    out.append(("bench_worst_of_autocall", bench.DSL_PRODUCT, THREE))
+    return out
+
+
+@pytest.mark.parametrize("mode", ["reference", "fast", "external"])
+def test_native_kernel_prices_equal_interpreter(gpu, mode):
+    """Every corpus product, priced by the interpreter and then by its native
+    kernel, on the same draws: prices and standard errors bit for bit; each
+    native price is one launch of the native kernel."""
+    from openferric_amd import RngMode, parse_and_compile
+    from openferric_amd.dsl import jit_stats
+    rng = {"reference": RngMode.Reference, "fast": RngMode.Fast, "external": RngMode.External}[mode]
+    checked = 0
+    for name, src, spec in _all_products():
+        mkt, ora, corr, rate = _mk(spec)
+        paths, steps = 3_001, 12
+        z = None
+        if rng == RngMode.External:
+            z = np.transpose(np.random.default_rng(9).standard_normal((paths, steps, len(ora))),
+                             (1, 2, 0))
+        eng = _engine(paths, steps, rng, normals=z, seed=3)
+        prod = parse_and_compile(src)
+        try:
+            want = eng.price_multi_asset(prod, mkt)
+        except Exception as e:   # the product's own error: the native kernel must give it too
+            try:
+                prod.jit_compile(len(ora))
+            except type(e):
+                continue   # raised when building the program, before any kernel
+            with pytest.raises(type(e), match=re.escape(str(e))):
+                eng.price_multi_asset(prod, mkt)
+            continue
+        ms = prod.jit_compile(len(ora))
+        assert ms > 0.0
+        assert prod.jit_compile(len(ora)) == 0.0   # attached once per context
+        before = jit_stats()[1]
+        got = eng.price_multi_asset(prod, mkt)
+        assert jit_stats()[1] == before + 1, name
+        assert got.price == want.price and got.stderr == want.stderr, (name, got.price, want.price)
+        checked += 1
+    assert checked >= 15, checked
+
+
+def test_native_kernel_greeks_markets_and_errors(gpu):
+    """The native kernel under several markets in one call (the fused
+    simulation) and the interpreter's error for an out-of-range price()
+    index, with the masked index of a branch no path takes still allowed."""
+    from openferric_amd import MultiAssetMarket, NumericalError, RngMode, parse_and_compile
+    src, spec, paths, steps = CASES[2]
+    mkt, *_ = _mk(spec)
+    bumps = [mkt, MultiAssetMarket(mkt.assets, mkt.correlation, mkt.rate + 0.01)]
+    eng = _engine(paths, steps, RngMode.Reference, seed=13)
+    prod = parse_and_compile(src)
+    want = [r.price for r in eng.price_markets(prod, bumps)]
+    g_want = eng.greeks_multi_asset(prod, mkt, 1)
+    prod.jit_compile(3)
+    assert [r.price for r in eng.price_markets(prod, bumps)] == want
+    g = eng.greeks_multi_asset(prod, mkt, 1)
+    assert (g.delta, g.gamma, g.vega, g.rho) == (g_want.delta, g_want.gamma, g_want.vega,
+                                                 g_want.rho)
+    one = MultiAssetMarket.single(100.0, 0.2, 0.01, 0.0)
+    e1 = _engine(1000, 4, RngMode.Reference)
+    bad = parse_and_compile(P.OUT_OF_RANGE_PRICE)
+    bad.jit_compile(1)
+    with pytest.raises(NumericalError, match=r"eval error: asset index 5 out of range \(have 1 "
+                                             r"assets\)"):
+        e1.price_multi_asset(bad, one)
+    masked = parse_and_compile(P.MASKED_BAD_INDEX)
+    want_m = e1.price_multi_asset(masked, one).price
+    masked.jit_compile(1)
+    assert e1.price_multi_asset(masked, one).price == want_m
+
+
+def test_native_kernel_invalid_arguments(gpu):
+    from openferric_amd import InvalidInput, parse_and_compile
+    prod = parse_and_compile(P.CALL)
+    with pytest.raises(InvalidInput, match="n_assets must be in 1..16"):
+        prod.jit_compile(0)
+    with pytest.raises(InvalidInput, match="n_assets must be in 1..16"):
+        prod.jit_compile(17)
