@@ -30,6 +30,7 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int NWAVES = LSM_BLOCK / WAVE;
+constexpr int LSM_AK = 8;   // reference draws per inverse-CDF block
 
 __device__ __forceinline__ double lsm_intrinsic(bool put, double s, double k) {
     return put ? fmax(k - s, 0.0) : fmax(s - k, 0.0);
@@ -260,11 +261,13 @@ __global__ __launch_bounds__(LSM_BLOCK) void lsm_close_kernel(LsmArgs a) {
 
 // Drives one path's draws: step(z1, z2) once per path step. Reference draws
 // walk the path's slice of the engine's single stream (z1 then z2 per Heston
-// step), external normals read [step][path], the fast mode runs Box-Muller on
-// a per-path xoroshiro128+ stream.
+// step), 8 at a time with the tail values inverted per wave (slots: the
+// wave's LDS for them; a tile is one option, so every lane has the same step
+// count); external normals read [step][path], the fast mode runs Box-Muller
+// on a per-path xoroshiro128+ stream.
 template <int RNG, bool TWO, class Step>
 __device__ __forceinline__ void drive(const LsmDev &o, const LsmArgs &a, int64_t p,
-                                      const double *tab, Step &&step) {
+                                      const double *tab, double *slots, Step &&step) {
     const int n = o.m.steps;
     if constexpr (RNG == OFG_RNG_REFERENCE) {
         const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
@@ -273,8 +276,8 @@ __device__ __forceinline__ void drive(const LsmDev &o, const LsmArgs &a, int64_t
         rng.s1 = sd[1];
         rng.s2 = sd[2];
         rng.s3 = sd[3];
-        reference_steps<8, TWO ? 2 : 1>(rng, n,
-                                        [&](int j, double z1, double z2) { step(z1, z2, j + 1); });
+        reference_steps_wave<LSM_AK, TWO ? 2 : 1>(
+            rng, n, [&](int j, double z1, double z2) { step(z1, z2, j + 1); }, slots);
     } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
         const double *e1 = a.normals1 + o.normals_offset + p;
         const double *e2 = TWO ? a.normals2 + o.normals_offset + p : nullptr;
@@ -322,7 +325,8 @@ __device__ __forceinline__ void drive(const LsmDev &o, const LsmArgs &a, int64_t
 // terminal cash flow discounted to the latest date.
 template <int RNG, int DYN>
 __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int64_t p,
-                                         const double *tab, const RoundPlan &rp, LsmAcc &acc) {
+                                         const double *tab, double *slots, const RoundPlan &rp,
+                                         LsmAcc &acc) {
     constexpr bool FAST = RNG == OFG_RNG_FAST;
     const LsmDate *dates = a.dates + o.date_offset;
     double *sp = a.spots + o.s_offset + p;
@@ -337,10 +341,13 @@ __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int6
             next = k < o.n_ex ? dates[k].step : 0x7fffffff;
         }
     };
+    // reference draws: growth factors by the 2^(j/256) table (tab), as the
+    // pricing kernel's reference mode
+    constexpr bool TABLE = RNG == OFG_RNG_REFERENCE;
     if constexpr (DYN == LSM_HESTON) {
-        drive<RNG, true>(o, a, p, tab, [&](double z1, double z2, int t) {
+        drive<RNG, true>(o, a, p, tab, slots, [&](double z1, double z2, int t) {
             double sn, vn, growth, e;
-            heston_euler<!FAST>(s, v, o.m, z1, z2, sn, vn, growth, e, tab);
+            heston_euler<!FAST, TABLE>(s, v, o.m, z1, z2, sn, vn, growth, e, tab);
             s = sn;   // fmax(s_next, 1e-12) (:272)
             v = vn;
             record(t);
@@ -348,17 +355,24 @@ __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int6
     } else if constexpr (DYN == LSM_LOCAL_VOL) {
         // flat surface: sigma(S, t) = vol; s *= exp(drift + sigma sqrt(dt) z),
         // floored at 1e-12 (:231-240)
-        drive<RNG, false>(o, a, p, tab, [&](double z, double, int t) {
+        drive<RNG, false>(o, a, p, tab, slots, [&](double z, double, int t) {
             double g;
-            if constexpr (FAST) g = exp2_256(o.m.drift2 + o.m.diffusion2 * z, tab);
-            else g = exp_fp64(o.m.drift + o.m.diffusion * z);
+            if constexpr (FAST) {
+                g = exp2_256(o.m.drift2 + o.m.diffusion2 * z, tab);
+            } else if constexpr (TABLE) {
+                const double x = o.m.drift + o.m.diffusion * z;
+                g = exp2_256t(fmin(fmax(x * OFG_INV_LN2_256, -OFG_EXP2_256_YMAX),
+                                   OFG_EXP2_256_YMAX), tab);
+            } else {
+                g = exp_fp64(o.m.drift + o.m.diffusion * z);
+            }
             s = fmax(s * g, 1.0e-12);
             record(t);
         });
     } else {
-        drive<RNG, false>(o, a, p, tab, [&](double z, double, int t) {
+        drive<RNG, false>(o, a, p, tab, slots, [&](double z, double, int t) {
             double x;
-            s *= gbm_growth<FAST>(o.m, z, tab, x);
+            s *= gbm_growth<FAST, TABLE>(o.m, z, tab, x);
             record(t);
         });
     }
@@ -379,15 +393,15 @@ __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int6
 // path point (:704-729).
 template <int RNG>
 __device__ __forceinline__ void barrier_path(const LsmDev &o, const LsmArgs &a, int64_t p,
-                                             const double *tab, LsmAcc &acc) {
+                                             const double *tab, double *slots, LsmAcc &acc) {
     constexpr bool FAST = RNG == OFG_RNG_FAST;
     const double level = o.barrier_level;
     const bool down = o.barrier_down != 0;
     double s = o.m.s0;
     bool hit = down ? s <= level : s >= level;
-    drive<RNG, false>(o, a, p, tab, [&](double z, double, int) {
+    drive<RNG, false>(o, a, p, tab, slots, [&](double z, double, int) {
         double x;
-        s *= gbm_growth<FAST>(o.m, z, tab, x);
+        s *= gbm_growth<FAST, RNG == OFG_RNG_REFERENCE>(o.m, z, tab, x);
         hit = hit || (down ? s <= level : s >= level);
     });
     const bool active = o.knock_in ? hit : !hit;
@@ -399,9 +413,17 @@ __device__ __forceinline__ void barrier_path(const LsmDev &o, const LsmArgs &a, 
 
 template <int RNG>
 __global__ __launch_bounds__(LSM_BLOCK) void lsm_paths_kernel(LsmArgs a) {
-    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
+    constexpr bool REF = RNG == OFG_RNG_REFERENCE;
+    // the fast mode's tables, or the reference draws' 2^(j/256) table and
+    // each wave's slots for the tail values of an inverse-CDF block
+    __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : REF ? 256 : 1];
+    __shared__ double tail_slots[REF ? LSM_BLOCK * LSM_AK : 1];
+    double *slots = tail_slots + (REF ? (threadIdx.x / WAVE) * WAVE * LSM_AK : 0);
     if constexpr (RNG == OFG_RNG_FAST) {
         for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += LSM_BLOCK) tab[i] = OFG_FAST_TABLE[i];
+        __syncthreads();
+    } else if constexpr (REF) {
+        for (int i = threadIdx.x; i < 256; i += LSM_BLOCK) tab[i] = OFG_FAST_TABLE[OFG_TAB_EXP2 + i];
         __syncthreads();
     }
     const int tile = blockIdx.x;
@@ -414,10 +436,11 @@ __global__ __launch_bounds__(LSM_BLOCK) void lsm_paths_kernel(LsmArgs a) {
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * LSM_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        if (o.barrier) barrier_path<RNG>(o, a, p, tab, acc);
-        else if (o.dynamics == LSM_HESTON) lsm_path<RNG, LSM_HESTON>(o, a, p, tab, rp, acc);
-        else if (o.dynamics == LSM_LOCAL_VOL) lsm_path<RNG, LSM_LOCAL_VOL>(o, a, p, tab, rp, acc);
-        else lsm_path<RNG, LSM_GBM>(o, a, p, tab, rp, acc);
+        if (o.barrier) barrier_path<RNG>(o, a, p, tab, slots, acc);
+        else if (o.dynamics == LSM_HESTON) lsm_path<RNG, LSM_HESTON>(o, a, p, tab, slots, rp, acc);
+        else if (o.dynamics == LSM_LOCAL_VOL)
+            lsm_path<RNG, LSM_LOCAL_VOL>(o, a, p, tab, slots, rp, acc);
+        else lsm_path<RNG, LSM_GBM>(o, a, p, tab, slots, rp, acc);
     }
     finish_tile(a, o, tile, acc);
 }

@@ -325,6 +325,30 @@ __device__ __forceinline__ void acklam_block_wave(Draw &&draw, double (&z)[K], d
         if (slot[k] != ~0u) z[k] = slots[slot[k]];
     wave_lds_sync();   // the buffer is reused by the next block
 }
+
+// reference_steps (below) with each block's tail values inverted per wave
+// (acklam_block_wave): the same draws in the same order, the same normals.
+// n_steps must be the same on every lane of the wave; slots holds the
+// wave's 64 K doubles.
+template <int K, int PER, class RNG, class F>
+__device__ __forceinline__ void reference_steps_wave(RNG &rng, int n_steps, F &&f,
+                                                     double *slots) {
+    static_assert(K % PER == 0, "whole steps per block");
+    constexpr int SPB = K / PER;
+    for (int j = 0; j < n_steps; j += SPB) {
+        double z[K];
+        acklam_block_wave<K>(
+            [&](int i) { return j + i / PER < n_steps ? uniform_open01(rng.next_f64()) : 0.5; },
+            z, slots);
+#pragma unroll
+        for (int i = 0; i < SPB; ++i) {
+            if (j + i < n_steps) {
+                if constexpr (PER == 2) f(j + i, z[2 * i], z[2 * i + 1]);
+                else f(j + i, z[i], 0.0);
+            }
+        }
+    }
+}
 #endif  // __HIP__
 
 // n_steps path steps of PER reference normals each (Heston: z1 then z2),
