@@ -143,15 +143,36 @@ struct StepParams<MAXA, false> {
 // entries 2^(j/256) of the GBM factors; without (a kernel whose LDS holds
 // the programs' register files) each lane inverts its own tail values and
 // GBM factors take exp_fp64.
+// Reference draws: the paths of one thread are consecutive in the engine's
+// single stream (thread_stream_start), so the seed kernel positions one
+// stream per thread and the thread draws on from path to path. g: the
+// thread's index among its scenario's tiles.
+template <int RNG>
+__device__ __forceinline__ Xoshiro256pp thread_stream(const DslArgs &a, const DslScen &o,
+                                                      int64_t g) {
+    Xoshiro256pp ref;
+    ref.s0 = ref.s1 = ref.s2 = ref.s3 = 0;
+    if constexpr (RNG == OFG_RNG_REFERENCE) {
+        const uint64_t *sd = a.seeds + o.seed_offset + 4 * g;
+        ref.s0 = sd[0];
+        ref.s1 = sd[1];
+        ref.s2 = sd[2];
+        ref.s3 = sd[3];
+    }
+    return ref;
+}
+
 // NA > 0: exactly NA assets (NA = MAXA), every one stepping as GBM, so the
 // per-asset kind tests and asset-count guards are gone from the step.
 // The snapshot steps are walked with a cursor over a.snap_list (a scalar
 // load per snapshot, not per step).
+// ref: the reference stream, positioned at this path's first draw and left
+// after its last (the thread's next path follows it in the stream).
 template <int RNG, int MAXA, bool WTAIL = true, int NA = 0>
 __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o,
                                               const StepParams<MAXA> &sp, int64_t p,
                                               double *snap, int64_t stride, const double *tab,
-                                              double *slots) {
+                                              double *slots, Xoshiro256pp &ref) {
     static_assert(NA == 0 || NA == MAXA, "exact asset count");
     const int n = NA > 0 ? NA : o.n_assets;
     double s[MAXA];
@@ -167,15 +188,7 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
         next_step = sl[0];
         next_k = sl[1];
     }
-    Xoshiro256pp ref;
     Xoroshiro128p fast;
-    if constexpr (RNG == OFG_RNG_REFERENCE) {
-        const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
-        ref.s0 = sd[0];
-        ref.s1 = sd[1];
-        ref.s2 = sd[2];
-        ref.s3 = sd[3];
-    }
     if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
     const double *zp = a.normals + o.normals_offset + p;
     // fast draws: Box-Muller pairs run across steps, so an odd asset count
@@ -400,10 +413,12 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_kernel(DslArgs a) {
     double *snap = a.snaps + (int64_t)tile * a.n_snaps * o.n_assets * DSL_BLOCK + threadIdx.x;
     const int64_t base = (int64_t)(tile - o.tile_first) * DSL_BLOCK * o.ppt;
     double sum = 0.0, sq = 0.0, err = -1.0;
+    Xoshiro256pp ref =
+        thread_stream<RNG>(a, o, (int64_t)(tile - o.tile_first) * DSL_BLOCK + threadIdx.x);
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        simulate_path<RNG, MAXA, false>(a, o, sp, p, snap, DSL_BLOCK, tab, nullptr);
+        simulate_path<RNG, MAXA, false>(a, o, sp, p, snap, DSL_BLOCK, tab, nullptr, ref);
         const double x = evaluate_path(a, o, snap, DSL_BLOCK, regs + threadIdx.x, err);
         sum += x;
         sq += x * x;
@@ -452,11 +467,13 @@ __global__ __launch_bounds__(DSL_BLOCK) void dsl_sim_kernel(DslArgs a) {
     const DslScen &o = a.scen[a.tile_scen[tile]];
     const StepParams<MAXA> sp(o);   // read once, before any snapshot store
     const int64_t base = (int64_t)(tile - o.tile_first) * DSL_BLOCK * o.ppt;
+    Xoshiro256pp ref =
+        thread_stream<RNG>(a, o, (int64_t)(tile - o.tile_first) * DSL_BLOCK + threadIdx.x);
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
         simulate_path<RNG, MAXA, true, NA>(a, o, sp, p, a.snaps + o.snap_off + p, o.paths, tab,
-                                           slots);
+                                           slots, ref);
     }
 }
 
@@ -490,6 +507,7 @@ dsl_sim_markets_kernel(DslArgs a) {
         own[m] = m < ns && a.scen[m].own_corr;
     }
     const int64_t base = (int64_t)blockIdx.x * DSL_BLOCK * o.ppt;
+    Xoshiro256pp ref = thread_stream<RNG>(a, o, (int64_t)blockIdx.x * DSL_BLOCK + threadIdx.x);
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * DSL_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
@@ -514,15 +532,7 @@ dsl_sim_markets_kernel(DslArgs a) {
             next_step = sl[0];
             next_k = sl[1];
         }
-        Xoshiro256pp ref;
         Xoroshiro128p fast;
-        if constexpr (RNG == OFG_RNG_REFERENCE) {
-            const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
-            ref.s0 = sd[0];
-            ref.s1 = sd[1];
-            ref.s2 = sd[2];
-            ref.s3 = sd[3];
-        }
         if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
         const double *zp = a.normals + o.normals_offset + p;
         double zpair = 0.0;   // fast draws: pairs run across steps, as simulate_path

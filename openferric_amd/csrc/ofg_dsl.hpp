@@ -76,7 +76,8 @@ struct alignas(16) DslScen {
     int32_t ppt;            // paths per thread
     int64_t paths;
     int64_t normals_offset; // external normals [step][asset][path]
-    int64_t seed_offset;    // reference-stream start states [path][word]
+    int64_t seed_offset;    // reference-stream start states [thread][word]: a thread's
+                            // paths are consecutive in the stream
     uint64_t seed;          // fast draws: stream_seed(seed, path)
     int32_t df_off;         // discount factors of the observations, exp(-r t)
     int32_t own_corr;       // fused simulation: 1 when the Cholesky factor differs from market 0's

@@ -33,6 +33,9 @@
 using namespace ofg_host;
 using namespace ofg_corr;
 
+// the seed kernel's thread-start mode assumes 256-thread tiles
+static_assert(ofg::DSL_BLOCK == 256, "tile width of ofg::thread_stream_start");
+
 struct ofg_dsl_product {
     std::unique_ptr<ofg_dsl::Product> prod;
     uint64_t uid = 0;   // names the product's native kernels in a context
@@ -190,8 +193,13 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
     std::map<uint64_t, std::vector<ofg_jump::Poly>> poly_cache;
     int64_t seed_threads = 0;
     if (rng == OFG_RNG_REFERENCE) {
-        const ofg::LsmSeedJob j =
-            ofg_jump::make_seed_job(eng->seed, paths, steps * na, 0, 0, polys, poly_cache);
+        // one stream state per simulation thread (every market's tiles have
+        // this many paths per thread), its paths consecutive in the stream
+        const int64_t blk = ofg::DSL_BLOCK;
+        const int32_t ppt =
+            (int32_t)std::max<int64_t>(1, (paths + blk * tile_cap - 1) / (blk * tile_cap));
+        const ofg::LsmSeedJob j = ofg_jump::make_thread_seed_job(eng->seed, paths, steps * na, ppt,
+                                                                 0, 0, polys, poly_cache);
         jobs.push_back(j);
         seed_threads = ofg_jump::seed_job_threads(j);
     }
@@ -298,7 +306,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         if (rng == OFG_RNG_REFERENCE) {
             up(b.jobs, jobs.data(), jobs.size() * sizeof(jobs[0]), "H2D seed jobs");
             up(b.polys, polys.data(), polys.size() * sizeof(uint64_t), "H2D jump polynomials");
-            b.seeds.ensure((size_t)(4 * paths) * sizeof(uint64_t));
+            b.seeds.ensure((size_t)(4 * seed_threads) * sizeof(uint64_t));
             a.seeds = b.seeds.as<uint64_t>();
         }
         a.scen = b.scen.as<ofg::DslScen>();

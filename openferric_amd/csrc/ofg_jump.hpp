@@ -169,7 +169,45 @@ inline ofg::LsmSeedJob make_seed_job(uint64_t seed, int64_t paths, int64_t draws
 }
 
 inline int64_t seed_job_threads(const ofg::LsmSeedJob &j) {
+    if (j.tile_ppt > 0) {   // every thread of the consuming kernel's tiles
+        const int64_t per_tile = 256 * (int64_t)j.tile_ppt;
+        return (j.paths + per_tile - 1) / per_tile * 256;
+    }
     return (j.paths + j.chunk - 1) / j.chunk;
+}
+
+// Seed job for a kernel whose threads each simulate consecutive paths of the
+// stream (ofg::thread_stream_start): one state per thread, reached by jumps
+// of start * draws draws composed from the polynomials x^(draws 2^k).
+inline ofg::LsmSeedJob make_thread_seed_job(uint64_t seed, int64_t paths, int64_t draws,
+                                            int32_t tile_ppt, int64_t seed_offset,
+                                            int64_t thread_first, std::vector<uint64_t> &polys,
+                                            std::map<uint64_t, std::vector<Poly>> &cache) {
+    ofg::LsmSeedJob j{};
+    ofg::Xoshiro256pp st;
+    st.seed(seed);   // Xoshiro256PlusPlus::seed_from_u64 (fast_rng.rs:35-48)
+    j.base[0] = st.s0;
+    j.base[1] = st.s1;
+    j.base[2] = st.s2;
+    j.base[3] = st.s3;
+    j.paths = paths;
+    j.draws = draws;
+    j.chunk = 1;
+    j.tile_ppt = tile_ppt;
+    int32_t bits = 0;   // start < paths
+    while (bits < 63 && (int64_t(1) << bits) < paths) ++bits;
+    j.n_polys = bits;
+    j.seed_offset = seed_offset;
+    j.thread_first = thread_first;
+    const uint64_t key = (uint64_t)draws;
+    std::vector<Poly> &qs = cache[key];
+    if ((int32_t)qs.size() < bits) {
+        if (qs.empty()) qs.push_back(x_pow(key));
+        while ((int32_t)qs.size() < bits) qs.push_back(mulmod(qs.back(), qs.back()));
+    }
+    j.poly_offset = (int32_t)(polys.size() / 4);
+    for (int32_t k = 0; k < bits; ++k) polys.insert(polys.end(), qs[(size_t)k].w, qs[(size_t)k].w + 4);
+    return j;
 }
 
 }  // namespace ofg_jump

@@ -530,13 +530,27 @@ __global__ __launch_bounds__(256) void lsm_seed_kernel(const LsmSeedJob *__restr
     }
     const LsmSeedJob &j = jobs[lo];
     const int64_t g = gid - j.thread_first;
-    const int64_t first = g * j.chunk;
-    if (first >= j.paths) return;
     Xoshiro256pp st;
     st.s0 = j.base[0];
     st.s1 = j.base[1];
     st.s2 = j.base[2];
     st.s3 = j.base[3];
+    if (j.tile_ppt > 0) {
+        // one state per thread of the consuming kernel, at its first path
+        int64_t count;
+        const int64_t start = thread_stream_start(g, j.paths, j.tile_ppt, &count);
+        if (count <= 0) return;
+        for (int k = 0; k < j.n_polys; ++k)
+            if ((start >> k) & 1) xoshiro_apply_poly(polys + 4 * (j.poly_offset + k), st);
+        uint64_t *out = seeds + j.seed_offset + 4 * g;
+        out[0] = st.s0;
+        out[1] = st.s1;
+        out[2] = st.s2;
+        out[3] = st.s3;
+        return;
+    }
+    const int64_t first = g * j.chunk;
+    if (first >= j.paths) return;
     for (int k = 0; k < j.n_polys; ++k)
         if ((g >> k) & 1) xoshiro_apply_poly(polys + 4 * (j.poly_offset + k), st);
     uint64_t *out = seeds + j.seed_offset;

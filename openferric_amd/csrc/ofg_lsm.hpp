@@ -113,8 +113,27 @@ struct LsmSeedJob {
     int32_t chunk;
     int32_t n_polys;
     int32_t poly_offset;         // into polys, 4 words per polynomial
-    int32_t pad;
+    // > 0: one state per thread of a kernel whose 256-thread tiles hold
+    // tile_ppt paths per thread, each thread's paths consecutive in the
+    // stream (thread_stream_start); chunk is then 1 and the polynomials
+    // jump by draws * 2^k
+    int32_t tile_ppt;
 };
+
+// Start path in the stream of thread g of such a kernel and its path
+// count: the valid paths of the tiles in g-major order. Tile t holds paths
+// [t 256 ppt, (t + 1) 256 ppt) in storage order p = base + j 256 + lane;
+// with R of them valid (R = q 256 + r), lane l has q + (l < r) of them.
+__host__ __device__ inline int64_t thread_stream_start(int64_t g, int64_t paths, int32_t ppt,
+                                                       int64_t *count) {
+    const int64_t per_tile = 256 * (int64_t)ppt;
+    const int64_t base = (g / 256) * per_tile, l = g % 256;
+    const int64_t rem = paths - base;
+    const int64_t R = rem < 0 ? 0 : (rem < per_tile ? rem : per_tile);
+    const int64_t q = R / 256, r = R % 256;
+    if (count) *count = q + (l < r ? 1 : 0);
+    return base + l * q + (l < r ? l : r);
+}
 
 hipError_t launch_lsm_seeds(const LsmSeedJob *jobs, int32_t n_jobs, int64_t total_threads,
                             const uint64_t *polys, uint64_t *seeds, hipStream_t stream);
