@@ -32,6 +32,9 @@
 #include <vector>
 
 using namespace ofg_host;
+
+// the seed kernel's thread-start mode assumes 256-thread tiles
+static_assert(ofg::MA_BLOCK == 256, "tile width of ofg::thread_stream_start");
 using namespace ofg_corr;
 
 namespace ofg_host {
@@ -431,12 +434,20 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
                     s.d.normals_offset = my_normals;
                     if (rng == OFG_RNG_REFERENCE) {
                         if (seed_off < 0) {   // one stream per basket, shared by its bumps
-                            const ofg::LsmSeedJob j = ofg_jump::make_seed_job(
-                                b.seed, b.num_paths, draws, seed_words, seed_threads, polys, poly_cache);
+                            // one state per kernel thread (every scenario of
+                            // the basket has these tiles), its paths
+                            // consecutive in the stream
+                            const int64_t blk = ofg::MA_BLOCK;
+                            const int32_t ppt = (int32_t)std::max<int64_t>(
+                                1, (b.num_paths + blk * ofg::MA_MAX_TILES - 1) /
+                                       (blk * ofg::MA_MAX_TILES));
+                            const ofg::LsmSeedJob j = ofg_jump::make_thread_seed_job(
+                                b.seed, b.num_paths, draws, ppt, seed_words, seed_threads, polys,
+                                poly_cache);
                             jobs.push_back(j);
                             seed_threads += ofg_jump::seed_job_threads(j);
                             seed_off = seed_words;
-                            seed_words += 4 * b.num_paths;
+                            seed_words += 4 * ((b.num_paths + blk * ppt - 1) / (blk * ppt)) * blk;
                         }
                         s.d.seed_offset = seed_off;
                     }

@@ -191,10 +191,11 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
     std::vector<ofg::LsmSeedJob> jobs;
     std::vector<uint64_t> polys;
     std::map<uint64_t, std::vector<ofg_jump::Poly>> poly_cache;
-    int64_t seed_threads = 0;
+    int64_t seed_threads = 0, seed_states = 0;
     if (rng == OFG_RNG_REFERENCE) {
         // one stream state per simulation thread (every market's tiles have
-        // this many paths per thread), its paths consecutive in the stream
+        // this many paths per thread), its paths consecutive in the stream;
+        // a seed thread serves one or more simulation threads
         const int64_t blk = ofg::DSL_BLOCK;
         const int32_t ppt =
             (int32_t)std::max<int64_t>(1, (paths + blk * tile_cap - 1) / (blk * tile_cap));
@@ -202,6 +203,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
                                                                  0, 0, polys, poly_cache);
         jobs.push_back(j);
         seed_threads = ofg_jump::seed_job_threads(j);
+        seed_states = (paths + blk * ppt - 1) / (blk * ppt) * blk;
     }
     for (size_t s = 0; s < n_mk; ++s) {
         const ofg_dsl_market &m = mk[s];
@@ -306,7 +308,7 @@ int32_t dsl_price_impl(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_d
         if (rng == OFG_RNG_REFERENCE) {
             up(b.jobs, jobs.data(), jobs.size() * sizeof(jobs[0]), "H2D seed jobs");
             up(b.polys, polys.data(), polys.size() * sizeof(uint64_t), "H2D jump polynomials");
-            b.seeds.ensure((size_t)(4 * seed_threads) * sizeof(uint64_t));
+            b.seeds.ensure((size_t)(4 * seed_states) * sizeof(uint64_t));
             a.seeds = b.seeds.as<uint64_t>();
         }
         a.scen = b.scen.as<ofg::DslScen>();

@@ -169,16 +169,20 @@ inline ofg::LsmSeedJob make_seed_job(uint64_t seed, int64_t paths, int64_t draws
 }
 
 inline int64_t seed_job_threads(const ofg::LsmSeedJob &j) {
-    if (j.tile_ppt > 0) {   // every thread of the consuming kernel's tiles
+    if (j.tile_ppt > 0) {   // every `chunk` threads of the consuming kernel's tiles
         const int64_t per_tile = 256 * (int64_t)j.tile_ppt;
-        return (j.paths + per_tile - 1) / per_tile * 256;
+        const int64_t n_cons = (j.paths + per_tile - 1) / per_tile * 256;
+        return (n_cons + j.chunk - 1) / j.chunk;
     }
     return (j.paths + j.chunk - 1) / j.chunk;
 }
 
 // Seed job for a kernel whose threads each simulate consecutive paths of the
-// stream (ofg::thread_stream_start): one state per thread, reached by jumps
-// of start * draws draws composed from the polynomials x^(draws 2^k).
+// stream (ofg::thread_stream_start): one state per consumer thread. A seed
+// thread jumps to the first of `chunk` consecutive consumer threads (start
+// * draws draws, composed from the polynomials x^(draws 2^k)) and walks the
+// draws of their paths to the next ones' starts; chunk shrinks to 1 when a
+// consumer thread's draws outnumber a jump's cost.
 inline ofg::LsmSeedJob make_thread_seed_job(uint64_t seed, int64_t paths, int64_t draws,
                                             int32_t tile_ppt, int64_t seed_offset,
                                             int64_t thread_first, std::vector<uint64_t> &polys,
@@ -192,7 +196,8 @@ inline ofg::LsmSeedJob make_thread_seed_job(uint64_t seed, int64_t paths, int64_
     j.base[3] = st.s3;
     j.paths = paths;
     j.draws = draws;
-    j.chunk = 1;
+    j.chunk = (int32_t)std::max<int64_t>(
+        1, std::min<int64_t>(64, 8192 / std::max<int64_t>(1, (int64_t)tile_ppt * draws)));
     j.tile_ppt = tile_ppt;
     int32_t bits = 0;   // start < paths
     while (bits < 63 && (int64_t(1) << bits) < paths) ++bits;

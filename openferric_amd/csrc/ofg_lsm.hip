@@ -261,23 +261,20 @@ __global__ __launch_bounds__(LSM_BLOCK) void lsm_close_kernel(LsmArgs a) {
 
 // Drives one path's draws: step(z1, z2) once per path step. Reference draws
 // walk the path's slice of the engine's single stream (z1 then z2 per Heston
-// step), 8 at a time with the tail values inverted per wave (slots: the
+// step) from the thread's stream state `ref`, 8 at a time with the tail values inverted per wave (slots: the
 // wave's LDS for them; a tile is one option, so every lane has the same step
 // count); external normals read [step][path], the fast mode runs Box-Muller
 // on a per-path xoroshiro128+ stream.
 template <int RNG, bool TWO, class Step>
 __device__ __forceinline__ void drive(const LsmDev &o, const LsmArgs &a, int64_t p,
-                                      const double *tab, double *slots, Step &&step) {
+                                      const double *tab, double *slots, Xoshiro256pp &ref,
+                                      Step &&step) {
     const int n = o.m.steps;
     if constexpr (RNG == OFG_RNG_REFERENCE) {
-        const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
-        Xoshiro256pp rng;
-        rng.s0 = sd[0];
-        rng.s1 = sd[1];
-        rng.s2 = sd[2];
-        rng.s3 = sd[3];
+        // ref: this thread's stream, at the path's first draw (the thread's
+        // paths follow one another in the option's stream)
         reference_steps_wave<LSM_AK, TWO ? 2 : 1>(
-            rng, n, [&](int j, double z1, double z2) { step(z1, z2, j + 1); }, slots);
+            ref, n, [&](int j, double z1, double z2) { step(z1, z2, j + 1); }, slots);
     } else if constexpr (RNG == OFG_RNG_EXTERNAL) {
         const double *e1 = a.normals1 + o.normals_offset + p;
         const double *e2 = TWO ? a.normals2 + o.normals_offset + p : nullptr;
@@ -325,8 +322,8 @@ __device__ __forceinline__ void drive(const LsmDev &o, const LsmArgs &a, int64_t
 // terminal cash flow discounted to the latest date.
 template <int RNG, int DYN>
 __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int64_t p,
-                                         const double *tab, double *slots, const RoundPlan &rp,
-                                         LsmAcc &acc) {
+                                         const double *tab, double *slots, Xoshiro256pp &ref,
+                                         const RoundPlan &rp, LsmAcc &acc) {
     constexpr bool FAST = RNG == OFG_RNG_FAST;
     const LsmDate *dates = a.dates + o.date_offset;
     double *sp = a.spots + o.s_offset + p;
@@ -345,7 +342,7 @@ __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int6
     // pricing kernel's reference mode
     constexpr bool TABLE = RNG == OFG_RNG_REFERENCE;
     if constexpr (DYN == LSM_HESTON) {
-        drive<RNG, true>(o, a, p, tab, slots, [&](double z1, double z2, int t) {
+        drive<RNG, true>(o, a, p, tab, slots, ref, [&](double z1, double z2, int t) {
             double sn, vn, growth, e;
             heston_euler<!FAST, TABLE>(s, v, o.m, z1, z2, sn, vn, growth, e, tab);
             s = sn;   // fmax(s_next, 1e-12) (:272)
@@ -355,7 +352,7 @@ __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int6
     } else if constexpr (DYN == LSM_LOCAL_VOL) {
         // flat surface: sigma(S, t) = vol; s *= exp(drift + sigma sqrt(dt) z),
         // floored at 1e-12 (:231-240)
-        drive<RNG, false>(o, a, p, tab, slots, [&](double z, double, int t) {
+        drive<RNG, false>(o, a, p, tab, slots, ref, [&](double z, double, int t) {
             double g;
             if constexpr (FAST) {
                 g = exp2_256(o.m.drift2 + o.m.diffusion2 * z, tab);
@@ -370,7 +367,7 @@ __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int6
             record(t);
         });
     } else {
-        drive<RNG, false>(o, a, p, tab, slots, [&](double z, double, int t) {
+        drive<RNG, false>(o, a, p, tab, slots, ref, [&](double z, double, int t) {
             double x;
             s *= gbm_growth<FAST, TABLE>(o.m, z, tab, x);
             record(t);
@@ -393,13 +390,14 @@ __device__ __forceinline__ void lsm_path(const LsmDev &o, const LsmArgs &a, int6
 // path point (:704-729).
 template <int RNG>
 __device__ __forceinline__ void barrier_path(const LsmDev &o, const LsmArgs &a, int64_t p,
-                                             const double *tab, double *slots, LsmAcc &acc) {
+                                             const double *tab, double *slots,
+                                             Xoshiro256pp &ref, LsmAcc &acc) {
     constexpr bool FAST = RNG == OFG_RNG_FAST;
     const double level = o.barrier_level;
     const bool down = o.barrier_down != 0;
     double s = o.m.s0;
     bool hit = down ? s <= level : s >= level;
-    drive<RNG, false>(o, a, p, tab, slots, [&](double z, double, int) {
+    drive<RNG, false>(o, a, p, tab, slots, ref, [&](double z, double, int) {
         double x;
         s *= gbm_growth<FAST, RNG == OFG_RNG_REFERENCE>(o.m, z, tab, x);
         hit = hit || (down ? s <= level : s >= level);
@@ -433,14 +431,27 @@ __global__ __launch_bounds__(LSM_BLOCK) void lsm_paths_kernel(LsmArgs a) {
     LsmAcc acc;
     acc_init(acc, o.put != 0);
     const int64_t base = (int64_t)(tile - o.tile_first) * LSM_BLOCK * o.ppt;
+    // reference draws: the thread's paths are consecutive in the option's
+    // stream (thread_stream_start); its state comes from the seed kernel
+    Xoshiro256pp ref;
+    ref.s0 = ref.s1 = ref.s2 = ref.s3 = 0;
+    if constexpr (REF) {
+        const uint64_t *sd =
+            a.seeds + o.seed_offset + 4 * ((int64_t)(tile - o.tile_first) * LSM_BLOCK + threadIdx.x);
+        ref.s0 = sd[0];
+        ref.s1 = sd[1];
+        ref.s2 = sd[2];
+        ref.s3 = sd[3];
+    }
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * LSM_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        if (o.barrier) barrier_path<RNG>(o, a, p, tab, slots, acc);
-        else if (o.dynamics == LSM_HESTON) lsm_path<RNG, LSM_HESTON>(o, a, p, tab, slots, rp, acc);
+        if (o.barrier) barrier_path<RNG>(o, a, p, tab, slots, ref, acc);
+        else if (o.dynamics == LSM_HESTON)
+            lsm_path<RNG, LSM_HESTON>(o, a, p, tab, slots, ref, rp, acc);
         else if (o.dynamics == LSM_LOCAL_VOL)
-            lsm_path<RNG, LSM_LOCAL_VOL>(o, a, p, tab, slots, rp, acc);
-        else lsm_path<RNG, LSM_GBM>(o, a, p, tab, slots, rp, acc);
+            lsm_path<RNG, LSM_LOCAL_VOL>(o, a, p, tab, slots, ref, rp, acc);
+        else lsm_path<RNG, LSM_GBM>(o, a, p, tab, slots, ref, rp, acc);
     }
     finish_tile(a, o, tile, acc);
 }
@@ -536,17 +547,30 @@ __global__ __launch_bounds__(256) void lsm_seed_kernel(const LsmSeedJob *__restr
     st.s2 = j.base[2];
     st.s3 = j.base[3];
     if (j.tile_ppt > 0) {
-        // one state per thread of the consuming kernel, at its first path
+        // one state per thread of the consuming kernel, at its first path:
+        // a jump to the first of `chunk` consecutive consumer threads, then
+        // their paths' draws walked (their streams follow one another)
+        const int64_t per_tile = 256 * (int64_t)j.tile_ppt;
+        const int64_t n_cons = (j.paths + per_tile - 1) / per_tile * 256;
+        const int64_t g0 = g * j.chunk;
+        if (g0 >= n_cons) return;
         int64_t count;
-        const int64_t start = thread_stream_start(g, j.paths, j.tile_ppt, &count);
+        const int64_t start = thread_stream_start(g0, j.paths, j.tile_ppt, &count);
         if (count <= 0) return;
         for (int k = 0; k < j.n_polys; ++k)
             if ((start >> k) & 1) xoshiro_apply_poly(polys + 4 * (j.poly_offset + k), st);
-        uint64_t *out = seeds + j.seed_offset + 4 * g;
-        out[0] = st.s0;
-        out[1] = st.s1;
-        out[2] = st.s2;
-        out[3] = st.s3;
+        const int64_t g_end = g0 + j.chunk < n_cons ? g0 + j.chunk : n_cons;
+        for (int64_t gg = g0; gg < g_end; ++gg) {
+            (void)thread_stream_start(gg, j.paths, j.tile_ppt, &count);
+            if (count <= 0) break;   // the rest of the last tile has no paths
+            uint64_t *out = seeds + j.seed_offset + 4 * gg;
+            out[0] = st.s0;
+            out[1] = st.s1;
+            out[2] = st.s2;
+            out[3] = st.s3;
+            if (gg + 1 < g_end)
+                for (int64_t d = 0; d < count * j.draws; ++d) (void)st.next_u64();
+        }
         return;
     }
     const int64_t first = g * j.chunk;

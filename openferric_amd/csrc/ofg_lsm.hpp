@@ -115,8 +115,8 @@ struct LsmSeedJob {
     int32_t poly_offset;         // into polys, 4 words per polynomial
     // > 0: one state per thread of a kernel whose 256-thread tiles hold
     // tile_ppt paths per thread, each thread's paths consecutive in the
-    // stream (thread_stream_start); chunk is then 1 and the polynomials
-    // jump by draws * 2^k
+    // stream (thread_stream_start); chunk then counts consumer threads per
+    // seed thread and the polynomials jump by draws * 2^k
     int32_t tile_ppt;
 };
 

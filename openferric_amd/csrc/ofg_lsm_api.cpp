@@ -31,6 +31,9 @@
 using namespace ofg_host;
 using namespace ofg_jump;
 
+// the seed kernel's thread-start mode assumes 256-thread tiles
+static_assert(ofg::LSM_BLOCK == 256, "tile width of ofg::thread_stream_start");
+
 namespace ofg_host {
 
 struct LsmBuffers {
@@ -318,12 +321,14 @@ LsmPlan build_lsm_plan(const ofg_lsm_option *in, size_t n, int32_t rng_mode) {
         p.max_rounds = std::max(p.max_rounds, d.barrier ? 0 : d.n_ex);
 
         if (rng_mode == OFG_RNG_REFERENCE) {
-            const ofg::LsmSeedJob j =
-                make_seed_job(o.seed, paths, steps * (dynamics == ofg::LSM_HESTON ? 2 : 1),
-                              p.seeds, p.seed_threads, p.polys, poly_cache);
+            // one stream state per path thread, its paths consecutive in
+            // the option's stream
+            const ofg::LsmSeedJob j = make_thread_seed_job(
+                o.seed, paths, steps * (dynamics == ofg::LSM_HESTON ? 2 : 1), d.ppt, p.seeds,
+                p.seed_threads, p.polys, poly_cache);
             p.jobs.push_back(j);
             p.seed_threads += seed_job_threads(j);
-            p.seeds += 4 * paths;
+            p.seeds += 4 * (int64_t)d.n_tiles * ofg::LSM_BLOCK;
         }
         h.simulate = true;
         h.dev = opt_index;

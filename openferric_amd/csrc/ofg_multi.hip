@@ -257,23 +257,33 @@ __device__ __forceinline__ double clamp_copula_uniform(double u) {   // clamp(1e
 // :786-800; times the quanto FX rate, :429). The correlated
 // normal of asset i accumulates L[i][j] z_j column by column as each z_j is
 // drawn, the same terms in the same order as correlate_normals' running sum.
+// The seed kernel's state for thread g (reference draws only).
+template <int RNG>
+__device__ __forceinline__ Xoshiro256pp thread_seed_state(const uint64_t *seeds, int64_t offset,
+                                                          int64_t g) {
+    Xoshiro256pp st;
+    st.s0 = st.s1 = st.s2 = st.s3 = 0;
+    if constexpr (RNG == OFG_RNG_REFERENCE) {
+        const uint64_t *sd = seeds + offset + 4 * g;
+        st.s0 = sd[0];
+        st.s1 = sd[1];
+        st.s2 = sd[2];
+        st.s3 = sd[3];
+    }
+    return st;
+}
+
+// ref: the thread's reference stream at this path's first draw (a thread's
+// paths are consecutive in the basket's stream, ofg::thread_stream_start).
 template <int RNG, bool TCOP>
 __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const double *normals,
-                                              const uint64_t *seeds, const double *tab) {
+                                              Xoshiro256pp &ref, const double *tab) {
     constexpr bool FAST = RNG == OFG_RNG_FAST;
     const int n = o.n;
     double c[BK_MAX];
 #pragma unroll
     for (int i = 0; i < BK_MAX; ++i) c[i] = 0.0;
-    Xoshiro256pp ref;
     Xoroshiro128p fast;
-    if constexpr (RNG == OFG_RNG_REFERENCE) {
-        const uint64_t *sd = seeds + o.seed_offset + 4 * p;
-        ref.s0 = sd[0];
-        ref.s1 = sd[1];
-        ref.s2 = sd[2];
-        ref.s3 = sd[3];
-    }
     if constexpr (FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
     double zpair = 0.0;
     // draw k of the path (external normals: [draw][path])
@@ -409,10 +419,12 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_kernel(BkArgs a) {
     const BkDev &o = a.scen[sc];
     const int64_t base = (int64_t)(tile - o.tile_first) * MA_BLOCK * o.ppt;
     double sum = 0.0, sq = 0.0;
+    Xoshiro256pp ref = thread_seed_state<RNG>(
+        a.seeds, o.seed_offset, (int64_t)(tile - o.tile_first) * MA_BLOCK + threadIdx.x);
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * MA_BLOCK + threadIdx.x;
         if (p >= o.paths) break;
-        const double x = basket_path<RNG, TCOP>(o, p, a.normals, a.seeds, tab);
+        const double x = basket_path<RNG, TCOP>(o, p, a.normals, ref, tab);
         sum += x;
         sq += x * x;
     }
@@ -851,21 +863,17 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_greeks_kernel(BkgArgs a) {
             acc_sq += sq;
         }
     };
+    // reference draws: the thread's paths are consecutive in the basket's
+    // stream (ofg::thread_stream_start), from one state per thread
+    Xoshiro256pp ref = thread_seed_state<RNG>(
+        a.seeds, o.seed_offset, (int64_t)(tile - o.tile_first) * MA_BLOCK + threadIdx.x);
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * MA_BLOCK + threadIdx.x;
         const bool valid = p < o.paths;
         if (!__any(valid)) break;
         double z[MAXN], c[MAXN], g[MAXN], st[MAXN];
         if (valid) {
-            Xoshiro256pp ref;
             Xoroshiro128p fast;
-            if constexpr (RNG == OFG_RNG_REFERENCE) {
-                const uint64_t *sd = a.seeds + o.seed_offset + 4 * p;
-                ref.s0 = sd[0];
-                ref.s1 = sd[1];
-                ref.s2 = sd[2];
-                ref.s3 = sd[3];
-            }
             if constexpr (RNG == OFG_RNG_FAST) fast.seed(stream_seed(o.seed, (uint64_t)p));
             double zpair = 0.0;
             if constexpr (RNG == OFG_RNG_REFERENCE) {
