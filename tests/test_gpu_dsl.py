@@ -235,6 +235,24 @@ def test_many_paths_per_thread(gpu):
     assert _close(r.price, p) and abs(r.stderr - se) <= 1e-8 * se
 
 
+def test_reference_draws_several_paths_per_thread(gpu):
+    """300,001 paths on the reference's stream: two paths per simulation
+    thread, consecutive in the stream, and a ragged last tile (the seed
+    kernel's one state per thread); one market and two markets simulated
+    together, both equal to the oracle's stream order."""
+    from openferric_amd import MultiAssetMarket, RngMode, parse_and_compile
+    mkt, ora, corr, rate = _mk(TWO)
+    paths, steps = 300_001, 2
+    eng = _engine(paths, steps, RngMode.Reference, seed=21)
+    prod = parse_and_compile(P.WORST_DIGITAL)
+    r = eng.price_multi_asset(prod, mkt)
+    p, se = D.price(P.WORST_DIGITAL, ora, corr, rate, paths, steps, None, seed=21)
+    assert _close(r.price, p) and abs(r.stderr - se) <= 1e-8 * se, (r.price, p)
+    both = eng.price_markets(prod, [mkt, MultiAssetMarket(mkt.assets, mkt.correlation,
+                                                          mkt.rate + 0.01)])
+    assert _close(both[0].price, p), (both[0].price, p)
+
+
 def test_fused_form_matches_split_form(gpu, monkeypatch):
     """The single-kernel form (used when all snapshots exceed the buffer
     budget) prices as the two-kernel form."""
