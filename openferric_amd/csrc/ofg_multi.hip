@@ -83,7 +83,9 @@ __device__ __forceinline__ double autocall_path(const AcDev &o, const AcObs *obs
                     g = exp_fp64(o.drift[i] + o.vol_dt[i] * c);
                 }
                 s[i] = fmax(s[i] * g, 1.0e-12);
-                worst = fmin(worst, s[i] / o.s0[i]);
+                // the fast mode's performance multiplies by 1 / s0 (within an
+                // ulp of the reference's division, which the parity modes keep)
+                worst = fmin(worst, FAST ? s[i] * o.inv_s0[i] : s[i] / o.s0[i]);
             }
         }
         worst_final = worst;
@@ -514,12 +516,14 @@ __device__ __forceinline__ void acg_correlate(const AcGreeksDev &o, int set, con
     }
 }
 
-template <int MAXN>
+// Worst performance; the fast mode multiplies by 1 / s0, as the one-scenario
+// kernel's fast mode does.
+template <bool FAST, int MAXN>
 __device__ __forceinline__ double acg_worst(const AcGreeksDev &o, const double *sp) {
     double w = INFINITY;
 #pragma unroll
     for (int i = 0; i < MAXN; ++i)
-        if (i < o.n) w = fmin(w, sp[i] / o.s0[i]);
+        if (i < o.n) w = fmin(w, FAST ? sp[i] * o.inv_s0[i] : sp[i] / o.s0[i]);
     return w;
 }
 
@@ -653,16 +657,18 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
             };
             double rb[MAXN];
 #pragma unroll
-            for (int i = 0; i < MAXN; ++i) rb[i] = i < n ? sb[i] / o.s0[i] : INFINITY;
+            for (int i = 0; i < MAXN; ++i)
+                rb[i] = i < n ? (RNG == OFG_RNG_FAST ? sb[i] * o.inv_s0[i] : sb[i] / o.s0[i])
+                              : INFINITY;
             double w0 = INFINITY;
 #pragma unroll
             for (int i = 0; i < MAXN; ++i) w0 = fmin(w0, rb[i]);
             visit(0, w0);
-            visit(1, acg_worst<MAXN>(o, sv[0]));
-            visit(2, acg_worst<MAXN>(o, sv[1]));
+            visit(1, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sv[0]));
+            visit(2, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sv[1]));
             if (o.has_cega) {
-                visit(3, acg_worst<MAXN>(o, sc[0]));
-                visit(4, acg_worst<MAXN>(o, sc[1]));
+                visit(3, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sc[0]));
+                visit(4, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sc[1]));
             }
 #pragma unroll
             for (int q = 0; q < MAXN; ++q) {
@@ -670,8 +676,12 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
                     double wu = INFINITY, wd = INFINITY;
 #pragma unroll
                     for (int i = 0; i < MAXN; ++i) {
-                        wu = fmin(wu, i == q ? su[i] / o.s0_up[i] : rb[i]);
-                        wd = fmin(wd, i == q ? sd[i] / o.s0_dn[i] : rb[i]);
+                        wu = fmin(wu, i == q ? (RNG == OFG_RNG_FAST ? su[i] * o.inv_up[i]
+                                                                    : su[i] / o.s0_up[i])
+                                             : rb[i]);
+                        wd = fmin(wd, i == q ? (RNG == OFG_RNG_FAST ? sd[i] * o.inv_dn[i]
+                                                                    : sd[i] / o.s0_dn[i])
+                                             : rb[i]);
                     }
                     visit(5 + 2 * q, wu);
                     visit(6 + 2 * q, wd);
@@ -706,16 +716,18 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
         };
         double rb[MAXN];
 #pragma unroll
-        for (int i = 0; i < MAXN; ++i) rb[i] = i < n ? sb[i] / o.s0[i] : INFINITY;
+        for (int i = 0; i < MAXN; ++i)
+            rb[i] = i < n ? (RNG == OFG_RNG_FAST ? sb[i] * o.inv_s0[i] : sb[i] / o.s0[i])
+                          : INFINITY;
         double w0 = INFINITY;
 #pragma unroll
         for (int i = 0; i < MAXN; ++i) w0 = fmin(w0, rb[i]);
         finish(0, w0);
-        finish(1, acg_worst<MAXN>(o, sv[0]));
-        finish(2, acg_worst<MAXN>(o, sv[1]));
+        finish(1, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sv[0]));
+        finish(2, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sv[1]));
         if (o.has_cega) {
-            finish(3, acg_worst<MAXN>(o, sc[0]));
-            finish(4, acg_worst<MAXN>(o, sc[1]));
+            finish(3, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sc[0]));
+            finish(4, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sc[1]));
         }
 #pragma unroll
         for (int q = 0; q < MAXN; ++q) {
@@ -723,8 +735,12 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
                 double wu = INFINITY, wd = INFINITY;
 #pragma unroll
                 for (int i = 0; i < MAXN; ++i) {
-                    wu = fmin(wu, i == q ? su[i] / o.s0_up[i] : rb[i]);
-                    wd = fmin(wd, i == q ? sd[i] / o.s0_dn[i] : rb[i]);
+                    wu = fmin(wu, i == q ? (RNG == OFG_RNG_FAST ? su[i] * o.inv_up[i]
+                                                                : su[i] / o.s0_up[i])
+                                         : rb[i]);
+                    wd = fmin(wd, i == q ? (RNG == OFG_RNG_FAST ? sd[i] * o.inv_dn[i]
+                                                                : sd[i] / o.s0_dn[i])
+                                         : rb[i]);
                 }
                 finish(5 + 2 * q, wu);
                 finish(6 + 2 * q, wd);

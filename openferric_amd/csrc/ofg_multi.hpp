@@ -32,6 +32,7 @@ struct alignas(16) AcDev {
     int64_t normals_offset;   // external normals of the contract, [step][asset][path]
     uint64_t seed;            // fast draws: stream_seed(seed, path)
     double s0[MA_MAX], drift[MA_MAX], vol_dt[MA_MAX];
+    double inv_s0[MA_MAX];                    // fast mode: performances s * (1 / s0)
     double drift2[MA_MAX], vol_dt2[MA_MAX];   // fast mode: scaled by 256/ln2
     double chol[MA_MAX * (MA_MAX + 1) / 2];   // lower factor, row i at i(i+1)/2
     double notional, maturity, autocall_barrier, coupon_rate, coupon_barrier;
@@ -89,6 +90,8 @@ struct alignas(16) AcGreeksDev {
     int64_t paths, normals_offset;
     uint64_t seed;
     double s0[MA_MAX], s0_up[MA_MAX], s0_dn[MA_MAX];
+    // fast mode: performances s * (1 / s0), the reciprocals of the three
+    double inv_s0[MA_MAX], inv_up[MA_MAX], inv_dn[MA_MAX];
     double drift[3][MA_MAX], vol_dt[3][MA_MAX];     // base, vol up, vol down
     double drift2[3][MA_MAX], vol_dt2[3][MA_MAX];   // fast mode: scaled by 256/ln2
     double chol[3][MA_MAX * (MA_MAX + 1) / 2];     // base, correlation up, down

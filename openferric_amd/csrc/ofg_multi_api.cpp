@@ -144,6 +144,7 @@ ofg::AcDev make_scenario(const ofg_autocall &c, const std::vector<double> &spots
     for (int i = 0; i < n; ++i) {
         const double s0 = spots[(size_t)c.underlyings[i]], v = vols[(size_t)c.underlyings[i]];
         d.s0[i] = s0;
+        d.inv_s0[i] = 1.0 / s0;
         d.drift[i] = (c.rate - c.div_yield - 0.5 * v * v) * dt;
         d.vol_dt[i] = v * sqrt_dt;
         d.drift2[i] = d.drift[i] * OFG_INV_LN2_256;
@@ -343,6 +344,9 @@ int32_t autocall_impl(ofg_ctx *ctx, const ofg_autocall *in, size_t n, const ofg_
             g.s0[a] = b0.s0[a];
             g.s0_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].s0[a];
             g.s0_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].s0[a];
+            g.inv_s0[a] = b0.inv_s0[a];
+            g.inv_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].inv_s0[a];
+            g.inv_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].inv_s0[a];
             for (int v = 0; v < 3; ++v) {
                 g.drift[v][a] = sets[v]->drift[a];
                 g.vol_dt[v][a] = sets[v]->vol_dt[a];
