@@ -99,11 +99,13 @@ struct LsmArgs {
     int32_t rng_mode;
 };
 
-// Start states of the reference stream for a group of paths: thread g of
-// option job j jumps the seeded state by g * chunk * draws draws with the
-// polynomials polys[k] = x^(chunk * draws * 2^k) mod P(x) (P the
-// characteristic polynomial of the xoshiro256 transition), then writes
-// `chunk` consecutive paths' states, stepping `draws` draws between them.
+// Start states of the reference stream for a group of paths (P the
+// characteristic polynomial of the xoshiro256 transition). Per path
+// (tile_ppt = 0): thread g of job j jumps the seeded state by
+// g * chunk * draws draws with the polynomials polys[k] =
+// x^(chunk * draws * 2^k) mod P(x), then writes `chunk` consecutive paths'
+// states, stepping `draws` draws between them. Per consumer thread
+// (tile_ppt > 0): see tile_ppt below.
 struct LsmSeedJob {
     uint64_t base[4];            // xoshiro256++ state after seed_from_u64(seed)
     int64_t paths;
