@@ -703,6 +703,14 @@ int32_t ofg_dsl_price(ofg_ctx *ctx, const ofg_dsl_product *product, const ofg_ds
 int32_t ofg_dsl_jit_compile(ofg_ctx *ctx, const ofg_dsl_product *product, int32_t n_assets,
                             double *compile_ms);
 
+/* The HIP source that ofg_dsl_jit_compile compiles for this product and
+ * asset count (no device needed): *len receives its length in bytes; when
+ * buf is not NULL, up to cap - 1 bytes and a terminating NUL are copied. No
+ * counterpart in the reference (whose JIT emits machine code directly); for
+ * inspection and tests. */
+int32_t ofg_dsl_jit_source(const ofg_dsl_product *product, int32_t n_assets, char *buf, size_t cap,
+                           size_t *len);
+
 /* Native evaluation kernels attached to the context and how many times
  * ofg_dsl_price has launched one. No counterpart in the reference; for tests
  * and measurement. Either output may be NULL. */

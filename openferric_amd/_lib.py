@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = [
     "ofg_dsl_price", "ofg_price_noalloc", "ofg_finite_difference_greeks",
     "ofg_bump_and_reprice", "ofg_european_exact", "ofg_spread_price",
     "ofg_inject_alloc_failure", "ofg_fill_normals", "ofg_price_device",
-    "ofg_dsl_jit_compile", "ofg_dsl_jit_stats",
+    "ofg_dsl_jit_compile", "ofg_dsl_jit_stats", "ofg_dsl_jit_source",
 ]
 
 
@@ -341,6 +341,9 @@ def load() -> C.CDLL:
         L.ofg_dsl_jit_compile.restype = C.c_int32
         L.ofg_dsl_jit_stats.argtypes = [pv, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.ofg_dsl_jit_stats.restype = C.c_int32
+        L.ofg_dsl_jit_source.argtypes = [pv, C.c_int32, C.c_char_p, C.c_size_t,
+                                         C.POINTER(C.c_size_t)]
+        L.ofg_dsl_jit_source.restype = C.c_int32
         _lib = L
         return L
 

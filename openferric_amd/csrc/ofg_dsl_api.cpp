@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -426,15 +427,46 @@ extern "C" int32_t ofg_dsl_jit_compile(ofg_ctx *ctx, const ofg_dsl_product *prod
         if (n_assets < 1 || n_assets > ofg::DSL_MAX_ASSETS)
             fail(OFG_INVALID_INPUT, "n_assets must be in 1..16");
         const ofg_dsl::Product &prod = *product->prod;
+        const std::pair<uint64_t, int32_t> key{product->uid, n_assets};
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            if (ctx->dsl && ctx->dsl->jit.count(key)) return OFG_OK;   // already attached
+        }
+        // generate and compile without holding the context (host work only)
+        const auto t0 = std::chrono::steady_clock::now();
         const ofg_dsl_jit::ProgramImage im = ofg_dsl_jit::assemble(prod);
-        const std::string src = ofg_dsl_jit::kernel_source(im, prod.notional, n_assets);
+        const std::vector<char> code =
+            ofg_dsl_jit::compile(ofg_dsl_jit::kernel_source(im, prod.notional, n_assets));
         std::lock_guard<std::mutex> lock(ctx->mu);
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         if (!ctx->dsl) ctx->dsl = new DslBuffers();
         DslBuffers &b = *ctx->dsl;
-        const std::pair<uint64_t, int32_t> key{product->uid, n_assets};
-        if (b.jit.count(key)) return OFG_OK;   // already attached
-        b.jit[key] = ofg_dsl_jit::compile(src, compile_ms);
+        if (b.jit.count(key)) return OFG_OK;   // another thread attached it meanwhile
+        b.jit[key] = ofg_dsl_jit::load(code);
+        if (compile_ms)
+            *compile_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+        return OFG_OK;
+    });
+}
+
+extern "C" int32_t ofg_dsl_jit_source(const ofg_dsl_product *product, int32_t n_assets,
+                                      char *buf, size_t cap, size_t *len) {
+    return guarded([&]() {
+        if (len) *len = 0;
+        if (!product || !product->prod || !len) fail(OFG_INVALID_INPUT, "null argument");
+        if (n_assets < 1 || n_assets > ofg::DSL_MAX_ASSETS)
+            fail(OFG_INVALID_INPUT, "n_assets must be in 1..16");
+        const ofg_dsl::Product &prod = *product->prod;
+        const std::string src =
+            ofg_dsl_jit::kernel_source(ofg_dsl_jit::assemble(prod), prod.notional, n_assets);
+        *len = src.size();
+        if (buf && cap > 0) {
+            const size_t m = std::min(cap - 1, src.size());
+            std::memcpy(buf, src.data(), m);
+            buf[m] = '\0';
+        }
         return OFG_OK;
     });
 }

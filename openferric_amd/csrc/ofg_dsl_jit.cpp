@@ -12,7 +12,6 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -234,8 +233,7 @@ extern "C" __global__ void __launch_bounds__(256) ofg_dsl_eval_jit(DslJitArgs a)
     return s;
 }
 
-Kernel compile(const std::string &source, double *compile_ms) {
-    const auto t0 = std::chrono::steady_clock::now();
+std::vector<char> compile(const std::string &source) {
     hiprtcProgram prog = nullptr;
     if (hiprtcCreateProgram(&prog, source.c_str(), "ofg_dsl_eval_jit.hip", 0, nullptr, nullptr) !=
         HIPRTC_SUCCESS)
@@ -259,6 +257,10 @@ Kernel compile(const std::string &source, double *compile_ms) {
     }
     hiprtcDestroyProgram(&prog);
     if (code.empty()) fail(OFG_DEVICE_ERROR, "product kernel compilation gave no code");
+    return code;
+}
+
+Kernel load(const std::vector<char> &code) {
     Kernel k;
     ofg_host::hip_check(hipModuleLoadData(&k.module, code.data()), "hipModuleLoadData");
     const hipError_t e = hipModuleGetFunction(&k.fn, k.module, "ofg_dsl_eval_jit");
@@ -266,9 +268,6 @@ Kernel compile(const std::string &source, double *compile_ms) {
         (void)hipModuleUnload(k.module);
         ofg_host::hip_check(e, "hipModuleGetFunction");
     }
-    if (compile_ms)
-        *compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
-                          .count();
     return k;
 }
 

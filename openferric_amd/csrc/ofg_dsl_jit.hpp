@@ -40,8 +40,11 @@ struct Kernel {
     hipFunction_t fn = nullptr;
 };
 
-// Compiles the source for gfx950 and loads it on the current device; fails
-// with OFG_DEVICE_ERROR and hipRTC's log when compiling or loading fails.
-Kernel compile(const std::string &source, double *compile_ms);
+// Compiles the source for gfx950 (host work only, no device needed); fails
+// with OFG_DEVICE_ERROR and hipRTC's log when compiling fails.
+std::vector<char> compile(const std::string &source);
+
+// Loads a compiled code object on the current device.
+Kernel load(const std::vector<char> &code);
 
 }  // namespace ofg_dsl_jit

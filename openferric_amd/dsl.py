@@ -59,6 +59,17 @@ class CompiledProduct:
         _raise(L.ofg_dsl_jit_compile(_lib.context(device), self._h, n, C.byref(ms)))
         return ms.value
 
+    def jit_source(self, n_assets: Optional[int] = None) -> str:
+        """The HIP source of this product's native evaluation kernel
+        (ofg_dsl_jit_source)."""
+        L = _lib.load()
+        n = self.num_underlyings if n_assets is None else int(n_assets)
+        size = C.c_size_t(0)
+        _raise(L.ofg_dsl_jit_source(self._h, max(1, n), None, 0, C.byref(size)))
+        buf = C.create_string_buffer(size.value + 1)
+        _raise(L.ofg_dsl_jit_source(self._h, max(1, n), buf, size.value + 1, C.byref(size)))
+        return buf.value.decode()
+
     def max_local_slots(self) -> int:
         info = _lib.OfgDslInfo()
         _raise(_lib.load().ofg_dsl_get_info(self._h, C.byref(info)))

@@ -255,3 +255,60 @@ def test_reference_example_files_compile():
         p, q = _lib_compile(src), D.compile_source(src)
         assert [p.schedule_dates(k) for k in range(p.num_schedules)] == [s[0] for s in q.schedules]
         assert p.num_state_vars == len(q.state), f
+
+
+# ---- native evaluation kernels: the generated source compiles (host only) ----
+
+def _hiprtc():
+    import ctypes as C
+    # no on-disk code cache: the compiles of this test leave nothing behind
+    os.environ["AMD_COMGR_CACHE"] = "0"
+    path = "/opt/rocm/lib/libhiprtc.so"
+    if not os.path.exists(path):
+        pytest.skip("hipRTC is not installed")
+    L = C.CDLL(path)
+    L.hiprtcCreateProgram.argtypes = [C.POINTER(C.c_void_p), C.c_char_p, C.c_char_p, C.c_int,
+                                      C.c_void_p, C.c_void_p]
+    L.hiprtcCompileProgram.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p)]
+    L.hiprtcGetProgramLogSize.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+    L.hiprtcGetProgramLog.argtypes = [C.c_void_p, C.c_char_p]
+    L.hiprtcGetCodeSize.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+    L.hiprtcDestroyProgram.argtypes = [C.POINTER(C.c_void_p)]
+    return L
+
+
+def test_native_kernel_sources_compile_for_gfx950():
+    """The source ofg_dsl_jit_compile would compile, for corpus products of
+    1 to 4 underlyings: it compiles with hipRTC for gfx950 under the library's
+    flags (no device needed), names the kernel, writes every constant as an
+    exact hexadecimal literal and resolves the registers to named locals."""
+    import ctypes as C
+    from openferric_amd import parse_and_compile
+    L = _hiprtc()
+    opts = [b"--offload-arch=gfx950", b"-O3", b"-std=c++17", b"-ffp-contract=off",
+            b"-Xclang", b"-target-feature", b"-Xclang", b"-packed-fp32-ops"]
+    products = [(P.CALL, 1), (P.WORST_DIGITAL, 2), (P.KITCHEN_SINK, 4), (P.RANGE_ACCRUAL, 3),
+                (P.standard_autocall(3, 1.0, 0.08, 1.5), 3),
+                (P.phoenix_memory(4, 1.05, 0.75, 0.1, 2.0), 4)]
+    for src, n in products:
+        prod = parse_and_compile(src)
+        text = prod.jit_source(n)
+        assert 'extern "C" __global__' in text and "ofg_dsl_eval_jit" in text
+        assert "double r0 = 0.0;" in text and "a.code" not in text
+        h = C.c_void_p()
+        assert L.hiprtcCreateProgram(C.byref(h), text.encode(), b"k.hip", 0, None, None) == 0
+        rc = L.hiprtcCompileProgram(h, len(opts), (C.c_char_p * len(opts))(*opts))
+        size = C.c_size_t(0)
+        L.hiprtcGetProgramLogSize(h, C.byref(size))
+        log = C.create_string_buffer(size.value + 1)
+        L.hiprtcGetProgramLog(h, log)
+        code = C.c_size_t(0)
+        L.hiprtcGetCodeSize(h, C.byref(code))
+        L.hiprtcDestroyProgram(C.byref(h))
+        assert rc == 0, log.value.decode()[:2000]
+        assert code.value > 1000
+    # constants are exact: 0.08 as its hexadecimal literal
+    text = parse_and_compile(P.standard_autocall(3, 1.0, 0.08, 1.5)).jit_source(3)
+    assert float.hex(0.08) in text
+    with pytest.raises(Exception, match="n_assets must be in 1..16"):
+        parse_and_compile(P.CALL).jit_source(17)

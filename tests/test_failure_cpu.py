@@ -19,7 +19,7 @@ STATUS_ENTRY_POINTS = [
     "ofg_autocall_price", "ofg_basket_price", "ofg_repair_correlation",
     "ofg_stress_correlation", "ofg_dsl_compile", "ofg_dsl_get_info", "ofg_dsl_schedule_dates",
     "ofg_dsl_price", "ofg_stream_jump", "ofg_fill_normals", "ofg_price_device",
-    "ofg_dsl_jit_compile", "ofg_dsl_jit_stats",
+    "ofg_dsl_jit_compile", "ofg_dsl_jit_stats", "ofg_dsl_jit_source",
 ]
 
 
