@@ -12,7 +12,7 @@ each argument's own shape.
 Forward pass: when no argument requires grad (or grad mode is off) it is the
 plain pricing launch; otherwise the forward pathwise evaluation
 (`OFG_GREEKS_FORWARD`: one sweep per path, the adjoint's estimator, about
-1.06 pricing passes under GBM and 1.28 under Heston) fills the price and the
+1.05 pricing passes under GBM and 1.26 under Heston) fills the price and the
 full gradient in the same launch, and backward scales that gradient by
 grad_output. Arguments on a HIP device are read in place and the results
 written in place through `ofg_price_device` (no host copy of the data; the
