@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -59,13 +60,23 @@ ProgramImage assemble(const ofg_dsl::Product &prod) {
 
 namespace {
 
-// A double as an exact C++ literal.
+// A double as an exact C++ hexadecimal literal, formatted from its bits
+// (independent of the host's locale): 0x1.<52-bit fraction>p<exponent> for
+// normal numbers, 0x0.<fraction>p-1022 for subnormals.
 std::string lit(double v) {
     if (std::isnan(v)) return "__builtin_nan(\"\")";
     if (std::isinf(v)) return v > 0 ? "__builtin_inf()" : "(-__builtin_inf())";
-    char buf[64];
-    std::snprintf(buf, sizeof buf, "%a", v);
-    return v < 0 ? "(" + std::string(buf) + ")" : std::string(buf);
+    uint64_t bits;
+    std::memcpy(&bits, &v, sizeof bits);
+    const bool neg = (bits >> 63) != 0;
+    const int e = (int)((bits >> 52) & 0x7FF);
+    const uint64_t frac = bits & 0xFFFFFFFFFFFFFull;
+    char buf[48];
+    if (e == 0)
+        std::snprintf(buf, sizeof buf, "0x0.%013llxp-1022", (unsigned long long)frac);
+    else
+        std::snprintf(buf, sizeof buf, "0x1.%013llxp%+d", (unsigned long long)frac, e - 1023);
+    return neg ? "(-" + std::string(buf) + ")" : std::string(buf);
 }
 
 std::string R(int i) { return "r" + std::to_string(i); }

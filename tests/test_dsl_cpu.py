@@ -307,8 +307,14 @@ def test_native_kernel_sources_compile_for_gfx950():
         L.hiprtcDestroyProgram(C.byref(h))
         assert rc == 0, log.value.decode()[:2000]
         assert code.value > 1000
-    # constants are exact: 0.08 as its hexadecimal literal
+    # constants are exact: every hexadecimal literal of the source reads back
+    # as one of the product's constants
+    import re
     text = parse_and_compile(P.standard_autocall(3, 1.0, 0.08, 1.5)).jit_source(3)
     assert float.hex(0.08) in text
+    src = P.CALL.replace("max(price(0) - 100, 0)", "0.1 + 0.3 * 123.456 - 1_000_000 / 7")
+    text = parse_and_compile(src).jit_source(1)
+    lits = {float.fromhex(m) for m in re.findall(r"-?0x[01]\.[0-9a-f]{13}p[+-]\d+", text)}
+    assert {0.1, 0.3, 123.456, 1_000_000.0, 7.0} <= lits, lits
     with pytest.raises(Exception, match="n_assets must be in 1..16"):
         parse_and_compile(P.CALL).jit_source(17)
