@@ -9,6 +9,7 @@
 #include "../../include/openferric_amd.h"
 #include "ofg_host.hpp"
 
+#include <dlfcn.h>
 #include <hip/hiprtc.h>
 
 #include <algorithm>
@@ -149,11 +150,55 @@ std::string statement(const ofg::DslIns &in, const ProgramImage &im, double noti
     }
 }
 
-std::string rtc_log(hiprtcProgram p) {
+// hipRTC, opened on first use: the library itself loads without it (only
+// ofg_dsl_jit_compile needs the compiler).
+struct Rtc {
+    hiprtcResult (*create)(hiprtcProgram *, const char *, const char *, int, const char **,
+                           const char **) = nullptr;
+    hiprtcResult (*compile)(hiprtcProgram, int, const char **) = nullptr;
+    hiprtcResult (*log_size)(hiprtcProgram, size_t *) = nullptr;
+    hiprtcResult (*log)(hiprtcProgram, char *) = nullptr;
+    hiprtcResult (*code_size)(hiprtcProgram, size_t *) = nullptr;
+    hiprtcResult (*code)(hiprtcProgram, char *) = nullptr;
+    hiprtcResult (*destroy)(hiprtcProgram *) = nullptr;
+    const char *(*error)(hiprtcResult) = nullptr;
+    std::string problem;   // why it is unavailable, if it is
+};
+
+template <class F>
+bool bind(void *lib, const char *name, F &f) {
+    f = reinterpret_cast<F>(dlsym(lib, name));
+    return f != nullptr;
+}
+
+const Rtc &rtc() {
+    static const Rtc r = [] {
+        Rtc x;
+        void *lib = dlopen("libhiprtc.so", RTLD_NOW | RTLD_LOCAL);
+        if (!lib) lib = dlopen("/opt/rocm/lib/libhiprtc.so", RTLD_NOW | RTLD_LOCAL);
+        if (!lib) {
+            const char *e = dlerror();
+            x.problem = std::string("hipRTC is not available: ") + (e ? e : "libhiprtc.so");
+            return x;
+        }
+        if (!(bind(lib, "hiprtcCreateProgram", x.create) &&
+              bind(lib, "hiprtcCompileProgram", x.compile) &&
+              bind(lib, "hiprtcGetProgramLogSize", x.log_size) &&
+              bind(lib, "hiprtcGetProgramLog", x.log) &&
+              bind(lib, "hiprtcGetCodeSize", x.code_size) && bind(lib, "hiprtcGetCode", x.code) &&
+              bind(lib, "hiprtcDestroyProgram", x.destroy) &&
+              bind(lib, "hiprtcGetErrorString", x.error)))
+            x.problem = "hipRTC is missing an entry point";
+        return x;
+    }();
+    return r;
+}
+
+std::string rtc_log(const Rtc &r, hiprtcProgram p) {
     size_t n = 0;
-    if (hiprtcGetProgramLogSize(p, &n) != HIPRTC_SUCCESS || n == 0) return "";
+    if (r.log_size(p, &n) != HIPRTC_SUCCESS || n == 0) return "";
     std::string s(n, '\0');
-    if (hiprtcGetProgramLog(p, &s[0]) != HIPRTC_SUCCESS) return "";
+    if (r.log(p, &s[0]) != HIPRTC_SUCCESS) return "";
     return s;
 }
 
@@ -245,28 +290,30 @@ extern "C" __global__ void __launch_bounds__(256) ofg_dsl_eval_jit(DslJitArgs a)
 }
 
 std::vector<char> compile(const std::string &source) {
+    const Rtc &r = rtc();
+    if (!r.problem.empty()) fail(OFG_DEVICE_ERROR, r.problem);
     hiprtcProgram prog = nullptr;
-    if (hiprtcCreateProgram(&prog, source.c_str(), "ofg_dsl_eval_jit.hip", 0, nullptr, nullptr) !=
+    if (r.create(&prog, source.c_str(), "ofg_dsl_eval_jit.hip", 0, nullptr, nullptr) !=
         HIPRTC_SUCCESS)
         fail(OFG_DEVICE_ERROR, "hiprtcCreateProgram failed");
     // the library's own build flags: contraction off (the interpreter's
     // separate multiplies and adds), no packed FP32 instructions
     const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
                           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+    const hiprtcResult rc = r.compile(prog, (int)(sizeof opts / sizeof opts[0]), opts);
     if (rc != HIPRTC_SUCCESS) {
-        const std::string log = rtc_log(prog);
-        hiprtcDestroyProgram(&prog);
-        fail(OFG_DEVICE_ERROR, std::string("product kernel compilation failed: ") +
-                                   hiprtcGetErrorString(rc) + "\n" + log.substr(0, 2000));
+        const std::string log = rtc_log(r, prog);
+        r.destroy(&prog);
+        fail(OFG_DEVICE_ERROR, std::string("product kernel compilation failed: ") + r.error(rc) +
+                                   "\n" + log.substr(0, 2000));
     }
     size_t n = 0;
     std::vector<char> code;
-    if (hiprtcGetCodeSize(prog, &n) == HIPRTC_SUCCESS && n > 0) {
+    if (r.code_size(prog, &n) == HIPRTC_SUCCESS && n > 0) {
         code.resize(n);
-        if (hiprtcGetCode(prog, code.data()) != HIPRTC_SUCCESS) code.clear();
+        if (r.code(prog, code.data()) != HIPRTC_SUCCESS) code.clear();
     }
-    hiprtcDestroyProgram(&prog);
+    r.destroy(&prog);
     if (code.empty()) fail(OFG_DEVICE_ERROR, "product kernel compilation gave no code");
     return code;
 }
