@@ -302,6 +302,7 @@ Scenario prepare(const ofg_basket &b, const std::vector<double> &spots,
     for (int i = 0; i < n; ++i) {
         const double v = vols[(size_t)i];
         d.s0[i] = spots[(size_t)i];
+        d.inv_s0[i] = 1.0 / spots[(size_t)i];
         if (b.product == 2) {   // quanto drift (basket.rs:408-417)
             const double quanto_adj = b.asset_fx_corr[i] * v * b.fx_vol;
             d.drift[i] = (b.foreign_rate - b.dividends[i] - quanto_adj - 0.5 * v * v) * t;
@@ -554,6 +555,9 @@ int32_t basket_impl(ofg_ctx *ctx, const ofg_basket *in, size_t n, const ofg_engi
             g.w[a] = b0.w[a];
             g.s0_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].d.s0[a];
             g.s0_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].d.s0[a];
+            g.inv_s0[a] = b0.inv_s0[a];
+            g.inv_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].d.inv_s0[a];
+            g.inv_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].d.inv_s0[a];
             for (int v = 0; v < 3; ++v) {
                 g.drift[v][a] = sets[v]->drift[a];
                 g.vol_t[v][a] = sets[v]->vol_t[a];

@@ -396,9 +396,14 @@ __device__ __forceinline__ double basket_path(const BkDev &o, int64_t p, const d
             }
             const double st = o.s0[i] * g;
             if (i == o.leader) leader = st;
-            if (o.type == BK_AVERAGE || outperf) u += o.w[i] * st;
-            else if (o.type == BK_BEST_OF) u = fmax(u, st / o.s0[i]);
-            else u = fmin(u, st / o.s0[i]);
+            // the fast mode's performance multiplies by 1 / s0 (within an ulp
+            // of the reference's division, which the parity modes keep)
+            if (o.type == BK_AVERAGE || outperf)
+                u += o.w[i] * st;
+            else if (o.type == BK_BEST_OF)
+                u = fmax(u, FAST ? st * o.inv_s0[i] : st / o.s0[i]);
+            else
+                u = fmin(u, FAST ? st * o.inv_s0[i] : st / o.s0[i]);
         }
     }
     if (outperf) u = leader / fmax(u, 1.0e-12);   // outperformance_payoff (basket.rs:786-800)
@@ -808,20 +813,25 @@ void launch_acg(const AcgArgs &a, hipStream_t stream) {
 // ------------------------------------------------------------------------
 
 // terminal_payoff with asset k's terminal spot and initial spot replaced
-// (k = -1: none), the reference's fold order.
-template <int MAXN>
+// (k = -1: none; invk = 1 / s0k), the reference's fold order. The fast mode's
+// performances multiply by 1 / s0 as the one-scenario kernel's fast mode
+// does.
+template <bool FAST, int MAXN>
 __device__ __forceinline__ double bkg_payoff(const BkGreeksDev &o, const double *st, int k,
-                                             double stk, double s0k) {
+                                             double stk, double s0k, double invk) {
     const int n = o.n;
     double u = o.type == BK_AVERAGE ? -0.0 : (o.type == BK_BEST_OF ? -INFINITY : INFINITY);
 #pragma unroll
     for (int i = 0; i < MAXN; ++i) {
         if (i < n) {
             const double sti = i == k ? stk : st[i];
-            const double s0i = i == k ? s0k : o.s0[i];
-            if (o.type == BK_AVERAGE) u += o.w[i] * sti;
-            else if (o.type == BK_BEST_OF) u = fmax(u, sti / s0i);
-            else u = fmin(u, sti / s0i);
+            if (o.type == BK_AVERAGE) {
+                u += o.w[i] * sti;
+            } else {
+                const double perf =
+                    FAST ? sti * (i == k ? invk : o.inv_s0[i]) : sti / (i == k ? s0k : o.s0[i]);
+                u = o.type == BK_BEST_OF ? fmax(u, perf) : fmin(u, perf);
+            }
         }
     }
     return o.is_call ? fmax(u - o.strike, 0.0) : fmax(o.strike - u, 0.0);
@@ -926,20 +936,22 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_greeks_kernel(BkgArgs a) {
         bkg_factors<RNG, MAXN>(o, 0, c, g, tab);
 #pragma unroll
         for (int i = 0; i < MAXN; ++i) st[i] = i < n ? o.s0[i] * g[i] : 0.0;
-        add(0, keep * bkg_payoff<MAXN>(o, st, -1, 0.0, 0.0));
+        add(0, keep * bkg_payoff<RNG == OFG_RNG_FAST, MAXN>(o, st, -1, 0.0, 0.0, 0.0));
         for (int k = 0; k < n; ++k) {
             double gk = g[0];
 #pragma unroll
             for (int i = 1; i < MAXN; ++i)
                 if (i == k) gk = g[i];
-            add(1 + 2 * k, keep * bkg_payoff<MAXN>(o, st, k, o.s0_up[k] * gk, o.s0_up[k]));
-            add(2 + 2 * k, keep * bkg_payoff<MAXN>(o, st, k, o.s0_dn[k] * gk, o.s0_dn[k]));
+            add(1 + 2 * k, keep * bkg_payoff<RNG == OFG_RNG_FAST, MAXN>(o, st, k, o.s0_up[k] * gk, o.s0_up[k],
+                                                                o.inv_up[k]));
+            add(2 + 2 * k, keep * bkg_payoff<RNG == OFG_RNG_FAST, MAXN>(o, st, k, o.s0_dn[k] * gk, o.s0_dn[k],
+                                                                o.inv_dn[k]));
         }
         for (int v = 1; v <= 2; ++v) {   // vol up, vol down: the base correlated normals
             bkg_factors<RNG, MAXN>(o, v, c, g, tab);
 #pragma unroll
             for (int i = 0; i < MAXN; ++i) st[i] = i < n ? o.s0[i] * g[i] : 0.0;
-            add(2 * n + v, keep * bkg_payoff<MAXN>(o, st, -1, 0.0, 0.0));
+            add(2 * n + v, keep * bkg_payoff<RNG == OFG_RNG_FAST, MAXN>(o, st, -1, 0.0, 0.0, 0.0));
         }
         if (o.has_cega) {
             for (int v = 1; v <= 2; ++v) {   // correlation up, down: the same draws
@@ -947,7 +959,7 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_greeks_kernel(BkgArgs a) {
                 bkg_factors<RNG, MAXN>(o, 0, c, g, tab);
 #pragma unroll
                 for (int i = 0; i < MAXN; ++i) st[i] = i < n ? o.s0[i] * g[i] : 0.0;
-                add(2 * n + 2 + v, keep * bkg_payoff<MAXN>(o, st, -1, 0.0, 0.0));
+                add(2 * n + 2 + v, keep * bkg_payoff<RNG == OFG_RNG_FAST, MAXN>(o, st, -1, 0.0, 0.0, 0.0));
             }
         }
     }

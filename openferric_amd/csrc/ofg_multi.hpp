@@ -137,6 +137,7 @@ struct alignas(16) BkDev {
     uint64_t seed;            // fast draws: stream_seed(seed, path)
     double strike, discount;
     double s0[BK_MAX], drift[BK_MAX], vol_t[BK_MAX], w[BK_MAX];   // w: weights or lagger weights
+    double inv_s0[BK_MAX];                     // fast mode: performances s * (1 / s0)
     double drift2[BK_MAX], vol_t2[BK_MAX];     // fast mode: scaled by 256/ln2
     double chol[BK_MAX * (BK_MAX + 1) / 2];   // lower factor, row i at i(i+1)/2
     // factor model (n_factors > 0): normals from loadings instead of chol
@@ -188,6 +189,8 @@ struct alignas(16) BkGreeksDev {
     uint64_t seed;
     double strike, discount;
     double s0[BK_MAX], s0_up[BK_MAX], s0_dn[BK_MAX], w[BK_MAX];
+    // fast mode: performances s * (1 / s0), the reciprocals of the three
+    double inv_s0[BK_MAX], inv_up[BK_MAX], inv_dn[BK_MAX];
     double drift[3][BK_MAX], vol_t[3][BK_MAX];     // base, vol up, vol down
     double drift2[3][BK_MAX], vol_t2[3][BK_MAX];   // fast mode: scaled by 256/ln2
     double chol[3][BK_MAX * (BK_MAX + 1) / 2];    // base, correlation up, down
