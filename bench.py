@@ -718,6 +718,27 @@ def reference_draws_leg(opts, reps=2):
             "frac_fp64_peak": flops / (k * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
 
 
+def host_call_leg(structs, cfg, ps, batch_prices, reps=2):
+    """The same batch through one ofg_price call on host arrays, as a caller
+    holding the descriptors in host memory makes it: the call uploads the
+    descriptors and its work list, allocates its device buffers, launches the
+    persistent kernel and the reduction and copies the results back. Wall
+    time per call after one untimed call (PCIe and allocation included)."""
+    from openferric_amd.engines import price_raw
+    price_raw(list(structs), cfg)
+    walls, res = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = price_raw(list(structs), cfg)
+        walls.append(time.perf_counter() - t0)
+    w = float(np.mean(walls))
+    prices = np.array([r.price for r in res])
+    return {"workload": "the headline batch (4096 options) through one ofg_price call on host "
+                        "buffers per repetition",
+            "wall_ms": 1000.0 * w, "path_steps_per_s": ps / w,
+            "max_abs_diff_vs_batch": float(np.max(np.abs(prices - batch_prices)))}
+
+
 CONFIG1 = dict(payoff="vanilla", option_type="call", spot=100.0, strike=100.0, expiry=1.0,
                rate=0.05, q=0.0, vol=0.2, num_steps=252, num_paths=100_000, obs_times=None)
 
@@ -884,6 +905,8 @@ def main():
                     help="skip the single European (100000 antithetic paths x 252 steps) leg")
     ap.add_argument("--no-reference-draws", action="store_true",
                     help="skip the batch on the reference's draws")
+    ap.add_argument("--no-host-call", action="store_true",
+                    help="skip the whole-batch ofg_price call from host buffers")
     ap.add_argument("--no-lsm", action="store_true", help="skip the Longstaff-Schwartz leg")
     ap.add_argument("--lsm-only", action="store_true",
                     help="run only the Longstaff-Schwartz leg (profiling)")
@@ -1008,6 +1031,9 @@ def main():
                      "kernel_ms_avg": k_ms,
                      "flop_model": "65 FLOP per GBM path-step, 66 with Asian running sum"},
     }
+    if not args.no_host_call:
+        print("[bench] host-buffer call", file=sys.stderr, flush=True)
+        out["host_call"] = host_call_leg(structs, cfg, ps, prices)
     if not args.no_reference_draws:
         print("[bench] batch on the reference's draws", file=sys.stderr, flush=True)
         out["reference_draws"] = reference_draws_leg(opts)
