@@ -34,21 +34,42 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Fast mode: one step of an underlying's log performance Y = 256 log2(S / s0)
+// (the spot's step S = max(S g, 1e-12) with g = 2^(y/256), taken in logs).
+__device__ __forceinline__ double ac_log_step(double y, double drift2, double vol_dt2, double c,
+                                              double floor) {
+    return fmax(y + fma(vol_dt2, c, drift2), floor);
+}
+
+// Fast mode: the worst performance 2^(y/256) of the worst log performance.
+__device__ __forceinline__ double ac_perf(double y, const double *tab) {
+    return exp2_256(fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX), tab);
+}
+
+// One path of simulate_autocallable_paths. The parity modes step the spots
+// as the reference does (S *= exp(drift + vol_dt c), performances S / s0);
+// the fast mode steps the log performances (ac_log_step) and compares them
+// with the barriers' logs, so the exponential is taken once per path, for the
+// knock-in redemption, instead of once per underlying and step.
 template <int RNG>
 __device__ __forceinline__ double autocall_path(const AcDev &o, const AcObs *obs, int64_t p,
                                                 const double *normals, const double *tab) {
     constexpr bool FAST = RNG == OFG_RNG_FAST;
     const int n = o.n;
+    // spots, or log performances in the fast mode
     double s[MA_MAX];
 #pragma unroll
-    for (int i = 0; i < MA_MAX; ++i) s[i] = o.s0[i];
+    for (int i = 0; i < MA_MAX; ++i) s[i] = FAST ? 0.0 : o.s0[i];
+    const double ki_b = FAST ? o.ki_y : o.ki_barrier;
+    const double coupon_b = FAST ? o.coupon_y : o.coupon_barrier;
+    const double call_b = FAST ? o.autocall_y : o.autocall_barrier;
     Xoroshiro128p rng;
     if constexpr (FAST) rng.seed(stream_seed(o.seed, (uint64_t)p));
     const double *zp = normals + o.normals_offset + p;
     int k = 0;
     int next_step = o.n_obs > 0 ? obs[0].step : 0x7fffffff;
     bool called = false, ki = false;
-    double worst_final = 1.0, pv = 0.0, pending = 0.0, prev_t = 0.0;
+    double worst_final = FAST ? 0.0 : 1.0, pv = 0.0, pending = 0.0, prev_t = 0.0;
     for (int step = 1; step <= o.steps; ++step) {
         double z[MA_MAX];
         if constexpr (FAST) {
@@ -74,27 +95,23 @@ __device__ __forceinline__ double autocall_path(const AcDev &o, const AcObs *obs
                 double c = 0.0;
 #pragma unroll
                 for (int j = 0; j <= i; ++j) c += li[j] * z[j];
-                double g;
                 if constexpr (FAST) {
-                    double y = o.drift2[i] + o.vol_dt2[i] * c;
-                    y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
-                    g = exp2_256(y, tab);
+                    s[i] = ac_log_step(s[i], o.drift2[i], o.vol_dt2[i], c, o.yfloor[i]);
+                    worst = fmin(worst, s[i]);
                 } else {
-                    g = exp_fp64(o.drift[i] + o.vol_dt[i] * c);
+                    const double g = exp_fp64(o.drift[i] + o.vol_dt[i] * c);
+                    s[i] = fmax(s[i] * g, 1.0e-12);
+                    worst = fmin(worst, s[i] / o.s0[i]);
                 }
-                s[i] = fmax(s[i] * g, 1.0e-12);
-                // the fast mode's performance multiplies by 1 / s0 (within an
-                // ulp of the reference's division, which the parity modes keep)
-                worst = fmin(worst, FAST ? s[i] * o.inv_s0[i] : s[i] / o.s0[i]);
             }
         }
         worst_final = worst;
-        if (worst <= o.ki_barrier) ki = true;
+        if (worst <= ki_b) ki = true;
         if (step == next_step) {
             const AcObs ob = obs[k];
             if (o.phoenix) {
                 const double accrual = o.notional * o.coupon_rate * fmax(ob.time - prev_t, 0.0);
-                if (worst >= o.coupon_barrier) {
+                if (worst >= coupon_b) {
                     const double due = o.memory ? pending + accrual : accrual;
                     pv += ob.df * due;
                     pending = 0.0;
@@ -103,7 +120,7 @@ __device__ __forceinline__ double autocall_path(const AcDev &o, const AcObs *obs
                 }
                 prev_t = ob.time;
             }
-            if (worst >= o.autocall_barrier) {
+            if (worst >= call_b) {
                 if (o.phoenix) pv += ob.df * o.notional;
                 else pv = ob.call_pv;
                 called = true;
@@ -115,6 +132,7 @@ __device__ __forceinline__ double autocall_path(const AcDev &o, const AcObs *obs
     }
     if (!called) {
         double redemption;
+        if constexpr (FAST) worst_final = ac_perf(worst_final, tab);
         if (ki) redemption = o.notional * (worst_final / o.ki_strike);
         else if (!o.phoenix) redemption = o.redeem_plain;
         else redemption = o.notional;
@@ -490,20 +508,17 @@ __global__ __launch_bounds__(MA_BLOCK) void basket_close_kernel(BkArgs a) {
 // ------------------------------------------------------------------------
 
 template <int RNG, int MAXN>
-__device__ __forceinline__ void acg_move(const AcGreeksDev &o, int set, const double *c, double *sp,
-                                         const double *tab) {
+__device__ __forceinline__ void acg_move(const AcGreeksDev &o, int set, const double *c,
+                                         double *sp) {
 #pragma unroll
     for (int i = 0; i < MAXN; ++i) {
         if (i < o.n) {
-            double g;
             if constexpr (RNG == OFG_RNG_FAST) {
-                double y = o.drift2[set][i] + o.vol_dt2[set][i] * c[i];
-                y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
-                g = exp2_256(y, tab);
+                sp[i] = ac_log_step(sp[i], o.drift2[set][i], o.vol_dt2[set][i], c[i], o.yfloor[i]);
             } else {
-                g = exp_fp64(o.drift[set][i] + o.vol_dt[set][i] * c[i]);
+                const double g = exp_fp64(o.drift[set][i] + o.vol_dt[set][i] * c[i]);
+                sp[i] = fmax(sp[i] * g, 1.0e-12);
             }
-            sp[i] = fmax(sp[i] * g, 1.0e-12);
         }
     }
 }
@@ -521,14 +536,14 @@ __device__ __forceinline__ void acg_correlate(const AcGreeksDev &o, int set, con
     }
 }
 
-// Worst performance; the fast mode multiplies by 1 / s0, as the one-scenario
-// kernel's fast mode does.
+// Worst performance; in the fast mode the worst log performance, as the
+// one-scenario kernel's fast mode forms it.
 template <bool FAST, int MAXN>
 __device__ __forceinline__ double acg_worst(const AcGreeksDev &o, const double *sp) {
     double w = INFINITY;
 #pragma unroll
     for (int i = 0; i < MAXN; ++i)
-        if (i < o.n) w = fmin(w, FAST ? sp[i] * o.inv_s0[i] : sp[i] / o.s0[i]);
+        if (i < o.n) w = fmin(w, FAST ? sp[i] : sp[i] / o.s0[i]);
     return w;
 }
 
@@ -545,6 +560,7 @@ template <int RNG, int MAXN>
 __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
     acg_waves_target<MAXN>()))) void autocall_greeks_kernel(AcgArgs a) {
     constexpr int MAXS = 5 + 2 * MAXN;
+    constexpr bool FAST = RNG == OFG_RNG_FAST;
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     __shared__ double red[NWAVES][ACG_MAXS][2];
     if constexpr (RNG == OFG_RNG_FAST) {
@@ -561,6 +577,9 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
     // only with correlation bumps, 5 + 2q / 6 + 2q for q < n
     uint32_t present = 0x7u | (o.has_cega ? 0x18u : 0u);
     for (int q = 0; q < n; ++q) present |= 3u << (5 + 2 * q);
+    const double ki_b = FAST ? o.ki_y : o.ki_barrier;
+    const double coupon_b = FAST ? o.coupon_y : o.coupon_barrier;
+    const double call_b = FAST ? o.autocall_y : o.autocall_barrier;
     double acc = 0.0, acc_sq = 0.0;
     for (int j = 0; j < o.ppt; ++j) {
         const int64_t p = base + (int64_t)j * MA_BLOCK + threadIdx.x;
@@ -573,12 +592,13 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
             pend[s] = 0.0;
         }
         uint32_t called = valid ? ~present : 0xffffffffu, ki = 0u;
+        // spots, or log performances in the fast mode (ac_log_step)
         double sb[MAXN], sv[2][MAXN], sc[2][MAXN], su[MAXN], sd[MAXN];
 #pragma unroll
         for (int i = 0; i < MAXN; ++i) {
-            sb[i] = sv[0][i] = sv[1][i] = sc[0][i] = sc[1][i] = o.s0[i];
-            su[i] = o.s0_up[i];
-            sd[i] = o.s0_dn[i];
+            sb[i] = sv[0][i] = sv[1][i] = sc[0][i] = sc[1][i] = FAST ? 0.0 : o.s0[i];
+            su[i] = FAST ? 0.0 : o.s0_up[i];
+            sd[i] = FAST ? 0.0 : o.s0_dn[i];
         }
         Xoroshiro128p rng;
         if constexpr (RNG == OFG_RNG_FAST) rng.seed(stream_seed(o.seed, (uint64_t)p));
@@ -586,6 +606,40 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
         int k = 0;
         int next_step = o.n_obs > 0 ? obs[0].step : 0x7fffffff;
         double prev_t = 0.0;
+        // every scenario's worst performance (worst log performance in the
+        // fast mode), handed to f(s, w) in scenario order
+        auto each = [&](auto &&f) {
+            double rb[MAXN];
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i)
+                rb[i] = i < n ? (FAST ? sb[i] : sb[i] / o.s0[i]) : INFINITY;
+            double w0 = INFINITY;
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i) w0 = fmin(w0, rb[i]);
+            f(0, w0);
+            f(1, acg_worst<FAST, MAXN>(o, sv[0]));
+            f(2, acg_worst<FAST, MAXN>(o, sv[1]));
+            if (o.has_cega) {
+                f(3, acg_worst<FAST, MAXN>(o, sc[0]));
+                f(4, acg_worst<FAST, MAXN>(o, sc[1]));
+            }
+#pragma unroll
+            for (int q = 0; q < MAXN; ++q) {
+                if (q < n) {
+                    double wu = INFINITY, wd = INFINITY;
+#pragma unroll
+                    for (int i = 0; i < MAXN; ++i) {
+                        wu = fmin(wu, i == q ? (FAST ? su[i] : su[i] / o.s0_up[i]) : rb[i]);
+                        wd = fmin(wd, i == q ? (FAST ? sd[i] : sd[i] / o.s0_dn[i]) : rb[i]);
+                    }
+                    f(5 + 2 * q, wu);
+                    f(6 + 2 * q, wd);
+                }
+            }
+        };
+        // a lane leaves once every scenario of its path is called; the lanes
+        // still stepping share the step, hence the observation index and the
+        // observation branch
         for (int step = 1; step <= o.steps && called != 0xffffffffu; ++step) {
             double z[MAXN];
             if constexpr (RNG == OFG_RNG_FAST) {
@@ -600,10 +654,11 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
                     }
                 }
             } else {
+                // only the contract's paths read normals
                 const int64_t row = (int64_t)(step - 1) * n;
 #pragma unroll
                 for (int i = 0; i < MAXN; ++i)
-                    z[i] = i < n ? zp[(row + i) * o.paths] : 0.0;
+                    z[i] = i < n && valid ? zp[(row + i) * o.paths] : 0.0;
             }
             double c[MAXN];
             acg_correlate<MAXN>(o, 0, z, c);
@@ -611,27 +666,30 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
             for (int i = 0; i < MAXN; ++i) {
                 if (i < n) {
-                    double f;
-                    if constexpr (RNG == OFG_RNG_FAST) {
-                        double y = o.drift2[0][i] + o.vol_dt2[0][i] * c[i];
-                        y = fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX);
-                        f = exp2_256(y, tab);
+                    if constexpr (FAST) {
+                        const double d2 = o.drift2[0][i], v2 = o.vol_dt2[0][i];
+                        sb[i] = ac_log_step(sb[i], d2, v2, c[i], o.yfloor[i]);
+                        su[i] = ac_log_step(su[i], d2, v2, c[i], o.yfloor_up[i]);
+                        sd[i] = ac_log_step(sd[i], d2, v2, c[i], o.yfloor_dn[i]);
                     } else {
-                        f = exp_fp64(o.drift[0][i] + o.vol_dt[0][i] * c[i]);
+                        const double f = exp_fp64(o.drift[0][i] + o.vol_dt[0][i] * c[i]);
+                        sb[i] = fmax(sb[i] * f, 1.0e-12);
+                        su[i] = fmax(su[i] * f, 1.0e-12);
+                        sd[i] = fmax(sd[i] * f, 1.0e-12);
                     }
-                    sb[i] = fmax(sb[i] * f, 1.0e-12);
-                    su[i] = fmax(su[i] * f, 1.0e-12);
-                    sd[i] = fmax(sd[i] * f, 1.0e-12);
                 }
             }
-            acg_move<RNG, MAXN>(o, 1, c, sv[0], tab);
-            acg_move<RNG, MAXN>(o, 2, c, sv[1], tab);
+            acg_move<RNG, MAXN>(o, 1, c, sv[0]);
+            acg_move<RNG, MAXN>(o, 2, c, sv[1]);
             if (o.has_cega) {
                 acg_correlate<MAXN>(o, 1, z, c);
-                acg_move<RNG, MAXN>(o, 0, c, sc[0], tab);
+                acg_move<RNG, MAXN>(o, 0, c, sc[0]);
                 acg_correlate<MAXN>(o, 2, z, c);
-                acg_move<RNG, MAXN>(o, 0, c, sc[1], tab);
+                acg_move<RNG, MAXN>(o, 0, c, sc[1]);
             }
+            // knock-in of every scenario (a called scenario's bit is never
+            // read) and, on an observation step (the same step on every
+            // lane), the coupons and calls of the scenarios not called yet
             const bool obs_now = step == next_step;
             AcObs ob{};
             double accrual = 0.0;
@@ -639,59 +697,25 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
                 ob = obs[k];
                 accrual = o.notional * o.coupon_rate * fmax(ob.time - prev_t, 0.0);
             }
-            // knock-in and observation of scenario s with worst performance w
-            auto visit = [&](int s, double w) {
-                const uint32_t bit = 1u << s;
-                if (called & bit) return;
-                if (w <= o.ki_barrier) ki |= bit;
-                if (!obs_now) return;
+            each([&](int sc_, double w) {
+                const uint32_t bit = 1u << sc_;
+                if (w <= ki_b) ki |= bit;
+                if (!obs_now || (called & bit)) return;
                 if (o.phoenix) {
-                    if (w >= o.coupon_barrier) {
-                        const double due = o.memory ? pend[s] + accrual : accrual;
-                        pv[s] += ob.df * due;
-                        pend[s] = 0.0;
+                    if (w >= coupon_b) {
+                        const double due = o.memory ? pend[sc_] + accrual : accrual;
+                        pv[sc_] += ob.df * due;
+                        pend[sc_] = 0.0;
                     } else if (o.memory) {
-                        pend[s] += accrual;
+                        pend[sc_] += accrual;
                     }
                 }
-                if (w >= o.autocall_barrier) {
-                    if (o.phoenix) pv[s] += ob.df * o.notional;
-                    else pv[s] = ob.call_pv;
+                if (w >= call_b) {
+                    if (o.phoenix) pv[sc_] += ob.df * o.notional;
+                    else pv[sc_] = ob.call_pv;
                     called |= bit;
                 }
-            };
-            double rb[MAXN];
-#pragma unroll
-            for (int i = 0; i < MAXN; ++i)
-                rb[i] = i < n ? (RNG == OFG_RNG_FAST ? sb[i] * o.inv_s0[i] : sb[i] / o.s0[i])
-                              : INFINITY;
-            double w0 = INFINITY;
-#pragma unroll
-            for (int i = 0; i < MAXN; ++i) w0 = fmin(w0, rb[i]);
-            visit(0, w0);
-            visit(1, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sv[0]));
-            visit(2, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sv[1]));
-            if (o.has_cega) {
-                visit(3, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sc[0]));
-                visit(4, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sc[1]));
-            }
-#pragma unroll
-            for (int q = 0; q < MAXN; ++q) {
-                if (q < n) {
-                    double wu = INFINITY, wd = INFINITY;
-#pragma unroll
-                    for (int i = 0; i < MAXN; ++i) {
-                        wu = fmin(wu, i == q ? (RNG == OFG_RNG_FAST ? su[i] * o.inv_up[i]
-                                                                    : su[i] / o.s0_up[i])
-                                             : rb[i]);
-                        wd = fmin(wd, i == q ? (RNG == OFG_RNG_FAST ? sd[i] * o.inv_dn[i]
-                                                                    : sd[i] / o.s0_dn[i])
-                                             : rb[i]);
-                    }
-                    visit(5 + 2 * q, wu);
-                    visit(6 + 2 * q, wd);
-                }
-            }
+            });
             if (obs_now) {
                 if (o.phoenix) prev_t = ob.time;
                 ++k;
@@ -706,6 +730,7 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
                 double v = pv[s];
                 if (!((called >> s) & 1u)) {
                     double redemption;
+                    if constexpr (FAST) wf = ac_perf(wf, tab);
                     if ((ki >> s) & 1u) redemption = o.notional * (wf / o.ki_strike);
                     else if (!o.phoenix) redemption = o.redeem_plain;
                     else redemption = o.notional;
@@ -719,38 +744,7 @@ __global__ __launch_bounds__(MA_BLOCK) __attribute__((amdgpu_waves_per_eu(
                 acc_sq += sq;
             }
         };
-        double rb[MAXN];
-#pragma unroll
-        for (int i = 0; i < MAXN; ++i)
-            rb[i] = i < n ? (RNG == OFG_RNG_FAST ? sb[i] * o.inv_s0[i] : sb[i] / o.s0[i])
-                          : INFINITY;
-        double w0 = INFINITY;
-#pragma unroll
-        for (int i = 0; i < MAXN; ++i) w0 = fmin(w0, rb[i]);
-        finish(0, w0);
-        finish(1, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sv[0]));
-        finish(2, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sv[1]));
-        if (o.has_cega) {
-            finish(3, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sc[0]));
-            finish(4, acg_worst<RNG == OFG_RNG_FAST, MAXN>(o, sc[1]));
-        }
-#pragma unroll
-        for (int q = 0; q < MAXN; ++q) {
-            if (q < n) {
-                double wu = INFINITY, wd = INFINITY;
-#pragma unroll
-                for (int i = 0; i < MAXN; ++i) {
-                    wu = fmin(wu, i == q ? (RNG == OFG_RNG_FAST ? su[i] * o.inv_up[i]
-                                                                : su[i] / o.s0_up[i])
-                                         : rb[i]);
-                    wd = fmin(wd, i == q ? (RNG == OFG_RNG_FAST ? sd[i] * o.inv_dn[i]
-                                                                : sd[i] / o.s0_dn[i])
-                                         : rb[i]);
-                }
-                finish(5 + 2 * q, wu);
-                finish(6 + 2 * q, wd);
-            }
-        }
+        each(finish);
     }
     if (lane < ACG_MAXS) {
         red[wave][lane][0] = acc;

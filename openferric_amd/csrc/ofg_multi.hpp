@@ -32,11 +32,16 @@ struct alignas(16) AcDev {
     int64_t normals_offset;   // external normals of the contract, [step][asset][path]
     uint64_t seed;            // fast draws: stream_seed(seed, path)
     double s0[MA_MAX], drift[MA_MAX], vol_dt[MA_MAX];
-    double inv_s0[MA_MAX];                    // fast mode: performances s * (1 / s0)
-    double drift2[MA_MAX], vol_dt2[MA_MAX];   // fast mode: scaled by 256/ln2
+    // fast mode: each underlying's log performance is carried as
+    // Y = 256 log2(S / s0), advanced by drift2 + vol_dt2 c (the increments
+    // scaled by 256/ln2) and held at yfloor = 256 log2(1e-12 / s0), the
+    // spot floor of the step; the barriers are compared as 256 log2(b)
+    double yfloor[MA_MAX];
+    double drift2[MA_MAX], vol_dt2[MA_MAX];
     double chol[MA_MAX * (MA_MAX + 1) / 2];   // lower factor, row i at i(i+1)/2
     double notional, maturity, autocall_barrier, coupon_rate, coupon_barrier;
     double ki_barrier, ki_strike;
+    double autocall_y, coupon_y, ki_y;        // fast mode: 256 log2 of the barriers
     double df_maturity;       // exp(-r maturity)
     double redeem_plain;      // notional (1 + c maturity): standard note, no knock-in
     double shift;             // payoffs are summed as pv - shift (variance accuracy)
@@ -90,13 +95,14 @@ struct alignas(16) AcGreeksDev {
     int64_t paths, normals_offset;
     uint64_t seed;
     double s0[MA_MAX], s0_up[MA_MAX], s0_dn[MA_MAX];
-    // fast mode: performances s * (1 / s0), the reciprocals of the three
-    double inv_s0[MA_MAX], inv_up[MA_MAX], inv_dn[MA_MAX];
+    // fast mode: the log-performance floors of the three spots (AcDev)
+    double yfloor[MA_MAX], yfloor_up[MA_MAX], yfloor_dn[MA_MAX];
     double drift[3][MA_MAX], vol_dt[3][MA_MAX];     // base, vol up, vol down
     double drift2[3][MA_MAX], vol_dt2[3][MA_MAX];   // fast mode: scaled by 256/ln2
     double chol[3][MA_MAX * (MA_MAX + 1) / 2];     // base, correlation up, down
     double notional, autocall_barrier, coupon_rate, coupon_barrier;
     double ki_barrier, ki_strike, df_maturity, redeem_plain, shift;
+    double autocall_y, coupon_y, ki_y;
 };
 
 struct AcgArgs {

@@ -144,7 +144,7 @@ ofg::AcDev make_scenario(const ofg_autocall &c, const std::vector<double> &spots
     for (int i = 0; i < n; ++i) {
         const double s0 = spots[(size_t)c.underlyings[i]], v = vols[(size_t)c.underlyings[i]];
         d.s0[i] = s0;
-        d.inv_s0[i] = 1.0 / s0;
+        d.yfloor[i] = std::log(1.0e-12 / s0) * OFG_INV_LN2_256;
         d.drift[i] = (c.rate - c.div_yield - 0.5 * v * v) * dt;
         d.vol_dt[i] = v * sqrt_dt;
         d.drift2[i] = d.drift[i] * OFG_INV_LN2_256;
@@ -158,6 +158,10 @@ ofg::AcDev make_scenario(const ofg_autocall &c, const std::vector<double> &spots
     d.coupon_barrier = c.coupon_barrier;
     d.ki_barrier = c.ki_barrier;
     d.ki_strike = c.ki_strike;
+    d.autocall_y = std::log(c.autocall_barrier) * OFG_INV_LN2_256;
+    d.coupon_y = c.coupon_barrier > 0.0 ? std::log(c.coupon_barrier) * OFG_INV_LN2_256
+                                        : -INFINITY;
+    d.ki_y = std::log(c.ki_barrier) * OFG_INV_LN2_256;
     d.df_maturity = std::exp(-c.rate * c.maturity);
     d.redeem_plain = c.notional * (1.0 + c.coupon_rate * c.maturity);
     d.shift = c.notional;
@@ -344,9 +348,9 @@ int32_t autocall_impl(ofg_ctx *ctx, const ofg_autocall *in, size_t n, const ofg_
             g.s0[a] = b0.s0[a];
             g.s0_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].s0[a];
             g.s0_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].s0[a];
-            g.inv_s0[a] = b0.inv_s0[a];
-            g.inv_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].inv_s0[a];
-            g.inv_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].inv_s0[a];
+            g.yfloor[a] = b0.yfloor[a];
+            g.yfloor_up[a] = scen[(size_t)pl.delta_up[(size_t)a]].yfloor[a];
+            g.yfloor_dn[a] = scen[(size_t)pl.delta_dn[(size_t)a]].yfloor[a];
             for (int v = 0; v < 3; ++v) {
                 g.drift[v][a] = sets[v]->drift[a];
                 g.vol_dt[v][a] = sets[v]->vol_dt[a];
@@ -361,6 +365,9 @@ int32_t autocall_impl(ofg_ctx *ctx, const ofg_autocall *in, size_t n, const ofg_
         g.coupon_barrier = b0.coupon_barrier;
         g.ki_barrier = b0.ki_barrier;
         g.ki_strike = b0.ki_strike;
+        g.autocall_y = b0.autocall_y;
+        g.coupon_y = b0.coupon_y;
+        g.ki_y = b0.ki_y;
         g.df_maturity = b0.df_maturity;
         g.redeem_plain = b0.redeem_plain;
         g.shift = b0.shift;
