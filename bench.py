@@ -648,6 +648,9 @@ def dsl_leg(reps=3, cpu=True):
                 "frac_fp64_peak": asset_steps / (k * 1e-3) * flop_multi_asset_step(3) / 1e12
                 / FP64_PEAK_TFLOPS}
 
+    out["frac_note"] = ("frac_fp64_peak counts the reference's step (an exponential per "
+                        "asset-step); the fast mode's GBM-only kernels step log growths and "
+                        "take the exponential per stored snapshot")
     modes = ((RngMode.Reference, "reference_draws"), (RngMode.Fast, "fast_draws"))
     # the program interpreter (the default), then the product's native
     # evaluation kernel (compiled at run time, the same prices bit for bit)

@@ -63,6 +63,18 @@ __device__ __forceinline__ double gbm_factor(double p0, double p1, double z, con
     }
 }
 
+// Fast mode of the GBM-only kernels: a spot is carried as its log growth
+// Y = 256 log2(S / s0), advanced by gbm_factor's exponent, and formed as
+// s0 2^(Y/256) only where a snapshot is stored (the same discretisation, one
+// exponential per snapshot instead of one per step).
+__device__ __forceinline__ double gbm_log_growth(double p0, double p1, double z) {
+    return (p0 + p1 * z) * OFG_INV_LN2_256;
+}
+
+__device__ __forceinline__ double gbm_spot(double s0, double y, const double *tab) {
+    return s0 * exp2_256(fmin(fmax(y, -OFG_EXP2_256_YMAX), OFG_EXP2_256_YMAX), tab);
+}
+
 // AssetStepper::step (engine.rs:95-122)
 template <bool FAST, bool TABLE = false>
 __device__ __forceinline__ double step_asset(int kind, double prev, double p0, double p1, double p2,
@@ -174,16 +186,18 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
                                               double *snap, int64_t stride, const double *tab,
                                               double *slots, Xoshiro256pp &ref) {
     static_assert(NA == 0 || NA == MAXA, "exact asset count");
+    // log growths instead of spots (gbm_log_growth)
+    constexpr bool LOGS = NA > 0 && RNG == OFG_RNG_FAST;
     const int n = NA > 0 ? NA : o.n_assets;
     double s[MAXA];
 #pragma unroll
-    for (int i = 0; i < MAXA; ++i) s[i] = o.s0[i];
+    for (int i = 0; i < MAXA; ++i) s[i] = LOGS ? 0.0 : o.s0[i];
     const int32_t *sl = a.snap_list;
     int next_step = sl[0], next_k = sl[1];
     if (next_step == 0) {
 #pragma unroll
         for (int i = 0; i < MAXA; ++i)
-            if (i < n) snap[(int64_t)(next_k * n + i) * stride] = s[i];
+            if (i < n) snap[(int64_t)(next_k * n + i) * stride] = o.s0[i];
         sl += 2;
         next_step = sl[0];
         next_k = sl[1];
@@ -226,7 +240,9 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
 #pragma unroll
         for (int i = 0; i < MAXA; ++i) {
             if (i < n) {
-                if constexpr (NA > 0)
+                if constexpr (LOGS)
+                    s[i] += gbm_log_growth(sp.p0(i), sp.p1(i), c[i]);
+                else if constexpr (NA > 0)
                     s[i] = s[i] * gbm_factor<RNG == OFG_RNG_FAST, WTAIL && RNG == OFG_RNG_REFERENCE>(
                                       sp.p0(i), sp.p1(i), c[i], tab);
                 else
@@ -237,7 +253,9 @@ __device__ __forceinline__ void simulate_path(const DslArgs &a, const DslScen &o
         if (step == next_step) {
 #pragma unroll
             for (int i = 0; i < MAXA; ++i)
-                if (i < n) snap[(int64_t)(next_k * n + i) * stride] = s[i];
+                if (i < n)
+                    snap[(int64_t)(next_k * n + i) * stride] =
+                        LOGS ? gbm_spot(o.s0[i], s[i], tab) : s[i];
             sl += 2;
             next_step = sl[0];
             next_k = sl[1];
@@ -491,6 +509,8 @@ template <int RNG, int MAXA, int NA = 0>
 __global__ __launch_bounds__(DSL_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
 dsl_sim_markets_kernel(DslArgs a) {
     static_assert(NA == 0 || NA == MAXA, "exact asset count");
+    // log growths instead of spots (gbm_log_growth), as simulate_path
+    constexpr bool LOGS = NA > 0 && RNG == OFG_RNG_FAST;
     __shared__ double tab[RNG == OFG_RNG_FAST ? OFG_FAST_TABLE_LEN : 1];
     if constexpr (RNG == OFG_RNG_FAST) {
         for (int i = threadIdx.x; i < OFG_FAST_TABLE_LEN; i += DSL_BLOCK) tab[i] = OFG_FAST_TABLE[i];
@@ -515,7 +535,7 @@ dsl_sim_markets_kernel(DslArgs a) {
 #pragma unroll
         for (int m = 0; m < DSL_FUSE_MAXS; ++m)
 #pragma unroll
-            for (int i = 0; i < MAXA; ++i) s[m][i] = m < ns ? a.scen[m].s0[i] : 0.0;
+            for (int i = 0; i < MAXA; ++i) s[m][i] = m < ns && !LOGS ? a.scen[m].s0[i] : 0.0;
         // snapshot steps by a cursor over a.snap_list
         const int32_t *sl = a.snap_list;
         int next_step = sl[0], next_k = sl[1];
@@ -527,7 +547,7 @@ dsl_sim_markets_kernel(DslArgs a) {
                     for (int i = 0; i < MAXA; ++i)
                         if (i < n)
                             a.snaps[a.scen[m].snap_off + (int64_t)(next_k * n + i) * o.paths + p] =
-                                s[m][i];
+                                a.scen[m].s0[i];
             sl += 2;
             next_step = sl[0];
             next_k = sl[1];
@@ -571,7 +591,8 @@ dsl_sim_markets_kernel(DslArgs a) {
                     z[j2] = 0.0;
                 }
             }
-            // market 0: correlated normals and GBM factors
+            // market 0: correlated normals and GBM factors (log growths when
+            // LOGS)
 #pragma unroll
             for (int i = 0; i < MAXA; ++i) c0[i] = 0.0;
 #pragma unroll
@@ -581,9 +602,9 @@ dsl_sim_markets_kernel(DslArgs a) {
                     if (i < n && j2 < n) c0[i] += sp0.chol(i * (i + 1) / 2 + j2) * z[j2];
 #pragma unroll
             for (int i = 0; i < MAXA; ++i)
-                g0[i] = (i < n && (NA > 0 || sp0.kind(i) == DSL_GBM))
-                            ? gbm_factor<RNG == OFG_RNG_FAST>(sp0.p0(i), sp0.p1(i), c0[i], tab)
-                            : 1.0;
+                g0[i] = !(i < n && (NA > 0 || sp0.kind(i) == DSL_GBM)) ? 1.0
+                        : LOGS ? gbm_log_growth(sp0.p0(i), sp0.p1(i), c0[i])
+                               : gbm_factor<RNG == OFG_RNG_FAST>(sp0.p0(i), sp0.p1(i), c0[i], tab);
             const bool snap_now = step == next_step;
             const int ks = next_k;
 #pragma unroll
@@ -606,7 +627,11 @@ dsl_sim_markets_kernel(DslArgs a) {
 #pragma unroll
                 for (int i = 0; i < MAXA; ++i) {
                     if (i < n) {
-                        if ((share[m] >> i) & 1u) s[m][i] = s[m][i] * g0[i];
+                        if constexpr (LOGS)
+                            s[m][i] += ((share[m] >> i) & 1u)
+                                           ? g0[i]
+                                           : gbm_log_growth(q.p0[i], q.p1[i], c[i]);
+                        else if ((share[m] >> i) & 1u) s[m][i] = s[m][i] * g0[i];
                         else if constexpr (NA > 0)
                             s[m][i] = s[m][i] * gbm_factor<RNG == OFG_RNG_FAST>(q.p0[i], q.p1[i],
                                                                                 c[i], tab);
@@ -618,7 +643,9 @@ dsl_sim_markets_kernel(DslArgs a) {
                 if (snap_now) {
 #pragma unroll
                     for (int i = 0; i < MAXA; ++i)
-                        if (i < n) a.snaps[q.snap_off + (int64_t)(ks * n + i) * o.paths + p] = s[m][i];
+                        if (i < n)
+                            a.snaps[q.snap_off + (int64_t)(ks * n + i) * o.paths + p] =
+                                LOGS ? gbm_spot(q.s0[i], s[m][i], tab) : s[m][i];
                 }
             }
             if (snap_now) {
